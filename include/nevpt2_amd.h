@@ -1,0 +1,66 @@
+/*
+ * nevpt2_amd -- C ABI of the MI355X (gfx950) NEVPT2 RDM-contraction hot path.
+ *
+ * Conventions (identical to the Fortran reference, so a Fortran caller passes
+ * its arrays unchanged through ISO_C_BINDING):
+ *   - every array is column-major with the state index fastest, e.g.
+ *     daaa(metat,n,n,n,n,n,n), taa(metat,n,n,n,n), dal(metat,n,n);
+ *   - the packed 4-RDM is ro4(nwords,n,n,n,n,metat), nwords = n(n+1)(n+2)(n+3)/24
+ *     (koopro4QD.F90:263, indices_norm.F90:53);
+ *   - one-electron quantities (oneint, aj) are packed over the norb triangle,
+ *     pair (p,q) at max(p,q)*(max(p,q)-1)/2+min(p,q) (1-based, koop_matrices.F90:180);
+ *   - Cholesky vectors cholesky(nchovec, norb*(norb+1)/2), pair column as above
+ *     (ijkl_utils.F90:257, 676-677);
+ *   - all dimensions are int64 (the reference builds with -fdefault-integer-8,
+ *     cmake/FortranFlags.cmake:17-21).
+ * Every entry point returns 0 on success, -1 on a HIP error, -2 on invalid
+ * arguments; nv2_last_error() returns the message (the reference stops with a
+ * message instead, e.g. qdnevpt.F:521-526).
+ */
+#ifndef NEVPT2_AMD_H
+#define NEVPT2_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* arithmetic of the 4-RDM contractions (bamat) and PC S-builds */
+#define NV2_PREC_FP64 0   /* drop-in form, matches the reference */
+#define NV2_PREC_FP32 1   /* FP32 storage and arithmetic of the 4-RDM contractions */
+
+int nv2_last_error(char *buf, int64_t len);
+int nv2_device_count(void);
+
+/* Partial traces of the packed 4-RDM: daaa = Tr4 ro4/(N-3), taa = Tr3 daaa/(N-2),
+ * dal = Tr2 taa/(N-1).  Replaces bro3/bro2/bro1 (koopro4QD.F90:4881, 4975, 5028). */
+int nv2_bro(int64_t nact, int64_t metat, int64_t nele, const double *ro4, double *daaa, double *taa, double *dal);
+
+/* Dyall core-dressed one-electron matrix aj and active ERIs atwo(n,n,n,n) from
+ * Cholesky vectors.  Replaces get_aj_atwo (koopro4QD.F90:3900). */
+int nv2_get_aj_atwo(int64_t ncore, int64_t nact, int64_t norb, int64_t nchovec, const double *cholesky,
+                    const double *oneint, double *aj, double *atwo);
+
+/* Koopmans matrices of rank <= 3: 1-IPKO/1-EAKO (koopE, koop_matrices.F90:165),
+ * 2-HRDM (koop_matrices.F90:78-88), 2-IPKO/2-EAKO (koop2E, :253),
+ * gk0pa/gk0pd/gk0pf (koopman0pE, :389) and the B/C/D matrices of the (+-1)'
+ * classes (bbmat/bcmat/bdmat, koopro4QD.F90:4231, 4301, 4399).  Together with
+ * nv2_bamat this replaces koop_matrices_ctl (koop_matrices.F90:16) and the
+ * A-D block of koopro4QD_driver (koopro4QD.F90:352-424).  Shapes:
+ * (metat,n,n) for ipko1, eako1, gk0pf, dmat, dtmat; (metat,n,n,n,n) otherwise. */
+int nv2_koop_matrices(int64_t metat, int64_t nact, int64_t ncore, int64_t norb, const double *atwo,
+                      const double *daaa, const double *taa, const double *dal, const double *aj, double *ipko1,
+                      double *eako1, double *hrdm, double *ipko2, double *eako2, double *gk0pa, double *gk0pd,
+                      double *gk0pf, double *bmat, double *btmat, double *cmat, double *ctmat, double *dmat,
+                      double *dtmat);
+
+/* A-matrices of the (+-1)' classes, amat/atmat(metat,n,n,n,n,n,n) (amatIP/amatEA).
+ * Replaces bamat (koopro4QD.F90:3974), C1 branch. */
+int nv2_bamat(int64_t nact, int64_t ncore, int64_t norb, int64_t metat, const double *atwo, const double *ro4,
+              const double *daaa, const double *taa, const double *dal, const double *aj, double *amat,
+              double *atmat, int32_t prec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

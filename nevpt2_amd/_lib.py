@@ -1,0 +1,74 @@
+"""ctypes binding of libnevpt2_amd.so (the C ABI in include/nevpt2_amd.h).
+
+The library is built in-tree by ``python -m nevpt2_amd.build``.  Loading fails
+loudly when it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libnevpt2_amd.so")
+
+i64 = C.c_int64
+i32 = C.c_int32
+dp = C.POINTER(C.c_double)
+
+PREC_FP64 = 0
+PREC_FP32 = 1
+
+_lib = None
+
+
+class NV2Error(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise NV2Error(f"{LIBPATH} is not built; run `python -m nevpt2_amd.build`")
+        _lib = C.CDLL(LIBPATH)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    L.nv2_last_error.argtypes = [C.c_char_p, i64]
+    L.nv2_device_count.restype = C.c_int
+    L.nv2_bro.argtypes = [i64, i64, i64, dp, dp, dp, dp]
+    L.nv2_get_aj_atwo.argtypes = [i64, i64, i64, i64, dp, dp, dp, dp]
+    L.nv2_koop_matrices.argtypes = [i64, i64, i64, i64] + [dp] * 19
+    L.nv2_bamat.argtypes = [i64, i64, i64, i64] + [dp] * 8 + [i32]
+    for name in dir(L):
+        pass
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(1024)
+    lib().nv2_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        raise NV2Error(f"{what} failed (rc={rc}): {last_error()}")
+
+
+def ptr(a: np.ndarray):
+    if not (a.flags.f_contiguous or a.flags.c_contiguous) or a.dtype != np.float64:
+        raise ValueError("arrays passed to libnevpt2_amd must be contiguous float64")
+    return a.ctypes.data_as(dp)
+
+
+def fin(a) -> np.ndarray:
+    """Input array as Fortran-contiguous float64 (the reference layout)."""
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+def zeros(*shape) -> np.ndarray:
+    return np.zeros(shape, order="F")

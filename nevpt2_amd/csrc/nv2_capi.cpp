@@ -1,0 +1,185 @@
+// C ABI of the NEVPT2 hot-path library (declarations in include/nevpt2_amd.h).
+//
+// The per-routine entry points take host arrays in the reference layouts,
+// stage them to HBM, run the gfx950 kernels and copy the results back; they
+// are the drop-in replacements for the Fortran routines they name.
+#include <hip/hip_runtime.h>
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+
+#include "nv2_common.h"
+#include "nv2_stage1.h"
+#include "../../include/nevpt2_amd.h"
+
+namespace nv2 {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+// device buffer owned for the duration of one call
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    ~DBuf() { if (p) (void)hipFree(p); }
+    int alloc(size_t b) {
+        bytes = b;
+        if (b == 0) return 0;
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) { set_error("hipMalloc(%zu bytes): %s", b, hipGetErrorString(e)); p = nullptr; return -1; }
+        return 0;
+    }
+    int up(const void *h, size_t b) {
+        if (alloc(b)) return -1;
+        if (b && h) {
+            hipError_t e = hipMemcpy(p, h, b, hipMemcpyHostToDevice);
+            if (e != hipSuccess) { set_error("hipMemcpy H2D: %s", hipGetErrorString(e)); return -1; }
+        }
+        return 0;
+    }
+    int down(void *h) const {
+        if (!bytes || !h) return 0;
+        hipError_t e = hipMemcpy(h, p, bytes, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { set_error("hipMemcpy D2H: %s", hipGetErrorString(e)); return -1; }
+        return 0;
+    }
+    double *d() const { return (double *)p; }
+};
+
+static int check_dims(i64 nact, i64 metat) {
+    if (nact < 1 || nact > 16) { set_error("nact=%lld outside 1..16", (long long)nact); return -2; }
+    if (metat < 1) { set_error("metat=%lld must be >= 1", (long long)metat); return -2; }
+    return 0;
+}
+
+static inline i64 nwords(i64 n) { return n * (n + 1) * (n + 2) * (n + 3) / 24; }
+static inline i64 p(i64 n, int k) { i64 r = 1; while (k--) r *= n; return r; }
+
+}  // namespace nv2
+
+using namespace nv2;
+
+#define TRY(x) do { int _r = (x); if (_r) return _r; } while (0)
+
+extern "C" {
+
+int nv2_last_error(char *buf, int64_t len) {
+    if (!buf || len <= 0) return -2;
+    strncpy(buf, g_err, (size_t)len - 1);
+    buf[len - 1] = 0;
+    return 0;
+}
+
+int nv2_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+int nv2_bro(int64_t nact, int64_t metat, int64_t nele, const double *ro4, double *daaa, double *taa, double *dal) {
+    TRY(check_dims(nact, metat));
+    if (nele < 4) { set_error("nv2_bro needs nele >= 4 (got %lld)", (long long)nele); return -2; }
+    const i64 n = nact, M = metat;
+    DBuf dro4, d3, d2, d1;
+    TRY(dro4.up(ro4, sizeof(double) * nwords(n) * p(n, 4) * M));
+    TRY(d3.alloc(sizeof(double) * M * p(n, 6)));
+    TRY(d2.alloc(sizeof(double) * M * p(n, 4)));
+    TRY(d1.alloc(sizeof(double) * M * p(n, 2)));
+    Rdm R{(int)n, (int)M, nwords(n), nullptr, nullptr, nullptr, dro4.d()};
+    TRY(launch_bro(R, (int)nele, d3.d(), d2.d(), d1.d(), 0));
+    NV2_HIP(hipDeviceSynchronize());
+    TRY(d3.down(daaa)); TRY(d2.down(taa)); TRY(d1.down(dal));
+    return 0;
+}
+
+int nv2_get_aj_atwo(int64_t ncore, int64_t nact, int64_t norb, int64_t nchovec, const double *cholesky,
+                    const double *oneint, double *aj, double *atwo) {
+    TRY(check_dims(nact, 1));
+    const i64 np = norb * (norb + 1) / 2;
+    DBuf dB, d1, daj, datwo;
+    TRY(dB.up(cholesky, sizeof(double) * nchovec * np));
+    TRY(d1.up(oneint, sizeof(double) * np));
+    TRY(daj.alloc(sizeof(double) * np));
+    TRY(datwo.alloc(sizeof(double) * p(nact, 4)));
+    TRY(launch_aj_atwo((int)nact, (int)ncore, (int)norb, (int)nchovec, dB.d(), d1.d(), daj.d(), datwo.d(), 0));
+    NV2_HIP(hipDeviceSynchronize());
+    TRY(daj.down(aj)); TRY(datwo.down(atwo));
+    return 0;
+}
+
+// all Koopmans-type matrices that need at most the 3-RDM: koop_matrices_ctl + bbmat/bcmat/bdmat
+int nv2_koop_matrices(int64_t metat, int64_t nact, int64_t ncore, int64_t norb, const double *atwo,
+                      const double *daaa, const double *taa, const double *dal, const double *aj, double *ipko1,
+                      double *eako1, double *hrdm, double *ipko2, double *eako2, double *gk0pa, double *gk0pd,
+                      double *gk0pf, double *bmat, double *btmat, double *cmat, double *ctmat, double *dmat,
+                      double *dtmat) {
+    TRY(check_dims(nact, metat));
+    const i64 n = nact, M = metat, np = norb * (norb + 1) / 2;
+    DBuf dat, d3, d2, d1, daj, AJ, Fh, F1;
+    TRY(dat.up(atwo, sizeof(double) * p(n, 4)));
+    TRY(d3.up(daaa, sizeof(double) * M * p(n, 6)));
+    TRY(d2.up(taa, sizeof(double) * M * p(n, 4)));
+    TRY(d1.up(dal, sizeof(double) * M * p(n, 2)));
+    TRY(daj.up(aj, sizeof(double) * np));
+    TRY(AJ.alloc(sizeof(double) * n * n)); TRY(Fh.alloc(sizeof(double) * n * n)); TRY(F1.alloc(sizeof(double) * n * n));
+    const size_t s2 = sizeof(double) * M * n * n, s4 = sizeof(double) * M * p(n, 4);
+    DBuf o[14];
+    size_t sz[14] = {s2, s2, s4, s4, s4, s4, s4, s2, s4, s4, s4, s4, s2, s2};
+    for (int k = 0; k < 14; k++) TRY(o[k].alloc(sz[k]));
+    Stage1Scratch w; w.AJ = AJ.d(); w.Fhalf = Fh.d(); w.Fone = F1.d();
+    Stage1Out so;
+    so.ipko1 = o[0].d(); so.eako1 = o[1].d(); so.hrdm = o[2].d(); so.ipko2 = o[3].d(); so.eako2 = o[4].d();
+    so.gk0pa = o[5].d(); so.gk0pd = o[6].d(); so.gk0pf = o[7].d();
+    so.bmat = o[8].d(); so.btmat = o[9].d(); so.cmat = o[10].d(); so.ctmat = o[11].d();
+    so.dmat = o[12].d(); so.dtmat = o[13].d();
+    Rdm R{(int)n, (int)M, nwords(n), d3.d(), d2.d(), d1.d(), nullptr};
+    TRY(launch_koop_small(R, (int)ncore, daj.d(), dat.d(), w, so, 0));
+    NV2_HIP(hipDeviceSynchronize());
+    double *hs[14] = {ipko1, eako1, hrdm, ipko2, eako2, gk0pa, gk0pd, gk0pf, bmat, btmat, cmat, ctmat, dmat, dtmat};
+    for (int k = 0; k < 14; k++) TRY(o[k].down(hs[k]));
+    return 0;
+}
+
+int nv2_bamat(int64_t nact, int64_t ncore, int64_t norb, int64_t metat, const double *atwo, const double *ro4,
+              const double *daaa, const double *taa, const double *dal, const double *aj, double *amat,
+              double *atmat, int32_t prec) {
+    TRY(check_dims(nact, metat));
+    if (prec != NV2_PREC_FP64 && prec != NV2_PREC_FP32) { set_error("nv2_bamat: unsupported precision %d", prec); return -2; }
+    const i64 n = nact, M = metat, np = norb * (norb + 1) / 2;
+    DBuf dat, dro4, d3, d2, d1, daj, Fh, E4, C[4], dam, dat6;
+    TRY(dat.up(atwo, sizeof(double) * p(n, 4)));
+    TRY(dro4.up(ro4, sizeof(double) * nwords(n) * p(n, 4) * M));
+    TRY(d3.up(daaa, sizeof(double) * M * p(n, 6)));
+    TRY(d2.up(taa, sizeof(double) * M * p(n, 4)));
+    TRY(d1.up(dal, sizeof(double) * M * p(n, 2)));
+    TRY(daj.up(aj, sizeof(double) * np));
+    TRY(Fh.alloc(sizeof(double) * n * n));
+    TRY(E4.alloc((prec == NV2_PREC_FP64 ? sizeof(double) : sizeof(float)) * p(n, 8)));
+    for (int k = 0; k < 4; k++) TRY(C[k].alloc(sizeof(double) * M * p(n, 6)));
+    TRY(dam.alloc(sizeof(double) * M * p(n, 6)));
+    TRY(dat6.alloc(sizeof(double) * M * p(n, 6)));
+    Rdm R{(int)n, (int)M, nwords(n), d3.d(), d2.d(), d1.d(), dro4.d()};
+    TRY(launch_make_f((int)n, (int)ncore, daj.d(), dat.d(), 0.5, Fh.d(), 0));
+    ContractPlan *plan = contract_plan_create((int)n, (int)M);
+    int rc = 0;
+    for (int s = 0; s < M && !rc; s++) {
+        rc = launch_build_e4(R, s, E4.p, prec, 0);
+        if (!rc) rc = launch_e4_contractions(plan, E4.p, prec, dat.d(), s, C[0].d(), C[1].d(), C[2].d(), C[3].d(), 0);
+    }
+    if (!rc) rc = launch_bamat_direct(R, dat.d(), Fh.d(), C[0].d(), C[1].d(), C[2].d(), C[3].d(), dam.d(), dat6.d(), 0);
+    if (!rc) { hipError_t e = hipDeviceSynchronize(); if (e != hipSuccess) { set_error("bamat: %s", hipGetErrorString(e)); rc = -1; } }
+    contract_plan_destroy(plan);
+    if (rc) return rc;
+    TRY(dam.down(amat)); TRY(dat6.down(atmat));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,530 @@
+// Stage 1 of the NEVPT2 hot path on gfx950: Koopmans-matrix builds.
+//
+// O(n^7) builds (koopE, 2-hole RDM, koop2E, koopman0pE, bbmat, bcmat, bdmat,
+// partial traces) run one thread per output element with the reference's
+// inner loop nest, reading RDMs that stay resident in HBM/MALL.  The O(n^9)
+// 4-RDM part of bamat is split off: the full <E E E E> tensor E4 is
+// materialised once per state and contracted with the active ERIs as four
+// strided GEMMs (nv2_contract.hip); everything lower-order in bamat runs in
+// nv2_bamat_direct.
+#include "nv2_common.h"
+#include "nv2_stage1.h"
+
+namespace nv2 {
+
+static __device__ double ro3t(const Rdm &R, int s, int a, int b, int c, int ap, int bp, int cp) {
+    // three-hole density (eexx_functions.F90:164-207)
+    double v = -D3(s, ap, bp, cp, a, b, c);
+    if (ap == a) v += 2.0 * D2(s, bp, cp, b, c);
+    if (bp == b) v += 2.0 * D2(s, ap, cp, a, c);
+    if (cp == c) v += 2.0 * D2(s, ap, bp, a, b);
+    if (bp == c) v -= D2(s, ap, cp, a, b);
+    if (ap == c) v -= D2(s, cp, bp, a, b);
+    if (cp == b) v -= D2(s, bp, ap, c, a);
+    if (ap == b) v -= D2(s, bp, cp, a, c);
+    if (cp == a) v -= D2(s, ap, bp, c, b);
+    if (bp == a) v -= D2(s, ap, cp, b, c);
+    if (cp == c && bp == b) v -= 4.0 * D1(s, ap, a);
+    if (cp == b && bp == c) v += 2.0 * D1(s, ap, a);
+    if (cp == c && ap == a) v -= 4.0 * D1(s, bp, b);
+    if (cp == a && ap == c) v += 2.0 * D1(s, bp, b);
+    if (ap == a && bp == b) v -= 4.0 * D1(s, cp, c);
+    if (ap == b && bp == a) v += 2.0 * D1(s, cp, c);
+    if (cp == c && ap == b) v += 2.0 * D1(s, bp, a);
+    if (cp == b && ap == c) v -= D1(s, bp, a);
+    if (cp == c && bp == a) v += 2.0 * D1(s, ap, b);
+    if (cp == a && bp == c) v -= D1(s, ap, b);
+    if (bp == b && ap == c) v += 2.0 * D1(s, cp, a);
+    if (bp == c && ap == b) v -= D1(s, cp, a);
+    if (bp == b && cp == a) v += 2.0 * D1(s, ap, c);
+    if (bp == a && cp == b) v -= D1(s, ap, c);
+    if (ap == a && cp == b) v += 2.0 * D1(s, bp, c);
+    if (ap == b && cp == a) v -= D1(s, bp, c);
+    if (ap == a && bp == c) v += 2.0 * D1(s, cp, b);
+    if (ap == c && bp == a) v -= D1(s, cp, b);
+    if (ap == a && bp == b && cp == c) v += 8.0;
+    if (ap == b && bp == a && cp == c) v -= 4.0;
+    if (ap == a && bp == c && cp == b) v -= 4.0;
+    if (ap == c && bp == b && cp == a) v -= 4.0;
+    if (ap == b && bp == c && cp == a) v += 2.0;
+    if (ap == c && bp == a && cp == b) v += 2.0;
+    return v;
+}
+
+#define AT(p, q, r, t) atwo[(p) + n * ((q) + n * ((r) + n * (t)))]
+#define FF(p, q) F[(p) + n * (q)]
+
+// decode a flat Fortran index with the state fastest: out = s + M*(i0 + n*(i1 + ...))
+__device__ __forceinline__ void decode(i64 idx, int M, int n, int k, int *s, int *ix) {
+    *s = (int)(idx % M);
+    idx /= M;
+    for (int t = 0; t < k; t++) { ix[t] = (int)(idx % n); idx /= n; }
+}
+
+// ---------------------------------------------------------------- partial traces
+__global__ void k_bro3(Rdm R, int nele, double *daaa) {
+    const int n = R.n, M = R.M;
+    i64 tot = (i64)M * n * n * n * n * n * n;
+    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= tot) return;
+    int s, x[6];
+    decode(idx, M, n, 6, &s, x);
+    double v = 0.0;
+    for (int ll = 0; ll < n; ll++) v += g4(R, s, x[0], x[1], x[2], ll, x[3], x[4], x[5], ll);
+    daaa[idx] = v / (double)(nele - 3);
+}
+__global__ void k_bro2(Rdm R, int nele, double *taa) {
+    const int n = R.n, M = R.M;
+    i64 tot = (i64)M * n * n * n * n;
+    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= tot) return;
+    int s, x[4];
+    decode(idx, M, n, 4, &s, x);
+    double v = 0.0;
+    for (int ll = 0; ll < n; ll++) v += D3(s, x[0], x[1], ll, x[2], x[3], ll);
+    taa[idx] = v / (double)(nele - 2);
+}
+__global__ void k_bro1(Rdm R, int nele, double *dal) {
+    const int n = R.n, M = R.M;
+    i64 tot = (i64)M * n * n;
+    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= tot) return;
+    int s, x[2];
+    decode(idx, M, n, 2, &s, x);
+    double v = 0.0;
+    for (int ll = 0; ll < n; ll++) v += D2(s, x[0], ll, x[1], ll);
+    dal[idx] = v / (double)(nele - 1);
+}
+
+// ---------------------------------------------------------------- Dyall h, active ERIs (koopro4QD.F90:3900-3937)
+__global__ void k_atwo(int n, int ncore, int L, const double *B, double *atwo) {
+    i64 tot = (i64)n * n * n * n;
+    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= tot) return;
+    int i = idx % n, j = (idx / n) % n, k = (idx / n / n) % n, l = idx / n / n / n;
+    const double *u = B + (i64)L * tri(i + ncore, j + ncore), *v = B + (i64)L * tri(k + ncore, l + ncore);
+    double x = 0.0;
+    for (int m = 0; m < L; m++) x += u[m] * v[m];
+    atwo[idx] = x;
+}
+__global__ void k_aj(int n, int ncore, int L, const double *B, double *aj) {
+    int a = blockIdx.x, b = threadIdx.x;          // a <= b active
+    if (b < a || b >= n) return;
+    i64 ac = a + ncore, bc = b + ncore, jab = tri(ac, bc);
+    double acc = aj[jab];
+    for (int j = 0; j < ncore; j++) {
+        const double *u = B + (i64)L * tri(ac, bc), *v = B + (i64)L * tri(j, j);
+        double x = 0, y = 0;
+        for (int l = 0; l < L; l++) x += u[l] * v[l];
+        u = B + (i64)L * tri(ac, j); v = B + (i64)L * tri(j, bc);
+        for (int l = 0; l < L; l++) y += u[l] * v[l];
+        acc = acc + 2.0 * x - y;
+    }
+    aj[jab] = acc;
+}
+
+// active f(ab) = aj(ab) - alpha * sum_c atwo(a,c,c,b) (e.g. koopro4QD.F90:3993-4005)
+__global__ void k_make_f(int n, int ncore, const double *aj, const double *atwo, double alpha, double *F) {
+    int a = blockIdx.x, b = threadIdx.x;
+    if (b < a || b >= n) return;
+    double v = aj[tri(a + ncore, b + ncore)];
+    for (int c = 0; c < n; c++) v = v - AT(a, c, c, b) * alpha;
+    FF(a, b) = v;
+    FF(b, a) = v;
+}
+// active block of aj as an n x n matrix
+__global__ void k_aj_active(int n, int ncore, const double *aj, double *A) {
+    int a = blockIdx.x, b = threadIdx.x;
+    if (b >= n) return;
+    A[a + n * b] = aj[tri(a + ncore, b + ncore)];
+}
+
+// ---------------------------------------------------------------- koopE (koop_matrices.F90:165-250); F = aj
+__global__ void k_koopE(Rdm R, const double *atwo, const double *F, double *ipa, double *eaa) {
+    const int n = R.n, M = R.M;
+    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (i64)M * n * n) return;
+    int s, x[2];
+    decode(idx, M, n, 2, &s, x);
+    const int ap = x[0], a = x[1];
+    double ip = 0.0;
+    for (int c = 0; c < n; c++) {
+        ip = ip - FF(a, c) * D1(s, ap, c);
+        for (int d = 0; d < n; d++)
+            for (int ie = 0; ie < n; ie++) ip = ip - AT(c, ie, a, d) * D2(s, ap, c, d, ie);
+    }
+    ip = 2.0 * ip;
+    // eaa(ap,a) = ipa(ap,a) + 2*dumm with dumm from (ap,a)
+    double dumm = 2.0 * FF(ap, a);
+    for (int d = 0; d < n; d++)
+        for (int ie = 0; ie < n; ie++) dumm = dumm + (2.0 * AT(d, ie, ap, a) - AT(ap, ie, d, a)) * D1(s, d, ie);
+    ipa[idx] = ip;
+    eaa[idx] = ip + 2.0 * dumm;
+}
+
+// ---------------------------------------------------------------- 2-hole RDM (koop_matrices.F90:78-88)
+__global__ void k_hrdm(Rdm R, double *out) {
+    const int n = R.n, M = R.M;
+    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (i64)M * n * n * n * n) return;
+    int s, x[4];
+    decode(idx, M, n, 4, &s, x);
+    out[idx] = ro2t(R, s, x[0], x[1], x[2], x[3]);
+}
+
+// ---------------------------------------------------------------- koop2E (koop_matrices.F90:253-386)
+__global__ void k_koop2E(Rdm R, const double *atwo, const double *F, double *koopaa, double *koopeaa) {
+    const int n = R.n, M = R.M;
+    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (i64)M * n * n * n * n) return;
+    int s, x[4];
+    decode(idx, M, n, 4, &s, x);
+    const int ap = x[0], bp = x[1], a = x[2], b = x[3];
+    double ip = 0.0;
+    for (int d = 0; d < n; d++) {
+        ip = ip - FF(a, d) * D2(s, ap, bp, d, b) - FF(b, d) * D2(s, ap, bp, a, d);
+        for (int ie = 0; ie < n; ie++)
+            for (int ief = 0; ief < n; ief++) {
+                double dint = AT(d, ie, a, ief), dint2 = AT(d, ie, b, ief);
+                double dum = D3(s, ap, bp, d, ief, b, ie);
+                if (d == ief) dum = dum + 0.5 * D2(s, ap, bp, ie, b);
+                if (d == b) dum = dum + 0.5 * D2(s, ap, bp, ief, ie);
+                ip = ip - dint * dum;
+                dum = D3(s, ap, bp, d, a, ief, ie);
+                if (d == a) dum = dum + 0.5 * D2(s, ap, bp, ie, ief);
+                if (d == ief) dum = dum + 0.5 * D2(s, ap, bp, a, ie);
+                ip = ip - dint2 * dum;
+            }
+    }
+    double ea = 0.0;
+    for (int c = 0; c < n; c++) {
+        ea = ea + FF(a, c) * ro2t(R, s, ap, bp, c, b) + FF(b, c) * ro2t(R, s, ap, bp, a, c);
+        for (int d = 0; d < n; d++)
+            for (int ie = 0; ie < n; ie++) {
+                double dint = AT(c, ie, d, a);
+                double dum = -ro3t(R, s, ap, bp, ie, d, b, c);
+                if (c == ie) dum = dum + 2.0 * ro2t(R, s, ap, bp, d, b);
+                if (d == ie) dum = dum - 0.5 * ro2t(R, s, ap, bp, c, b);
+                if (b == ie) dum = dum - 0.5 * ro2t(R, s, ap, bp, d, c);
+                ea = ea + dint * dum;
+                dint = AT(c, ie, d, b);
+                dum = -ro3t(R, s, ap, bp, ie, a, d, c);
+                if (c == ie) dum = dum + 2.0 * ro2t(R, s, ap, bp, a, d);
+                if (a == ie) dum = dum - 0.5 * ro2t(R, s, ap, bp, c, d);
+                if (d == ie) dum = dum - 0.5 * ro2t(R, s, ap, bp, a, c);
+                ea = ea + dint * dum;
+            }
+    }
+    koopaa[idx] = ip;
+    koopeaa[idx] = ea;
+}
+
+// ---------------------------------------------------------------- koopman0pE (koop_matrices.F90:389-588)
+// thread index (s,bp,ap,a,b) matches the output layout koopaa(s,bp,ap,a,b)
+__global__ void k_koop0p(Rdm R, const double *atwo, const double *F, double *koopaa, double *koopbb) {
+    const int n = R.n, M = R.M;
+    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (i64)M * n * n * n * n) return;
+    int s, x[4];
+    decode(idx, M, n, 4, &s, x);
+    const int bp = x[0], ap = x[1], a = x[2], b = x[3];
+    double ka = 0.0;
+    for (int c = 0; c < n; c++) {
+        ka = ka + 2.0 * (FF(c, a) * ee2(R, s, bp, ap, c, b) - FF(b, c) * ee2(R, s, bp, ap, a, c));
+        for (int d = 0; d < n; d++)
+            for (int ie = 0; ie < n; ie++) {
+                double dum1 = eee2(R, s, bp, ap, c, ie, d, b) + eee2(R, s, bp, ap, d, b, c, ie);
+                double dum2 = eee2(R, s, bp, ap, a, ie, c, d) + eee2(R, s, bp, ap, c, d, a, ie);
+                ka = ka + AT(c, ie, d, a) * dum1 - AT(b, ie, c, d) * dum2;
+            }
+    }
+    double kb = 0.0;
+    for (int c = 0; c < n; c++) {
+        double dum1 = -ee2(R, s, a, ap, bp, c);
+        if (a == ap) dum1 = dum1 + 2.0 * D1(s, bp, c);
+        if (bp == ap) dum1 = dum1 + D1(s, a, c);
+        double dum2 = -ee2(R, s, c, ap, bp, b);
+        if (ap == c) dum2 = dum2 + 2.0 * D1(s, bp, b);
+        if (bp == ap) dum2 = dum2 + D1(s, c, b);
+        kb = kb - FF(b, c) * dum1 + FF(c, a) * dum2;
+        for (int d = 0; d < n; d++)
+            for (int ie = 0; ie < n; ie++) {
+                double e1 = -eee2(R, s, c, ie, a, ap, bp, d) - eee2(R, s, a, ap, bp, d, c, ie);
+                if (ap == a) e1 = e1 + 2.0 * (ee2(R, s, c, ie, bp, d) + ee2(R, s, bp, d, c, ie));
+                if (bp == ap) e1 = e1 + ee2(R, s, c, ie, a, d) + ee2(R, s, a, d, c, ie);
+                if (ap == c) {
+                    e1 = e1 - ee2(R, s, a, ie, bp, d);
+                    if (ie == a) e1 = e1 + 2.0 * D1(s, bp, d);
+                }
+                if (bp == ie) {
+                    e1 = e1 + ee2(R, s, a, ap, c, d);
+                    if (ap == a) e1 = e1 - 2.0 * D1(s, c, d);
+                }
+                double e2 = -eee2(R, s, c, ie, d, ap, bp, b) - eee2(R, s, d, ap, bp, b, c, ie);
+                if (ap == d) e2 = e2 + 2.0 * (ee2(R, s, c, ie, bp, b) + ee2(R, s, bp, b, c, ie));
+                if (bp == ap) e2 = e2 + ee2(R, s, c, ie, d, b) + ee2(R, s, d, b, c, ie);
+                if (ap == c) {
+                    e2 = e2 - ee2(R, s, d, ie, bp, b);
+                    if (d == ie) e2 = e2 + 2.0 * D1(s, bp, b);
+                }
+                if (bp == ie) {
+                    e2 = e2 + ee2(R, s, d, ap, c, b);
+                    if (d == ap) e2 = e2 - 2.0 * D1(s, c, b);
+                }
+                kb = kb - 0.5 * AT(c, ie, b, d) * e1 + 0.5 * AT(c, ie, d, a) * e2;
+            }
+    }
+    koopaa[idx] = ka;
+    koopbb[idx] = kb;
+}
+__global__ void k_koop0pf(Rdm R, const double *atwo, const double *F, double *koopab) {
+    const int n = R.n, M = R.M;
+    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (i64)M * n * n) return;
+    int s, x[2];
+    decode(idx, M, n, 2, &s, x);
+    const int a = x[0], b = x[1];
+    double v = 0.0;
+    for (int c = 0; c < n; c++) {
+        v = v + 2.0 * (FF(c, a) * D1(s, c, b) - FF(b, c) * D1(s, a, c));
+        for (int d = 0; d < n; d++)
+            for (int ie = 0; ie < n; ie++) {
+                double dum1 = ee2(R, s, c, ie, d, b) + ee2(R, s, d, b, c, ie);
+                double dum2 = ee2(R, s, a, ie, c, d) + ee2(R, s, c, d, a, ie);
+                v = v + AT(c, ie, d, a) * dum1 - AT(b, ie, c, d) * dum2;
+            }
+    }
+    koopab[idx] = v;
+}
+
+// ---------------------------------------------------------------- bbmat (koopro4QD.F90:4231-4299); F alpha=1
+__global__ void k_bbmat(Rdm R, const double *atwo, const double *F, const double *AJ, double *bmat, double *btmat) {
+    const int n = R.n, M = R.M;
+    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (i64)M * n * n * n * n) return;
+    int s, x[4];
+    decode(idx, M, n, 4, &s, x);
+    const int ap = x[0], bp = x[1], cp = x[2], a = x[3];
+    double bv = 0.0, bt = 0.0;
+    for (int c = 0; c < n; c++) {
+        bv = bv - FF(c, a) * ee2(R, s, cp, ap, bp, c);
+        bt = bt + AJ[c + n * a] * ee2tr(R, s, cp, ap, bp, c);
+        for (int ie = 0; ie < n; ie++)
+            for (int ief = 0; ief < n; ief++) {
+                double dint = AT(a, ie, c, ief);
+                bv = bv - eee2(R, s, cp, ap, bp, ie, c, ief) * dint;
+                bt = bt + eee2t(R, s, cp, ap, bp, ie, c, ief) * dint;
+            }
+    }
+    bmat[idx] = bv;
+    btmat[idx] = bt;
+}
+
+// ---------------------------------------------------------------- bcmat (koopro4QD.F90:4301-4397); F alpha=1/2
+__global__ void k_bcmat(Rdm R, const double *atwo, const double *F, double *cmat, double *ctmat) {
+    const int n = R.n, M = R.M;
+    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (i64)M * n * n * n * n) return;
+    int s, x[4];
+    decode(idx, M, n, 4, &s, x);
+    const int ap = x[0], a = x[1], b = x[2], c = x[3];
+    double cv = 0.0, ct = 0.0;
+    for (int d = 0; d < n; d++) {
+        double fda = FF(d, a), fcd = FF(c, d), fbd = FF(b, d);
+        cv = cv + fda * ee2(R, s, ap, b, d, c) - fcd * ee2(R, s, ap, b, a, d) - fbd * ee2(R, s, ap, d, a, c);
+        ct = ct + fda * ee2t(R, s, ap, b, d, c) - fcd * ee2t(R, s, ap, b, a, d) + fbd * ee2t(R, s, ap, d, a, c);
+        for (int ie = 0; ie < n; ie++) {
+            double dint1 = AT(c, ie, d, a), dint2 = AT(d, ie, ie, a), dint3 = AT(c, d, d, ie), dint4 = AT(b, d, d, ie);
+            double dum1 = ee2(R, s, ap, b, d, ie), dum2 = ee2(R, s, ap, b, d, c);
+            double dum3 = ee2(R, s, ap, b, a, ie), dum4 = ee2(R, s, ap, ie, a, c);
+            double dum1t = ee2t(R, s, ap, b, d, ie), dum2t = ee2t(R, s, ap, b, d, c);
+            double dum3t = ee2t(R, s, ap, b, a, ie), dum4t = ee2t(R, s, ap, ie, a, c);
+            cv = cv + dum1 * dint1 - 0.5 * (dum2 * dint2 + dum3 * dint3 - dum4 * dint4);
+            ct = ct + dum1t * dint1 - 0.5 * (dum2t * dint2 + dum3t * dint3 - dum4t * dint4);
+            for (int ief = 0; ief < n; ief++) {
+                double di1 = AT(d, ief, ie, a), di2 = AT(d, ief, c, ie), di3 = AT(d, ief, b, ie);
+                double du1 = eee2(R, s, ap, b, d, ief, ie, c), du1t = eee2tl(R, s, ap, b, d, ief, ie, c);
+                double du2 = eee2(R, s, ap, b, d, ief, a, ie), du2t = eee2tl(R, s, ap, b, d, ief, a, ie);
+                double du3 = eee2(R, s, ap, ie, d, ief, a, c), du3t = eee2tl(R, s, ap, ie, d, ief, a, c);
+                cv = cv + du1 * di1 - du2 * di2 - du3 * di3;
+                ct = ct + du1t * di1 - du2t * di2 + du3t * di3;
+            }
+        }
+    }
+    cmat[idx] = cv;
+    ctmat[idx] = ct;
+}
+
+// ---------------------------------------------------------------- bdmat (koopro4QD.F90:4399-4459); F alpha=1
+__global__ void k_bdmat(Rdm R, const double *atwo, const double *F, const double *AJ, double *dmat, double *dtmat) {
+    const int n = R.n, M = R.M;
+    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (i64)M * n * n) return;
+    int s, x[2];
+    decode(idx, M, n, 2, &s, x);
+    const int ap = x[0], a = x[1];
+    double dv = 0.0, dt = 0.0;
+    for (int c = 0; c < n; c++) {
+        dv = dv - FF(c, a) * D1(s, ap, c);
+        double dum = -D1(s, ap, c);
+        if (ap == c) dum = dum + 2.0;
+        dt = dt + AJ[c + n * a] * dum;
+        for (int ie = 0; ie < n; ie++)
+            for (int ief = 0; ief < n; ief++) {
+                double dint = AT(a, ie, c, ief);
+                dv = dv - ee2(R, s, ap, ie, c, ief) * dint;
+                dt = dt + ee2t(R, s, ap, ie, c, ief) * dint;
+            }
+    }
+    dmat[idx] = dv;
+    dtmat[idx] = dt;
+}
+
+// ---------------------------------------------------------------- bamat: E4 = <E E E E> for one state
+// E4 is stored C-ordered over (x1..x8), x8 fastest: offset = sum_k x_k n^(8-k)
+template <typename T>
+__global__ void k_build_e4(Rdm R, int s, T *E4) {
+    const int n = R.n;
+    i64 n8 = (i64)n * n * n * n * n * n * n * n;
+    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n8) return;
+    int x[8];
+    i64 t = idx;
+    for (int k = 7; k >= 0; k--) { x[k] = (int)(t % n); t /= n; }
+    E4[idx] = (T)eeee(R, s, x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7]);
+}
+
+// lower-order part of bamat plus the combination with the four E4 contractions.
+// Contraction buffers Ck share amat's layout (state fastest, then ap,bp,cp,a,b,c):
+//   C1 = sum eeee(cp,ap,bp,b,d,f,i,c) w(d,f,i,a)    C2 = sum eeee(cp,ap,bp,b,d,f,a,i) w(d,f,c,i)
+//   C3 = sum eeee(cp,ap,bp,i,d,f,a,c) w(d,f,b,i)    C4 = sum eeee(cp,ap,i,bp,d,f,a,c) w(d,f,b,i)
+// amat = direct + C1 - C2 - C3 ; atmat = direct_t - C1[bp<->b] + C2[bp<->b] - C4
+__global__ void k_bamat_direct(Rdm R, const double *atwo, const double *F, const double *C1, const double *C2,
+                               const double *C3, const double *C4, double *amat, double *atmat) {
+    const int n = R.n, M = R.M;
+    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (i64)M * n * n * n * n * n * n) return;
+    int s, x[6];
+    decode(idx, M, n, 6, &s, x);
+    const int ap = x[0], bp = x[1], cp = x[2], a = x[3], b = x[4], c = x[5];
+    // index of the bp<->b swapped element
+    const i64 sw = s + (i64)M * (ap + n * (b + n * (cp + n * (a + n * (bp + (i64)n * c)))));
+    double av = 0.0, tv = 0.0;
+    for (int d = 0; d < n; d++) {
+        // f * <EEE> terms, own iteration (koopro4QD.F90:4018-4037)
+        av = av + FF(d, a) * eee2(R, s, cp, ap, bp, b, d, c) - FF(c, d) * eee2(R, s, cp, ap, bp, b, a, d)
+             - FF(b, d) * eee2(R, s, cp, ap, bp, d, a, c);
+        tv = tv + FF(d, a) * eee2t(R, s, cp, ap, bp, b, d, c) - FF(c, d) * eee2t(R, s, cp, ap, bp, b, a, d)
+             + FF(b, d) * eee2t(R, s, cp, ap, bp, d, a, c);
+        for (int ie = 0; ie < n; ie++) {
+            // 3-RDM x ERI terms (koopro4QD.F90:4038-4068)
+            double dint1 = AT(c, ie, d, a), dint2 = AT(d, ie, ie, a), dint3 = AT(c, d, d, ie), dint4 = AT(b, d, d, ie);
+            double dum1 = eee2(R, s, cp, ap, bp, b, d, ie), dum2 = eee2(R, s, cp, ap, bp, b, d, c);
+            double dum3 = eee2(R, s, cp, ap, bp, b, a, ie), dum4 = eee2(R, s, cp, ap, bp, ie, a, c);
+            av = av + dum1 * dint1 - 0.5 * (dum2 * dint2 + dum3 * dint3 - dum4 * dint4);
+            tv = tv + 0.5 * eee2t(R, s, cp, ap, bp, ie, a, c) * dint4;
+            // swapped iteration (its bp is our b, its b is our bp) writes atmat(ap,bp,cp,a,b,c)
+            double sd1 = -eee2(R, s, cp, ap, b, bp, d, ie), sd2 = -eee2(R, s, cp, ap, b, bp, d, c);
+            double sd3 = -eee2(R, s, cp, ap, b, bp, a, ie);
+            if (b == bp) {
+                sd1 = sd1 + 2.0 * ee2(R, s, cp, ap, d, ie);
+                sd2 = sd2 + 2.0 * ee2(R, s, cp, ap, d, c);
+                sd3 = sd3 + 2.0 * ee2(R, s, cp, ap, a, ie);
+            }
+            tv = tv + sd1 * dint1 - 0.5 * (sd2 * dint2 + sd3 * dint3);
+            // delta part of eeeet(cp,ap,bp,ie,...) at ie == bp (koopro4QD.F90:4088-4092)
+            if (ie == bp)
+                for (int ief = 0; ief < n; ief++) tv = tv + 2.0 * eee2(R, s, cp, ap, d, ief, a, c) * AT(d, ief, b, ie);
+            // delta parts of du1t/du2t for the swapped iteration (koopro4QD.F90:4074-4086)
+            if (b == bp)
+                for (int ief = 0; ief < n; ief++)
+                    tv = tv + 2.0 * eee2(R, s, cp, ap, d, ief, ie, c) * AT(d, ief, ie, a)
+                         - 2.0 * eee2(R, s, cp, ap, d, ief, a, ie) * AT(d, ief, c, ie);
+        }
+    }
+    amat[idx] = av + C1[idx] - C2[idx] - C3[idx];
+    atmat[idx] = tv - C1[sw] + C2[sw] - C4[idx];
+}
+
+// ---------------------------------------------------------------- host launchers
+static inline unsigned nblk(i64 tot, int bs) { return (unsigned)((tot + bs - 1) / bs); }
+
+int launch_bro(const Rdm &R, int nele, double *daaa, double *taa, double *dal, hipStream_t st) {
+    const i64 n = R.n, M = R.M;
+    hipLaunchKernelGGL(k_bro3, dim3(nblk(M * n * n * n * n * n * n, 256)), dim3(256), 0, st, R, nele, daaa);
+    NV2_LAUNCH_CHECK();
+    Rdm R3 = R; R3.d3 = daaa;
+    hipLaunchKernelGGL(k_bro2, dim3(nblk(M * n * n * n * n, 256)), dim3(256), 0, st, R3, nele, taa);
+    NV2_LAUNCH_CHECK();
+    Rdm R2 = R3; R2.d2 = taa;
+    hipLaunchKernelGGL(k_bro1, dim3(nblk(M * n * n, 256)), dim3(256), 0, st, R2, nele, dal);
+    NV2_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_aj_atwo(int n, int ncore, int norb, int L, const double *B, const double *oneint, double *aj,
+                   double *atwo, hipStream_t st) {
+    NV2_HIP(hipMemcpyAsync(aj, oneint, sizeof(double) * (i64)norb * (norb + 1) / 2, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_aj, dim3(n), dim3(64), 0, st, n, ncore, L, B, aj);
+    NV2_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_atwo, dim3(nblk((i64)n * n * n * n, 256)), dim3(256), 0, st, n, ncore, L, B, atwo);
+    NV2_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_make_f(int n, int ncore, const double *aj, const double *atwo, double alpha, double *F, hipStream_t st) {
+    hipLaunchKernelGGL(k_make_f, dim3(n), dim3(64), 0, st, n, ncore, aj, atwo, alpha, F);
+    NV2_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_koop_small(const Rdm &R, int ncore, const double *aj, const double *atwo, Stage1Scratch &w,
+                      Stage1Out &o, hipStream_t st) {
+    const i64 n = R.n, M = R.M;
+    const i64 t2 = M * n * n, t4 = M * n * n * n * n;
+    hipLaunchKernelGGL(k_aj_active, dim3(n), dim3(64), 0, st, (int)n, ncore, aj, w.AJ);
+    NV2_LAUNCH_CHECK();
+    // koopE with f = aj
+    hipLaunchKernelGGL(k_koopE, dim3(nblk(t2, 256)), dim3(256), 0, st, R, atwo, w.AJ, o.ipko1, o.eako1);
+    NV2_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_hrdm, dim3(nblk(t4, 256)), dim3(256), 0, st, R, o.hrdm);
+    NV2_LAUNCH_CHECK();
+    if (launch_make_f((int)n, ncore, aj, atwo, 0.5, w.Fhalf, st)) return -1;
+    if (launch_make_f((int)n, ncore, aj, atwo, 1.0, w.Fone, st)) return -1;
+    hipLaunchKernelGGL(k_koop2E, dim3(nblk(t4, 64)), dim3(64), 0, st, R, atwo, w.Fhalf, o.ipko2, o.eako2);
+    NV2_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_koop0p, dim3(nblk(t4, 64)), dim3(64), 0, st, R, atwo, w.Fhalf, o.gk0pa, o.gk0pd);
+    NV2_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_koop0pf, dim3(nblk(t2, 64)), dim3(64), 0, st, R, atwo, w.Fhalf, o.gk0pf);
+    NV2_LAUNCH_CHECK();
+    if (o.bmat) {
+        hipLaunchKernelGGL(k_bbmat, dim3(nblk(t4, 64)), dim3(64), 0, st, R, atwo, w.Fone, w.AJ, o.bmat, o.btmat);
+        NV2_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_bcmat, dim3(nblk(t4, 64)), dim3(64), 0, st, R, atwo, w.Fhalf, o.cmat, o.ctmat);
+        NV2_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_bdmat, dim3(nblk(t2, 64)), dim3(64), 0, st, R, atwo, w.Fone, w.AJ, o.dmat, o.dtmat);
+        NV2_LAUNCH_CHECK();
+    }
+    return 0;
+}
+
+int launch_build_e4(const Rdm &R, int s, void *E4, int prec, hipStream_t st) {
+    const i64 n = R.n, n8 = n * n * n * n * n * n * n * n;
+    if (prec == 0)
+        hipLaunchKernelGGL(k_build_e4<double>, dim3(nblk(n8, 256)), dim3(256), 0, st, R, s, (double *)E4);
+    else
+        hipLaunchKernelGGL(k_build_e4<float>, dim3(nblk(n8, 256)), dim3(256), 0, st, R, s, (float *)E4);
+    NV2_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_bamat_direct(const Rdm &R, const double *atwo, const double *Fhalf, const double *C1, const double *C2,
+                        const double *C3, const double *C4, double *amat, double *atmat, hipStream_t st) {
+    const i64 n = R.n, M = R.M, t6 = M * n * n * n * n * n * n;
+    hipLaunchKernelGGL(k_bamat_direct, dim3(nblk(t6, 64)), dim3(64), 0, st, R, atwo, Fhalf, C1, C2, C3, C4, amat,
+                       atmat);
+    NV2_LAUNCH_CHECK();
+    return 0;
+}
+
+}  // namespace nv2
