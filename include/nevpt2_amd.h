@@ -60,6 +60,42 @@ int nv2_bamat(int64_t nact, int64_t ncore, int64_t norb, int64_t metat, const do
               const double *daaa, const double *taa, const double *dal, const double *aj, double *amat,
               double *atmat, int32_t prec);
 
+/* Second-order energies of one state: the canonical core/virtual orbitals, the
+ * Cholesky transform and the eight perturber classes in SC and PC form.
+ * Replaces the per-state body of qdnevpt (qdnevpt.F:369-784, 858-993) with the
+ * class drivers v0, v1k, vm1k, v2mod, vm2mod, v0pmod, vmupE, vpupE
+ * (qdnevpt.F:1661-6445).  Koopmans-matrix arguments are the stage-1 outputs
+ * (nevpt.h5 datasets of the same names); the eight A-D matrices may be NULL,
+ * which skips the (+-1)' classes as .NO4RDM does.  out[4*k + {0,1,2,3}] =
+ * (norm_sc, e_sc, norm_pc, e_pc) of class k in the order (0),(+1),(-1),(+2),
+ * (-2),(+1)',(-1)',(0)' -- the columns of the reference's result table
+ * (qdnevpt.F:1019-1045).  istate is 0-based. */
+int nv2_qdnevpt_state(int64_t ncore, int64_t nact, int64_t nvirt, int64_t metat, int64_t nchovec,
+                      const double *cholesky, const double *oneint, const double *dal, const double *taa,
+                      const double *daaa, const double *ipko1, const double *eako1, const double *hrdm,
+                      const double *ipko2, const double *eako2, const double *gk0pa, const double *gk0pd,
+                      const double *gk0pf, const double *amatIP, const double *bmatIP, const double *cmatIP,
+                      const double *dmatIP, const double *amatEA, const double *bmatEA, const double *cmatEA,
+                      const double *dmatEA, int64_t istate, double thr_lindep, double *out);
+
+/* Resident evaluation context (no counterpart routine in the reference: it
+ * replaces the koopro4QD.x -> nevpt.h5 -> QDnevpt2.x file pipeline,
+ * koopro4QD_main.F90 / qdnevpt_main.F90, with device-resident hand-over).
+ * Dimensions are fixed at creation; the packed 4-RDM and the integrals are
+ * uploaded once and stay in HBM.  nv2_ctx_run is one full evaluation: partial
+ * traces, every Koopmans matrix, and the SC/PC class energies of every state,
+ * out[32*s + 4*k + j] as in nv2_qdnevpt_state.  nv2_ctx_get copies any
+ * resident array back by its nevpt.h5 dataset name ("amatIP", "3-RDM", ...). */
+#define NV2_NPHASE 6   /* bro, koop<=3-RDM, E4 build, E4 contractions, bamat rest, stage 2 */
+typedef struct nv2_ctx nv2_ctx;
+nv2_ctx *nv2_ctx_create(int64_t ncore, int64_t nact, int64_t nvirt, int64_t metat, int64_t nele, int64_t nchovec);
+int nv2_ctx_set_integrals(nv2_ctx *ctx, const double *cholesky, const double *oneint);
+int nv2_ctx_set_ro4(nv2_ctx *ctx, const double *ro4);
+int nv2_ctx_run(nv2_ctx *ctx, int32_t prec, double thr_lindep, double *out);
+int nv2_ctx_timings(nv2_ctx *ctx, double *ms);
+int nv2_ctx_get(nv2_ctx *ctx, const char *name, double *host, int64_t count);
+void nv2_ctx_destroy(nv2_ctx *ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,8 +44,16 @@ def _declare(L):
     L.nv2_get_aj_atwo.argtypes = [i64, i64, i64, i64, dp, dp, dp, dp]
     L.nv2_koop_matrices.argtypes = [i64, i64, i64, i64] + [dp] * 19
     L.nv2_bamat.argtypes = [i64, i64, i64, i64] + [dp] * 8 + [i32]
-    for name in dir(L):
-        pass
+    L.nv2_qdnevpt_state.argtypes = [i64] * 5 + [dp] * 21 + [i64, C.c_double, dp]
+    L.nv2_ctx_create.restype = C.c_void_p
+    L.nv2_ctx_create.argtypes = [i64] * 6
+    L.nv2_ctx_set_integrals.argtypes = [C.c_void_p, dp, dp]
+    L.nv2_ctx_set_ro4.argtypes = [C.c_void_p, dp]
+    L.nv2_ctx_run.argtypes = [C.c_void_p, i32, C.c_double, dp]
+    L.nv2_ctx_timings.argtypes = [C.c_void_p, dp]
+    L.nv2_ctx_get.argtypes = [C.c_void_p, C.c_char_p, dp, i64]
+    L.nv2_ctx_destroy.argtypes = [C.c_void_p]
+    L.nv2_ctx_destroy.restype = None
 
 
 def last_error() -> str:

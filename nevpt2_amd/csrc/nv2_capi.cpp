@@ -10,6 +10,7 @@
 
 #include "nv2_common.h"
 #include "nv2_stage1.h"
+#include "nv2_stage2.h"
 #include "../../include/nevpt2_amd.h"
 
 namespace nv2 {
@@ -180,6 +181,38 @@ int nv2_bamat(int64_t nact, int64_t ncore, int64_t norb, int64_t metat, const do
     if (rc) return rc;
     TRY(dam.down(amat)); TRY(dat6.down(atmat));
     return 0;
+}
+
+int nv2_qdnevpt_state(int64_t ncore, int64_t nact, int64_t nvirt, int64_t metat, int64_t nchovec,
+                      const double *cholesky, const double *oneint, const double *dal, const double *taa,
+                      const double *daaa, const double *ipko1, const double *eako1, const double *hrdm,
+                      const double *ipko2, const double *eako2, const double *gk0pa, const double *gk0pd,
+                      const double *gk0pf, const double *amatIP, const double *bmatIP, const double *cmatIP,
+                      const double *dmatIP, const double *amatEA, const double *bmatEA, const double *cmatEA,
+                      const double *dmatEA, int64_t istate, double thr_lindep, double *out) {
+    TRY(check_dims(nact, metat));
+    if (istate < 0 || istate >= metat) { set_error("istate %lld out of range", (long long)istate); return -2; }
+    const i64 n = nact, M = metat, norb = ncore + nact + nvirt, np = norb * (norb + 1) / 2;
+    const size_t s2 = sizeof(double) * M * n * n, s4 = sizeof(double) * M * p(n, 4), s6 = sizeof(double) * M * p(n, 6);
+    DBuf dB, d1e, dd1, dd2, dd3, k[16];
+    TRY(dB.up(cholesky, sizeof(double) * nchovec * np));
+    TRY(d1e.up(oneint, sizeof(double) * np));
+    TRY(dd1.up(dal, s2)); TRY(dd2.up(taa, s4)); TRY(dd3.up(daaa, s6));
+    const double *hk[16] = {ipko1, eako1, hrdm, ipko2, eako2, gk0pa, gk0pd, gk0pf,
+                            amatIP, bmatIP, cmatIP, dmatIP, amatEA, bmatEA, cmatEA, dmatEA};
+    size_t sz[16] = {s2, s2, s4, s4, s4, s4, s4, s2, s6, s4, s4, s2, s6, s4, s4, s2};
+    for (int q = 0; q < 16; q++) if (hk[q]) TRY(k[q].up(hk[q], sz[q]));
+    S2Inputs in{(int)ncore, (int)n, (int)nvirt, (int)nchovec, (int)M, dB.d(), d1e.d(), dd1.d(), dd2.d(), dd3.d(),
+                k[0].d(), k[1].d(), k[2].d(), k[3].d(), k[4].d(), k[5].d(), k[6].d(), k[7].d(),
+                amatIP ? k[8].d() : nullptr, k[9].d(), k[10].d(), k[11].d(),
+                amatEA ? k[12].d() : nullptr, k[13].d(), k[14].d(), k[15].d()};
+    rocblas_handle h;
+    if (rocblas_create_handle(&h) != rocblas_status_success) { set_error("rocblas_create_handle failed"); return -1; }
+    Stage2 *st2 = stage2_create(in, h, 0);
+    int rc = st2 ? stage2_state(st2, (int)istate, thr_lindep, 0, out, nullptr) : -1;
+    stage2_destroy(st2);
+    rocblas_destroy_handle(h);
+    return rc;
 }
 
 }  // extern "C"

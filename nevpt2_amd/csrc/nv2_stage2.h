@@ -1,0 +1,31 @@
+// Internal stage-2 interfaces: canonicalisation, Cholesky integral assembly and
+// the eight perturber classes (SC and PC) for one state, all on the device.
+#pragma once
+#include <rocblas/rocblas.h>
+#include "nv2_common.h"
+
+namespace nv2 {
+
+struct S2Inputs {             // device pointers, reference layouts (state index fastest)
+    int nc, n, nv, L, M;
+    const double *B;          // cholesky(L, norb(norb+1)/2)
+    const double *oneint;     // norb(norb+1)/2
+    const double *dal, *taa, *daaa;
+    const double *ipko1, *eako1, *hrdm, *ipko2, *eako2, *gk0pa, *gk0pd, *gk0pf;
+    const double *amatIP, *bmatIP, *cmatIP, *dmatIP, *amatEA, *bmatEA, *cmatEA, *dmatEA;  // may be null (no 4-RDM)
+};
+
+struct S2Stats {              // diagnostics for threshold-sensitive seeds
+    double min_metric_eig[8];
+    int nnew[8];
+    double eps_min_virt, eps_max_core;
+};
+
+struct Stage2;
+Stage2 *stage2_create(const S2Inputs &in, rocblas_handle h, hipStream_t st);
+void stage2_destroy(Stage2 *p);
+// out[k*4 + {0,1,2,3}] = (norm_sc, e_sc, norm_pc, e_pc) for class k in the reference order
+// (0),(+1),(-1),(+2),(-2),(+1)',(-1)',(0)'
+int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *stats);
+
+}  // namespace nv2

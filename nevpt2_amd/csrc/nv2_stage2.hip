@@ -1,0 +1,1233 @@
+// Stage 2 of the NEVPT2 hot path on gfx950: canonical core/virtual orbitals,
+// Cholesky integral assembly, and the eight perturber classes in SC and PC
+// form for one state (qdnevpt.F:369-784, 858-993, 1661-6445, 7599-8086).
+//
+// Plain library GEMMs (the Cholesky transforms and integral blocks, the PC
+// metric transforms and the n^9 S-builds of the (+-1)' classes) go to rocBLAS;
+// the metric/Koopmans eigen-solves go to rocSOLVER syevd with the reference's
+// triangle choices.  Every denominator-weighted energy sum is a hand-written
+// kernel that reads the resident integral blocks once and reduces per block;
+// the block partials are summed on the device with compensated (Neumaier)
+// FP64 accumulation.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+#include <cmath>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "nv2_common.h"
+#include "nv2_stage2.h"
+
+namespace nv2 {
+
+#define RB(call)                                                                          \
+    do {                                                                                  \
+        rocblas_status _s = (call);                                                       \
+        if (_s != rocblas_status_success) {                                               \
+            set_error("%s:%d %s: rocblas status %d", __FILE__, __LINE__, #call, (int)_s); \
+            return -1;                                                                    \
+        }                                                                                 \
+    } while (0)
+#define TRYS(x) do { int _r = (x); if (_r) return _r; } while (0)
+
+static inline unsigned nb(i64 tot, int bs) { return (unsigned)((tot + bs - 1) / bs); }
+
+// ------------------------------------------------------------------ reductions
+__device__ __forceinline__ double wsum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+// block sum of four values; thread 0 writes part[4*block + k]
+__device__ void block_store4(double a, double b, double c, double d, double *part) {
+    __shared__ double sh[4][16];
+    a = wsum(a); b = wsum(b); c = wsum(c); d = wsum(d);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if (lane == 0) { sh[0][wid] = a; sh[1][wid] = b; sh[2][wid] = c; sh[3][wid] = d; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const i64 bid = (i64)blockIdx.x + (i64)gridDim.x * blockIdx.y;
+        for (int k = 0; k < 4; k++) {
+            double s = 0.0;
+            for (int w = 0; w < nw; w++) s += sh[k][w];
+            part[4 * bid + k] = s;
+        }
+    }
+}
+__device__ double block_sum(double v, double *sh16) {
+    v = wsum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    __syncthreads();
+    if (lane == 0) sh16[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int w = 0; w < nw; w++) s += sh16[w];
+    return s;
+}
+// Neumaier-compensated sum of the block partials; out[k] += sum_b part[4b+k]
+__global__ void k_reduce_parts(const double *part, i64 nparts, double *out) {
+    const int k = blockIdx.x;
+    double s = 0.0, c = 0.0;
+    for (i64 i = threadIdx.x; i < nparts; i += blockDim.x) {
+        double x = part[4 * i + k], t = s + x;
+        c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
+        s = t;
+    }
+    __shared__ double ss[256], cc[256];
+    ss[threadIdx.x] = s; cc[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double S = 0.0, Cc = 0.0;
+        for (int t = 0; t < (int)blockDim.x; t++) {
+            double x = ss[t], u = S + x;
+            Cc += (fabs(S) >= fabs(x)) ? (S - u) + x : (x - u) + S;
+            S = u;
+            Cc += cc[t];
+        }
+        out[k] += S + Cc;
+    }
+}
+
+// ------------------------------------------------------------------ small utility kernels
+__global__ void k_slice(const double *src, int M, int s, i64 cnt, double *dst) {
+    i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) dst[i] = src[s + (i64)M * i];
+}
+// out[l + L*(p + np*q)] = B[l, tri(p0+p, q0+q)]
+__global__ void k_unpack(const double *B, int L, int p0, int np, int q0, int nq, double *out) {
+    i64 tot = (i64)L * np * nq;
+    i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= tot) return;
+    int l = i % L;
+    i64 r = i / L;
+    int p = r % np, q = r / np;
+    out[i] = B[l + (i64)L * tri(p0 + p, q0 + q)];
+}
+__global__ void k_unpack_h(const double *oneint, int norb, double *h) {
+    i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (i64)norb * norb) return;
+    int p = i % norb, q = i / norb;
+    h[i] = oneint[tri(p, q)];
+}
+// Jcore[l] = sum_k B[l,tri(k,k)] ; Jact[l] = sum_ab B[l,tri(a,b)] da(b,a)
+__global__ void k_jvec(const double *B, int L, int nc, int n, const double *da, double *Jc, double *Ja) {
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= L) return;
+    double jc = 0.0, ja = 0.0;
+    for (int k = 0; k < nc; k++) jc += B[l + (i64)L * tri(k, k)];
+    for (int a = 0; a < n; a++)
+        for (int b = 0; b < n; b++) ja += B[l + (i64)L * tri(nc + a, nc + b)] * da[b + n * a];
+    if (Jc) Jc[l] = jc;
+    Ja[l] = ja;
+}
+// F[p,q] += J . B[:, tri(p0+p, p0+q)] (one wavefront per element)
+__global__ void k_coul(const double *B, int L, int p0, int np, const double *J, double *F) {
+    i64 e = (i64)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (e >= (i64)np * np) return;
+    int p = e % np, q = e / np, lane = threadIdx.x & 63;
+    const double *col = B + (i64)L * tri(p0 + p, p0 + q);
+    double s = 0.0;
+    for (int l = lane; l < L; l += 64) s += col[l] * J[l];
+    s = wsum(s);
+    if (lane == 0) F[e] += s;
+}
+__global__ void k_jdiag(const double *Tcc, int L, int nc, double *Jc) {
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= L) return;
+    double s = 0.0;
+    for (int j = 0; j < nc; j++) s += Tcc[l + (i64)L * (j + (i64)nc * j)];
+    Jc[l] = s;
+}
+// caux[:,j] = U[:, first+j] / sqrt(w[first+j])
+__global__ void k_caux(const double *U, const double *w, int N, int first, int nnew, double *caux) {
+    i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (i64)N * nnew) return;
+    int r = i % N, j = i / N;
+    caux[i] = U[r + (i64)N * (first + j)] / sqrt(w[first + j]);
+}
+// symmetric matrix from the upper triangle (row <= col), as dsymm('L','U') reads it
+__global__ void k_sym_upper(const double *K, int N, double *Ks) {
+    i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (i64)N * N) return;
+    int r = i % N, c = i / N;
+    Ks[i] = (r <= c) ? K[i] : K[c + (i64)N * r];
+}
+__global__ void k_fill_diag_minus(const double *da, int n, double two, double *S) {   // S = two*I - da (or da)
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * n) return;
+    int r = i % n, c = i / n;
+    S[i] = (two != 0.0) ? ((r == c ? two : 0.0) - da[i]) : da[i];
+}
+// out[c + cols*r] = in[r + rows*c]
+__global__ void k_transpose(const double *in, int rows, int cols, double *out) {
+    i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (i64)rows * cols) return;
+    int r = i % rows, c = i / rows;
+    out[c + (i64)cols * r] = in[i];
+}
+__global__ void k_scale(const double *src, i64 cnt, double f, double *dst) {
+    i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) dst[i] = f * src[i];
+}
+
+// ------------------------------------------------------------------ stage-2 accessors on per-state slices
+struct Sl {           // da(n,n), taa(n^4), ro3(n^6) column-major
+    int n;
+    const double *da, *taa, *ro3;
+};
+__device__ __forceinline__ double DA(const Sl &S, int p, int q) { return S.da[p + S.n * q]; }
+__device__ __forceinline__ double TA(const Sl &S, int p, int q, int r, int t) {
+    const i64 n = S.n;
+    return S.taa[p + n * (q + n * (r + n * t))];
+}
+__device__ __forceinline__ double R3(const Sl &S, int p, int q, int r, int t, int u, int v) {
+    const i64 n = S.n;
+    return S.ro3[p + n * (q + n * (r + n * (t + n * (u + n * v))))];
+}
+// qdnevpt.F:6462-6469
+__device__ __forceinline__ double s_ee2(const Sl &S, int a, int b, int c, int d) {
+    double v = TA(S, b, d, a, c);
+    if (b == c) v += DA(S, d, a);
+    return v;
+}
+// qdnevpt.F:6490-6497
+__device__ __forceinline__ double s_ee2tr(const Sl &S, int a, int b, int c, int d) {
+    double v = -s_ee2(S, a, b, d, c);
+    if (c == d) v += 2.0 * DA(S, b, a);
+    return v;
+}
+// qdnevpt.F:6450-6460
+__device__ __forceinline__ double s_eee2(const Sl &S, int a, int b, int c, int d, int e, int f) {
+    double v = R3(S, b, d, f, a, c, e);
+    if (e == d) v += TA(S, b, f, a, c);
+    if (e == b) v += TA(S, f, d, a, c);
+    if (b == c) v += TA(S, d, f, a, e);
+    if (b == c && e == d) v += DA(S, f, a);
+    return v;
+}
+// qdnevpt.F:6480-6488
+__device__ __forceinline__ double s_eee2t(const Sl &S, int a, int b, int c, int d, int e, int f) {
+    double v = -s_eee2(S, a, b, d, c, e, f);
+    if (c == d) v += 2.0 * s_ee2(S, a, b, e, f);
+    return v;
+}
+
+// ------------------------------------------------------------------ class (0) (qdnevpt.F:1661-1773)
+__global__ void k_v0(const double *I, i64 ld, const int2 *pairs, int nc, int nv, const double *ec, const double *ev,
+                     double *part) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int2 pr = pairs[blockIdx.y];
+    const int ii = pr.x, ij = pr.y;
+    double E = 0.0, P = 0.0;
+    if (r < nv) {
+        const double dij = ec[ii] + ec[ij];
+        const bool zij = ii == ij;
+        for (int s = r; s < nv; s++) {
+            // traint(r,s) = (ij r | s ii)
+            double ab1 = I[(ij + (i64)nc * r) + ld * (ii + (i64)nc * s)];
+            double dint;
+            const bool zrs = r == s;
+            if (zrs && zij) dint = ab1;
+            else if (zrs != zij) dint = sqrt(2.0) * ab1;
+            else {
+                double ab2 = I[(ij + (i64)nc * s) + ld * (ii + (i64)nc * r)];
+                dint = 2.0 * sqrt(ab1 * ab1 + ab2 * ab2 - ab1 * ab2);
+            }
+            double co = -dint / (ev[r] + ev[s] - dij);
+            E += dint * co;
+            P += co * co;
+        }
+    }
+    block_store4(E, P, E, P, part);
+}
+
+// ------------------------------------------------------------------ class (+1) (qdnevpt.F:1775-1934, 3076-3254)
+template <int NM>
+__global__ void k_v1(const double *I, i64 ld, const int2 *pairs, int nc, int n, int nv, const double *ec,
+                     const double *ev, const double *gk, const double *ro, const double *spc, const double *w, int nnew,
+                     double *part) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int2 pr = pairs[blockIdx.y];
+    const int ii = pr.x, ij = pr.y;
+    double Esc = 0, Psc = 0, Epc = 0, Ppc = 0;
+    if (r < nv) {
+        double Tj[NM], Tk[NM];
+#pragma unroll
+        for (int a = 0; a < NM; a++) {
+            Tj[a] = a < n ? I[(ii + (i64)nc * a) + ld * (ij + (i64)nc * r)] : 0.0;     // (a ii | r ij)
+            Tk[a] = a < n ? I[(ij + (i64)nc * a) + ld * (ii + (i64)nc * r)] : 0.0;     // (a ij | r ii)
+        }
+        const bool same = ii == ij;
+        double dn = 0.0, ek = 0.0;
+#pragma unroll
+        for (int a = 0; a < NM; a++) {
+            if (a >= n) break;
+#pragma unroll
+            for (int b = 0; b < NM; b++) {
+                if (b >= n) break;
+                double cint = same ? 0.5 * Tj[a] * Tj[b]
+                                   : Tj[a] * Tj[b] + Tk[a] * Tk[b] - 0.5 * Tj[a] * Tk[b] - 0.5 * Tk[a] * Tj[b];
+                dn += 2.0 * ro[a + n * b] * cint;
+                ek += gk[a + n * b] * cint;
+            }
+        }
+        if (fabs(dn) > 0.0) {
+            double den = ec[ii] + ec[ij] - ev[r] - ek / dn;
+            double co = dn / den;
+            Esc += co;
+            Psc += co / den;
+        }
+        for (int mu = 0; mu < nnew; mu++) {
+            double rj = 0.0, rjp = 0.0;
+#pragma unroll
+            for (int a = 0; a < NM; a++) {
+                if (a >= n) break;
+                double s = spc[a + n * mu];
+                rj += (same ? Tj[a] : Tj[a] + Tk[a]) * s;
+                rjp += (Tj[a] - Tk[a]) * s;
+            }
+            double deno = w[mu] + ev[r] - ec[ii] - ec[ij];
+            double contri = same ? rj * rj / deno : (0.5 * rj * rj + 1.5 * rjp * rjp) / deno;
+            Epc -= contri;
+            Ppc += contri / deno;
+        }
+    }
+    block_store4(Esc, Psc, Epc, Ppc, part);
+}
+
+// ------------------------------------------------------------------ class (-1) (qdnevpt.F:2095-2262, 2400-2561)
+template <int NM>
+__global__ void k_vm1(const double *I, i64 ld, const int2 *pairs, i64 npairs, int nc, int n, const double *ec,
+                      const double *ev, const double *da, const double *gk, const double *spc, const double *w,
+                      int nnew, double *part) {
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    double Esc = 0, Psc = 0, Epc = 0, Ppc = 0;
+    if (t < npairs * nc) {
+        const int i = t % nc;
+        const int2 pr = pairs[t / nc];
+        const int ir = pr.x, is = pr.y;
+        double Tj[NM], Tk[NM];
+#pragma unroll
+        for (int a = 0; a < NM; a++) {
+            Tj[a] = a < n ? I[(i + (i64)nc * ir) + ld * (a + (i64)n * is)] : 0.0;    // (r i | s a)
+            Tk[a] = a < n ? I[(i + (i64)nc * is) + ld * (a + (i64)n * ir)] : 0.0;    // (s i | r a)
+        }
+        const bool same = ir == is;
+        double dn = 0.0, ek = 0.0;
+#pragma unroll
+        for (int a = 0; a < NM; a++) {
+            if (a >= n) break;
+#pragma unroll
+            for (int b = 0; b < NM; b++) {
+                if (b >= n) break;
+                double cint = same ? 0.5 * Tj[a] * Tj[b]
+                                   : Tj[a] * Tj[b] + Tk[a] * Tk[b] - 0.5 * Tj[a] * Tk[b] - 0.5 * Tk[a] * Tj[b];
+                dn += 2.0 * da[a + n * b] * cint;
+                ek += gk[a + n * b] * cint;
+            }
+        }
+        if (fabs(dn) >= 1.0e-14) {
+            double den = -ek / dn + ec[i] - ev[ir] - ev[is];
+            double co = dn / den;
+            Esc += co;
+            Psc += co / den;
+        }
+        for (int mu = 0; mu < nnew; mu++) {
+            double r1 = 0.0, r2 = 0.0;
+#pragma unroll
+            for (int a = 0; a < NM; a++) {
+                if (a >= n) break;
+                double s = spc[a + n * mu];
+                r1 += (same ? Tj[a] : Tj[a] + Tk[a]) * s;
+                r2 += (Tj[a] - Tk[a]) * s;
+            }
+            double deno = w[mu] + ev[ir] + ev[is] - ec[i];
+            double contri = same ? r1 * r1 / deno : (0.5 * r1 * r1 + 1.5 * r2 * r2) / deno;
+            Epc -= contri;
+            Ppc += contri / deno;
+        }
+    }
+    block_store4(Esc, Psc, Epc, Ppc, part);
+}
+
+// ------------------------------------------------------------------ classes (+2)/(-2), SC (qdnevpt.F:3327-3445, 3740-3849)
+// one workgroup per external pair; T(c,d) is the pair's integral block, X the metric RDM and K the Koopmans matrix:
+//   dn = sum cint*X, dk = sum cint*K, cint = T(c',d')T(c,d) [x 1/4 and symmetrised when the pair is diagonal]
+__global__ void k_pm2_sc(const double *I, i64 ld, const int2 *pairs, int mode, int nc, int n, const double *X,
+                         const double *K, const double *eo, double *part) {
+    // mode 0: (+2), pair (i<=j) core, T(a,b) = (i a | j b) from I_caca ; mode 1: (-2), pair (r<=s) virt,
+    // T(a,b) = (s a | r b) from I_avav
+    __shared__ double T[256];
+    __shared__ double sh[16];
+    const int2 pr = pairs[blockIdx.x];
+    const int x = pr.x, y = pr.y;
+    const int nn = n * n;
+    for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+        int a = e % n, b = e / n;
+        T[e] = mode == 0 ? I[(x + (i64)nc * a) + ld * (y + (i64)nc * b)] : I[(a + (i64)n * y) + ld * (b + (i64)n * x)];
+    }
+    __syncthreads();
+    const bool same = x == y;
+    double dn = 0.0, dk = 0.0;
+    for (int e = threadIdx.x; e < nn; e += blockDim.x) {      // (c',d') row
+        const int C = e % n, D = e / n;
+        double sx = 0.0, sk = 0.0;
+        for (int f = 0; f < nn; f++) {
+            const int c = f % n, d = f / n;
+            const i64 o = C + (i64)n * (D + (i64)n * (c + (i64)n * d));
+            double xv = X[o], kv = K[o];
+            if (same) {
+                xv += X[C + (i64)n * (D + (i64)n * (d + (i64)n * c))];
+                kv += K[D + (i64)n * (C + (i64)n * (c + (i64)n * d))];
+            }
+            sx += T[f] * xv;
+            sk += T[f] * kv;
+        }
+        const double f = same ? 0.25 : 1.0;
+        dn += f * T[e] * sx;
+        dk += f * T[e] * sk;
+    }
+    dn = block_sum(dn, sh);
+    dk = block_sum(dk, sh);
+    double E = 0.0, P = 0.0;
+    if (threadIdx.x == 0 && fabs(dn) >= 1.0e-14) {
+        double den = mode == 0 ? eo[x] + eo[y] - dk / dn : -dk / dn - eo[x] - eo[y];
+        double co = dn / den;
+        E = co;
+        P = co / den;
+    }
+    if (threadIdx.x != 0) { E = 0.0; P = 0.0; }
+    block_store4(E, P, 0.0, 0.0, part);
+}
+
+// ------------------------------------------------------------------ classes (+2)/(-2), PC (qdnevpt.F:2563-3072)
+// off-diagonal pairs: v(ab) = T(b,a) with ab = a*n+b ; contribution -(v.s_mu)^2/deno
+__global__ void k_pm2_pc(const double *I, i64 ld, const int2 *pairs, i64 npairs, int mode, int nc, int n,
+                         const double *spc, const double *w, int nnew, const double *eo, double *part) {
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    double E = 0.0, P = 0.0;
+    if (t < npairs * nnew) {
+        const int mu = t % nnew;
+        const int2 pr = pairs[t / nnew];
+        const int x = pr.x, y = pr.y;
+        const int nn = n * n;
+        double acc = 0.0;
+        for (int ab = 0; ab < nn; ab++) {
+            const int a = ab / n, b = ab % n;
+            // (+2): traint(b,a) = (i b | j a) ; (-2): dint1(b,a) = (r b | s a)
+            double v = mode == 0 ? I[(x + (i64)nc * b) + ld * (y + (i64)nc * a)] : I[(b + (i64)n * x) + ld * (a + (i64)n * y)];
+            acc += v * spc[ab + (i64)nn * mu];
+        }
+        double deno = mode == 0 ? w[mu] - eo[x] - eo[y] : w[mu] + eo[x] + eo[y];
+        double contri = acc * acc / deno;
+        E = -contri;
+        P = contri / deno;
+    }
+    block_store4(0.0, 0.0, E, P, part);
+}
+// diagonal pairs (i=j or r=s) with the n(n+1)/2 metric
+__global__ void k_pm2p_pc(const double *I, i64 ld, int next, int mode, int nc, int n, const double *spc,
+                          const double *w, int nnew, const double *eo, double *part) {
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    double E = 0.0, P = 0.0;
+    const int m = n * (n + 1) / 2;
+    if (t < (i64)next * nnew) {
+        const int mu = t % nnew, x = t / nnew;
+        double acc = 0.0;
+        int ab = 0;
+        for (int a = 0; a < n; a++)
+            for (int b = a; b < n; b++, ab++) {
+                double v;
+                if (mode == 0) {   // (i b | i a), doubled off the diagonal (qdnevpt.F:3018-3047)
+                    v = I[(x + (i64)nc * b) + ld * (x + (i64)nc * a)];
+                    if (a != b) v *= 2.0;
+                } else {           // (r a | r b), doubled off the diagonal (qdnevpt.F:2760-2779)
+                    v = I[(a + (i64)n * x) + ld * (b + (i64)n * x)];
+                    if (a != b) v *= 2.0;
+                }
+                acc += v * spc[ab + (i64)m * mu];
+            }
+        double deno = mode == 0 ? w[mu] - 2.0 * eo[x] : w[mu] + 2.0 * eo[x];
+        double contri = acc * acc / deno;
+        E = -contri;
+        P = contri / deno;
+    }
+    block_store4(0.0, 0.0, E, P, part);
+}
+
+// metric / Koopmans matrices of the (+-2) PC solves: fillcm2 and fillcm2p (qdnevpt.F:4156-4193)
+__global__ void k_fillcm2(const double *X, int n, double *S) {
+    const i64 nn = (i64)n * n;
+    i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nn * nn) return;
+    i64 ij = e % nn, kl = e / nn;
+    int i = ij / n, j = ij % n, k = kl / n, l = kl % n;
+    S[e] = X[i + n * (j + (i64)n * (k + (i64)n * l))];
+}
+__global__ void k_fillcm2p(const double *X, int n, const int2 *up, double *S) {
+    const int m = n * (n + 1) / 2;
+    i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (i64)m * m) return;
+    int p = e % m, q = e / m;
+    int i = up[p].x, j = up[p].y, k = up[q].x, l = up[q].y;
+    S[e] = X[i + n * (j + (i64)n * (k + (i64)n * l))] + X[i + n * (j + (i64)n * (l + (i64)n * k))];
+}
+
+// ------------------------------------------------------------------ class (0)' (qdnevpt.F:4326-4742, 4771-4999)
+// X1 = ee2(b',a',a,b); X2 = ee2tr(b',b,a',a) + d(a,b) da(b',a'); G1 = gk0pa(b',a',a,b); G2 = gk0pd(b',a',a,b)
+// all stored at index A + n*(B + n*(a + n*b)) with (A,B) = (a',b')
+__global__ void k_v0p_prep(Sl S, const double *ga, const double *gd, double *X1, double *X2, double *G1, double *G2) {
+    const int n = S.n;
+    i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (i64)n * n * n * n) return;
+    int A = e % n, B = (e / n) % n, a = (e / n / n) % n, b = e / n / n / n;
+    X1[e] = s_ee2(S, B, A, a, b);
+    X2[e] = s_ee2tr(S, B, b, A, a) + (a == b ? DA(S, B, A) : 0.0);
+    const i64 g = B + (i64)n * (A + (i64)n * (a + (i64)n * b));
+    G1[e] = ga[g];
+    G2[e] = gd[g];
+}
+// fillc0p (qdnevpt.F:4195-4264): 2n^2 metric S and Koopmans matrix K, index ab = a*n+b
+__global__ void k_fillc0p(Sl S, const double *ga, const double *gd, double *Sm, double *Km) {
+    const int n = S.n, nn = n * n, N = 2 * nn;
+    i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (i64)N * N) return;
+    int row = e % N, col = e / N;
+    int br = row / nn, bc = col / nn, ab = row % nn, abp = col % nn;
+    int a = ab / n, b = ab % n, ap = abp / n, bp = abp % n;
+    const double gav = ga[b + (i64)n * (a + (i64)n * (ap + (i64)n * bp))];
+    double s, k;
+    if (br == 0 && bc == 0) { k = gav; s = 2.0 * s_ee2(S, bp, ap, a, b); }
+    else if (br != bc) { k = -0.5 * gav; s = -s_ee2(S, bp, ap, a, b); }
+    else {
+        k = gd[b + (i64)n * (a + (i64)n * (ap + (i64)n * bp))];
+        s = -TA(S, bp, a, b, ap);
+        if (a == ap) s += 2.0 * DA(S, bp, b);
+    }
+    Sm[e] = s;
+    Km[e] = k;
+}
+// e20pcont's V[ab,a'b'] = ee2(b',a',a,b) and V2[ab,a'b'] = -daa(a',b,a,b') + 2 d(a',a) da(b,b')
+__global__ void k_v0p_vmats(Sl S, double *V, double *V2) {
+    const int n = S.n, nn = n * n;
+    i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (i64)nn * nn) return;
+    int row = e % nn, col = e / nn;
+    int a = row / n, b = row % n, ap = col / n, bp = col % n;
+    V[e] = s_ee2(S, bp, ap, a, b);
+    double v2 = -TA(S, ap, b, a, bp);
+    if (ap == a) v2 += 2.0 * DA(S, b, bp);
+    V2[e] = v2;
+}
+__global__ void k_v0p_sc(const double *Icvaa, i64 ldj, const double *Icaav, i64 ldk, int nc, int n, int nv,
+                         const double *X1, const double *X2, const double *G1, const double *G2, const double *da,
+                         const double *gf, const double *heff, const double *ec, const double *ev, double *part) {
+    __shared__ double dj[256], dk[256], sh[16];
+    const int i = blockIdx.x % nc, r = blockIdx.x / nc;
+    const int nn = n * n;
+    for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+        int a = e % n, b = e / n;
+        dj[e] = Icvaa[(i + (i64)nc * r) + ldj * (a + (i64)n * b)];       // (i r | a b)
+        dk[e] = Icaav[(i + (i64)nc * a) + ldk * (b + (i64)n * r)];       // (i a | r b)
+    }
+    __syncthreads();
+    double dn = 0.0, dko = 0.0, dummy = 0.0, dummyk = 0.0;
+    for (int e = threadIdx.x; e < nn; e += blockDim.x) {      // (a',b')
+        const double djp = dj[e], dkp = dk[e];
+        double s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+        for (int f = 0; f < nn; f++) {                        // (a,b)
+            const i64 o = e + (i64)nn * f;
+            const double djv = dj[f], dkv = dk[f];
+            const double dint1 = 2.0 * djp * djv - djp * dkv - dkp * djv;
+            const double dint2 = dkp * dkv;
+            const double dint11 = djp * djv - 0.5 * djp * dkv - 0.5 * djv * dkp;
+            s1 += dint1 * X1[o] + dint2 * X2[o];
+            s2 += dint11 * G1[o] + dint2 * G2[o];
+        }
+        dn += s1;
+        dko += s2;
+        const int a = e % n, b = e / n;
+        const double d1 = 2.0 * dj[e] - dk[e];
+        dummy += da[b + n * a] * d1;
+        dummyk += gf[a + n * b] * d1 * 0.5;
+        (void)s3; (void)s4;
+    }
+    dn = block_sum(dn, sh);
+    dko = block_sum(dko, sh);
+    dummy = block_sum(dummy, sh);
+    dummyk = block_sum(dummyk, sh);
+    double E = 0.0, P = 0.0;
+    if (threadIdx.x == 0) {
+        const double hh = heff[i + (i64)nc * r];
+        dn = dn + 2.0 * hh * dummy + 2.0 * hh * hh;
+        dko = dko + hh * dummyk;
+        if (fabs(dn) > 1.0e-14) {
+            double den = ec[i] - ev[r] - dko / dn;
+            double co = dn / den;
+            E = co;
+            P = co / den;
+        }
+    }
+    block_store4(E, P, 0.0, 0.0, part);
+}
+__global__ void k_v0p_pc(const double *Icvaa, i64 ldj, const double *Icaav, i64 ldk, int nc, int n, int nv,
+                         const double *sm, const double *sp, const double *spp, const double *dc1, const double *dc2,
+                         const double *w, int nnew, const double *heff, const double *ec, const double *ev,
+                         double *part) {
+    __shared__ double dj[256], dk[256];
+    const int i = blockIdx.x % nc, r = blockIdx.x / nc;
+    const int nn = n * n;
+    for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+        int a = e / n, b = e % n;                                          // ref order ab = a*n+b
+        dj[e] = Icvaa[(i + (i64)nc * r) + ldj * (a + (i64)n * b)];
+        dk[e] = Icaav[(i + (i64)nc * a) + ldk * (b + (i64)n * r)];
+    }
+    __syncthreads();
+    const double hh = heff[i + (i64)nc * r];
+    double E = 0.0, P = 0.0;
+    for (int mu = threadIdx.x; mu < nnew; mu += blockDim.x) {
+        double rimu = 0.0, rimup = 0.0;
+        for (int ab = 0; ab < nn; ab++) {
+            const i64 o = ab + (i64)nn * mu;
+            rimu += (2.0 * dj[ab] - dk[ab]) * sm[o];
+            rimup += -dj[ab] * sp[o] + dk[ab] * spp[o];
+        }
+        rimu += 2.0 * hh * dc1[mu];
+        rimup -= hh * dc2[mu];
+        const double deno = w[mu] + ev[r] - ec[i];
+        const double co = rimu + rimup, cir = co / deno;
+        E -= co * cir;
+        P += cir * cir;
+    }
+    block_store4(0.0, 0.0, E, P, part);
+}
+// dc[mu] = sum_ab da(b,a) coef[off + ab, mu], ab = a*n+b
+__global__ void k_dacoef(const double *da, int n, const double *coef, int ldc, int off, int nnew, double *dc) {
+    int mu = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mu >= nnew) return;
+    double s = 0.0;
+    for (int a = 0; a < n; a++)
+        for (int b = 0; b < n; b++) s += da[b + n * a] * coef[off + a * n + b + (i64)ldc * mu];
+    dc[mu] = s;
+}
+
+// ------------------------------------------------------------------ classes (+1)' / (-1)' (qdnevpt.F:5437-6445, 5001-5435)
+// index order abc = a*n^2 + b*n + c (the reference's iabc)
+// Spc[abc, A'B'C'] = X3(c',a',b',b,a,c) with X3 = eee2 (minus) or eee2t (plus); Kpc[abc, A'B'C'] = amat(a',b',c',a,b,c)
+__global__ void k_fill_pm1p(Sl S, int plus, const double *amat, double *Spc, double *Kpc) {
+    const int n = S.n;
+    const i64 N = (i64)n * n * n;
+    i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N * N) return;
+    i64 row = e % N, col = e / N;
+    int a = row / (n * n), b = (row / n) % n, c = row % n;
+    int ap = col / (n * n), bp = (col / n) % n, cp = col % n;
+    Spc[e] = plus ? s_eee2t(S, cp, ap, bp, b, a, c) : s_eee2(S, cp, ap, bp, b, a, c);
+    Kpc[e] = amat[ap + (i64)n * (bp + (i64)n * (cp + (i64)n * (a + (i64)n * (b + (i64)n * c))))];
+}
+// Q2[A'B'C', a] = X2(c',a',b',a) (ee2 or ee2tr); BC[A'B'C', a] = bmat(a',b',c',a) + cmat(a,a',b',c')
+__global__ void k_fill_pm1p_q(Sl S, int plus, const double *bmat, const double *cmat, double *Q2, double *BC) {
+    const int n = S.n;
+    const i64 N = (i64)n * n * n;
+    i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N * n) return;
+    i64 row = e % N;
+    int a = e / N;
+    int ap = row / (n * n), bp = (row / n) % n, cp = row % n;
+    Q2[e] = plus ? s_ee2tr(S, cp, ap, bp, a) : s_ee2(S, cp, ap, bp, a);
+    BC[e] = bmat[ap + (i64)n * (bp + (i64)n * (cp + (i64)n * a))] + cmat[a + (i64)n * (ap + (i64)n * (bp + (i64)n * cp))];
+}
+// Vmat[abc, x] = dint1(a,c,b): (a c | b r) from I_aaav (minus) or (a c | b i) from I_aaca (plus)
+__global__ void k_fill_pm1p_v(const double *I, int plus, int n, int nc, int next, double *V) {
+    const i64 N = (i64)n * n * n, ld = (i64)n * n;
+    i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N * next) return;
+    i64 row = e % N;
+    int x = e / N;
+    int a = row / (n * n), b = (row / n) % n, c = row % n;
+    V[e] = plus ? I[(a + (i64)n * c) + ld * (x + (i64)nc * b)] : I[(a + (i64)n * c) + ld * (b + (i64)n * x)];
+}
+// SC sums per external orbital x
+__global__ void k_pm1p_sc(int plus, int n, int next, const double *H, const double *W1, const double *W2,
+                          const double *V, const double *Z1, const double *Z2, const double *Y1, const double *Y2,
+                          const double *eo, double *part) {
+    __shared__ double sh[16];
+    const int x = blockIdx.x;
+    const i64 N = (i64)n * n * n;
+    double dm = 0.0, dk = 0.0;
+    for (i64 e = threadIdx.x; e < N; e += blockDim.x) {
+        const double v = V[e + N * x];
+        dm += 2.0 * v * Z1[e + N * x] + Y1[e + N * x] * v;
+        dk += v * Z2[e + N * x] + Y2[e + N * x] * v;
+    }
+    for (int a = threadIdx.x; a < n; a += blockDim.x) {
+        const double h = H[a + (i64)n * x];
+        dm += h * W1[a + (i64)n * x];
+        dk += h * W2[a + (i64)n * x];
+    }
+    dm = block_sum(dm, sh);
+    dk = block_sum(dk, sh);
+    double E = 0.0, P = 0.0;
+    if (threadIdx.x == 0 && dm != 0.0) {
+        double den = plus ? eo[x] - dk / dm : -dk / dm - eo[x];
+        double co = dm / den;
+        E = co;
+        P = co / den;
+    }
+    block_store4(E, P, 0.0, 0.0, part);
+}
+// PC sums: R[mu, x] from the S-build, deno = w(mu) -/+ e_x
+__global__ void k_pm1p_pc(int plus, int nnew, int next, const double *R, const double *w, const double *eo,
+                          double *part) {
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    double E = 0.0, P = 0.0;
+    if (t < (i64)nnew * next) {
+        int mu = t % nnew, x = t / nnew;
+        double r = R[t];
+        double deno = plus ? w[mu] - eo[x] : w[mu] + eo[x];
+        double co = r / deno;
+        E = -r * co;
+        P = co * co;
+    }
+    block_store4(0.0, 0.0, E, P, part);
+}
+
+// ------------------------------------------------------------------ host side
+struct Buf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct Stage2 {
+    S2Inputs in;
+    rocblas_handle h;
+    hipStream_t st;
+    int nc, n, nv, norb, L, M;
+    std::map<std::string, Buf> bufs;
+    int *dinfo = nullptr;
+    double *part = nullptr;
+    size_t part_cap = 0;
+    double *acc = nullptr;         // 8 classes x 4
+    int2 *pairs_c = nullptr, *pairs_cs = nullptr, *pairs_v = nullptr, *pairs_vs = nullptr, *up = nullptr;
+    i64 np_c, np_cs, np_v, np_vs;
+    bool blocks_ready = false;
+
+    double *get(const std::string &name, i64 count) {
+        Buf &b = bufs[name];
+        size_t bytes = sizeof(double) * (size_t)(count > 0 ? count : 1);
+        if (b.cap < bytes) {
+            if (b.p) { (void)hipStreamSynchronize(st); (void)hipFree(b.p); }
+            b.p = nullptr;
+            if (hipMalloc(&b.p, bytes) != hipSuccess) { b.cap = 0; return nullptr; }
+            b.cap = bytes;
+        }
+        return (double *)b.p;
+    }
+};
+
+#define GET(var, name, cnt) double *var = p->get(name, (cnt)); if (!var) { set_error("hipMalloc failed for %s", name); return -1; }
+
+static int gemm(Stage2 *p, bool tA, bool tB, int m, int nn, int k, double alpha, const double *A, i64 lda,
+                const double *B, i64 ldb, double beta, double *C, i64 ldc) {
+    if (m <= 0 || nn <= 0) return 0;
+    if (k <= 0) {
+        if (beta == 0.0) NV2_HIP(hipMemset2DAsync(C, sizeof(double) * ldc, 0, sizeof(double) * m, nn, p->st));
+        return 0;
+    }
+    RB(rocblas_dgemm(p->h, tA ? rocblas_operation_transpose : rocblas_operation_none,
+                     tB ? rocblas_operation_transpose : rocblas_operation_none, m, nn, k, &alpha, A, (int)lda, B,
+                     (int)ldb, &beta, C, (int)ldc));
+    return 0;
+}
+static int gemm_sb(Stage2 *p, int m, int nn, int k, const double *A, i64 lda, i64 sA, const double *B, i64 ldb,
+                   i64 sB, double *C, i64 ldc, i64 sC, int batch) {
+    if (m <= 0 || nn <= 0 || batch <= 0) return 0;
+    const double one = 1.0, zero = 0.0;
+    RB(rocblas_dgemm_strided_batched(p->h, rocblas_operation_none, rocblas_operation_none, m, nn, k, &one, A, (int)lda,
+                                     sA, B, (int)ldb, sB, &zero, C, (int)ldc, sC, batch));
+    return 0;
+}
+// eigen-decomposition in place (ascending eigenvalues in w), reference triangle choice
+static int syevd(Stage2 *p, bool upper, int N, double *A, double *w) {
+    if (N <= 0) return 0;
+    GET(E, "syevd_E", N);
+    RB(rocsolver_dsyevd(p->h, rocblas_evect_original, upper ? rocblas_fill_upper : rocblas_fill_lower, N, A, N, w, E,
+                        p->dinfo));
+    int info = 0;
+    NV2_HIP(hipMemcpyAsync(&info, p->dinfo, sizeof(int), hipMemcpyDeviceToHost, p->st));
+    NV2_HIP(hipStreamSynchronize(p->st));
+    if (info != 0) { set_error("syevd(N=%d) failed, info=%d", N, info); return -3; }
+    return 0;
+}
+static double *parts(Stage2 *p, i64 nblocks) {
+    size_t need = sizeof(double) * 4 * (size_t)nblocks;
+    if (p->part_cap < need) {
+        if (p->part) { (void)hipStreamSynchronize(p->st); (void)hipFree(p->part); }
+        p->part = nullptr;
+        if (hipMalloc(&p->part, need) != hipSuccess) { p->part_cap = 0; return nullptr; }
+        p->part_cap = need;
+    }
+    return p->part;
+}
+static int reduce_into(Stage2 *p, i64 nblocks, int cls) {
+    hipLaunchKernelGGL(k_reduce_parts, dim3(4), dim3(256), 0, p->st, p->part, nblocks, p->acc + 4 * cls);
+    NV2_LAUNCH_CHECK();
+    return 0;
+}
+
+// PC metric solve (see oracle pc_solve): returns coef (N x nnew, col-major, ld N) and w (nnew) in named buffers
+static int pc_solve(Stage2 *p, const char *tag, int N, double *S, double *K, double thr, double **coef_out,
+                    double **w_out, int *nnew_out, double *minw) {
+    std::string t(tag);
+    GET(w1, (t + "_w1").c_str(), N);
+    TRYS(syevd(p, true, N, S, w1));
+    std::vector<double> hw(N);
+    NV2_HIP(hipMemcpyAsync(hw.data(), w1, sizeof(double) * N, hipMemcpyDeviceToHost, p->st));
+    NV2_HIP(hipStreamSynchronize(p->st));
+    int nnew = 0;
+    for (int j = 0; j < N; j++) if (hw[j] >= thr) nnew++;
+    const int first = N - nnew;
+    if (minw) *minw = N ? hw[0] : 0.0;
+    *nnew_out = nnew;
+    GET(coef, (t + "_coef").c_str(), (i64)N * (nnew ? nnew : 1));
+    GET(w2, (t + "_w2").c_str(), nnew ? nnew : 1);
+    *coef_out = coef;
+    *w_out = w2;
+    if (nnew == 0) return 0;
+    GET(caux, "pc_caux", (i64)N * nnew);
+    hipLaunchKernelGGL(k_caux, dim3(nb((i64)N * nnew, 256)), dim3(256), 0, p->st, S, w1, N, first, nnew, caux);
+    NV2_LAUNCH_CHECK();
+    GET(Ks, "pc_Ks", (i64)N * N);
+    hipLaunchKernelGGL(k_sym_upper, dim3(nb((i64)N * N, 256)), dim3(256), 0, p->st, K, N, Ks);
+    NV2_LAUNCH_CHECK();
+    GET(T, "pc_T", (i64)N * nnew);
+    TRYS(gemm(p, false, false, N, nnew, N, 1.0, Ks, N, caux, N, 0.0, T, N));
+    GET(Kp, "pc_Kp", (i64)nnew * nnew);
+    TRYS(gemm(p, true, false, nnew, nnew, N, 1.0, caux, N, T, N, 0.0, Kp, nnew));
+    TRYS(syevd(p, true, nnew, Kp, w2));
+    TRYS(gemm(p, false, false, N, nnew, nnew, 1.0, caux, N, Kp, nnew, 0.0, coef, N));
+    return 0;
+}
+
+Stage2 *stage2_create(const S2Inputs &in, rocblas_handle h, hipStream_t st) {
+    Stage2 *p = new Stage2();
+    p->in = in;
+    p->h = h;
+    p->st = st;
+    p->nc = in.nc; p->n = in.n; p->nv = in.nv; p->L = in.L; p->M = in.M;
+    p->norb = in.nc + in.n + in.nv;
+    if (hipMalloc(&p->dinfo, sizeof(int)) != hipSuccess) { delete p; return nullptr; }
+    if (hipMalloc(&p->acc, sizeof(double) * 32) != hipSuccess) { delete p; return nullptr; }
+    // pair tables: core i<=j, core i<j, virt r<=s, virt r<s, active upper pairs
+    std::vector<int2> pc, pcs, pv, pvs, up;
+    for (int i = 0; i < in.nc; i++)
+        for (int j = i; j < in.nc; j++) { pc.push_back({i, j}); if (j > i) pcs.push_back({i, j}); }
+    for (int r = 0; r < in.nv; r++)
+        for (int s = r; s < in.nv; s++) { pv.push_back({r, s}); if (s > r) pvs.push_back({r, s}); }
+    for (int a = 0; a < in.n; a++)
+        for (int b = a; b < in.n; b++) up.push_back({a, b});
+    auto upl = [&](std::vector<int2> &v, int2 **d) {
+        size_t b = sizeof(int2) * (v.size() ? v.size() : 1);
+        if (hipMalloc(d, b) != hipSuccess) return false;
+        if (v.size()) hipMemcpy(*d, v.data(), sizeof(int2) * v.size(), hipMemcpyHostToDevice);
+        return true;
+    };
+    upl(pc, &p->pairs_c); upl(pcs, &p->pairs_cs); upl(pv, &p->pairs_v); upl(pvs, &p->pairs_vs); upl(up, &p->up);
+    p->np_c = pc.size(); p->np_cs = pcs.size(); p->np_v = pv.size(); p->np_vs = pvs.size();
+    return p;
+}
+
+void stage2_destroy(Stage2 *p) {
+    if (!p) return;
+    for (auto &kv : p->bufs) if (kv.second.p) (void)hipFree(kv.second.p);
+    (void)hipFree(p->dinfo); (void)hipFree(p->part); (void)hipFree(p->acc);
+    (void)hipFree(p->pairs_c); (void)hipFree(p->pairs_cs); (void)hipFree(p->pairs_v); (void)hipFree(p->pairs_vs);
+    (void)hipFree(p->up);
+    delete p;
+}
+
+static int slice(Stage2 *p, const double *src, int s, i64 cnt, double *dst) {
+    hipLaunchKernelGGL(k_slice, dim3(nb(cnt, 256)), dim3(256), 0, p->st, src, p->M, s, cnt, dst);
+    NV2_LAUNCH_CHECK();
+    return 0;
+}
+
+// state-independent unpacking of the Cholesky blocks and h
+static int prepare_blocks(Stage2 *p) {
+    if (p->blocks_ready) return 0;
+    const int nc = p->nc, n = p->n, nv = p->nv, L = p->L, norb = p->norb;
+    const int c0 = 0, a0 = nc, v0 = nc + n;
+    struct { const char *nm; int p0, np, q0, nq; } blk[] = {
+        {"Bcc", c0, nc, c0, nc}, {"Bcv", c0, nc, v0, nv}, {"Bca", c0, nc, a0, n},
+        {"Bac", a0, n, c0, nc},  {"Bav", a0, n, v0, nv},  {"Baa", a0, n, a0, n}};
+    for (auto &b : blk) {
+        i64 cnt = (i64)L * b.np * b.nq;
+        GET(d, b.nm, cnt);
+        if (cnt) {
+            hipLaunchKernelGGL(k_unpack, dim3(nb(cnt, 256)), dim3(256), 0, p->st, p->in.B, L, b.p0, b.np, b.q0, b.nq, d);
+            NV2_LAUNCH_CHECK();
+        }
+    }
+    GET(hf, "hfull", (i64)norb * norb);
+    hipLaunchKernelGGL(k_unpack_h, dim3(nb((i64)norb * norb, 256)), dim3(256), 0, p->st, p->in.oneint, norb, hf);
+    NV2_LAUNCH_CHECK();
+    p->blocks_ready = true;
+    return 0;
+}
+
+int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *stats) {
+    (void)prec;
+    const int nc = p->nc, n = p->n, nv = p->nv, L = p->L, norb = p->norb, M = p->M;
+    const i64 n2 = (i64)n * n, n4 = n2 * n2, n6 = n4 * n2;
+    hipStream_t st = p->st;
+    RB(rocblas_set_stream(p->h, st));
+    RB(rocblas_set_pointer_mode(p->h, rocblas_pointer_mode_host));
+    TRYS(prepare_blocks(p));
+    NV2_HIP(hipMemsetAsync(p->acc, 0, sizeof(double) * 32, st));
+    S2Stats sd{};
+    // ---- per-state slices
+    GET(da, "da", n2); GET(taa, "taa", n4); GET(ro3, "ro3", n6);
+    TRYS(slice(p, p->in.dal, s, n2, da)); TRYS(slice(p, p->in.taa, s, n4, taa)); TRYS(slice(p, p->in.daaa, s, n6, ro3));
+    Sl S{n, da, taa, ro3};
+    // ---- Fock operators and canonical orbitals (qdnevpt.F:369-698)
+    GET(Jc, "Jc", L); GET(Ja, "Ja", L);
+    hipLaunchKernelGGL(k_jvec, dim3(nb(L, 256)), dim3(256), 0, st, p->in.B, L, nc, n, da, Jc, Ja);
+    NV2_LAUNCH_CHECK();
+    double *hf = p->get("hfull", (i64)norb * norb);
+    GET(eps, "eps", norb);
+    NV2_HIP(hipMemsetAsync(eps, 0, sizeof(double) * norb, st));
+    GET(Cc, "Ccore", (i64)nc * nc);
+    GET(Cv, "Cvirt", (i64)nv * nv);
+    GET(Jt, "Jtot", L);
+    double *Bcc = p->get("Bcc", 0), *Bcv = p->get("Bcv", 0), *Bca = p->get("Bca", 0), *Bac = p->get("Bac", 0);
+    double *Bav = p->get("Bav", 0), *Baa = p->get("Baa", 0);
+    // Jt = 2 Jcore + Jact (Coulomb of both core and active densities)
+    {
+        const double one = 1.0, two = 2.0;
+        NV2_HIP(hipMemcpyAsync(Jt, Ja, sizeof(double) * L, hipMemcpyDeviceToDevice, st));
+        if (nc) RB(rocblas_daxpy(p->h, L, &two, Jc, 1, Jt, 1));
+        (void)one;
+    }
+    if (nc) {
+        NV2_HIP(hipMemcpy2DAsync(Cc, sizeof(double) * nc, hf, sizeof(double) * norb, sizeof(double) * nc, nc,
+                                 hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_coul, dim3(nb((i64)nc * nc, 4)), dim3(256), 0, st, p->in.B, L, 0, nc, Jt, Cc);
+        NV2_LAUNCH_CHECK();
+        TRYS(gemm(p, true, false, nc, nc, L * nc, -1.0, Bcc, (i64)L * nc, Bcc, (i64)L * nc, 1.0, Cc, nc));
+        GET(G, "fockG", (i64)L * n * (nc > nv ? nc : nv));
+        TRYS(gemm_sb(p, L, n, n, Bac, L, (i64)L * n, da, n, 0, G, L, (i64)L * n, nc));
+        TRYS(gemm(p, true, false, nc, nc, L * n, -0.5, G, (i64)L * n, Bac, (i64)L * n, 1.0, Cc, nc));
+        GET(wc, "wcore", nc);
+        TRYS(syevd(p, false, nc, Cc, wc));
+        NV2_HIP(hipMemcpyAsync(eps, wc, sizeof(double) * nc, hipMemcpyDeviceToDevice, st));
+    }
+    {
+        NV2_HIP(hipMemcpy2DAsync(Cv, sizeof(double) * nv, hf + (i64)(nc + n) * norb + nc + n, sizeof(double) * norb,
+                                 sizeof(double) * nv, nv, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_coul, dim3(nb((i64)nv * nv, 4)), dim3(256), 0, st, p->in.B, L, nc + n, nv, Jt, Cv);
+        NV2_LAUNCH_CHECK();
+        if (nc) TRYS(gemm(p, true, false, nv, nv, L * nc, -1.0, Bcv, (i64)L * nc, Bcv, (i64)L * nc, 1.0, Cv, nv));
+        GET(G, "fockG", (i64)L * n * (nc > nv ? nc : nv));
+        TRYS(gemm_sb(p, L, n, n, Bav, L, (i64)L * n, da, n, 0, G, L, (i64)L * n, nv));
+        TRYS(gemm(p, true, false, nv, nv, L * n, -0.5, G, (i64)L * n, Bav, (i64)L * n, 1.0, Cv, nv));
+        GET(wv, "wvirt", nv);
+        TRYS(syevd(p, false, nv, Cv, wv));
+        NV2_HIP(hipMemcpyAsync(eps + nc + n, wv, sizeof(double) * nv, hipMemcpyDeviceToDevice, st));
+    }
+    const double *ec = eps, *ev = eps + nc + n;
+    // ---- Cholesky transform (qdnevpt.F:7599-7717)
+    GET(T1, "T1", (i64)L * nc * (nv > nc ? nv : nc));
+    GET(Tcc, "Tcc", (i64)L * nc * nc); GET(Tcv, "Tcv", (i64)L * nc * nv); GET(Tca, "Tca", (i64)L * nc * n);
+    GET(Tav, "Tav", (i64)L * n * nv);
+    if (nc) {
+        TRYS(gemm(p, false, false, L * nc, nc, nc, 1.0, Bcc, (i64)L * nc, Cc, nc, 0.0, T1, (i64)L * nc));
+        TRYS(gemm_sb(p, L, nc, nc, T1, L, (i64)L * nc, Cc, nc, 0, Tcc, L, (i64)L * nc, nc));
+        TRYS(gemm(p, false, false, L * nc, nv, nv, 1.0, Bcv, (i64)L * nc, Cv, nv, 0.0, T1, (i64)L * nc));
+        TRYS(gemm_sb(p, L, nc, nc, T1, L, (i64)L * nc, Cc, nc, 0, Tcv, L, (i64)L * nc, nv));
+        TRYS(gemm_sb(p, L, nc, nc, Bca, L, (i64)L * nc, Cc, nc, 0, Tca, L, (i64)L * nc, n));
+    }
+    TRYS(gemm(p, false, false, L * n, nv, nv, 1.0, Bav, (i64)L * n, Cv, nv, 0.0, Tav, (i64)L * n));
+    const double *Taa = Baa;
+    // ---- integral blocks: I_xy[u,v] = sum_L T_x[L,u] T_y[L,v]
+    const i64 Ncv = (i64)nc * nv, Nca = (i64)nc * n, Nav = (i64)n * nv, Naa = n2;
+    GET(Icvcv, "Icvcv", Ncv * Ncv); GET(Icacv, "Icacv", Nca * Ncv); GET(Icvav, "Icvav", Ncv * Nav);
+    GET(Icaca, "Icaca", Nca * Nca); GET(Iavav, "Iavav", Nav * Nav); GET(Icvaa, "Icvaa", Ncv * Naa);
+    GET(Icaav, "Icaav", Nca * Nav); GET(Iaaav, "Iaaav", Naa * Nav); GET(Iaaca, "Iaaca", Naa * Nca);
+    TRYS(gemm(p, true, false, Ncv, Ncv, L, 1.0, Tcv, L, Tcv, L, 0.0, Icvcv, Ncv));
+    TRYS(gemm(p, true, false, Nca, Ncv, L, 1.0, Tca, L, Tcv, L, 0.0, Icacv, Nca));
+    TRYS(gemm(p, true, false, Ncv, Nav, L, 1.0, Tcv, L, Tav, L, 0.0, Icvav, Ncv));
+    TRYS(gemm(p, true, false, Nca, Nca, L, 1.0, Tca, L, Tca, L, 0.0, Icaca, Nca));
+    TRYS(gemm(p, true, false, Nav, Nav, L, 1.0, Tav, L, Tav, L, 0.0, Iavav, Nav));
+    TRYS(gemm(p, true, false, Ncv, Naa, L, 1.0, Tcv, L, Taa, L, 0.0, Icvaa, Ncv));
+    TRYS(gemm(p, true, false, Nca, Nav, L, 1.0, Tca, L, Tav, L, 0.0, Icaav, Nca));
+    TRYS(gemm(p, true, false, Naa, Nav, L, 1.0, Taa, L, Tav, L, 0.0, Iaaav, Naa));
+    TRYS(gemm(p, true, false, Naa, Nca, L, 1.0, Taa, L, Tca, L, 0.0, Iaaca, Naa));
+    // ---- one-electron integrals in the canonical bases and the Dyall-type heff vectors
+    GET(Jc2, "Jc2", L);
+    if (nc) {
+        hipLaunchKernelGGL(k_jdiag, dim3(nb(L, 256)), dim3(256), 0, st, Tcc, L, nc, Jc2);
+        NV2_LAUNCH_CHECK();
+    } else NV2_HIP(hipMemsetAsync(Jc2, 0, sizeof(double) * L, st));
+    GET(tmp, "h_tmp", (i64)norb * norb);
+    // heff0p[k,r] = (C^T h_cv V)[k,r] + 2 (Jc2^T Tcv)[k,r] - sum_{L,j} Tcc[L,j,k] Tcv[L,j,r]
+    GET(h0p, "heff0p", Ncv);
+    GET(hm1p, "heffm1p", Nav);
+    GET(hp1p, "heffp1p", Nca);
+    if (nc) {
+        TRYS(gemm(p, false, false, nc, nv, nv, 1.0, hf + (i64)(nc + n) * norb, norb, Cv, nv, 0.0, tmp, nc));
+        TRYS(gemm(p, true, false, nc, nv, nc, 1.0, Cc, nc, tmp, nc, 0.0, h0p, nc));
+        TRYS(gemm(p, true, false, 1, (int)Ncv, L, 2.0, Jc2, L, Tcv, L, 1.0, h0p, 1));
+        TRYS(gemm(p, true, false, nc, nv, L * nc, -1.0, Tcc, (i64)L * nc, Tcv, (i64)L * nc, 1.0, h0p, nc));
+    }
+    // heffm1p[a,r] = (h_av V)[a,r] + 2 (Jc2^T Tav)[a,r] - sum_{L,j} Tca[L,j,a] Tcv[L,j,r] - sum_{L,b} Taa[L,b,a] Tav[L,b,r]
+    TRYS(gemm(p, false, false, n, nv, nv, 1.0, hf + (i64)(nc + n) * norb + nc, norb, Cv, nv, 0.0, hm1p, n));
+    TRYS(gemm(p, true, false, 1, (int)Nav, L, 2.0, Jc2, L, Tav, L, 1.0, hm1p, 1));
+    if (nc) TRYS(gemm(p, true, false, n, nv, L * nc, -1.0, Tca, (i64)L * nc, Tcv, (i64)L * nc, 1.0, hm1p, n));
+    TRYS(gemm(p, true, false, n, nv, L * n, -1.0, Taa, (i64)L * n, Tav, (i64)L * n, 1.0, hm1p, n));
+    // heffp1p[i,a] = (C^T h_ca)[i,a] + 2 (Jc2^T Tca)[i,a] - sum_{L,j} Tcc[L,j,i] Tca[L,j,a]
+    if (nc) {
+        TRYS(gemm(p, true, false, nc, n, nc, 1.0, Cc, nc, hf + (i64)nc * norb, norb, 0.0, hp1p, nc));
+        TRYS(gemm(p, true, false, 1, (int)Nca, L, 2.0, Jc2, L, Tca, L, 1.0, hp1p, 1));
+        TRYS(gemm(p, true, false, nc, n, L * nc, -1.0, Tcc, (i64)L * nc, Tca, (i64)L * nc, 1.0, hp1p, nc));
+    }
+    // ---- Koopmans matrices of this state
+    GET(k_eako1, "k_eako1", n2); GET(k_ipko1, "k_ipko1", n2); GET(k_hrdm, "k_hrdm", n4); GET(k_eako2, "k_eako2", n4);
+    GET(k_ipko2, "k_ipko2", n4); GET(k_ga, "k_ga", n4); GET(k_gd, "k_gd", n4); GET(k_gf, "k_gf", n2);
+    TRYS(slice(p, p->in.eako1, s, n2, k_eako1)); TRYS(slice(p, p->in.ipko1, s, n2, k_ipko1));
+    TRYS(slice(p, p->in.hrdm, s, n4, k_hrdm)); TRYS(slice(p, p->in.eako2, s, n4, k_eako2));
+    TRYS(slice(p, p->in.ipko2, s, n4, k_ipko2)); TRYS(slice(p, p->in.gk0pa, s, n4, k_ga));
+    TRYS(slice(p, p->in.gk0pd, s, n4, k_gd)); TRYS(slice(p, p->in.gk0pf, s, n2, k_gf));
+
+    // ================================================================ class (0)
+    if (nc && nv) {
+        dim3 g(nb(nv, 128), (unsigned)p->np_c);
+        double *pt = parts(p, (i64)g.x * g.y);
+        hipLaunchKernelGGL(k_v0, g, dim3(128), 0, st, Icvcv, Ncv, p->pairs_c, nc, nv, ec, ev, pt);
+        NV2_LAUNCH_CHECK();
+        TRYS(reduce_into(p, (i64)g.x * g.y, 0));
+    }
+    // ================================================================ class (+1)
+    {
+        GET(Sm, "pc_S", n2); GET(Km, "pc_K", n2); GET(ro, "ro_v1", n2);
+        hipLaunchKernelGGL(k_fill_diag_minus, dim3(nb(n2, 256)), dim3(256), 0, st, da, n, 2.0, Sm);
+        NV2_LAUNCH_CHECK();
+        NV2_HIP(hipMemcpyAsync(ro, Sm, sizeof(double) * n2, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_scale, dim3(nb(n2, 256)), dim3(256), 0, st, k_eako1, n2, 0.5, Km);
+        NV2_LAUNCH_CHECK();
+        double *coef, *w; int nnew;
+        TRYS(pc_solve(p, "v1", n, Sm, Km, thr, &coef, &w, &nnew, &sd.min_metric_eig[1]));
+        sd.nnew[1] = nnew;
+        GET(spc, "spc_v1", (i64)n * (nnew ? nnew : 1));
+        TRYS(gemm(p, true, false, n, nnew, n, 1.0, ro, n, coef, n, 0.0, spc, n));
+        if (nc && nv) {
+            dim3 g(nb(nv, 64), (unsigned)p->np_c);
+            double *pt = parts(p, (i64)g.x * g.y);
+#define V1(NM) hipLaunchKernelGGL(k_v1<NM>, g, dim3(64), 0, st, Icacv, Nca, p->pairs_c, nc, n, nv, ec, ev, k_eako1, ro, spc, w, nnew, pt)
+            if (n <= 8) V1(8); else V1(16);
+#undef V1
+            NV2_LAUNCH_CHECK();
+            TRYS(reduce_into(p, (i64)g.x * g.y, 1));
+        }
+    }
+    // ================================================================ class (-1)
+    {
+        GET(Sm, "pc_S", n2); GET(Km, "pc_K", n2);
+        NV2_HIP(hipMemcpyAsync(Sm, da, sizeof(double) * n2, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_scale, dim3(nb(n2, 256)), dim3(256), 0, st, k_ipko1, n2, 0.5, Km);
+        NV2_LAUNCH_CHECK();
+        double *coef, *w; int nnew;
+        TRYS(pc_solve(p, "vm1", n, Sm, Km, thr, &coef, &w, &nnew, &sd.min_metric_eig[2]));
+        sd.nnew[2] = nnew;
+        GET(spc, "spc_vm1", (i64)n * (nnew ? nnew : 1));
+        TRYS(gemm(p, true, false, n, nnew, n, 1.0, da, n, coef, n, 0.0, spc, n));
+        if (nc && nv) {
+            const i64 tot = p->np_v * nc;
+            double *pt = parts(p, nb(tot, 128));
+#define VM1(NM) hipLaunchKernelGGL(k_vm1<NM>, dim3(nb(tot, 128)), dim3(128), 0, st, Icvav, Ncv, p->pairs_v, p->np_v, nc, n, ec, ev, da, k_ipko1, spc, w, nnew, pt)
+            if (n <= 8) VM1(8); else VM1(16);
+#undef VM1
+            NV2_LAUNCH_CHECK();
+            TRYS(reduce_into(p, nb(tot, 128), 2));
+        }
+    }
+    // ================================================================ classes (+2) and (-2)
+    for (int mode = 0; mode < 2; mode++) {
+        const int cls = mode == 0 ? 3 : 4;
+        const double *X = mode == 0 ? k_hrdm : taa;
+        const double *K = mode == 0 ? k_eako2 : k_ipko2;
+        const double *I = mode == 0 ? Icaca : Iavav;
+        const i64 ld = mode == 0 ? Nca : Nav;
+        const int next = mode == 0 ? nc : nv;
+        const double *eo = mode == 0 ? ec : ev;
+        const int2 *pr = mode == 0 ? p->pairs_c : p->pairs_v;
+        const i64 npr = mode == 0 ? p->np_c : p->np_v;
+        const int2 *prs = mode == 0 ? p->pairs_cs : p->pairs_vs;
+        const i64 nprs = mode == 0 ? p->np_cs : p->np_vs;
+        if (next && npr) {
+            double *pt = parts(p, npr);
+            hipLaunchKernelGGL(k_pm2_sc, dim3((unsigned)npr), dim3(256), 0, st, I, ld, pr, mode, nc, n, X, K, eo, pt);
+            NV2_LAUNCH_CHECK();
+            TRYS(reduce_into(p, npr, cls));
+        }
+        // PC, off-diagonal pairs: n^2 metric
+        GET(Sm, "pc_S", n4); GET(Km, "pc_K", n4); GET(Xm, "pc_X", n4);
+        hipLaunchKernelGGL(k_fillcm2, dim3(nb(n4, 256)), dim3(256), 0, st, X, n, Sm);
+        hipLaunchKernelGGL(k_fillcm2, dim3(nb(n4, 256)), dim3(256), 0, st, K, n, Km);
+        hipLaunchKernelGGL(k_fillcm2, dim3(nb(n4, 256)), dim3(256), 0, st, X, n, Xm);
+        NV2_LAUNCH_CHECK();
+        double *coef, *w; int nnew;
+        TRYS(pc_solve(p, mode == 0 ? "v2" : "vm2", (int)n2, Sm, Km, thr, &coef, &w, &nnew, &sd.min_metric_eig[cls]));
+        sd.nnew[cls] = nnew;
+        GET(spc, "spc_pm2", n2 * (nnew ? nnew : 1));
+        TRYS(gemm(p, true, false, (int)n2, nnew, (int)n2, 1.0, Xm, n2, coef, n2, 0.0, spc, n2));
+        if (nprs && nnew) {
+            const i64 tot = nprs * nnew;
+            double *pt = parts(p, nb(tot, 128));
+            hipLaunchKernelGGL(k_pm2_pc, dim3(nb(tot, 128)), dim3(128), 0, st, I, ld, prs, nprs, mode, nc, n, spc, w,
+                               nnew, eo, pt);
+            NV2_LAUNCH_CHECK();
+            TRYS(reduce_into(p, nb(tot, 128), cls));
+        }
+        // PC, diagonal pairs: n(n+1)/2 metric
+        const int m = n * (n + 1) / 2;
+        hipLaunchKernelGGL(k_fillcm2p, dim3(nb((i64)m * m, 256)), dim3(256), 0, st, X, n, p->up, Sm);
+        hipLaunchKernelGGL(k_fillcm2p, dim3(nb((i64)m * m, 256)), dim3(256), 0, st, K, n, p->up, Km);
+        hipLaunchKernelGGL(k_fillcm2p, dim3(nb((i64)m * m, 256)), dim3(256), 0, st, X, n, p->up, Xm);
+        NV2_LAUNCH_CHECK();
+        double mw2;
+        TRYS(pc_solve(p, mode == 0 ? "v2p" : "vm2p", m, Sm, Km, thr, &coef, &w, &nnew, &mw2));
+        if (mw2 < sd.min_metric_eig[cls]) sd.min_metric_eig[cls] = mw2;
+        GET(spc2, "spc_pm2p", (i64)m * (nnew ? nnew : 1));
+        TRYS(gemm(p, true, false, m, nnew, m, 0.5, Xm, m, coef, m, 0.0, spc2, m));
+        if (next && nnew) {
+            const i64 tot = (i64)next * nnew;
+            double *pt = parts(p, nb(tot, 128));
+            hipLaunchKernelGGL(k_pm2p_pc, dim3(nb(tot, 128)), dim3(128), 0, st, I, ld, next, mode, nc, n, spc2, w,
+                               nnew, eo, pt);
+            NV2_LAUNCH_CHECK();
+            TRYS(reduce_into(p, nb(tot, 128), cls));
+        }
+    }
+    // ================================================================ class (0)'
+    {
+        GET(X1, "x1", n4); GET(X2, "x2", n4); GET(G1, "g1", n4); GET(G2, "g2", n4);
+        hipLaunchKernelGGL(k_v0p_prep, dim3(nb(n4, 256)), dim3(256), 0, st, S, k_ga, k_gd, X1, X2, G1, G2);
+        NV2_LAUNCH_CHECK();
+        if (nc && nv) {
+            const i64 nbk = (i64)nc * nv;
+            double *pt = parts(p, nbk);
+            hipLaunchKernelGGL(k_v0p_sc, dim3((unsigned)nbk), dim3(256), 0, st, Icvaa, Ncv, Icaav, Nca, nc, n, nv, X1,
+                               X2, G1, G2, da, k_gf, h0p, ec, ev, pt);
+            NV2_LAUNCH_CHECK();
+            TRYS(reduce_into(p, nbk, 7));
+        }
+        const int N = 2 * (int)n2;
+        GET(Sm, "pc_S", (i64)N * N); GET(Km, "pc_K", (i64)N * N);
+        hipLaunchKernelGGL(k_fillc0p, dim3(nb((i64)N * N, 256)), dim3(256), 0, st, S, k_ga, k_gd, Sm, Km);
+        NV2_LAUNCH_CHECK();
+        double *coef, *w; int nnew;
+        TRYS(pc_solve(p, "v0p", N, Sm, Km, thr, &coef, &w, &nnew, &sd.min_metric_eig[7]));
+        sd.nnew[7] = nnew;
+        if (nnew && nc && nv) {
+            GET(V, "v0p_V", n4); GET(V2, "v0p_V2", n4);
+            hipLaunchKernelGGL(k_v0p_vmats, dim3(nb(n4, 256)), dim3(256), 0, st, S, V, V2);
+            NV2_LAUNCH_CHECK();
+            GET(sm, "v0p_sm", n2 * nnew); GET(sp, "v0p_sp", n2 * nnew); GET(spp, "v0p_spp", n2 * nnew);
+            TRYS(gemm(p, false, false, (int)n2, nnew, (int)n2, 1.0, V, n2, coef, N, 0.0, sm, n2));
+            TRYS(gemm(p, false, false, (int)n2, nnew, (int)n2, 1.0, V, n2, coef + n2, N, 0.0, sp, n2));
+            TRYS(gemm(p, false, false, (int)n2, nnew, (int)n2, 1.0, V2, n2, coef + n2, N, 0.0, spp, n2));
+            GET(dc1, "v0p_dc1", nnew); GET(dc2, "v0p_dc2", nnew);
+            hipLaunchKernelGGL(k_dacoef, dim3(nb(nnew, 128)), dim3(128), 0, st, da, n, coef, N, 0, nnew, dc1);
+            hipLaunchKernelGGL(k_dacoef, dim3(nb(nnew, 128)), dim3(128), 0, st, da, n, coef, N, (int)n2, nnew, dc2);
+            NV2_LAUNCH_CHECK();
+            const i64 nbk = (i64)nc * nv;
+            double *pt = parts(p, nbk);
+            hipLaunchKernelGGL(k_v0p_pc, dim3((unsigned)nbk), dim3(256), 0, st, Icvaa, Ncv, Icaav, Nca, nc, n, nv, sm,
+                               sp, spp, dc1, dc2, w, nnew, h0p, ec, ev, pt);
+            NV2_LAUNCH_CHECK();
+            TRYS(reduce_into(p, nbk, 7));
+        }
+    }
+    // ================================================================ classes (-1)' and (+1)'
+    if (p->in.amatIP && p->in.amatEA) {
+        const i64 N = n2 * n;
+        for (int plus = 0; plus < 2; plus++) {
+            const int cls = plus ? 5 : 6;
+            const int next = plus ? nc : nv;
+            const double *eo = plus ? ec : ev;
+            GET(am, "k_amat", n6); GET(bm, "k_bmat", n4); GET(cm, "k_cmat", n4); GET(dm, "k_dmat", n2);
+            TRYS(slice(p, plus ? p->in.amatEA : p->in.amatIP, s, n6, am));
+            TRYS(slice(p, plus ? p->in.bmatEA : p->in.bmatIP, s, n4, bm));
+            TRYS(slice(p, plus ? p->in.cmatEA : p->in.cmatIP, s, n4, cm));
+            TRYS(slice(p, plus ? p->in.dmatEA : p->in.dmatIP, s, n2, dm));
+            GET(Spc, "pm1_S", N * N); GET(Kpc, "pm1_K", N * N); GET(Mx, "pm1_M", N * N);
+            hipLaunchKernelGGL(k_fill_pm1p, dim3(nb(N * N, 256)), dim3(256), 0, st, S, plus, am, Spc, Kpc);
+            NV2_LAUNCH_CHECK();
+            NV2_HIP(hipMemcpyAsync(Mx, Spc, sizeof(double) * N * N, hipMemcpyDeviceToDevice, st));   // keeps S for the sums
+            GET(Q2, "pm1_Q2", N * n); GET(BC, "pm1_BC", N * n);
+            hipLaunchKernelGGL(k_fill_pm1p_q, dim3(nb(N * n, 256)), dim3(256), 0, st, S, plus, bm, cm, Q2, BC);
+            NV2_LAUNCH_CHECK();
+            if (next) {
+                GET(V, "pm1_V", N * next);
+                hipLaunchKernelGGL(k_fill_pm1p_v, dim3(nb(N * next, 256)), dim3(256), 0, st, plus ? Iaaca : Iaaav,
+                                   plus, n, nc, next, V);
+                NV2_LAUNCH_CHECK();
+                const double *H = plus ? nullptr : hm1p;
+                GET(Hp, "pm1_H", (i64)n * next);
+                if (plus) {   // H[a,i] = heffp1p[i,a]
+                    hipLaunchKernelGGL(k_transpose, dim3(nb((i64)n * nc, 256)), dim3(256), 0, st, hp1p, nc, n, Hp);
+                    NV2_LAUNCH_CHECK();
+                    H = Hp;
+                }
+                GET(ro1, "pm1_ro", n2);
+                hipLaunchKernelGGL(k_fill_diag_minus, dim3(nb(n2, 256)), dim3(256), 0, st, da, n, plus ? 2.0 : 0.0, ro1);
+                NV2_LAUNCH_CHECK();
+                GET(W1, "pm1_W1", (i64)n * next); GET(W2, "pm1_W2", (i64)n * next);
+                GET(Z1, "pm1_Z1", N * next); GET(Z2, "pm1_Z2", N * next); GET(Y1, "pm1_Y1", N * next);
+                GET(Y2, "pm1_Y2", N * next);
+                TRYS(gemm(p, false, false, n, next, n, 1.0, ro1, n, H, n, 0.0, W1, n));
+                TRYS(gemm(p, false, false, n, next, n, 1.0, dm, n, H, n, 0.0, W2, n));
+                TRYS(gemm(p, false, false, (int)N, next, n, 1.0, Q2, N, H, n, 0.0, Z1, N));
+                TRYS(gemm(p, false, false, (int)N, next, n, 1.0, BC, N, H, n, 0.0, Z2, N));
+                TRYS(gemm(p, false, false, (int)N, next, (int)N, 1.0, Mx, N, V, N, 0.0, Y1, N));   // Spc V = M^T v
+                TRYS(gemm(p, false, false, (int)N, next, (int)N, 1.0, Kpc, N, V, N, 0.0, Y2, N));  // Kpc V = Am^T v
+                double *pt = parts(p, next);
+                hipLaunchKernelGGL(k_pm1p_sc, dim3(next), dim3(256), 0, st, plus, n, next, H, W1, W2, V, Z1, Z2, Y1, Y2,
+                                   eo, pt);
+                NV2_LAUNCH_CHECK();
+                TRYS(reduce_into(p, next, cls));
+                // PC: metric solve at N = n^3, S-build, sums
+                double *coef, *w; int nnew;
+                TRYS(pc_solve(p, plus ? "vp1p" : "vm1p", (int)N, Spc, Kpc, thr, &coef, &w, &nnew, &sd.min_metric_eig[cls]));
+                sd.nnew[cls] = nnew;
+                if (nnew) {
+                    GET(SbT, "pm1_SbT", N * nnew); GET(spT, "pm1_spT", (i64)n * nnew); GET(R, "pm1_R", (i64)nnew * next);
+                    TRYS(gemm(p, false, false, (int)N, nnew, (int)N, 1.0, Mx, N, coef, N, 0.0, SbT, N));    // S-build
+                    TRYS(gemm(p, true, false, n, nnew, (int)N, 1.0, Q2, N, coef, N, 0.0, spT, n));
+                    TRYS(gemm(p, true, false, nnew, next, (int)N, 1.0, SbT, N, V, N, 0.0, R, nnew));
+                    TRYS(gemm(p, true, false, nnew, next, n, 1.0, spT, n, H, n, 1.0, R, nnew));
+                    const i64 tot = (i64)nnew * next;
+                    double *pt2 = parts(p, nb(tot, 256));
+                    hipLaunchKernelGGL(k_pm1p_pc, dim3(nb(tot, 256)), dim3(256), 0, st, plus, nnew, next, R, w, eo, pt2);
+                    NV2_LAUNCH_CHECK();
+                    TRYS(reduce_into(p, nb(tot, 256), cls));
+                }
+            }
+        }
+    }
+    // ---- results: (norm_sc, e_sc, norm_pc, e_pc) per class, SC==PC for class (0)
+    double accs[32];
+    NV2_HIP(hipMemcpyAsync(accs, p->acc, sizeof(accs), hipMemcpyDeviceToHost, st));
+    NV2_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < 8; k++) {
+        out[4 * k + 0] = accs[4 * k + 1];
+        out[4 * k + 1] = accs[4 * k + 0];
+        out[4 * k + 2] = accs[4 * k + 3];
+        out[4 * k + 3] = accs[4 * k + 2];
+    }
+    if (stats) *stats = sd;
+    (void)M;
+    return 0;
+}
+
+}  // namespace nv2

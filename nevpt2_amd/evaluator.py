@@ -1,0 +1,66 @@
+"""Device-resident NEVPT2 evaluation (stage 1 + stage 2 in one call).
+
+Replaces the reference's two-executable pipeline koopro4QD.x -> nevpt.h5 ->
+QDnevpt2.x (koopro4QD_main.F90, qdnevpt_main.F90) for the hot path: the packed
+4-RDM and the Cholesky integrals are uploaded once and every intermediate stays
+in HBM.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib as L
+
+PHASES = ["partial traces", "Koopmans <=3-RDM + B/C/D", "E4 build", "E4 contractions", "bamat rest", "stage 2"]
+
+
+class Evaluator:
+    def __init__(self, ncore, nact, nvirt, metat, nele, nchovec):
+        self.nc, self.n, self.nv, self.M, self.nele, self.L = ncore, nact, nvirt, metat, nele, nchovec
+        self.norb = ncore + nact + nvirt
+        h = L.lib().nv2_ctx_create(ncore, nact, nvirt, metat, nele, nchovec)
+        if not h:
+            raise L.NV2Error(f"nv2_ctx_create failed: {L.last_error()}")
+        self._h = h
+
+    @classmethod
+    def for_problem(cls, P):
+        ev = cls(P.ncore, P.nact, P.nvirt, P.metat, P.nele, P.B.shape[0])
+        ev.set_integrals(P.B, P.oneint)
+        ev.set_ro4(P.ro4)
+        return ev
+
+    def set_integrals(self, cholesky, oneint):
+        B = L.fin(cholesky)
+        one = np.ascontiguousarray(oneint, dtype=np.float64)
+        L.check(L.lib().nv2_ctx_set_integrals(self._h, L.ptr(B), L.ptr(one)), "nv2_ctx_set_integrals")
+
+    def set_ro4(self, ro4):
+        L.check(L.lib().nv2_ctx_set_ro4(self._h, L.ptr(L.fin(ro4))), "nv2_ctx_set_ro4")
+
+    def run(self, prec=L.PREC_FP64, thr_lindep=1.0e-6):
+        """Per-state class table, shape (metat, 8, 4): (norm_sc, e_sc, norm_pc, e_pc)."""
+        out = np.zeros(32 * self.M)
+        L.check(L.lib().nv2_ctx_run(self._h, int(prec), float(thr_lindep), L.ptr(out)), "nv2_ctx_run")
+        return out.reshape(self.M, 8, 4)
+
+    def timings(self):
+        ms = np.zeros(6)
+        L.check(L.lib().nv2_ctx_timings(self._h, L.ptr(ms)), "nv2_ctx_timings")
+        return dict(zip(PHASES, ms))
+
+    def get(self, name, shape):
+        a = np.zeros(shape, order="F")
+        L.check(L.lib().nv2_ctx_get(self._h, name.encode(), L.ptr(a), a.size), f"nv2_ctx_get({name})")
+        return a
+
+    def close(self):
+        if self._h:
+            L.lib().nv2_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

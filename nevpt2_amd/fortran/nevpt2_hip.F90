@@ -1,0 +1,164 @@
+! ISO_C_BINDING interface to libnevpt2_amd.so (declarations in include/nevpt2_amd.h)
+! plus the input.nevpt reader of the Fortran host.
+!
+! The interface block is what a Fortran caller of the reference binds instead of
+! koopro4QD_driver/qdnevpt (koopro4QD.F90:25, qdnevpt.F:15): dimensions are
+! int64 (the reference builds with -fdefault-integer-8) and arrays keep the
+! reference's column-major layouts, state index first.
+module nevpt2_hip
+  use iso_c_binding
+  implicit none
+
+  integer(c_int32_t), parameter :: NV2_PREC_FP64 = 0, NV2_PREC_FP32 = 1
+  integer, parameter :: NV2_NPHASE = 6
+
+  interface
+    function nv2_last_error(buf, len) bind(C, name="nv2_last_error") result(rc)
+      import :: c_int, c_char, c_int64_t
+      character(kind=c_char), intent(out) :: buf(*)
+      integer(c_int64_t), value :: len
+      integer(c_int) :: rc
+    end function
+    function nv2_ctx_create(ncore, nact, nvirt, metat, nele, nchovec) bind(C, name="nv2_ctx_create") result(ctx)
+      import :: c_ptr, c_int64_t
+      integer(c_int64_t), value :: ncore, nact, nvirt, metat, nele, nchovec
+      type(c_ptr) :: ctx
+    end function
+    function nv2_ctx_set_integrals(ctx, cholesky, oneint) bind(C, name="nv2_ctx_set_integrals") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: cholesky(*), oneint(*)
+      integer(c_int) :: rc
+    end function
+    function nv2_ctx_set_ro4(ctx, ro4) bind(C, name="nv2_ctx_set_ro4") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: ro4(*)
+      integer(c_int) :: rc
+    end function
+    function nv2_ctx_run(ctx, prec, thr_lindep, out) bind(C, name="nv2_ctx_run") result(rc)
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: prec
+      real(c_double), value :: thr_lindep
+      real(c_double), intent(out) :: out(*)
+      integer(c_int) :: rc
+    end function
+    function nv2_ctx_timings(ctx, ms) bind(C, name="nv2_ctx_timings") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: ms(*)
+      integer(c_int) :: rc
+    end function
+    subroutine nv2_ctx_destroy(ctx) bind(C, name="nv2_ctx_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end subroutine
+  end interface
+
+  ! ------------------------------------------------------------------ input.nevpt (nevpt2_cfg.F90 defaults)
+  type nv2_config
+    real(8) :: thr = 1.0d-10
+    real(8) :: thr_lindep = 1.0d-6
+    integer :: nr_states = -1
+    integer :: nr_active_electrons = -1
+    logical :: do_cholesky = .false.
+    logical :: no_4rdm_terms = .false.
+    logical :: no_pc = .false.
+    logical :: zorder = .true.
+    integer :: prec = 0
+    character(len=256) :: file04 = ' '
+  end type
+
+contains
+
+  function upcase(s) result(u)
+    character(*), intent(in) :: s
+    character(len=len(s)) :: u
+    integer :: i, c
+    u = s
+    do i = 1, len(s)
+      c = ichar(s(i:i))
+      if (c >= ichar('a') .and. c <= ichar('z')) u(i:i) = char(c - 32)
+    end do
+  end function
+
+  ! Sections *MODEL, *KOOPRO, *QDNEVP; 7-character, case-insensitive keywords;
+  ! a keyword's value sits on the next line; input ends at *END OF
+  ! (parse_input.F90:1-67, input_reader_sections.F90:28-161).  Unknown keywords stop.
+  subroutine read_input(fname, cfg)
+    character(*), intent(in) :: fname
+    type(nv2_config), intent(inout) :: cfg
+    character(len=7) :: word, section
+    character(len=256) :: line
+    integer :: u, ios
+    open(newunit=u, file=fname, status='old', action='read', iostat=ios)
+    if (ios /= 0) stop 'ERROR: cannot open input.nevpt'
+    section = '       '
+    do
+      read(u, '(a)', iostat=ios) line
+      if (ios /= 0) exit
+      word = upcase(line(1:7))
+      if (word == '       ') cycle
+      if (word(1:1) == '!' .or. word(1:1) == '#') cycle
+      if (word(1:1) == '*') then
+        section = word
+        if (word == '*END OF' .or. word == '**END O') exit
+        cycle
+      end if
+      select case (section)
+      case ('*MODEL ')
+        select case (word)
+        case ('.CHOLES'); cfg%do_cholesky = .true.
+        case ('.NO4RDM'); cfg%no_4rdm_terms = .true.
+        case ('.DMRGPT')
+        case ('.MIXED '); cfg%prec = 1
+        case default; call bad_kw(word, section)
+        end select
+      case ('*KOOPRO')
+        select case (word)
+        case ('.THRESH'); read(u, *) cfg%thr
+        case ('.SPIN  '); read(u, *)
+        case ('.STATES'); read(u, *) cfg%nr_states
+        case ('.NACTEL'); read(u, *) cfg%nr_active_electrons
+        case ('.NCORE ', '.NACTOR'); read(u, *)
+        case ('.FILE04'); read(u, '(a)') cfg%file04
+        case ('.FILE50'); read(u, *)
+        case ('.NO ZOR'); cfg%zorder = .false.
+        case default; call bad_kw(word, section)
+        end select
+      case ('*QDNEVP')
+        select case (word)
+        case ('.LINDEP'); read(u, *) cfg%thr_lindep
+        case ('.STATES'); read(u, *) cfg%nr_states
+        case ('.FILE04'); read(u, '(a)') cfg%file04
+        case ('.NO PC '); cfg%no_pc = .true.
+        case ('.NATORB')
+        case ('.FROZEN'); call bad_kw(word, section)
+        case default; call bad_kw(word, section)
+        end select
+      case default
+      end select
+    end do
+    close(u)
+  end subroutine
+
+  subroutine bad_kw(word, section)
+    character(*), intent(in) :: word, section
+    write(*, '(a)') ' illegal keyword '//word//' in section '//section
+    stop 'illegal keyword in input'
+  end subroutine
+
+  function last_error() result(msg)
+    character(len=1024) :: msg
+    character(kind=c_char) :: buf(1024)
+    integer :: i, rc
+    rc = nv2_last_error(buf, 1024_c_int64_t)
+    msg = ' '
+    do i = 1, 1024
+      if (buf(i) == c_null_char) exit
+      msg(i:i) = buf(i)
+    end do
+  end function
+
+end module nevpt2_hip

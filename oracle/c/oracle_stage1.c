@@ -484,7 +484,7 @@ void orc_koopman0pE(i64 n, i64 ncore, i64 M, const double *atwo, const double *d
 /* ---------------- bamat, C1 branch (koopro4QD.F90:3974-4104) */
 #define A6(arr, s, p, q, r, t, u, v) arr[(s) + M * ((p) + n * ((q) + n * ((r) + n * ((t) + n * ((u) + n * (v))))))]
 void orc_bamat(i64 n, i64 ncore, i64 M, const double *atwo, const double *ro4, const double *daaa,
-               const double *taa, const double *dal, const double *aj, double *amat, double *atmat) {
+               const double *taa, const double *dal, const double *aj, double *amat, double *atmat, i64 pair_limit) {
     MKR(n, M, daaa, taa, dal, ro4);
     double *F = malloc(sizeof(double) * n * n);
     make_f(n, ncore, aj, atwo, 0.5, F);
@@ -493,9 +493,11 @@ void orc_bamat(i64 n, i64 ncore, i64 M, const double *atwo, const double *ro4, c
     memset(atmat, 0, sizeof(double) * M * n6);
     /* a and c occupy the same slots in every target (amat(.,ap,bp,cp,a,b,c), atmat(.,ap,b,cp,a,bp,c)),
        so iterations with different (a,c) never touch the same element */
-#pragma omp parallel for collapse(2) schedule(dynamic)
-    for (i64 a = 0; a < n; a++)
-        for (i64 c = 0; c < n; c++)
+    /* pair_limit >= 0 restricts the work to the first pair_limit (a,c) pairs (timing samples only) */
+    const i64 npairs = (pair_limit >= 0 && pair_limit < n * n) ? pair_limit : n * n;
+#pragma omp parallel for schedule(dynamic)
+    for (i64 ac = 0; ac < npairs; ac++) {
+        const i64 a = ac / n, c = ac % n;
             for (i64 b = 0; b < n; b++)
                 for (i64 ap = 0; ap < n; ap++)
                     for (i64 bp = 0; bp < n; bp++)
@@ -554,6 +556,7 @@ void orc_bamat(i64 n, i64 ncore, i64 M, const double *atwo, const double *ro4, c
                                         }
                                 }
                             }
+    }
     free(F);
 }
 

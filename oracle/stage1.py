@@ -123,10 +123,11 @@ def koopman0pE(n, ncore, M, atwo, daaa, taa, dal, aj):
     return ga, gd, gf
 
 
-def bamat(n, ncore, M, atwo, ro4, daaa, taa, dal, aj):
+def bamat(n, ncore, M, atwo, ro4, daaa, taa, dal, aj, pair_limit=-1):
+    """A-matrices; pair_limit >= 0 computes only the first (a,c) outer pairs (timing samples)."""
     am, at = _z(M, *(n,) * 6), _z(M, *(n,) * 6)
     lib().orc_bamat(i64(n), i64(ncore), i64(M), P(F(atwo)), P(F(ro4)), P(F(daaa)), P(F(taa)), P(F(dal)),
-                    P(np.ascontiguousarray(aj)), P(am), P(at))
+                    P(np.ascontiguousarray(aj)), P(am), P(at), i64(pair_limit))
     return am, at
 
 

@@ -19,6 +19,18 @@ import numpy as np
 
 THR_LINDEP = 1.0e-6     # thr_lindep default (nevpt2_cfg.F90)
 
+# timing samples (bench cpu_baseline only): outer external loops run on a fraction of their range
+_FRAC = None
+_SCALES = []
+
+
+def _outer(N):
+    if _FRAC is None or N == 0:
+        return range(N)
+    k = max(1, int(np.ceil(_FRAC * N)))
+    _SCALES.append(N / k)
+    return range(min(N, k))
+
 
 # ---------------------------------------------------------------------------- helpers
 def tri_idx(norb):
@@ -149,7 +161,7 @@ def v0(c: Ctx):
     e = c.eps
     ec, ev = e[:nc], e[nc + c.n:]
     E = P = 0.0
-    for ii in range(nc):
+    for ii in _outer(nc):
         for ij in range(ii, nc):
             # traint(r,s) = (ij r | s ii)
             T = np.einsum("lr,ls->rs", c.Bcv[:, ij, :], c.Bcv[:, ii, :])
@@ -182,7 +194,7 @@ def v1k(c: Ctx, thr=THR_LINDEP):
     coef, w, nnew, _ = pc_solve(2.0 * np.eye(n) - c.da, Kpc, thr)
     s_pc = ro.T @ coef                               # s(a,mu) = sum_a' ro(a',a) coef(a',mu)
     Epc = Ppc = 0.0
-    for ii in range(nc):
+    for ii in _outer(nc):
         for ij in range(ii, nc):
             Tj = np.einsum("la,lr->ar", c.Bca[:, ii, :], c.Bcv[:, ij, :])     # (a ii | r ij)
             Tk = np.einsum("la,lr->ar", c.Bca[:, ij, :], c.Bcv[:, ii, :])     # (a ij | r ii)
@@ -222,7 +234,7 @@ def vm1k(c: Ctx, thr=THR_LINDEP):
     coef, w, nnew, _ = pc_solve(c.da.copy(), gkm1a * 0.5, thr)
     s_pc = c.da.T @ coef                               # s(a,mu) = sum_a' da(a',a) coef(a',mu)
     Esc = Psc = Epc = Ppc = 0.0
-    for ir in range(nv):
+    for ir in _outer(nv):
         for is_ in range(ir, nv):
             Tj = np.einsum("li,la->ia", c.Bcv[:, :, ir], c.Bav[:, :, is_])     # (r i | s a)
             Tk = np.einsum("li,la->ia", c.Bcv[:, :, is_], c.Bav[:, :, ir])     # (s i | r a)
@@ -275,7 +287,7 @@ def v2mod(c: Ctx, thr=THR_LINDEP):
     ec = e[:nc]
     daat, gkp2aa = c.koop["2-HRDM"], c.koop["2-EAKO"]
     Esc = Psc = 0.0
-    for ir in range(nc):
+    for ir in _outer(nc):
         for is_ in range(ir, nc):
             T = np.einsum("la,lb->ab", c.Bca[:, ir, :], c.Bca[:, is_, :])   # (i a | j b)
             cint = np.einsum("CD,cd->CDcd", T, T)                            # dint1(c',d')*dint2(c,d)
@@ -296,7 +308,7 @@ def v2mod(c: Ctx, thr=THR_LINDEP):
     coef, w, nnew, _ = pc_solve(fillcm2(daat), fillcm2(gkp2aa), thr)
     s_pc = fillcm2(daat).T @ coef                     # s(ab,mu) = sum_a'b' daat(a',b',a,b) coef(a'b',mu)
     Epc = Ppc = 0.0
-    for i in range(nc):
+    for i in _outer(nc):
         for j in range(i + 1, nc):
             T = np.einsum("la,lb->ab", c.Bca[:, i, :], c.Bca[:, j, :])     # (i a | j b)
             v = T.T.reshape(-1)                                              # traint(b,a) at ab = a*n+b
@@ -311,7 +323,7 @@ def v2mod(c: Ctx, thr=THR_LINDEP):
     coef, w, nnew, _ = pc_solve(Sp, Kp, thr)
     Mh = np.array([[0.5 * (daat[ap, bp, a, b] + daat[ap, bp, b, a]) for (a, b) in iu] for (ap, bp) in iu])
     s_pc = Mh.T @ coef
-    for i in range(nc):
+    for i in _outer(nc):
         T = np.einsum("la,lb->ab", c.Bca[:, i, :], c.Bca[:, i, :])
         v = np.array([T[a, a] if a == b else 2.0 * T[b, a] for (a, b) in iu])
         aij = v @ s_pc
@@ -329,7 +341,7 @@ def vm2mod(c: Ctx, thr=THR_LINDEP):
     daa, gkm2aa = c.daa, c.koop["2-IPKO"]
     Bvr = c.Bav.transpose(0, 2, 1)                    # [l, r, a] = B'(r a)
     Esc = Psc = 0.0
-    for ir in range(nv):
+    for ir in _outer(nv):
         for is_ in range(ir, nv):
             D = np.einsum("la,lb->ab", Bvr[:, is_, :], Bvr[:, ir, :])     # dint1(a,b) = (s a | r b)
             cint = np.einsum("cd,CD->CDcd", D, D)
@@ -349,7 +361,7 @@ def vm2mod(c: Ctx, thr=THR_LINDEP):
     coef, w, nnew, _ = pc_solve(fillcm2(daa), fillcm2(gkm2aa), thr)
     s_pc = fillcm2(daa).T @ coef
     Epc = Ppc = 0.0
-    for ir in range(nv):
+    for ir in _outer(nv):
         for is_ in range(ir + 1, nv):
             D = np.einsum("la,lb->ab", Bvr[:, ir, :], Bvr[:, is_, :])     # (r a | s b)
             v = D.T.reshape(-1)                                              # dint1(b,a) at ab=a*n+b
@@ -363,7 +375,7 @@ def vm2mod(c: Ctx, thr=THR_LINDEP):
     coef, w, nnew, _ = pc_solve(Sp, Kp, thr)
     Mh = np.array([[0.5 * (daa[ap, bp, a, b] + daa[ap, bp, b, a]) for (a, b) in iu] for (ap, bp) in iu])
     s_pc = Mh.T @ coef
-    for ir in range(nv):
+    for ir in _outer(nv):
         D = np.einsum("la,lb->ab", Bvr[:, ir, :], Bvr[:, ir, :])
         v = np.array([D[a, b] * (1.0 if a == b else 2.0) for (a, b) in iu])
         rs = v @ s_pc
@@ -391,7 +403,7 @@ def v0pmod(c: Ctx, thr=THR_LINDEP):
     G1 = np.einsum("BAab->ABab", ga)
     G2 = np.einsum("BAab->ABab", gd)
     Esc = Psc = 0.0
-    for ii in range(nc):
+    for ii in _outer(nc):
         Dj = np.einsum("lr,lab->rab", c.Bcv[:, ii, :], c.Baa)                 # (i r | a b)
         Dk = np.einsum("la,lrb->rab", c.Bca[:, ii, :], c.Bav.transpose(0, 2, 1)[:, :, :])  # (i a | r b)
         for ir in range(nv):
@@ -434,7 +446,7 @@ def v0pmod(c: Ctx, thr=THR_LINDEP):
     spp = V2p @ coef[nn:]
     daT = da.T.reshape(-1)                                   # da(b,a) at ab = a*n+b
     Epc = Ppc = 0.0
-    for i in range(nc):
+    for i in _outer(nc):
         Dj = np.einsum("lr,lab->rab", c.Bcv[:, i, :], c.Baa).reshape(nv, nn)
         Dk = np.einsum("la,lrb->rab", c.Bca[:, i, :], c.Bav.transpose(0, 2, 1)).reshape(nv, nn)
         hh = heff[i]                                         # (r)
@@ -482,7 +494,7 @@ def _pm1_prime(c: Ctx, plus: bool, thr=THR_LINDEP):
     Q2 = np.einsum("CABa->ABCa", X2).reshape(n ** 3, n)       # X2(c',a',b',a)
     BC = (np.einsum("ABCa->ABCa", bmat) + np.einsum("aABC->ABCa", cmat)).reshape(n ** 3, n)
     Esc = Psc = 0.0
-    for x in range(next_):
+    for x in _outer(next_):
         v = np.einsum("acb->abc", D[x]).reshape(-1)          # v[a'b'c'] = dint1(a',c',b')
         h = heff[:, x]
         dumm = float(h @ ro1 @ h)
@@ -504,7 +516,7 @@ def _pm1_prime(c: Ctx, plus: bool, thr=THR_LINDEP):
     Sb = coefT @ M                                                   # S[mu, abc] = sum coefT[mu,a'b'c'] X3(c',a',b',b,a,c)
     sp = coefT @ Q2                                                  # sp[mu, a]
     Epc = Ppc = 0.0
-    for x in range(next_):
+    for x in _outer(next_):
         v = np.einsum("acb->abc", D[x]).reshape(-1)                  # dint1(a,c,b) at abc
         rmu = sp @ heff[:, x] + Sb @ v
         deno = w - ext_e[x] if plus else w + ext_e[x]
@@ -538,3 +550,28 @@ def state_energies(P, koop, s=0, thr=THR_LINDEP, with_4rdm=True):
             e_sc, p_sc, e_pc, p_pc = fn(c, thr)
             out[k] = (p_sc, e_sc, p_pc, e_pc)
     return out, c
+
+
+def state_energies_timed(P, koop, s=0, frac=0.05, thr=THR_LINDEP):
+    """Timing sample for the CPU baseline: each class's outer external loops run on
+    ``frac`` of their range; each class's time is scaled by the largest range/sample
+    ratio it used.  Returns (table of the partial sums, per-class seconds, total seconds)."""
+    import time
+    global _FRAC
+    t0 = time.perf_counter()
+    c = Ctx(P, koop, s)
+    t_ctx = time.perf_counter() - t0
+    out = np.zeros((8, 4))
+    secs = {}
+    _FRAC = frac
+    try:
+        for k, fn in ((0, v0), (1, v1k), (2, vm1k), (3, v2mod), (4, vm2mod), (7, v0pmod), (6, vmupE), (5, vpupE)):
+            _SCALES.clear()
+            t0 = time.perf_counter()
+            e_sc, p_sc, e_pc, p_pc = fn(c) if fn is v0 else fn(c, thr)
+            dt = time.perf_counter() - t0
+            secs[CLASS_NAMES[k]] = dt * (max(_SCALES) if _SCALES else 1.0)
+            out[k] = (p_sc, e_sc, p_pc, e_pc)
+    finally:
+        _FRAC = None
+    return out, secs, t_ctx + sum(secs.values())

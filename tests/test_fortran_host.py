@@ -1,0 +1,59 @@
+"""The Fortran host (input.nevpt reader + ISO_C_BINDING calls into the GPU library)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from nevpt2_amd import synth, bundle
+from oracle import stage1 as S1, stage2 as S2
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "nevpt2_amd", "nevpt2_amd.x")
+
+INPUT = """*MODEL
+.CHOLES
+*KOOPRO
+.STATES
+1
+.NACTEL
+{nele}
+.NO ZOR
+*QDNEVP
+.LINDEP
+1.0d-6
+.FILE04
+{bundle}
+*END OF
+"""
+
+
+def _need_exe():
+    if not os.path.exists(EXE):
+        pytest.fail("nevpt2_amd.x not built (run __graft_entry__.build())")
+
+
+def test_host_rejects_unknown_keyword(tmp_path):
+    _need_exe()
+    (tmp_path / "input.nevpt").write_text("*QDNEVP\n.BOGUS \n*END OF\n")
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert "illegal keyword .BOGUS" in r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+def test_host_end_to_end(gpu, tmp_path):
+    _need_exe()
+    print("[fortran host] CAS(4,4) nc=2 nv=5", flush=True)
+    P = synth.Problem(4, 2, 5, seed=3)
+    bundle.write_bundle(tmp_path / "in.bin", P, e0=[-1.5])
+    (tmp_path / "input.nevpt").write_text(INPUT.format(nele=P.nele, bundle="in.bin"))
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    tab = bundle.read_table(r.stdout)
+    koop = S1.koopmans_all(P)
+    ref, _ = S2.state_energies(P, koop, 0)
+    # the printed table carries 15 decimals
+    tol = 1e-9 + 1e-11 * np.abs(ref)
+    assert (np.abs(tab[0][:, 1] - ref[:, 1]) <= tol[:, 1]).all()
+    assert (np.abs(tab[0][:, 3] - ref[:, 3]) <= tol[:, 3]).all()
+    assert " @   1" in r.stdout

@@ -1,0 +1,36 @@
+"""GPU parity of stage 2 (all eight classes, SC and PC) against the numpy oracle."""
+import numpy as np
+import pytest
+
+from nevpt2_amd import synth, qdnevpt
+from oracle import stage1 as S1, stage2 as S2
+
+CASES = [  # (nact, ncore, nvirt, seed)
+    (4, 2, 3, 11),
+    (6, 3, 8, 1),
+    (6, 4, 12, 2),
+]
+
+
+@pytest.fixture(scope="module", params=CASES, ids=lambda c: f"cas{c[0]}_c{c[1]}_v{c[2]}")
+def case(request):
+    n, nc, nv, seed = request.param
+    print(f"[stage2 case] nact={n} ncore={nc} nvirt={nv} seed={seed}", flush=True)
+    P = synth.Problem(n, nc, nv, seed=seed)
+    koop = S1.koopmans_all(P)
+    ref, _ = S2.state_energies(P, koop, 0)
+    return P, koop, ref
+
+
+@pytest.mark.gpu
+def test_class_energies_fp64(gpu, case):
+    P, koop, ref = case
+    got = qdnevpt.state_energies(P.ncore, P.nact, P.nvirt, P.metat, P.B, P.oneint, P.dal, P.taa, P.daaa, koop, 0)
+    for k, name in enumerate(S2.CLASS_NAMES):
+        # energies to 1e-9 Eh per class; random CI vectors can put PC denominators near zero
+        # (intruders), which amplifies round-off, so the bound also grows with |E| at 1e-11
+        tol = lambda e: 1e-9 + 1e-11 * abs(e)
+        assert abs(got[k, 1] - ref[k, 1]) <= tol(ref[k, 1]), (name, "e_sc", got[k, 1], ref[k, 1])
+        assert abs(got[k, 3] - ref[k, 3]) <= tol(ref[k, 3]), (name, "e_pc", got[k, 3], ref[k, 3])
+        for j in (0, 2):
+            assert abs(got[k, j] - ref[k, j]) <= 1e-9 * max(1.0, abs(ref[k, j])), (name, j, got[k, j], ref[k, j])
