@@ -52,6 +52,11 @@ __device__ __forceinline__ i64 ix6(const Rdm &R, int s, int p, int q, int r, int
 #define D2(s, p, q, r, t) (R.d2[ix4(R, s, p, q, r, t)])
 #define D3(s, p, q, r, t, u, v) (R.d3[ix6(R, s, p, q, r, t, u, v)])
 
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
 // lindice of a 0-based descending tuple, 0-based result (indices_norm.F90:41-44)
 __host__ __device__ __forceinline__ i64 lind0(i64 a, i64 b, i64 c, i64 d) {
     a++; b++; c++; d++;

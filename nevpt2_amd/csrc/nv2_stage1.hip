@@ -1,8 +1,9 @@
 // Stage 1 of the NEVPT2 hot path on gfx950: Koopmans-matrix builds.
 //
-// O(n^7) builds (koopE, 2-hole RDM, koop2E, koopman0pE, bbmat, bcmat, bdmat,
-// partial traces) run one thread per output element with the reference's
-// inner loop nest, reading RDMs that stay resident in HBM/MALL.  The O(n^9)
+// O(n^7) builds (koopE, koop2E, koopman0pE, bbmat, bcmat, bdmat) run one
+// wavefront per output element, the lanes sharing the reference's inner loop
+// nest; RDMs stay resident in HBM/MALL.  Partial traces and the 2-hole RDM run
+// one thread per element.  The O(n^9)
 // 4-RDM part of bamat is split off: the full <E E E E> tensor E4 is
 // materialised once per state and contracted with the active ERIs as four
 // strided GEMMs (nv2_contract.hip); everything lower-order in bamat runs in
@@ -107,20 +108,24 @@ __global__ void k_atwo(int n, int ncore, int L, const double *B, double *atwo) {
     for (int m = 0; m < L; m++) x += u[m] * v[m];
     atwo[idx] = x;
 }
+// one wavefront per active pair a<=b; lanes split the Cholesky index
 __global__ void k_aj(int n, int ncore, int L, const double *B, double *aj) {
-    int a = blockIdx.x, b = threadIdx.x;          // a <= b active
-    if (b < a || b >= n) return;
-    i64 ac = a + ncore, bc = b + ncore, jab = tri(ac, bc);
-    double acc = aj[jab];
+    const int pr = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (pr >= n * n) return;
+    const int a = pr / n, b = pr % n;
+    if (b < a) return;                          // uniform per wavefront
+    const i64 ac = a + ncore, bc = b + ncore, jab = tri(ac, bc);
+    double acc = 0.0;
     for (int j = 0; j < ncore; j++) {
         const double *u = B + (i64)L * tri(ac, bc), *v = B + (i64)L * tri(j, j);
+        const double *u2 = B + (i64)L * tri(ac, j), *v2 = B + (i64)L * tri(j, bc);
         double x = 0, y = 0;
-        for (int l = 0; l < L; l++) x += u[l] * v[l];
-        u = B + (i64)L * tri(ac, j); v = B + (i64)L * tri(j, bc);
-        for (int l = 0; l < L; l++) y += u[l] * v[l];
+        for (int l = lane; l < L; l += 64) { x += u[l] * v[l]; y += u2[l] * v2[l]; }
+        x = wave_sum(x);
+        y = wave_sum(y);
         acc = acc + 2.0 * x - y;
     }
-    aj[jab] = acc;
+    if (lane == 0) aj[jab] = aj[jab] + acc;
 }
 
 // active f(ab) = aj(ab) - alpha * sum_c atwo(a,c,c,b) (e.g. koopro4QD.F90:3993-4005)
@@ -139,27 +144,34 @@ __global__ void k_aj_active(int n, int ncore, const double *aj, double *A) {
     A[a + n * b] = aj[tri(a + ncore, b + ncore)];
 }
 
+// The O(n^7) builds below run one wavefront per output element; the 64 lanes
+// split the flattened n^3 inner loop of the reference and reduce with shuffles.
+#define WAVE_SETUP(ntot)                                                          \
+    const i64 wid = ((i64)blockIdx.x * blockDim.x + threadIdx.x) >> 6;          \
+    const int lane = threadIdx.x & 63;                                            \
+    if (wid >= (ntot)) return;
+
 // ---------------------------------------------------------------- koopE (koop_matrices.F90:165-250); F = aj
 __global__ void k_koopE(Rdm R, const double *atwo, const double *F, double *ipa, double *eaa) {
     const int n = R.n, M = R.M;
-    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (i64)M * n * n) return;
+    WAVE_SETUP((i64)M * n * n);
     int s, x[2];
-    decode(idx, M, n, 2, &s, x);
+    decode(wid, M, n, 2, &s, x);
     const int ap = x[0], a = x[1];
-    double ip = 0.0;
-    for (int c = 0; c < n; c++) {
-        ip = ip - FF(a, c) * D1(s, ap, c);
-        for (int d = 0; d < n; d++)
-            for (int ie = 0; ie < n; ie++) ip = ip - AT(c, ie, a, d) * D2(s, ap, c, d, ie);
+    double ip = 0.0, dumm = 0.0;
+    const int n3 = n * n * n;
+    for (int t = lane; t < n3; t += 64) {
+        const int c = t / (n * n), d = (t / n) % n, ie = t % n;
+        if (d == 0 && ie == 0) ip = ip - FF(a, c) * D1(s, ap, c);
+        ip = ip - AT(c, ie, a, d) * D2(s, ap, c, d, ie);
+        if (c == 0) dumm = dumm + (2.0 * AT(d, ie, ap, a) - AT(ap, ie, d, a)) * D1(s, d, ie);   // (d,ie) = t
     }
-    ip = 2.0 * ip;
-    // eaa(ap,a) = ipa(ap,a) + 2*dumm with dumm from (ap,a)
-    double dumm = 2.0 * FF(ap, a);
-    for (int d = 0; d < n; d++)
-        for (int ie = 0; ie < n; ie++) dumm = dumm + (2.0 * AT(d, ie, ap, a) - AT(ap, ie, d, a)) * D1(s, d, ie);
-    ipa[idx] = ip;
-    eaa[idx] = ip + 2.0 * dumm;
+    ip = 2.0 * wave_sum(ip);
+    dumm = 2.0 * FF(ap, a) + wave_sum(dumm);
+    if (lane == 0) {
+        ipa[wid] = ip;
+        eaa[wid] = ip + 2.0 * dumm;
+    }
 }
 
 // ---------------------------------------------------------------- 2-hole RDM (koop_matrices.F90:78-88)
@@ -175,165 +187,169 @@ __global__ void k_hrdm(Rdm R, double *out) {
 // ---------------------------------------------------------------- koop2E (koop_matrices.F90:253-386)
 __global__ void k_koop2E(Rdm R, const double *atwo, const double *F, double *koopaa, double *koopeaa) {
     const int n = R.n, M = R.M;
-    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (i64)M * n * n * n * n) return;
+    WAVE_SETUP((i64)M * n * n * n * n);
     int s, x[4];
-    decode(idx, M, n, 4, &s, x);
+    decode(wid, M, n, 4, &s, x);
     const int ap = x[0], bp = x[1], a = x[2], b = x[3];
-    double ip = 0.0;
-    for (int d = 0; d < n; d++) {
-        ip = ip - FF(a, d) * D2(s, ap, bp, d, b) - FF(b, d) * D2(s, ap, bp, a, d);
-        for (int ie = 0; ie < n; ie++)
-            for (int ief = 0; ief < n; ief++) {
-                double dint = AT(d, ie, a, ief), dint2 = AT(d, ie, b, ief);
-                double dum = D3(s, ap, bp, d, ief, b, ie);
-                if (d == ief) dum = dum + 0.5 * D2(s, ap, bp, ie, b);
-                if (d == b) dum = dum + 0.5 * D2(s, ap, bp, ief, ie);
-                ip = ip - dint * dum;
-                dum = D3(s, ap, bp, d, a, ief, ie);
-                if (d == a) dum = dum + 0.5 * D2(s, ap, bp, ie, ief);
-                if (d == ief) dum = dum + 0.5 * D2(s, ap, bp, a, ie);
-                ip = ip - dint2 * dum;
-            }
+    double ip = 0.0, ea = 0.0;
+    const int n3 = n * n * n;
+    for (int t = lane; t < n3; t += 64) {
+        const int d = t / (n * n), ie = (t / n) % n, ief = t % n;
+        if (ie == 0 && ief == 0) ip = ip - FF(a, d) * D2(s, ap, bp, d, b) - FF(b, d) * D2(s, ap, bp, a, d);
+        double dint = AT(d, ie, a, ief), dint2 = AT(d, ie, b, ief);
+        double dum = D3(s, ap, bp, d, ief, b, ie);
+        if (d == ief) dum = dum + 0.5 * D2(s, ap, bp, ie, b);
+        if (d == b) dum = dum + 0.5 * D2(s, ap, bp, ief, ie);
+        ip = ip - dint * dum;
+        dum = D3(s, ap, bp, d, a, ief, ie);
+        if (d == a) dum = dum + 0.5 * D2(s, ap, bp, ie, ief);
+        if (d == ief) dum = dum + 0.5 * D2(s, ap, bp, a, ie);
+        ip = ip - dint2 * dum;
+        // EA part: loop indices (c, d2, ie2) = (d, ie, ief)
+        const int c = d, dd = ie, e2 = ief;
+        if (dd == 0 && e2 == 0) ea = ea + FF(a, c) * ro2t(R, s, ap, bp, c, b) + FF(b, c) * ro2t(R, s, ap, bp, a, c);
+        dint = AT(c, e2, dd, a);
+        dum = -ro3t(R, s, ap, bp, e2, dd, b, c);
+        if (c == e2) dum = dum + 2.0 * ro2t(R, s, ap, bp, dd, b);
+        if (dd == e2) dum = dum - 0.5 * ro2t(R, s, ap, bp, c, b);
+        if (b == e2) dum = dum - 0.5 * ro2t(R, s, ap, bp, dd, c);
+        ea = ea + dint * dum;
+        dint = AT(c, e2, dd, b);
+        dum = -ro3t(R, s, ap, bp, e2, a, dd, c);
+        if (c == e2) dum = dum + 2.0 * ro2t(R, s, ap, bp, a, dd);
+        if (a == e2) dum = dum - 0.5 * ro2t(R, s, ap, bp, c, dd);
+        if (dd == e2) dum = dum - 0.5 * ro2t(R, s, ap, bp, a, c);
+        ea = ea + dint * dum;
     }
-    double ea = 0.0;
-    for (int c = 0; c < n; c++) {
-        ea = ea + FF(a, c) * ro2t(R, s, ap, bp, c, b) + FF(b, c) * ro2t(R, s, ap, bp, a, c);
-        for (int d = 0; d < n; d++)
-            for (int ie = 0; ie < n; ie++) {
-                double dint = AT(c, ie, d, a);
-                double dum = -ro3t(R, s, ap, bp, ie, d, b, c);
-                if (c == ie) dum = dum + 2.0 * ro2t(R, s, ap, bp, d, b);
-                if (d == ie) dum = dum - 0.5 * ro2t(R, s, ap, bp, c, b);
-                if (b == ie) dum = dum - 0.5 * ro2t(R, s, ap, bp, d, c);
-                ea = ea + dint * dum;
-                dint = AT(c, ie, d, b);
-                dum = -ro3t(R, s, ap, bp, ie, a, d, c);
-                if (c == ie) dum = dum + 2.0 * ro2t(R, s, ap, bp, a, d);
-                if (a == ie) dum = dum - 0.5 * ro2t(R, s, ap, bp, c, d);
-                if (d == ie) dum = dum - 0.5 * ro2t(R, s, ap, bp, a, c);
-                ea = ea + dint * dum;
-            }
+    ip = wave_sum(ip);
+    ea = wave_sum(ea);
+    if (lane == 0) {
+        koopaa[wid] = ip;
+        koopeaa[wid] = ea;
     }
-    koopaa[idx] = ip;
-    koopeaa[idx] = ea;
 }
 
 // ---------------------------------------------------------------- koopman0pE (koop_matrices.F90:389-588)
-// thread index (s,bp,ap,a,b) matches the output layout koopaa(s,bp,ap,a,b)
+// wave index (s,bp,ap,a,b) matches the output layout koopaa(s,bp,ap,a,b)
 __global__ void k_koop0p(Rdm R, const double *atwo, const double *F, double *koopaa, double *koopbb) {
     const int n = R.n, M = R.M;
-    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (i64)M * n * n * n * n) return;
+    WAVE_SETUP((i64)M * n * n * n * n);
     int s, x[4];
-    decode(idx, M, n, 4, &s, x);
+    decode(wid, M, n, 4, &s, x);
     const int bp = x[0], ap = x[1], a = x[2], b = x[3];
-    double ka = 0.0;
-    for (int c = 0; c < n; c++) {
-        ka = ka + 2.0 * (FF(c, a) * ee2(R, s, bp, ap, c, b) - FF(b, c) * ee2(R, s, bp, ap, a, c));
-        for (int d = 0; d < n; d++)
-            for (int ie = 0; ie < n; ie++) {
-                double dum1 = eee2(R, s, bp, ap, c, ie, d, b) + eee2(R, s, bp, ap, d, b, c, ie);
-                double dum2 = eee2(R, s, bp, ap, a, ie, c, d) + eee2(R, s, bp, ap, c, d, a, ie);
-                ka = ka + AT(c, ie, d, a) * dum1 - AT(b, ie, c, d) * dum2;
-            }
+    double ka = 0.0, kb = 0.0;
+    const int n3 = n * n * n;
+    for (int t = lane; t < n3; t += 64) {
+        const int c = t / (n * n), d = (t / n) % n, ie = t % n;
+        if (d == 0 && ie == 0) {
+            ka = ka + 2.0 * (FF(c, a) * ee2(R, s, bp, ap, c, b) - FF(b, c) * ee2(R, s, bp, ap, a, c));
+            double dum1 = -ee2(R, s, a, ap, bp, c);
+            if (a == ap) dum1 = dum1 + 2.0 * D1(s, bp, c);
+            if (bp == ap) dum1 = dum1 + D1(s, a, c);
+            double dum2 = -ee2(R, s, c, ap, bp, b);
+            if (ap == c) dum2 = dum2 + 2.0 * D1(s, bp, b);
+            if (bp == ap) dum2 = dum2 + D1(s, c, b);
+            kb = kb - FF(b, c) * dum1 + FF(c, a) * dum2;
+        }
+        {
+            double dum1 = eee2(R, s, bp, ap, c, ie, d, b) + eee2(R, s, bp, ap, d, b, c, ie);
+            double dum2 = eee2(R, s, bp, ap, a, ie, c, d) + eee2(R, s, bp, ap, c, d, a, ie);
+            ka = ka + AT(c, ie, d, a) * dum1 - AT(b, ie, c, d) * dum2;
+        }
+        double e1 = -eee2(R, s, c, ie, a, ap, bp, d) - eee2(R, s, a, ap, bp, d, c, ie);
+        if (ap == a) e1 = e1 + 2.0 * (ee2(R, s, c, ie, bp, d) + ee2(R, s, bp, d, c, ie));
+        if (bp == ap) e1 = e1 + ee2(R, s, c, ie, a, d) + ee2(R, s, a, d, c, ie);
+        if (ap == c) {
+            e1 = e1 - ee2(R, s, a, ie, bp, d);
+            if (ie == a) e1 = e1 + 2.0 * D1(s, bp, d);
+        }
+        if (bp == ie) {
+            e1 = e1 + ee2(R, s, a, ap, c, d);
+            if (ap == a) e1 = e1 - 2.0 * D1(s, c, d);
+        }
+        double e2 = -eee2(R, s, c, ie, d, ap, bp, b) - eee2(R, s, d, ap, bp, b, c, ie);
+        if (ap == d) e2 = e2 + 2.0 * (ee2(R, s, c, ie, bp, b) + ee2(R, s, bp, b, c, ie));
+        if (bp == ap) e2 = e2 + ee2(R, s, c, ie, d, b) + ee2(R, s, d, b, c, ie);
+        if (ap == c) {
+            e2 = e2 - ee2(R, s, d, ie, bp, b);
+            if (d == ie) e2 = e2 + 2.0 * D1(s, bp, b);
+        }
+        if (bp == ie) {
+            e2 = e2 + ee2(R, s, d, ap, c, b);
+            if (d == ap) e2 = e2 - 2.0 * D1(s, c, b);
+        }
+        kb = kb - 0.5 * AT(c, ie, b, d) * e1 + 0.5 * AT(c, ie, d, a) * e2;
     }
-    double kb = 0.0;
-    for (int c = 0; c < n; c++) {
-        double dum1 = -ee2(R, s, a, ap, bp, c);
-        if (a == ap) dum1 = dum1 + 2.0 * D1(s, bp, c);
-        if (bp == ap) dum1 = dum1 + D1(s, a, c);
-        double dum2 = -ee2(R, s, c, ap, bp, b);
-        if (ap == c) dum2 = dum2 + 2.0 * D1(s, bp, b);
-        if (bp == ap) dum2 = dum2 + D1(s, c, b);
-        kb = kb - FF(b, c) * dum1 + FF(c, a) * dum2;
-        for (int d = 0; d < n; d++)
-            for (int ie = 0; ie < n; ie++) {
-                double e1 = -eee2(R, s, c, ie, a, ap, bp, d) - eee2(R, s, a, ap, bp, d, c, ie);
-                if (ap == a) e1 = e1 + 2.0 * (ee2(R, s, c, ie, bp, d) + ee2(R, s, bp, d, c, ie));
-                if (bp == ap) e1 = e1 + ee2(R, s, c, ie, a, d) + ee2(R, s, a, d, c, ie);
-                if (ap == c) {
-                    e1 = e1 - ee2(R, s, a, ie, bp, d);
-                    if (ie == a) e1 = e1 + 2.0 * D1(s, bp, d);
-                }
-                if (bp == ie) {
-                    e1 = e1 + ee2(R, s, a, ap, c, d);
-                    if (ap == a) e1 = e1 - 2.0 * D1(s, c, d);
-                }
-                double e2 = -eee2(R, s, c, ie, d, ap, bp, b) - eee2(R, s, d, ap, bp, b, c, ie);
-                if (ap == d) e2 = e2 + 2.0 * (ee2(R, s, c, ie, bp, b) + ee2(R, s, bp, b, c, ie));
-                if (bp == ap) e2 = e2 + ee2(R, s, c, ie, d, b) + ee2(R, s, d, b, c, ie);
-                if (ap == c) {
-                    e2 = e2 - ee2(R, s, d, ie, bp, b);
-                    if (d == ie) e2 = e2 + 2.0 * D1(s, bp, b);
-                }
-                if (bp == ie) {
-                    e2 = e2 + ee2(R, s, d, ap, c, b);
-                    if (d == ap) e2 = e2 - 2.0 * D1(s, c, b);
-                }
-                kb = kb - 0.5 * AT(c, ie, b, d) * e1 + 0.5 * AT(c, ie, d, a) * e2;
-            }
+    ka = wave_sum(ka);
+    kb = wave_sum(kb);
+    if (lane == 0) {
+        koopaa[wid] = ka;
+        koopbb[wid] = kb;
     }
-    koopaa[idx] = ka;
-    koopbb[idx] = kb;
 }
 __global__ void k_koop0pf(Rdm R, const double *atwo, const double *F, double *koopab) {
     const int n = R.n, M = R.M;
-    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (i64)M * n * n) return;
+    WAVE_SETUP((i64)M * n * n);
     int s, x[2];
-    decode(idx, M, n, 2, &s, x);
+    decode(wid, M, n, 2, &s, x);
     const int a = x[0], b = x[1];
     double v = 0.0;
-    for (int c = 0; c < n; c++) {
-        v = v + 2.0 * (FF(c, a) * D1(s, c, b) - FF(b, c) * D1(s, a, c));
-        for (int d = 0; d < n; d++)
-            for (int ie = 0; ie < n; ie++) {
-                double dum1 = ee2(R, s, c, ie, d, b) + ee2(R, s, d, b, c, ie);
-                double dum2 = ee2(R, s, a, ie, c, d) + ee2(R, s, c, d, a, ie);
-                v = v + AT(c, ie, d, a) * dum1 - AT(b, ie, c, d) * dum2;
-            }
+    const int n3 = n * n * n;
+    for (int t = lane; t < n3; t += 64) {
+        const int c = t / (n * n), d = (t / n) % n, ie = t % n;
+        if (d == 0 && ie == 0) v = v + 2.0 * (FF(c, a) * D1(s, c, b) - FF(b, c) * D1(s, a, c));
+        double dum1 = ee2(R, s, c, ie, d, b) + ee2(R, s, d, b, c, ie);
+        double dum2 = ee2(R, s, a, ie, c, d) + ee2(R, s, c, d, a, ie);
+        v = v + AT(c, ie, d, a) * dum1 - AT(b, ie, c, d) * dum2;
     }
-    koopab[idx] = v;
+    v = wave_sum(v);
+    if (lane == 0) koopab[wid] = v;
 }
 
 // ---------------------------------------------------------------- bbmat (koopro4QD.F90:4231-4299); F alpha=1
 __global__ void k_bbmat(Rdm R, const double *atwo, const double *F, const double *AJ, double *bmat, double *btmat) {
     const int n = R.n, M = R.M;
-    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (i64)M * n * n * n * n) return;
+    WAVE_SETUP((i64)M * n * n * n * n);
     int s, x[4];
-    decode(idx, M, n, 4, &s, x);
+    decode(wid, M, n, 4, &s, x);
     const int ap = x[0], bp = x[1], cp = x[2], a = x[3];
     double bv = 0.0, bt = 0.0;
-    for (int c = 0; c < n; c++) {
-        bv = bv - FF(c, a) * ee2(R, s, cp, ap, bp, c);
-        bt = bt + AJ[c + n * a] * ee2tr(R, s, cp, ap, bp, c);
-        for (int ie = 0; ie < n; ie++)
-            for (int ief = 0; ief < n; ief++) {
-                double dint = AT(a, ie, c, ief);
-                bv = bv - eee2(R, s, cp, ap, bp, ie, c, ief) * dint;
-                bt = bt + eee2t(R, s, cp, ap, bp, ie, c, ief) * dint;
-            }
+    const int n3 = n * n * n;
+    for (int t = lane; t < n3; t += 64) {
+        const int c = t / (n * n), ie = (t / n) % n, ief = t % n;
+        if (ie == 0 && ief == 0) {
+            bv = bv - FF(c, a) * ee2(R, s, cp, ap, bp, c);
+            bt = bt + AJ[c + n * a] * ee2tr(R, s, cp, ap, bp, c);
+        }
+        double dint = AT(a, ie, c, ief);
+        bv = bv - eee2(R, s, cp, ap, bp, ie, c, ief) * dint;
+        bt = bt + eee2t(R, s, cp, ap, bp, ie, c, ief) * dint;
     }
-    bmat[idx] = bv;
-    btmat[idx] = bt;
+    bv = wave_sum(bv);
+    bt = wave_sum(bt);
+    if (lane == 0) {
+        bmat[wid] = bv;
+        btmat[wid] = bt;
+    }
 }
 
 // ---------------------------------------------------------------- bcmat (koopro4QD.F90:4301-4397); F alpha=1/2
 __global__ void k_bcmat(Rdm R, const double *atwo, const double *F, double *cmat, double *ctmat) {
     const int n = R.n, M = R.M;
-    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (i64)M * n * n * n * n) return;
+    WAVE_SETUP((i64)M * n * n * n * n);
     int s, x[4];
-    decode(idx, M, n, 4, &s, x);
+    decode(wid, M, n, 4, &s, x);
     const int ap = x[0], a = x[1], b = x[2], c = x[3];
     double cv = 0.0, ct = 0.0;
-    for (int d = 0; d < n; d++) {
-        double fda = FF(d, a), fcd = FF(c, d), fbd = FF(b, d);
-        cv = cv + fda * ee2(R, s, ap, b, d, c) - fcd * ee2(R, s, ap, b, a, d) - fbd * ee2(R, s, ap, d, a, c);
-        ct = ct + fda * ee2t(R, s, ap, b, d, c) - fcd * ee2t(R, s, ap, b, a, d) + fbd * ee2t(R, s, ap, d, a, c);
-        for (int ie = 0; ie < n; ie++) {
+    const int n3 = n * n * n;
+    for (int t = lane; t < n3; t += 64) {
+        const int d = t / (n * n), ie = (t / n) % n, ief = t % n;
+        if (ie == 0 && ief == 0) {
+            double fda = FF(d, a), fcd = FF(c, d), fbd = FF(b, d);
+            cv = cv + fda * ee2(R, s, ap, b, d, c) - fcd * ee2(R, s, ap, b, a, d) - fbd * ee2(R, s, ap, d, a, c);
+            ct = ct + fda * ee2t(R, s, ap, b, d, c) - fcd * ee2t(R, s, ap, b, a, d) + fbd * ee2t(R, s, ap, d, a, c);
+        }
+        if (ief == 0) {
             double dint1 = AT(c, ie, d, a), dint2 = AT(d, ie, ie, a), dint3 = AT(c, d, d, ie), dint4 = AT(b, d, d, ie);
             double dum1 = ee2(R, s, ap, b, d, ie), dum2 = ee2(R, s, ap, b, d, c);
             double dum3 = ee2(R, s, ap, b, a, ie), dum4 = ee2(R, s, ap, ie, a, c);
@@ -341,43 +357,49 @@ __global__ void k_bcmat(Rdm R, const double *atwo, const double *F, double *cmat
             double dum3t = ee2t(R, s, ap, b, a, ie), dum4t = ee2t(R, s, ap, ie, a, c);
             cv = cv + dum1 * dint1 - 0.5 * (dum2 * dint2 + dum3 * dint3 - dum4 * dint4);
             ct = ct + dum1t * dint1 - 0.5 * (dum2t * dint2 + dum3t * dint3 - dum4t * dint4);
-            for (int ief = 0; ief < n; ief++) {
-                double di1 = AT(d, ief, ie, a), di2 = AT(d, ief, c, ie), di3 = AT(d, ief, b, ie);
-                double du1 = eee2(R, s, ap, b, d, ief, ie, c), du1t = eee2tl(R, s, ap, b, d, ief, ie, c);
-                double du2 = eee2(R, s, ap, b, d, ief, a, ie), du2t = eee2tl(R, s, ap, b, d, ief, a, ie);
-                double du3 = eee2(R, s, ap, ie, d, ief, a, c), du3t = eee2tl(R, s, ap, ie, d, ief, a, c);
-                cv = cv + du1 * di1 - du2 * di2 - du3 * di3;
-                ct = ct + du1t * di1 - du2t * di2 + du3t * di3;
-            }
         }
+        double di1 = AT(d, ief, ie, a), di2 = AT(d, ief, c, ie), di3 = AT(d, ief, b, ie);
+        double du1 = eee2(R, s, ap, b, d, ief, ie, c), du1t = eee2tl(R, s, ap, b, d, ief, ie, c);
+        double du2 = eee2(R, s, ap, b, d, ief, a, ie), du2t = eee2tl(R, s, ap, b, d, ief, a, ie);
+        double du3 = eee2(R, s, ap, ie, d, ief, a, c), du3t = eee2tl(R, s, ap, ie, d, ief, a, c);
+        cv = cv + du1 * di1 - du2 * di2 - du3 * di3;
+        ct = ct + du1t * di1 - du2t * di2 + du3t * di3;
     }
-    cmat[idx] = cv;
-    ctmat[idx] = ct;
+    cv = wave_sum(cv);
+    ct = wave_sum(ct);
+    if (lane == 0) {
+        cmat[wid] = cv;
+        ctmat[wid] = ct;
+    }
 }
 
 // ---------------------------------------------------------------- bdmat (koopro4QD.F90:4399-4459); F alpha=1
 __global__ void k_bdmat(Rdm R, const double *atwo, const double *F, const double *AJ, double *dmat, double *dtmat) {
     const int n = R.n, M = R.M;
-    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (i64)M * n * n) return;
+    WAVE_SETUP((i64)M * n * n);
     int s, x[2];
-    decode(idx, M, n, 2, &s, x);
+    decode(wid, M, n, 2, &s, x);
     const int ap = x[0], a = x[1];
     double dv = 0.0, dt = 0.0;
-    for (int c = 0; c < n; c++) {
-        dv = dv - FF(c, a) * D1(s, ap, c);
-        double dum = -D1(s, ap, c);
-        if (ap == c) dum = dum + 2.0;
-        dt = dt + AJ[c + n * a] * dum;
-        for (int ie = 0; ie < n; ie++)
-            for (int ief = 0; ief < n; ief++) {
-                double dint = AT(a, ie, c, ief);
-                dv = dv - ee2(R, s, ap, ie, c, ief) * dint;
-                dt = dt + ee2t(R, s, ap, ie, c, ief) * dint;
-            }
+    const int n3 = n * n * n;
+    for (int t = lane; t < n3; t += 64) {
+        const int c = t / (n * n), ie = (t / n) % n, ief = t % n;
+        if (ie == 0 && ief == 0) {
+            dv = dv - FF(c, a) * D1(s, ap, c);
+            double dum = -D1(s, ap, c);
+            if (ap == c) dum = dum + 2.0;
+            dt = dt + AJ[c + n * a] * dum;
+        }
+        double dint = AT(a, ie, c, ief);
+        dv = dv - ee2(R, s, ap, ie, c, ief) * dint;
+        dt = dt + ee2t(R, s, ap, ie, c, ief) * dint;
     }
-    dmat[idx] = dv;
-    dtmat[idx] = dt;
+    dv = wave_sum(dv);
+    dt = wave_sum(dt);
+    if (lane == 0) {
+        dmat[wid] = dv;
+        dtmat[wid] = dt;
+    }
 }
 
 // ---------------------------------------------------------------- bamat: E4 = <E E E E> for one state
@@ -465,7 +487,7 @@ int launch_bro(const Rdm &R, int nele, double *daaa, double *taa, double *dal, h
 int launch_aj_atwo(int n, int ncore, int norb, int L, const double *B, const double *oneint, double *aj,
                    double *atwo, hipStream_t st) {
     NV2_HIP(hipMemcpyAsync(aj, oneint, sizeof(double) * (i64)norb * (norb + 1) / 2, hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(k_aj, dim3(n), dim3(64), 0, st, n, ncore, L, B, aj);
+    hipLaunchKernelGGL(k_aj, dim3(nblk((i64)n * n * 64, 256)), dim3(256), 0, st, n, ncore, L, B, aj);
     NV2_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_atwo, dim3(nblk((i64)n * n * n * n, 256)), dim3(256), 0, st, n, ncore, L, B, atwo);
     NV2_LAUNCH_CHECK();
@@ -485,24 +507,24 @@ int launch_koop_small(const Rdm &R, int ncore, const double *aj, const double *a
     hipLaunchKernelGGL(k_aj_active, dim3(n), dim3(64), 0, st, (int)n, ncore, aj, w.AJ);
     NV2_LAUNCH_CHECK();
     // koopE with f = aj
-    hipLaunchKernelGGL(k_koopE, dim3(nblk(t2, 256)), dim3(256), 0, st, R, atwo, w.AJ, o.ipko1, o.eako1);
+    hipLaunchKernelGGL(k_koopE, dim3(nblk(t2 * 64, 256)), dim3(256), 0, st, R, atwo, w.AJ, o.ipko1, o.eako1);
     NV2_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_hrdm, dim3(nblk(t4, 256)), dim3(256), 0, st, R, o.hrdm);
     NV2_LAUNCH_CHECK();
     if (launch_make_f((int)n, ncore, aj, atwo, 0.5, w.Fhalf, st)) return -1;
     if (launch_make_f((int)n, ncore, aj, atwo, 1.0, w.Fone, st)) return -1;
-    hipLaunchKernelGGL(k_koop2E, dim3(nblk(t4, 64)), dim3(64), 0, st, R, atwo, w.Fhalf, o.ipko2, o.eako2);
+    hipLaunchKernelGGL(k_koop2E, dim3(nblk(t4 * 64, 256)), dim3(256), 0, st, R, atwo, w.Fhalf, o.ipko2, o.eako2);
     NV2_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_koop0p, dim3(nblk(t4, 64)), dim3(64), 0, st, R, atwo, w.Fhalf, o.gk0pa, o.gk0pd);
+    hipLaunchKernelGGL(k_koop0p, dim3(nblk(t4 * 64, 256)), dim3(256), 0, st, R, atwo, w.Fhalf, o.gk0pa, o.gk0pd);
     NV2_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_koop0pf, dim3(nblk(t2, 64)), dim3(64), 0, st, R, atwo, w.Fhalf, o.gk0pf);
+    hipLaunchKernelGGL(k_koop0pf, dim3(nblk(t2 * 64, 256)), dim3(256), 0, st, R, atwo, w.Fhalf, o.gk0pf);
     NV2_LAUNCH_CHECK();
     if (o.bmat) {
-        hipLaunchKernelGGL(k_bbmat, dim3(nblk(t4, 64)), dim3(64), 0, st, R, atwo, w.Fone, w.AJ, o.bmat, o.btmat);
+        hipLaunchKernelGGL(k_bbmat, dim3(nblk(t4 * 64, 256)), dim3(256), 0, st, R, atwo, w.Fone, w.AJ, o.bmat, o.btmat);
         NV2_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_bcmat, dim3(nblk(t4, 64)), dim3(64), 0, st, R, atwo, w.Fhalf, o.cmat, o.ctmat);
+        hipLaunchKernelGGL(k_bcmat, dim3(nblk(t4 * 64, 256)), dim3(256), 0, st, R, atwo, w.Fhalf, o.cmat, o.ctmat);
         NV2_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_bdmat, dim3(nblk(t2, 64)), dim3(64), 0, st, R, atwo, w.Fone, w.AJ, o.dmat, o.dtmat);
+        hipLaunchKernelGGL(k_bdmat, dim3(nblk(t2 * 64, 256)), dim3(256), 0, st, R, atwo, w.Fone, w.AJ, o.dmat, o.dtmat);
         NV2_LAUNCH_CHECK();
     }
     return 0;

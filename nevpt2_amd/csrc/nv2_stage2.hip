@@ -12,6 +12,8 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <map>
 #include <string>
 #include <vector>
@@ -747,12 +749,29 @@ static int gemm_sb(Stage2 *p, int m, int nn, int k, const double *A, i64 lda, i6
                                      sA, B, (int)ldb, sB, &zero, C, (int)ldc, sC, batch));
     return 0;
 }
-// eigen-decomposition in place (ascending eigenvalues in w), reference triangle choice
+// eigen-decomposition in place (ascending eigenvalues in w), reference triangle choice.
+// NV2_EIGEN selects the rocSOLVER algorithm: "dc" (syevd, default), "jacobi" (syevj), "dcj" (syevdj)
+static int eigen_algo() {
+    static int algo = -1;
+    if (algo < 0) {
+        const char *e = getenv("NV2_EIGEN");
+        algo = (e && !strcmp(e, "jacobi")) ? 1 : (e && !strcmp(e, "dcj")) ? 2 : 0;
+    }
+    return algo;
+}
 static int syevd(Stage2 *p, bool upper, int N, double *A, double *w) {
     if (N <= 0) return 0;
     GET(E, "syevd_E", N);
-    RB(rocsolver_dsyevd(p->h, rocblas_evect_original, upper ? rocblas_fill_upper : rocblas_fill_lower, N, A, N, w, E,
-                        p->dinfo));
+    const rocblas_fill uplo = upper ? rocblas_fill_upper : rocblas_fill_lower;
+    const int algo = eigen_algo();
+    if (algo == 1) {
+        RB(rocsolver_dsyevj(p->h, rocblas_esort_ascending, rocblas_evect_original, uplo, N, A, N, 0.0, E, 100,
+                            p->dinfo + 1, w, p->dinfo));
+    } else if (algo == 2) {
+        RB(rocsolver_dsyevdj(p->h, rocblas_evect_original, uplo, N, A, N, w, p->dinfo));
+    } else {
+        RB(rocsolver_dsyevd(p->h, rocblas_evect_original, uplo, N, A, N, w, E, p->dinfo));
+    }
     int info = 0;
     NV2_HIP(hipMemcpyAsync(&info, p->dinfo, sizeof(int), hipMemcpyDeviceToHost, p->st));
     NV2_HIP(hipStreamSynchronize(p->st));
@@ -816,7 +835,7 @@ Stage2 *stage2_create(const S2Inputs &in, rocblas_handle h, hipStream_t st) {
     p->st = st;
     p->nc = in.nc; p->n = in.n; p->nv = in.nv; p->L = in.L; p->M = in.M;
     p->norb = in.nc + in.n + in.nv;
-    if (hipMalloc(&p->dinfo, sizeof(int)) != hipSuccess) { delete p; return nullptr; }
+    if (hipMalloc(&p->dinfo, 2 * sizeof(int)) != hipSuccess) { delete p; return nullptr; }
     if (hipMalloc(&p->acc, sizeof(double) * 32) != hipSuccess) { delete p; return nullptr; }
     // pair tables: core i<=j, core i<j, virt r<=s, virt r<s, active upper pairs
     std::vector<int2> pc, pcs, pv, pvs, up;

@@ -320,8 +320,10 @@ class Problem:
             ij = 0
             for i in range(M):
                 for j in range(i):
-                    # pair (i, j), i > j, in the reference's ijcou order (koopro4QD.F90:879-881)
-                    self.daaaoff[ij] = spinfree_rdm(3, self.C[j], self.C[i], blocks=blk)
+                    # pair (i, j), i > j, in the reference's ijcou order (koopro4QD.F90:879-881);
+                    # bra = state i, ket = state j: the class drivers pair daoff(ijcou,p,q) of
+                    # (istate > jstate) with <istate|E_pq|jstate> (e.g. qdnevpt.F:1912-1918, 3395-3406)
+                    self.daaaoff[ij] = spinfree_rdm(3, self.C[i], self.C[j], blocks=blk)
                     ij += 1
             self.taaoff = rdmtrace.bro2(self.daaaoff, n, self.nele)
             self.daloff = rdmtrace.bro1(self.taaoff, n, self.nele)
