@@ -109,6 +109,7 @@ nv2_ctx *nv2_ctx_create(int64_t ncore, int64_t nact, int64_t nvirt, int64_t meta
     for (auto nm : m4) TRYC(ctx_alloc(c, nm, M * n4));
     TRYC(ctx_alloc(c, "amatIP", M * n6)); TRYC(ctx_alloc(c, "amatEA", M * n6));
     for (const char *nm : {"C1", "C2", "C3", "C4"}) TRYC(ctx_alloc(c, nm, M * n6));
+    TRYC(ctx_alloc(c, "bamat_work", (size_t)bamat_lo_workspace((int)n)));
     c->e4_bytes = sizeof(double) * (size_t)(n4 * n4);
     if (hipMalloc(&c->E4, c->e4_bytes) != hipSuccess) { set_error("hipMalloc(E4) failed"); nv2_ctx_destroy(c); return nullptr; }
     c->plan = contract_plan_create((int)n, (int)M);
@@ -181,8 +182,10 @@ int nv2_ctx_run(nv2_ctx *c, int32_t prec, double thr_lindep, double *out) {
     NV2_HIP(hipEventElapsedTime(&t12, c->ev[1], c->ev[2]));
     NV2_HIP(hipEventRecord(c->ev[4], st));
     // phase 4: lower-order part of bamat and final A-matrices
-    if (launch_bamat_direct(R, c->find("atwo"), c->find("Fhalf"), c->find("C1"), c->find("C2"), c->find("C3"),
-                            c->find("C4"), c->find("amatIP"), c->find("amatEA"), st)) return -1;
+    for (int s = 0; s < M; s++)
+        if (launch_bamat_lo(c->rb, R, s, c->find("atwo"), c->find("Fhalf"), c->find("bamat_work"), c->find("C1"),
+                            c->find("C2"), c->find("C3"), c->find("C4"), c->find("amatIP"), c->find("amatEA"), st))
+            return -1;
     NV2_HIP(hipEventRecord(c->ev[5], st));
     // phase 5: stage 2 for every state
     for (int s = 0; s < M; s++) {

@@ -1,5 +1,7 @@
 // Internal stage-1 interfaces (device pointers, reference layouts).
 #pragma once
+#include <rocblas/rocblas.h>
+
 #include "nv2_common.h"
 
 namespace nv2 {
@@ -28,6 +30,12 @@ int launch_koop_small(const Rdm &R, int ncore, const double *aj, const double *a
 int launch_build_e4(const Rdm &R, int s, void *E4, int prec, hipStream_t st);
 int launch_bamat_direct(const Rdm &R, const double *atwo, const double *Fhalf, const double *C1, const double *C2,
                         const double *C3, const double *C4, double *amat, double *atmat, hipStream_t st);
+// lower-order part of bamat for state s as GEMMs + one combine kernel (nv2_bamat_lo.hip);
+// work holds bamat_lo_workspace(n) doubles
+i64 bamat_lo_workspace(int n);
+int launch_bamat_lo(rocblas_handle h, const Rdm &R, int s, const double *atwo, const double *Fhalf, double *work,
+                    const double *C1, const double *C2, const double *C3, const double *C4, double *amat,
+                    double *atmat, hipStream_t st);
 
 // 4-RDM contractions of bamat for one state s; C1..C4 in amat's layout (see nv2_stage1.hip)
 struct ContractPlan;
