@@ -91,10 +91,28 @@ typedef struct nv2_ctx nv2_ctx;
 nv2_ctx *nv2_ctx_create(int64_t ncore, int64_t nact, int64_t nvirt, int64_t metat, int64_t nele, int64_t nchovec);
 int nv2_ctx_set_integrals(nv2_ctx *ctx, const double *cholesky, const double *oneint);
 int nv2_ctx_set_ro4(nv2_ctx *ctx, const double *ro4);
+/* Transition RDMs of the state pairs for QD-NEVPT2 (metat >= 2): the nevpt.h5
+ * datasets 3-TRDM (P,n^6), 2-TRDM (P,n^4), 1-TRDM (P,n,n), P = metat(metat-1)/2,
+ * pair (I>J) at (I-1)(I-2)/2+J (koopro4QD.F90:433-477, read at qdnevpt.F:792-852).
+ * Once set, nv2_ctx_run also forms the off-diagonal class terms that every class
+ * driver adds to e2mp(j,i)/e2en(j,i) (e.g. qdnevpt.F:1936-1943, 2056-2070) and
+ * nv2_ctx_get returns "Heff-SC" and "Heff-PC" (metat x metat, element (j,i) at
+ * j + metat*i, second order without E0) and "QD" (per state i: 8 classes x
+ * (SC row e2mp(.,i), PC row e2en(.,i)), 16*metat doubles per state). */
+int nv2_ctx_set_trdm(nv2_ctx *ctx, const double *trdm3, const double *trdm2, const double *trdm1);
 int nv2_ctx_run(nv2_ctx *ctx, int32_t prec, double thr_lindep, double *out);
 int nv2_ctx_timings(nv2_ctx *ctx, double *ms);
 int nv2_ctx_get(nv2_ctx *ctx, const char *name, double *host, int64_t count);
 void nv2_ctx_destroy(nv2_ctx *ctx);
+
+/* QD-NEVPT2 effective Hamiltonian (qdnevpt.F:1046-1204 with hermitian_heff,
+ * :8089-8184): adds e0 to the diagonal of the second-order heff2 (metat x metat,
+ * element (j,i) at j + metat*i), takes the right eigenvectors of the
+ * non-Hermitian matrix in ascending eigenvalue order, and returns the
+ * hermitised Heff~ = C' diag(w) C'^T, C' = V (V^T V)^{-1/2}, and its
+ * eigenvalues (the final QD energies).  Returns -3 when the spectrum is not
+ * real.  metat <= 16. */
+int nv2_qd_heff(int64_t metat, const double *heff2, const double *e0, double *heff, double *energies);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,8 @@ def _declare(L):
     L.nv2_ctx_create.argtypes = [i64] * 6
     L.nv2_ctx_set_integrals.argtypes = [C.c_void_p, dp, dp]
     L.nv2_ctx_set_ro4.argtypes = [C.c_void_p, dp]
+    L.nv2_ctx_set_trdm.argtypes = [C.c_void_p, dp, dp, dp]
+    L.nv2_qd_heff.argtypes = [i64, dp, dp, dp, dp]
     L.nv2_ctx_run.argtypes = [C.c_void_p, i32, C.c_double, dp]
     L.nv2_ctx_timings.argtypes = [C.c_void_p, dp]
     L.nv2_ctx_get.argtypes = [C.c_void_p, C.c_char_p, dp, i64]

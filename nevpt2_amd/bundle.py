@@ -3,9 +3,12 @@ Cholesky vectors, one-electron integrals and reference energies.
 
 It replaces, for the hot path, the reference's file04 + ijkl.h5 + *.RasOrb.N
 inputs (koopro4QD.F90:147-176, ijkl_utils.F90:210-263, info_state_energy.F90:41-107).
-Layout (little endian): 8 bytes "NV2BUNDL", int64[8] = (version=1, ncore, nact,
+Layout (little endian): 8 bytes "NV2BUNDL", int64[8] = (version, ncore, nact,
 nvirt, metat, nele, nchovec, 0), then float64 e0(metat), oneint(npair),
 cholesky(nchovec*npair) [column-major], ro4(nwords*nact^4*metat) [column-major].
+Version 2 (metat > 1) appends the transition RDMs of the P = metat(metat-1)/2
+state pairs, 3-TRDM(P,n^6), 2-TRDM(P,n^4), 1-TRDM(P,n^2), column-major
+(the nevpt.h5 datasets of koopro4QD.F90:473-477).
 """
 from __future__ import annotations
 
@@ -15,7 +18,7 @@ import numpy as np
 def write_bundle(path, P, e0=None):
     M = P.metat
     e0 = np.zeros(M) if e0 is None else np.asarray(e0, dtype=np.float64)
-    hdr = np.array([1, P.ncore, P.nact, P.nvirt, M, P.nele, P.B.shape[0], 0], dtype="<i8")
+    hdr = np.array([2 if M > 1 else 1, P.ncore, P.nact, P.nvirt, M, P.nele, P.B.shape[0], 0], dtype="<i8")
     with open(path, "wb") as f:
         f.write(b"NV2BUNDL")
         f.write(hdr.tobytes())
@@ -23,6 +26,9 @@ def write_bundle(path, P, e0=None):
         f.write(np.ascontiguousarray(P.oneint, dtype="<f8").tobytes())
         f.write(np.asarray(P.B, dtype="<f8").tobytes(order="F"))
         f.write(np.asarray(P.ro4, dtype="<f8").tobytes(order="F"))
+        if M > 1:
+            for a in (P.daaaoff, P.taaoff, P.daloff):
+                f.write(np.asarray(a, dtype="<f8").tobytes(order="F"))
 
 
 def read_table(stdout: str):
@@ -36,3 +42,13 @@ def read_table(stdout: str):
             vals = line[6:]
             cur.append([float(vals[i:i + 20]) for i in range(0, 80, 20)])
     return np.array(tabs)
+
+
+def read_qd_summary(stdout: str):
+    """Parse the host's QD summary lines ' @ state  E0  SC  PC' into an array (metat, 3)."""
+    rows = []
+    for line in stdout.splitlines():
+        if line.startswith(" @"):
+            parts = line[2:].split()
+            rows.append([float(x) for x in parts[1:4]])
+    return np.array(rows)

@@ -38,6 +38,9 @@ struct nv2_ctx {
     Stage2 *s2 = nullptr;
     hipEvent_t ev[10];
     double ms[NV2_NPHASE];
+    bool has_trdm = false;
+    std::vector<double> qd;                 // per state s: 8 classes x (SC e2mp(j,s), PC e2en(j,s)), 16*M
+    std::vector<double> heff_sc, heff_pc;   // second-order Heff, H[j + M*i] = e2mp(j,i) / e2en(j,i)
     double *find(const char *nm) {
         for (auto &b : bufs) if (b.name == nm) return b.p;
         return nullptr;
@@ -109,6 +112,10 @@ nv2_ctx *nv2_ctx_create(int64_t ncore, int64_t nact, int64_t nvirt, int64_t meta
     for (auto nm : m4) TRYC(ctx_alloc(c, nm, M * n4));
     TRYC(ctx_alloc(c, "amatIP", M * n6)); TRYC(ctx_alloc(c, "amatEA", M * n6));
     for (const char *nm : {"C1", "C2", "C3", "C4"}) TRYC(ctx_alloc(c, nm, M * n6));
+    const i64 P = M * (M - 1) / 2;
+    if (P) {   // transition RDMs of the state pairs (nevpt.h5 1/2/3-TRDM)
+        TRYC(ctx_alloc(c, "1-TRDM", P * n2)); TRYC(ctx_alloc(c, "2-TRDM", P * n4)); TRYC(ctx_alloc(c, "3-TRDM", P * n6));
+    }
     TRYC(ctx_alloc(c, "bamat_work", (size_t)bamat_lo_workspace((int)n)));
     c->e4_bytes = sizeof(double) * (size_t)(n4 * n4);
     if (hipMalloc(&c->E4, c->e4_bytes) != hipSuccess) { set_error("hipMalloc(E4) failed"); nv2_ctx_destroy(c); return nullptr; }
@@ -119,6 +126,7 @@ nv2_ctx *nv2_ctx_create(int64_t ncore, int64_t nact, int64_t nvirt, int64_t meta
                 c->find("2-IPKO"), c->find("2-EAKO"), c->find("gk0pa"), c->find("gk0pd"), c->find("gk0pf"),
                 c->find("amatIP"), c->find("bmatIP"), c->find("cmatIP"), c->find("dmatIP"), c->find("amatEA"),
                 c->find("bmatEA"), c->find("cmatEA"), c->find("dmatEA")};
+    in.trdm1 = c->find("1-TRDM"); in.trdm2 = c->find("2-TRDM"); in.trdm3 = c->find("3-TRDM");
     c->s2 = stage2_create(in, c->rb, c->st);
     if (!c->s2) { set_error("stage2_create failed"); nv2_ctx_destroy(c); return nullptr; }
     return c;
@@ -129,6 +137,20 @@ int nv2_ctx_set_integrals(nv2_ctx *c, const double *cholesky, const double *onei
     NV2_HIP(hipMemcpyAsync(c->find("cholesky"), cholesky, sizeof(double) * (size_t)c->L * c->np, hipMemcpyHostToDevice, c->st));
     NV2_HIP(hipMemcpyAsync(c->find("oneint"), oneint, sizeof(double) * c->np, hipMemcpyHostToDevice, c->st));
     NV2_HIP(hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int nv2_ctx_set_trdm(nv2_ctx *c, const double *trdm3, const double *trdm2, const double *trdm1) {
+    if (!c) return -2;
+    if (c->M < 2) { set_error("nv2_ctx_set_trdm: transition RDMs need metat >= 2"); return -2; }
+    const char *nm[3] = {"3-TRDM", "2-TRDM", "1-TRDM"};
+    const double *src[3] = {trdm3, trdm2, trdm1};
+    for (int k = 0; k < 3; k++) {
+        CtxBuf *b = c->entry(nm[k]);
+        NV2_HIP(hipMemcpyAsync(b->p, src[k], sizeof(double) * b->count, hipMemcpyHostToDevice, c->st));
+    }
+    NV2_HIP(hipStreamSynchronize(c->st));
+    c->has_trdm = true;
     return 0;
 }
 
@@ -187,10 +209,26 @@ int nv2_ctx_run(nv2_ctx *c, int32_t prec, double thr_lindep, double *out) {
                             c->find("C2"), c->find("C3"), c->find("C4"), c->find("amatIP"), c->find("amatEA"), st))
             return -1;
     NV2_HIP(hipEventRecord(c->ev[5], st));
-    // phase 5: stage 2 for every state
+    // phase 5: stage 2 for every state, with the QD couplings when transition RDMs are present
+    const bool qd = c->has_trdm && M > 1;
+    c->qd.assign(qd ? (size_t)16 * M * M : 0, 0.0);
     for (int s = 0; s < M; s++) {
-        int rc = stage2_state(c->s2, s, thr_lindep, prec, out + 32 * s, nullptr);
+        int rc = stage2_state(c->s2, s, thr_lindep, prec, out + 32 * s, nullptr, qd ? c->qd.data() + 16 * M * s : nullptr);
         if (rc) return rc;
+    }
+    c->heff_sc.assign((size_t)M * M, 0.0);
+    c->heff_pc.assign((size_t)M * M, 0.0);
+    for (int i = 0; i < M; i++) {   // column i: perturbers of state i (qdnevpt.F e2mp(j,i), e2en(j,i))
+        for (int k = 0; k < 8; k++) {
+            c->heff_sc[i + M * i] += out[32 * i + 4 * k + 1];
+            c->heff_pc[i + M * i] += out[32 * i + 4 * k + 3];
+            if (!qd) continue;
+            for (int j = 0; j < M; j++) {
+                if (j == i) continue;
+                c->heff_sc[j + M * i] += c->qd[16 * M * i + k * 2 * M + j];
+                c->heff_pc[j + M * i] += c->qd[16 * M * i + k * 2 * M + M + j];
+            }
+        }
     }
     NV2_HIP(hipEventRecord(c->ev[6], st));
     NV2_HIP(hipEventSynchronize(c->ev[6]));
@@ -209,6 +247,16 @@ int nv2_ctx_timings(nv2_ctx *c, double *ms) {
 
 int nv2_ctx_get(nv2_ctx *c, const char *name, double *host, int64_t count) {
     if (!c || !name) return -2;
+    const std::vector<double> *hv = !strcmp(name, "Heff-SC") ? &c->heff_sc : !strcmp(name, "Heff-PC") ? &c->heff_pc
+                                  : !strcmp(name, "QD") ? &c->qd : nullptr;
+    if (hv) {
+        if ((size_t)count != hv->size()) {
+            set_error("nv2_ctx_get(%s): count %lld != %zu (run the context first)", name, (long long)count, hv->size());
+            return -2;
+        }
+        for (size_t k = 0; k < hv->size(); k++) host[k] = (*hv)[k];
+        return 0;
+    }
     CtxBuf *b = c->entry(name);
     if (!b) { set_error("nv2_ctx_get: unknown array '%s'", name); return -2; }
     if ((size_t)count != b->count) { set_error("nv2_ctx_get(%s): count %lld != %zu", name, (long long)count, b->count); return -2; }

@@ -13,6 +13,7 @@ struct S2Inputs {             // device pointers, reference layouts (state index
     const double *dal, *taa, *daaa;
     const double *ipko1, *eako1, *hrdm, *ipko2, *eako2, *gk0pa, *gk0pd, *gk0pf;
     const double *amatIP, *bmatIP, *cmatIP, *dmatIP, *amatEA, *bmatEA, *cmatEA, *dmatEA;  // may be null (no 4-RDM)
+    const double *trdm1 = nullptr, *trdm2 = nullptr, *trdm3 = nullptr;   // (P, n^k), P = M(M-1)/2; null: no QD
 };
 
 struct S2Stats {              // diagnostics for threshold-sensitive seeds
@@ -25,7 +26,9 @@ struct Stage2;
 Stage2 *stage2_create(const S2Inputs &in, rocblas_handle h, hipStream_t st);
 void stage2_destroy(Stage2 *p);
 // out[k*4 + {0,1,2,3}] = (norm_sc, e_sc, norm_pc, e_pc) for class k in the reference order
-// (0),(+1),(-1),(+2),(-2),(+1)',(-1)',(0)'
-int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *stats);
+// (0),(+1),(-1),(+2),(-2),(+1)',(-1)',(0)'.  With qd_out (16*M doubles) and transition RDMs
+// present, the QD couplings of state s: qd_out[k*2M + j] = SC e2mp(j,s), qd_out[k*2M + M + j] =
+// PC e2en(j,s) of class k (zero for j == s).
+int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *stats, double *qd_out = nullptr);
 
 }  // namespace nv2

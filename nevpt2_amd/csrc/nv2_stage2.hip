@@ -214,6 +214,201 @@ __device__ __forceinline__ double s_eee2t(const Sl &S, int a, int b, int c, int 
     return v;
 }
 
+// qdnevpt.F:6499-6506
+__device__ __forceinline__ double s_ee2tl(const Sl &S, int a, int b, int c, int d) {
+    double v = -s_ee2(S, b, a, c, d);
+    if (a == b) v += 2.0 * DA(S, d, c);
+    return v;
+}
+// ro2toff (qdnevpt.F:6508-6518): two-hole density of a state pair, without constant terms
+__device__ __forceinline__ double s_ro2toff(const Sl &S, int ap, int bp, int a, int b) {
+    double v = TA(S, ap, bp, a, b);
+    if (ap == a) v -= 2.0 * DA(S, bp, b);
+    if (bp == b) v -= 2.0 * DA(S, ap, a);
+    if (ap == b) v += DA(S, bp, a);
+    if (bp == a) v += DA(S, ap, b);
+    return v;
+}
+
+// ------------------------------------------------------------------ QD-NEVPT2 couplings
+// For state i (self) every class also sums, per other state j, the reference's
+// off-diagonal terms e2mp(j,i) (SC) and e2en(j,i) (PC).  They reuse i's
+// integrals, Koopmans matrices and denominators; the RDM-dependent factors come
+// from the transition RDMs of the pair, prepared per j by k_qd_fill into the
+// arrays a..e (their meaning is documented at each kernel).  M = 0 disables.
+#define QM 8
+struct QDArgs {
+    int M, self;
+    double *qpart;                 // per block: QM SC sums then QM PC sums
+    const double *a[QM], *b[QM], *c[QM], *d[QM], *e[QM];
+};
+__device__ __forceinline__ bool qd_on(const QDArgs &q, int j) { return j < q.M && j != q.self; }
+
+// block sums of sc[QM], pc[QM]; thread 0 writes qpart[2*QM*block + k]
+__device__ void block_storeQ(const double *sc, const double *pc, const QDArgs &q) {
+    __shared__ double sh[2 * QM][16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int k = 0; k < QM; k++) {
+        double x = wsum(sc[k]), y = wsum(pc[k]);
+        if (lane == 0) { sh[k][wid] = x; sh[QM + k][wid] = y; }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const i64 bid = (i64)blockIdx.x + (i64)gridDim.x * blockIdx.y;
+        for (int k = 0; k < 2 * QM; k++) {
+            double s = 0.0;
+            for (int w = 0; w < nw; w++) s += sh[k][w];
+            q.qpart[2 * QM * bid + k] = s;
+        }
+    }
+}
+// Neumaier sum of the per-block QD partials: out[k] += sum_b part[2*QM*b + k]
+__global__ void k_reduce_q(const double *part, i64 nparts, double *out) {
+    const int k = blockIdx.x;
+    double s = 0.0, c = 0.0;
+    for (i64 i = threadIdx.x; i < nparts; i += blockDim.x) {
+        double x = part[2 * QM * i + k], t = s + x;
+        c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
+        s = t;
+    }
+    __shared__ double ss[256], cc[256];
+    ss[threadIdx.x] = s; cc[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double S = 0.0, Cc = 0.0;
+        for (int t = 0; t < (int)blockDim.x; t++) {
+            double x = ss[t], u = S + x;
+            Cc += (fabs(S) >= fabs(x)) ? (S - u) + x : (x - u) + S;
+            S = u;
+            Cc += cc[t];
+        }
+        out[k] += S + Cc;
+    }
+}
+
+// Per-pair arrays of the off-diagonal terms.  S holds the transition RDMs of the
+// pair (i,j) as stored by the reference (pair I>J, koopro4QD.F90:2969-2974);
+// lt = (j < i) selects the reference's index orientation for that case.
+enum QdKind {
+    QK_V1_RO,     // (+1) SC  ro(a,b) at a+n*b                 qdnevpt.F:1912-1918
+    QK_V1_R,      // (+1) PC  R(ap,a) at ap+n*a                qdnevpt.F:3128-3138
+    QK_VM1_RO,    // (-1) SC                                   qdnevpt.F:2227-2234
+    QK_VM1_R,     // (-1) PC                                   qdnevpt.F:2445-2452
+    QK_P2_X,      // (+2) SC  X(C,D,c,d) at C+n(D+n(c+n d))    qdnevpt.F:3401-3414
+    QK_P2_XS,     // (+2) SC, diagonal core pair
+    QK_M2_X,      // (-2) SC                                   qdnevpt.F:3806-3819
+    QK_M2_XS,
+    QK_P2_Y,      // (+2) PC  Y[(A,B),(a,b)] at (A*n+B)+n^2(a*n+b)   qdnevpt.F:2859-2870
+    QK_M2_Y,      // (-2) PC                                         qdnevpt.F:2613-2624
+    QK_P2P_Y,     // (+2) PC diagonal pairs, upper-pair index        qdnevpt.F:2990-3003
+    QK_M2P_Y,     // (-2) PC diagonal pairs                          qdnevpt.F:2737-2750
+    QK_0P_X1,     // (0)' SC  ee2(b',a',a,b) at A+n(B+n(a+n b)) (same order for j<i and j>i) qdnevpt.F:4531
+    QK_0P_X2,     // (0)' SC  ee2tr(b',b,a',a) + d(a,b) daoff(a',b')                    qdnevpt.F:4533-4536
+    QK_0P_DS,     // (0)' SC  daoff(b,a)+daoff(a,b) at a+n*b                            qdnevpt.F:4569
+    QK_0P_V,      // (0)' PC  V[ab,a'b'] (ab = a*n+b rows)                              qdnevpt.F:4862-4880
+    QK_0P_V2,
+    QK_0P_RO,     // (0)' PC  ro(ab)                                                    qdnevpt.F:4935-4945
+    QK_P1P_R1,    // (+1)' SC -daoff(a',a) / -daoff(a,a') at a'+n*a                      qdnevpt.F:5602-5611
+    QK_M1P_R1,    // (-1)' SC                                                           qdnevpt.F:6077-6088
+    QK_P1P_QSC,   // (+1)' SC ee2tr(c',a',b',a)+ee2tl(a,b',a',c') at a'b'c' + n^3 a      qdnevpt.F:5642-5644
+    QK_M1P_QSC,   // (-1)' SC ee2(c',a',b',a)+ee2(a,b',a',c')                          qdnevpt.F:6098-6100
+    QK_P1P_MSC,   // (+1)' SC eee2t pattern, as [abc, a'b'c'] (transposed M)           qdnevpt.F:5661-5668
+    QK_M1P_MSC,   // (-1)' SC eee2 pattern                                             qdnevpt.F:6120-6131
+    QK_P1P_MPC,   // (+1)' PC eee2t pattern, [abc, a'b'c']                             qdnevpt.F:5082-5091
+    QK_M1P_MPC,   // (-1)' PC eee2 pattern                                             qdnevpt.F:5295-5304
+    QK_P1P_QPC,   // (+1)' PC ee2tl(a,b',a',c') (j<i) / ee2tr(c',a',b',a) (j>i)         qdnevpt.F:5084-5097
+    QK_M1P_QPC    // (-1)' PC ee2(a,b',a',c') / ee2(c',a',b',a)                         qdnevpt.F:5297-5310
+};
+__global__ void k_qd_fill(Sl S, int kind, int lt, const int2 *up, double *out, i64 total) {
+    const int n = S.n, nn = n * n;
+    const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    double v = 0.0;
+    switch (kind) {
+    case QK_V1_RO: { int a = e % n, b = e / n; v = lt ? -DA(S, b, a) : -DA(S, a, b); break; }
+    case QK_V1_R: { int ap = e % n, a = e / n; v = lt ? -DA(S, a, ap) : -DA(S, ap, a); break; }
+    case QK_VM1_RO: { int a = e % n, b = e / n; v = lt ? DA(S, a, b) : DA(S, b, a); break; }
+    case QK_VM1_R: { int ap = e % n, a = e / n; v = lt ? DA(S, ap, a) : DA(S, a, ap); break; }
+    case QK_P2_X: case QK_P2_XS: {
+        int C = e % n, D = (e / n) % n, c = (e / nn) % n, d = e / (nn * n);
+        if (lt) { v = s_ro2toff(S, C, D, c, d); if (kind == QK_P2_XS) v += s_ro2toff(S, D, C, c, d); }
+        else { v = s_ro2toff(S, c, d, C, D); if (kind == QK_P2_XS) v += s_ro2toff(S, c, d, D, C); }
+        break;
+    }
+    case QK_M2_X: case QK_M2_XS: {
+        int C = e % n, D = (e / n) % n, c = (e / nn) % n, d = e / (nn * n);
+        if (lt) { v = TA(S, c, d, C, D); if (kind == QK_M2_XS) v += TA(S, c, d, D, C); }
+        else { v = TA(S, C, D, c, d); if (kind == QK_M2_XS) v += TA(S, D, C, c, d); }
+        break;
+    }
+    case QK_P2_Y: case QK_M2_Y: {
+        int row = e % nn, col = e / nn;
+        int A = row / n, B = row % n, a = col / n, b = col % n;
+        if (kind == QK_P2_Y) v = lt ? s_ro2toff(S, b, a, B, A) : s_ro2toff(S, B, A, b, a);
+        else v = lt ? TA(S, B, A, b, a) : TA(S, b, a, B, A);
+        break;
+    }
+    case QK_P2P_Y: case QK_M2P_Y: {
+        const int m = n * (n + 1) / 2;
+        int row = e % m, col = e / m;
+        int A = up[row].x, B = up[row].y, a = up[col].x, b = up[col].y;
+        if (kind == QK_P2P_Y)
+            v = lt ? 0.5 * (s_ro2toff(S, b, a, B, A) + s_ro2toff(S, b, a, A, B))
+                   : 0.5 * (s_ro2toff(S, B, A, b, a) + s_ro2toff(S, A, B, b, a));
+        else
+            v = lt ? 0.5 * (TA(S, B, A, b, a) + TA(S, A, B, b, a)) : 0.5 * (TA(S, b, a, A, B) + TA(S, b, a, B, A));
+        break;
+    }
+    case QK_0P_X1: { int A = e % n, B = (e / n) % n, a = (e / nn) % n, b = e / (nn * n); v = s_ee2(S, B, A, a, b); break; }
+    case QK_0P_X2: {
+        int A = e % n, B = (e / n) % n, a = (e / nn) % n, b = e / (nn * n);
+        v = s_ee2tr(S, B, b, A, a) + (a == b ? DA(S, A, B) : 0.0);
+        break;
+    }
+    case QK_0P_DS: { int a = e % n, b = e / n; v = DA(S, a, b) + DA(S, b, a); break; }
+    case QK_0P_V: case QK_0P_V2: {
+        int row = e % nn, col = e / nn;
+        int a = row / n, b = row % n, A = col / n, B = col % n;
+        if (kind == QK_0P_V) v = lt ? s_ee2(S, b, a, A, B) : s_ee2(S, B, A, a, b);
+        else {
+            v = lt ? -TA(S, a, B, A, b) : -TA(S, A, b, a, B);
+            if (A == a) v += 2.0 * (lt ? DA(S, B, b) : DA(S, b, B));
+        }
+        break;
+    }
+    case QK_0P_RO: { int a = e / n, b = e % n; v = lt ? DA(S, b, a) : DA(S, a, b); break; }
+    case QK_P1P_R1: { int A = e % n, a = e / n; v = lt ? -DA(S, A, a) : -DA(S, a, A); break; }
+    case QK_M1P_R1: { int A = e % n, a = e / n; v = lt ? DA(S, a, A) : DA(S, A, a); break; }
+    case QK_P1P_QSC: case QK_M1P_QSC: case QK_P1P_QPC: case QK_M1P_QPC: {
+        const i64 N = (i64)nn * n;
+        i64 row = e % N;
+        int a = e / N;
+        int A = row / nn, B = (row / n) % n, C = row % n;
+        const bool plus = kind == QK_P1P_QSC || kind == QK_P1P_QPC;
+        const double t1 = plus ? s_ee2tr(S, C, A, B, a) : s_ee2(S, C, A, B, a);     // X2(c',a',b',a)
+        const double t2 = plus ? s_ee2tl(S, a, B, A, C) : s_ee2(S, a, B, A, C);     // (a,b',a',c')
+        if (kind == QK_P1P_QSC || kind == QK_M1P_QSC) v = t1 + t2;
+        else v = lt ? t2 : t1;
+        break;
+    }
+    case QK_P1P_MSC: case QK_M1P_MSC: case QK_P1P_MPC: case QK_M1P_MPC: {
+        const i64 N = (i64)nn * n;
+        i64 row = e % N, col = e / N;
+        int a = row / nn, b = (row / n) % n, c = row % n;
+        int A = col / nn, B = (col / n) % n, C = col % n;
+        const bool plus = kind == QK_P1P_MSC || kind == QK_P1P_MPC;
+        // SC: X3(c',a',b',b,a,c) for j<i, X3(c,a,b,b',a',c') for j>i; PC the other way round
+        const bool first = (kind == QK_P1P_MSC || kind == QK_M1P_MSC) ? (lt != 0) : (lt == 0);
+        if (first) v = plus ? s_eee2t(S, C, A, B, b, a, c) : s_eee2(S, C, A, B, b, a, c);
+        else v = plus ? s_eee2t(S, c, a, b, B, A, C) : s_eee2(S, c, a, b, B, A, C);
+        break;
+    }
+    default: break;
+    }
+    out[e] = v;
+}
+
 // ------------------------------------------------------------------ class (0) (qdnevpt.F:1661-1773)
 __global__ void k_v0(const double *I, i64 ld, const int2 *pairs, int nc, int nv, const double *ec, const double *ev,
                      double *part) {
@@ -247,11 +442,15 @@ __global__ void k_v0(const double *I, i64 ld, const int2 *pairs, int nc, int nv,
 template <int NM>
 __global__ void k_v1(const double *I, i64 ld, const int2 *pairs, int nc, int n, int nv, const double *ec,
                      const double *ev, const double *gk, const double *ro, const double *spc, const double *w, int nnew,
-                     double *part) {
+                     double *part, QDArgs q) {
+    // QD: q.a[j] = ro of pair (i,j) at a+n*b, q.b[j] = s(a,mu) of pair (i,j)
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int2 pr = pairs[blockIdx.y];
     const int ii = pr.x, ij = pr.y;
     double Esc = 0, Psc = 0, Epc = 0, Ppc = 0;
+    double qsc[QM], qpc[QM];
+#pragma unroll
+    for (int j = 0; j < QM; j++) { qsc[j] = 0.0; qpc[j] = 0.0; }
     if (r < nv) {
         double Tj[NM], Tk[NM];
 #pragma unroll
@@ -260,7 +459,9 @@ __global__ void k_v1(const double *I, i64 ld, const int2 *pairs, int nc, int n, 
             Tk[a] = a < n ? I[(ij + (i64)nc * a) + ld * (ii + (i64)nc * r)] : 0.0;     // (a ij | r ii)
         }
         const bool same = ii == ij;
-        double dn = 0.0, ek = 0.0;
+        double dn = 0.0, ek = 0.0, dnj[QM];
+#pragma unroll
+        for (int j = 0; j < QM; j++) dnj[j] = 0.0;
 #pragma unroll
         for (int a = 0; a < NM; a++) {
             if (a >= n) break;
@@ -271,6 +472,9 @@ __global__ void k_v1(const double *I, i64 ld, const int2 *pairs, int nc, int n, 
                                    : Tj[a] * Tj[b] + Tk[a] * Tk[b] - 0.5 * Tj[a] * Tk[b] - 0.5 * Tk[a] * Tj[b];
                 dn += 2.0 * ro[a + n * b] * cint;
                 ek += gk[a + n * b] * cint;
+#pragma unroll
+                for (int j = 0; j < QM; j++)
+                    if (qd_on(q, j)) dnj[j] += 2.0 * q.a[j][a + n * b] * cint;
             }
         }
         if (fabs(dn) > 0.0) {
@@ -278,6 +482,8 @@ __global__ void k_v1(const double *I, i64 ld, const int2 *pairs, int nc, int n, 
             double co = dn / den;
             Esc += co;
             Psc += co / den;
+#pragma unroll
+            for (int j = 0; j < QM; j++) qsc[j] += dnj[j] / den;
         }
         for (int mu = 0; mu < nnew; mu++) {
             double rj = 0.0, rjp = 0.0;
@@ -292,18 +498,34 @@ __global__ void k_v1(const double *I, i64 ld, const int2 *pairs, int nc, int n, 
             double contri = same ? rj * rj / deno : (0.5 * rj * rj + 1.5 * rjp * rjp) / deno;
             Epc -= contri;
             Ppc += contri / deno;
+#pragma unroll
+            for (int j = 0; j < QM; j++) {
+                if (!qd_on(q, j)) continue;
+                double xj = 0.0, xjp = 0.0;
+                for (int a = 0; a < n; a++) {
+                    double sj = q.b[j][a + n * mu];
+                    xj += (same ? Tj[a] : Tj[a] + Tk[a]) * sj;
+                    xjp += (Tj[a] - Tk[a]) * sj;
+                }
+                qpc[j] -= same ? xj * rj / deno : (0.5 * xj * rj + 1.5 * xjp * rjp) / deno;
+            }
         }
     }
     block_store4(Esc, Psc, Epc, Ppc, part);
+    if (q.M) block_storeQ(qsc, qpc, q);
 }
 
 // ------------------------------------------------------------------ class (-1) (qdnevpt.F:2095-2262, 2400-2561)
 template <int NM>
 __global__ void k_vm1(const double *I, i64 ld, const int2 *pairs, i64 npairs, int nc, int n, const double *ec,
                       const double *ev, const double *da, const double *gk, const double *spc, const double *w,
-                      int nnew, double *part) {
+                      int nnew, double *part, QDArgs q) {
+    // QD: q.a[j] = ro of pair (i,j) at a+n*b, q.b[j] = s(a,mu) of pair (i,j)
     const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     double Esc = 0, Psc = 0, Epc = 0, Ppc = 0;
+    double qsc[QM], qpc[QM];
+#pragma unroll
+    for (int j = 0; j < QM; j++) { qsc[j] = 0.0; qpc[j] = 0.0; }
     if (t < npairs * nc) {
         const int i = t % nc;
         const int2 pr = pairs[t / nc];
@@ -315,7 +537,9 @@ __global__ void k_vm1(const double *I, i64 ld, const int2 *pairs, i64 npairs, in
             Tk[a] = a < n ? I[(i + (i64)nc * is) + ld * (a + (i64)n * ir)] : 0.0;    // (s i | r a)
         }
         const bool same = ir == is;
-        double dn = 0.0, ek = 0.0;
+        double dn = 0.0, ek = 0.0, dnj[QM];
+#pragma unroll
+        for (int j = 0; j < QM; j++) dnj[j] = 0.0;
 #pragma unroll
         for (int a = 0; a < NM; a++) {
             if (a >= n) break;
@@ -326,6 +550,9 @@ __global__ void k_vm1(const double *I, i64 ld, const int2 *pairs, i64 npairs, in
                                    : Tj[a] * Tj[b] + Tk[a] * Tk[b] - 0.5 * Tj[a] * Tk[b] - 0.5 * Tk[a] * Tj[b];
                 dn += 2.0 * da[a + n * b] * cint;
                 ek += gk[a + n * b] * cint;
+#pragma unroll
+                for (int j = 0; j < QM; j++)
+                    if (qd_on(q, j)) dnj[j] += 2.0 * q.a[j][a + n * b] * cint;
             }
         }
         if (fabs(dn) >= 1.0e-14) {
@@ -333,6 +560,8 @@ __global__ void k_vm1(const double *I, i64 ld, const int2 *pairs, i64 npairs, in
             double co = dn / den;
             Esc += co;
             Psc += co / den;
+#pragma unroll
+            for (int j = 0; j < QM; j++) qsc[j] += dnj[j] / den;
         }
         for (int mu = 0; mu < nnew; mu++) {
             double r1 = 0.0, r2 = 0.0;
@@ -347,16 +576,29 @@ __global__ void k_vm1(const double *I, i64 ld, const int2 *pairs, i64 npairs, in
             double contri = same ? r1 * r1 / deno : (0.5 * r1 * r1 + 1.5 * r2 * r2) / deno;
             Epc -= contri;
             Ppc += contri / deno;
+#pragma unroll
+            for (int j = 0; j < QM; j++) {
+                if (!qd_on(q, j)) continue;
+                double xj = 0.0, xjp = 0.0;
+                for (int a = 0; a < n; a++) {
+                    double sj = q.b[j][a + n * mu];
+                    xj += (same ? Tj[a] : Tj[a] + Tk[a]) * sj;
+                    xjp += (Tj[a] - Tk[a]) * sj;
+                }
+                qpc[j] -= same ? xj * r1 / deno : (0.5 * xj * r1 + 1.5 * xjp * r2) / deno;
+            }
         }
     }
     block_store4(Esc, Psc, Epc, Ppc, part);
+    if (q.M) block_storeQ(qsc, qpc, q);
 }
 
 // ------------------------------------------------------------------ classes (+2)/(-2), SC (qdnevpt.F:3327-3445, 3740-3849)
 // one workgroup per external pair; T(c,d) is the pair's integral block, X the metric RDM and K the Koopmans matrix:
 //   dn = sum cint*X, dk = sum cint*K, cint = T(c',d')T(c,d) [x 1/4 and symmetrised when the pair is diagonal]
 __global__ void k_pm2_sc(const double *I, i64 ld, const int2 *pairs, int mode, int nc, int n, const double *X,
-                         const double *K, const double *eo, double *part) {
+                         const double *K, const double *eo, double *part, QDArgs q) {
+    // QD: q.a[j] = X of pair (i,j) for x != y, q.b[j] = the symmetrised X for x == y (both at C+n(D+n(c+n d)))
     // mode 0: (+2), pair (i<=j) core, T(a,b) = (i a | j b) from I_caca ; mode 1: (-2), pair (r<=s) virt,
     // T(a,b) = (s a | r b) from I_avav
     __shared__ double T[256];
@@ -370,7 +612,9 @@ __global__ void k_pm2_sc(const double *I, i64 ld, const int2 *pairs, int mode, i
     }
     __syncthreads();
     const bool same = x == y;
-    double dn = 0.0, dk = 0.0;
+    double dn = 0.0, dk = 0.0, dnj[QM];
+#pragma unroll
+    for (int j = 0; j < QM; j++) dnj[j] = 0.0;
     for (int e = threadIdx.x; e < nn; e += blockDim.x) {      // (c',d') row
         const int C = e % n, D = e / n;
         double sx = 0.0, sk = 0.0;
@@ -388,54 +632,88 @@ __global__ void k_pm2_sc(const double *I, i64 ld, const int2 *pairs, int mode, i
         const double f = same ? 0.25 : 1.0;
         dn += f * T[e] * sx;
         dk += f * T[e] * sk;
+#pragma unroll
+        for (int j = 0; j < QM; j++) {
+            if (!qd_on(q, j)) continue;
+            const double *Xj = same ? q.b[j] : q.a[j];
+            double sj = 0.0;
+            for (int g = 0; g < nn; g++) sj += T[g] * Xj[e + (i64)nn * g];
+            dnj[j] += f * T[e] * sj;
+        }
     }
     dn = block_sum(dn, sh);
     dk = block_sum(dk, sh);
-    double E = 0.0, P = 0.0;
+#pragma unroll
+    for (int j = 0; j < QM; j++)
+        if (qd_on(q, j)) dnj[j] = block_sum(dnj[j], sh);
+    double E = 0.0, P = 0.0, qsc[QM], qpc[QM];
+#pragma unroll
+    for (int j = 0; j < QM; j++) { qsc[j] = 0.0; qpc[j] = 0.0; }
     if (threadIdx.x == 0 && fabs(dn) >= 1.0e-14) {
         double den = mode == 0 ? eo[x] + eo[y] - dk / dn : -dk / dn - eo[x] - eo[y];
         double co = dn / den;
         E = co;
         P = co / den;
+#pragma unroll
+        for (int j = 0; j < QM; j++)
+            if (qd_on(q, j)) qsc[j] = dnj[j] / den;
     }
     if (threadIdx.x != 0) { E = 0.0; P = 0.0; }
     block_store4(E, P, 0.0, 0.0, part);
+    if (q.M) block_storeQ(qsc, qpc, q);
 }
 
 // ------------------------------------------------------------------ classes (+2)/(-2), PC (qdnevpt.F:2563-3072)
 // off-diagonal pairs: v(ab) = T(b,a) with ab = a*n+b ; contribution -(v.s_mu)^2/deno
 __global__ void k_pm2_pc(const double *I, i64 ld, const int2 *pairs, i64 npairs, int mode, int nc, int n,
-                         const double *spc, const double *w, int nnew, const double *eo, double *part) {
+                         const double *spc, const double *w, int nnew, const double *eo, double *part, QDArgs q) {
+    // QD: q.a[j] = s(ab,mu) of pair (i,j)
     const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    double E = 0.0, P = 0.0;
+    double E = 0.0, P = 0.0, qsc[QM], qpc[QM];
+#pragma unroll
+    for (int j = 0; j < QM; j++) { qsc[j] = 0.0; qpc[j] = 0.0; }
     if (t < npairs * nnew) {
         const int mu = t % nnew;
         const int2 pr = pairs[t / nnew];
         const int x = pr.x, y = pr.y;
         const int nn = n * n;
-        double acc = 0.0;
+        double acc = 0.0, accj[QM];
+#pragma unroll
+        for (int j = 0; j < QM; j++) accj[j] = 0.0;
         for (int ab = 0; ab < nn; ab++) {
             const int a = ab / n, b = ab % n;
             // (+2): traint(b,a) = (i b | j a) ; (-2): dint1(b,a) = (r b | s a)
             double v = mode == 0 ? I[(x + (i64)nc * b) + ld * (y + (i64)nc * a)] : I[(b + (i64)n * x) + ld * (a + (i64)n * y)];
             acc += v * spc[ab + (i64)nn * mu];
+#pragma unroll
+            for (int j = 0; j < QM; j++)
+                if (qd_on(q, j)) accj[j] += v * q.a[j][ab + (i64)nn * mu];
         }
         double deno = mode == 0 ? w[mu] - eo[x] - eo[y] : w[mu] + eo[x] + eo[y];
         double contri = acc * acc / deno;
         E = -contri;
         P = contri / deno;
+#pragma unroll
+        for (int j = 0; j < QM; j++)
+            if (qd_on(q, j)) qpc[j] = -accj[j] * acc / deno;
     }
     block_store4(0.0, 0.0, E, P, part);
+    if (q.M) block_storeQ(qsc, qpc, q);
 }
 // diagonal pairs (i=j or r=s) with the n(n+1)/2 metric
 __global__ void k_pm2p_pc(const double *I, i64 ld, int next, int mode, int nc, int n, const double *spc,
-                          const double *w, int nnew, const double *eo, double *part) {
+                          const double *w, int nnew, const double *eo, double *part, QDArgs q) {
+    // QD: q.a[j] = s(ab,mu) of pair (i,j) over upper pairs
     const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    double E = 0.0, P = 0.0;
+    double E = 0.0, P = 0.0, qsc[QM], qpc[QM];
+#pragma unroll
+    for (int j = 0; j < QM; j++) { qsc[j] = 0.0; qpc[j] = 0.0; }
     const int m = n * (n + 1) / 2;
     if (t < (i64)next * nnew) {
         const int mu = t % nnew, x = t / nnew;
-        double acc = 0.0;
+        double acc = 0.0, accj[QM];
+#pragma unroll
+        for (int j = 0; j < QM; j++) accj[j] = 0.0;
         int ab = 0;
         for (int a = 0; a < n; a++)
             for (int b = a; b < n; b++, ab++) {
@@ -448,13 +726,20 @@ __global__ void k_pm2p_pc(const double *I, i64 ld, int next, int mode, int nc, i
                     if (a != b) v *= 2.0;
                 }
                 acc += v * spc[ab + (i64)m * mu];
+#pragma unroll
+                for (int j = 0; j < QM; j++)
+                    if (qd_on(q, j)) accj[j] += v * q.a[j][ab + (i64)m * mu];
             }
         double deno = mode == 0 ? w[mu] - 2.0 * eo[x] : w[mu] + 2.0 * eo[x];
         double contri = acc * acc / deno;
         E = -contri;
         P = contri / deno;
+#pragma unroll
+        for (int j = 0; j < QM; j++)
+            if (qd_on(q, j)) qpc[j] = -accj[j] * acc / deno;
     }
     block_store4(0.0, 0.0, E, P, part);
+    if (q.M) block_storeQ(qsc, qpc, q);
 }
 
 // metric / Koopmans matrices of the (+-2) PC solves: fillcm2 and fillcm2p (qdnevpt.F:4156-4193)
@@ -523,7 +808,9 @@ __global__ void k_v0p_vmats(Sl S, double *V, double *V2) {
 }
 __global__ void k_v0p_sc(const double *Icvaa, i64 ldj, const double *Icaav, i64 ldk, int nc, int n, int nv,
                          const double *X1, const double *X2, const double *G1, const double *G2, const double *da,
-                         const double *gf, const double *heff, const double *ec, const double *ev, double *part) {
+                         const double *gf, const double *heff, const double *ec, const double *ev, double *part,
+                         QDArgs q) {
+    // QD: q.a[j] = X1, q.b[j] = X2 (layout of X1/X2), q.c[j] = daoff(b,a)+daoff(a,b) at a+n*b
     __shared__ double dj[256], dk[256], sh[16];
     const int i = blockIdx.x % nc, r = blockIdx.x / nc;
     const int nn = n * n;
@@ -533,10 +820,12 @@ __global__ void k_v0p_sc(const double *Icvaa, i64 ldj, const double *Icaav, i64 
         dk[e] = Icaav[(i + (i64)nc * a) + ldk * (b + (i64)n * r)];       // (i a | r b)
     }
     __syncthreads();
-    double dn = 0.0, dko = 0.0, dummy = 0.0, dummyk = 0.0;
+    double dn = 0.0, dko = 0.0, dummy = 0.0, dummyk = 0.0, dnj[QM];
+#pragma unroll
+    for (int j = 0; j < QM; j++) dnj[j] = 0.0;
     for (int e = threadIdx.x; e < nn; e += blockDim.x) {      // (a',b')
         const double djp = dj[e], dkp = dk[e];
-        double s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+        double s1 = 0, s2 = 0;
         for (int f = 0; f < nn; f++) {                        // (a,b)
             const i64 o = e + (i64)nn * f;
             const double djv = dj[f], dkv = dk[f];
@@ -545,6 +834,9 @@ __global__ void k_v0p_sc(const double *Icvaa, i64 ldj, const double *Icaav, i64 
             const double dint11 = djp * djv - 0.5 * djp * dkv - 0.5 * djv * dkp;
             s1 += dint1 * X1[o] + dint2 * X2[o];
             s2 += dint11 * G1[o] + dint2 * G2[o];
+#pragma unroll
+            for (int j = 0; j < QM; j++)
+                if (qd_on(q, j)) dnj[j] += dint1 * q.a[j][o] + dint2 * q.b[j][o];
         }
         dn += s1;
         dko += s2;
@@ -552,13 +844,21 @@ __global__ void k_v0p_sc(const double *Icvaa, i64 ldj, const double *Icaav, i64 
         const double d1 = 2.0 * dj[e] - dk[e];
         dummy += da[b + n * a] * d1;
         dummyk += gf[a + n * b] * d1 * 0.5;
-        (void)s3; (void)s4;
+        const double hh = heff[i + (i64)nc * r];
+#pragma unroll
+        for (int j = 0; j < QM; j++)
+            if (qd_on(q, j)) dnj[j] += hh * q.c[j][a + n * b] * d1;
     }
     dn = block_sum(dn, sh);
     dko = block_sum(dko, sh);
     dummy = block_sum(dummy, sh);
     dummyk = block_sum(dummyk, sh);
-    double E = 0.0, P = 0.0;
+#pragma unroll
+    for (int j = 0; j < QM; j++)
+        if (qd_on(q, j)) dnj[j] = block_sum(dnj[j], sh);
+    double E = 0.0, P = 0.0, qsc[QM], qpc[QM];
+#pragma unroll
+    for (int j = 0; j < QM; j++) { qsc[j] = 0.0; qpc[j] = 0.0; }
     if (threadIdx.x == 0) {
         const double hh = heff[i + (i64)nc * r];
         dn = dn + 2.0 * hh * dummy + 2.0 * hh * hh;
@@ -568,14 +868,19 @@ __global__ void k_v0p_sc(const double *Icvaa, i64 ldj, const double *Icaav, i64 
             double co = dn / den;
             E = co;
             P = co / den;
+#pragma unroll
+            for (int j = 0; j < QM; j++)
+                if (qd_on(q, j)) qsc[j] = dnj[j] / den;
         }
     }
     block_store4(E, P, 0.0, 0.0, part);
+    if (q.M) block_storeQ(qsc, qpc, q);
 }
 __global__ void k_v0p_pc(const double *Icvaa, i64 ldj, const double *Icaav, i64 ldk, int nc, int n, int nv,
                          const double *sm, const double *sp, const double *spp, const double *dc1, const double *dc2,
                          const double *w, int nnew, const double *heff, const double *ec, const double *ev,
-                         double *part) {
+                         double *part, QDArgs q) {
+    // QD: q.a/b/c[j] = s, s', s'' of pair (i,j) (nn x nnew); q.d/e[j] = ro.coef[:nn], ro.coef[nn:] (nnew)
     __shared__ double dj[256], dk[256];
     const int i = blockIdx.x % nc, r = blockIdx.x / nc;
     const int nn = n * n;
@@ -586,7 +891,9 @@ __global__ void k_v0p_pc(const double *Icvaa, i64 ldj, const double *Icaav, i64 
     }
     __syncthreads();
     const double hh = heff[i + (i64)nc * r];
-    double E = 0.0, P = 0.0;
+    double E = 0.0, P = 0.0, qsc[QM], qpc[QM];
+#pragma unroll
+    for (int j = 0; j < QM; j++) { qsc[j] = 0.0; qpc[j] = 0.0; }
     for (int mu = threadIdx.x; mu < nnew; mu += blockDim.x) {
         double rimu = 0.0, rimup = 0.0;
         for (int ab = 0; ab < nn; ab++) {
@@ -600,8 +907,22 @@ __global__ void k_v0p_pc(const double *Icvaa, i64 ldj, const double *Icaav, i64 
         const double co = rimu + rimup, cir = co / deno;
         E -= co * cir;
         P += cir * cir;
+#pragma unroll
+        for (int j = 0; j < QM; j++) {
+            if (!qd_on(q, j)) continue;
+            double rj = 0.0, rjp = 0.0;
+            for (int ab = 0; ab < nn; ab++) {
+                const i64 o = ab + (i64)nn * mu;
+                rj += (2.0 * dj[ab] - dk[ab]) * q.a[j][o];
+                rjp += -dj[ab] * q.b[j][o] + dk[ab] * q.c[j][o];
+            }
+            rj += 2.0 * hh * q.d[j][mu];
+            rjp -= hh * q.e[j][mu];
+            qpc[j] -= (rj + rjp) * cir;
+        }
     }
     block_store4(0.0, 0.0, E, P, part);
+    if (q.M) block_storeQ(qsc, qpc, q);
 }
 // dc[mu] = sum_ab da(b,a) coef[off + ab, mu], ab = a*n+b
 __global__ void k_dacoef(const double *da, int n, const double *coef, int ldc, int off, int nnew, double *dc) {
@@ -652,37 +973,58 @@ __global__ void k_fill_pm1p_v(const double *I, int plus, int n, int nc, int next
 // SC sums per external orbital x
 __global__ void k_pm1p_sc(int plus, int n, int next, const double *H, const double *W1, const double *W2,
                           const double *V, const double *Z1, const double *Z2, const double *Y1, const double *Y2,
-                          const double *eo, double *part) {
+                          const double *eo, double *part, QDArgs q) {
+    // QD: q.a[j] = R1_j H (n x next), q.b[j] = Q2_j H and q.c[j] = M_j V (n^3 x next) of pair (i,j)
     __shared__ double sh[16];
     const int x = blockIdx.x;
     const i64 N = (i64)n * n * n;
-    double dm = 0.0, dk = 0.0;
+    double dm = 0.0, dk = 0.0, dmj[QM];
+#pragma unroll
+    for (int j = 0; j < QM; j++) dmj[j] = 0.0;
     for (i64 e = threadIdx.x; e < N; e += blockDim.x) {
         const double v = V[e + N * x];
         dm += 2.0 * v * Z1[e + N * x] + Y1[e + N * x] * v;
         dk += v * Z2[e + N * x] + Y2[e + N * x] * v;
+#pragma unroll
+        for (int j = 0; j < QM; j++)
+            if (qd_on(q, j)) dmj[j] += v * q.b[j][e + N * x] + q.c[j][e + N * x] * v;
     }
     for (int a = threadIdx.x; a < n; a += blockDim.x) {
         const double h = H[a + (i64)n * x];
         dm += h * W1[a + (i64)n * x];
         dk += h * W2[a + (i64)n * x];
+#pragma unroll
+        for (int j = 0; j < QM; j++)
+            if (qd_on(q, j)) dmj[j] += h * q.a[j][a + (i64)n * x];
     }
     dm = block_sum(dm, sh);
     dk = block_sum(dk, sh);
-    double E = 0.0, P = 0.0;
+#pragma unroll
+    for (int j = 0; j < QM; j++)
+        if (qd_on(q, j)) dmj[j] = block_sum(dmj[j], sh);
+    double E = 0.0, P = 0.0, qsc[QM], qpc[QM];
+#pragma unroll
+    for (int j = 0; j < QM; j++) { qsc[j] = 0.0; qpc[j] = 0.0; }
     if (threadIdx.x == 0 && dm != 0.0) {
         double den = plus ? eo[x] - dk / dm : -dk / dm - eo[x];
         double co = dm / den;
         E = co;
         P = co / den;
+#pragma unroll
+        for (int j = 0; j < QM; j++)
+            if (qd_on(q, j)) qsc[j] = dmj[j] / den;
     }
     block_store4(E, P, 0.0, 0.0, part);
+    if (q.M) block_storeQ(qsc, qpc, q);
 }
 // PC sums: R[mu, x] from the S-build, deno = w(mu) -/+ e_x
 __global__ void k_pm1p_pc(int plus, int nnew, int next, const double *R, const double *w, const double *eo,
-                          double *part) {
+                          double *part, QDArgs q) {
+    // QD: q.a[j] = R of pair (i,j) (nnew x next)
     i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    double E = 0.0, P = 0.0;
+    double E = 0.0, P = 0.0, qsc[QM], qpc[QM];
+#pragma unroll
+    for (int j = 0; j < QM; j++) { qsc[j] = 0.0; qpc[j] = 0.0; }
     if (t < (i64)nnew * next) {
         int mu = t % nnew, x = t / nnew;
         double r = R[t];
@@ -690,8 +1032,12 @@ __global__ void k_pm1p_pc(int plus, int nnew, int next, const double *R, const d
         double co = r / deno;
         E = -r * co;
         P = co * co;
+#pragma unroll
+        for (int j = 0; j < QM; j++)
+            if (qd_on(q, j)) qpc[j] = -q.a[j][t] * co;
     }
     block_store4(0.0, 0.0, E, P, part);
+    if (q.M) block_storeQ(qsc, qpc, q);
 }
 
 // ------------------------------------------------------------------ host side
@@ -710,6 +1056,9 @@ struct Stage2 {
     double *part = nullptr;
     size_t part_cap = 0;
     double *acc = nullptr;         // 8 classes x 4
+    double *accq = nullptr;        // 8 classes x 2*QM (QD off-diagonal sums)
+    double *qpart = nullptr;
+    size_t qpart_cap = 0;
     int2 *pairs_c = nullptr, *pairs_cs = nullptr, *pairs_v = nullptr, *pairs_vs = nullptr, *up = nullptr;
     i64 np_c, np_cs, np_v, np_vs;
     bool blocks_ready = false;
@@ -793,6 +1142,26 @@ static int reduce_into(Stage2 *p, i64 nblocks, int cls) {
     NV2_LAUNCH_CHECK();
     return 0;
 }
+// QD partial buffer for a launch of nblocks blocks (q.M == 0 leaves q disabled)
+static int qd_prepare(Stage2 *p, QDArgs &q, i64 nblocks) {
+    if (!q.M) return 0;
+    size_t need = sizeof(double) * 2 * QM * (size_t)(nblocks > 0 ? nblocks : 1);
+    if (p->qpart_cap < need) {
+        if (p->qpart) { (void)hipStreamSynchronize(p->st); (void)hipFree(p->qpart); }
+        p->qpart = nullptr;
+        p->qpart_cap = 0;
+        if (hipMalloc(&p->qpart, need) != hipSuccess) { set_error("hipMalloc(qpart) failed"); return -1; }
+        p->qpart_cap = need;
+    }
+    q.qpart = p->qpart;
+    return 0;
+}
+static int qd_reduce(Stage2 *p, const QDArgs &q, i64 nblocks, int cls) {
+    if (!q.M) return 0;
+    hipLaunchKernelGGL(k_reduce_q, dim3(2 * QM), dim3(256), 0, p->st, p->qpart, nblocks, p->accq + 2 * QM * cls);
+    NV2_LAUNCH_CHECK();
+    return 0;
+}
 
 // PC metric solve (see oracle pc_solve): returns coef (N x nnew, col-major, ld N) and w (nnew) in named buffers
 static int pc_solve(Stage2 *p, const char *tag, int N, double *S, double *K, double thr, double **coef_out,
@@ -837,6 +1206,7 @@ Stage2 *stage2_create(const S2Inputs &in, rocblas_handle h, hipStream_t st) {
     p->norb = in.nc + in.n + in.nv;
     if (hipMalloc(&p->dinfo, 2 * sizeof(int)) != hipSuccess) { delete p; return nullptr; }
     if (hipMalloc(&p->acc, sizeof(double) * 32) != hipSuccess) { delete p; return nullptr; }
+    if (hipMalloc(&p->accq, sizeof(double) * 8 * 2 * QM) != hipSuccess) { delete p; return nullptr; }
     // pair tables: core i<=j, core i<j, virt r<=s, virt r<s, active upper pairs
     std::vector<int2> pc, pcs, pv, pvs, up;
     for (int i = 0; i < in.nc; i++)
@@ -859,7 +1229,8 @@ Stage2 *stage2_create(const S2Inputs &in, rocblas_handle h, hipStream_t st) {
 void stage2_destroy(Stage2 *p) {
     if (!p) return;
     for (auto &kv : p->bufs) if (kv.second.p) (void)hipFree(kv.second.p);
-    (void)hipFree(p->dinfo); (void)hipFree(p->part); (void)hipFree(p->acc);
+    (void)hipFree(p->dinfo); (void)hipFree(p->part); (void)hipFree(p->acc); (void)hipFree(p->accq);
+    (void)hipFree(p->qpart);
     (void)hipFree(p->pairs_c); (void)hipFree(p->pairs_cs); (void)hipFree(p->pairs_v); (void)hipFree(p->pairs_vs);
     (void)hipFree(p->up);
     delete p;
@@ -894,7 +1265,22 @@ static int prepare_blocks(Stage2 *p) {
     return 0;
 }
 
-int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *stats) {
+static double *qd_fill(Stage2 *p, const Sl &S, int kind, int lt, i64 total, const std::string &name) {
+    double *b = p->get(name, total);
+    if (!b) { set_error("hipMalloc failed for %s", name.c_str()); return nullptr; }
+    if (total > 0) {
+        hipLaunchKernelGGL(k_qd_fill, dim3(nb(total, 256)), dim3(256), 0, p->st, S, kind, lt, p->up, b, total);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) { set_error("k_qd_fill: %s", hipGetErrorString(e)); return nullptr; }
+    }
+    return b;
+}
+#define QFILL(var, kind, j, total, name) \
+    double *var = qd_fill(p, Sj[j], (kind), (j) < s, (total), std::string(name) + std::to_string(j)); \
+    if (!var) return -1;
+#define QGET(var, name, j, cnt) GET(var, (std::string(name) + std::to_string(j)).c_str(), (cnt))
+
+int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *stats, double *qd_out) {
     (void)prec;
     const int nc = p->nc, n = p->n, nv = p->nv, L = p->L, norb = p->norb, M = p->M;
     const i64 n2 = (i64)n * n, n4 = n2 * n2, n6 = n4 * n2;
@@ -1016,6 +1402,27 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
     TRYS(slice(p, p->in.hrdm, s, n4, k_hrdm)); TRYS(slice(p, p->in.eako2, s, n4, k_eako2));
     TRYS(slice(p, p->in.ipko2, s, n4, k_ipko2)); TRYS(slice(p, p->in.gk0pa, s, n4, k_ga));
     TRYS(slice(p, p->in.gk0pd, s, n4, k_gd)); TRYS(slice(p, p->in.gk0pf, s, n2, k_gf));
+    // ---- QD: transition RDMs of every pair (s, j), j != s (pair index of (max,min), qdnevpt.F:1912)
+    QDArgs q{};
+    std::vector<Sl> Sj(QM);
+    const bool qd = qd_out && M > 1 && p->in.trdm1 && p->in.trdm2 && p->in.trdm3;
+    if (qd) {
+        if (M > QM) { set_error("QD-NEVPT2: at most %d states are supported (metat=%d)", QM, M); return -2; }
+        q.M = M;
+        q.self = s;
+        NV2_HIP(hipMemsetAsync(p->accq, 0, sizeof(double) * 8 * 2 * QM, st));
+        const int P = M * (M - 1) / 2;
+        for (int j = 0; j < M; j++) {
+            if (j == s) continue;
+            const int hi = j > s ? j : s, lo = j > s ? s : j, pr = hi * (hi - 1) / 2 + lo;
+            QGET(t1, "qd_d1_", j, n2); QGET(t2, "qd_d2_", j, n4); QGET(t3, "qd_d3_", j, n6);
+            hipLaunchKernelGGL(k_slice, dim3(nb(n2, 256)), dim3(256), 0, st, p->in.trdm1, P, pr, n2, t1);
+            hipLaunchKernelGGL(k_slice, dim3(nb(n4, 256)), dim3(256), 0, st, p->in.trdm2, P, pr, n4, t2);
+            hipLaunchKernelGGL(k_slice, dim3(nb(n6, 256)), dim3(256), 0, st, p->in.trdm3, P, pr, n6, t3);
+            NV2_LAUNCH_CHECK();
+            Sj[j] = Sl{n, t1, t2, t3};
+        }
+    }
 
     // ================================================================ class (0)
     if (nc && nv) {
@@ -1038,14 +1445,24 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         sd.nnew[1] = nnew;
         GET(spc, "spc_v1", (i64)n * (nnew ? nnew : 1));
         TRYS(gemm(p, true, false, n, nnew, n, 1.0, ro, n, coef, n, 0.0, spc, n));
+        for (int j = 0; q.M && j < M; j++) {
+            if (j == s) continue;
+            QFILL(roj, QK_V1_RO, j, n2, "qd_v1ro_");
+            QFILL(Rj, QK_V1_R, j, n2, "qd_v1R_");
+            QGET(sj, "qd_v1s_", j, (i64)n * (nnew ? nnew : 1));
+            TRYS(gemm(p, true, false, n, nnew, n, 1.0, Rj, n, coef, n, 0.0, sj, n));
+            q.a[j] = roj; q.b[j] = sj;
+        }
         if (nc && nv) {
             dim3 g(nb(nv, 64), (unsigned)p->np_c);
             double *pt = parts(p, (i64)g.x * g.y);
-#define V1(NM) hipLaunchKernelGGL(k_v1<NM>, g, dim3(64), 0, st, Icacv, Nca, p->pairs_c, nc, n, nv, ec, ev, k_eako1, ro, spc, w, nnew, pt)
+            TRYS(qd_prepare(p, q, (i64)g.x * g.y));
+#define V1(NM) hipLaunchKernelGGL(k_v1<NM>, g, dim3(64), 0, st, Icacv, Nca, p->pairs_c, nc, n, nv, ec, ev, k_eako1, ro, spc, w, nnew, pt, q)
             if (n <= 8) V1(8); else V1(16);
 #undef V1
             NV2_LAUNCH_CHECK();
             TRYS(reduce_into(p, (i64)g.x * g.y, 1));
+            TRYS(qd_reduce(p, q, (i64)g.x * g.y, 1));
         }
     }
     // ================================================================ class (-1)
@@ -1059,14 +1476,24 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         sd.nnew[2] = nnew;
         GET(spc, "spc_vm1", (i64)n * (nnew ? nnew : 1));
         TRYS(gemm(p, true, false, n, nnew, n, 1.0, da, n, coef, n, 0.0, spc, n));
+        for (int j = 0; q.M && j < M; j++) {
+            if (j == s) continue;
+            QFILL(roj, QK_VM1_RO, j, n2, "qd_vm1ro_");
+            QFILL(Rj, QK_VM1_R, j, n2, "qd_vm1R_");
+            QGET(sj, "qd_vm1s_", j, (i64)n * (nnew ? nnew : 1));
+            TRYS(gemm(p, true, false, n, nnew, n, 1.0, Rj, n, coef, n, 0.0, sj, n));
+            q.a[j] = roj; q.b[j] = sj;
+        }
         if (nc && nv) {
             const i64 tot = p->np_v * nc;
             double *pt = parts(p, nb(tot, 128));
-#define VM1(NM) hipLaunchKernelGGL(k_vm1<NM>, dim3(nb(tot, 128)), dim3(128), 0, st, Icvav, Ncv, p->pairs_v, p->np_v, nc, n, ec, ev, da, k_ipko1, spc, w, nnew, pt)
+            TRYS(qd_prepare(p, q, nb(tot, 128)));
+#define VM1(NM) hipLaunchKernelGGL(k_vm1<NM>, dim3(nb(tot, 128)), dim3(128), 0, st, Icvav, Ncv, p->pairs_v, p->np_v, nc, n, ec, ev, da, k_ipko1, spc, w, nnew, pt, q)
             if (n <= 8) VM1(8); else VM1(16);
 #undef VM1
             NV2_LAUNCH_CHECK();
             TRYS(reduce_into(p, nb(tot, 128), 2));
+            TRYS(qd_reduce(p, q, nb(tot, 128), 2));
         }
     }
     // ================================================================ classes (+2) and (-2)
@@ -1082,11 +1509,19 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         const i64 npr = mode == 0 ? p->np_c : p->np_v;
         const int2 *prs = mode == 0 ? p->pairs_cs : p->pairs_vs;
         const i64 nprs = mode == 0 ? p->np_cs : p->np_vs;
+        for (int j = 0; q.M && j < M; j++) {
+            if (j == s) continue;
+            QFILL(xj, mode == 0 ? QK_P2_X : QK_M2_X, j, n4, "qd_pm2x_");
+            QFILL(xsj, mode == 0 ? QK_P2_XS : QK_M2_XS, j, n4, "qd_pm2xs_");
+            q.a[j] = xj; q.b[j] = xsj;
+        }
         if (next && npr) {
             double *pt = parts(p, npr);
-            hipLaunchKernelGGL(k_pm2_sc, dim3((unsigned)npr), dim3(256), 0, st, I, ld, pr, mode, nc, n, X, K, eo, pt);
+            TRYS(qd_prepare(p, q, npr));
+            hipLaunchKernelGGL(k_pm2_sc, dim3((unsigned)npr), dim3(256), 0, st, I, ld, pr, mode, nc, n, X, K, eo, pt, q);
             NV2_LAUNCH_CHECK();
             TRYS(reduce_into(p, npr, cls));
+            TRYS(qd_reduce(p, q, npr, cls));
         }
         // PC, off-diagonal pairs: n^2 metric
         GET(Sm, "pc_S", n4); GET(Km, "pc_K", n4); GET(Xm, "pc_X", n4);
@@ -1099,13 +1534,22 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         sd.nnew[cls] = nnew;
         GET(spc, "spc_pm2", n2 * (nnew ? nnew : 1));
         TRYS(gemm(p, true, false, (int)n2, nnew, (int)n2, 1.0, Xm, n2, coef, n2, 0.0, spc, n2));
+        for (int j = 0; q.M && j < M; j++) {
+            if (j == s) continue;
+            QFILL(yj, mode == 0 ? QK_P2_Y : QK_M2_Y, j, n4, "qd_pm2y_");
+            QGET(sj, "qd_pm2s_", j, n2 * (nnew ? nnew : 1));
+            TRYS(gemm(p, true, false, (int)n2, nnew, (int)n2, 1.0, yj, n2, coef, n2, 0.0, sj, n2));
+            q.a[j] = sj;
+        }
         if (nprs && nnew) {
             const i64 tot = nprs * nnew;
             double *pt = parts(p, nb(tot, 128));
+            TRYS(qd_prepare(p, q, nb(tot, 128)));
             hipLaunchKernelGGL(k_pm2_pc, dim3(nb(tot, 128)), dim3(128), 0, st, I, ld, prs, nprs, mode, nc, n, spc, w,
-                               nnew, eo, pt);
+                               nnew, eo, pt, q);
             NV2_LAUNCH_CHECK();
             TRYS(reduce_into(p, nb(tot, 128), cls));
+            TRYS(qd_reduce(p, q, nb(tot, 128), cls));
         }
         // PC, diagonal pairs: n(n+1)/2 metric
         const int m = n * (n + 1) / 2;
@@ -1118,13 +1562,22 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         if (mw2 < sd.min_metric_eig[cls]) sd.min_metric_eig[cls] = mw2;
         GET(spc2, "spc_pm2p", (i64)m * (nnew ? nnew : 1));
         TRYS(gemm(p, true, false, m, nnew, m, 0.5, Xm, m, coef, m, 0.0, spc2, m));
+        for (int j = 0; q.M && j < M; j++) {
+            if (j == s) continue;
+            QFILL(yj, mode == 0 ? QK_P2P_Y : QK_M2P_Y, j, (i64)m * m, "qd_pm2py_");
+            QGET(sj, "qd_pm2ps_", j, (i64)m * (nnew ? nnew : 1));
+            TRYS(gemm(p, true, false, m, nnew, m, 1.0, yj, m, coef, m, 0.0, sj, m));
+            q.a[j] = sj;
+        }
         if (next && nnew) {
             const i64 tot = (i64)next * nnew;
             double *pt = parts(p, nb(tot, 128));
+            TRYS(qd_prepare(p, q, nb(tot, 128)));
             hipLaunchKernelGGL(k_pm2p_pc, dim3(nb(tot, 128)), dim3(128), 0, st, I, ld, next, mode, nc, n, spc2, w,
-                               nnew, eo, pt);
+                               nnew, eo, pt, q);
             NV2_LAUNCH_CHECK();
             TRYS(reduce_into(p, nb(tot, 128), cls));
+            TRYS(qd_reduce(p, q, nb(tot, 128), cls));
         }
     }
     // ================================================================ class (0)'
@@ -1132,13 +1585,22 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         GET(X1, "x1", n4); GET(X2, "x2", n4); GET(G1, "g1", n4); GET(G2, "g2", n4);
         hipLaunchKernelGGL(k_v0p_prep, dim3(nb(n4, 256)), dim3(256), 0, st, S, k_ga, k_gd, X1, X2, G1, G2);
         NV2_LAUNCH_CHECK();
+        for (int j = 0; q.M && j < M; j++) {
+            if (j == s) continue;
+            QFILL(x1j, QK_0P_X1, j, n4, "qd_0px1_");
+            QFILL(x2j, QK_0P_X2, j, n4, "qd_0px2_");
+            QFILL(dsj, QK_0P_DS, j, n2, "qd_0pds_");
+            q.a[j] = x1j; q.b[j] = x2j; q.c[j] = dsj;
+        }
         if (nc && nv) {
             const i64 nbk = (i64)nc * nv;
             double *pt = parts(p, nbk);
+            TRYS(qd_prepare(p, q, nbk));
             hipLaunchKernelGGL(k_v0p_sc, dim3((unsigned)nbk), dim3(256), 0, st, Icvaa, Ncv, Icaav, Nca, nc, n, nv, X1,
-                               X2, G1, G2, da, k_gf, h0p, ec, ev, pt);
+                               X2, G1, G2, da, k_gf, h0p, ec, ev, pt, q);
             NV2_LAUNCH_CHECK();
             TRYS(reduce_into(p, nbk, 7));
+            TRYS(qd_reduce(p, q, nbk, 7));
         }
         const int N = 2 * (int)n2;
         GET(Sm, "pc_S", (i64)N * N); GET(Km, "pc_K", (i64)N * N);
@@ -1159,12 +1621,28 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             hipLaunchKernelGGL(k_dacoef, dim3(nb(nnew, 128)), dim3(128), 0, st, da, n, coef, N, 0, nnew, dc1);
             hipLaunchKernelGGL(k_dacoef, dim3(nb(nnew, 128)), dim3(128), 0, st, da, n, coef, N, (int)n2, nnew, dc2);
             NV2_LAUNCH_CHECK();
+            for (int j = 0; q.M && j < M; j++) {
+                if (j == s) continue;
+                QFILL(vj, QK_0P_V, j, n4, "qd_0pv_");
+                QFILL(v2j, QK_0P_V2, j, n4, "qd_0pv2_");
+                QFILL(roj, QK_0P_RO, j, n2, "qd_0pro_");
+                QGET(smj, "qd_0psm_", j, n2 * nnew); QGET(spj, "qd_0psp_", j, n2 * nnew);
+                QGET(sppj, "qd_0pspp_", j, n2 * nnew); QGET(r0j, "qd_0pr0_", j, nnew); QGET(r1j, "qd_0pr1_", j, nnew);
+                TRYS(gemm(p, false, false, (int)n2, nnew, (int)n2, 1.0, vj, n2, coef, N, 0.0, smj, n2));
+                TRYS(gemm(p, false, false, (int)n2, nnew, (int)n2, 1.0, vj, n2, coef + n2, N, 0.0, spj, n2));
+                TRYS(gemm(p, false, false, (int)n2, nnew, (int)n2, 1.0, v2j, n2, coef + n2, N, 0.0, sppj, n2));
+                TRYS(gemm(p, true, false, 1, nnew, (int)n2, 1.0, roj, n2, coef, N, 0.0, r0j, 1));
+                TRYS(gemm(p, true, false, 1, nnew, (int)n2, 1.0, roj, n2, coef + n2, N, 0.0, r1j, 1));
+                q.a[j] = smj; q.b[j] = spj; q.c[j] = sppj; q.d[j] = r0j; q.e[j] = r1j;
+            }
             const i64 nbk = (i64)nc * nv;
             double *pt = parts(p, nbk);
+            TRYS(qd_prepare(p, q, nbk));
             hipLaunchKernelGGL(k_v0p_pc, dim3((unsigned)nbk), dim3(256), 0, st, Icvaa, Ncv, Icaav, Nca, nc, n, nv, sm,
-                               sp, spp, dc1, dc2, w, nnew, h0p, ec, ev, pt);
+                               sp, spp, dc1, dc2, w, nnew, h0p, ec, ev, pt, q);
             NV2_LAUNCH_CHECK();
             TRYS(reduce_into(p, nbk, 7));
+            TRYS(qd_reduce(p, q, nbk, 7));
         }
     }
     // ================================================================ classes (-1)' and (+1)'
@@ -1210,11 +1688,26 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
                 TRYS(gemm(p, false, false, (int)N, next, n, 1.0, BC, N, H, n, 0.0, Z2, N));
                 TRYS(gemm(p, false, false, (int)N, next, (int)N, 1.0, Mx, N, V, N, 0.0, Y1, N));   // Spc V = M^T v
                 TRYS(gemm(p, false, false, (int)N, next, (int)N, 1.0, Kpc, N, V, N, 0.0, Y2, N));  // Kpc V = Am^T v
+                for (int j = 0; q.M && j < M; j++) {
+                    if (j == s) continue;
+                    QFILL(r1j, plus ? QK_P1P_R1 : QK_M1P_R1, j, n2, "qd_pm1r1_");
+                    QFILL(qj, plus ? QK_P1P_QSC : QK_M1P_QSC, j, N * n, "qd_pm1q_");
+                    double *mj = qd_fill(p, Sj[j], plus ? QK_P1P_MSC : QK_M1P_MSC, j < s, N * N, "qd_pm1M");
+                    if (!mj) return -1;
+                    QGET(w1j, "qd_pm1w1_", j, (i64)n * next); QGET(z1j, "qd_pm1z1_", j, N * next);
+                    QGET(y1j, "qd_pm1y1_", j, N * next);
+                    TRYS(gemm(p, false, false, n, next, n, 1.0, r1j, n, H, n, 0.0, w1j, n));
+                    TRYS(gemm(p, false, false, (int)N, next, n, 1.0, qj, N, H, n, 0.0, z1j, N));
+                    TRYS(gemm(p, false, false, (int)N, next, (int)N, 1.0, mj, N, V, N, 0.0, y1j, N));
+                    q.a[j] = w1j; q.b[j] = z1j; q.c[j] = y1j;
+                }
                 double *pt = parts(p, next);
+                TRYS(qd_prepare(p, q, next));
                 hipLaunchKernelGGL(k_pm1p_sc, dim3(next), dim3(256), 0, st, plus, n, next, H, W1, W2, V, Z1, Z2, Y1, Y2,
-                                   eo, pt);
+                                   eo, pt, q);
                 NV2_LAUNCH_CHECK();
                 TRYS(reduce_into(p, next, cls));
+                TRYS(qd_reduce(p, q, next, cls));
                 // PC: metric solve at N = n^3, S-build, sums
                 double *coef, *w; int nnew;
                 TRYS(pc_solve(p, plus ? "vp1p" : "vm1p", (int)N, Spc, Kpc, thr, &coef, &w, &nnew, &sd.min_metric_eig[cls]));
@@ -1225,11 +1718,27 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
                     TRYS(gemm(p, true, false, n, nnew, (int)N, 1.0, Q2, N, coef, N, 0.0, spT, n));
                     TRYS(gemm(p, true, false, nnew, next, (int)N, 1.0, SbT, N, V, N, 0.0, R, nnew));
                     TRYS(gemm(p, true, false, nnew, next, n, 1.0, spT, n, H, n, 1.0, R, nnew));
+                    for (int j = 0; q.M && j < M; j++) {
+                        if (j == s) continue;
+                        double *mj = qd_fill(p, Sj[j], plus ? QK_P1P_MPC : QK_M1P_MPC, j < s, N * N, "qd_pm1M");
+                        if (!mj) return -1;
+                        QFILL(qj, plus ? QK_P1P_QPC : QK_M1P_QPC, j, N * n, "qd_pm1qp_");
+                        GET(sbj, "qd_pm1SbT", N * nnew); GET(spj, "qd_pm1spT", (i64)n * nnew);
+                        QGET(rj, "qd_pm1R_", j, (i64)nnew * next);
+                        TRYS(gemm(p, false, false, (int)N, nnew, (int)N, 1.0, mj, N, coef, N, 0.0, sbj, N));
+                        TRYS(gemm(p, true, false, n, nnew, (int)N, 1.0, qj, N, coef, N, 0.0, spj, n));
+                        TRYS(gemm(p, true, false, nnew, next, (int)N, 1.0, sbj, N, V, N, 0.0, rj, nnew));
+                        TRYS(gemm(p, true, false, nnew, next, n, 1.0, spj, n, H, n, 1.0, rj, nnew));
+                        q.a[j] = rj;
+                    }
                     const i64 tot = (i64)nnew * next;
                     double *pt2 = parts(p, nb(tot, 256));
-                    hipLaunchKernelGGL(k_pm1p_pc, dim3(nb(tot, 256)), dim3(256), 0, st, plus, nnew, next, R, w, eo, pt2);
+                    TRYS(qd_prepare(p, q, nb(tot, 256)));
+                    hipLaunchKernelGGL(k_pm1p_pc, dim3(nb(tot, 256)), dim3(256), 0, st, plus, nnew, next, R, w, eo, pt2,
+                                       q);
                     NV2_LAUNCH_CHECK();
                     TRYS(reduce_into(p, nb(tot, 256), cls));
+                    TRYS(qd_reduce(p, q, nb(tot, 256), cls));
                 }
             }
         }
@@ -1244,8 +1753,19 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         out[4 * k + 2] = accs[4 * k + 3];
         out[4 * k + 3] = accs[4 * k + 2];
     }
+    if (q.M) {   // qd_out[k*2M + j] = e2mp(j,s) of class k, qd_out[k*2M + M + j] = e2en(j,s)
+        std::vector<double> hq(8 * 2 * QM);
+        NV2_HIP(hipMemcpyAsync(hq.data(), p->accq, sizeof(double) * hq.size(), hipMemcpyDeviceToHost, st));
+        NV2_HIP(hipStreamSynchronize(st));
+        for (int k = 0; k < 8; k++)
+            for (int j = 0; j < M; j++) {
+                qd_out[k * 2 * M + j] = j == s ? 0.0 : hq[k * 2 * QM + j];
+                qd_out[k * 2 * M + M + j] = j == s ? 0.0 : hq[k * 2 * QM + QM + j];
+            }
+    } else if (qd_out) {
+        for (int k = 0; k < 16 * M; k++) qd_out[k] = 0.0;
+    }
     if (stats) *stats = sd;
-    (void)M;
     return 0;
 }
 

@@ -28,7 +28,30 @@ class Evaluator:
         ev = cls(P.ncore, P.nact, P.nvirt, P.metat, P.nele, P.B.shape[0])
         ev.set_integrals(P.B, P.oneint)
         ev.set_ro4(P.ro4)
+        if P.metat > 1:
+            ev.set_trdm(P.daaaoff, P.taaoff, P.daloff)
         return ev
+
+    def set_trdm(self, trdm3, trdm2, trdm1):
+        """Transition RDMs (3-TRDM, 2-TRDM, 1-TRDM) of the state pairs, for QD-NEVPT2."""
+        L.check(L.lib().nv2_ctx_set_trdm(self._h, L.ptr(L.fin(trdm3)), L.ptr(L.fin(trdm2)), L.ptr(L.fin(trdm1))),
+                "nv2_ctx_set_trdm")
+
+    def heff(self):
+        """Second-order effective Hamiltonians (SC, PC) without E0; element [j, i] = e2mp(j,i)."""
+        M = self.M
+        sc = np.zeros(M * M)
+        pc = np.zeros(M * M)
+        L.check(L.lib().nv2_ctx_get(self._h, b"Heff-SC", L.ptr(sc), sc.size), "nv2_ctx_get(Heff-SC)")
+        L.check(L.lib().nv2_ctx_get(self._h, b"Heff-PC", L.ptr(pc), pc.size), "nv2_ctx_get(Heff-PC)")
+        return sc.reshape(M, M, order="F"), pc.reshape(M, M, order="F")
+
+    def qd_terms(self):
+        """Per-class QD couplings: array [i, k, 0|1, j] = SC|PC contribution of class k to Heff(j, i)."""
+        M = self.M
+        q = np.zeros(16 * M * M)
+        L.check(L.lib().nv2_ctx_get(self._h, b"QD", L.ptr(q), q.size), "nv2_ctx_get(QD)")
+        return q.reshape(M, 8, 2, M)
 
     def set_integrals(self, cholesky, oneint):
         B = L.fin(cholesky)

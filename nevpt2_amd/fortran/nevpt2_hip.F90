@@ -50,6 +50,27 @@ module nevpt2_hip
       real(c_double), intent(out) :: ms(*)
       integer(c_int) :: rc
     end function
+    function nv2_ctx_set_trdm(ctx, trdm3, trdm2, trdm1) bind(C, name="nv2_ctx_set_trdm") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: trdm3(*), trdm2(*), trdm1(*)
+      integer(c_int) :: rc
+    end function
+    function nv2_ctx_get(ctx, name, host, count) bind(C, name="nv2_ctx_get") result(rc)
+      import :: c_ptr, c_int, c_char, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: name(*)
+      real(c_double), intent(out) :: host(*)
+      integer(c_int64_t), value :: count
+      integer(c_int) :: rc
+    end function
+    function nv2_qd_heff(metat, heff2, e0, heff, energies) bind(C, name="nv2_qd_heff") result(rc)
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: metat
+      real(c_double), intent(in) :: heff2(*), e0(*)
+      real(c_double), intent(out) :: heff(*), energies(*)
+      integer(c_int) :: rc
+    end function
     subroutine nv2_ctx_destroy(ctx) bind(C, name="nv2_ctx_destroy")
       import :: c_ptr
       type(c_ptr), value :: ctx

@@ -1,7 +1,8 @@
 ! Fortran host of the MI355X NEVPT2 hot path: reads input.nevpt, the input
 ! bundle named by .FILE04 (packed 4-RDM, Cholesky vectors, one-electron
-! integrals, reference energies), runs one device-resident evaluation and prints
-! the reference's per-class table and summary (qdnevpt.F:1004-1230).
+! integrals, reference energies and, for several roots, the transition RDMs),
+! runs one device-resident evaluation and prints the reference's per-class
+! table, the QD effective Hamiltonians and the summary (qdnevpt.F:1004-1230).
 program nevpt2_amd_main
   use iso_c_binding
   use nevpt2_hip
@@ -10,6 +11,9 @@ program nevpt2_amd_main
   type(c_ptr) :: ctx
   integer(c_int64_t) :: hdr(8), ncore, nact, nvirt, metat, nele, nchovec, norb, npair, nwords
   real(c_double), allocatable :: e0(:), oneint(:), chol(:), ro4(:), tab(:), ms(:)
+  real(c_double), allocatable :: t3(:), t2(:), t1(:), hsc(:), hpc(:), hh(:), esc(:), epc(:)
+  integer(c_int64_t) :: npr
+  logical :: qd
   character(len=8) :: magic
   integer :: u, ios, s, k, i
   integer(c_int) :: rc
@@ -32,6 +36,13 @@ program nevpt2_amd_main
   nwords = nact * (nact + 1) * (nact + 2) * (nact + 3) / 24
   allocate(e0(metat), oneint(npair), chol(nchovec * npair), ro4(nwords * nact**4 * metat))
   read(u) e0, oneint, chol, ro4
+  qd = metat > 1 .and. hdr(1) >= 2
+  if (metat > 1 .and. .not. qd) stop 'ERROR: several roots need the transition RDMs (bundle version 2)'
+  if (qd) then
+    npr = metat * (metat - 1) / 2
+    allocate(t3(npr * nact**6), t2(npr * nact**4), t1(npr * nact**2))
+    read(u) t3, t2, t1
+  end if
   close(u)
 
   ctx = nv2_ctx_create(ncore, nact, nvirt, metat, nele, nchovec)
@@ -41,6 +52,7 @@ program nevpt2_amd_main
   end if
   rc = nv2_ctx_set_integrals(ctx, chol, oneint)
   if (rc == 0) rc = nv2_ctx_set_ro4(ctx, ro4)
+  if (rc == 0 .and. qd) rc = nv2_ctx_set_trdm(ctx, t3, t2, t1)
   allocate(tab(32 * metat), ms(NV2_NPHASE))
   if (rc == 0) rc = nv2_ctx_run(ctx, int(cfg%prec, c_int32_t), real(cfg%thr_lindep, c_double), tab)
   if (rc /= 0) then
@@ -49,10 +61,19 @@ program nevpt2_amd_main
     stop 'ERROR: device evaluation failed'
   end if
   rc = nv2_ctx_timings(ctx, ms)
+  allocate(hsc(metat * metat), hpc(metat * metat), hh(metat * metat), esc(metat), epc(metat))
+  rc = nv2_ctx_get(ctx, 'Heff-SC' // c_null_char, hsc, metat * metat)
+  if (rc == 0) rc = nv2_ctx_get(ctx, 'Heff-PC' // c_null_char, hpc, metat * metat)
   call nv2_ctx_destroy(ctx)
+  if (rc /= 0) stop 'ERROR: effective Hamiltonian'
 
-  write(6, '(//a//)') ' state-specific NEVPT2: FINAL RESULTS'
+  if (qd) then
+    write(6, '(//a//)') ' QD-NEVPT2: FINAL RESULTS'
+  else
+    write(6, '(//a//)') ' state-specific NEVPT2: FINAL RESULTS'
+  end if
   write(6, '(a)') ' Second order contribution of each class to the energies'
+  if (qd) write(6, '(a)') ' (to the diagonal elements of Heff)'
   do s = 1, int(metat)
     write(6, '(/a,i4)') ' state number:', s
     write(6, '(a)') ' -----------------'
@@ -68,22 +89,80 @@ program nevpt2_amd_main
     write(6, '(1x,85a1)') ('-', i = 1, 85)
     write(6, '(1x,a,4f20.15)') 'Total', nsc_tot, sc_tot, npc_tot, pc_tot
   end do
+  ! energies: diagonal sums (state-specific) or eigenvalues of the hermitised Heff (QD)
+  do s = 1, int(metat)
+    esc(s) = e0(s) + hsc(s + metat * (s - 1))
+    epc(s) = e0(s) + hpc(s + metat * (s - 1))
+  end do
+  if (qd) then
+    call print_heff('SC', hsc)
+    rc = nv2_qd_heff(metat, hsc, e0, hh, esc)
+    if (rc /= 0) stop 'ERROR: SC effective Hamiltonian'
+    call print_heff_h('SC', hh)
+    if (.not. cfg%no_pc) then
+      call print_heff('PC', hpc)
+      rc = nv2_qd_heff(metat, hpc, e0, hh, epc)
+      if (rc /= 0) stop 'ERROR: PC effective Hamiltonian'
+      call print_heff_h('PC', hh)
+    end if
+  end if
   write(6, '(/)')
-  write(6, '(a)') '                   Summary of the NEVPT2 energies'
+  if (qd) then
+    write(6, '(a)') '                 Summary of the QD-NEVPT2 energies'
+  else
+    write(6, '(a)') '                   Summary of the NEVPT2 energies'
+  end if
   write(6, '(1x,72("-"))')
   write(6, '(a/)') ' state   diagonalization     strongly-contracted     partially-contracted'
   do s = 1, int(metat)
-    sc_tot = e0(s); pc_tot = e0(s)
-    do k = 1, 8
-      i = 32 * (s - 1) + 4 * (k - 1)
-      sc_tot = sc_tot + tab(i + 2); pc_tot = pc_tot + tab(i + 4)
-    end do
     if (cfg%no_pc) then
-      write(6, '(a2,i4,2f19.7,9x,a)') ' @', s, e0(s), sc_tot, ' not requested'
+      write(6, '(a2,i4,2f19.7,9x,a)') ' @', s, e0(s), esc(s), ' not requested'
     else
-      write(6, '(a2,i4,f18.7,6x,f18.7,7x,f18.7)') ' @', s, e0(s), sc_tot, pc_tot
+      write(6, '(a2,i4,f18.7,6x,f18.7,7x,f18.7)') ' @', s, e0(s), esc(s), epc(s)
     end if
   end do
   write(6, '(1x,72("-"))')
   write(6, '(a,6f10.3)') ' device phase times (ms): ', ms
+
+contains
+
+  ! matout layout (qdnevpt.F:1639-1659): five columns per block
+  subroutine matout(a, m)
+    real(c_double), intent(in) :: a(:)
+    integer(c_int64_t), intent(in) :: m
+    integer :: i0, i1, r, c
+    do i0 = 1, int(m), 5
+      i1 = min(i0 + 4, int(m))
+      write(6, '(7x,5(8x,i2,7x))') (c, c = i0, i1)
+      do r = 1, int(m)
+        write(6, '(1x,i2,4x,5(f17.10))') r, (a(r + m * (c - 1)), c = i0, i1)
+      end do
+    end do
+  end subroutine
+
+  subroutine print_heff(key, h)
+    character(len=2), intent(in) :: key
+    real(c_double), intent(in) :: h(:)
+    real(c_double), allocatable :: z(:)
+    integer :: s
+    write(6, '(/a)') ' *******************************************'
+    write(6, '(a)') ' * Second order Effective Hamiltonian (' // key // ') *'
+    write(6, '(a/)') ' *******************************************'
+    call matout(h, metat)
+    z = h
+    do s = 1, int(metat)
+      z(s + metat * (s - 1)) = z(s + metat * (s - 1)) + e0(s)
+    end do
+    write(6, '(a)') ' **************************************************'
+    write(6, '(a)') ' * Zero + second order Effective Hamiltonian (' // key // ') *'
+    write(6, '(a/)') ' **************************************************'
+    call matout(z, metat)
+  end subroutine
+
+  subroutine print_heff_h(key, h)
+    character(len=2), intent(in) :: key
+    real(c_double), intent(in) :: h(:)
+    write(6, '(a)') '  Heff~ (Hermitian) --> ' // key
+    call matout(h, metat)
+  end subroutine
 end program nevpt2_amd_main

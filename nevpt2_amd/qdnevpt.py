@@ -51,3 +51,14 @@ def format_table(tab, state=1):
     tot = tab.sum(axis=0)
     lines.append(" Total" + "".join(f"{v:20.15f}" for v in tot))
     return "\n".join(lines)
+
+
+def qd_heff(heff2, e0):
+    """Hermitised zero+second-order Heff and its eigenvalues (the QD-NEVPT2 energies) on the GPU."""
+    H = L.fin(np.asarray(heff2, dtype=np.float64))
+    M = H.shape[0]
+    e = np.ascontiguousarray(e0, dtype=np.float64)
+    out = np.zeros((M, M), order="F")
+    w = np.zeros(M)
+    L.check(L.lib().nv2_qd_heff(M, L.ptr(H), L.ptr(e), L.ptr(out), L.ptr(w)), "nv2_qd_heff")
+    return w, out

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from nevpt2_amd import synth, bundle
-from oracle import stage1 as S1, stage2 as S2
+from oracle import stage1 as S1, stage2 as S2, qd as Q
 from conftest import energy_tol
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -16,7 +16,7 @@ INPUT = """*MODEL
 .CHOLES
 *KOOPRO
 .STATES
-1
+{nstates}
 .NACTEL
 {nele}
 .NO ZOR
@@ -47,7 +47,7 @@ def test_host_end_to_end(gpu, tmp_path):
     print("[fortran host] CAS(4,4) nc=2 nv=5", flush=True)
     P = synth.Problem(4, 2, 5, seed=3)
     bundle.write_bundle(tmp_path / "in.bin", P, e0=[-1.5])
-    (tmp_path / "input.nevpt").write_text(INPUT.format(nele=P.nele, bundle="in.bin"))
+    (tmp_path / "input.nevpt").write_text(INPUT.format(nstates=1, nele=P.nele, bundle="in.bin"))
     r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     tab = bundle.read_table(r.stdout)
@@ -57,3 +57,28 @@ def test_host_end_to_end(gpu, tmp_path):
     assert (np.abs(tab[0][:, 1] - ref[:, 1]) <= energy_tol(ref[:, 0])).all()
     assert (np.abs(tab[0][:, 3] - ref[:, 3]) <= energy_tol(ref[:, 2])).all()
     assert " @   1" in r.stdout
+
+
+@pytest.mark.gpu
+def test_host_qd_two_roots(gpu, tmp_path):
+    """Two roots: the host reads the transition RDMs, prints Heff and the QD summary,
+    whose energies are the eigenvalues of the hermitised Heff (qdnevpt.F:1046-1230)."""
+    _need_exe()
+    print("[fortran host] QD CAS(4,4) nc=2 nv=3, 2 roots", flush=True)
+    P = synth.Problem(4, 2, 3, seed=1, nroots=2)     # a seed whose SC and PC Heff have real spectra
+    e0 = np.array([-1.5, -1.2])
+    bundle.write_bundle(tmp_path / "in.bin", P, e0=e0)
+    (tmp_path / "input.nevpt").write_text(INPUT.format(nstates=2, nele=P.nele, bundle="in.bin"))
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "QD-NEVPT2: FINAL RESULTS" in r.stdout
+    assert "Heff~ (Hermitian) --> SC" in r.stdout
+    summ = bundle.read_qd_summary(r.stdout)
+    koop = S1.koopmans_all(P)
+    Hsc, Hpc, _, _ = Q.heff_matrices(P, koop)
+    esc, _ = Q.diagonalize_heff(Hsc, e0)
+    epc, _ = Q.diagonalize_heff(Hpc, e0)
+    # the summary prints 7 decimals
+    assert np.abs(summ[:, 0] - e0).max() < 1e-6
+    assert np.abs(summ[:, 1] - esc).max() < 2e-6 * max(1.0, np.abs(esc).max())
+    assert np.abs(summ[:, 2] - epc).max() < 2e-6 * max(1.0, np.abs(epc).max())

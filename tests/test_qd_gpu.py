@@ -1,0 +1,68 @@
+"""QD-NEVPT2 on the GPU: off-diagonal class couplings and the effective Hamiltonian
+against the oracle (oracle/qd.py), through the resident context's C ABI."""
+import numpy as np
+import pytest
+
+from nevpt2_amd import synth, qdnevpt
+from nevpt2_amd.evaluator import Evaluator
+from oracle import stage1 as S1, qd as Q
+from conftest import energy_tol
+
+CASES = [  # (nact, ncore, nvirt, seed, roots)
+    (4, 2, 3, 5, 3),
+    (6, 3, 5, 2, 2),
+]
+
+
+@pytest.fixture(scope="module", params=CASES, ids=lambda c: f"cas{c[0]}_c{c[1]}_v{c[2]}_m{c[4]}")
+def qdcase(request):
+    n, nc, nv, seed, M = request.param
+    print(f"[qd case] nact={n} ncore={nc} nvirt={nv} seed={seed} roots={M}", flush=True)
+    P = synth.Problem(n, nc, nv, seed=seed, nroots=M)
+    koop = S1.koopmans_all(P)
+    Hsc, Hpc, tabs, off = Q.heff_matrices(P, koop)
+    return P, koop, Hsc, Hpc, tabs, off
+
+
+@pytest.mark.gpu
+def test_qd_couplings(gpu, qdcase):
+    P, koop, Hsc, Hpc, tabs, off = qdcase
+    ev = Evaluator.for_problem(P)
+    try:
+        got = ev.run()
+        gsc, gpc = ev.heff()
+        terms = ev.qd_terms()
+    finally:
+        ev.close()
+    M = P.metat
+    # diagonal tables as in single-state runs
+    for i in range(M):
+        tol = energy_tol(tabs[i][:, [0, 0, 2, 2]])
+        assert (np.abs(got[i][:, [1, 3]] - tabs[i][:, [1, 3]]) <= tol[:, [1, 3]]).all(), (i, got[i], tabs[i])
+    # per-class couplings: relative to the class's largest coupling, round-off amplified like the energies
+    for k, (osc, opc) in off.items():
+        for which, ref in ((0, osc), (1, opc)):
+            g = np.array([[terms[i, k, which, j] for i in range(M)] for j in range(M)])
+            scale = max(1.0, np.abs(ref).max())
+            err = np.abs(g - ref).max()
+            assert err <= 1e-9 * scale, (k, which, err, g, ref)
+    assert np.abs(gsc - Hsc).max() <= 1e-9 * max(1.0, np.abs(Hsc).max())
+    assert np.abs(gpc - Hpc).max() <= 1e-9 * max(1.0, np.abs(Hpc).max())
+
+
+@pytest.mark.gpu
+def test_qd_heff_diagonalisation(gpu, qdcase):
+    P, koop, Hsc, Hpc, tabs, off = qdcase
+    M = P.metat
+    e0 = -1.0 + 0.15 * np.arange(M)
+    for H in (Hsc, Hpc):
+        try:
+            ref_e, ref_h = Q.diagonalize_heff(H, e0)
+        except ValueError:
+            with pytest.raises(Exception):
+                qdnevpt.qd_heff(H, e0)
+            continue
+        w, Ht = qdnevpt.qd_heff(H, e0)
+        scale = max(1.0, np.abs(ref_e).max())
+        assert np.abs(w - ref_e).max() <= 1e-10 * scale, (w, ref_e)
+        assert np.abs(Ht - ref_h).max() <= 1e-8 * scale, (Ht, ref_h)
