@@ -16,6 +16,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "nv2_common.h"
@@ -1046,10 +1047,20 @@ struct Buf {
     size_t cap = 0;
 };
 
+// independent PC eigen-solves run on worker streams (each with its own rocBLAS handle)
+#define NW 2
 struct Stage2 {
     S2Inputs in;
     rocblas_handle h;
     hipStream_t st;
+    rocblas_handle wh[NW] = {};
+    hipStream_t ws[NW] = {};
+    hipEvent_t wev[NW + 1] = {};
+    double *hpin = nullptr;        // pinned host copies of metric eigenvalues
+    size_t hpin_cap = 0;
+    int *hinfo = nullptr;          // pinned host copies of solver info words
+    int *dinfo_w = nullptr;        // device info words, 2 per solve slot
+    double *wE[NW] = {};           // syevd workspace per worker
     int nc, n, nv, norb, L, M;
     std::map<std::string, Buf> bufs;
     int *dinfo = nullptr;
@@ -1067,7 +1078,7 @@ struct Stage2 {
         Buf &b = bufs[name];
         size_t bytes = sizeof(double) * (size_t)(count > 0 ? count : 1);
         if (b.cap < bytes) {
-            if (b.p) { (void)hipStreamSynchronize(st); (void)hipFree(b.p); }
+            if (b.p) { (void)hipDeviceSynchronize(); (void)hipFree(b.p); }
             b.p = nullptr;
             if (hipMalloc(&b.p, bytes) != hipSuccess) { b.cap = 0; return nullptr; }
             b.cap = bytes;
@@ -1078,17 +1089,21 @@ struct Stage2 {
 
 #define GET(var, name, cnt) double *var = p->get(name, (cnt)); if (!var) { set_error("hipMalloc failed for %s", name); return -1; }
 
-static int gemm(Stage2 *p, bool tA, bool tB, int m, int nn, int k, double alpha, const double *A, i64 lda,
-                const double *B, i64 ldb, double beta, double *C, i64 ldc) {
+static int gemm_on(rocblas_handle h, hipStream_t st, bool tA, bool tB, int m, int nn, int k, double alpha,
+                   const double *A, i64 lda, const double *B, i64 ldb, double beta, double *C, i64 ldc) {
     if (m <= 0 || nn <= 0) return 0;
     if (k <= 0) {
-        if (beta == 0.0) NV2_HIP(hipMemset2DAsync(C, sizeof(double) * ldc, 0, sizeof(double) * m, nn, p->st));
+        if (beta == 0.0) NV2_HIP(hipMemset2DAsync(C, sizeof(double) * ldc, 0, sizeof(double) * m, nn, st));
         return 0;
     }
-    RB(rocblas_dgemm(p->h, tA ? rocblas_operation_transpose : rocblas_operation_none,
+    RB(rocblas_dgemm(h, tA ? rocblas_operation_transpose : rocblas_operation_none,
                      tB ? rocblas_operation_transpose : rocblas_operation_none, m, nn, k, &alpha, A, (int)lda, B,
                      (int)ldb, &beta, C, (int)ldc));
     return 0;
+}
+static int gemm(Stage2 *p, bool tA, bool tB, int m, int nn, int k, double alpha, const double *A, i64 lda,
+                const double *B, i64 ldb, double beta, double *C, i64 ldc) {
+    return gemm_on(p->h, p->st, tA, tB, m, nn, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 static int gemm_sb(Stage2 *p, int m, int nn, int k, const double *A, i64 lda, i64 sA, const double *B, i64 ldb,
                    i64 sB, double *C, i64 ldc, i64 sC, int batch) {
@@ -1127,6 +1142,61 @@ static int syevd(Stage2 *p, bool upper, int N, double *A, double *w) {
     if (info != 0) { set_error("syevd(N=%d) failed, info=%d", N, info); return -3; }
     return 0;
 }
+// the same for nb problems A_b = A + b*sA (ld N), eigenvalues ev + b*sD, on worker w without
+// waiting; info words go to dinfo_w[idx + b] and are copied to hinfo[idx + b]
+static int syevd_async(Stage2 *p, int w, int idx, bool upper, int N, double *A, i64 sA, double *ev, i64 sD, int nb) {
+    if (N <= 0) return 0;
+    double *E = p->wE[w];
+    const rocblas_fill uplo = upper ? rocblas_fill_upper : rocblas_fill_lower;
+    int *info = p->dinfo_w + idx;
+    const int algo = eigen_algo();
+    if (algo == 0 && nb > 1) {
+        RB(rocsolver_dsyevd_strided_batched(p->wh[w], rocblas_evect_original, uplo, N, A, N, sA, ev, sD, E, N, info, nb));
+    } else {
+        for (int b = 0; b < nb; b++) {
+            double *Ab = A + b * sA, *eb = ev + b * sD;
+            if (algo == 1) {
+                RB(rocsolver_dsyevj(p->wh[w], rocblas_esort_ascending, rocblas_evect_original, uplo, N, Ab, N, 0.0, E,
+                                    100, p->dinfo_w + 62, eb, info + b));
+            } else if (algo == 2) {
+                RB(rocsolver_dsyevdj(p->wh[w], rocblas_evect_original, uplo, N, Ab, N, eb, info + b));
+            } else {
+                RB(rocsolver_dsyevd(p->wh[w], rocblas_evect_original, uplo, N, Ab, N, eb, E, info + b));
+            }
+        }
+    }
+    NV2_HIP(hipMemcpyAsync(p->hinfo + idx, info, sizeof(int) * nb, hipMemcpyDeviceToHost, p->ws[w]));
+    return 0;
+}
+// Pads problem b of a batch (n_b x n_b used block of an m x m matrix, stride sA) to m x m
+// with a diagonal value above every eigenvalue of every used block: twice the largest
+// Gershgorin row sum plus one, so the padded block stays decoupled and sorts last while
+// the matrix norm, and with it the solver's tolerances, stays that of the used blocks.
+__global__ void k_pad_gersh(double *A, int ld, i64 sA, int nb, int n0, int n1, int m) {
+    __shared__ double sh[256];
+    double mx = 0.0;
+    for (int b = 0; b < nb; b++) {
+        const int nu = b == 0 ? n0 : n1;
+        const double *Ab = A + b * sA;
+        for (int i = threadIdx.x; i < nu; i += blockDim.x) {
+            double r = 0.0;
+            for (int j = 0; j < nu; j++) r += fabs(Ab[i + (i64)ld * j]);
+            mx = fmax(mx, r);
+        }
+    }
+    sh[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+        __syncthreads();
+    }
+    const double pad = 2.0 * sh[0] + 1.0;
+    for (int b = 0; b < nb; b++) {
+        const int nu = b == 0 ? n0 : n1;
+        double *Ab = A + b * sA;
+        for (int i = nu + threadIdx.x; i < m; i += blockDim.x) Ab[i + (i64)ld * i] = pad;
+    }
+}
 static double *parts(Stage2 *p, i64 nblocks) {
     size_t need = sizeof(double) * 4 * (size_t)nblocks;
     if (p->part_cap < need) {
@@ -1163,37 +1233,91 @@ static int qd_reduce(Stage2 *p, const QDArgs &q, i64 nblocks, int cls) {
     return 0;
 }
 
-// PC metric solve (see oracle pc_solve): returns coef (N x nnew, col-major, ld N) and w (nnew) in named buffers
-static int pc_solve(Stage2 *p, const char *tag, int N, double *S, double *K, double thr, double **coef_out,
-                    double **w_out, int *nnew_out, double *minw) {
-    std::string t(tag);
-    GET(w1, (t + "_w1").c_str(), N);
-    TRYS(syevd(p, true, N, S, w1));
-    std::vector<double> hw(N);
-    NV2_HIP(hipMemcpyAsync(hw.data(), w1, sizeof(double) * N, hipMemcpyDeviceToHost, p->st));
-    NV2_HIP(hipStreamSynchronize(p->st));
-    int nnew = 0;
-    for (int j = 0; j < N; j++) if (hw[j] >= thr) nnew++;
-    const int first = N - nnew;
-    if (minw) *minw = N ? hw[0] : 0.0;
-    *nnew_out = nnew;
-    GET(coef, (t + "_coef").c_str(), (i64)N * (nnew ? nnew : 1));
-    GET(w2, (t + "_w2").c_str(), nnew ? nnew : 1);
-    *coef_out = coef;
-    *w_out = w2;
-    if (nnew == 0) return 0;
-    GET(caux, "pc_caux", (i64)N * nnew);
-    hipLaunchKernelGGL(k_caux, dim3(nb((i64)N * nnew, 256)), dim3(256), 0, p->st, S, w1, N, first, nnew, caux);
+// One or two PC metric problems of equal size N (qdnevpt.F:4744-4769 and e.g. :1974-2037),
+// solved in two launches on worker stream w: the metric eigen-decompositions (one batched
+// call), then, after the host has counted the kept eigenvalues of each, the transformed
+// Koopmans matrices K' and their eigen-decompositions.  The second problem's K' is padded
+// to the larger kept dimension with a decoupled diagonal block far above its spectrum, so
+// both still go through one batched call; the padding never reaches the coefficients.
+struct PcJob {
+    std::string tag;
+    int N = 0, nb = 1, w = 0, slot = 0;        // info words at 4*slot + {metric b, K' 2+b}
+    double *S = nullptr, *K = nullptr;         // nb consecutive N x N matrices; S is overwritten
+    double *w1 = nullptr, *hw1 = nullptr, *coef = nullptr, *w2 = nullptr;
+    double *caux = nullptr, *Ks = nullptr, *T = nullptr, *Kp = nullptr;   // nb x N x N scratch each
+    int nnew[2] = {0, 0}, nmax = 0;
+    double minw[2] = {0.0, 0.0};
+    i64 NN() const { return (i64)N * N; }
+    double *Sb(int b) const { return S + b * NN(); }
+    double *Kb(int b) const { return K + b * NN(); }
+    double *coefb(int b) const { return coef + b * NN(); }
+    double *w2b(int b) const { return w2 + (i64)b * N; }
+};
+// every buffer a job touches is allocated here, on the calling thread, before the workers start
+static int pc_job_buffers(Stage2 *p, PcJob &j, size_t hoff) {
+    const i64 N = j.N, NN = N * N * j.nb;
+    GET(w1, (j.tag + "_w1").c_str(), N * j.nb); GET(w2, (j.tag + "_w2").c_str(), N * j.nb);
+    GET(coef, (j.tag + "_coef").c_str(), NN); GET(caux, (j.tag + "_caux").c_str(), NN);
+    GET(Ks, (j.tag + "_Ks").c_str(), NN); GET(T, (j.tag + "_T").c_str(), NN); GET(Kp, (j.tag + "_Kp").c_str(), NN);
+    j.w1 = w1; j.w2 = w2; j.coef = coef; j.caux = caux; j.Ks = Ks; j.T = T; j.Kp = Kp;
+    j.hw1 = p->hpin + hoff;
+    for (int k = 0; k < 4; k++) p->hinfo[4 * j.slot + k] = 0;
+    return 0;
+}
+static int pc_metric_launch(Stage2 *p, PcJob &j) {
+    TRYS(syevd_async(p, j.w, 4 * j.slot, true, j.N, j.S, j.NN(), j.w1, j.N, j.nb));
+    if (j.N) NV2_HIP(hipMemcpyAsync(j.hw1, j.w1, sizeof(double) * j.N * j.nb, hipMemcpyDeviceToHost, p->ws[j.w]));
+    return 0;
+}
+// after the worker stream has finished: threshold the metric spectra, launch K' and its solves
+static int pc_kprime_launch(Stage2 *p, PcJob &j, double thr) {
+    const int N = j.N;
+    j.nmax = 0;
+    for (int b = 0; b < j.nb; b++) {
+        const int inf = p->hinfo[4 * j.slot + b];
+        if (inf != 0) { set_error("syevd(N=%d, %s metric %d) failed, info=%d", N, j.tag.c_str(), b, inf); return -3; }
+        int nnew = 0;
+        for (int k = 0; k < N; k++) if (j.hw1[(i64)b * N + k] >= thr) nnew++;
+        j.nnew[b] = nnew;
+        j.minw[b] = N ? j.hw1[(i64)b * N] : 0.0;
+        if (nnew > j.nmax) j.nmax = nnew;
+    }
+    const int nmax = j.nmax;
+    if (nmax == 0) return 0;
+    hipStream_t st = p->ws[j.w];
+    rocblas_handle h = p->wh[j.w];
+    const i64 NN = j.NN();
+    NV2_HIP(hipMemsetAsync(j.caux, 0, sizeof(double) * NN * j.nb, st));
+    for (int b = 0; b < j.nb; b++) {
+        if (j.nnew[b])
+            hipLaunchKernelGGL(k_caux, dim3(nb((i64)N * j.nnew[b], 256)), dim3(256), 0, st, j.Sb(b), j.w1 + (i64)b * N,
+                               N, N - j.nnew[b], j.nnew[b], j.caux + b * NN);
+        hipLaunchKernelGGL(k_sym_upper, dim3(nb(NN, 256)), dim3(256), 0, st, j.Kb(b), N, j.Ks + b * NN);
+    }
     NV2_LAUNCH_CHECK();
-    GET(Ks, "pc_Ks", (i64)N * N);
-    hipLaunchKernelGGL(k_sym_upper, dim3(nb((i64)N * N, 256)), dim3(256), 0, p->st, K, N, Ks);
-    NV2_LAUNCH_CHECK();
-    GET(T, "pc_T", (i64)N * nnew);
-    TRYS(gemm(p, false, false, N, nnew, N, 1.0, Ks, N, caux, N, 0.0, T, N));
-    GET(Kp, "pc_Kp", (i64)nnew * nnew);
-    TRYS(gemm(p, true, false, nnew, nnew, N, 1.0, caux, N, T, N, 0.0, Kp, nnew));
-    TRYS(syevd(p, true, nnew, Kp, w2));
-    TRYS(gemm(p, false, false, N, nnew, nnew, 1.0, caux, N, Kp, nnew, 0.0, coef, N));
+    const double one = 1.0, zero = 0.0;
+    RB(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, N, nmax, N, &one, j.Ks, N, NN,
+                                     j.caux, N, NN, &zero, j.T, N, NN, j.nb));
+    RB(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, nmax, nmax, N, &one,
+                                     j.caux, N, NN, j.T, N, NN, &zero, j.Kp, nmax, NN, j.nb));
+    if (j.nb > 1 && (j.nnew[0] < nmax || j.nnew[1] < nmax)) {
+        hipLaunchKernelGGL(k_pad_gersh, dim3(1), dim3(256), 0, st, j.Kp, nmax, NN, j.nb, j.nnew[0], j.nnew[1], nmax);
+        NV2_LAUNCH_CHECK();
+    }
+    TRYS(syevd_async(p, j.w, 4 * j.slot + 2, true, nmax, j.Kp, NN, j.w2, N, j.nb));
+    RB(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, N, nmax, nmax, &one, j.caux, N,
+                                     NN, j.Kp, nmax, NN, &zero, j.coef, N, NN, j.nb));
+    return 0;
+}
+// one host thread per worker stream: the solver library blocks its calling thread at times,
+// so separate threads are what lets the chains of different problems overlap on the device
+static int pc_worker(Stage2 *p, PcJob *J, int njob, int w, double thr) {
+    for (int k = 0; k < njob; k++)
+        if (J[k].w == w) TRYS(pc_metric_launch(p, J[k]));
+    NV2_HIP(hipStreamSynchronize(p->ws[w]));
+    for (int k = 0; k < njob; k++)
+        if (J[k].w == w) TRYS(pc_kprime_launch(p, J[k], thr));
+    NV2_HIP(hipStreamSynchronize(p->ws[w]));
     return 0;
 }
 
@@ -1207,6 +1331,21 @@ Stage2 *stage2_create(const S2Inputs &in, rocblas_handle h, hipStream_t st) {
     if (hipMalloc(&p->dinfo, 2 * sizeof(int)) != hipSuccess) { delete p; return nullptr; }
     if (hipMalloc(&p->acc, sizeof(double) * 32) != hipSuccess) { delete p; return nullptr; }
     if (hipMalloc(&p->accq, sizeof(double) * 8 * 2 * QM) != hipSuccess) { delete p; return nullptr; }
+    if (hipMalloc(&p->dinfo_w, sizeof(int) * 64) != hipSuccess) { delete p; return nullptr; }
+    if (hipHostMalloc(&p->hinfo, sizeof(int) * 32, hipHostMallocDefault) != hipSuccess) { delete p; return nullptr; }
+    {
+        const i64 n = in.n;
+        p->hpin_cap = (size_t)(2 * n * n * n + 4 * n * n + 2 * n * (n + 1) + 2 * n + 64);
+        if (hipHostMalloc(&p->hpin, sizeof(double) * p->hpin_cap, hipHostMallocDefault) != hipSuccess) { delete p; return nullptr; }
+    }
+    for (int w = 0; w < NW; w++) {
+        if (hipStreamCreateWithFlags(&p->ws[w], hipStreamNonBlocking) != hipSuccess) { delete p; return nullptr; }
+        if (rocblas_create_handle(&p->wh[w]) != rocblas_status_success) { delete p; return nullptr; }
+        rocblas_set_stream(p->wh[w], p->ws[w]);
+        rocblas_set_pointer_mode(p->wh[w], rocblas_pointer_mode_host);
+    }
+    for (int e = 0; e < NW + 1; e++)
+        if (hipEventCreateWithFlags(&p->wev[e], hipEventDisableTiming) != hipSuccess) { delete p; return nullptr; }
     // pair tables: core i<=j, core i<j, virt r<=s, virt r<s, active upper pairs
     std::vector<int2> pc, pcs, pv, pvs, up;
     for (int i = 0; i < in.nc; i++)
@@ -1229,8 +1368,16 @@ Stage2 *stage2_create(const S2Inputs &in, rocblas_handle h, hipStream_t st) {
 void stage2_destroy(Stage2 *p) {
     if (!p) return;
     for (auto &kv : p->bufs) if (kv.second.p) (void)hipFree(kv.second.p);
+    (void)hipDeviceSynchronize();
     (void)hipFree(p->dinfo); (void)hipFree(p->part); (void)hipFree(p->acc); (void)hipFree(p->accq);
-    (void)hipFree(p->qpart);
+    (void)hipFree(p->qpart); (void)hipFree(p->dinfo_w);
+    if (p->hinfo) (void)hipHostFree(p->hinfo);
+    if (p->hpin) (void)hipHostFree(p->hpin);
+    for (int w = 0; w < NW; w++) {
+        if (p->wh[w]) rocblas_destroy_handle(p->wh[w]);
+        if (p->ws[w]) (void)hipStreamDestroy(p->ws[w]);
+    }
+    for (int e = 0; e < NW + 1; e++) if (p->wev[e]) (void)hipEventDestroy(p->wev[e]);
     (void)hipFree(p->pairs_c); (void)hipFree(p->pairs_cs); (void)hipFree(p->pairs_v); (void)hipFree(p->pairs_vs);
     (void)hipFree(p->up);
     delete p;
@@ -1424,7 +1571,110 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         }
     }
 
-    // ================================================================ class (0)
+    // ================================================================ PC metric problems
+    // The metric and Koopmans matrices of every PC class are filled on the main stream; their
+    // eigen-solves (chains of small latency-bound solver kernels) then run concurrently on the
+    // worker streams while the SC sums proceed on the main stream.
+    // batched pairs: (+1)/(-1) n, (+2)/(-2) n^2, (+2)/(-2) diagonal pairs n(n+1)/2, (0)' alone 2n^2,
+    // (+1)'/(-1)' n^3; problem b of a job: b = 0 for the "+" class, 1 for the "-" class
+    enum { JPM1, JPM2, JPM2P, J0P, JPM1P, NJOB };
+    PcJob J[NJOB];
+    const int m = n * (n + 1) / 2;
+    const bool has4 = p->in.amatIP && p->in.amatEA;
+    const i64 N3 = n2 * n;
+    {
+        const char *tags[NJOB] = {"pm1", "pm2", "pm2p", "v0p", "pm1p"};
+        const int Ns[NJOB] = {n, (int)n2, m, 2 * (int)n2, has4 ? (int)N3 : 0};
+        for (int k = 0; k < NJOB; k++) {
+            J[k].tag = tags[k];
+            J[k].N = Ns[k];
+            J[k].nb = k == J0P ? 1 : 2;
+            J[k].slot = k;
+            J[k].w = k == JPM1P ? 1 : 0;
+            GET(Sm, (J[k].tag + "_S").c_str(), (i64)Ns[k] * Ns[k] * J[k].nb);
+            GET(Km, (J[k].tag + "_K").c_str(), (i64)Ns[k] * Ns[k] * J[k].nb);
+            J[k].S = Sm;
+            J[k].K = Km;
+        }
+    }
+    // (+1): S = 2 - da, K = 1-EAKO/2 ; (-1): S = da, K = 1-IPKO/2
+    GET(ro_v1, "ro_v1", n2);
+    hipLaunchKernelGGL(k_fill_diag_minus, dim3(nb(n2, 256)), dim3(256), 0, st, da, n, 2.0, J[JPM1].Sb(0));
+    hipLaunchKernelGGL(k_fill_diag_minus, dim3(nb(n2, 256)), dim3(256), 0, st, da, n, 2.0, ro_v1);
+    hipLaunchKernelGGL(k_scale, dim3(nb(n2, 256)), dim3(256), 0, st, k_eako1, n2, 0.5, J[JPM1].Kb(0));
+    NV2_HIP(hipMemcpyAsync(J[JPM1].Sb(1), da, sizeof(double) * n2, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_scale, dim3(nb(n2, 256)), dim3(256), 0, st, k_ipko1, n2, 0.5, J[JPM1].Kb(1));
+    NV2_LAUNCH_CHECK();
+    // (+-2): fillcm2 / fillcm2p of (2-HRDM, 2-EAKO) and (2-RDM, 2-IPKO)
+    double *Xm2[2], *Xm2p[2];
+    for (int mode = 0; mode < 2; mode++) {
+        const double *X = mode == 0 ? k_hrdm : taa;
+        const double *K = mode == 0 ? k_eako2 : k_ipko2;
+        PcJob &a = J[JPM2], &b = J[JPM2P];
+        GET(xm, mode == 0 ? "pc_Xm_p2" : "pc_Xm_m2", n4);
+        GET(xmp, mode == 0 ? "pc_Xmp_p2" : "pc_Xmp_m2", (i64)m * m);
+        hipLaunchKernelGGL(k_fillcm2, dim3(nb(n4, 256)), dim3(256), 0, st, X, n, a.Sb(mode));
+        hipLaunchKernelGGL(k_fillcm2, dim3(nb(n4, 256)), dim3(256), 0, st, K, n, a.Kb(mode));
+        hipLaunchKernelGGL(k_fillcm2, dim3(nb(n4, 256)), dim3(256), 0, st, X, n, xm);
+        hipLaunchKernelGGL(k_fillcm2p, dim3(nb((i64)m * m, 256)), dim3(256), 0, st, X, n, p->up, b.Sb(mode));
+        hipLaunchKernelGGL(k_fillcm2p, dim3(nb((i64)m * m, 256)), dim3(256), 0, st, K, n, p->up, b.Kb(mode));
+        hipLaunchKernelGGL(k_fillcm2p, dim3(nb((i64)m * m, 256)), dim3(256), 0, st, X, n, p->up, xmp);
+        NV2_LAUNCH_CHECK();
+        Xm2[mode] = xm;
+        Xm2p[mode] = xmp;
+    }
+    // (0)': fillc0p
+    hipLaunchKernelGGL(k_fillc0p, dim3(nb(4 * n4, 256)), dim3(256), 0, st, S, k_ga, k_gd, J[J0P].S, J[J0P].K);
+    NV2_LAUNCH_CHECK();
+    // (+-1)': n^3 metric (eee2 / eee2t) and amat, with a copy of the metric kept for the sums
+    double *Mx[2] = {nullptr, nullptr}, *amk[2] = {nullptr, nullptr}, *bmk[2] = {nullptr, nullptr};
+    double *cmk[2] = {nullptr, nullptr}, *dmk[2] = {nullptr, nullptr};
+    if (has4) {
+        for (int plus = 0; plus < 2; plus++) {
+            PcJob &jb = J[JPM1P];
+            const int bb = plus ? 0 : 1;
+            const std::string t = plus ? "_p" : "_m";
+            GET(am, ("k_amat" + t).c_str(), n6); GET(bm, ("k_bmat" + t).c_str(), n4);
+            GET(cm, ("k_cmat" + t).c_str(), n4); GET(dm, ("k_dmat" + t).c_str(), n2);
+            TRYS(slice(p, plus ? p->in.amatEA : p->in.amatIP, s, n6, am));
+            TRYS(slice(p, plus ? p->in.bmatEA : p->in.bmatIP, s, n4, bm));
+            TRYS(slice(p, plus ? p->in.cmatEA : p->in.cmatIP, s, n4, cm));
+            TRYS(slice(p, plus ? p->in.dmatEA : p->in.dmatIP, s, n2, dm));
+            GET(mx, ("pm1_M" + t).c_str(), N3 * N3);
+            hipLaunchKernelGGL(k_fill_pm1p, dim3(nb(N3 * N3, 256)), dim3(256), 0, st, S, plus, am, jb.Sb(bb), jb.Kb(bb));
+            NV2_LAUNCH_CHECK();
+            NV2_HIP(hipMemcpyAsync(mx, jb.Sb(bb), sizeof(double) * N3 * N3, hipMemcpyDeviceToDevice, st));
+            Mx[plus] = mx; amk[plus] = am; bmk[plus] = bm; cmk[plus] = cm; dmk[plus] = dm;
+        }
+    }
+    // start the eigen-solves on the worker threads/streams once the fills are done
+    {
+        size_t hoff = 0;
+        int maxN[NW] = {};
+        for (int k = 0; k < NJOB; k++) {
+            TRYS(pc_job_buffers(p, J[k], hoff));
+            hoff += (size_t)J[k].N * J[k].nb;
+            if (J[k].N * J[k].nb > maxN[J[k].w]) maxN[J[k].w] = J[k].N * J[k].nb;
+        }
+        for (int w = 0; w < NW; w++) {
+            GET(E, ("syevd_E" + std::to_string(w)).c_str(), maxN[w] ? maxN[w] : 1);
+            p->wE[w] = E;
+        }
+    }
+    NV2_HIP(hipEventRecord(p->wev[NW], st));
+    for (int w = 0; w < NW; w++) NV2_HIP(hipStreamWaitEvent(p->ws[w], p->wev[NW], 0));
+    int wrc[NW] = {};
+    std::vector<std::thread> workers;
+    struct Joiner {     // every exit from here on waits for the worker threads first
+        std::vector<std::thread> &v;
+        ~Joiner() { for (auto &t : v) if (t.joinable()) t.join(); }
+    } joiner{workers};
+    for (int w = 0; w < NW; w++)
+        workers.emplace_back([p, &J, w, thr, &wrc]() { wrc[w] = pc_worker(p, J, NJOB, w, thr); });
+    auto join_workers = [&]() { for (auto &t : workers) if (t.joinable()) t.join(); };
+
+    // ================================================================ SC sums that need no PC solve (main stream)
+    // class (0): SC == PC
     if (nc && nv) {
         dim3 g(nb(nv, 128), (unsigned)p->np_c);
         double *pt = parts(p, (i64)g.x * g.y);
@@ -1432,19 +1682,136 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         NV2_LAUNCH_CHECK();
         TRYS(reduce_into(p, (i64)g.x * g.y, 0));
     }
-    // ================================================================ class (+1)
+    // (+2)/(-2) SC
+    for (int mode = 0; mode < 2; mode++) {
+        const int cls = mode == 0 ? 3 : 4;
+        const double *X = mode == 0 ? k_hrdm : taa;
+        const double *K = mode == 0 ? k_eako2 : k_ipko2;
+        const double *I = mode == 0 ? Icaca : Iavav;
+        const i64 ld = mode == 0 ? Nca : Nav;
+        const int next = mode == 0 ? nc : nv;
+        const double *eo = mode == 0 ? ec : ev;
+        const int2 *pr = mode == 0 ? p->pairs_c : p->pairs_v;
+        const i64 npr = mode == 0 ? p->np_c : p->np_v;
+        for (int j = 0; q.M && j < M; j++) {
+            if (j == s) continue;
+            QFILL(xj, mode == 0 ? QK_P2_X : QK_M2_X, j, n4, "qd_pm2x_");
+            QFILL(xsj, mode == 0 ? QK_P2_XS : QK_M2_XS, j, n4, "qd_pm2xs_");
+            q.a[j] = xj; q.b[j] = xsj;
+        }
+        if (next && npr) {
+            double *pt = parts(p, npr);
+            TRYS(qd_prepare(p, q, npr));
+            hipLaunchKernelGGL(k_pm2_sc, dim3((unsigned)npr), dim3(256), 0, st, I, ld, pr, mode, nc, n, X, K, eo, pt, q);
+            NV2_LAUNCH_CHECK();
+            TRYS(reduce_into(p, npr, cls));
+            TRYS(qd_reduce(p, q, npr, cls));
+        }
+    }
+    // (0)' SC
+    GET(X1, "x1", n4); GET(X2, "x2", n4); GET(G1, "g1", n4); GET(G2, "g2", n4);
+    hipLaunchKernelGGL(k_v0p_prep, dim3(nb(n4, 256)), dim3(256), 0, st, S, k_ga, k_gd, X1, X2, G1, G2);
+    NV2_LAUNCH_CHECK();
+    for (int j = 0; q.M && j < M; j++) {
+        if (j == s) continue;
+        QFILL(x1j, QK_0P_X1, j, n4, "qd_0px1_");
+        QFILL(x2j, QK_0P_X2, j, n4, "qd_0px2_");
+        QFILL(dsj, QK_0P_DS, j, n2, "qd_0pds_");
+        q.a[j] = x1j; q.b[j] = x2j; q.c[j] = dsj;
+    }
+    if (nc && nv) {
+        const i64 nbk = (i64)nc * nv;
+        double *pt = parts(p, nbk);
+        TRYS(qd_prepare(p, q, nbk));
+        hipLaunchKernelGGL(k_v0p_sc, dim3((unsigned)nbk), dim3(256), 0, st, Icvaa, Ncv, Icaav, Nca, nc, n, nv, X1,
+                           X2, G1, G2, da, k_gf, h0p, ec, ev, pt, q);
+        NV2_LAUNCH_CHECK();
+        TRYS(reduce_into(p, nbk, 7));
+        TRYS(qd_reduce(p, q, nbk, 7));
+    }
+    // (-1)' and (+1)' SC
+    double *Vpm[2] = {nullptr, nullptr}, *Hpm[2] = {nullptr, nullptr}, *Q2pm[2] = {nullptr, nullptr};
+    for (int plus = 0; has4 && plus < 2; plus++) {
+        const int cls = plus ? 5 : 6;
+        const int next = plus ? nc : nv;
+        const double *eo = plus ? ec : ev;
+        const std::string t = plus ? "_p" : "_m";
+        PcJob &jb = J[JPM1P];
+        const int bb = plus ? 0 : 1;
+        GET(Q2, ("pm1_Q2" + t).c_str(), N3 * n); GET(BC, ("pm1_BC" + t).c_str(), N3 * n);
+        hipLaunchKernelGGL(k_fill_pm1p_q, dim3(nb(N3 * n, 256)), dim3(256), 0, st, S, plus, bmk[plus], cmk[plus], Q2, BC);
+        NV2_LAUNCH_CHECK();
+        Q2pm[plus] = Q2;
+        if (!next) continue;
+        GET(V, ("pm1_V" + t).c_str(), N3 * next);
+        hipLaunchKernelGGL(k_fill_pm1p_v, dim3(nb(N3 * next, 256)), dim3(256), 0, st, plus ? Iaaca : Iaaav, plus, n, nc,
+                           next, V);
+        NV2_LAUNCH_CHECK();
+        double *H = hm1p;
+        if (plus) {   // H[a,i] = heffp1p[i,a]
+            GET(Hp, "pm1_H_p", (i64)n * next);
+            hipLaunchKernelGGL(k_transpose, dim3(nb((i64)n * nc, 256)), dim3(256), 0, st, hp1p, nc, n, Hp);
+            NV2_LAUNCH_CHECK();
+            H = Hp;
+        }
+        Vpm[plus] = V;
+        Hpm[plus] = H;
+        GET(ro1, ("pm1_ro" + t).c_str(), n2);
+        hipLaunchKernelGGL(k_fill_diag_minus, dim3(nb(n2, 256)), dim3(256), 0, st, da, n, plus ? 2.0 : 0.0, ro1);
+        NV2_LAUNCH_CHECK();
+        GET(W1, ("pm1_W1" + t).c_str(), (i64)n * next); GET(W2, ("pm1_W2" + t).c_str(), (i64)n * next);
+        GET(Z1, ("pm1_Z1" + t).c_str(), N3 * next); GET(Z2, ("pm1_Z2" + t).c_str(), N3 * next);
+        GET(Y1, ("pm1_Y1" + t).c_str(), N3 * next); GET(Y2, ("pm1_Y2" + t).c_str(), N3 * next);
+        TRYS(gemm(p, false, false, n, next, n, 1.0, ro1, n, H, n, 0.0, W1, n));
+        TRYS(gemm(p, false, false, n, next, n, 1.0, dmk[plus], n, H, n, 0.0, W2, n));
+        TRYS(gemm(p, false, false, (int)N3, next, n, 1.0, Q2, N3, H, n, 0.0, Z1, N3));
+        TRYS(gemm(p, false, false, (int)N3, next, n, 1.0, BC, N3, H, n, 0.0, Z2, N3));
+        TRYS(gemm(p, false, false, (int)N3, next, (int)N3, 1.0, Mx[plus], N3, V, N3, 0.0, Y1, N3));   // M^T v
+        TRYS(gemm(p, false, false, (int)N3, next, (int)N3, 1.0, jb.Kb(bb), N3, V, N3, 0.0, Y2, N3));  // Am^T v
+        for (int j = 0; q.M && j < M; j++) {
+            if (j == s) continue;
+            QFILL(r1j, plus ? QK_P1P_R1 : QK_M1P_R1, j, n2, "qd_pm1r1_");
+            QFILL(qj, plus ? QK_P1P_QSC : QK_M1P_QSC, j, N3 * n, "qd_pm1q_");
+            double *mj = qd_fill(p, Sj[j], plus ? QK_P1P_MSC : QK_M1P_MSC, j < s, N3 * N3, "qd_pm1M");
+            if (!mj) return -1;
+            QGET(w1j, "qd_pm1w1_", j, (i64)n * next); QGET(z1j, "qd_pm1z1_", j, N3 * next);
+            QGET(y1j, "qd_pm1y1_", j, N3 * next);
+            TRYS(gemm(p, false, false, n, next, n, 1.0, r1j, n, H, n, 0.0, w1j, n));
+            TRYS(gemm(p, false, false, (int)N3, next, n, 1.0, qj, N3, H, n, 0.0, z1j, N3));
+            TRYS(gemm(p, false, false, (int)N3, next, (int)N3, 1.0, mj, N3, V, N3, 0.0, y1j, N3));
+            q.a[j] = w1j; q.b[j] = z1j; q.c[j] = y1j;
+        }
+        double *pt = parts(p, next);
+        TRYS(qd_prepare(p, q, next));
+        hipLaunchKernelGGL(k_pm1p_sc, dim3(next), dim3(256), 0, st, plus, n, next, H, W1, W2, V, Z1, Z2, Y1, Y2, eo,
+                           pt, q);
+        NV2_LAUNCH_CHECK();
+        TRYS(reduce_into(p, next, cls));
+        TRYS(qd_reduce(p, q, next, cls));
+    }
+
+    // ================================================================ join the PC solves
+    join_workers();
+    for (int w = 0; w < NW; w++) if (wrc[w]) return wrc[w];
+    for (int w = 0; w < NW; w++) {
+        NV2_HIP(hipEventRecord(p->wev[w], p->ws[w]));
+        NV2_HIP(hipStreamWaitEvent(st, p->wev[w], 0));
+    }
+    const int cls_of[NJOB][2] = {{1, 2}, {3, 4}, {3, 4}, {7, 7}, {5, 6}};
+    for (int k = 0; k < NJOB; k++)
+        for (int b = 0; b < J[k].nb; b++) {
+            const int c = cls_of[k][b];
+            if (k == JPM2P) { if (J[k].minw[b] < sd.min_metric_eig[c]) sd.min_metric_eig[c] = J[k].minw[b]; }
+            else { sd.min_metric_eig[c] = J[k].minw[b]; sd.nnew[c] = J[k].nnew[b]; }
+        }
+
+    // ================================================================ class (+1), SC and PC (fused kernel)
     {
-        GET(Sm, "pc_S", n2); GET(Km, "pc_K", n2); GET(ro, "ro_v1", n2);
-        hipLaunchKernelGGL(k_fill_diag_minus, dim3(nb(n2, 256)), dim3(256), 0, st, da, n, 2.0, Sm);
-        NV2_LAUNCH_CHECK();
-        NV2_HIP(hipMemcpyAsync(ro, Sm, sizeof(double) * n2, hipMemcpyDeviceToDevice, st));
-        hipLaunchKernelGGL(k_scale, dim3(nb(n2, 256)), dim3(256), 0, st, k_eako1, n2, 0.5, Km);
-        NV2_LAUNCH_CHECK();
-        double *coef, *w; int nnew;
-        TRYS(pc_solve(p, "v1", n, Sm, Km, thr, &coef, &w, &nnew, &sd.min_metric_eig[1]));
-        sd.nnew[1] = nnew;
+        const PcJob &jb = J[JPM1];
+        const int nnew = jb.nnew[0];
+        double *coef = jb.coefb(0), *w = jb.w2b(0);
         GET(spc, "spc_v1", (i64)n * (nnew ? nnew : 1));
-        TRYS(gemm(p, true, false, n, nnew, n, 1.0, ro, n, coef, n, 0.0, spc, n));
+        TRYS(gemm(p, true, false, n, nnew, n, 1.0, ro_v1, n, coef, n, 0.0, spc, n));
         for (int j = 0; q.M && j < M; j++) {
             if (j == s) continue;
             QFILL(roj, QK_V1_RO, j, n2, "qd_v1ro_");
@@ -1457,7 +1824,7 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             dim3 g(nb(nv, 64), (unsigned)p->np_c);
             double *pt = parts(p, (i64)g.x * g.y);
             TRYS(qd_prepare(p, q, (i64)g.x * g.y));
-#define V1(NM) hipLaunchKernelGGL(k_v1<NM>, g, dim3(64), 0, st, Icacv, Nca, p->pairs_c, nc, n, nv, ec, ev, k_eako1, ro, spc, w, nnew, pt, q)
+#define V1(NM) hipLaunchKernelGGL(k_v1<NM>, g, dim3(64), 0, st, Icacv, Nca, p->pairs_c, nc, n, nv, ec, ev, k_eako1, ro_v1, spc, w, nnew, pt, q)
             if (n <= 8) V1(8); else V1(16);
 #undef V1
             NV2_LAUNCH_CHECK();
@@ -1465,15 +1832,11 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             TRYS(qd_reduce(p, q, (i64)g.x * g.y, 1));
         }
     }
-    // ================================================================ class (-1)
+    // ================================================================ class (-1), SC and PC (fused kernel)
     {
-        GET(Sm, "pc_S", n2); GET(Km, "pc_K", n2);
-        NV2_HIP(hipMemcpyAsync(Sm, da, sizeof(double) * n2, hipMemcpyDeviceToDevice, st));
-        hipLaunchKernelGGL(k_scale, dim3(nb(n2, 256)), dim3(256), 0, st, k_ipko1, n2, 0.5, Km);
-        NV2_LAUNCH_CHECK();
-        double *coef, *w; int nnew;
-        TRYS(pc_solve(p, "vm1", n, Sm, Km, thr, &coef, &w, &nnew, &sd.min_metric_eig[2]));
-        sd.nnew[2] = nnew;
+        const PcJob &jb = J[JPM1];
+        const int nnew = jb.nnew[1];
+        double *coef = jb.coefb(1), *w = jb.w2b(1);
         GET(spc, "spc_vm1", (i64)n * (nnew ? nnew : 1));
         TRYS(gemm(p, true, false, n, nnew, n, 1.0, da, n, coef, n, 0.0, spc, n));
         for (int j = 0; q.M && j < M; j++) {
@@ -1496,119 +1859,69 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             TRYS(qd_reduce(p, q, nb(tot, 128), 2));
         }
     }
-    // ================================================================ classes (+2) and (-2)
+    // ================================================================ classes (+2) and (-2), PC
     for (int mode = 0; mode < 2; mode++) {
         const int cls = mode == 0 ? 3 : 4;
-        const double *X = mode == 0 ? k_hrdm : taa;
-        const double *K = mode == 0 ? k_eako2 : k_ipko2;
         const double *I = mode == 0 ? Icaca : Iavav;
         const i64 ld = mode == 0 ? Nca : Nav;
         const int next = mode == 0 ? nc : nv;
         const double *eo = mode == 0 ? ec : ev;
-        const int2 *pr = mode == 0 ? p->pairs_c : p->pairs_v;
-        const i64 npr = mode == 0 ? p->np_c : p->np_v;
         const int2 *prs = mode == 0 ? p->pairs_cs : p->pairs_vs;
         const i64 nprs = mode == 0 ? p->np_cs : p->np_vs;
-        for (int j = 0; q.M && j < M; j++) {
-            if (j == s) continue;
-            QFILL(xj, mode == 0 ? QK_P2_X : QK_M2_X, j, n4, "qd_pm2x_");
-            QFILL(xsj, mode == 0 ? QK_P2_XS : QK_M2_XS, j, n4, "qd_pm2xs_");
-            q.a[j] = xj; q.b[j] = xsj;
+        {   // off-diagonal external pairs: n^2 metric
+            const PcJob &jb = J[JPM2];
+            const int nnew = jb.nnew[mode];
+            double *coef = jb.coefb(mode), *w2 = jb.w2b(mode);
+            GET(spc, "spc_pm2", n2 * (nnew ? nnew : 1));
+            TRYS(gemm(p, true, false, (int)n2, nnew, (int)n2, 1.0, Xm2[mode], n2, coef, n2, 0.0, spc, n2));
+            for (int j = 0; q.M && j < M; j++) {
+                if (j == s) continue;
+                QFILL(yj, mode == 0 ? QK_P2_Y : QK_M2_Y, j, n4, "qd_pm2y_");
+                QGET(sj, "qd_pm2s_", j, n2 * (nnew ? nnew : 1));
+                TRYS(gemm(p, true, false, (int)n2, nnew, (int)n2, 1.0, yj, n2, coef, n2, 0.0, sj, n2));
+                q.a[j] = sj;
+            }
+            if (nprs && nnew) {
+                const i64 tot = nprs * nnew;
+                double *pt = parts(p, nb(tot, 128));
+                TRYS(qd_prepare(p, q, nb(tot, 128)));
+                hipLaunchKernelGGL(k_pm2_pc, dim3(nb(tot, 128)), dim3(128), 0, st, I, ld, prs, nprs, mode, nc, n, spc,
+                                   w2, nnew, eo, pt, q);
+                NV2_LAUNCH_CHECK();
+                TRYS(reduce_into(p, nb(tot, 128), cls));
+                TRYS(qd_reduce(p, q, nb(tot, 128), cls));
+            }
         }
-        if (next && npr) {
-            double *pt = parts(p, npr);
-            TRYS(qd_prepare(p, q, npr));
-            hipLaunchKernelGGL(k_pm2_sc, dim3((unsigned)npr), dim3(256), 0, st, I, ld, pr, mode, nc, n, X, K, eo, pt, q);
-            NV2_LAUNCH_CHECK();
-            TRYS(reduce_into(p, npr, cls));
-            TRYS(qd_reduce(p, q, npr, cls));
-        }
-        // PC, off-diagonal pairs: n^2 metric
-        GET(Sm, "pc_S", n4); GET(Km, "pc_K", n4); GET(Xm, "pc_X", n4);
-        hipLaunchKernelGGL(k_fillcm2, dim3(nb(n4, 256)), dim3(256), 0, st, X, n, Sm);
-        hipLaunchKernelGGL(k_fillcm2, dim3(nb(n4, 256)), dim3(256), 0, st, K, n, Km);
-        hipLaunchKernelGGL(k_fillcm2, dim3(nb(n4, 256)), dim3(256), 0, st, X, n, Xm);
-        NV2_LAUNCH_CHECK();
-        double *coef, *w; int nnew;
-        TRYS(pc_solve(p, mode == 0 ? "v2" : "vm2", (int)n2, Sm, Km, thr, &coef, &w, &nnew, &sd.min_metric_eig[cls]));
-        sd.nnew[cls] = nnew;
-        GET(spc, "spc_pm2", n2 * (nnew ? nnew : 1));
-        TRYS(gemm(p, true, false, (int)n2, nnew, (int)n2, 1.0, Xm, n2, coef, n2, 0.0, spc, n2));
-        for (int j = 0; q.M && j < M; j++) {
-            if (j == s) continue;
-            QFILL(yj, mode == 0 ? QK_P2_Y : QK_M2_Y, j, n4, "qd_pm2y_");
-            QGET(sj, "qd_pm2s_", j, n2 * (nnew ? nnew : 1));
-            TRYS(gemm(p, true, false, (int)n2, nnew, (int)n2, 1.0, yj, n2, coef, n2, 0.0, sj, n2));
-            q.a[j] = sj;
-        }
-        if (nprs && nnew) {
-            const i64 tot = nprs * nnew;
-            double *pt = parts(p, nb(tot, 128));
-            TRYS(qd_prepare(p, q, nb(tot, 128)));
-            hipLaunchKernelGGL(k_pm2_pc, dim3(nb(tot, 128)), dim3(128), 0, st, I, ld, prs, nprs, mode, nc, n, spc, w,
-                               nnew, eo, pt, q);
-            NV2_LAUNCH_CHECK();
-            TRYS(reduce_into(p, nb(tot, 128), cls));
-            TRYS(qd_reduce(p, q, nb(tot, 128), cls));
-        }
-        // PC, diagonal pairs: n(n+1)/2 metric
-        const int m = n * (n + 1) / 2;
-        hipLaunchKernelGGL(k_fillcm2p, dim3(nb((i64)m * m, 256)), dim3(256), 0, st, X, n, p->up, Sm);
-        hipLaunchKernelGGL(k_fillcm2p, dim3(nb((i64)m * m, 256)), dim3(256), 0, st, K, n, p->up, Km);
-        hipLaunchKernelGGL(k_fillcm2p, dim3(nb((i64)m * m, 256)), dim3(256), 0, st, X, n, p->up, Xm);
-        NV2_LAUNCH_CHECK();
-        double mw2;
-        TRYS(pc_solve(p, mode == 0 ? "v2p" : "vm2p", m, Sm, Km, thr, &coef, &w, &nnew, &mw2));
-        if (mw2 < sd.min_metric_eig[cls]) sd.min_metric_eig[cls] = mw2;
-        GET(spc2, "spc_pm2p", (i64)m * (nnew ? nnew : 1));
-        TRYS(gemm(p, true, false, m, nnew, m, 0.5, Xm, m, coef, m, 0.0, spc2, m));
-        for (int j = 0; q.M && j < M; j++) {
-            if (j == s) continue;
-            QFILL(yj, mode == 0 ? QK_P2P_Y : QK_M2P_Y, j, (i64)m * m, "qd_pm2py_");
-            QGET(sj, "qd_pm2ps_", j, (i64)m * (nnew ? nnew : 1));
-            TRYS(gemm(p, true, false, m, nnew, m, 1.0, yj, m, coef, m, 0.0, sj, m));
-            q.a[j] = sj;
-        }
-        if (next && nnew) {
-            const i64 tot = (i64)next * nnew;
-            double *pt = parts(p, nb(tot, 128));
-            TRYS(qd_prepare(p, q, nb(tot, 128)));
-            hipLaunchKernelGGL(k_pm2p_pc, dim3(nb(tot, 128)), dim3(128), 0, st, I, ld, next, mode, nc, n, spc2, w,
-                               nnew, eo, pt, q);
-            NV2_LAUNCH_CHECK();
-            TRYS(reduce_into(p, nb(tot, 128), cls));
-            TRYS(qd_reduce(p, q, nb(tot, 128), cls));
+        {   // diagonal external pairs: n(n+1)/2 metric
+            const PcJob &jb = J[JPM2P];
+            const int nnew = jb.nnew[mode];
+            double *coef = jb.coefb(mode), *w2 = jb.w2b(mode);
+            GET(spc2, "spc_pm2p", (i64)m * (nnew ? nnew : 1));
+            TRYS(gemm(p, true, false, m, nnew, m, 0.5, Xm2p[mode], m, coef, m, 0.0, spc2, m));
+            for (int j = 0; q.M && j < M; j++) {
+                if (j == s) continue;
+                QFILL(yj, mode == 0 ? QK_P2P_Y : QK_M2P_Y, j, (i64)m * m, "qd_pm2py_");
+                QGET(sj, "qd_pm2ps_", j, (i64)m * (nnew ? nnew : 1));
+                TRYS(gemm(p, true, false, m, nnew, m, 1.0, yj, m, coef, m, 0.0, sj, m));
+                q.a[j] = sj;
+            }
+            if (next && nnew) {
+                const i64 tot = (i64)next * nnew;
+                double *pt = parts(p, nb(tot, 128));
+                TRYS(qd_prepare(p, q, nb(tot, 128)));
+                hipLaunchKernelGGL(k_pm2p_pc, dim3(nb(tot, 128)), dim3(128), 0, st, I, ld, next, mode, nc, n, spc2,
+                                   w2, nnew, eo, pt, q);
+                NV2_LAUNCH_CHECK();
+                TRYS(reduce_into(p, nb(tot, 128), cls));
+                TRYS(qd_reduce(p, q, nb(tot, 128), cls));
+            }
         }
     }
-    // ================================================================ class (0)'
+    // ================================================================ class (0)', PC
     {
-        GET(X1, "x1", n4); GET(X2, "x2", n4); GET(G1, "g1", n4); GET(G2, "g2", n4);
-        hipLaunchKernelGGL(k_v0p_prep, dim3(nb(n4, 256)), dim3(256), 0, st, S, k_ga, k_gd, X1, X2, G1, G2);
-        NV2_LAUNCH_CHECK();
-        for (int j = 0; q.M && j < M; j++) {
-            if (j == s) continue;
-            QFILL(x1j, QK_0P_X1, j, n4, "qd_0px1_");
-            QFILL(x2j, QK_0P_X2, j, n4, "qd_0px2_");
-            QFILL(dsj, QK_0P_DS, j, n2, "qd_0pds_");
-            q.a[j] = x1j; q.b[j] = x2j; q.c[j] = dsj;
-        }
-        if (nc && nv) {
-            const i64 nbk = (i64)nc * nv;
-            double *pt = parts(p, nbk);
-            TRYS(qd_prepare(p, q, nbk));
-            hipLaunchKernelGGL(k_v0p_sc, dim3((unsigned)nbk), dim3(256), 0, st, Icvaa, Ncv, Icaav, Nca, nc, n, nv, X1,
-                               X2, G1, G2, da, k_gf, h0p, ec, ev, pt, q);
-            NV2_LAUNCH_CHECK();
-            TRYS(reduce_into(p, nbk, 7));
-            TRYS(qd_reduce(p, q, nbk, 7));
-        }
-        const int N = 2 * (int)n2;
-        GET(Sm, "pc_S", (i64)N * N); GET(Km, "pc_K", (i64)N * N);
-        hipLaunchKernelGGL(k_fillc0p, dim3(nb((i64)N * N, 256)), dim3(256), 0, st, S, k_ga, k_gd, Sm, Km);
-        NV2_LAUNCH_CHECK();
-        double *coef, *w; int nnew;
-        TRYS(pc_solve(p, "v0p", N, Sm, Km, thr, &coef, &w, &nnew, &sd.min_metric_eig[7]));
-        sd.nnew[7] = nnew;
+        const PcJob &jb = J[J0P];
+        const int nnew = jb.nnew[0], N = jb.N;
+        double *coef = jb.coefb(0), *w = jb.w2b(0);
         if (nnew && nc && nv) {
             GET(V, "v0p_V", n4); GET(V2, "v0p_V2", n4);
             hipLaunchKernelGGL(k_v0p_vmats, dim3(nb(n4, 256)), dim3(256), 0, st, S, V, V2);
@@ -1645,108 +1958,54 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             TRYS(qd_reduce(p, q, nbk, 7));
         }
     }
-    // ================================================================ classes (-1)' and (+1)'
-    if (p->in.amatIP && p->in.amatEA) {
-        const i64 N = n2 * n;
-        for (int plus = 0; plus < 2; plus++) {
-            const int cls = plus ? 5 : 6;
-            const int next = plus ? nc : nv;
-            const double *eo = plus ? ec : ev;
-            GET(am, "k_amat", n6); GET(bm, "k_bmat", n4); GET(cm, "k_cmat", n4); GET(dm, "k_dmat", n2);
-            TRYS(slice(p, plus ? p->in.amatEA : p->in.amatIP, s, n6, am));
-            TRYS(slice(p, plus ? p->in.bmatEA : p->in.bmatIP, s, n4, bm));
-            TRYS(slice(p, plus ? p->in.cmatEA : p->in.cmatIP, s, n4, cm));
-            TRYS(slice(p, plus ? p->in.dmatEA : p->in.dmatIP, s, n2, dm));
-            GET(Spc, "pm1_S", N * N); GET(Kpc, "pm1_K", N * N); GET(Mx, "pm1_M", N * N);
-            hipLaunchKernelGGL(k_fill_pm1p, dim3(nb(N * N, 256)), dim3(256), 0, st, S, plus, am, Spc, Kpc);
-            NV2_LAUNCH_CHECK();
-            NV2_HIP(hipMemcpyAsync(Mx, Spc, sizeof(double) * N * N, hipMemcpyDeviceToDevice, st));   // keeps S for the sums
-            GET(Q2, "pm1_Q2", N * n); GET(BC, "pm1_BC", N * n);
-            hipLaunchKernelGGL(k_fill_pm1p_q, dim3(nb(N * n, 256)), dim3(256), 0, st, S, plus, bm, cm, Q2, BC);
-            NV2_LAUNCH_CHECK();
-            if (next) {
-                GET(V, "pm1_V", N * next);
-                hipLaunchKernelGGL(k_fill_pm1p_v, dim3(nb(N * next, 256)), dim3(256), 0, st, plus ? Iaaca : Iaaav,
-                                   plus, n, nc, next, V);
-                NV2_LAUNCH_CHECK();
-                const double *H = plus ? nullptr : hm1p;
-                GET(Hp, "pm1_H", (i64)n * next);
-                if (plus) {   // H[a,i] = heffp1p[i,a]
-                    hipLaunchKernelGGL(k_transpose, dim3(nb((i64)n * nc, 256)), dim3(256), 0, st, hp1p, nc, n, Hp);
-                    NV2_LAUNCH_CHECK();
-                    H = Hp;
-                }
-                GET(ro1, "pm1_ro", n2);
-                hipLaunchKernelGGL(k_fill_diag_minus, dim3(nb(n2, 256)), dim3(256), 0, st, da, n, plus ? 2.0 : 0.0, ro1);
-                NV2_LAUNCH_CHECK();
-                GET(W1, "pm1_W1", (i64)n * next); GET(W2, "pm1_W2", (i64)n * next);
-                GET(Z1, "pm1_Z1", N * next); GET(Z2, "pm1_Z2", N * next); GET(Y1, "pm1_Y1", N * next);
-                GET(Y2, "pm1_Y2", N * next);
-                TRYS(gemm(p, false, false, n, next, n, 1.0, ro1, n, H, n, 0.0, W1, n));
-                TRYS(gemm(p, false, false, n, next, n, 1.0, dm, n, H, n, 0.0, W2, n));
-                TRYS(gemm(p, false, false, (int)N, next, n, 1.0, Q2, N, H, n, 0.0, Z1, N));
-                TRYS(gemm(p, false, false, (int)N, next, n, 1.0, BC, N, H, n, 0.0, Z2, N));
-                TRYS(gemm(p, false, false, (int)N, next, (int)N, 1.0, Mx, N, V, N, 0.0, Y1, N));   // Spc V = M^T v
-                TRYS(gemm(p, false, false, (int)N, next, (int)N, 1.0, Kpc, N, V, N, 0.0, Y2, N));  // Kpc V = Am^T v
-                for (int j = 0; q.M && j < M; j++) {
-                    if (j == s) continue;
-                    QFILL(r1j, plus ? QK_P1P_R1 : QK_M1P_R1, j, n2, "qd_pm1r1_");
-                    QFILL(qj, plus ? QK_P1P_QSC : QK_M1P_QSC, j, N * n, "qd_pm1q_");
-                    double *mj = qd_fill(p, Sj[j], plus ? QK_P1P_MSC : QK_M1P_MSC, j < s, N * N, "qd_pm1M");
-                    if (!mj) return -1;
-                    QGET(w1j, "qd_pm1w1_", j, (i64)n * next); QGET(z1j, "qd_pm1z1_", j, N * next);
-                    QGET(y1j, "qd_pm1y1_", j, N * next);
-                    TRYS(gemm(p, false, false, n, next, n, 1.0, r1j, n, H, n, 0.0, w1j, n));
-                    TRYS(gemm(p, false, false, (int)N, next, n, 1.0, qj, N, H, n, 0.0, z1j, N));
-                    TRYS(gemm(p, false, false, (int)N, next, (int)N, 1.0, mj, N, V, N, 0.0, y1j, N));
-                    q.a[j] = w1j; q.b[j] = z1j; q.c[j] = y1j;
-                }
-                double *pt = parts(p, next);
-                TRYS(qd_prepare(p, q, next));
-                hipLaunchKernelGGL(k_pm1p_sc, dim3(next), dim3(256), 0, st, plus, n, next, H, W1, W2, V, Z1, Z2, Y1, Y2,
-                                   eo, pt, q);
-                NV2_LAUNCH_CHECK();
-                TRYS(reduce_into(p, next, cls));
-                TRYS(qd_reduce(p, q, next, cls));
-                // PC: metric solve at N = n^3, S-build, sums
-                double *coef, *w; int nnew;
-                TRYS(pc_solve(p, plus ? "vp1p" : "vm1p", (int)N, Spc, Kpc, thr, &coef, &w, &nnew, &sd.min_metric_eig[cls]));
-                sd.nnew[cls] = nnew;
-                if (nnew) {
-                    GET(SbT, "pm1_SbT", N * nnew); GET(spT, "pm1_spT", (i64)n * nnew); GET(R, "pm1_R", (i64)nnew * next);
-                    TRYS(gemm(p, false, false, (int)N, nnew, (int)N, 1.0, Mx, N, coef, N, 0.0, SbT, N));    // S-build
-                    TRYS(gemm(p, true, false, n, nnew, (int)N, 1.0, Q2, N, coef, N, 0.0, spT, n));
-                    TRYS(gemm(p, true, false, nnew, next, (int)N, 1.0, SbT, N, V, N, 0.0, R, nnew));
-                    TRYS(gemm(p, true, false, nnew, next, n, 1.0, spT, n, H, n, 1.0, R, nnew));
-                    for (int j = 0; q.M && j < M; j++) {
-                        if (j == s) continue;
-                        double *mj = qd_fill(p, Sj[j], plus ? QK_P1P_MPC : QK_M1P_MPC, j < s, N * N, "qd_pm1M");
-                        if (!mj) return -1;
-                        QFILL(qj, plus ? QK_P1P_QPC : QK_M1P_QPC, j, N * n, "qd_pm1qp_");
-                        GET(sbj, "qd_pm1SbT", N * nnew); GET(spj, "qd_pm1spT", (i64)n * nnew);
-                        QGET(rj, "qd_pm1R_", j, (i64)nnew * next);
-                        TRYS(gemm(p, false, false, (int)N, nnew, (int)N, 1.0, mj, N, coef, N, 0.0, sbj, N));
-                        TRYS(gemm(p, true, false, n, nnew, (int)N, 1.0, qj, N, coef, N, 0.0, spj, n));
-                        TRYS(gemm(p, true, false, nnew, next, (int)N, 1.0, sbj, N, V, N, 0.0, rj, nnew));
-                        TRYS(gemm(p, true, false, nnew, next, n, 1.0, spj, n, H, n, 1.0, rj, nnew));
-                        q.a[j] = rj;
-                    }
-                    const i64 tot = (i64)nnew * next;
-                    double *pt2 = parts(p, nb(tot, 256));
-                    TRYS(qd_prepare(p, q, nb(tot, 256)));
-                    hipLaunchKernelGGL(k_pm1p_pc, dim3(nb(tot, 256)), dim3(256), 0, st, plus, nnew, next, R, w, eo, pt2,
-                                       q);
-                    NV2_LAUNCH_CHECK();
-                    TRYS(reduce_into(p, nb(tot, 256), cls));
-                    TRYS(qd_reduce(p, q, nb(tot, 256), cls));
-                }
-            }
+    // ================================================================ classes (-1)' and (+1)', PC: S-build and sums
+    for (int plus = 0; has4 && plus < 2; plus++) {
+        const int cls = plus ? 5 : 6;
+        const int next = plus ? nc : nv;
+        const double *eo = plus ? ec : ev;
+        const PcJob &jb = J[JPM1P];
+        const int bb = plus ? 0 : 1;
+        const int nnew = jb.nnew[bb];
+        if (!next || !nnew) continue;
+        const std::string t = plus ? "_p" : "_m";
+        double *coef = jb.coefb(bb), *w = jb.w2b(bb), *V = Vpm[plus], *H = Hpm[plus], *Q2 = Q2pm[plus];
+        GET(SbT, "pm1_SbT", N3 * nnew); GET(spT, "pm1_spT", (i64)n * nnew); GET(R, ("pm1_R" + t).c_str(), (i64)nnew * next);
+        TRYS(gemm(p, false, false, (int)N3, nnew, (int)N3, 1.0, Mx[plus], N3, coef, N3, 0.0, SbT, N3));    // S-build
+        TRYS(gemm(p, true, false, n, nnew, (int)N3, 1.0, Q2, N3, coef, N3, 0.0, spT, n));
+        TRYS(gemm(p, true, false, nnew, next, (int)N3, 1.0, SbT, N3, V, N3, 0.0, R, nnew));
+        TRYS(gemm(p, true, false, nnew, next, n, 1.0, spT, n, H, n, 1.0, R, nnew));
+        for (int j = 0; q.M && j < M; j++) {
+            if (j == s) continue;
+            double *mj = qd_fill(p, Sj[j], plus ? QK_P1P_MPC : QK_M1P_MPC, j < s, N3 * N3, "qd_pm1M");
+            if (!mj) return -1;
+            QFILL(qj, plus ? QK_P1P_QPC : QK_M1P_QPC, j, N3 * n, "qd_pm1qp_");
+            GET(sbj, "qd_pm1SbT", N3 * nnew); GET(spj, "qd_pm1spT", (i64)n * nnew);
+            QGET(rj, "qd_pm1R_", j, (i64)nnew * next);
+            TRYS(gemm(p, false, false, (int)N3, nnew, (int)N3, 1.0, mj, N3, coef, N3, 0.0, sbj, N3));
+            TRYS(gemm(p, true, false, n, nnew, (int)N3, 1.0, qj, N3, coef, N3, 0.0, spj, n));
+            TRYS(gemm(p, true, false, nnew, next, (int)N3, 1.0, sbj, N3, V, N3, 0.0, rj, nnew));
+            TRYS(gemm(p, true, false, nnew, next, n, 1.0, spj, n, H, n, 1.0, rj, nnew));
+            q.a[j] = rj;
         }
+        const i64 tot = (i64)nnew * next;
+        double *pt2 = parts(p, nb(tot, 256));
+        TRYS(qd_prepare(p, q, nb(tot, 256)));
+        hipLaunchKernelGGL(k_pm1p_pc, dim3(nb(tot, 256)), dim3(256), 0, st, plus, nnew, next, R, w, eo, pt2, q);
+        NV2_LAUNCH_CHECK();
+        TRYS(reduce_into(p, nb(tot, 256), cls));
+        TRYS(qd_reduce(p, q, nb(tot, 256), cls));
     }
     // ---- results: (norm_sc, e_sc, norm_pc, e_pc) per class, SC==PC for class (0)
     double accs[32];
     NV2_HIP(hipMemcpyAsync(accs, p->acc, sizeof(accs), hipMemcpyDeviceToHost, st));
     NV2_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < NJOB; k++)
+        for (int b = 0; b < J[k].nb; b++)
+            if (J[k].nmax && p->hinfo[4 * J[k].slot + 2 + b] != 0) {
+                set_error("syevd(N=%d, %s Koopmans %d) failed, info=%d", J[k].nmax, J[k].tag.c_str(), b,
+                          p->hinfo[4 * J[k].slot + 2 + b]);
+                return -3;
+            }
     for (int k = 0; k < 8; k++) {
         out[4 * k + 0] = accs[4 * k + 1];
         out[4 * k + 1] = accs[4 * k + 0];
