@@ -1101,8 +1101,42 @@ static int gemm_on(rocblas_handle h, hipStream_t st, bool tA, bool tB, int m, in
                      (int)ldb, &beta, C, (int)ldc));
     return 0;
 }
+__global__ void k_splitk_reduce(const double *P, int ns, int m, int nn, double alpha, double beta, double *C, i64 ldc) {
+    const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (i64)m * nn) return;
+    const i64 i = e % m, j = e / m;
+    double acc = 0.0;
+    for (int c = 0; c < ns; c++) acc += P[(i64)c * m * nn + e];
+    const double old = beta == 0.0 ? 0.0 : beta * C[i + ldc * j];
+    C[i + ldc * j] = old + alpha * acc;
+}
+// GEMMs with a small output and a long reduction (the Fock and heff contractions over the
+// Cholesky index times an orbital index, k up to L*nc) run split over k as a strided batch,
+// so enough workgroups share the reduction; the slices are summed by k_splitk_reduce.
 static int gemm(Stage2 *p, bool tA, bool tB, int m, int nn, int k, double alpha, const double *A, i64 lda,
                 const double *B, i64 ldb, double beta, double *C, i64 ldc) {
+    if (m > 0 && nn > 0 && k >= 4096 && (i64)m * nn <= 512 * 512) {
+        int ns = k / 1024;
+        if (ns > 64) ns = 64;
+        const int kc = k / ns;          // the last slice takes the remainder below
+        const i64 mn = (i64)m * nn;
+        double *P = p->get("splitk", mn * (ns + 1));
+        if (!P) { set_error("hipMalloc failed for splitk"); return -1; }
+        const double one = 1.0, zero = 0.0;
+        const i64 sA = tA ? kc : kc * lda, sB = tB ? kc * ldb : kc;
+        RB(rocblas_dgemm_strided_batched(p->h, tA ? rocblas_operation_transpose : rocblas_operation_none,
+                                         tB ? rocblas_operation_transpose : rocblas_operation_none, m, nn, kc, &one, A,
+                                         (int)lda, sA, B, (int)ldb, sB, &zero, P, m, mn, ns));
+        int nsum = ns;
+        const int rem = k - kc * ns;
+        if (rem > 0) {
+            TRYS(gemm_on(p->h, p->st, tA, tB, m, nn, rem, 1.0, A + ns * sA, lda, B + ns * sB, ldb, 0.0, P + ns * mn, m));
+            nsum++;
+        }
+        hipLaunchKernelGGL(k_splitk_reduce, dim3(nb(mn, 256)), dim3(256), 0, p->st, P, nsum, m, nn, alpha, beta, C, ldc);
+        NV2_LAUNCH_CHECK();
+        return 0;
+    }
     return gemm_on(p->h, p->st, tA, tB, m, nn, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 static int gemm_sb(Stage2 *p, int m, int nn, int k, const double *A, i64 lda, i64 sA, const double *B, i64 ldb,
@@ -1441,107 +1475,6 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
     GET(da, "da", n2); GET(taa, "taa", n4); GET(ro3, "ro3", n6);
     TRYS(slice(p, p->in.dal, s, n2, da)); TRYS(slice(p, p->in.taa, s, n4, taa)); TRYS(slice(p, p->in.daaa, s, n6, ro3));
     Sl S{n, da, taa, ro3};
-    // ---- Fock operators and canonical orbitals (qdnevpt.F:369-698)
-    GET(Jc, "Jc", L); GET(Ja, "Ja", L);
-    hipLaunchKernelGGL(k_jvec, dim3(nb(L, 256)), dim3(256), 0, st, p->in.B, L, nc, n, da, Jc, Ja);
-    NV2_LAUNCH_CHECK();
-    double *hf = p->get("hfull", (i64)norb * norb);
-    GET(eps, "eps", norb);
-    NV2_HIP(hipMemsetAsync(eps, 0, sizeof(double) * norb, st));
-    GET(Cc, "Ccore", (i64)nc * nc);
-    GET(Cv, "Cvirt", (i64)nv * nv);
-    GET(Jt, "Jtot", L);
-    double *Bcc = p->get("Bcc", 0), *Bcv = p->get("Bcv", 0), *Bca = p->get("Bca", 0), *Bac = p->get("Bac", 0);
-    double *Bav = p->get("Bav", 0), *Baa = p->get("Baa", 0);
-    // Jt = 2 Jcore + Jact (Coulomb of both core and active densities)
-    {
-        const double one = 1.0, two = 2.0;
-        NV2_HIP(hipMemcpyAsync(Jt, Ja, sizeof(double) * L, hipMemcpyDeviceToDevice, st));
-        if (nc) RB(rocblas_daxpy(p->h, L, &two, Jc, 1, Jt, 1));
-        (void)one;
-    }
-    if (nc) {
-        NV2_HIP(hipMemcpy2DAsync(Cc, sizeof(double) * nc, hf, sizeof(double) * norb, sizeof(double) * nc, nc,
-                                 hipMemcpyDeviceToDevice, st));
-        hipLaunchKernelGGL(k_coul, dim3(nb((i64)nc * nc, 4)), dim3(256), 0, st, p->in.B, L, 0, nc, Jt, Cc);
-        NV2_LAUNCH_CHECK();
-        TRYS(gemm(p, true, false, nc, nc, L * nc, -1.0, Bcc, (i64)L * nc, Bcc, (i64)L * nc, 1.0, Cc, nc));
-        GET(G, "fockG", (i64)L * n * (nc > nv ? nc : nv));
-        TRYS(gemm_sb(p, L, n, n, Bac, L, (i64)L * n, da, n, 0, G, L, (i64)L * n, nc));
-        TRYS(gemm(p, true, false, nc, nc, L * n, -0.5, G, (i64)L * n, Bac, (i64)L * n, 1.0, Cc, nc));
-        GET(wc, "wcore", nc);
-        TRYS(syevd(p, false, nc, Cc, wc));
-        NV2_HIP(hipMemcpyAsync(eps, wc, sizeof(double) * nc, hipMemcpyDeviceToDevice, st));
-    }
-    {
-        NV2_HIP(hipMemcpy2DAsync(Cv, sizeof(double) * nv, hf + (i64)(nc + n) * norb + nc + n, sizeof(double) * norb,
-                                 sizeof(double) * nv, nv, hipMemcpyDeviceToDevice, st));
-        hipLaunchKernelGGL(k_coul, dim3(nb((i64)nv * nv, 4)), dim3(256), 0, st, p->in.B, L, nc + n, nv, Jt, Cv);
-        NV2_LAUNCH_CHECK();
-        if (nc) TRYS(gemm(p, true, false, nv, nv, L * nc, -1.0, Bcv, (i64)L * nc, Bcv, (i64)L * nc, 1.0, Cv, nv));
-        GET(G, "fockG", (i64)L * n * (nc > nv ? nc : nv));
-        TRYS(gemm_sb(p, L, n, n, Bav, L, (i64)L * n, da, n, 0, G, L, (i64)L * n, nv));
-        TRYS(gemm(p, true, false, nv, nv, L * n, -0.5, G, (i64)L * n, Bav, (i64)L * n, 1.0, Cv, nv));
-        GET(wv, "wvirt", nv);
-        TRYS(syevd(p, false, nv, Cv, wv));
-        NV2_HIP(hipMemcpyAsync(eps + nc + n, wv, sizeof(double) * nv, hipMemcpyDeviceToDevice, st));
-    }
-    const double *ec = eps, *ev = eps + nc + n;
-    // ---- Cholesky transform (qdnevpt.F:7599-7717)
-    GET(T1, "T1", (i64)L * nc * (nv > nc ? nv : nc));
-    GET(Tcc, "Tcc", (i64)L * nc * nc); GET(Tcv, "Tcv", (i64)L * nc * nv); GET(Tca, "Tca", (i64)L * nc * n);
-    GET(Tav, "Tav", (i64)L * n * nv);
-    if (nc) {
-        TRYS(gemm(p, false, false, L * nc, nc, nc, 1.0, Bcc, (i64)L * nc, Cc, nc, 0.0, T1, (i64)L * nc));
-        TRYS(gemm_sb(p, L, nc, nc, T1, L, (i64)L * nc, Cc, nc, 0, Tcc, L, (i64)L * nc, nc));
-        TRYS(gemm(p, false, false, L * nc, nv, nv, 1.0, Bcv, (i64)L * nc, Cv, nv, 0.0, T1, (i64)L * nc));
-        TRYS(gemm_sb(p, L, nc, nc, T1, L, (i64)L * nc, Cc, nc, 0, Tcv, L, (i64)L * nc, nv));
-        TRYS(gemm_sb(p, L, nc, nc, Bca, L, (i64)L * nc, Cc, nc, 0, Tca, L, (i64)L * nc, n));
-    }
-    TRYS(gemm(p, false, false, L * n, nv, nv, 1.0, Bav, (i64)L * n, Cv, nv, 0.0, Tav, (i64)L * n));
-    const double *Taa = Baa;
-    // ---- integral blocks: I_xy[u,v] = sum_L T_x[L,u] T_y[L,v]
-    const i64 Ncv = (i64)nc * nv, Nca = (i64)nc * n, Nav = (i64)n * nv, Naa = n2;
-    GET(Icvcv, "Icvcv", Ncv * Ncv); GET(Icacv, "Icacv", Nca * Ncv); GET(Icvav, "Icvav", Ncv * Nav);
-    GET(Icaca, "Icaca", Nca * Nca); GET(Iavav, "Iavav", Nav * Nav); GET(Icvaa, "Icvaa", Ncv * Naa);
-    GET(Icaav, "Icaav", Nca * Nav); GET(Iaaav, "Iaaav", Naa * Nav); GET(Iaaca, "Iaaca", Naa * Nca);
-    TRYS(gemm(p, true, false, Ncv, Ncv, L, 1.0, Tcv, L, Tcv, L, 0.0, Icvcv, Ncv));
-    TRYS(gemm(p, true, false, Nca, Ncv, L, 1.0, Tca, L, Tcv, L, 0.0, Icacv, Nca));
-    TRYS(gemm(p, true, false, Ncv, Nav, L, 1.0, Tcv, L, Tav, L, 0.0, Icvav, Ncv));
-    TRYS(gemm(p, true, false, Nca, Nca, L, 1.0, Tca, L, Tca, L, 0.0, Icaca, Nca));
-    TRYS(gemm(p, true, false, Nav, Nav, L, 1.0, Tav, L, Tav, L, 0.0, Iavav, Nav));
-    TRYS(gemm(p, true, false, Ncv, Naa, L, 1.0, Tcv, L, Taa, L, 0.0, Icvaa, Ncv));
-    TRYS(gemm(p, true, false, Nca, Nav, L, 1.0, Tca, L, Tav, L, 0.0, Icaav, Nca));
-    TRYS(gemm(p, true, false, Naa, Nav, L, 1.0, Taa, L, Tav, L, 0.0, Iaaav, Naa));
-    TRYS(gemm(p, true, false, Naa, Nca, L, 1.0, Taa, L, Tca, L, 0.0, Iaaca, Naa));
-    // ---- one-electron integrals in the canonical bases and the Dyall-type heff vectors
-    GET(Jc2, "Jc2", L);
-    if (nc) {
-        hipLaunchKernelGGL(k_jdiag, dim3(nb(L, 256)), dim3(256), 0, st, Tcc, L, nc, Jc2);
-        NV2_LAUNCH_CHECK();
-    } else NV2_HIP(hipMemsetAsync(Jc2, 0, sizeof(double) * L, st));
-    GET(tmp, "h_tmp", (i64)norb * norb);
-    // heff0p[k,r] = (C^T h_cv V)[k,r] + 2 (Jc2^T Tcv)[k,r] - sum_{L,j} Tcc[L,j,k] Tcv[L,j,r]
-    GET(h0p, "heff0p", Ncv);
-    GET(hm1p, "heffm1p", Nav);
-    GET(hp1p, "heffp1p", Nca);
-    if (nc) {
-        TRYS(gemm(p, false, false, nc, nv, nv, 1.0, hf + (i64)(nc + n) * norb, norb, Cv, nv, 0.0, tmp, nc));
-        TRYS(gemm(p, true, false, nc, nv, nc, 1.0, Cc, nc, tmp, nc, 0.0, h0p, nc));
-        TRYS(gemm(p, true, false, 1, (int)Ncv, L, 2.0, Jc2, L, Tcv, L, 1.0, h0p, 1));
-        TRYS(gemm(p, true, false, nc, nv, L * nc, -1.0, Tcc, (i64)L * nc, Tcv, (i64)L * nc, 1.0, h0p, nc));
-    }
-    // heffm1p[a,r] = (h_av V)[a,r] + 2 (Jc2^T Tav)[a,r] - sum_{L,j} Tca[L,j,a] Tcv[L,j,r] - sum_{L,b} Taa[L,b,a] Tav[L,b,r]
-    TRYS(gemm(p, false, false, n, nv, nv, 1.0, hf + (i64)(nc + n) * norb + nc, norb, Cv, nv, 0.0, hm1p, n));
-    TRYS(gemm(p, true, false, 1, (int)Nav, L, 2.0, Jc2, L, Tav, L, 1.0, hm1p, 1));
-    if (nc) TRYS(gemm(p, true, false, n, nv, L * nc, -1.0, Tca, (i64)L * nc, Tcv, (i64)L * nc, 1.0, hm1p, n));
-    TRYS(gemm(p, true, false, n, nv, L * n, -1.0, Taa, (i64)L * n, Tav, (i64)L * n, 1.0, hm1p, n));
-    // heffp1p[i,a] = (C^T h_ca)[i,a] + 2 (Jc2^T Tca)[i,a] - sum_{L,j} Tcc[L,j,i] Tca[L,j,a]
-    if (nc) {
-        TRYS(gemm(p, true, false, nc, n, nc, 1.0, Cc, nc, hf + (i64)nc * norb, norb, 0.0, hp1p, nc));
-        TRYS(gemm(p, true, false, 1, (int)Nca, L, 2.0, Jc2, L, Tca, L, 1.0, hp1p, 1));
-        TRYS(gemm(p, true, false, nc, n, L * nc, -1.0, Tcc, (i64)L * nc, Tca, (i64)L * nc, 1.0, hp1p, nc));
-    }
     // ---- Koopmans matrices of this state
     GET(k_eako1, "k_eako1", n2); GET(k_ipko1, "k_ipko1", n2); GET(k_hrdm, "k_hrdm", n4); GET(k_eako2, "k_eako2", n4);
     GET(k_ipko2, "k_ipko2", n4); GET(k_ga, "k_ga", n4); GET(k_gd, "k_gd", n4); GET(k_gf, "k_gf", n2);
@@ -1673,6 +1606,107 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         workers.emplace_back([p, &J, w, thr, &wrc]() { wrc[w] = pc_worker(p, J, NJOB, w, thr); });
     auto join_workers = [&]() { for (auto &t : workers) if (t.joinable()) t.join(); };
 
+    // ---- Fock operators and canonical orbitals (qdnevpt.F:369-698)
+    GET(Jc, "Jc", L); GET(Ja, "Ja", L);
+    hipLaunchKernelGGL(k_jvec, dim3(nb(L, 256)), dim3(256), 0, st, p->in.B, L, nc, n, da, Jc, Ja);
+    NV2_LAUNCH_CHECK();
+    double *hf = p->get("hfull", (i64)norb * norb);
+    GET(eps, "eps", norb);
+    NV2_HIP(hipMemsetAsync(eps, 0, sizeof(double) * norb, st));
+    GET(Cc, "Ccore", (i64)nc * nc);
+    GET(Cv, "Cvirt", (i64)nv * nv);
+    GET(Jt, "Jtot", L);
+    double *Bcc = p->get("Bcc", 0), *Bcv = p->get("Bcv", 0), *Bca = p->get("Bca", 0), *Bac = p->get("Bac", 0);
+    double *Bav = p->get("Bav", 0), *Baa = p->get("Baa", 0);
+    // Jt = 2 Jcore + Jact (Coulomb of both core and active densities)
+    {
+        const double one = 1.0, two = 2.0;
+        NV2_HIP(hipMemcpyAsync(Jt, Ja, sizeof(double) * L, hipMemcpyDeviceToDevice, st));
+        if (nc) RB(rocblas_daxpy(p->h, L, &two, Jc, 1, Jt, 1));
+        (void)one;
+    }
+    if (nc) {
+        NV2_HIP(hipMemcpy2DAsync(Cc, sizeof(double) * nc, hf, sizeof(double) * norb, sizeof(double) * nc, nc,
+                                 hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_coul, dim3(nb((i64)nc * nc, 4)), dim3(256), 0, st, p->in.B, L, 0, nc, Jt, Cc);
+        NV2_LAUNCH_CHECK();
+        TRYS(gemm(p, true, false, nc, nc, L * nc, -1.0, Bcc, (i64)L * nc, Bcc, (i64)L * nc, 1.0, Cc, nc));
+        GET(G, "fockG", (i64)L * n * (nc > nv ? nc : nv));
+        TRYS(gemm_sb(p, L, n, n, Bac, L, (i64)L * n, da, n, 0, G, L, (i64)L * n, nc));
+        TRYS(gemm(p, true, false, nc, nc, L * n, -0.5, G, (i64)L * n, Bac, (i64)L * n, 1.0, Cc, nc));
+        GET(wc, "wcore", nc);
+        TRYS(syevd(p, false, nc, Cc, wc));
+        NV2_HIP(hipMemcpyAsync(eps, wc, sizeof(double) * nc, hipMemcpyDeviceToDevice, st));
+    }
+    {
+        NV2_HIP(hipMemcpy2DAsync(Cv, sizeof(double) * nv, hf + (i64)(nc + n) * norb + nc + n, sizeof(double) * norb,
+                                 sizeof(double) * nv, nv, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_coul, dim3(nb((i64)nv * nv, 4)), dim3(256), 0, st, p->in.B, L, nc + n, nv, Jt, Cv);
+        NV2_LAUNCH_CHECK();
+        if (nc) TRYS(gemm(p, true, false, nv, nv, L * nc, -1.0, Bcv, (i64)L * nc, Bcv, (i64)L * nc, 1.0, Cv, nv));
+        GET(G, "fockG", (i64)L * n * (nc > nv ? nc : nv));
+        TRYS(gemm_sb(p, L, n, n, Bav, L, (i64)L * n, da, n, 0, G, L, (i64)L * n, nv));
+        TRYS(gemm(p, true, false, nv, nv, L * n, -0.5, G, (i64)L * n, Bav, (i64)L * n, 1.0, Cv, nv));
+        GET(wv, "wvirt", nv);
+        TRYS(syevd(p, false, nv, Cv, wv));
+        NV2_HIP(hipMemcpyAsync(eps + nc + n, wv, sizeof(double) * nv, hipMemcpyDeviceToDevice, st));
+    }
+    const double *ec = eps, *ev = eps + nc + n;
+    // ---- Cholesky transform (qdnevpt.F:7599-7717)
+    GET(T1, "T1", (i64)L * nc * (nv > nc ? nv : nc));
+    GET(Tcc, "Tcc", (i64)L * nc * nc); GET(Tcv, "Tcv", (i64)L * nc * nv); GET(Tca, "Tca", (i64)L * nc * n);
+    GET(Tav, "Tav", (i64)L * n * nv);
+    if (nc) {
+        TRYS(gemm(p, false, false, L * nc, nc, nc, 1.0, Bcc, (i64)L * nc, Cc, nc, 0.0, T1, (i64)L * nc));
+        TRYS(gemm_sb(p, L, nc, nc, T1, L, (i64)L * nc, Cc, nc, 0, Tcc, L, (i64)L * nc, nc));
+        TRYS(gemm(p, false, false, L * nc, nv, nv, 1.0, Bcv, (i64)L * nc, Cv, nv, 0.0, T1, (i64)L * nc));
+        TRYS(gemm_sb(p, L, nc, nc, T1, L, (i64)L * nc, Cc, nc, 0, Tcv, L, (i64)L * nc, nv));
+        TRYS(gemm_sb(p, L, nc, nc, Bca, L, (i64)L * nc, Cc, nc, 0, Tca, L, (i64)L * nc, n));
+    }
+    TRYS(gemm(p, false, false, L * n, nv, nv, 1.0, Bav, (i64)L * n, Cv, nv, 0.0, Tav, (i64)L * n));
+    const double *Taa = Baa;
+    // ---- integral blocks: I_xy[u,v] = sum_L T_x[L,u] T_y[L,v]
+    const i64 Ncv = (i64)nc * nv, Nca = (i64)nc * n, Nav = (i64)n * nv, Naa = n2;
+    GET(Icvcv, "Icvcv", Ncv * Ncv); GET(Icacv, "Icacv", Nca * Ncv); GET(Icvav, "Icvav", Ncv * Nav);
+    GET(Icaca, "Icaca", Nca * Nca); GET(Iavav, "Iavav", Nav * Nav); GET(Icvaa, "Icvaa", Ncv * Naa);
+    GET(Icaav, "Icaav", Nca * Nav); GET(Iaaav, "Iaaav", Naa * Nav); GET(Iaaca, "Iaaca", Naa * Nca);
+    TRYS(gemm(p, true, false, Ncv, Ncv, L, 1.0, Tcv, L, Tcv, L, 0.0, Icvcv, Ncv));
+    TRYS(gemm(p, true, false, Nca, Ncv, L, 1.0, Tca, L, Tcv, L, 0.0, Icacv, Nca));
+    TRYS(gemm(p, true, false, Ncv, Nav, L, 1.0, Tcv, L, Tav, L, 0.0, Icvav, Ncv));
+    TRYS(gemm(p, true, false, Nca, Nca, L, 1.0, Tca, L, Tca, L, 0.0, Icaca, Nca));
+    TRYS(gemm(p, true, false, Nav, Nav, L, 1.0, Tav, L, Tav, L, 0.0, Iavav, Nav));
+    TRYS(gemm(p, true, false, Ncv, Naa, L, 1.0, Tcv, L, Taa, L, 0.0, Icvaa, Ncv));
+    TRYS(gemm(p, true, false, Nca, Nav, L, 1.0, Tca, L, Tav, L, 0.0, Icaav, Nca));
+    TRYS(gemm(p, true, false, Naa, Nav, L, 1.0, Taa, L, Tav, L, 0.0, Iaaav, Naa));
+    TRYS(gemm(p, true, false, Naa, Nca, L, 1.0, Taa, L, Tca, L, 0.0, Iaaca, Naa));
+    // ---- one-electron integrals in the canonical bases and the Dyall-type heff vectors
+    GET(Jc2, "Jc2", L);
+    if (nc) {
+        hipLaunchKernelGGL(k_jdiag, dim3(nb(L, 256)), dim3(256), 0, st, Tcc, L, nc, Jc2);
+        NV2_LAUNCH_CHECK();
+    } else NV2_HIP(hipMemsetAsync(Jc2, 0, sizeof(double) * L, st));
+    GET(tmp, "h_tmp", (i64)norb * norb);
+    // heff0p[k,r] = (C^T h_cv V)[k,r] + 2 (Jc2^T Tcv)[k,r] - sum_{L,j} Tcc[L,j,k] Tcv[L,j,r]
+    GET(h0p, "heff0p", Ncv);
+    GET(hm1p, "heffm1p", Nav);
+    GET(hp1p, "heffp1p", Nca);
+    if (nc) {
+        TRYS(gemm(p, false, false, nc, nv, nv, 1.0, hf + (i64)(nc + n) * norb, norb, Cv, nv, 0.0, tmp, nc));
+        TRYS(gemm(p, true, false, nc, nv, nc, 1.0, Cc, nc, tmp, nc, 0.0, h0p, nc));
+        TRYS(gemm(p, true, false, 1, (int)Ncv, L, 2.0, Jc2, L, Tcv, L, 1.0, h0p, 1));
+        TRYS(gemm(p, true, false, nc, nv, L * nc, -1.0, Tcc, (i64)L * nc, Tcv, (i64)L * nc, 1.0, h0p, nc));
+    }
+    // heffm1p[a,r] = (h_av V)[a,r] + 2 (Jc2^T Tav)[a,r] - sum_{L,j} Tca[L,j,a] Tcv[L,j,r] - sum_{L,b} Taa[L,b,a] Tav[L,b,r]
+    TRYS(gemm(p, false, false, n, nv, nv, 1.0, hf + (i64)(nc + n) * norb + nc, norb, Cv, nv, 0.0, hm1p, n));
+    TRYS(gemm(p, true, false, 1, (int)Nav, L, 2.0, Jc2, L, Tav, L, 1.0, hm1p, 1));
+    if (nc) TRYS(gemm(p, true, false, n, nv, L * nc, -1.0, Tca, (i64)L * nc, Tcv, (i64)L * nc, 1.0, hm1p, n));
+    TRYS(gemm(p, true, false, n, nv, L * n, -1.0, Taa, (i64)L * n, Tav, (i64)L * n, 1.0, hm1p, n));
+    // heffp1p[i,a] = (C^T h_ca)[i,a] + 2 (Jc2^T Tca)[i,a] - sum_{L,j} Tcc[L,j,i] Tca[L,j,a]
+    if (nc) {
+        TRYS(gemm(p, true, false, nc, n, nc, 1.0, Cc, nc, hf + (i64)nc * norb, norb, 0.0, hp1p, nc));
+        TRYS(gemm(p, true, false, 1, (int)Nca, L, 2.0, Jc2, L, Tca, L, 1.0, hp1p, 1));
+        TRYS(gemm(p, true, false, nc, n, L * nc, -1.0, Tcc, (i64)L * nc, Tca, (i64)L * nc, 1.0, hp1p, nc));
+    }
     // ================================================================ SC sums that need no PC solve (main stream)
     // class (0): SC == PC
     if (nc && nv) {
