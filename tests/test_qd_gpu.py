@@ -8,17 +8,18 @@ from nevpt2_amd.evaluator import Evaluator
 from oracle import stage1 as S1, qd as Q
 from conftest import energy_tol
 
-CASES = [  # (nact, ncore, nvirt, seed, roots)
-    (4, 2, 3, 5, 3),
-    (6, 3, 5, 2, 2),
+CASES = [  # (nact, ncore, nvirt, seed, roots, active electrons)
+    (4, 2, 3, 5, 3, 4),
+    (6, 3, 5, 2, 2, 6),
+    (7, 2, 3, 4, 2, 6),  # odd active space, nele < nact
 ]
 
 
 @pytest.fixture(scope="module", params=CASES, ids=lambda c: f"cas{c[0]}_c{c[1]}_v{c[2]}_m{c[4]}")
 def qdcase(request):
-    n, nc, nv, seed, M = request.param
-    print(f"[qd case] nact={n} ncore={nc} nvirt={nv} seed={seed} roots={M}", flush=True)
-    P = synth.Problem(n, nc, nv, seed=seed, nroots=M)
+    n, nc, nv, seed, M, nele = request.param
+    print(f"[qd case] nact={n} ncore={nc} nvirt={nv} seed={seed} roots={M} nele={nele}", flush=True)
+    P = synth.Problem(n, nc, nv, seed=seed, nroots=M, nele=nele)
     koop = S1.koopmans_all(P)
     Hsc, Hpc, tabs, off = Q.heff_matrices(P, koop)
     return P, koop, Hsc, Hpc, tabs, off

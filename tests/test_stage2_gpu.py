@@ -6,18 +6,19 @@ from nevpt2_amd import synth, qdnevpt
 from oracle import stage1 as S1, stage2 as S2
 from conftest import energy_tol
 
-CASES = [  # (nact, ncore, nvirt, seed)
-    (4, 2, 3, 11),
-    (6, 3, 8, 1),
-    (6, 4, 12, 2),
+CASES = [  # (nact, ncore, nvirt, seed, active electrons)
+    (4, 2, 3, 11, 4),
+    (6, 3, 8, 1, 6),
+    (6, 4, 12, 2, 6),
+    (7, 2, 4, 4, 6),     # odd active space, nele < nact
 ]
 
 
 @pytest.fixture(scope="module", params=CASES, ids=lambda c: f"cas{c[0]}_c{c[1]}_v{c[2]}")
 def case(request):
-    n, nc, nv, seed = request.param
-    print(f"[stage2 case] nact={n} ncore={nc} nvirt={nv} seed={seed}", flush=True)
-    P = synth.Problem(n, nc, nv, seed=seed)
+    n, nc, nv, seed, nele = request.param
+    print(f"[stage2 case] nact={n} ncore={nc} nvirt={nv} seed={seed} nele={nele}", flush=True)
+    P = synth.Problem(n, nc, nv, seed=seed, nele=nele)
     koop = S1.koopmans_all(P)
     ref, _ = S2.state_energies(P, koop, 0)
     return P, koop, ref
