@@ -21,6 +21,7 @@
 
 #include "nv2_common.h"
 #include "nv2_stage2.h"
+#include "nv2_trd.h"
 
 namespace nv2 {
 
@@ -1018,24 +1019,28 @@ __global__ void k_pm1p_sc(int plus, int n, int next, const double *H, const doub
     block_store4(E, P, 0.0, 0.0, part);
     if (q.M) block_storeQ(qsc, qpc, q);
 }
-// PC sums: R[mu, x] from the S-build, deno = w(mu) -/+ e_x
-__global__ void k_pm1p_pc(int plus, int nnew, int next, const double *R, const double *w, const double *eo,
-                          double *part, QDArgs q) {
-    // QD: q.a[j] = R of pair (i,j) (nnew x next)
-    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+// (+-1)' PC sums of one external orbital x per thread: (T + s_x) u = z_x with s_x = -e_x for (+1)'
+// (denominator w_mu - e_i) and +e_x for (-1)' (w_mu + e_r); E -= z.u, norm += u.u, and for QD the
+// coupling -z_j.u with z_j = q.a[j] (same layout as Z)
+__global__ void k_pm1p_shift(int plus, int nnew, int next, const double *d, const double *e, const double *Z, i64 ldz,
+                             const double *eo, double *rows, double *part, QDArgs q) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
     double E = 0.0, P = 0.0, qsc[QM], qpc[QM];
 #pragma unroll
     for (int j = 0; j < QM; j++) { qsc[j] = 0.0; qpc[j] = 0.0; }
-    if (t < (i64)nnew * next) {
-        int mu = t % nnew, x = t / nnew;
-        double r = R[t];
-        double deno = plus ? w[mu] - eo[x] : w[mu] + eo[x];
-        double co = r / deno;
-        E = -r * co;
-        P = co * co;
+    if (x < next) {
+        const double sh = plus ? -eo[x] : eo[x];
+        const double *z = Z + ldz * x;
+        double ez = 0.0, uu = 0.0;
+        trd_shift_solve(nnew, d, e, sh, z, 1, rows + x, next, [&](int i, double u) {
+            ez += z[i] * u;
+            uu += u * u;
 #pragma unroll
-        for (int j = 0; j < QM; j++)
-            if (qd_on(q, j)) qpc[j] = -q.a[j][t] * co;
+            for (int j = 0; j < QM; j++)
+                if (qd_on(q, j)) qpc[j] -= q.a[j][ldz * x + i] * u;
+        });
+        E = -ez;
+        P = uu;
     }
     block_store4(0.0, 0.0, E, P, part);
     if (q.M) block_storeQ(qsc, qpc, q);
@@ -1061,6 +1066,7 @@ struct Stage2 {
     int *hinfo = nullptr;          // pinned host copies of solver info words
     int *dinfo_w = nullptr;        // device info words, 2 per solve slot
     double *wE[NW] = {};           // syevd workspace per worker
+    TrdPlan *trdplan[NW] = {};     // captured tridiagonalisation of worker w (touched by that worker only)
     int nc, n, nv, norb, L, M;
     std::map<std::string, Buf> bufs;
     int *dinfo = nullptr;
@@ -1276,6 +1282,8 @@ static int qd_reduce(Stage2 *p, const QDArgs &q, i64 nblocks, int cls) {
 struct PcJob {
     std::string tag;
     int N = 0, nb = 1, w = 0, slot = 0;        // info words at 4*slot + {metric b, K' 2+b}
+    bool trd = false;                          // K' only tridiagonalised (nv2_trd.hip), no eigenvectors
+    double *trdw = nullptr;                    // its scratch; T and tau of problem b from trd_outputs(nmax)
     double *S = nullptr, *K = nullptr;         // nb consecutive N x N matrices; S is overwritten
     double *w1 = nullptr, *hw1 = nullptr, *coef = nullptr, *w2 = nullptr;
     double *caux = nullptr, *Ks = nullptr, *T = nullptr, *Kp = nullptr;   // nb x N x N scratch each
@@ -1295,8 +1303,17 @@ static int pc_job_buffers(Stage2 *p, PcJob &j, size_t hoff) {
     GET(Ks, (j.tag + "_Ks").c_str(), NN); GET(T, (j.tag + "_T").c_str(), NN); GET(Kp, (j.tag + "_Kp").c_str(), NN);
     j.w1 = w1; j.w2 = w2; j.coef = coef; j.caux = caux; j.Ks = Ks; j.T = T; j.Kp = Kp;
     j.hw1 = p->hpin + hoff;
+    if (j.trd) {
+        GET(tw, (j.tag + "_trdw").c_str(), trd_workspace(j.N, j.nb));
+        j.trdw = tw;
+    }
     for (int k = 0; k < 4; k++) p->hinfo[4 * j.slot + k] = 0;
     return 0;
+}
+// K' = Q T Q^T of a trd job; the launch sequence is captured once per shape and buffer set
+// (one plan per worker: a repeated evaluation of one problem replays it)
+static int pc_trd_launch(Stage2 *p, PcJob &j) {
+    return trd_reduce(p->ws[j.w], j.nmax, j.Kp, j.NN(), j.nb, j.trdw, &p->trdplan[j.w]);
 }
 static int pc_metric_launch(Stage2 *p, PcJob &j) {
     TRYS(syevd_async(p, j.w, 4 * j.slot, true, j.N, j.S, j.NN(), j.w1, j.N, j.nb));
@@ -1338,6 +1355,7 @@ static int pc_kprime_launch(Stage2 *p, PcJob &j, double thr) {
         hipLaunchKernelGGL(k_pad_gersh, dim3(1), dim3(256), 0, st, j.Kp, nmax, NN, j.nb, j.nnew[0], j.nnew[1], nmax);
         NV2_LAUNCH_CHECK();
     }
+    if (j.trd) return pc_trd_launch(p, j);
     TRYS(syevd_async(p, j.w, 4 * j.slot + 2, true, nmax, j.Kp, NN, j.w2, N, j.nb));
     RB(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, N, nmax, nmax, &one, j.caux, N,
                                      NN, j.Kp, nmax, NN, &zero, j.coef, N, NN, j.nb));
@@ -1410,6 +1428,7 @@ void stage2_destroy(Stage2 *p) {
     for (int w = 0; w < NW; w++) {
         if (p->wh[w]) rocblas_destroy_handle(p->wh[w]);
         if (p->ws[w]) (void)hipStreamDestroy(p->ws[w]);
+        trd_plan_destroy(p->trdplan[w]);
     }
     for (int e = 0; e < NW + 1; e++) if (p->wev[e]) (void)hipEventDestroy(p->wev[e]);
     (void)hipFree(p->pairs_c); (void)hipFree(p->pairs_cs); (void)hipFree(p->pairs_v); (void)hipFree(p->pairs_vs);
@@ -1524,6 +1543,7 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             J[k].nb = k == J0P ? 1 : 2;
             J[k].slot = k;
             J[k].w = k == JPM1P ? 1 : 0;
+            J[k].trd = k == JPM1P;
             GET(Sm, (J[k].tag + "_S").c_str(), (i64)Ns[k] * Ns[k] * J[k].nb);
             GET(Km, (J[k].tag + "_K").c_str(), (i64)Ns[k] * Ns[k] * J[k].nb);
             J[k].S = Sm;
@@ -1765,6 +1785,7 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
     }
     // (-1)' and (+1)' SC
     double *Vpm[2] = {nullptr, nullptr}, *Hpm[2] = {nullptr, nullptr}, *Q2pm[2] = {nullptr, nullptr};
+    double *Xpm[2] = {nullptr, nullptr};
     for (int plus = 0; has4 && plus < 2; plus++) {
         const int cls = plus ? 5 : 6;
         const int next = plus ? nc : nv;
@@ -1802,6 +1823,11 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         TRYS(gemm(p, false, false, (int)N3, next, n, 1.0, BC, N3, H, n, 0.0, Z2, N3));
         TRYS(gemm(p, false, false, (int)N3, next, (int)N3, 1.0, Mx[plus], N3, V, N3, 0.0, Y1, N3));   // M^T v
         TRYS(gemm(p, false, false, (int)N3, next, (int)N3, 1.0, jb.Kb(bb), N3, V, N3, 0.0, Y2, N3));  // Am^T v
+        // PC right-hand sides X = M^T V + Q2 H (the S-build of qdnevpt.F:5277-5324 applied to the integrals)
+        GET(Xpc, ("pm1_X" + t).c_str(), N3 * next);
+        TRYS(gemm(p, true, false, (int)N3, next, (int)N3, 1.0, Mx[plus], N3, V, N3, 0.0, Xpc, N3));
+        TRYS(gemm(p, false, false, (int)N3, next, n, 1.0, Q2, N3, H, n, 1.0, Xpc, N3));
+        Xpm[plus] = Xpc;
         for (int j = 0; q.M && j < M; j++) {
             if (j == s) continue;
             QFILL(r1j, plus ? QK_P1P_R1 : QK_M1P_R1, j, n2, "qd_pm1r1_");
@@ -1992,42 +2018,49 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             TRYS(qd_reduce(p, q, nbk, 7));
         }
     }
-    // ================================================================ classes (-1)' and (+1)', PC: S-build and sums
+    // ================================================================ classes (-1)' and (+1)', PC sums
+    // sum_mu r_mu^2 / (w_mu + s_x) with r = C^T caux^T X_x and K' = C W C^T (qdnevpt.F:5351-5422,
+    // 5130-5205) is y^T (K' + s_x)^-1 y with y = caux^T X_x; with K' = Q T Q^T it is
+    // z^T (T + s_x)^-1 z, z = Q^T y: one tridiagonal solve per external orbital, no eigenvectors of K'
     for (int plus = 0; has4 && plus < 2; plus++) {
         const int cls = plus ? 5 : 6;
         const int next = plus ? nc : nv;
         const double *eo = plus ? ec : ev;
         const PcJob &jb = J[JPM1P];
         const int bb = plus ? 0 : 1;
-        const int nnew = jb.nnew[bb];
+        const int nnew = jb.nnew[bb], nmax = jb.nmax;
         if (!next || !nnew) continue;
-        const std::string t = plus ? "_p" : "_m";
-        double *coef = jb.coefb(bb), *w = jb.w2b(bb), *V = Vpm[plus], *H = Hpm[plus], *Q2 = Q2pm[plus];
-        GET(SbT, "pm1_SbT", N3 * nnew); GET(spT, "pm1_spT", (i64)n * nnew); GET(R, ("pm1_R" + t).c_str(), (i64)nnew * next);
-        TRYS(gemm(p, false, false, (int)N3, nnew, (int)N3, 1.0, Mx[plus], N3, coef, N3, 0.0, SbT, N3));    // S-build
-        TRYS(gemm(p, true, false, n, nnew, (int)N3, 1.0, Q2, N3, coef, N3, 0.0, spT, n));
-        TRYS(gemm(p, true, false, nnew, next, (int)N3, 1.0, SbT, N3, V, N3, 0.0, R, nnew));
-        TRYS(gemm(p, true, false, nnew, next, n, 1.0, spT, n, H, n, 1.0, R, nnew));
+        const i64 NN = jb.NN();
+        const double *caux = jb.caux + bb * NN, *Kp = jb.Kp + bb * NN;
+        const TrdOut to = trd_outputs(nmax, jb.nb, jb.trdw);
+        const double *dT = to.d + (i64)bb * nmax, *eT = to.e + (i64)bb * nmax, *tauT = to.tau + (i64)bb * nmax;
+        const int ncq = 1 + (q.M ? M - 1 : 0);              // right-hand sides: this state, then the QD partners
+        GET(Ypc, "pm1_Ypc", (i64)nmax * next * ncq);
+        GET(Wq, "pm1_applyq", trd_apply_workspace(nmax, 1, next * ncq));
+        TRYS(gemm(p, true, false, nmax, next, (int)N3, 1.0, caux, N3, Xpm[plus], N3, 0.0, Ypc, nmax));
+        int col = 1;
         for (int j = 0; q.M && j < M; j++) {
             if (j == s) continue;
             double *mj = qd_fill(p, Sj[j], plus ? QK_P1P_MPC : QK_M1P_MPC, j < s, N3 * N3, "qd_pm1M");
             if (!mj) return -1;
             QFILL(qj, plus ? QK_P1P_QPC : QK_M1P_QPC, j, N3 * n, "qd_pm1qp_");
-            GET(sbj, "qd_pm1SbT", N3 * nnew); GET(spj, "qd_pm1spT", (i64)n * nnew);
-            QGET(rj, "qd_pm1R_", j, (i64)nnew * next);
-            TRYS(gemm(p, false, false, (int)N3, nnew, (int)N3, 1.0, mj, N3, coef, N3, 0.0, sbj, N3));
-            TRYS(gemm(p, true, false, n, nnew, (int)N3, 1.0, qj, N3, coef, N3, 0.0, spj, n));
-            TRYS(gemm(p, true, false, nnew, next, (int)N3, 1.0, sbj, N3, V, N3, 0.0, rj, nnew));
-            TRYS(gemm(p, true, false, nnew, next, n, 1.0, spj, n, H, n, 1.0, rj, nnew));
-            q.a[j] = rj;
+            GET(xj, "qd_pm1X", N3 * next);
+            TRYS(gemm(p, true, false, (int)N3, next, (int)N3, 1.0, mj, N3, Vpm[plus], N3, 0.0, xj, N3));
+            TRYS(gemm(p, false, false, (int)N3, next, n, 1.0, qj, N3, Hpm[plus], n, 1.0, xj, N3));
+            double *yj = Ypc + (i64)nmax * next * col++;
+            TRYS(gemm(p, true, false, nmax, next, (int)N3, 1.0, caux, N3, xj, N3, 0.0, yj, nmax));
+            q.a[j] = yj;
         }
-        const i64 tot = (i64)nnew * next;
-        double *pt2 = parts(p, nb(tot, 256));
-        TRYS(qd_prepare(p, q, nb(tot, 256)));
-        hipLaunchKernelGGL(k_pm1p_pc, dim3(nb(tot, 256)), dim3(256), 0, st, plus, nnew, next, R, w, eo, pt2, q);
+        TRYS(trd_apply_q(p->h, st, nmax, Kp, NN, 1, tauT, true, next * ncq, Ypc, 0, Wq));   // z = Q^T y
+        GET(rows, "pm1_rows", (i64)4 * nnew * next);
+        const unsigned nbk = nb(next, 64);
+        double *pt2 = parts(p, nbk);
+        TRYS(qd_prepare(p, q, nbk));
+        hipLaunchKernelGGL(k_pm1p_shift, dim3(nbk), dim3(64), 0, st, plus, nnew, next, dT, eT, Ypc, (i64)nmax, eo, rows,
+                           pt2, q);
         NV2_LAUNCH_CHECK();
-        TRYS(reduce_into(p, nb(tot, 256), cls));
-        TRYS(qd_reduce(p, q, nb(tot, 256), cls));
+        TRYS(reduce_into(p, nbk, cls));
+        TRYS(qd_reduce(p, q, nbk, cls));
     }
     // ---- results: (norm_sc, e_sc, norm_pc, e_pc) per class, SC==PC for class (0)
     double accs[32];
