@@ -10,13 +10,13 @@
 // Q = H_0 H_1 ... H_{N-2}.  The input is symmetrised from its upper triangle first (the
 // triangle dsyev('V','U') reads).
 //
-// Two launches per column k, both batched over the matrices: k_trd_cols (one wavefront per
-// trailing column) applies the rank-2 update of step k-1 and forms (A v_k)_c as a column
-// dot product (A is symmetric, so there are no partial sums to reduce); k_trd_house (one
-// workgroup per matrix) forms w_k and generates reflector k+1.  Fusing the two through a
-// last-workgroup-done counter needs an agent-scope release per workgroup, which writes back
-// the freshly updated columns from every L2 and measured twice as slow.  The 2(N-2)+3
-// launches are captured in a hipGraph once per shape and buffer set, then replayed.
+// One launch per column k (k_trd_col), batched over the matrices.  Every workgroup repeats the
+// O(N) Householder step of column k itself instead of waiting for one workgroup to publish it
+// (a last-workgroup-done handshake needs an agent-scope release per workgroup, which writes back
+// the freshly updated columns from every L2 and measured twice as slow); then one wavefront per
+// trailing column applies the rank-2 update of step k-1 and forms (A v_k)_c as a column dot
+// product (A is symmetric, so there are no partial sums to reduce).  The N+2 launches are
+// captured in a hipGraph once per shape and buffer set, then replayed.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
@@ -27,9 +27,7 @@ namespace nv2 {
 
 #define TRYT(x) do { int _r = (x); if (_r) return _r; } while (0)
 
-#define TRD_WG 256      // threads of a k_trd_cols workgroup (4 wavefronts)
-#define TRD_HOUSE 512   // threads of the k_trd_house workgroup
-#define TRD_U 4         // independent loads in flight per lane
+#define TRD_WG 256      // threads of a k_trd_col workgroup (4 wavefronts)
 
 __global__ void k_trd_sym_upper(double *A, int N, i64 sA) {
     const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -55,131 +53,146 @@ __device__ inline double trd_block_sum(double x, double *red) {
     return s;
 }
 
-// Householder step of matrix b by one workgroup.  k = -1: reflector of column 0 from the
-// input.  k >= 0: w_k = p - (tau_k/2)(p^T v_k) v_k from p = tau_k A v_k, then reflector k+1
-// of column k+1 brought up to date with step k (stored values carry steps <= k-1).
-__device__ void trd_house(double *Ab, int N, int k, const double *vk, double *wk, double *vn, const double *pb,
-                          double *db, double *eb, double *taub, double *red, double *sc) {
-    const int c1 = k + 1, nt = blockDim.x;
-    double alpha = 0.0;
-    if (k >= 0) {
-        double s = 0.0;
-        for (int r = c1 + threadIdx.x; r < N; r += nt) s += pb[r] * vk[r];
-        s = trd_block_sum(s, red);
-        alpha = -0.5 * taub[k] * s;
-        for (int r = c1 + threadIdx.x; r < N; r += nt) wk[r] = pb[r] + alpha * vk[r];
-    }
-    const double vc1 = k >= 0 ? vk[c1] : 0.0, wc1 = k >= 0 ? pb[c1] + alpha * vc1 : 0.0;
-    double xn = 0.0;                    // |x|^2, x = rows c1+2.. of the column
-    for (int r = c1 + 2 + threadIdx.x; r < N; r += nt) {
-        double a = Ab[r + (i64)N * c1];
-        if (k >= 0) a -= vk[r] * wc1 + (pb[r] + alpha * vk[r]) * vc1;
-        xn += a * a;
-    }
-    xn = trd_block_sum(xn, red);
-    if (threadIdx.x == 0) {
-        double a0 = Ab[c1 + (i64)N * c1], a1 = Ab[(c1 + 1) + (i64)N * c1];
-        if (k >= 0) {
-            const double v1 = vk[c1 + 1], w1 = pb[c1 + 1] + alpha * v1;
-            a0 -= 2.0 * vc1 * wc1;
-            a1 -= v1 * wc1 + w1 * vc1;
-        }
-        double beta, t, scale;
-        if (xn == 0.0) { beta = a1; t = 0.0; scale = 0.0; }
-        else {
-            beta = -copysign(sqrt(a1 * a1 + xn), a1);
-            t = (beta - a1) / beta;
-            scale = 1.0 / (a1 - beta);
-        }
-        db[c1] = a0;
-        eb[c1] = beta;
-        taub[c1] = t;
-        *sc = scale;
-        Ab[c1 + (i64)N * c1] = a0;
-        Ab[(c1 + 1) + (i64)N * c1] = beta;
-        if (c1 + 2 == N) {              // last reflector: T(N-1,N-1) as well
-            double an = Ab[(N - 1) + (i64)N * (N - 1)];
-            if (k >= 0) an -= 2.0 * vk[N - 1] * (pb[N - 1] + alpha * vk[N - 1]);
-            db[N - 1] = an;
-        }
-    }
-    __syncthreads();
-    const double scale = *sc;
-    for (int r = threadIdx.x; r < N; r += nt) {   // each element read and written by one thread
-        double v = 0.0;
-        if (r == c1 + 1) v = 1.0;
-        else if (r >= c1 + 2) {
-            double a = Ab[r + (i64)N * c1];
-            if (k >= 0) a -= vk[r] * wc1 + (pb[r] + alpha * vk[r]) * vc1;
-            v = a * scale;
-            Ab[r + (i64)N * c1] = v;
-        }
-        vn[r] = v;
-    }
-}
+#define TRD_NMAX 4096    // largest order (three vectors of it staged in LDS)
+#define TRD_PRE 8        // trailing-column rows per lane loaded in one pass
 
-// step k >= 0, phase 1, matrix b = blockIdx.z: every wavefront takes q trailing columns c
-// (rows and columns o = k+1 .. N-1), applies the pending rank-2 update of step k-1 to the
-// stored column and forms p_c = tau_k (A v_k)_c as the column's dot product with v_k (A is
-// symmetric, so there are no partial sums to combine).
-__global__ __launch_bounds__(TRD_WG) void k_trd_cols(double *A, int N, i64 sA, int k, int q, const double *vbuf,
-                                                    const double *wbuf, i64 sV, const double *tau, double *p) {
+// Launch k of the reduction, matrix b = blockIdx.z.  Every workgroup first repeats the
+// Householder step of column k (cheap: O(N - k) loads and two block sums), so no workgroup
+// ever waits for another:
+//   k >= 1: w_{k-1} = p_{k-1} - (tau_{k-1}/2)(p_{k-1}^T v_{k-1}) v_{k-1}, with
+//           p_{k-1} = tau_{k-1} A v_{k-1} from launch k-1; column k brought up to date with
+//           step k-1 (its stored values carry steps <= k-2);
+//   then    reflector v_k, tau_k from that column (d_k, e_k of T).
+// Then every wavefront takes q trailing columns c >= k+1 (rows >= k+1), applies step k-1 to
+// the stored column and forms p_k[c] = tau_k (A v_k)_c as the column's dot product with v_k.
+// Workgroup 0 publishes v_k, d_k, e_k, tau_k and writes column k-1 of A in the dsytrd 'L'
+// storage (column k is still being read by the other workgroups of this launch).
+__global__ __launch_bounds__(TRD_WG) void k_trd_col(double *A, int N, i64 sA, int k, int q, double *vbuf,
+                                                   double *pbuf, i64 sV, double *d, double *e, double *tau) {
     extern __shared__ double sh[];
-    const int b = blockIdx.z, o = k + 1, m = N - o;
+    __shared__ double red[TRD_WG / 64];
+    __shared__ double sc[4];
+    const int b = blockIdx.z, m0 = N - k;                         // rows k .. N-1
     double *Ab = A + b * sA;
-    const double *vk = vbuf + b * 2 * sV + (i64)(k & 1) * sV;          // v_k
-    const double *vp = vbuf + b * 2 * sV + (i64)((k + 1) & 1) * sV;    // v_{k-1}
-    const double *wp = wbuf + b * 2 * sV + (i64)((k + 1) & 1) * sV;    // w_{k-1}
+    double *db = d + (i64)b * N, *eb = e + (i64)b * N, *taub = tau + (i64)b * N;
+    const double *vp = vbuf + b * 2 * sV + (i64)((k + 1) & 1) * sV;   // v_{k-1}
+    double *vk_out = vbuf + b * 2 * sV + (i64)(k & 1) * sV;           // v_k
+    const double *pp = pbuf + b * 2 * sV + (i64)((k + 1) & 1) * sV;   // p_{k-1}
+    double *pk = pbuf + b * 2 * sV + (i64)(k & 1) * sV;               // p_k
+    double *svp = sh, *swp = sh + m0, *sv = sh + 2 * m0;              // v_{k-1}, w_{k-1}, v_k at row k + t
     const bool upd = k >= 1;
-    double *sv = sh, *svp = sh + m, *swp = sh + 2 * m;
-    for (int t = threadIdx.x; t < m; t += TRD_WG) {
-        sv[t] = vk[o + t];
-        svp[t] = upd ? vp[o + t] : 0.0;
-        swp[t] = upd ? wp[o + t] : 0.0;
+    const bool pub = blockIdx.x == 0;
+    const int m = m0 - 1;                                          // trailing columns / rows k+1 .. N-1
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the first rows of this wavefront's first trailing column, loaded before the Householder step
+    // so that both memory latencies overlap
+    const int cl0 = ((int)blockIdx.x * (TRD_WG / 64) + wave) * q;
+    double pre[TRD_PRE];
+#pragma unroll
+    for (int u = 0; u < TRD_PRE; u++) {
+        const int r = lane + 64 * u;
+        pre[u] = cl0 < m && r < m ? Ab[(i64)N * (k + 1 + cl0) + (k + 1) + r] : 0.0;
+    }
+    // ---- Householder step of column k, rows k..N-1 staged in LDS at index t = row - k:
+    // svp = v_{k-1}, swp = p_{k-1} then w_{k-1}, sv = column k then v_k
+    double s = 0.0;
+    for (int t = threadIdx.x; t < m0; t += TRD_WG) {
+        const double c = Ab[(k + t) + (i64)N * k];
+        const double v = upd ? vp[k + t] : 0.0, pv = upd ? pp[k + t] : 0.0;
+        svp[t] = v;
+        swp[t] = pv;
+        sv[t] = c;
+        s += pv * v;
+        if (t == 0) { sc[0] = v; sc[1] = pv; }
+    }
+    double alpha = 0.0;
+    if (upd) alpha = -0.5 * taub[k - 1] * trd_block_sum(s, red);
+    else __syncthreads();
+    const double vkk = sc[0], wkk = sc[1] + alpha * sc[0];         // v_{k-1}, w_{k-1} at row k
+    double xn = 0.0;
+    for (int t = threadIdx.x; t < m0; t += TRD_WG) {                // same t -> thread map as above
+        const double w = swp[t] + alpha * svp[t];
+        swp[t] = w;
+        double c = sv[t];
+        if (upd) c -= svp[t] * wkk + w * vkk;
+        sv[t] = c;
+        if (t >= 2) xn += c * c;
+    }
+    xn = trd_block_sum(xn, red);                                   // its barriers publish sv[0], sv[1]
+    const double a0 = sv[0], a1 = m0 > 1 ? sv[1] : 0.0;
+    double beta, tk, scale;
+    if (xn == 0.0) { beta = a1; tk = 0.0; scale = 0.0; }
+    else {
+        beta = -copysign(sqrt(a1 * a1 + xn), a1);
+        tk = (beta - a1) / beta;
+        scale = 1.0 / (a1 - beta);
+    }
+    __syncthreads();                                               // every thread has read sv[0], sv[1]
+    for (int t = threadIdx.x; t < m0; t += TRD_WG) {
+        const double v = t == 1 ? 1.0 : (t >= 2 ? sv[t] * scale : 0.0);
+        sv[t] = v;
+        if (pub) vk_out[k + t] = v;
+    }
+    if (pub) {
+        for (int r = threadIdx.x; r < k; r += TRD_WG) vk_out[r] = 0.0;
+        if (threadIdx.x == 0) {
+            db[k] = a0;
+            eb[k] = beta;
+            taub[k] = tk;
+            if (upd) {                                             // column k-1 in dsytrd 'L' storage
+                Ab[(k - 1) + (i64)N * (k - 1)] = db[k - 1];
+                Ab[k + (i64)N * (k - 1)] = eb[k - 1];
+            }
+        }
+        if (upd)
+            for (int t = 1 + threadIdx.x; t < m0; t += TRD_WG) Ab[(k + t) + (i64)N * (k - 1)] = vp[k + t];
     }
     __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const double tk = tau[(i64)b * N + k];
-    for (int j = 0; j < q; j++) {
-        const int cl = ((int)blockIdx.x * (TRD_WG / 64) + wave) * q + j;
+    // ---- trailing columns c = k+1 .. N-1 (local index cl = c - k - 1), rows k+1 .. N-1,
+    // 64*TRD_PRE rows per pass with all loads of a pass in flight together
+    for (int jq = 0; jq < q; jq++) {
+        const int cl = cl0 + jq;
         if (cl >= m) break;                                            // uniform per wavefront
-        double *col = Ab + (i64)N * (o + cl) + o;
-        const double vcl = svp[cl], wcl = swp[cl];
+        double *col = Ab + (i64)N * (k + 1 + cl) + (k + 1);
+        const double vcl = svp[1 + cl], wcl = swp[1 + cl];
         double acc = 0.0;
-        for (int r0 = lane; r0 < m; r0 += 64 * TRD_U) {
-            double a[TRD_U];
+        for (int r0 = 0; r0 < m; r0 += 64 * TRD_PRE) {
+            double a[TRD_PRE];
+            if (jq == 0 && r0 == 0) {
 #pragma unroll
-            for (int u = 0; u < TRD_U; u++) {
-                const int r = r0 + 64 * u;
-                a[u] = r < m ? col[r] : 0.0;
+                for (int u = 0; u < TRD_PRE; u++) a[u] = pre[u];
+            } else {
+#pragma unroll
+                for (int u = 0; u < TRD_PRE; u++) {
+                    const int r = r0 + lane + 64 * u;
+                    a[u] = r < m ? col[r] : 0.0;
+                }
             }
 #pragma unroll
-            for (int u = 0; u < TRD_U; u++) {
-                const int r = r0 + 64 * u;
+            for (int u = 0; u < TRD_PRE; u++) {
+                const int r = r0 + lane + 64 * u;
                 if (r < m) {
                     if (upd) {
-                        a[u] -= svp[r] * wcl + swp[r] * vcl;
+                        a[u] -= svp[1 + r] * wcl + swp[1 + r] * vcl;
                         col[r] = a[u];
                     }
-                    acc += a[u] * sv[r];
+                    acc += a[u] * sv[1 + r];
                 }
             }
         }
         acc = trd_wave_sum(acc);
-        if (lane == 0) p[b * sV + o + cl] = tk * acc;
+        if (lane == 0) pk[k + 1 + cl] = tk * acc;
     }
 }
 
-// step k, phase 2 (k = -1: the first reflector), one workgroup per matrix
-__global__ __launch_bounds__(TRD_HOUSE) void k_trd_house(double *A, int N, i64 sA, int k, double *vbuf, double *wbuf,
-                                                        i64 sV, const double *p, double *d, double *e,
-                                                        double *tau) {
-    __shared__ double red[TRD_HOUSE / 64];
-    __shared__ double sc;
+// after the last launch (k = N-2): T(N-1,N-1) and column N-2 of A
+__global__ void k_trd_last(double *A, int N, i64 sA, double *d, const double *e) {
     const int b = blockIdx.x;
-    trd_house(A + b * sA, N, k, vbuf + b * 2 * sV + (i64)(k & 1) * sV, wbuf + b * 2 * sV + (i64)(k & 1) * sV,
-              vbuf + b * 2 * sV + (i64)((k + 1) & 1) * sV, p + b * sV, d + (i64)b * N, e + (i64)b * N,
-              tau + (i64)b * N, red, &sc);
+    double *Ab = A + b * sA;
+    if (threadIdx.x) return;
+    d[(i64)b * N + N - 1] = Ab[(N - 1) + (i64)N * (N - 1)];
+    Ab[(N - 2) + (i64)N * (N - 2)] = d[(i64)b * N + N - 2];
+    Ab[(N - 1) + (i64)N * (N - 2)] = e[(i64)b * N + N - 2];
 }
 
 __global__ void k_trd_zero(double *x, i64 cnt) {
@@ -191,11 +204,11 @@ __global__ void k_trd_small(double *A, int N, i64 sA, double *d, double *e, doub
 
 static inline unsigned nblk(i64 tot, int bs) { return (unsigned)((tot + bs - 1) / bs); }
 
-// work: vbuf (2N per matrix), wbuf (2N), p (N), d, e, tau (N each)
-i64 trd_workspace(int N, int nb) { return (i64)8 * N * nb; }
+// work: vbuf and pbuf (2N per matrix each, double-buffered by launch parity), then d, e, tau
+i64 trd_workspace(int N, int nb) { return (i64)7 * N * nb; }
 TrdOut trd_outputs(int N, int nb, double *work) {
     TrdOut o;
-    o.d = work + (i64)5 * N * nb;
+    o.d = work + (i64)4 * N * nb;
     o.e = o.d + (i64)N * nb;
     o.tau = o.e + (i64)N * nb;
     return o;
@@ -203,22 +216,19 @@ TrdOut trd_outputs(int N, int nb, double *work) {
 
 static int trd_launches(hipStream_t st, int N, double *A, i64 sA, int nb, double *work) {
     const i64 sV = N;
-    double *vbuf = work, *wbuf = vbuf + (i64)nb * 2 * sV, *p = wbuf + (i64)nb * 2 * sV;
+    double *vbuf = work, *pbuf = vbuf + (i64)nb * 2 * sV;
     const TrdOut o = trd_outputs(N, nb, work);
-    hipLaunchKernelGGL(k_trd_zero, dim3(nblk((i64)5 * N * nb, 256)), dim3(256), 0, st, work, (i64)5 * N * nb);
+    hipLaunchKernelGGL(k_trd_zero, dim3(nblk((i64)4 * N * nb, 256)), dim3(256), 0, st, work, (i64)4 * N * nb);
     hipLaunchKernelGGL(k_trd_sym_upper, dim3(nblk((i64)N * N, 256), 1, nb), dim3(256), 0, st, A, N, sA);
-    hipLaunchKernelGGL(k_trd_house, dim3(nb), dim3(TRD_HOUSE), 0, st, A, N, sA, -1, vbuf, wbuf, sV, p, o.d, o.e,
-                       o.tau);
-    for (int k = 0; k <= N - 3; k++) {
-        const int m = N - k - 1;
-        int q = (int)(((i64)m * nb + 1535) / 1536);          // about 1536 wavefronts over the batch
+    for (int k = 0; k <= N - 2; k++) {
+        const int m = N - k - 1;                              // trailing columns
+        int q = (int)(((i64)m * nb + 1535) / 1536);           // about 1536 wavefronts over the batch
         if (q < 1) q = 1;
         const unsigned gx = nblk(m, (TRD_WG / 64) * q);
-        hipLaunchKernelGGL(k_trd_cols, dim3(gx, 1, nb), dim3(TRD_WG), sizeof(double) * 3 * m, st, A, N, sA, k, q,
-                           vbuf, wbuf, sV, o.tau, p);
-        hipLaunchKernelGGL(k_trd_house, dim3(nb), dim3(TRD_HOUSE), 0, st, A, N, sA, k, vbuf, wbuf, sV, p, o.d, o.e,
-                           o.tau);
+        hipLaunchKernelGGL(k_trd_col, dim3(gx, 1, nb), dim3(TRD_WG), sizeof(double) * 3 * (N - k), st, A, N, sA, k, q,
+                           vbuf, pbuf, sV, o.d, o.e, o.tau);
     }
+    hipLaunchKernelGGL(k_trd_last, dim3(nb), dim3(64), 0, st, A, N, sA, o.d, o.e);
     return 0;
 }
 
@@ -245,10 +255,9 @@ int trd_reduce(hipStream_t st, int N, double *A, i64 sA, int nb, double *work, T
         NV2_LAUNCH_CHECK();
         return 0;
     }
-    const int lds_max = 160 * 1024;   // the LDS of one CU
-    if (sizeof(double) * 3 * (size_t)N > (size_t)lds_max) { set_error("trd_reduce: N=%d exceeds the LDS staging", N); return -2; }
+    if (N > TRD_NMAX) { set_error("trd_reduce: order %d above %d", N, TRD_NMAX); return -2; }
     static const hipError_t attr =    // once per process (thread-safe static initialisation)
-        hipFuncSetAttribute((const void *)k_trd_cols, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+        hipFuncSetAttribute((const void *)k_trd_col, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
     NV2_HIP(attr);
     if (!plan) {
         TRYT(trd_launches(st, N, A, sA, nb, work));
