@@ -1,61 +1,51 @@
-// The O(n^9) part of bamat: four contractions of E4 = <E E E E> with the active
-// ERIs (koopro4QD.F90:4069-4094).  Each is a batched GEMM
-//     out[t, m, col] = sum_k E4[atoff[t] + amoff[m] + akoff[k]] * Bk[k, col]
-// with t = (cp,ap,bp) batches, m a pair of free E4 indices, k a triple of summed
-// indices and col the single free ERI index (N = n <= 16).  Index tables turn
-// the permuted tensor views into plain offsets.  Outputs land directly in
-// amat's layout (state fastest).
+// The O(n^9) part of bamat: four contractions of E4 = <E E E E> with the active ERIs
+// (koopro4QD.F90:4069-4094), on the matrix cores.
+//
+// With E4 stored x8 fastest (E4[x1..x8], offset sum x_p n^(8-p)) and cp = x1, ap = x2:
+//   C1(ap,bp,cp,a,b,c) = sum_{x5 x6 x7} E4[cp,ap,bp,b,x5,x6,x7,c] B1[(x5,x6,x7), a]
+//   C2(ap,bp,cp,a,b,c) = sum_{x5 x6 x8} E4[cp,ap,bp,b,x5,x6,a,x8] B2[(x5,x6,x8), c]
+//   C3(ap,bp,cp,a,b,c) = sum_{x4 x5 x6} E4[cp,ap,bp,x4,x5,x6,a,c] B3[(x4,x5,x6), b]
+//   C4(ap,bp,cp,a,b,c) = sum_{x3 x5 x6} E4[cp,ap,x3,bp,x5,x6,a,c] B3[(x3,x5,x6), b]
+// (B1[(d,j,i),a] = w(d,j,i,a), B2[(d,j,i),c] = w(d,j,c,i), B3[(i,d,j),b] = w(d,j,b,i)).
+//
+// Every index runs over n <= 16 active orbitals, so each contraction is a set of 16x16 MFMA
+// tiles.  Pass A (one workgroup per (cp,ap,bp), four wavefronts) streams the contiguous
+// sub-slab E4[cp,ap,bp,:,:,:,:,:] (n^5 values) once through LDS in chunks (x4,x5) of n^3
+// values and feeds C1, C2 and C3 from each chunk (one wavefront each for C1 and C2, two for
+// C3); pass B (one
+// workgroup per (cp,ap,bp'), four wavefronts) streams E4[cp,ap,:,bp',...] for C4, whose sum
+// runs over x3.  E4 is read twice instead of four times.  FP64 uses v_mfma_f64_16x16x4f64;
+// the FP32 form stores E4 in float and uses v_mfma_f32_16x16x4f32 with FP32 accumulation.
+// Outputs land directly in amat's layout (state fastest).
 #include "nv2_common.h"
 #include "nv2_stage1.h"
 
 namespace nv2 {
 
-struct ContractPlan {
-    int n, M;
-    i64 *atoff[4], *amoff[4], *akoff[4], *otoff[4], *omoff[4], *ocoff[4];
-    double *Bk[4];
+typedef double mf_d4 __attribute__((ext_vector_type(4)));
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+
+// A operand of lane l is A[l % 16][l / 16], B operand B[l / 16][l % 16] (one K step of 4);
+// accumulator register r of lane l holds D[row(l, r)][l % 16]
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef mf_d4 V;
+    __device__ static V mma(double a, double b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct Mfma<float> {
+    typedef mf_f4 V;
+    __device__ static V mma(float a, float b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-static const int KC = 64;   // K chunk staged in LDS
+struct ContractPlan {
+    int n, M;
+    double *Bk[3];
+};
 
-template <typename TA, typename TC, int NN>
-__global__ __launch_bounds__(256) void k_contract(const TA *__restrict__ A, const i64 *__restrict__ atoff,
-                                                  const i64 *__restrict__ amoff, const i64 *__restrict__ akoff,
-                                                  const double *__restrict__ Bm, int K, int Mr, i64 rows,
-                                                  const i64 *__restrict__ otoff, const i64 *__restrict__ omoff,
-                                                  const i64 *__restrict__ ocoff, int M, int s,
-                                                  double *__restrict__ out) {
-    __shared__ TC Bs[KC * NN];
-    __shared__ i64 ks[KC];
-    const i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = r < rows;
-    const i64 t = valid ? r / Mr : 0, m = valid ? r % Mr : 0;
-    const i64 base = valid ? atoff[t] + amoff[m] : 0;
-    TC acc[NN];
-#pragma unroll
-    for (int c = 0; c < NN; c++) acc[c] = (TC)0;
-    for (int k0 = 0; k0 < K; k0 += KC) {
-        const int kc = min(KC, K - k0);
-        for (int e = threadIdx.x; e < kc * NN; e += blockDim.x) Bs[e] = (TC)Bm[(i64)k0 * NN + e];
-        for (int e = threadIdx.x; e < kc; e += blockDim.x) ks[e] = akoff[k0 + e];
-        __syncthreads();
-        if (valid) {
-            for (int kk = 0; kk < kc; kk++) {
-                const TC a = (TC)A[base + ks[kk]];
-#pragma unroll
-                for (int c = 0; c < NN; c++) acc[c] += a * Bs[kk * NN + c];
-            }
-        }
-        __syncthreads();
-    }
-    if (valid) {
-        const i64 ob = otoff[t] + omoff[m];
-#pragma unroll
-        for (int c = 0; c < NN; c++) out[s + (i64)M * (ob + ocoff[c])] = (double)acc[c];
-    }
-}
-
-// B matrices from atwo (Fortran layout): B1[(d,j,i),a]=w(d,j,i,a) B2[(d,j,i),c]=w(d,j,c,i) B3[(i,d,j),b]=w(d,j,b,i)
+// B matrices from atwo (Fortran layout): B1[(d,j,i),a]=w(d,j,i,a) B2[(d,j,i),c]=w(d,j,c,i) B3[(i,d,j),b]=w(d,j,b,i),
+// row-major [k][col]
 __global__ void k_make_bk(int n, const double *atwo, double *B1, double *B2, double *B3) {
     const i64 tot = (i64)n * n * n * n;
     i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -70,92 +60,226 @@ __global__ void k_make_bk(int n, const double *atwo, double *B1, double *B2, dou
 #undef W
 }
 
+template <typename V> __device__ inline V vzero() { V z = {0, 0, 0, 0}; return z; }
+
+// pass A: workgroup g = (cp*n + ap)*n + bp; wavefront 0 -> C1, 1 -> C2, 2 and 3 -> C3 (row
+// tiles split between them).  Per chunk (x4,x5) the E4 values and the three B slices the chunk
+// needs are staged together in LDS, so every MFMA operand comes from LDS.
+template <typename T, int NN>
+__global__ __launch_bounds__(256) void k_e4_pass_a(const T *__restrict__ E4, const double *__restrict__ B1,
+                                                   const double *__restrict__ B2, const double *__restrict__ B3,
+                                                   int M, int s, double *C1, double *C2, double *C3) {
+    constexpr int N2 = NN * NN, N3 = N2 * NN;
+    constexpr int KA = (N2 + 3) / 4;          // K steps of C1 / C2 per chunk (K = n^2)
+    constexpr int KC = (NN + 3) / 4;          // K steps of C3 per chunk (K = n)
+    constexpr int T3 = (N2 + 15) / 16;        // row tiles of C3 (rows (a,c))
+    constexpr int TW = (T3 + 1) / 2;          // C3 tiles per wavefront (2 and 3)
+    typedef typename Mfma<T>::V V;
+    __shared__ T ech[N3], b1s[N3], b2s[N3];
+    __shared__ T b3s[N2];
+    const int g = blockIdx.x;
+    const int bp = g % NN, ap = (g / NN) % NN, cp = g / N2;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const T *slab = E4 + (i64)g * N2 * N3;   // E4[cp,ap,bp,...]
+    const i64 obase = ap + (i64)NN * bp + (i64)N2 * cp;
+    V acc3[TW];
+#pragma unroll
+    for (int u = 0; u < TW; u++) acc3[u] = vzero<V>();
+    // the E4 chunk (x4,x5) is chunk x4*n + x5 of the contiguous sub-slab; the next one is loaded
+    // into registers while the current one is multiplied, so its HBM latency is hidden
+    constexpr int PF = (N3 + 255) / 256;
+    T pf[PF];
+#pragma unroll
+    for (int u = 0; u < PF; u++) {
+        const int e = threadIdx.x + 256 * u;
+        pf[u] = e < N3 ? slab[e] : (T)0;
+    }
+    for (int x4 = 0; x4 < NN; x4++) {
+        V acc = vzero<V>();                    // C1[x4] (wavefront 0) or C2[x4] (wavefront 1)
+        for (int x5 = 0; x5 < NN; x5++) {
+            __syncthreads();                   // the previous chunk is consumed
+#pragma unroll
+            for (int u = 0; u < PF; u++) {
+                const int e = threadIdx.x + 256 * u;
+                if (e < N3) ech[e] = pf[u];
+            }
+            for (int e = threadIdx.x; e < N3; e += 256) {
+                b1s[e] = (T)B1[(i64)x5 * N3 + e];
+                b2s[e] = (T)B2[(i64)x5 * N3 + e];
+            }
+            for (int e = threadIdx.x; e < N2; e += 256) b3s[e] = (T)B3[((i64)x4 * N2 + x5 * NN) * NN + e];
+            __syncthreads();
+            if (x4 * NN + x5 + 1 < N2) {       // prefetch chunk x4*n + x5 + 1
+                const T *nxt = slab + (i64)(x4 * NN + x5 + 1) * N3;
+#pragma unroll
+                for (int u = 0; u < PF; u++) {
+                    const int e = threadIdx.x + 256 * u;
+                    pf[u] = e < N3 ? nxt[e] : (T)0;
+                }
+            }
+            if (wave == 0) {                   // C1[x4][x8][a] += E[x6,x7,x8] B1[(x5,x6,x7),a], K = (x6,x7)
+#pragma unroll 5
+                for (int ks = 0; ks < KA; ks++) {
+                    const int kk = 4 * ks + lk;
+                    const bool ok = li < NN && kk < N2;
+                    const T a = ok ? ech[kk * NN + li] : (T)0;
+                    const T b = ok ? b1s[kk * NN + li] : (T)0;
+                    acc = Mfma<T>::mma(a, b, acc);
+                }
+            } else if (wave == 1) {            // C2[x4][x7][c] += E[x6,x7,x8] B2[(x5,x6,x8),c], K = (x6,x8)
+#pragma unroll 5
+                for (int ks = 0; ks < KA; ks++) {
+                    const int kk = 4 * ks + lk;
+                    const bool ok = li < NN && kk < N2;
+                    const T a = ok ? ech[(kk / NN) * N2 + li * NN + kk % NN] : (T)0;
+                    const T b = ok ? b2s[kk * NN + li] : (T)0;
+                    acc = Mfma<T>::mma(a, b, acc);
+                }
+            } else {                           // C3[(x7,x8)][b] += E[x6,x7,x8] B3[(x4,x5,x6),b], K = x6
+#pragma unroll
+                for (int ks = 0; ks < KC; ks++) {
+                    const int x6 = 4 * ks + lk;
+                    const T b = x6 < NN && li < NN ? b3s[x6 * NN + li] : (T)0;
+#pragma unroll
+                    for (int u = 0; u < TW; u++) {
+                        const int r = 16 * (2 * u + wave - 2) + li;
+                        const T a = x6 < NN && r < N2 ? ech[x6 * N2 + r] : (T)0;
+                        acc3[u] = Mfma<T>::mma(a, b, acc3[u]);
+                    }
+                }
+            }
+        }
+        if (wave < 2) {                        // rows x8 (C1) / x7 (C2), columns a (C1) / c (C2)
+            const int j = li;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = Mfma<T>::row(lane, r);
+                if (i < NN && j < NN) {
+                    if (wave == 0)
+                        C1[s + (i64)M * (obase + (i64)N3 * j + (i64)N2 * N2 * x4 + (i64)N2 * N3 * i)] = (double)acc[r];
+                    else
+                        C2[s + (i64)M * (obase + (i64)N3 * i + (i64)N2 * N2 * x4 + (i64)N2 * N3 * j)] = (double)acc[r];
+                }
+            }
+        }
+    }
+    if (wave >= 2) {                           // rows (a,c) = (x7,x8), columns b
+        const int j = li;
+#pragma unroll
+        for (int u = 0; u < TW; u++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int rr = 16 * (2 * u + wave - 2) + Mfma<T>::row(lane, r);
+                if (rr < N2 && j < NN) {
+                    const int a = rr / NN, c = rr % NN;
+                    C3[s + (i64)M * (obase + (i64)N3 * a + (i64)N2 * N2 * j + (i64)N2 * N3 * c)] = (double)acc3[u][r];
+                }
+            }
+    }
+}
+
+// pass B: workgroup g = (cp*n + ap)*n + bp; C4[(x7,x8)][b] = sum over chunks (x3,x5) and x6; the
+// row tiles are spread over the four wavefronts
+template <typename T, int NN>
+__global__ __launch_bounds__(256) void k_e4_pass_b(const T *__restrict__ E4, const double *__restrict__ B3, int M,
+                                                   int s, double *C4) {
+    constexpr int N2 = NN * NN, N3 = N2 * NN;
+    constexpr int KC = (NN + 3) / 4;
+    constexpr int T3 = (N2 + 15) / 16;
+    constexpr int TW = (T3 + 3) / 4;           // tiles per wavefront
+    typedef typename Mfma<T>::V V;
+    __shared__ T ech[N3];
+    __shared__ T b3s[N2];
+    const int g = blockIdx.x;
+    const int bp = g % NN, ap = (g / NN) % NN, cp = g / N2;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const T *slab = E4 + (i64)(g / NN) * N3 * N3 + (i64)bp * N2 * N2;    // E4[cp,ap,0,bp,...]
+    const i64 obase = ap + (i64)NN * bp + (i64)N2 * cp;
+    V acc[TW];
+#pragma unroll
+    for (int u = 0; u < TW; u++) acc[u] = vzero<V>();
+    constexpr int PF = (N3 + 255) / 256;       // next chunk prefetched into registers
+    T pf[PF];
+#pragma unroll
+    for (int u = 0; u < PF; u++) {
+        const int e = threadIdx.x + 256 * u;
+        pf[u] = e < N3 ? slab[e] : (T)0;
+    }
+    for (int x3 = 0; x3 < NN; x3++)
+        for (int x5 = 0; x5 < NN; x5++) {
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < PF; u++) {
+                const int e = threadIdx.x + 256 * u;
+                if (e < N3) ech[e] = pf[u];
+            }
+            for (int e = threadIdx.x; e < N2; e += 256) b3s[e] = (T)B3[((i64)x3 * N2 + x5 * NN) * NN + e];
+            __syncthreads();
+            {
+                const int nx5 = x5 + 1 < NN ? x5 + 1 : 0, nx3 = x5 + 1 < NN ? x3 : x3 + 1;
+                if (nx3 < NN) {
+                    const T *nxt = slab + (i64)nx3 * N2 * N3 + (i64)nx5 * N3;
+#pragma unroll
+                    for (int u = 0; u < PF; u++) {
+                        const int e = threadIdx.x + 256 * u;
+                        pf[u] = e < N3 ? nxt[e] : (T)0;
+                    }
+                }
+            }
+#pragma unroll
+            for (int ks = 0; ks < KC; ks++) {
+                const int x6 = 4 * ks + lk;
+                const T b = x6 < NN && li < NN ? b3s[x6 * NN + li] : (T)0;
+#pragma unroll
+                for (int u = 0; u < TW; u++) {
+                    const int r = 16 * (wave + 4 * u) + li;
+                    const T a = x6 < NN && r < N2 ? ech[x6 * N2 + r] : (T)0;
+                    acc[u] = Mfma<T>::mma(a, b, acc[u]);
+                }
+            }
+        }
+    const int j = li;
+#pragma unroll
+    for (int u = 0; u < TW; u++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int rr = 16 * (wave + 4 * u) + Mfma<T>::row(lane, r);
+            if (rr < N2 && j < NN) {
+                const int a = rr / NN, c = rr % NN;
+                C4[s + (i64)M * (obase + (i64)N3 * a + (i64)N2 * N2 * j + (i64)N2 * N3 * c)] = (double)acc[u][r];
+            }
+        }
+}
+
 ContractPlan *contract_plan_create(int n, int M) {
     ContractPlan *p = new ContractPlan();
     p->n = n; p->M = M;
-    const i64 N = n, n2 = N * N, n3 = n2 * N;
-    i64 s[9];
-    s[8] = 1;
-    for (int k = 7; k >= 1; k--) s[k] = s[k + 1] * N;    // s[k] = n^(8-k)
-    std::vector<i64> at(n3), am(n2), ak(n3), ot(n3), om(n2), oc(N);
-    auto up = [](std::vector<i64> &h) {
-        i64 *d = nullptr;
-        if (hipMalloc(&d, sizeof(i64) * h.size()) != hipSuccess) return (i64 *)nullptr;
-        hipMemcpy(d, h.data(), sizeof(i64) * h.size(), hipMemcpyHostToDevice);
-        return d;
-    };
-    for (int c = 0; c < 4; c++) {
-        for (i64 R = 0; R < N; R++)
-            for (i64 P = 0; P < N; P++)
-                for (i64 Q = 0; Q < N; Q++) {
-                    i64 t = (R * N + P) * N + Q;
-                    at[t] = R * s[1] + P * s[2] + (c == 3 ? Q * s[4] : Q * s[3]);
-                    ot[t] = P + N * Q + n2 * R;
-                }
-        for (i64 x = 0; x < N; x++)
-            for (i64 y = 0; y < N; y++) {
-                i64 m = x * N + y;
-                switch (c) {
-                    case 0: am[m] = x * s[4] + y * s[8]; om[m] = n2 * n2 * x + n2 * n3 * y; break;  // (b,c)
-                    case 1: am[m] = x * s[4] + y * s[7]; om[m] = n2 * n2 * x + n3 * y; break;       // (b,a)
-                    default: am[m] = x * s[7] + y * s[8]; om[m] = n3 * x + n2 * n3 * y; break;      // (a,c)
-                }
-            }
-        for (i64 x0 = 0; x0 < N; x0++)
-            for (i64 x1 = 0; x1 < N; x1++)
-                for (i64 x2 = 0; x2 < N; x2++) {
-                    i64 k = (x0 * N + x1) * N + x2;
-                    switch (c) {
-                        case 0: ak[k] = x0 * s[5] + x1 * s[6] + x2 * s[7]; break;   // (d,j,i)
-                        case 1: ak[k] = x0 * s[5] + x1 * s[6] + x2 * s[8]; break;   // (d,j,i), i at x8
-                        case 2: ak[k] = x0 * s[4] + x1 * s[5] + x2 * s[6]; break;   // (i,d,j), i at x4
-                        default: ak[k] = x0 * s[3] + x1 * s[5] + x2 * s[6]; break;  // (i,d,j), i at x3
-                    }
-                }
-        for (i64 col = 0; col < N; col++) {
-            switch (c) {
-                case 0: oc[col] = n3 * col; break;         // a
-                case 1: oc[col] = n2 * n3 * col; break;    // c
-                default: oc[col] = n2 * n2 * col; break;   // b
-            }
-        }
-        p->atoff[c] = up(at); p->amoff[c] = up(am); p->akoff[c] = up(ak);
-        p->otoff[c] = up(ot); p->omoff[c] = up(om); p->ocoff[c] = up(oc);
-    }
-    for (int c = 0; c < 3; c++) {
-        if (hipMalloc(&p->Bk[c], sizeof(double) * n3 * N) != hipSuccess) p->Bk[c] = nullptr;
-    }
-    p->Bk[3] = p->Bk[2];
+    const i64 n4 = (i64)n * n * n * n;
+    for (int c = 0; c < 3; c++)
+        if (hipMalloc(&p->Bk[c], sizeof(double) * n4) != hipSuccess) p->Bk[c] = nullptr;
     return p;
 }
 
 void contract_plan_destroy(ContractPlan *p) {
     if (!p) return;
-    for (int c = 0; c < 4; c++) {
-        hipFree(p->atoff[c]); hipFree(p->amoff[c]); hipFree(p->akoff[c]);
-        hipFree(p->otoff[c]); hipFree(p->omoff[c]); hipFree(p->ocoff[c]);
-    }
-    for (int c = 0; c < 3; c++) hipFree(p->Bk[c]);
+    for (int c = 0; c < 3; c++) (void)hipFree(p->Bk[c]);
     delete p;
 }
 
-template <typename TA, typename TC, int NN>
-static int run4(ContractPlan *p, const TA *E4, int s, double **outs, hipStream_t st) {
-    const i64 n = p->n, rows = n * n * n * n * n;
-    const int bs = 256;
-    for (int c = 0; c < 4; c++) {
-        hipLaunchKernelGGL((k_contract<TA, TC, NN>), dim3((unsigned)((rows + bs - 1) / bs)), dim3(bs), 0, st, E4,
-                           p->atoff[c], p->amoff[c], p->akoff[c], p->Bk[c], (int)(n * n * n), (int)(n * n), rows,
-                           p->otoff[c], p->omoff[c], p->ocoff[c], p->M, s, outs[c]);
-        NV2_LAUNCH_CHECK();
-    }
+template <typename T, int NN>
+static int run_passes(ContractPlan *p, const T *E4, int s, double **outs, hipStream_t st) {
+    const unsigned nwg = NN * NN * NN;
+    hipLaunchKernelGGL((k_e4_pass_a<T, NN>), dim3(nwg), dim3(256), 0, st, E4, p->Bk[0], p->Bk[1], p->Bk[2], p->M, s,
+                       outs[0], outs[1], outs[2]);
+    hipLaunchKernelGGL((k_e4_pass_b<T, NN>), dim3(nwg), dim3(256), 0, st, E4, p->Bk[2], p->M, s, outs[3]);
+    NV2_LAUNCH_CHECK();
     return 0;
 }
 
-template <typename TA, typename TC>
-static int dispatch(ContractPlan *p, const TA *E4, int s, double **outs, hipStream_t st) {
+template <typename T>
+static int dispatch(ContractPlan *p, const T *E4, int s, double **outs, hipStream_t st) {
     switch (p->n) {
-#define CASE(k) case k: return run4<TA, TC, k>(p, E4, s, outs, st);
+#define CASE(k) case k: return run_passes<T, k>(p, E4, s, outs, st);
         CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
         CASE(14) CASE(15) CASE(16)
 #undef CASE
@@ -165,13 +289,14 @@ static int dispatch(ContractPlan *p, const TA *E4, int s, double **outs, hipStre
 
 int launch_e4_contractions(ContractPlan *p, const void *E4, int prec, const double *atwo, int s, double *C1,
                            double *C2, double *C3, double *C4, hipStream_t st) {
+    if (!p->Bk[0] || !p->Bk[1] || !p->Bk[2]) { set_error("E4 contraction: B matrices not allocated"); return -1; }
     const i64 n = p->n, n4 = n * n * n * n;
     hipLaunchKernelGGL(k_make_bk, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, st, p->n, atwo, p->Bk[0],
                        p->Bk[1], p->Bk[2]);
     NV2_LAUNCH_CHECK();
     double *outs[4] = {C1, C2, C3, C4};
-    if (prec == 0) return dispatch<double, double>(p, (const double *)E4, s, outs, st);
-    return dispatch<float, float>(p, (const float *)E4, s, outs, st);
+    if (prec == 0) return dispatch<double>(p, (const double *)E4, s, outs, st);
+    return dispatch<float>(p, (const float *)E4, s, outs, st);
 }
 
 }  // namespace nv2
