@@ -114,6 +114,39 @@ void nv2_ctx_destroy(nv2_ctx *ctx);
  * real.  metat <= 16. */
 int nv2_qd_heff(int64_t metat, const double *heff2, const double *e0, double *heff, double *energies);
 
+/* Request loop (no counterpart in the reference, whose drivers evaluate one
+ * problem per process run: koopro4QD_main.F90:71, qdnevpt_main.F90:75).  A
+ * server runs `slots` evaluations concurrently on one GPU; requests are admitted
+ * first-in first-out when a slot is free and their device memory fits the HBM
+ * budget (<= 0: 80% of the free memory), so requests arriving while others run
+ * overlap with them on the device.  Contexts are kept resident per problem shape
+ * and reused by later requests of that shape.  A host-data request points at
+ * arrays in the nv2_ctx_set_* layouts, which must stay valid until its wait
+ * returns; out (32*metat) and, for metat > 1 with transition RDMs, heff
+ * (Heff-SC then Heff-PC, 2*metat^2) are filled on completion.
+ * nv2_server_submit_ctx runs an existing context whose inputs are resident
+ * (the caller keeps it, and must not run it elsewhere meanwhile).  Submit
+ * returns a request id (> 0) or a negative status; nv2_server_wait blocks until
+ * the request is done and returns its status, with the submit-to-completion
+ * latency and the time spent queued.  Destroy drains the queue first. */
+typedef struct nv2_request {
+    int64_t ncore, nact, nvirt, metat, nele, nchovec;
+    const double *cholesky, *oneint, *ro4;
+    const double *trdm3, *trdm2, *trdm1;   /* NULL for one state or no QD couplings */
+    int32_t prec;
+    double thr_lindep;
+    double *out;
+    double *heff;                           /* NULL when not wanted */
+} nv2_request;
+typedef struct nv2_server nv2_server;
+nv2_server *nv2_server_create(int32_t slots, int64_t hbm_budget_bytes);
+int64_t nv2_server_submit(nv2_server *srv, const nv2_request *req);
+int64_t nv2_server_submit_ctx(nv2_server *srv, nv2_ctx *ctx, int32_t prec, double thr_lindep, double *out);
+int nv2_server_wait(nv2_server *srv, int64_t id, double *latency_ms, double *queue_ms);
+void nv2_server_destroy(nv2_server *srv);
+/* device bytes one evaluation context of this shape holds (the admission estimate) */
+int64_t nv2_ctx_bytes(int64_t ncore, int64_t nact, int64_t nvirt, int64_t metat, int64_t nchovec);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,17 @@ def _declare(L):
     L.nv2_ctx_get.argtypes = [C.c_void_p, C.c_char_p, dp, i64]
     L.nv2_ctx_destroy.argtypes = [C.c_void_p]
     L.nv2_ctx_destroy.restype = None
+    L.nv2_server_create.restype = C.c_void_p
+    L.nv2_server_create.argtypes = [i32, i64]
+    L.nv2_server_submit.restype = i64
+    L.nv2_server_submit.argtypes = [C.c_void_p, C.c_void_p]
+    L.nv2_server_submit_ctx.restype = i64
+    L.nv2_server_submit_ctx.argtypes = [C.c_void_p, C.c_void_p, i32, C.c_double, dp]
+    L.nv2_server_wait.argtypes = [C.c_void_p, i64, dp, dp]
+    L.nv2_server_destroy.argtypes = [C.c_void_p]
+    L.nv2_server_destroy.restype = None
+    L.nv2_ctx_bytes.restype = i64
+    L.nv2_ctx_bytes.argtypes = [i64] * 5
 
 
 def last_error() -> str:
