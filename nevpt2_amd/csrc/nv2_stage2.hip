@@ -22,6 +22,7 @@
 #include "nv2_common.h"
 #include "nv2_stage2.h"
 #include "nv2_trd.h"
+#include "../../include/nevpt2_amd.h"
 
 namespace nv2 {
 
@@ -1066,7 +1067,6 @@ struct Stage2 {
     int *hinfo = nullptr;          // pinned host copies of solver info words
     int *dinfo_w = nullptr;        // device info words, 2 per solve slot
     double *wE[NW] = {};           // syevd workspace per worker
-    TrdPlan *trdplan[NW] = {};     // captured tridiagonalisation of worker w (touched by that worker only)
     int nc, n, nv, norb, L, M;
     std::map<std::string, Buf> bufs;
     int *dinfo = nullptr;
@@ -1282,8 +1282,10 @@ static int qd_reduce(Stage2 *p, const QDArgs &q, i64 nblocks, int cls) {
 struct PcJob {
     std::string tag;
     int N = 0, nb = 1, w = 0, slot = 0;        // info words at 4*slot + {metric b, K' 2+b}
-    bool trd = false;                          // K' only tridiagonalised (nv2_trd.hip), no eigenvectors
-    double *trdw = nullptr;                    // its scratch; T and tau of problem b from trd_outputs(nmax)
+    bool trd = false;                          // nv2_trd.hip: metric by reduction + dstedc, K' only reduced
+    double *trdw = nullptr;                    // reduction scratch; T and tau of problem b from trd_outputs(nmax)
+    double *trdm = nullptr, *trde = nullptr;   // metric: reduction scratch, off-diagonal copy for dstedc
+    double *trdq = nullptr;                    // scratch of applying Q
     double *S = nullptr, *K = nullptr;         // nb consecutive N x N matrices; S is overwritten
     double *w1 = nullptr, *hw1 = nullptr, *coef = nullptr, *w2 = nullptr;
     double *caux = nullptr, *Ks = nullptr, *T = nullptr, *Kp = nullptr;   // nb x N x N scratch each
@@ -1305,18 +1307,45 @@ static int pc_job_buffers(Stage2 *p, PcJob &j, size_t hoff) {
     j.hw1 = p->hpin + hoff;
     if (j.trd) {
         GET(tw, (j.tag + "_trdw").c_str(), trd_workspace(j.N, j.nb));
-        j.trdw = tw;
+        GET(tm, (j.tag + "_trdm").c_str(), trd_workspace(j.N, j.nb));
+        GET(te, (j.tag + "_trde").c_str(), (i64)j.N * j.nb);
+        GET(tq, (j.tag + "_trdq").c_str(), trd_apply_workspace(j.N, j.nb, j.N));
+        j.trdw = tw; j.trdm = tm; j.trde = te; j.trdq = tq;
     }
     for (int k = 0; k < 4; k++) p->hinfo[4 * j.slot + k] = 0;
     return 0;
 }
-// K' = Q T Q^T of a trd job; the launch sequence is captured once per shape and buffer set
-// (one plan per worker: a repeated evaluation of one problem replays it)
-static int pc_trd_launch(Stage2 *p, PcJob &j) {
-    return trd_reduce(p->ws[j.w], j.nmax, j.Kp, j.NN(), j.nb, j.trdw, &p->trdplan[j.w]);
+// K' = Q T Q^T of a trd job
+static int pc_trd_launch(Stage2 *p, PcJob &j) { return trd_reduce(p->ws[j.w], j.nmax, j.Kp, j.NN(), j.nb, j.trdw); }
+__global__ void k_identity_batched(double *Z, int N, i64 sZ) {
+    const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < (i64)N * N) Z[blockIdx.z * sZ + e] = (e % N) == (e / N) ? 1.0 : 0.0;
+}
+// metric eigen-decomposition of a trd job as dsyevd('V','U') leaves it (eigenvectors in S,
+// ascending eigenvalues in w1): S = Q T Q^T (nv2_trd.hip), T = Z W Z^T (rocsolver_dstedc on the
+// tridiagonal matrix), U = Q Z with Q applied in compact WY form.  Uses j.T as Z.
+static int pc_trd_metric(Stage2 *p, PcJob &j) {
+    hipStream_t st = p->ws[j.w];
+    rocblas_handle h = p->wh[j.w];
+    const int N = j.N;
+    const i64 NN = j.NN();
+    TRYS(trd_reduce(st, N, j.S, NN, j.nb, j.trdm));
+    const TrdOut o = trd_outputs(N, j.nb, j.trdm);
+    NV2_HIP(hipMemcpyAsync(j.w1, o.d, sizeof(double) * N * j.nb, hipMemcpyDeviceToDevice, st));
+    NV2_HIP(hipMemcpyAsync(j.trde, o.e, sizeof(double) * N * j.nb, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_identity_batched, dim3(nb(NN, 256), 1, j.nb), dim3(256), 0, st, j.T, N, NN);
+    NV2_LAUNCH_CHECK();
+    for (int b = 0; b < j.nb; b++)
+        RB(rocsolver_dstedc(h, rocblas_evect_tridiagonal, N, j.w1 + (i64)b * N, j.trde + (i64)b * N, j.T + b * NN, N,
+                            p->dinfo_w + 4 * j.slot + b));
+    TRYS(trd_apply_q(h, st, N, j.S, NN, j.nb, o.tau, false, N, j.T, NN, j.trdq));
+    NV2_HIP(hipMemcpyAsync(j.S, j.T, sizeof(double) * NN * j.nb, hipMemcpyDeviceToDevice, st));
+    NV2_HIP(hipMemcpyAsync(p->hinfo + 4 * j.slot, p->dinfo_w + 4 * j.slot, sizeof(int) * j.nb, hipMemcpyDeviceToHost, st));
+    return 0;
 }
 static int pc_metric_launch(Stage2 *p, PcJob &j) {
-    TRYS(syevd_async(p, j.w, 4 * j.slot, true, j.N, j.S, j.NN(), j.w1, j.N, j.nb));
+    if (j.trd && j.N >= 3 && eigen_algo() == 0) TRYS(pc_trd_metric(p, j));
+    else TRYS(syevd_async(p, j.w, 4 * j.slot, true, j.N, j.S, j.NN(), j.w1, j.N, j.nb));
     if (j.N) NV2_HIP(hipMemcpyAsync(j.hw1, j.w1, sizeof(double) * j.N * j.nb, hipMemcpyDeviceToHost, p->ws[j.w]));
     return 0;
 }
@@ -1428,7 +1457,6 @@ void stage2_destroy(Stage2 *p) {
     for (int w = 0; w < NW; w++) {
         if (p->wh[w]) rocblas_destroy_handle(p->wh[w]);
         if (p->ws[w]) (void)hipStreamDestroy(p->ws[w]);
-        trd_plan_destroy(p->trdplan[w]);
     }
     for (int e = 0; e < NW + 1; e++) if (p->wev[e]) (void)hipEventDestroy(p->wev[e]);
     (void)hipFree(p->pairs_c); (void)hipFree(p->pairs_cs); (void)hipFree(p->pairs_v); (void)hipFree(p->pairs_vs);
@@ -1617,13 +1645,17 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
     NV2_HIP(hipEventRecord(p->wev[NW], st));
     for (int w = 0; w < NW; w++) NV2_HIP(hipStreamWaitEvent(p->ws[w], p->wev[NW], 0));
     int wrc[NW] = {};
+    char werr[NW][512] = {};        // a worker's error message (the message buffer is per thread)
     std::vector<std::thread> workers;
     struct Joiner {     // every exit from here on waits for the worker threads first
         std::vector<std::thread> &v;
         ~Joiner() { for (auto &t : v) if (t.joinable()) t.join(); }
     } joiner{workers};
     for (int w = 0; w < NW; w++)
-        workers.emplace_back([p, &J, w, thr, &wrc]() { wrc[w] = pc_worker(p, J, NJOB, w, thr); });
+        workers.emplace_back([p, &J, w, thr, &wrc, &werr]() {
+            wrc[w] = pc_worker(p, J, NJOB, w, thr);
+            if (wrc[w]) nv2_last_error(werr[w], sizeof(werr[w]));
+        });
     auto join_workers = [&]() { for (auto &t : workers) if (t.joinable()) t.join(); };
 
     // ---- Fock operators and canonical orbitals (qdnevpt.F:369-698)
@@ -1852,7 +1884,8 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
 
     // ================================================================ join the PC solves
     join_workers();
-    for (int w = 0; w < NW; w++) if (wrc[w]) return wrc[w];
+    for (int w = 0; w < NW; w++)
+        if (wrc[w]) { set_error("PC solve worker %d: %s", w, werr[w]); return wrc[w]; }
     for (int w = 0; w < NW; w++) {
         NV2_HIP(hipEventRecord(p->wev[w], p->ws[w]));
         NV2_HIP(hipStreamWaitEvent(st, p->wev[w], 0));

@@ -13,12 +13,9 @@ struct TrdOut {
 // scratch of trd_reduce, in doubles; d, e, tau live inside it (trd_outputs)
 i64 trd_workspace(int N, int nb);
 TrdOut trd_outputs(int N, int nb, double *work);
-struct TrdPlan;         // captured launch sequence of one shape and buffer set
-void trd_plan_destroy(TrdPlan *pl);
 // A_b = Q_b T_b Q_b^T for the nb matrices A + b*sA (ld N), from their upper triangles;
-// reflectors left below the subdiagonal as dsytrd('L') does.  With a plan, the launches are
-// captured into a hipGraph on the first call (or when the shape or buffers change) and replayed.
-int trd_reduce(hipStream_t st, int N, double *A, i64 sA, int nb, double *work, TrdPlan **plan = nullptr);
+// reflectors left below the subdiagonal as dsytrd('L') does
+int trd_reduce(hipStream_t st, int N, double *A, i64 sA, int nb, double *work);
 // X_b <- Q_b X_b or Q_b^T X_b for X_b = X + b*sX (N x ncol, ld N); h must run on st
 int trd_apply_q(rocblas_handle h, hipStream_t st, int N, const double *A, i64 sA, int nb, const double *tau,
                 bool trans, int ncol, double *X, i64 sX, double *scratch);

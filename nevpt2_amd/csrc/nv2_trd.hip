@@ -15,8 +15,9 @@
 // (a last-workgroup-done handshake needs an agent-scope release per workgroup, which writes back
 // the freshly updated columns from every L2 and measured twice as slow); then one wavefront per
 // trailing column applies the rank-2 update of step k-1 and forms (A v_k)_c as a column dot
-// product (A is symmetric, so there are no partial sums to reduce).  The N+2 launches are
-// captured in a hipGraph once per shape and buffer set, then replayed.
+// product (A is symmetric, so there are no partial sums to reduce).  The launches are not
+// captured in a hipGraph: the callers run on worker threads beside others that synchronise the
+// device, which invalidates a capture in progress (and replay saved only ~4 % per column).
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
@@ -232,22 +233,7 @@ static int trd_launches(hipStream_t st, int N, double *A, i64 sA, int nb, double
     return 0;
 }
 
-struct TrdPlan {
-    int N = 0, nb = 0;
-    double *A = nullptr, *work = nullptr;
-    i64 sA = 0;
-    hipStream_t st = nullptr;
-    hipGraph_t g = nullptr;
-    hipGraphExec_t ge = nullptr;
-};
-void trd_plan_destroy(TrdPlan *pl) {
-    if (!pl) return;
-    if (pl->ge) (void)hipGraphExecDestroy(pl->ge);
-    if (pl->g) (void)hipGraphDestroy(pl->g);
-    delete pl;
-}
-
-int trd_reduce(hipStream_t st, int N, double *A, i64 sA, int nb, double *work, TrdPlan **plan) {
+int trd_reduce(hipStream_t st, int N, double *A, i64 sA, int nb, double *work) {
     if (N < 1 || nb < 1 || nb > 65535) { set_error("trd_reduce: unsupported N=%d nb=%d", N, nb); return -2; }
     if (N <= 2) {
         const TrdOut o = trd_outputs(N, nb, work);
@@ -259,25 +245,8 @@ int trd_reduce(hipStream_t st, int N, double *A, i64 sA, int nb, double *work, T
     static const hipError_t attr =    // once per process (thread-safe static initialisation)
         hipFuncSetAttribute((const void *)k_trd_col, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
     NV2_HIP(attr);
-    if (!plan) {
-        TRYT(trd_launches(st, N, A, sA, nb, work));
-        NV2_LAUNCH_CHECK();
-        return 0;
-    }
-    TrdPlan *pl = *plan;
-    if (!pl || pl->N != N || pl->nb != nb || pl->A != A || pl->sA != sA || pl->work != work || pl->st != st) {
-        // the N-1 launches are captured once per shape and buffer set, then replayed
-        trd_plan_destroy(pl);
-        *plan = pl = new TrdPlan();
-        pl->N = N; pl->nb = nb; pl->A = A; pl->sA = sA; pl->work = work; pl->st = st;
-        NV2_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        const int rc = trd_launches(st, N, A, sA, nb, work);
-        const hipError_t ec = hipStreamEndCapture(st, &pl->g);
-        if (rc) return rc;
-        NV2_HIP(ec);
-        NV2_HIP(hipGraphInstantiate(&pl->ge, pl->g, nullptr, nullptr, 0));
-    }
-    NV2_HIP(hipGraphLaunch(pl->ge, st));
+    TRYT(trd_launches(st, N, A, sA, nb, work));
+    NV2_LAUNCH_CHECK();
     return 0;
 }
 
