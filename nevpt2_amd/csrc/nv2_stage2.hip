@@ -1067,6 +1067,9 @@ struct Stage2 {
     int *hinfo = nullptr;          // pinned host copies of solver info words
     int *dinfo_w = nullptr;        // device info words, 2 per solve slot
     double *wE[NW] = {};           // syevd workspace per worker
+    hipStream_t xs = nullptr;      // second stream + handle for the "-" problem's dstedc of the n^3 metric
+    rocblas_handle xh = nullptr;
+    hipEvent_t xev[2] = {};
     int nc, n, nv, norb, L, M;
     std::map<std::string, Buf> bufs;
     int *dinfo = nullptr;
@@ -1335,9 +1338,29 @@ static int pc_trd_metric(Stage2 *p, PcJob &j) {
     NV2_HIP(hipMemcpyAsync(j.trde, o.e, sizeof(double) * N * j.nb, hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(k_identity_batched, dim3(nb(NN, 256), 1, j.nb), dim3(256), 0, st, j.T, N, NN);
     NV2_LAUNCH_CHECK();
-    for (int b = 0; b < j.nb; b++)
-        RB(rocsolver_dstedc(h, rocblas_evect_tridiagonal, N, j.w1 + (i64)b * N, j.trde + (i64)b * N, j.T + b * NN, N,
-                            p->dinfo_w + 4 * j.slot + b));
+    // the two tridiagonal problems are independent: problem 1 runs on its own stream and host
+    // thread (the solver blocks its calling thread at times) while problem 0 runs here
+    int rc1 = 0;
+    char err1[512] = "";
+    std::thread t1;
+    if (j.nb > 1) {
+        NV2_HIP(hipEventRecord(p->xev[0], st));
+        t1 = std::thread([&]() {
+            if (hipStreamWaitEvent(p->xs, p->xev[0], 0) != hipSuccess ||
+                rocsolver_dstedc(p->xh, rocblas_evect_tridiagonal, N, j.w1 + N, j.trde + N, j.T + NN, N,
+                                 p->dinfo_w + 4 * j.slot + 1) != rocblas_status_success ||
+                hipEventRecord(p->xev[1], p->xs) != hipSuccess) {
+                rc1 = -1;
+                snprintf(err1, sizeof(err1), "dstedc of the second metric problem failed");
+            }
+        });
+    }
+    const rocblas_status s0 = rocsolver_dstedc(h, rocblas_evect_tridiagonal, N, j.w1, j.trde, j.T, N,
+                                               p->dinfo_w + 4 * j.slot);
+    if (t1.joinable()) t1.join();
+    if (s0 != rocblas_status_success) { set_error("rocsolver_dstedc (metric, N=%d): status %d", N, (int)s0); return -1; }
+    if (rc1) { set_error("%s", err1); return rc1; }
+    if (j.nb > 1) NV2_HIP(hipStreamWaitEvent(st, p->xev[1], 0));
     TRYS(trd_apply_q(h, st, N, j.S, NN, j.nb, o.tau, false, N, j.T, NN, j.trdq));
     NV2_HIP(hipMemcpyAsync(j.S, j.T, sizeof(double) * NN * j.nb, hipMemcpyDeviceToDevice, st));
     NV2_HIP(hipMemcpyAsync(p->hinfo + 4 * j.slot, p->dinfo_w + 4 * j.slot, sizeof(int) * j.nb, hipMemcpyDeviceToHost, st));
@@ -1427,6 +1450,12 @@ Stage2 *stage2_create(const S2Inputs &in, rocblas_handle h, hipStream_t st) {
     }
     for (int e = 0; e < NW + 1; e++)
         if (hipEventCreateWithFlags(&p->wev[e], hipEventDisableTiming) != hipSuccess) { delete p; return nullptr; }
+    if (hipStreamCreateWithFlags(&p->xs, hipStreamNonBlocking) != hipSuccess) { delete p; return nullptr; }
+    if (rocblas_create_handle(&p->xh) != rocblas_status_success) { delete p; return nullptr; }
+    rocblas_set_stream(p->xh, p->xs);
+    rocblas_set_pointer_mode(p->xh, rocblas_pointer_mode_host);
+    for (int e = 0; e < 2; e++)
+        if (hipEventCreateWithFlags(&p->xev[e], hipEventDisableTiming) != hipSuccess) { delete p; return nullptr; }
     // pair tables: core i<=j, core i<j, virt r<=s, virt r<s, active upper pairs
     std::vector<int2> pc, pcs, pv, pvs, up;
     for (int i = 0; i < in.nc; i++)
@@ -1459,6 +1488,9 @@ void stage2_destroy(Stage2 *p) {
         if (p->ws[w]) (void)hipStreamDestroy(p->ws[w]);
     }
     for (int e = 0; e < NW + 1; e++) if (p->wev[e]) (void)hipEventDestroy(p->wev[e]);
+    if (p->xh) rocblas_destroy_handle(p->xh);
+    if (p->xs) (void)hipStreamDestroy(p->xs);
+    for (int e = 0; e < 2; e++) if (p->xev[e]) (void)hipEventDestroy(p->xev[e]);
     (void)hipFree(p->pairs_c); (void)hipFree(p->pairs_cs); (void)hipFree(p->pairs_v); (void)hipFree(p->pairs_vs);
     (void)hipFree(p->up);
     delete p;
