@@ -28,8 +28,6 @@ int launch_make_f(int n, int ncore, const double *aj, const double *atwo, double
 int launch_koop_small(const Rdm &R, int ncore, const double *aj, const double *atwo, Stage1Scratch &w,
                       Stage1Out &o, hipStream_t st);
 int launch_build_e4(const Rdm &R, int s, void *E4, int prec, hipStream_t st);
-int launch_bamat_direct(const Rdm &R, const double *atwo, const double *Fhalf, const double *C1, const double *C2,
-                        const double *C3, const double *C4, double *amat, double *atmat, hipStream_t st);
 // lower-order part of bamat for state s as GEMMs + one combine kernel (nv2_bamat_lo.hip);
 // work holds bamat_lo_workspace(n) doubles
 i64 bamat_lo_workspace(int n);

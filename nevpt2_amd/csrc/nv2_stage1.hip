@@ -5,9 +5,9 @@
 // nest; RDMs stay resident in HBM/MALL.  Partial traces and the 2-hole RDM run
 // one thread per element.  The O(n^9)
 // 4-RDM part of bamat is split off: the full <E E E E> tensor E4 is
-// materialised once per state and contracted with the active ERIs as four
-// strided GEMMs (nv2_contract.hip); everything lower-order in bamat runs in
-// nv2_bamat_direct.
+// materialised once per state (k_build_e4) and contracted with the active ERIs
+// on the matrix cores (nv2_contract.hip); everything lower-order in bamat runs
+// as GEMMs plus a combining kernel (nv2_bamat_lo.hip).
 #include "nv2_common.h"
 #include "nv2_stage1.h"
 
@@ -403,72 +403,21 @@ __global__ void k_bdmat(Rdm R, const double *atwo, const double *F, const double
 }
 
 // ---------------------------------------------------------------- bamat: E4 = <E E E E> for one state
-// E4 is stored C-ordered over (x1..x8), x8 fastest: offset = sum_k x_k n^(8-k)
-template <typename T>
-__global__ void k_build_e4(Rdm R, int s, T *E4) {
-    const int n = R.n;
-    i64 n8 = (i64)n * n * n * n * n * n * n * n;
-    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n8) return;
-    int x[8];
-    i64 t = idx;
-    for (int k = 7; k >= 0; k--) { x[k] = (int)(t % n); t /= n; }
-    E4[idx] = (T)eeee(R, s, x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7]);
+// E4 is stored C-ordered over (x1..x8), x8 fastest: offset = sum_k x_k n^(8-k).  Workgroup
+// g covers the n^3 elements (x6,x7,x8) of one (x1..x5) in chunks of 256, so every index comes
+// from 32-bit divisions by the compile-time n (a 64-bit division by a run-time n per index
+// costs more than the gather itself).
+template <typename T, int NN>
+__global__ __launch_bounds__(256) void k_build_e4(Rdm R, int s, T *E4) {
+    constexpr int N2 = NN * NN, N3 = N2 * NN, CH = (N3 + 255) / 256;
+    const unsigned hi = blockIdx.x / CH;                           // (x1..x5), x5 fastest
+    const int t = (int)(blockIdx.x % CH) * 256 + threadIdx.x;      // (x6,x7,x8), x8 fastest
+    if (t >= N3) return;
+    const int x8 = t % NN, x7 = (t / NN) % NN, x6 = t / N2;
+    const int x5 = hi % NN, x4 = (hi / NN) % NN, x3 = (hi / N2) % NN, x2 = (hi / N3) % NN, x1 = hi / (N3 * NN);
+    E4[(i64)hi * N3 + t] = (T)eeee(R, s, x1, x2, x3, x4, x5, x6, x7, x8);
 }
 
-// lower-order part of bamat plus the combination with the four E4 contractions.
-// Contraction buffers Ck share amat's layout (state fastest, then ap,bp,cp,a,b,c):
-//   C1 = sum eeee(cp,ap,bp,b,d,f,i,c) w(d,f,i,a)    C2 = sum eeee(cp,ap,bp,b,d,f,a,i) w(d,f,c,i)
-//   C3 = sum eeee(cp,ap,bp,i,d,f,a,c) w(d,f,b,i)    C4 = sum eeee(cp,ap,i,bp,d,f,a,c) w(d,f,b,i)
-// amat = direct + C1 - C2 - C3 ; atmat = direct_t - C1[bp<->b] + C2[bp<->b] - C4
-__global__ void k_bamat_direct(Rdm R, const double *atwo, const double *F, const double *C1, const double *C2,
-                               const double *C3, const double *C4, double *amat, double *atmat) {
-    const int n = R.n, M = R.M;
-    i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (i64)M * n * n * n * n * n * n) return;
-    int s, x[6];
-    decode(idx, M, n, 6, &s, x);
-    const int ap = x[0], bp = x[1], cp = x[2], a = x[3], b = x[4], c = x[5];
-    // index of the bp<->b swapped element
-    const i64 sw = s + (i64)M * (ap + n * (b + n * (cp + n * (a + n * (bp + (i64)n * c)))));
-    double av = 0.0, tv = 0.0;
-    for (int d = 0; d < n; d++) {
-        // f * <EEE> terms, own iteration (koopro4QD.F90:4018-4037)
-        av = av + FF(d, a) * eee2(R, s, cp, ap, bp, b, d, c) - FF(c, d) * eee2(R, s, cp, ap, bp, b, a, d)
-             - FF(b, d) * eee2(R, s, cp, ap, bp, d, a, c);
-        tv = tv + FF(d, a) * eee2t(R, s, cp, ap, bp, b, d, c) - FF(c, d) * eee2t(R, s, cp, ap, bp, b, a, d)
-             + FF(b, d) * eee2t(R, s, cp, ap, bp, d, a, c);
-        for (int ie = 0; ie < n; ie++) {
-            // 3-RDM x ERI terms (koopro4QD.F90:4038-4068)
-            double dint1 = AT(c, ie, d, a), dint2 = AT(d, ie, ie, a), dint3 = AT(c, d, d, ie), dint4 = AT(b, d, d, ie);
-            double dum1 = eee2(R, s, cp, ap, bp, b, d, ie), dum2 = eee2(R, s, cp, ap, bp, b, d, c);
-            double dum3 = eee2(R, s, cp, ap, bp, b, a, ie), dum4 = eee2(R, s, cp, ap, bp, ie, a, c);
-            av = av + dum1 * dint1 - 0.5 * (dum2 * dint2 + dum3 * dint3 - dum4 * dint4);
-            tv = tv + 0.5 * eee2t(R, s, cp, ap, bp, ie, a, c) * dint4;
-            // swapped iteration (its bp is our b, its b is our bp) writes atmat(ap,bp,cp,a,b,c)
-            double sd1 = -eee2(R, s, cp, ap, b, bp, d, ie), sd2 = -eee2(R, s, cp, ap, b, bp, d, c);
-            double sd3 = -eee2(R, s, cp, ap, b, bp, a, ie);
-            if (b == bp) {
-                sd1 = sd1 + 2.0 * ee2(R, s, cp, ap, d, ie);
-                sd2 = sd2 + 2.0 * ee2(R, s, cp, ap, d, c);
-                sd3 = sd3 + 2.0 * ee2(R, s, cp, ap, a, ie);
-            }
-            tv = tv + sd1 * dint1 - 0.5 * (sd2 * dint2 + sd3 * dint3);
-            // delta part of eeeet(cp,ap,bp,ie,...) at ie == bp (koopro4QD.F90:4088-4092)
-            if (ie == bp)
-                for (int ief = 0; ief < n; ief++) tv = tv + 2.0 * eee2(R, s, cp, ap, d, ief, a, c) * AT(d, ief, b, ie);
-            // delta parts of du1t/du2t for the swapped iteration (koopro4QD.F90:4074-4086)
-            if (b == bp)
-                for (int ief = 0; ief < n; ief++)
-                    tv = tv + 2.0 * eee2(R, s, cp, ap, d, ief, ie, c) * AT(d, ief, ie, a)
-                         - 2.0 * eee2(R, s, cp, ap, d, ief, a, ie) * AT(d, ief, c, ie);
-        }
-    }
-    amat[idx] = av + C1[idx] - C2[idx] - C3[idx];
-    atmat[idx] = tv - C1[sw] + C2[sw] - C4[idx];
-}
-
-// ---------------------------------------------------------------- host launchers
 static inline unsigned nblk(i64 tot, int bs) { return (unsigned)((tot + bs - 1) / bs); }
 
 int launch_bro(const Rdm &R, int nele, double *daaa, double *taa, double *dal, hipStream_t st) {
@@ -530,23 +479,27 @@ int launch_koop_small(const Rdm &R, int ncore, const double *aj, const double *a
     return 0;
 }
 
-int launch_build_e4(const Rdm &R, int s, void *E4, int prec, hipStream_t st) {
-    const i64 n = R.n, n8 = n * n * n * n * n * n * n * n;
-    if (prec == 0)
-        hipLaunchKernelGGL(k_build_e4<double>, dim3(nblk(n8, 256)), dim3(256), 0, st, R, s, (double *)E4);
-    else
-        hipLaunchKernelGGL(k_build_e4<float>, dim3(nblk(n8, 256)), dim3(256), 0, st, R, s, (float *)E4);
+template <typename T>
+static int build_e4_n(const Rdm &R, int s, T *E4, hipStream_t st) {
+    switch (R.n) {
+#define CASE(k)                                                                                                 \
+    case k: {                                                                                                   \
+        const unsigned ch = (k * k * k + 255) / 256, g = (unsigned)(k * k * k * k * k) * ch;                   \
+        hipLaunchKernelGGL((k_build_e4<T, k>), dim3(g), dim3(256), 0, st, R, s, E4);                           \
+        break;                                                                                                  \
+    }
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12)
+        CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+        default: set_error("E4 build supports 1 <= nact <= 16 (got %d)", R.n); return -2;
+    }
     NV2_LAUNCH_CHECK();
     return 0;
 }
 
-int launch_bamat_direct(const Rdm &R, const double *atwo, const double *Fhalf, const double *C1, const double *C2,
-                        const double *C3, const double *C4, double *amat, double *atmat, hipStream_t st) {
-    const i64 n = R.n, M = R.M, t6 = M * n * n * n * n * n * n;
-    hipLaunchKernelGGL(k_bamat_direct, dim3(nblk(t6, 64)), dim3(64), 0, st, R, atwo, Fhalf, C1, C2, C3, C4, amat,
-                       atmat);
-    NV2_LAUNCH_CHECK();
-    return 0;
+int launch_build_e4(const Rdm &R, int s, void *E4, int prec, hipStream_t st) {
+    if (prec == 0) return build_e4_n<double>(R, s, (double *)E4, st);
+    return build_e4_n<float>(R, s, (float *)E4, st);
 }
 
 }  // namespace nv2
