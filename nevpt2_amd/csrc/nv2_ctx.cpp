@@ -172,6 +172,8 @@ int nv2_ctx_run(nv2_ctx *c, int32_t prec, double thr_lindep, double *out) {
     // phase 0: partial traces (bro3/bro2/bro1)
     if (launch_bro(R, c->nele, c->find("3-RDM"), c->find("2-RDM"), c->find("1-RDM"), st)) return -1;
     NV2_HIP(hipEventRecord(c->ev[1], st));
+    // the (+-1)' metric of the first state depends on the 3-RDM only: its solve overlaps stage 1
+    if (stage2_pm1p_begin(c->s2, 0, thr_lindep)) return -1;
     // phase 1: Dyall h, active ERIs, Koopmans matrices of rank <= 3 and B/C/D
     if (launch_aj_atwo(n, c->nc, (int)c->norb, c->L, c->find("cholesky"), c->find("oneint"), c->find("aj"),
                        c->find("atwo"), st)) return -1;

@@ -30,5 +30,8 @@ void stage2_destroy(Stage2 *p);
 // present, the QD couplings of state s: qd_out[k*2M + j] = SC e2mp(j,s), qd_out[k*2M + M + j] =
 // PC e2en(j,s) of class k (zero for j == s).
 int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *stats, double *qd_out = nullptr);
+// starts the (+-1)' metric solve of state s on worker 1 ahead of stage2_state (it needs the 3-RDM
+// only); stage2_state(s) then waits for it instead of starting it
+int stage2_pm1p_begin(Stage2 *p, int s, double thr);
 
 }  // namespace nv2

@@ -11,7 +11,11 @@
 // FP64 accumulation.
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <mutex>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -940,7 +944,7 @@ __global__ void k_dacoef(const double *da, int n, const double *coef, int ldc, i
 // ------------------------------------------------------------------ classes (+1)' / (-1)' (qdnevpt.F:5437-6445, 5001-5435)
 // index order abc = a*n^2 + b*n + c (the reference's iabc)
 // Spc[abc, A'B'C'] = X3(c',a',b',b,a,c) with X3 = eee2 (minus) or eee2t (plus); Kpc[abc, A'B'C'] = amat(a',b',c',a,b,c)
-__global__ void k_fill_pm1p(Sl S, int plus, const double *amat, double *Spc, double *Kpc) {
+__global__ void k_fill_pm1p_s(Sl S, int plus, double *Spc) {
     const int n = S.n;
     const i64 N = (i64)n * n * n;
     i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -949,6 +953,14 @@ __global__ void k_fill_pm1p(Sl S, int plus, const double *amat, double *Spc, dou
     int a = row / (n * n), b = (row / n) % n, c = row % n;
     int ap = col / (n * n), bp = (col / n) % n, cp = col % n;
     Spc[e] = plus ? s_eee2t(S, cp, ap, bp, b, a, c) : s_eee2(S, cp, ap, bp, b, a, c);
+}
+__global__ void k_fill_pm1p_k(int n, const double *amat, double *Kpc) {
+    const i64 N = (i64)n * n * n;
+    i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N * N) return;
+    i64 row = e % N, col = e / N;
+    int a = row / (n * n), b = (row / n) % n, c = row % n;
+    int ap = col / (n * n), bp = (col / n) % n, cp = col % n;
     Kpc[e] = amat[ap + (i64)n * (bp + (i64)n * (cp + (i64)n * (a + (i64)n * (b + (i64)n * c))))];
 }
 // Q2[A'B'C', a] = X2(c',a',b',a) (ee2 or ee2tr); BC[A'B'C', a] = bmat(a',b',c',a) + cmat(a,a',b',c')
@@ -1055,8 +1067,11 @@ struct Buf {
 
 // independent PC eigen-solves run on worker streams (each with its own rocBLAS handle)
 #define NW 2
+struct Pm1pEarly;
 struct Stage2 {
     S2Inputs in;
+    Pm1pEarly *early = nullptr;    // the (+-1)' chain when started ahead of stage2_state
+    hipEvent_t evS = nullptr, evK = nullptr;   // its metric filled / its Koopmans matrix filled
     rocblas_handle h;
     hipStream_t st;
     rocblas_handle wh[NW] = {};
@@ -1415,13 +1430,109 @@ static int pc_kprime_launch(Stage2 *p, PcJob &j, double thr) {
 }
 // one host thread per worker stream: the solver library blocks its calling thread at times,
 // so separate threads are what lets the chains of different problems overlap on the device
-static int pc_worker(Stage2 *p, PcJob *J, int njob, int w, double thr) {
+static int pc_worker(Stage2 *p, PcJob *J, int njob, int w, double thr, double *ms_done,
+                     std::chrono::steady_clock::time_point t0) {
     for (int k = 0; k < njob; k++)
         if (J[k].w == w) TRYS(pc_metric_launch(p, J[k]));
     NV2_HIP(hipStreamSynchronize(p->ws[w]));
     for (int k = 0; k < njob; k++)
         if (J[k].w == w) TRYS(pc_kprime_launch(p, J[k], thr));
     NV2_HIP(hipStreamSynchronize(p->ws[w]));
+    *ms_done = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+static int slice(Stage2 *p, const double *src, int s, i64 cnt, double *dst);
+
+// The (+-1)' chain (n^3 metric, K', reduction) is the longest path of stage 2.  Its metric needs
+// the 3-RDM only, so it can start right after the partial traces, before the Koopmans matrices
+// exist; the worker thread then waits for K (filled by stage2_state once stage 1 is done) before
+// forming K'.
+struct Pm1pEarly {
+    bool active = false;
+    int state = -1;
+    double thr = 0.0;
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv;
+    bool k_ready = false, abort = false;
+    int rc = 0;
+    char err[512] = "";
+    PcJob job;
+};
+static size_t pm1p_hoff(int n) {   // pinned-host offset of its metric eigenvalues: after the other jobs'
+    const size_t nn = (size_t)n, m = nn * (nn + 1) / 2;
+    return 2 * nn + 2 * nn * nn + 2 * m + 2 * nn * nn;
+}
+static void pm1p_stop(Stage2 *p) {   // abort a chain still waiting for K, and join it
+    Pm1pEarly *E = p->early;
+    if (!E || !E->active) return;
+    {
+        std::lock_guard<std::mutex> lk(E->mu);
+        E->abort = true;
+    }
+    E->cv.notify_all();
+    if (E->th.joinable()) E->th.join();
+    E->active = false;
+}
+static int pm1p_worker(Stage2 *p, Pm1pEarly *E) {
+    PcJob &j = E->job;
+    NV2_HIP(hipStreamWaitEvent(p->ws[j.w], p->evS, 0));
+    TRYS(pc_metric_launch(p, j));
+    NV2_HIP(hipStreamSynchronize(p->ws[j.w]));
+    {
+        std::unique_lock<std::mutex> lk(E->mu);
+        E->cv.wait(lk, [E] { return E->k_ready || E->abort; });
+        if (E->abort) return 0;
+    }
+    NV2_HIP(hipStreamWaitEvent(p->ws[j.w], p->evK, 0));
+    TRYS(pc_kprime_launch(p, j, E->thr));
+    NV2_HIP(hipStreamSynchronize(p->ws[j.w]));
+    return 0;
+}
+
+int stage2_pm1p_begin(Stage2 *p, int s, double thr) {
+    if (!(p->in.amatIP && p->in.amatEA)) return 0;
+    if (!p->early) p->early = new Pm1pEarly();
+    Pm1pEarly *E = p->early;
+    if (E->active) {
+        if (E->state == s) return 0;
+        pm1p_stop(p);
+    }
+    hipStream_t st = p->st;
+    const int n = p->n;
+    const i64 n2 = (i64)n * n, n4 = n2 * n2, n6 = n4 * n2, N3 = n2 * n;
+    GET(da, "e_da", n2); GET(taa, "e_taa", n4); GET(ro3, "e_ro3", n6);
+    TRYS(slice(p, p->in.dal, s, n2, da)); TRYS(slice(p, p->in.taa, s, n4, taa)); TRYS(slice(p, p->in.daaa, s, n6, ro3));
+    Sl S{n, da, taa, ro3};
+    PcJob &j = E->job;
+    j = PcJob();
+    j.tag = "pm1p"; j.N = (int)N3; j.nb = 2; j.slot = 4; j.w = 1; j.trd = true;
+    GET(Sm, "pm1p_S", N3 * N3 * 2); GET(Km, "pm1p_K", N3 * N3 * 2);
+    j.S = Sm;
+    j.K = Km;
+    TRYS(pc_job_buffers(p, j, pm1p_hoff(n)));
+    GET(E1, "syevd_E1", N3 * 2);      // used only when the solver is forced to rocSOLVER (NV2_EIGEN)
+    p->wE[1] = E1;
+    for (int plus = 0; plus < 2; plus++) {   // metric, and the copy kept for the S-build
+        const int bb = plus ? 0 : 1;
+        GET(mx, plus ? "pm1_M_p" : "pm1_M_m", N3 * N3);
+        hipLaunchKernelGGL(k_fill_pm1p_s, dim3(nb(N3 * N3, 256)), dim3(256), 0, st, S, plus, j.Sb(bb));
+        NV2_LAUNCH_CHECK();
+        NV2_HIP(hipMemcpyAsync(mx, j.Sb(bb), sizeof(double) * N3 * N3, hipMemcpyDeviceToDevice, st));
+    }
+    NV2_HIP(hipEventRecord(p->evS, st));
+    E->state = s;
+    E->thr = thr;
+    E->k_ready = false;
+    E->abort = false;
+    E->rc = 0;
+    E->err[0] = 0;
+    E->active = true;
+    E->th = std::thread([p, E]() {
+        E->rc = pm1p_worker(p, E);
+        if (E->rc) nv2_last_error(E->err, sizeof(E->err));
+    });
     return 0;
 }
 
@@ -1450,6 +1561,8 @@ Stage2 *stage2_create(const S2Inputs &in, rocblas_handle h, hipStream_t st) {
     }
     for (int e = 0; e < NW + 1; e++)
         if (hipEventCreateWithFlags(&p->wev[e], hipEventDisableTiming) != hipSuccess) { delete p; return nullptr; }
+    if (hipEventCreateWithFlags(&p->evS, hipEventDisableTiming) != hipSuccess) { delete p; return nullptr; }
+    if (hipEventCreateWithFlags(&p->evK, hipEventDisableTiming) != hipSuccess) { delete p; return nullptr; }
     if (hipStreamCreateWithFlags(&p->xs, hipStreamNonBlocking) != hipSuccess) { delete p; return nullptr; }
     if (rocblas_create_handle(&p->xh) != rocblas_status_success) { delete p; return nullptr; }
     rocblas_set_stream(p->xh, p->xs);
@@ -1477,6 +1590,10 @@ Stage2 *stage2_create(const S2Inputs &in, rocblas_handle h, hipStream_t st) {
 
 void stage2_destroy(Stage2 *p) {
     if (!p) return;
+    pm1p_stop(p);
+    delete p->early;
+    if (p->evS) (void)hipEventDestroy(p->evS);
+    if (p->evK) (void)hipEventDestroy(p->evK);
     for (auto &kv : p->bufs) if (kv.second.p) (void)hipFree(kv.second.p);
     (void)hipDeviceSynchronize();
     (void)hipFree(p->dinfo); (void)hipFree(p->part); (void)hipFree(p->acc); (void)hipFree(p->accq);
@@ -1540,7 +1657,18 @@ static double *qd_fill(Stage2 *p, const Sl &S, int kind, int lt, i64 total, cons
     if (!var) return -1;
 #define QGET(var, name, j, cnt) GET(var, (std::string(name) + std::to_string(j)).c_str(), (cnt))
 
+// NV2_STAGE2_TIMING=1 prints, per state, the host-side milestones of the solve layout to stderr
+static bool stage2_timing() {
+    static const bool on = getenv("NV2_STAGE2_TIMING") && atoi(getenv("NV2_STAGE2_TIMING")) > 0;
+    return on;
+}
+static double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *stats, double *qd_out) {
+    const auto t_begin = std::chrono::steady_clock::now();
+    double ms_workers = 0.0, ms_wdone[NW] = {}, ms_scq = 0.0, ms_join = 0.0;
     (void)prec;
     const int nc = p->nc, n = p->n, nv = p->nv, L = p->L, norb = p->norb, M = p->M;
     const i64 n2 = (i64)n * n, n4 = n2 * n2, n6 = n4 * n2;
@@ -1594,10 +1722,21 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
     const int m = n * (n + 1) / 2;
     const bool has4 = p->in.amatIP && p->in.amatEA;
     const i64 N3 = n2 * n;
+    // the (+-1)' chain runs on worker 1 from its own start (stage2_pm1p_begin, possibly called
+    // right after the partial traces); every return from here on stops it if it still waits
+    if (has4) TRYS(stage2_pm1p_begin(p, s, thr));
+    struct EarlyGuard {
+        Stage2 *p;
+        ~EarlyGuard() { pm1p_stop(p); }
+    } eguard{p};
     {
         const char *tags[NJOB] = {"pm1", "pm2", "pm2p", "v0p", "pm1p"};
         const int Ns[NJOB] = {n, (int)n2, m, 2 * (int)n2, has4 ? (int)N3 : 0};
         for (int k = 0; k < NJOB; k++) {
+            if (k == JPM1P && has4) {
+                J[k] = p->early->job;          // its fields are final once worker 1 is joined
+                continue;
+            }
             J[k].tag = tags[k];
             J[k].N = Ns[k];
             J[k].nb = k == J0P ? 1 : 2;
@@ -1653,18 +1792,24 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             TRYS(slice(p, plus ? p->in.bmatEA : p->in.bmatIP, s, n4, bm));
             TRYS(slice(p, plus ? p->in.cmatEA : p->in.cmatIP, s, n4, cm));
             TRYS(slice(p, plus ? p->in.dmatEA : p->in.dmatIP, s, n2, dm));
-            GET(mx, ("pm1_M" + t).c_str(), N3 * N3);
-            hipLaunchKernelGGL(k_fill_pm1p, dim3(nb(N3 * N3, 256)), dim3(256), 0, st, S, plus, am, jb.Sb(bb), jb.Kb(bb));
+            GET(mx, ("pm1_M" + t).c_str(), N3 * N3);    // filled by stage2_pm1p_begin
+            hipLaunchKernelGGL(k_fill_pm1p_k, dim3(nb(N3 * N3, 256)), dim3(256), 0, st, n, am, jb.Kb(bb));
             NV2_LAUNCH_CHECK();
-            NV2_HIP(hipMemcpyAsync(mx, jb.Sb(bb), sizeof(double) * N3 * N3, hipMemcpyDeviceToDevice, st));
             Mx[plus] = mx; amk[plus] = am; bmk[plus] = bm; cmk[plus] = cm; dmk[plus] = dm;
         }
+        NV2_HIP(hipEventRecord(p->evK, st));             // K is there: the chain may form K'
+        {
+            std::lock_guard<std::mutex> lk(p->early->mu);
+            p->early->k_ready = true;
+        }
+        p->early->cv.notify_all();
     }
     // start the eigen-solves on the worker threads/streams once the fills are done
     {
         size_t hoff = 0;
         int maxN[NW] = {};
         for (int k = 0; k < NJOB; k++) {
+            if (k == JPM1P && has4) continue;          // allocated by stage2_pm1p_begin
             TRYS(pc_job_buffers(p, J[k], hoff));
             hoff += (size_t)J[k].N * J[k].nb;
             if (J[k].N * J[k].nb > maxN[J[k].w]) maxN[J[k].w] = J[k].N * J[k].nb;
@@ -1673,6 +1818,10 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             GET(E, ("syevd_E" + std::to_string(w)).c_str(), maxN[w] ? maxN[w] : 1);
             p->wE[w] = E;
         }
+    }
+    if (stage2_timing()) {           // fills done on the device
+        NV2_HIP(hipStreamSynchronize(st));
+        ms_workers = ms_since(t_begin);
     }
     NV2_HIP(hipEventRecord(p->wev[NW], st));
     for (int w = 0; w < NW; w++) NV2_HIP(hipStreamWaitEvent(p->ws[w], p->wev[NW], 0));
@@ -1683,12 +1832,11 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         std::vector<std::thread> &v;
         ~Joiner() { for (auto &t : v) if (t.joinable()) t.join(); }
     } joiner{workers};
-    for (int w = 0; w < NW; w++)
-        workers.emplace_back([p, &J, w, thr, &wrc, &werr]() {
-            wrc[w] = pc_worker(p, J, NJOB, w, thr);
+    for (int w = 0; w < (has4 ? 1 : NW); w++)          // with the (+-1)' chain, worker 1 is its thread
+        workers.emplace_back([p, &J, w, thr, &wrc, &werr, &ms_wdone, t_begin]() {
+            wrc[w] = pc_worker(p, J, NJOB, w, thr, &ms_wdone[w], t_begin);
             if (wrc[w]) nv2_last_error(werr[w], sizeof(werr[w]));
         });
-    auto join_workers = [&]() { for (auto &t : workers) if (t.joinable()) t.join(); };
 
     // ---- Fock operators and canonical orbitals (qdnevpt.F:369-698)
     GET(Jc, "Jc", L); GET(Ja, "Ja", L);
@@ -1914,21 +2062,29 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         TRYS(qd_reduce(p, q, next, cls));
     }
 
-    // ================================================================ join the PC solves
-    join_workers();
-    for (int w = 0; w < NW; w++)
-        if (wrc[w]) { set_error("PC solve worker %d: %s", w, werr[w]); return wrc[w]; }
-    for (int w = 0; w < NW; w++) {
+    // ================================================================ join worker 0: every PC class but (+-1)'
+    // (whose n^3 chain on worker 1 is the longest; its sums come last)
+    if (stage2_timing()) {           // the SC sums are done on the device
+        NV2_HIP(hipStreamSynchronize(st));
+        ms_scq = ms_since(t_begin);
+    }
+    auto join_worker = [&](int w) -> int {
+        if (w == 1 && has4) {
+            Pm1pEarly *E = p->early;
+            if (E->th.joinable()) E->th.join();
+            E->active = false;
+            if (E->rc) { set_error("(+-1)' PC solve: %s", E->err); return E->rc; }
+            J[JPM1P] = E->job;
+            ms_wdone[1] = ms_since(t_begin);
+        } else {
+            if (workers[w].joinable()) workers[w].join();
+            if (wrc[w]) { set_error("PC solve worker %d: %s", w, werr[w]); return wrc[w]; }
+        }
         NV2_HIP(hipEventRecord(p->wev[w], p->ws[w]));
         NV2_HIP(hipStreamWaitEvent(st, p->wev[w], 0));
-    }
-    const int cls_of[NJOB][2] = {{1, 2}, {3, 4}, {3, 4}, {7, 7}, {5, 6}};
-    for (int k = 0; k < NJOB; k++)
-        for (int b = 0; b < J[k].nb; b++) {
-            const int c = cls_of[k][b];
-            if (k == JPM2P) { if (J[k].minw[b] < sd.min_metric_eig[c]) sd.min_metric_eig[c] = J[k].minw[b]; }
-            else { sd.min_metric_eig[c] = J[k].minw[b]; sd.nnew[c] = J[k].nnew[b]; }
-        }
+        return 0;
+    };
+    TRYS(join_worker(0));
 
     // ================================================================ class (+1), SC and PC (fused kernel)
     {
@@ -2083,6 +2239,16 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             TRYS(qd_reduce(p, q, nbk, 7));
         }
     }
+    // ================================================================ join worker 1: the (+-1)' solves
+    TRYS(join_worker(1));
+    ms_join = ms_since(t_begin);
+    const int cls_of[NJOB][2] = {{1, 2}, {3, 4}, {3, 4}, {7, 7}, {5, 6}};
+    for (int k = 0; k < NJOB; k++)
+        for (int b = 0; b < J[k].nb; b++) {
+            const int c = cls_of[k][b];
+            if (k == JPM2P) { if (J[k].minw[b] < sd.min_metric_eig[c]) sd.min_metric_eig[c] = J[k].minw[b]; }
+            else { sd.min_metric_eig[c] = J[k].minw[b]; sd.nnew[c] = J[k].nnew[b]; }
+        }
     // ================================================================ classes (-1)' and (+1)', PC sums
     // sum_mu r_mu^2 / (w_mu + s_x) with r = C^T caux^T X_x and K' = C W C^T (qdnevpt.F:5351-5422,
     // 5130-5205) is y^T (K' + s_x)^-1 y with y = caux^T X_x; with K' = Q T Q^T it is
@@ -2126,6 +2292,11 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         NV2_LAUNCH_CHECK();
         TRYS(reduce_into(p, nbk, cls));
         TRYS(qd_reduce(p, q, nbk, cls));
+    }
+    if (stage2_timing()) {
+        NV2_HIP(hipStreamSynchronize(st));
+        fprintf(stderr, "[stage2 s=%d] fills %.2f  SC done %.2f  worker0 %.2f  worker1 %.2f  join %.2f  end %.2f ms\n", s,
+                ms_workers, ms_scq, ms_wdone[0], ms_wdone[1], ms_join, ms_since(t_begin));
     }
     // ---- results: (norm_sc, e_sc, norm_pc, e_pc) per class, SC==PC for class (0)
     double accs[32];
