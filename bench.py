@@ -28,11 +28,30 @@ CONFIGS = {   # name: (nact, ncore, nvirt, roots, description)
 }
 METRIC = "SC+PC-NEVPT2 evals/sec CAS(12-14) mixed vs FP64 at |ΔE|≤1µEh; p50/p99 lat"
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12}
+HBM_PEAK = 8.0e12
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "e4_contractions_cas10_fp64_pmc.md")
 
 
 def contraction_flops(n, M):
     """Algorithmic flops of bamat's four 4-RDM contractions: 4 x 2 n^9 per state."""
     return 8.0 * n ** 9 * M
+
+
+def contraction_bytes(n, M, esize):
+    """Algorithmic HBM bytes of the two contraction passes per evaluation: E4 (n^8 elements of
+    esize bytes) read once by each pass, and the four n^6 FP64 outputs written."""
+    return M * (2.0 * n ** 8 * esize + 4.0 * 8 * n ** 6)
+
+
+def measured_traffic(config, prec):
+    """HBM bytes of the two passes per evaluation from the committed rocprofv3 counter runs
+    (FETCH_SIZE and WRITE_SIZE, calibrated on a copy kernel), for the workload they were taken on."""
+    if config != "c2" or prec != "fp64" or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        if line.startswith("traffic_bytes_per_eval:"):
+            return float(line.split(":")[1])
+    return None
 
 
 def cpu_baseline(P, frac_pairs=0.05):
@@ -128,8 +147,8 @@ def main():
             elapsed = time.perf_counter() - t0
     t_contr = phases[PHASES.index("E4 contractions")] * 1e-3
     fl = contraction_flops(n, M)
-    achieved = fl / t_contr / 1e12 if t_contr > 0 else 0.0
-    peak = PEAK[args.prec] / 1e12
+    by = contraction_bytes(n, M, 8 if args.prec == "fp64" else 4)
+    achieved = by / t_contr / 1e9 if t_contr > 0 else 0.0
     out = {
         "metric": METRIC, "value": S * args.steps / elapsed, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
@@ -141,9 +160,12 @@ def main():
         "latency_ms": {"p50": float(np.percentile(latS, 50)), "p99": float(np.percentile(latS, 99)),
                        "unloaded_p50": float(np.percentile(lat1, 50)), "unloaded_p99": float(np.percentile(lat1, 99))},
         "phases_ms": dict(zip(PHASES, [float(x) for x in phases])),
-        "roofline": {"kernel": "bamat 4-RDM contractions (k_e4_pass_a + k_e4_pass_b)", "bound": "mfma",
-                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None, "algorithmic": f"8 n^9 M = {fl:.3e} flop per evaluation"},
+        "roofline": {"kernel": "bamat 4-RDM contractions (k_e4_pass_a + k_e4_pass_b)", "bound": "hbm",
+                     "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / (HBM_PEAK / 1e9),
+                     "traffic": measured_traffic(args.config, args.prec),
+                     "algorithmic": f"E4 read once per pass + outputs = {by:.3e} B per evaluation (two launches)",
+                     "flops": {"algorithmic": fl, "achieved_tflops": fl / t_contr / 1e12 if t_contr > 0 else 0.0,
+                               "peak_tflops": PEAK[args.prec] / 1e12}},
         "energies": {"e_sc_total": float(res[0, :, 1].sum()), "e_pc_total": float(res[0, :, 3].sum())},
     }
     if prec != 0:      # the mixed form against the FP64 path on the same resident inputs
