@@ -1300,9 +1300,9 @@ static int qd_reduce(Stage2 *p, const QDArgs &q, i64 nblocks, int cls) {
 struct PcJob {
     std::string tag;
     int N = 0, nb = 1, w = 0, slot = 0;        // info words at 4*slot + {metric b, K' 2+b}
-    bool trd = false;                          // nv2_trd.hip: metric by reduction + dstedc, K' only reduced
-    double *trdw = nullptr;                    // reduction scratch; T and tau of problem b from trd_outputs(nmax)
-    double *trdm = nullptr, *trde = nullptr;   // metric: reduction scratch, off-diagonal copy for dstedc
+    bool trd = false;                          // K' only reduced (the (+-1)' sums solve with its T)
+    double *trdw = nullptr;                    // K' reduction scratch; T and tau of problem b from trd_outputs(nmax)
+    double *trdm = nullptr, *trde = nullptr;   // eig_trd: reduction scratch, off-diagonal copy for dstedc
     double *trdq = nullptr;                    // scratch of applying Q
     double *S = nullptr, *K = nullptr;         // nb consecutive N x N matrices; S is overwritten
     double *w1 = nullptr, *hw1 = nullptr, *coef = nullptr, *w2 = nullptr;
@@ -1323,12 +1323,15 @@ static int pc_job_buffers(Stage2 *p, PcJob &j, size_t hoff) {
     GET(Ks, (j.tag + "_Ks").c_str(), NN); GET(T, (j.tag + "_T").c_str(), NN); GET(Kp, (j.tag + "_Kp").c_str(), NN);
     j.w1 = w1; j.w2 = w2; j.coef = coef; j.caux = caux; j.Ks = Ks; j.T = T; j.Kp = Kp;
     j.hw1 = p->hpin + hoff;
-    if (j.trd) {
-        GET(tw, (j.tag + "_trdw").c_str(), trd_workspace(j.N, j.nb));
+    if (j.N >= 3) {                  // scratch of eig_trd (every job) and of the K' reduction (trd jobs)
         GET(tm, (j.tag + "_trdm").c_str(), trd_workspace(j.N, j.nb));
         GET(te, (j.tag + "_trde").c_str(), (i64)j.N * j.nb);
         GET(tq, (j.tag + "_trdq").c_str(), trd_apply_workspace(j.N, j.nb, j.N));
-        j.trdw = tw; j.trdm = tm; j.trde = te; j.trdq = tq;
+        j.trdm = tm; j.trde = te; j.trdq = tq;
+        if (j.trd) {
+            GET(tw, (j.tag + "_trdw").c_str(), trd_workspace(j.N, j.nb));
+            j.trdw = tw;
+        }
     }
     for (int k = 0; k < 4; k++) p->hinfo[4 * j.slot + k] = 0;
     return 0;
@@ -1339,50 +1342,55 @@ __global__ void k_identity_batched(double *Z, int N, i64 sZ) {
     const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (e < (i64)N * N) Z[blockIdx.z * sZ + e] = (e % N) == (e / N) ? 1.0 : 0.0;
 }
-// metric eigen-decomposition of a trd job as dsyevd('V','U') leaves it (eigenvectors in S,
-// ascending eigenvalues in w1): S = Q T Q^T (nv2_trd.hip), T = Z W Z^T (rocsolver_dstedc on the
-// tridiagonal matrix), U = Q Z with Q applied in compact WY form.  Uses j.T as Z.
-static int pc_trd_metric(Stage2 *p, PcJob &j) {
+// eigen-decomposition of the nb matrices A + b*sA (order N, ld N, upper triangle read) as
+// dsyevd('V','U') leaves it: eigenvectors in A, ascending eigenvalues at ev + b*sD, info words at
+// dinfo_w[idx + b] (copied to hinfo).  A = Q T Q^T (nv2_trd.hip, one launch per column),
+// T = Z W Z^T (rocsolver_dstedc on the tridiagonal matrix), U = Q Z with Q applied in compact WY
+// form.  The job's T buffer holds Z.  On worker 1 the second problem's dstedc runs on the extra
+// stream and host thread (the solver blocks its calling thread at times).
+static int eig_trd(Stage2 *p, PcJob &j, int N, double *A, i64 sA, double *ev, i64 sD, int idx) {
     hipStream_t st = p->ws[j.w];
     rocblas_handle h = p->wh[j.w];
-    const int N = j.N;
-    const i64 NN = j.NN();
-    TRYS(trd_reduce(st, N, j.S, NN, j.nb, j.trdm));
-    const TrdOut o = trd_outputs(N, j.nb, j.trdm);
-    NV2_HIP(hipMemcpyAsync(j.w1, o.d, sizeof(double) * N * j.nb, hipMemcpyDeviceToDevice, st));
-    NV2_HIP(hipMemcpyAsync(j.trde, o.e, sizeof(double) * N * j.nb, hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(k_identity_batched, dim3(nb(NN, 256), 1, j.nb), dim3(256), 0, st, j.T, N, NN);
+    const int nbm = j.nb;
+    TRYS(trd_reduce(st, N, A, sA, nbm, j.trdm));
+    const TrdOut o = trd_outputs(N, nbm, j.trdm);
+    for (int b = 0; b < nbm; b++)
+        NV2_HIP(hipMemcpyAsync(ev + b * sD, o.d + (i64)b * N, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+    NV2_HIP(hipMemcpyAsync(j.trde, o.e, sizeof(double) * N * nbm, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_identity_batched, dim3(nb((i64)N * N, 256), 1, nbm), dim3(256), 0, st, j.T, N, sA);
     NV2_LAUNCH_CHECK();
-    // the two tridiagonal problems are independent: problem 1 runs on its own stream and host
-    // thread (the solver blocks its calling thread at times) while problem 0 runs here
+    const bool split = nbm > 1 && j.w == 1;
     int rc1 = 0;
     char err1[512] = "";
     std::thread t1;
-    if (j.nb > 1) {
+    if (split) {
         NV2_HIP(hipEventRecord(p->xev[0], st));
         t1 = std::thread([&]() {
             if (hipStreamWaitEvent(p->xs, p->xev[0], 0) != hipSuccess ||
-                rocsolver_dstedc(p->xh, rocblas_evect_tridiagonal, N, j.w1 + N, j.trde + N, j.T + NN, N,
-                                 p->dinfo_w + 4 * j.slot + 1) != rocblas_status_success ||
+                rocsolver_dstedc(p->xh, rocblas_evect_tridiagonal, N, ev + sD, j.trde + N, j.T + sA, N,
+                                 p->dinfo_w + idx + 1) != rocblas_status_success ||
                 hipEventRecord(p->xev[1], p->xs) != hipSuccess) {
                 rc1 = -1;
-                snprintf(err1, sizeof(err1), "dstedc of the second metric problem failed");
+                snprintf(err1, sizeof(err1), "dstedc of the second problem (N=%d) failed", N);
             }
         });
     }
-    const rocblas_status s0 = rocsolver_dstedc(h, rocblas_evect_tridiagonal, N, j.w1, j.trde, j.T, N,
-                                               p->dinfo_w + 4 * j.slot);
+    rocblas_status s0 = rocblas_status_success;
+    for (int b = 0; b < (split ? 1 : nbm) && s0 == rocblas_status_success; b++)
+        s0 = rocsolver_dstedc(h, rocblas_evect_tridiagonal, N, ev + b * sD, j.trde + (i64)b * N, j.T + b * sA, N,
+                              p->dinfo_w + idx + b);
     if (t1.joinable()) t1.join();
-    if (s0 != rocblas_status_success) { set_error("rocsolver_dstedc (metric, N=%d): status %d", N, (int)s0); return -1; }
+    if (s0 != rocblas_status_success) { set_error("rocsolver_dstedc (N=%d): status %d", N, (int)s0); return -1; }
     if (rc1) { set_error("%s", err1); return rc1; }
-    if (j.nb > 1) NV2_HIP(hipStreamWaitEvent(st, p->xev[1], 0));
-    TRYS(trd_apply_q(h, st, N, j.S, NN, j.nb, o.tau, false, N, j.T, NN, j.trdq));
-    NV2_HIP(hipMemcpyAsync(j.S, j.T, sizeof(double) * NN * j.nb, hipMemcpyDeviceToDevice, st));
-    NV2_HIP(hipMemcpyAsync(p->hinfo + 4 * j.slot, p->dinfo_w + 4 * j.slot, sizeof(int) * j.nb, hipMemcpyDeviceToHost, st));
+    if (split) NV2_HIP(hipStreamWaitEvent(st, p->xev[1], 0));
+    TRYS(trd_apply_q(h, st, N, A, sA, nbm, o.tau, false, N, j.T, sA, j.trdq));
+    for (int b = 0; b < nbm; b++)
+        NV2_HIP(hipMemcpyAsync(A + b * sA, j.T + b * sA, sizeof(double) * N * N, hipMemcpyDeviceToDevice, st));
+    NV2_HIP(hipMemcpyAsync(p->hinfo + idx, p->dinfo_w + idx, sizeof(int) * nbm, hipMemcpyDeviceToHost, st));
     return 0;
 }
 static int pc_metric_launch(Stage2 *p, PcJob &j) {
-    if (j.trd && j.N >= 3 && eigen_algo() == 0) TRYS(pc_trd_metric(p, j));
+    if (j.N >= 3 && eigen_algo() == 0) TRYS(eig_trd(p, j, j.N, j.S, j.NN(), j.w1, j.N, 4 * j.slot));
     else TRYS(syevd_async(p, j.w, 4 * j.slot, true, j.N, j.S, j.NN(), j.w1, j.N, j.nb));
     if (j.N) NV2_HIP(hipMemcpyAsync(j.hw1, j.w1, sizeof(double) * j.N * j.nb, hipMemcpyDeviceToHost, p->ws[j.w]));
     return 0;
@@ -1423,7 +1431,8 @@ static int pc_kprime_launch(Stage2 *p, PcJob &j, double thr) {
         NV2_LAUNCH_CHECK();
     }
     if (j.trd) return pc_trd_launch(p, j);
-    TRYS(syevd_async(p, j.w, 4 * j.slot + 2, true, nmax, j.Kp, NN, j.w2, N, j.nb));
+    if (nmax >= 3 && eigen_algo() == 0) TRYS(eig_trd(p, j, nmax, j.Kp, NN, j.w2, N, 4 * j.slot + 2));
+    else TRYS(syevd_async(p, j.w, 4 * j.slot + 2, true, nmax, j.Kp, NN, j.w2, N, j.nb));
     RB(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, N, nmax, nmax, &one, j.caux, N,
                                      NN, j.Kp, nmax, NN, &zero, j.coef, N, NN, j.nb));
     return 0;
@@ -1459,6 +1468,7 @@ struct Pm1pEarly {
     int rc = 0;
     char err[512] = "";
     PcJob job;
+    std::chrono::steady_clock::time_point t_start, t_metric, t_kready, t_end;   // for NV2_STAGE2_TIMING
 };
 static size_t pm1p_hoff(int n) {   // pinned-host offset of its metric eigenvalues: after the other jobs'
     const size_t nn = (size_t)n, m = nn * (nn + 1) / 2;
@@ -1477,17 +1487,21 @@ static void pm1p_stop(Stage2 *p) {   // abort a chain still waiting for K, and j
 }
 static int pm1p_worker(Stage2 *p, Pm1pEarly *E) {
     PcJob &j = E->job;
+    E->t_start = std::chrono::steady_clock::now();
     NV2_HIP(hipStreamWaitEvent(p->ws[j.w], p->evS, 0));
     TRYS(pc_metric_launch(p, j));
     NV2_HIP(hipStreamSynchronize(p->ws[j.w]));
+    E->t_metric = std::chrono::steady_clock::now();
     {
         std::unique_lock<std::mutex> lk(E->mu);
         E->cv.wait(lk, [E] { return E->k_ready || E->abort; });
         if (E->abort) return 0;
     }
+    E->t_kready = std::chrono::steady_clock::now();
     NV2_HIP(hipStreamWaitEvent(p->ws[j.w], p->evK, 0));
     TRYS(pc_kprime_launch(p, j, E->thr));
     NV2_HIP(hipStreamSynchronize(p->ws[j.w]));
+    E->t_end = std::chrono::steady_clock::now();
     return 0;
 }
 
@@ -2297,6 +2311,13 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         NV2_HIP(hipStreamSynchronize(st));
         fprintf(stderr, "[stage2 s=%d] fills %.2f  SC done %.2f  worker0 %.2f  worker1 %.2f  join %.2f  end %.2f ms\n", s,
                 ms_workers, ms_scq, ms_wdone[0], ms_wdone[1], ms_join, ms_since(t_begin));
+        if (has4) {
+            auto rel = [&](std::chrono::steady_clock::time_point t) {
+                return std::chrono::duration<double, std::milli>(t - t_begin).count();
+            };
+            fprintf(stderr, "[stage2 s=%d] (+-1)' chain: start %.2f  metric solved %.2f  K ready %.2f  K' reduced %.2f ms\n",
+                    s, rel(p->early->t_start), rel(p->early->t_metric), rel(p->early->t_kready), rel(p->early->t_end));
+        }
     }
     // ---- results: (norm_sc, e_sc, norm_pc, e_pc) per class, SC==PC for class (0)
     double accs[32];
