@@ -22,9 +22,11 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-CONFIGS = {   # name: (nact, ncore, nvirt, roots, description)
-    "c1": (6, 10, 40, 1, "CAS(6,6) nc=10 nv=40 single state"),
-    "c2": (10, 20, 200, 1, "CAS(10,10) nc=20 nv=200 single state"),
+CONFIGS = {   # name: (nact, ncore, nvirt, roots, determinants kept, description)
+    "c1": (6, 10, 40, 1, None, "CAS(6,6) nc=10 nv=40 single state"),
+    "c2": (10, 20, 200, 1, None, "CAS(10,10) nc=20 nv=200 single state"),
+    "c3": (14, 30, 400, 1, 200000, "CAS(14,14) nc=30 nv=400 single state, CI truncated to its 2e5 largest determinants"),
+    "c4": (12, 25, 300, 6, None, "CAS(12,12) nc=25 nv=300, 6 roots (QD-NEVPT2)"),
 }
 METRIC = "SC+PC-NEVPT2 evals/sec CAS(12-14) mixed vs FP64 at |ΔE|≤1µEh; p50/p99 lat"
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12}
@@ -90,6 +92,16 @@ def cpu_baseline(P, frac_pairs=0.05):
             "seconds_per_eval": {"stage1_small": t_small, "bamat": t_bamat, "stage2": t_s2, "total": t_total}}
 
 
+def host_rdms(P, ev):
+    """The oracle's inputs: ro4 and its partial traces as built on the GPU, copied to the host."""
+    n, M = P.nact, P.metat
+    nw = n * (n + 1) * (n + 2) * (n + 3) // 24
+    P.ro4 = ev.get("ro4", (nw, n, n, n, n, M))
+    P.daaa = ev.get("3-RDM", (M,) + (n,) * 6)
+    P.taa = ev.get("2-RDM", (M,) + (n,) * 4)
+    P.dal = ev.get("1-RDM", (M, n, n))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,11 +121,14 @@ def main():
     from nevpt2_amd import synth
     from nevpt2_amd.evaluator import Evaluator, PHASES
     from nevpt2_amd.serve import Server
-    n, nc, nv, M, desc = CONFIGS[args.config]
+    n, nc, nv, M, keep, desc = CONFIGS[args.config]
     S = max(1, args.concurrency)
     print(f"[bench] generating {desc} ...", file=sys.stderr, flush=True)
-    P = synth.Problem(n, nc, nv, seed=1, nroots=M)
-    evs = [Evaluator.for_problem(P) for _ in range(S)]   # each request: its own context and input copy in HBM
+    P = synth.Problem(n, nc, nv, seed=1, nroots=M, keep=keep, host_rdms=False)
+    t = time.perf_counter()
+    evs = [Evaluator.for_problem(P, rdms="gpu") for _ in range(S)]   # each request: its own context and inputs in HBM
+    t_rdm = (time.perf_counter() - t) / S
+    print(f"[bench] RDMs built on the GPU from the CI vector(s): {t_rdm:.2f} s per context", file=sys.stderr, flush=True)
     prec = 0 if args.prec == "fp64" else 1
     for ev in evs:
         for _ in range(args.warmup):
@@ -174,8 +189,10 @@ def main():
         out["vs_fp64"] = {"max_abs_dE_sc": float(d[:, :, 1].max()), "max_abs_dE_pc": float(d[:, :, 3].max()),
                           "abs_dE_total_sc": float(abs(res[:, :, 1].sum() - ref[:, :, 1].sum())),
                           "abs_dE_total_pc": float(abs(res[:, :, 3].sum() - ref[:, :, 3].sum())), "unit": "Eh"}
+    out["setup"] = {"rdm_build_s_per_context": t_rdm}
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU oracle on a bounded sample ...", file=sys.stderr, flush=True)
+        host_rdms(P, evs[0])
         out["cpu_baseline"] = cpu_baseline(P)
     for ev in evs:
         ev.close()

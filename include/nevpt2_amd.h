@@ -91,6 +91,19 @@ typedef struct nv2_ctx nv2_ctx;
 nv2_ctx *nv2_ctx_create(int64_t ncore, int64_t nact, int64_t nvirt, int64_t metat, int64_t nele, int64_t nchovec);
 int nv2_ctx_set_integrals(nv2_ctx *ctx, const double *cholesky, const double *oneint);
 int nv2_ctx_set_ro4(nv2_ctx *ctx, const double *ro4);
+/* Spin-free RDMs from CI vectors on the GPU, written into the context: the
+ * packed 4-RDM ro4 of every state and, for metat >= 2, the transition 3-/2-/1-RDMs
+ * of the state pairs (as nv2_ctx_set_trdm takes them, bra = state I > J = ket).
+ * Replaces rofour and ro3off/ro2off/ro1off (koopro4QD.F90:921-1570, 1978-2411,
+ * 2620-2860, 2936-3014) with their drivers state_specific_densities_driver /
+ * transition_densities_driver (koopro4QD.F90:504-917), which read the CI vector
+ * from file04 instead.  ci holds metat vectors C[I_alpha][I_beta] (beta string
+ * fastest) over all strings of nalpha / nbeta electrons in nact orbitals, each
+ * string list in lexicographic order of the ascending occupied-orbital tuple;
+ * a truncated expansion is the same array with zeros.  nalpha + nbeta must
+ * equal the context's nele; nact <= 16.  Takes device memory of its own while it
+ * runs (the CI vectors, the spin blocks and a GEMM panel of at most 2 GiB). */
+int nv2_ctx_set_ci(nv2_ctx *ctx, int64_t nalpha, int64_t nbeta, const double *ci);
 /* Transition RDMs of the state pairs for QD-NEVPT2 (metat >= 2): the nevpt.h5
  * datasets 3-TRDM (P,n^6), 2-TRDM (P,n^4), 1-TRDM (P,n,n), P = metat(metat-1)/2,
  * pair (I>J) at (I-1)(I-2)/2+J (koopro4QD.F90:433-477, read at qdnevpt.F:792-852).
@@ -104,6 +117,15 @@ int nv2_ctx_run(nv2_ctx *ctx, int32_t prec, double thr_lindep, double *out);
 int nv2_ctx_timings(nv2_ctx *ctx, double *ms);
 int nv2_ctx_get(nv2_ctx *ctx, const char *name, double *host, int64_t count);
 void nv2_ctx_destroy(nv2_ctx *ctx);
+
+/* One spin-free k-RDM (k = 1..4) or transition k-RDM from CI vectors on the GPU:
+ * out[p1 + n*p2 + ... + n^(2k-1)*qk] = <bra| sum over spins a+_{p1}..a+_{pk} a_{qk}..a_{q1} |ket>
+ * (creator p_i paired with annihilator q_i, the convention of the reference's
+ * RDM arrays, e.g. eexx_functions.F90:125-133).  bra/ket in the nv2_ctx_set_ci
+ * layout (one vector each); ket NULL means ket = bra.  The host-side form of the
+ * builders behind nv2_ctx_set_ci (koopro4QD.F90:921-3014). */
+int nv2_ci_rdm(int64_t nact, int64_t nalpha, int64_t nbeta, int32_t k, const double *bra, const double *ket,
+               double *out);
 
 /* QD-NEVPT2 effective Hamiltonian (qdnevpt.F:1046-1204 with hermitian_heff,
  * :8089-8184): adds e0 to the diagonal of the second-order heff2 (metat x metat,

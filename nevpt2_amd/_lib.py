@@ -50,6 +50,8 @@ def _declare(L):
     L.nv2_ctx_set_integrals.argtypes = [C.c_void_p, dp, dp]
     L.nv2_ctx_set_ro4.argtypes = [C.c_void_p, dp]
     L.nv2_ctx_set_trdm.argtypes = [C.c_void_p, dp, dp, dp]
+    L.nv2_ctx_set_ci.argtypes = [C.c_void_p, i64, i64, dp]
+    L.nv2_ci_rdm.argtypes = [i64, i64, i64, i32, dp, dp, dp]
     L.nv2_qd_heff.argtypes = [i64, dp, dp, dp, dp]
     L.nv2_ctx_run.argtypes = [C.c_void_p, i32, C.c_double, dp]
     L.nv2_ctx_timings.argtypes = [C.c_void_p, dp]

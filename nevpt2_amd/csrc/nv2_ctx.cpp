@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "nv2_common.h"
+#include "nv2_rdm.h"
 #include "nv2_stage1.h"
 #include "nv2_stage2.h"
 #include "../../include/nevpt2_amd.h"
@@ -152,6 +153,29 @@ int nv2_ctx_set_trdm(nv2_ctx *c, const double *trdm3, const double *trdm2, const
     NV2_HIP(hipStreamSynchronize(c->st));
     c->has_trdm = true;
     return 0;
+}
+
+int nv2_ctx_set_ci(nv2_ctx *c, int64_t nalpha, int64_t nbeta, const double *ci) {
+    if (!c || !ci) { set_error("nv2_ctx_set_ci: null argument"); return -2; }
+    if (nalpha < 0 || nbeta < 0 || nalpha > c->n || nbeta > c->n || nalpha + nbeta != c->nele) {
+        set_error("nv2_ctx_set_ci: nalpha=%lld nbeta=%lld do not fit CAS(%d,%d)", (long long)nalpha, (long long)nbeta,
+                  c->nele, c->n);
+        return -2;
+    }
+    const i64 ndet = ci_nstrings(c->n, (int)nalpha) * ci_nstrings(c->n, (int)nbeta);
+    double *dC = nullptr;
+    NV2_HIP(hipMalloc(&dC, sizeof(double) * ndet * c->M));
+    int rc = hipMemcpyAsync(dC, ci, sizeof(double) * ndet * c->M, hipMemcpyHostToDevice, c->st) == hipSuccess ? 0 : -1;
+    if (rc) set_error("nv2_ctx_set_ci: upload of the CI vectors failed");
+    const bool qd = c->M > 1;
+    if (!rc)
+        rc = ci_fill_rdms(c->n, (int)nalpha, (int)nbeta, c->M, dC, c->rb, c->st, c->find("ro4"),
+                          qd ? c->find("3-TRDM") : nullptr, qd ? c->find("2-TRDM") : nullptr,
+                          qd ? c->find("1-TRDM") : nullptr);
+    (void)hipStreamSynchronize(c->st);
+    (void)hipFree(dC);
+    if (!rc && qd) c->has_trdm = true;
+    return rc;
 }
 
 int nv2_ctx_set_ro4(nv2_ctx *c, const double *ro4) {

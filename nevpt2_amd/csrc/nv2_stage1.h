@@ -22,6 +22,8 @@ struct Stage1Out {            // every array: state index fastest, reference sha
 };
 
 int launch_bro(const Rdm &R, int nele, double *daaa, double *taa, double *dal, hipStream_t st);
+// bro2 and bro1 only, from R.d3 with R.M states (or state pairs)
+int launch_bro21(const Rdm &R, int nele, double *taa, double *dal, hipStream_t st);
 int launch_aj_atwo(int n, int ncore, int norb, int L, const double *B, const double *oneint, double *aj,
                    double *atwo, hipStream_t st);
 int launch_make_f(int n, int ncore, const double *aj, const double *atwo, double alpha, double *F, hipStream_t st);

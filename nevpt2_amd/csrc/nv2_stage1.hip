@@ -433,6 +433,16 @@ int launch_bro(const Rdm &R, int nele, double *daaa, double *taa, double *dal, h
     return 0;
 }
 
+int launch_bro21(const Rdm &R, int nele, double *taa, double *dal, hipStream_t st) {
+    const i64 n = R.n, M = R.M;
+    hipLaunchKernelGGL(k_bro2, dim3(nblk(M * n * n * n * n, 256)), dim3(256), 0, st, R, nele, taa);
+    NV2_LAUNCH_CHECK();
+    Rdm R2 = R; R2.d2 = taa;
+    hipLaunchKernelGGL(k_bro1, dim3(nblk(M * n * n, 256)), dim3(256), 0, st, R2, nele, dal);
+    NV2_LAUNCH_CHECK();
+    return 0;
+}
+
 int launch_aj_atwo(int n, int ncore, int norb, int L, const double *B, const double *oneint, double *aj,
                    double *atwo, hipStream_t st) {
     NV2_HIP(hipMemcpyAsync(aj, oneint, sizeof(double) * (i64)norb * (norb + 1) / 2, hipMemcpyDeviceToDevice, st));

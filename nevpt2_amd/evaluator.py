@@ -24,13 +24,23 @@ class Evaluator:
         self._h = h
 
     @classmethod
-    def for_problem(cls, P):
+    def for_problem(cls, P, rdms="host"):
+        """Context for a synthetic problem; rdms="gpu" builds the RDMs on the device from P.C."""
         ev = cls(P.ncore, P.nact, P.nvirt, P.metat, P.nele, P.B.shape[0])
         ev.set_integrals(P.B, P.oneint)
+        if rdms == "gpu":
+            ev.set_ci(P.C, P.nele // 2, P.nele - P.nele // 2)
+            return ev
         ev.set_ro4(P.ro4)
         if P.metat > 1:
             ev.set_trdm(P.daaaoff, P.taaoff, P.daloff)
         return ev
+
+    def set_ci(self, C, nalpha, nbeta):
+        """Builds ro4 (and, for several states, the transition RDMs) on the GPU from CI vectors
+        C[root, I_alpha, I_beta] over lexicographic strings."""
+        Cc = np.ascontiguousarray(C, dtype=np.float64)
+        L.check(L.lib().nv2_ctx_set_ci(self._h, int(nalpha), int(nbeta), L.ptr(Cc)), "nv2_ctx_set_ci")
 
     def set_trdm(self, trdm3, trdm2, trdm1):
         """Transition RDMs (3-TRDM, 2-TRDM, 1-TRDM) of the state pairs, for QD-NEVPT2."""
@@ -87,3 +97,13 @@ class Evaluator:
             self.close()
         except Exception:
             pass
+
+
+def ci_rdm(n, nalpha, nbeta, k, bra, ket=None):
+    """Spin-free k-RDM (or transition k-RDM) from CI vectors on the GPU, as an array [p1..pk, q1..qk]."""
+    b = np.ascontiguousarray(bra, dtype=np.float64)
+    kt = None if ket is None else np.ascontiguousarray(ket, dtype=np.float64)
+    out = np.zeros(n ** (2 * k))
+    L.check(L.lib().nv2_ci_rdm(int(n), int(nalpha), int(nbeta), int(k), L.ptr(b), None if kt is None else L.ptr(kt),
+                               L.ptr(out)), "nv2_ci_rdm")
+    return out.reshape((n,) * (2 * k), order="F")

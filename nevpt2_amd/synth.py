@@ -297,7 +297,9 @@ class Problem:
       transition RDMs (ncoppie, ...) when metat > 1: daaaoff, taaoff, daloff
     """
 
-    def __init__(self, nact, ncore, nvirt, seed=1, nroots=1, keep=None, rank_factor=3, nele=None):
+    def __init__(self, nact, ncore, nvirt, seed=1, nroots=1, keep=None, rank_factor=3, nele=None, host_rdms=True):
+        """host_rdms=False keeps only the CI vectors ``C`` (the RDMs are then built on the GPU from them,
+        ``Evaluator.for_problem(P, rdms="gpu")``); ro4 and the lower RDMs are None."""
         from . import rdmtrace
         self.nact, self.ncore, self.nvirt = nact, ncore, nvirt
         self.norb = nact + ncore + nvirt
@@ -306,6 +308,14 @@ class Problem:
         self.seed = seed
         n, M = nact, nroots
         self.C = random_ci(n, self.nele, seed, nroots=nroots, keep=keep)
+        self.nalpha = self.nbeta = self.nele // 2
+        self.ncoppie = M * (M - 1) // 2
+        self.B, self.oneint, self.eps0 = random_integrals(ncore, nact, nvirt, seed, rank_factor)
+        self.nchovec = self.B.shape[0]
+        if not host_rdms:
+            self.ro4 = self.daaa = self.taa = self.dal = None
+            self.daaaoff = self.taaoff = self.daloff = None
+            return
         blk = _Blocks(n, self.nele)
         self.ro4 = np.zeros((nwords(n), n, n, n, n, M), order="F")
         for s in range(M):
@@ -313,7 +323,6 @@ class Problem:
             self.ro4[..., s] = pack_ro4(G4)
             del G4
         self.daaa, self.taa, self.dal = rdmtrace.traces_from_ro4(self.ro4, n, self.nele)
-        self.ncoppie = M * (M - 1) // 2
         if self.ncoppie:
             P = self.ncoppie
             self.daaaoff = np.zeros((P,) + (n,) * 6, order="F")
@@ -327,5 +336,3 @@ class Problem:
                     ij += 1
             self.taaoff = rdmtrace.bro2(self.daaaoff, n, self.nele)
             self.daloff = rdmtrace.bro1(self.taaoff, n, self.nele)
-        self.B, self.oneint, self.eps0 = random_integrals(ncore, nact, nvirt, seed, rank_factor)
-        self.nchovec = self.B.shape[0]
