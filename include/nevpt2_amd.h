@@ -111,7 +111,11 @@ int nv2_ctx_set_ci(nv2_ctx *ctx, int64_t nalpha, int64_t nbeta, const double *ci
  * driver adds to e2mp(j,i)/e2en(j,i) (e.g. qdnevpt.F:1936-1943, 2056-2070) and
  * nv2_ctx_get returns "Heff-SC" and "Heff-PC" (metat x metat, element (j,i) at
  * j + metat*i, second order without E0) and "QD" (per state i: 8 classes x
- * (SC row e2mp(.,i), PC row e2en(.,i)), 16*metat doubles per state). */
+ * (SC row e2mp(.,i), PC row e2en(.,i)), 16*metat doubles per state).
+ * "PC-metric" (after a run, any metat): per state and class, 24 doubles per state,
+ * (smallest metric eigenvalue, eigenvalues kept, 1 if the metric was factored by
+ * Cholesky because none falls below thr_lindep -- the smallest is then reported
+ * as thr_lindep, a lower bound -- else 0), to flag threshold-sensitive inputs. */
 int nv2_ctx_set_trdm(nv2_ctx *ctx, const double *trdm3, const double *trdm2, const double *trdm1);
 int nv2_ctx_run(nv2_ctx *ctx, int32_t prec, double thr_lindep, double *out);
 int nv2_ctx_timings(nv2_ctx *ctx, double *ms);

@@ -17,8 +17,9 @@ struct S2Inputs {             // device pointers, reference layouts (state index
 };
 
 struct S2Stats {              // diagnostics for threshold-sensitive seeds
-    double min_metric_eig[8];
+    double min_metric_eig[8];    // on the Cholesky path: thr, a lower bound
     int nnew[8];
+    int chol[8];                 // 1: the metric was factored (every eigenvalue at or above thr)
     double eps_min_virt, eps_max_core;
 };
 

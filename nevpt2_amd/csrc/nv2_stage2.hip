@@ -25,6 +25,7 @@
 
 #include "nv2_common.h"
 #include "nv2_stage2.h"
+#include "nv2_chol.h"
 #include "nv2_trd.h"
 #include "../../include/nevpt2_amd.h"
 
@@ -1301,6 +1302,7 @@ struct PcJob {
     std::string tag;
     int N = 0, nb = 1, w = 0, slot = 0;        // info words at 4*slot + {metric b, K' 2+b}
     bool trd = false;                          // K' only reduced (the (+-1)' sums solve with its T)
+    bool chol = false;                         // metric factored as S = U^T U (every eigenvalue kept)
     double *trdw = nullptr;                    // K' reduction scratch; T and tau of problem b from trd_outputs(nmax)
     double *trdm = nullptr, *trde = nullptr;   // eig_trd: reduction scratch, off-diagonal copy for dstedc
     double *trdq = nullptr;                    // scratch of applying Q
@@ -1337,7 +1339,6 @@ static int pc_job_buffers(Stage2 *p, PcJob &j, size_t hoff) {
     return 0;
 }
 // K' = Q T Q^T of a trd job
-static int pc_trd_launch(Stage2 *p, PcJob &j) { return trd_reduce(p->ws[j.w], j.nmax, j.Kp, j.NN(), j.nb, j.trdw); }
 __global__ void k_identity_batched(double *Z, int N, i64 sZ) {
     const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (e < (i64)N * N) Z[blockIdx.z * sZ + e] = (e % N) == (e / N) ? 1.0 : 0.0;
@@ -1348,11 +1349,42 @@ __global__ void k_identity_batched(double *Z, int N, i64 sZ) {
 // T = Z W Z^T (rocsolver_dstedc on the tridiagonal matrix), U = Q Z with Q applied in compact WY
 // form.  The job's T buffer holds Z.  On worker 1 the second problem's dstedc runs on the extra
 // stream and host thread (the solver blocks its calling thread at times).
+static bool stage2_timing();
+// device time of the three steps of eig_trd for orders >= 500 (NV2_STAGE2_TIMING), printed at its end
+struct EigTiming {
+    hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool on = false;
+    EigTiming(int N) : on(stage2_timing() && N >= 500) {
+        if (on) for (auto &x : e) if (hipEventCreate(&x) != hipSuccess) on = false;
+    }
+    void mark(int k, hipStream_t st) { if (on) (void)hipEventRecord(e[k], st); }
+    void report(int N, int nbm) {
+        if (!on) return;
+        float t[3] = {0, 0, 0};
+        (void)hipEventSynchronize(e[3]);
+        for (int k = 0; k < 3; k++) (void)hipEventElapsedTime(&t[k], e[k], e[k + 1]);
+        fprintf(stderr, "[eig_trd N=%d x%d] reduce %.2f  dstedc %.2f  Q %.2f ms\n", N, nbm, t[0], t[1], t[2]);
+    }
+    ~EigTiming() { for (auto &x : e) if (x) (void)hipEventDestroy(x); }
+};
+static int pc_trd_launch(Stage2 *p, PcJob &j) {
+    EigTiming tm(j.nmax);
+    tm.mark(0, p->ws[j.w]);
+    TRYS(trd_reduce(p->ws[j.w], j.nmax, j.Kp, j.NN(), j.nb, j.trdw));
+    if (tm.on) {
+        for (int k = 1; k < 4; k++) tm.mark(k, p->ws[j.w]);
+        tm.report(j.nmax, j.nb);       // the reduction of K' only: the first figure
+    }
+    return 0;
+}
 static int eig_trd(Stage2 *p, PcJob &j, int N, double *A, i64 sA, double *ev, i64 sD, int idx) {
     hipStream_t st = p->ws[j.w];
     rocblas_handle h = p->wh[j.w];
     const int nbm = j.nb;
+    EigTiming tm(N);
+    tm.mark(0, st);
     TRYS(trd_reduce(st, N, A, sA, nbm, j.trdm));
+    tm.mark(1, st);
     const TrdOut o = trd_outputs(N, nbm, j.trdm);
     for (int b = 0; b < nbm; b++)
         NV2_HIP(hipMemcpyAsync(ev + b * sD, o.d + (i64)b * N, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
@@ -1383,13 +1415,54 @@ static int eig_trd(Stage2 *p, PcJob &j, int N, double *A, i64 sA, double *ev, i6
     if (s0 != rocblas_status_success) { set_error("rocsolver_dstedc (N=%d): status %d", N, (int)s0); return -1; }
     if (rc1) { set_error("%s", err1); return rc1; }
     if (split) NV2_HIP(hipStreamWaitEvent(st, p->xev[1], 0));
+    tm.mark(2, st);
     TRYS(trd_apply_q(h, st, N, A, sA, nbm, o.tau, false, N, j.T, sA, j.trdq));
+    tm.mark(3, st);
+    tm.report(N, nbm);
     for (int b = 0; b < nbm; b++)
         NV2_HIP(hipMemcpyAsync(A + b * sA, j.T + b * sA, sizeof(double) * N * N, hipMemcpyDeviceToDevice, st));
     NV2_HIP(hipMemcpyAsync(p->hinfo + idx, p->dinfo_w + idx, sizeof(int) * nbm, hipMemcpyDeviceToHost, st));
     return 0;
 }
-static int pc_metric_launch(Stage2 *p, PcJob &j) {
+// The (+-1)' metric without its eigen-decomposition.  When no eigenvalue falls below thr, the
+// thresholded inverse U_kept w^-1 U_kept^T of the reference is S^-1 itself, and every PC sum
+// r^T (cauxT K caux + s)^-1 r, r = caux^T x, is unchanged when caux is replaced by any G with
+// G G^T = S^-1 (G = caux O for an orthogonal O).  The Cholesky factor S = U^T U gives G = U^-1:
+// K' = U^-T K U^-1 and r = U^-T x by triangular solves.  "No eigenvalue below thr" is decided by
+// the Cholesky factorisation of S - thr*I succeeding; otherwise the eigen path runs.
+// NV2_METRIC_EIGEN=1 forces the eigen path.
+static bool chol_metric() {   // read at every solve, so that a caller can switch paths between runs
+    const char *e = getenv("NV2_METRIC_EIGEN");
+    return !(e && atoi(e) > 0);
+}
+__global__ void k_shift_diag(double *A, int N, i64 sA, int nbm, double d) {
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (i64)N * nbm) return;
+    const i64 b = t / N, i = t % N;
+    A[b * sA + i + (i64)N * i] += d;
+}
+static int pc_metric_launch(Stage2 *p, PcJob &j, double thr) {
+    if (j.trd && j.N >= 3 && chol_metric()) {     // info words: 4*slot + b for S - thr*I, + 2 + b for S
+        j.chol = true;
+        hipStream_t st = p->ws[j.w];
+        rocblas_handle h = p->wh[j.w];
+        const i64 NN = j.NN();
+        NV2_HIP(hipMemcpyAsync(j.T, j.S, sizeof(double) * NN * j.nb, hipMemcpyDeviceToDevice, st));
+        NV2_HIP(hipMemcpyAsync(j.caux, j.S, sizeof(double) * NN * j.nb, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_shift_diag, dim3(nb((i64)j.N * j.nb, 256)), dim3(256), 0, st, j.T, j.N, NN, j.nb, -thr);
+        NV2_LAUNCH_CHECK();
+        EigTiming tm(j.N);
+        tm.mark(0, st);
+        TRYS(chol_upper(h, st, j.N, j.T, NN, j.nb, p->dinfo_w + 4 * j.slot));
+        tm.mark(1, st);
+        TRYS(chol_upper(h, st, j.N, j.caux, NN, j.nb, p->dinfo_w + 4 * j.slot + 2));
+        tm.mark(2, st);
+        tm.mark(3, st);
+        tm.report(j.N, j.nb);            // "reduce": S - thr*I factored, "dstedc": S factored
+        NV2_HIP(hipMemcpyAsync(p->hinfo + 4 * j.slot, p->dinfo_w + 4 * j.slot, sizeof(int) * 4, hipMemcpyDeviceToHost,
+                               st));
+        return 0;
+    }
     if (j.N >= 3 && eigen_algo() == 0) TRYS(eig_trd(p, j, j.N, j.S, j.NN(), j.w1, j.N, 4 * j.slot));
     else TRYS(syevd_async(p, j.w, 4 * j.slot, true, j.N, j.S, j.NN(), j.w1, j.N, j.nb));
     if (j.N) NV2_HIP(hipMemcpyAsync(j.hw1, j.w1, sizeof(double) * j.N * j.nb, hipMemcpyDeviceToHost, p->ws[j.w]));
@@ -1398,6 +1471,35 @@ static int pc_metric_launch(Stage2 *p, PcJob &j) {
 // after the worker stream has finished: threshold the metric spectra, launch K' and its solves
 static int pc_kprime_launch(Stage2 *p, PcJob &j, double thr) {
     const int N = j.N;
+    if (j.chol) {                        // the factorisations of pc_metric_launch have finished
+        bool ok = true;
+        for (int b = 0; b < j.nb; b++) {
+            if (p->hinfo[4 * j.slot + b] != 0 || p->hinfo[4 * j.slot + 2 + b] != 0) ok = false;
+            p->hinfo[4 * j.slot + b] = p->hinfo[4 * j.slot + 2 + b] = 0;
+        }
+        hipStream_t st = p->ws[j.w];
+        rocblas_handle h = p->wh[j.w];
+        const i64 NN = j.NN();
+        if (stage2_timing() && N >= 500) fprintf(stderr, "[metric %s N=%d] Cholesky path: %s\n", j.tag.c_str(), N,
+                                                 ok ? "every eigenvalue above thr" : "not positive definite above thr");
+        if (ok) {
+            j.nmax = N;
+            for (int b = 0; b < j.nb; b++) { j.nnew[b] = N; j.minw[b] = thr; }   // thr: a lower bound
+            for (int b = 0; b < j.nb; b++)
+                hipLaunchKernelGGL(k_sym_upper, dim3(nb(NN, 256)), dim3(256), 0, st, j.Kb(b), N, j.Kp + b * NN);
+            NV2_LAUNCH_CHECK();
+            const double one = 1.0;
+            RB(rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
+                                             rocblas_diagonal_non_unit, N, N, &one, j.caux, N, NN, j.Kp, N, NN, j.nb));
+            RB(rocblas_dtrsm_strided_batched(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                                             rocblas_diagonal_non_unit, N, N, &one, j.caux, N, NN, j.Kp, N, NN, j.nb));
+            return pc_trd_launch(p, j);
+        }
+        j.chol = false;                  // some eigenvalue below thr: the eigen path after all
+        TRYS(eig_trd(p, j, N, j.S, NN, j.w1, N, 4 * j.slot));
+        NV2_HIP(hipMemcpyAsync(j.hw1, j.w1, sizeof(double) * N * j.nb, hipMemcpyDeviceToHost, st));
+        NV2_HIP(hipStreamSynchronize(st));
+    }
     j.nmax = 0;
     for (int b = 0; b < j.nb; b++) {
         const int inf = p->hinfo[4 * j.slot + b];
@@ -1409,6 +1511,8 @@ static int pc_kprime_launch(Stage2 *p, PcJob &j, double thr) {
         if (nnew > j.nmax) j.nmax = nnew;
     }
     const int nmax = j.nmax;
+    if (stage2_timing() && N >= 500)
+        fprintf(stderr, "[metric %s N=%d] kept %d / %d\n", j.tag.c_str(), N, j.nnew[0], j.nb > 1 ? j.nnew[1] : -1);
     if (nmax == 0) return 0;
     hipStream_t st = p->ws[j.w];
     rocblas_handle h = p->wh[j.w];
@@ -1442,7 +1546,7 @@ static int pc_kprime_launch(Stage2 *p, PcJob &j, double thr) {
 static int pc_worker(Stage2 *p, PcJob *J, int njob, int w, double thr, double *ms_done,
                      std::chrono::steady_clock::time_point t0) {
     for (int k = 0; k < njob; k++)
-        if (J[k].w == w) TRYS(pc_metric_launch(p, J[k]));
+        if (J[k].w == w) TRYS(pc_metric_launch(p, J[k], thr));
     NV2_HIP(hipStreamSynchronize(p->ws[w]));
     for (int k = 0; k < njob; k++)
         if (J[k].w == w) TRYS(pc_kprime_launch(p, J[k], thr));
@@ -1489,7 +1593,7 @@ static int pm1p_worker(Stage2 *p, Pm1pEarly *E) {
     PcJob &j = E->job;
     E->t_start = std::chrono::steady_clock::now();
     NV2_HIP(hipStreamWaitEvent(p->ws[j.w], p->evS, 0));
-    TRYS(pc_metric_launch(p, j));
+    TRYS(pc_metric_launch(p, j, E->thr));
     NV2_HIP(hipStreamSynchronize(p->ws[j.w]));
     E->t_metric = std::chrono::steady_clock::now();
     {
@@ -2261,7 +2365,7 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         for (int b = 0; b < J[k].nb; b++) {
             const int c = cls_of[k][b];
             if (k == JPM2P) { if (J[k].minw[b] < sd.min_metric_eig[c]) sd.min_metric_eig[c] = J[k].minw[b]; }
-            else { sd.min_metric_eig[c] = J[k].minw[b]; sd.nnew[c] = J[k].nnew[b]; }
+            else { sd.min_metric_eig[c] = J[k].minw[b]; sd.nnew[c] = J[k].nnew[b]; sd.chol[c] = J[k].chol ? 1 : 0; }
         }
     // ================================================================ classes (-1)' and (+1)', PC sums
     // sum_mu r_mu^2 / (w_mu + s_x) with r = C^T caux^T X_x and K' = C W C^T (qdnevpt.F:5351-5422,
@@ -2282,7 +2386,16 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         const int ncq = 1 + (q.M ? M - 1 : 0);              // right-hand sides: this state, then the QD partners
         GET(Ypc, "pm1_Ypc", (i64)nmax * next * ncq);
         GET(Wq, "pm1_applyq", trd_apply_workspace(nmax, 1, next * ncq));
-        TRYS(gemm(p, true, false, nmax, next, (int)N3, 1.0, caux, N3, Xpm[plus], N3, 0.0, Ypc, nmax));
+        // y = caux^T x, or U^-T x on the Cholesky path (nmax = N3 there)
+        auto project = [&](const double *x, double *y) -> int {
+            if (!jb.chol) return gemm(p, true, false, nmax, next, (int)N3, 1.0, caux, N3, x, N3, 0.0, y, nmax);
+            NV2_HIP(hipMemcpyAsync(y, x, sizeof(double) * N3 * next, hipMemcpyDeviceToDevice, st));
+            const double one = 1.0;
+            RB(rocblas_dtrsm(p->h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
+                             rocblas_diagonal_non_unit, (int)N3, next, &one, caux, (int)N3, y, nmax));
+            return 0;
+        };
+        TRYS(project(Xpm[plus], Ypc));
         int col = 1;
         for (int j = 0; q.M && j < M; j++) {
             if (j == s) continue;
@@ -2293,7 +2406,7 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             TRYS(gemm(p, true, false, (int)N3, next, (int)N3, 1.0, mj, N3, Vpm[plus], N3, 0.0, xj, N3));
             TRYS(gemm(p, false, false, (int)N3, next, n, 1.0, qj, N3, Hpm[plus], n, 1.0, xj, N3));
             double *yj = Ypc + (i64)nmax * next * col++;
-            TRYS(gemm(p, true, false, nmax, next, (int)N3, 1.0, caux, N3, xj, N3, 0.0, yj, nmax));
+            TRYS(project(xj, yj));
             q.a[j] = yj;
         }
         TRYS(trd_apply_q(p->h, st, nmax, Kp, NN, 1, tauT, true, next * ncq, Ypc, 0, Wq));   // z = Q^T y

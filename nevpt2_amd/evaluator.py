@@ -63,6 +63,12 @@ class Evaluator:
         L.check(L.lib().nv2_ctx_get(self._h, b"QD", L.ptr(q), q.size), "nv2_ctx_get(QD)")
         return q.reshape(M, 8, 2, M)
 
+    def pc_metric(self):
+        """Per state and class: (smallest metric eigenvalue, kept, Cholesky path flag), shape (metat, 8, 3)."""
+        a = np.zeros(24 * self.M)
+        L.check(L.lib().nv2_ctx_get(self._h, b"PC-metric", L.ptr(a), a.size), "nv2_ctx_get(PC-metric)")
+        return a.reshape(self.M, 8, 3)
+
     def set_integrals(self, cholesky, oneint):
         B = L.fin(cholesky)
         one = np.ascontiguousarray(oneint, dtype=np.float64)
