@@ -3,11 +3,16 @@
 // Cholesky step (it diagonalises every metric, qdnevpt.F:5723-5797); the factorisation stands in
 // for that eigen-decomposition when no eigenvalue falls below the threshold.
 //
-// Right-looking by block rows of CHOL_NB: the diagonal block is factored in LDS by one workgroup
-// per matrix (k_chol_diag), the block row to its right becomes U11^-T A12 (rocBLAS trsm), and the
-// trailing matrix takes the update A22 -= U12^T U12 as a full GEMM (its strict lower triangle is
-// never read again; rocBLAS syrk measured 2-3x slower here than the MFMA GEMM kernels).  About
-// 3 N / CHOL_NB launches instead of the several launches per column of an unblocked panel.
+// Right-looking by block rows of CHOL_NB, four launches per block row, all batched over the
+// matrices:
+//   k_chol_diag   factors the diagonal block in LDS (one workgroup per matrix) and writes
+//                 U11 and its inverse;
+//   GEMM          P = U11^-T A12 into a panel buffer (the triangular solve as a product with the
+//                 explicit inverse of a 64 x 64 block: rocBLAS trsm costs several helper kernels
+//                 per call);
+//   GEMM          A22 -= P^T P over the whole trailing block (its strict lower triangle is never
+//                 read again; rocBLAS syrk measured 2-3x slower here than the MFMA GEMM kernels);
+//   k_chol_put    copies P into the block row of U.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
@@ -18,11 +23,14 @@ namespace nv2 {
 #define CHOL_NB 64
 #define CHOL_WG 256
 
-// Factor the kb x kb diagonal block at (k0, k0) of matrix b in LDS.  info[b] = k0 + j + 1 for
-// the first pivot that is not positive (NaN included); a matrix whose info is already set is
-// left alone.
-__global__ __launch_bounds__(CHOL_WG) void k_chol_diag(double *A, int N, i64 sA, int k0, int kb, int *info) {
-    __shared__ double a[CHOL_NB][CHOL_NB + 1];      // a[r][c], upper triangle used
+// Factor the kb x kb diagonal block at (k0, k0) of matrix b in LDS and invert it.  info[b] =
+// k0 + j + 1 for the first pivot that is not positive (NaN included); a matrix whose info is
+// already set is left alone (its panel is then scratch).  Uinv + b * CHOL_NB^2 receives U11^-1
+// (upper triangular, ld CHOL_NB).
+__global__ __launch_bounds__(CHOL_WG) void k_chol_diag(double *A, int N, i64 sA, int k0, int kb, int *info,
+                                                       double *Uinv) {
+    __shared__ double a[CHOL_NB][CHOL_NB + 1];      // a[r][c]: U11 in the upper triangle
+    __shared__ double x[CHOL_NB][CHOL_NB + 1];      // x[r][c]: U11^-1
     __shared__ int bad;
     const int b = blockIdx.x;
     if (info[b] != 0) return;                       // uniform over the workgroup
@@ -56,10 +64,31 @@ __global__ __launch_bounds__(CHOL_WG) void k_chol_diag(double *A, int N, i64 sA,
         if (threadIdx.x == 0) info[b] = bad;
         return;
     }
+    // U11^-1 by rows from the bottom: x[i][c] = (delta_ic - sum_{i<k<=c} a[i][k] x[k][c]) / a[i][i], c >= i
+    for (int t = threadIdx.x; t < kb * kb; t += CHOL_WG) x[t % kb][t / kb] = 0.0;
+    __syncthreads();
+    for (int i = kb - 1; i >= 0; i--) {
+        for (int c = i + threadIdx.x; c < kb; c += CHOL_WG) {
+            double s = c == i ? 1.0 : 0.0;
+            for (int k = i + 1; k <= c; k++) s -= a[i][k] * x[k][c];
+            x[i][c] = s / a[i][i];
+        }
+        __syncthreads();
+    }
+    double *Ub = Uinv + (i64)b * CHOL_NB * CHOL_NB;
     for (int t = threadIdx.x; t < kb * kb; t += CHOL_WG) {
         const int r = t % kb, c = t / kb;
         if (r <= c) Ab[r + (i64)N * c] = a[r][c];
+        Ub[r + CHOL_NB * c] = x[r][c];
     }
+}
+
+// block row of U: A[k0 + r, k0 + kb + c] = P[r + CHOL_NB * c], r < kb, c < m
+__global__ void k_chol_put(double *A, int N, i64 sA, int k0, int kb, int m, const double *P, i64 sP) {
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (i64)kb * m) return;
+    const int r = (int)(t % kb), c = (int)(t / kb), b = blockIdx.z;
+    A[b * sA + (k0 + r) + (i64)N * (k0 + kb + c)] = P[b * sP + r + (i64)CHOL_NB * c];
 }
 
 __global__ void k_chol_zero_info(int *info, int nb) {
@@ -69,23 +98,30 @@ __global__ void k_chol_zero_info(int *info, int nb) {
 
 #define RBH(call) do { if ((call) != rocblas_status_success) { set_error("rocBLAS failed: %s", #call); return -1; } } while (0)
 
-int chol_upper(rocblas_handle h, hipStream_t st, int N, double *A, i64 sA, int nb, int *info) {
+i64 chol_workspace(int N, int nb) { return (i64)nb * CHOL_NB * ((i64)CHOL_NB + N); }
+
+int chol_upper(rocblas_handle h, hipStream_t st, int N, double *A, i64 sA, int nb, int *info, double *work) {
     if (N < 1 || nb < 1 || nb > 65535) { set_error("chol_upper: unsupported N=%d nb=%d", N, nb); return -2; }
     RBH(rocblas_set_stream(h, st));
+    double *Uinv = work, *P = work + (i64)nb * CHOL_NB * CHOL_NB;
+    const i64 sU = (i64)CHOL_NB * CHOL_NB, sP = (i64)CHOL_NB * N;
     hipLaunchKernelGGL(k_chol_zero_info, dim3((nb + 255) / 256), dim3(256), 0, st, info, nb);
     NV2_LAUNCH_CHECK();
-    const double one = 1.0, mone = -1.0;
+    const double one = 1.0, mone = -1.0, zero = 0.0;
     for (int k0 = 0; k0 < N; k0 += CHOL_NB) {
         const int kb = N - k0 < CHOL_NB ? N - k0 : CHOL_NB;
-        hipLaunchKernelGGL(k_chol_diag, dim3(nb), dim3(CHOL_WG), 0, st, A, N, sA, k0, kb, info);
+        hipLaunchKernelGGL(k_chol_diag, dim3(nb), dim3(CHOL_WG), 0, st, A, N, sA, k0, kb, info, Uinv);
         NV2_LAUNCH_CHECK();
         const int m = N - k0 - kb;
         if (m <= 0) break;
-        double *U11 = A + k0 + (i64)N * k0, *A12 = A + k0 + (i64)N * (k0 + kb), *A22 = A + (k0 + kb) + (i64)N * (k0 + kb);
-        RBH(rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
-                                          rocblas_diagonal_non_unit, kb, m, &one, U11, N, sA, A12, N, sA, nb));
-        RBH(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, m, m, kb, &mone, A12,
-                                          N, sA, A12, N, sA, &one, A22, N, sA, nb));
+        double *A12 = A + k0 + (i64)N * (k0 + kb), *A22 = A + (k0 + kb) + (i64)N * (k0 + kb);
+        RBH(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, kb, m, kb, &one, Uinv,
+                                          CHOL_NB, sU, A12, N, sA, &zero, P, CHOL_NB, sP, nb));
+        RBH(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, m, m, kb, &mone, P,
+                                          CHOL_NB, sP, P, CHOL_NB, sP, &one, A22, N, sA, nb));
+        hipLaunchKernelGGL(k_chol_put, dim3((unsigned)(((i64)kb * m + 255) / 256), 1, nb), dim3(256), 0, st, A, N, sA,
+                           k0, kb, m, P, sP);
+        NV2_LAUNCH_CHECK();
     }
     return 0;
 }

@@ -199,7 +199,7 @@ int nv2_ctx_run(nv2_ctx *c, int32_t prec, double thr_lindep, double *out) {
     NV2_HIP(hipEventRecord(c->ev[1], st));
     // the (+-1)' metric of the first state depends on the 3-RDM only: its solve overlaps stage 1
     if (stage2_pm1p_begin(c->s2, 0, thr_lindep)) return -1;
-    // phase 1: Dyall h, active ERIs, Koopmans matrices of rank <= 3 and B/C/D
+    // phase 1 (first part): Dyall h, active ERIs and the active-space f that bamat needs
     if (launch_aj_atwo(n, c->nc, (int)c->norb, c->L, c->find("cholesky"), c->find("oneint"), c->find("aj"),
                        c->find("atwo"), st)) return -1;
     Stage1Scratch w; w.AJ = c->find("AJ"); w.Fhalf = c->find("Fhalf"); w.Fone = c->find("Fone");
@@ -208,8 +208,7 @@ int nv2_ctx_run(nv2_ctx *c, int32_t prec, double thr_lindep, double *out) {
     so.ipko2 = c->find("2-IPKO"); so.eako2 = c->find("2-EAKO"); so.gk0pa = c->find("gk0pa");
     so.gk0pd = c->find("gk0pd"); so.gk0pf = c->find("gk0pf"); so.bmat = c->find("bmatIP"); so.btmat = c->find("bmatEA");
     so.cmat = c->find("cmatIP"); so.ctmat = c->find("cmatEA"); so.dmat = c->find("dmatIP"); so.dtmat = c->find("dmatEA");
-    if (launch_koop_small(R, c->nc, c->find("aj"), c->find("atwo"), w, so, st)) return -1;
-    NV2_HIP(hipEventRecord(c->ev[2], st));
+    if (launch_koop_prep(n, c->nc, c->find("aj"), c->find("atwo"), w, st)) return -1;
     // phases 2-3: E4 build and the four 4-RDM contractions of bamat, per state
     float t_build = 0.f, t_contr = 0.f;
     for (int s = 0; s < M; s++) {
@@ -226,15 +225,19 @@ int nv2_ctx_run(nv2_ctx *c, int32_t prec, double thr_lindep, double *out) {
         NV2_HIP(hipEventElapsedTime(&y, b, d));
         t_build += x; t_contr += y;
     }
-    float t01 = 0.f, t12 = 0.f;
+    float t01 = 0.f;
     NV2_HIP(hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
-    NV2_HIP(hipEventElapsedTime(&t12, c->ev[1], c->ev[2]));
     NV2_HIP(hipEventRecord(c->ev[4], st));
     // phase 4: lower-order part of bamat and final A-matrices
     for (int s = 0; s < M; s++)
         if (launch_bamat_lo(c->rb, R, s, c->find("atwo"), c->find("Fhalf"), c->find("bamat_work"), c->find("C1"),
                             c->find("C2"), c->find("C3"), c->find("C4"), c->find("amatIP"), c->find("amatEA"), st))
             return -1;
+    NV2_HIP(hipEventRecord(c->ev[2], st));
+    // the (+-1)' chain of the first state needs amat only: it runs on while the rest of stage 1 is built
+    if (stage2_pm1p_kready(c->s2, 0)) return -1;
+    // phase 1 (second part): Koopmans matrices of rank <= 3 and B/C/D
+    if (launch_koop_rest(R, c->find("atwo"), w, so, st)) return -1;
     NV2_HIP(hipEventRecord(c->ev[5], st));
     // phase 5: stage 2 for every state, with the QD couplings when transition RDMs are present
     const bool qd = c->has_trdm && M > 1;
@@ -266,10 +269,11 @@ int nv2_ctx_run(nv2_ctx *c, int32_t prec, double thr_lindep, double *out) {
     }
     NV2_HIP(hipEventRecord(c->ev[6], st));
     NV2_HIP(hipEventSynchronize(c->ev[6]));
-    float t45 = 0.f, t56 = 0.f;
-    NV2_HIP(hipEventElapsedTime(&t45, c->ev[4], c->ev[5]));
+    float t42 = 0.f, t25 = 0.f, t56 = 0.f;
+    NV2_HIP(hipEventElapsedTime(&t42, c->ev[4], c->ev[2]));
+    NV2_HIP(hipEventElapsedTime(&t25, c->ev[2], c->ev[5]));
     NV2_HIP(hipEventElapsedTime(&t56, c->ev[5], c->ev[6]));
-    c->ms[0] = t01; c->ms[1] = t12; c->ms[2] = t_build; c->ms[3] = t_contr; c->ms[4] = t45; c->ms[5] = t56;
+    c->ms[0] = t01; c->ms[1] = t25; c->ms[2] = t_build; c->ms[3] = t_contr; c->ms[4] = t42; c->ms[5] = t56;
     return 0;
 }
 

@@ -27,6 +27,10 @@ int launch_bro21(const Rdm &R, int nele, double *taa, double *dal, hipStream_t s
 int launch_aj_atwo(int n, int ncore, int norb, int L, const double *B, const double *oneint, double *aj,
                    double *atwo, hipStream_t st);
 int launch_make_f(int n, int ncore, const double *aj, const double *atwo, double alpha, double *F, hipStream_t st);
+// launch_koop_small in two parts: the active-space Dyall f (AJ, Fhalf, Fone), which bamat needs,
+// and the Koopmans matrices of rank <= 3 and B/C/D, which the (+-1)' chain does not
+int launch_koop_prep(int n, int ncore, const double *aj, const double *atwo, Stage1Scratch &w, hipStream_t st);
+int launch_koop_rest(const Rdm &R, const double *atwo, Stage1Scratch &w, Stage1Out &o, hipStream_t st);
 int launch_koop_small(const Rdm &R, int ncore, const double *aj, const double *atwo, Stage1Scratch &w,
                       Stage1Out &o, hipStream_t st);
 int launch_build_e4(const Rdm &R, int s, void *E4, int prec, hipStream_t st);

@@ -459,19 +459,22 @@ int launch_make_f(int n, int ncore, const double *aj, const double *atwo, double
     return 0;
 }
 
-int launch_koop_small(const Rdm &R, int ncore, const double *aj, const double *atwo, Stage1Scratch &w,
-                      Stage1Out &o, hipStream_t st) {
+int launch_koop_prep(int n, int ncore, const double *aj, const double *atwo, Stage1Scratch &w, hipStream_t st) {
+    hipLaunchKernelGGL(k_aj_active, dim3(n), dim3(64), 0, st, n, ncore, aj, w.AJ);
+    NV2_LAUNCH_CHECK();
+    if (launch_make_f(n, ncore, aj, atwo, 0.5, w.Fhalf, st)) return -1;
+    if (launch_make_f(n, ncore, aj, atwo, 1.0, w.Fone, st)) return -1;
+    return 0;
+}
+
+int launch_koop_rest(const Rdm &R, const double *atwo, Stage1Scratch &w, Stage1Out &o, hipStream_t st) {
     const i64 n = R.n, M = R.M;
     const i64 t2 = M * n * n, t4 = M * n * n * n * n;
-    hipLaunchKernelGGL(k_aj_active, dim3(n), dim3(64), 0, st, (int)n, ncore, aj, w.AJ);
-    NV2_LAUNCH_CHECK();
     // koopE with f = aj
     hipLaunchKernelGGL(k_koopE, dim3(nblk(t2 * 64, 256)), dim3(256), 0, st, R, atwo, w.AJ, o.ipko1, o.eako1);
     NV2_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_hrdm, dim3(nblk(t4, 256)), dim3(256), 0, st, R, o.hrdm);
     NV2_LAUNCH_CHECK();
-    if (launch_make_f((int)n, ncore, aj, atwo, 0.5, w.Fhalf, st)) return -1;
-    if (launch_make_f((int)n, ncore, aj, atwo, 1.0, w.Fone, st)) return -1;
     hipLaunchKernelGGL(k_koop2E, dim3(nblk(t4 * 64, 256)), dim3(256), 0, st, R, atwo, w.Fhalf, o.ipko2, o.eako2);
     NV2_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_koop0p, dim3(nblk(t4 * 64, 256)), dim3(256), 0, st, R, atwo, w.Fhalf, o.gk0pa, o.gk0pd);
@@ -487,6 +490,12 @@ int launch_koop_small(const Rdm &R, int ncore, const double *aj, const double *a
         NV2_LAUNCH_CHECK();
     }
     return 0;
+}
+
+int launch_koop_small(const Rdm &R, int ncore, const double *aj, const double *atwo, Stage1Scratch &w,
+                      Stage1Out &o, hipStream_t st) {
+    if (launch_koop_prep(R.n, ncore, aj, atwo, w, st)) return -1;
+    return launch_koop_rest(R, atwo, w, o, st);
 }
 
 template <typename T>

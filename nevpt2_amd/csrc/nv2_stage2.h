@@ -34,5 +34,8 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
 // starts the (+-1)' metric solve of state s on worker 1 ahead of stage2_state (it needs the 3-RDM
 // only); stage2_state(s) then waits for it instead of starting it
 int stage2_pm1p_begin(Stage2 *p, int s, double thr);
+// Fills the chain's Koopmans matrix from amat of state s (bamat done) and releases the chain;
+// stage2_state then skips that fill.  No-op without a chain started for state s.
+int stage2_pm1p_kready(Stage2 *p, int s);
 
 }  // namespace nv2

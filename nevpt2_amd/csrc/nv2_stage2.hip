@@ -1306,6 +1306,7 @@ struct PcJob {
     double *trdw = nullptr;                    // K' reduction scratch; T and tau of problem b from trd_outputs(nmax)
     double *trdm = nullptr, *trde = nullptr;   // eig_trd: reduction scratch, off-diagonal copy for dstedc
     double *trdq = nullptr;                    // scratch of applying Q
+    double *cholw = nullptr;                   // Cholesky scratch (trd jobs); T and caux are adjacent there
     double *S = nullptr, *K = nullptr;         // nb consecutive N x N matrices; S is overwritten
     double *w1 = nullptr, *hw1 = nullptr, *coef = nullptr, *w2 = nullptr;
     double *caux = nullptr, *Ks = nullptr, *T = nullptr, *Kp = nullptr;   // nb x N x N scratch each
@@ -1321,9 +1322,14 @@ struct PcJob {
 static int pc_job_buffers(Stage2 *p, PcJob &j, size_t hoff) {
     const i64 N = j.N, NN = N * N * j.nb;
     GET(w1, (j.tag + "_w1").c_str(), N * j.nb); GET(w2, (j.tag + "_w2").c_str(), N * j.nb);
-    GET(coef, (j.tag + "_coef").c_str(), NN); GET(caux, (j.tag + "_caux").c_str(), NN);
-    GET(Ks, (j.tag + "_Ks").c_str(), NN); GET(T, (j.tag + "_T").c_str(), NN); GET(Kp, (j.tag + "_Kp").c_str(), NN);
-    j.w1 = w1; j.w2 = w2; j.coef = coef; j.caux = caux; j.Ks = Ks; j.T = T; j.Kp = Kp;
+    GET(coef, (j.tag + "_coef").c_str(), NN);
+    GET(Ks, (j.tag + "_Ks").c_str(), NN); GET(Kp, (j.tag + "_Kp").c_str(), NN);
+    GET(Tc, (j.tag + "_Tcaux").c_str(), 2 * NN);   // T then caux: one batch for the two factorisations
+    j.w1 = w1; j.w2 = w2; j.coef = coef; j.caux = Tc + NN; j.Ks = Ks; j.T = Tc; j.Kp = Kp;
+    if (j.trd) {
+        GET(cw, (j.tag + "_cholw").c_str(), chol_workspace(j.N, 2 * j.nb));
+        j.cholw = cw;
+    }
     j.hw1 = p->hpin + hoff;
     if (j.N >= 3) {                  // scratch of eig_trd (every job) and of the K' reduction (trd jobs)
         GET(tm, (j.tag + "_trdm").c_str(), trd_workspace(j.N, j.nb));
@@ -1442,7 +1448,7 @@ __global__ void k_shift_diag(double *A, int N, i64 sA, int nbm, double d) {
     A[b * sA + i + (i64)N * i] += d;
 }
 static int pc_metric_launch(Stage2 *p, PcJob &j, double thr) {
-    if (j.trd && j.N >= 3 && chol_metric()) {     // info words: 4*slot + b for S - thr*I, + 2 + b for S
+    if (j.trd && j.N >= 3 && chol_metric()) {     // info words: 4*slot + b for S - thr*I, + nb + b for S
         j.chol = true;
         hipStream_t st = p->ws[j.w];
         rocblas_handle h = p->wh[j.w];
@@ -1453,12 +1459,10 @@ static int pc_metric_launch(Stage2 *p, PcJob &j, double thr) {
         NV2_LAUNCH_CHECK();
         EigTiming tm(j.N);
         tm.mark(0, st);
-        TRYS(chol_upper(h, st, j.N, j.T, NN, j.nb, p->dinfo_w + 4 * j.slot));
-        tm.mark(1, st);
-        TRYS(chol_upper(h, st, j.N, j.caux, NN, j.nb, p->dinfo_w + 4 * j.slot + 2));
-        tm.mark(2, st);
-        tm.mark(3, st);
-        tm.report(j.N, j.nb);            // "reduce": S - thr*I factored, "dstedc": S factored
+        // T_b (S - thr*I) and caux_b (S) factored as one batch: info words b and nb + b
+        TRYS(chol_upper(h, st, j.N, j.T, NN, 2 * j.nb, p->dinfo_w + 4 * j.slot, j.cholw));
+        for (int k = 1; k < 4; k++) tm.mark(k, st);
+        tm.report(j.N, j.nb);            // the factorisations: the first figure
         NV2_HIP(hipMemcpyAsync(p->hinfo + 4 * j.slot, p->dinfo_w + 4 * j.slot, sizeof(int) * 4, hipMemcpyDeviceToHost,
                                st));
         return 0;
@@ -1474,8 +1478,8 @@ static int pc_kprime_launch(Stage2 *p, PcJob &j, double thr) {
     if (j.chol) {                        // the factorisations of pc_metric_launch have finished
         bool ok = true;
         for (int b = 0; b < j.nb; b++) {
-            if (p->hinfo[4 * j.slot + b] != 0 || p->hinfo[4 * j.slot + 2 + b] != 0) ok = false;
-            p->hinfo[4 * j.slot + b] = p->hinfo[4 * j.slot + 2 + b] = 0;
+            if (p->hinfo[4 * j.slot + b] != 0 || p->hinfo[4 * j.slot + j.nb + b] != 0) ok = false;
+            p->hinfo[4 * j.slot + b] = p->hinfo[4 * j.slot + j.nb + b] = 0;
         }
         hipStream_t st = p->ws[j.w];
         rocblas_handle h = p->wh[j.w];
@@ -1569,6 +1573,7 @@ struct Pm1pEarly {
     std::mutex mu;
     std::condition_variable cv;
     bool k_ready = false, abort = false;
+    bool k_filled = false;         // K filled and signalled ahead of stage2_state (stage2_pm1p_kready)
     int rc = 0;
     char err[512] = "";
     PcJob job;
@@ -1643,6 +1648,7 @@ int stage2_pm1p_begin(Stage2 *p, int s, double thr) {
     E->state = s;
     E->thr = thr;
     E->k_ready = false;
+    E->k_filled = false;
     E->abort = false;
     E->rc = 0;
     E->err[0] = 0;
@@ -1651,6 +1657,31 @@ int stage2_pm1p_begin(Stage2 *p, int s, double thr) {
         E->rc = pm1p_worker(p, E);
         if (E->rc) nv2_last_error(E->err, sizeof(E->err));
     });
+    return 0;
+}
+
+// The chain's Koopmans matrix needs amat only: it is filled (and the chain released) as soon as bamat
+// has finished, while the other Koopmans matrices of stage 1 are still being built.
+int stage2_pm1p_kready(Stage2 *p, int s) {
+    Pm1pEarly *E = p->early;
+    if (!E || !E->active || E->state != s || E->k_filled) return 0;
+    hipStream_t st = p->st;
+    const int n = p->n;
+    const i64 n2 = (i64)n * n, n6 = n2 * n2 * n2, N3 = n2 * n;
+    for (int plus = 0; plus < 2; plus++) {
+        const std::string t = plus ? "_p" : "_m";
+        GET(am, ("k_amat" + t).c_str(), n6);
+        TRYS(slice(p, plus ? p->in.amatEA : p->in.amatIP, s, n6, am));
+        hipLaunchKernelGGL(k_fill_pm1p_k, dim3(nb(N3 * N3, 256)), dim3(256), 0, st, n, am, E->job.Kb(plus ? 0 : 1));
+        NV2_LAUNCH_CHECK();
+    }
+    NV2_HIP(hipEventRecord(p->evK, st));
+    {
+        std::lock_guard<std::mutex> lk(E->mu);
+        E->k_filled = true;
+        E->k_ready = true;
+    }
+    E->cv.notify_all();
     return 0;
 }
 
@@ -1911,16 +1942,20 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             TRYS(slice(p, plus ? p->in.cmatEA : p->in.cmatIP, s, n4, cm));
             TRYS(slice(p, plus ? p->in.dmatEA : p->in.dmatIP, s, n2, dm));
             GET(mx, ("pm1_M" + t).c_str(), N3 * N3);    // filled by stage2_pm1p_begin
-            hipLaunchKernelGGL(k_fill_pm1p_k, dim3(nb(N3 * N3, 256)), dim3(256), 0, st, n, am, jb.Kb(bb));
-            NV2_LAUNCH_CHECK();
+            if (!p->early->k_filled) {
+                hipLaunchKernelGGL(k_fill_pm1p_k, dim3(nb(N3 * N3, 256)), dim3(256), 0, st, n, am, jb.Kb(bb));
+                NV2_LAUNCH_CHECK();
+            }
             Mx[plus] = mx; amk[plus] = am; bmk[plus] = bm; cmk[plus] = cm; dmk[plus] = dm;
         }
-        NV2_HIP(hipEventRecord(p->evK, st));             // K is there: the chain may form K'
-        {
-            std::lock_guard<std::mutex> lk(p->early->mu);
-            p->early->k_ready = true;
+        if (!p->early->k_filled) {
+            NV2_HIP(hipEventRecord(p->evK, st));         // K is there: the chain may form K'
+            {
+                std::lock_guard<std::mutex> lk(p->early->mu);
+                p->early->k_ready = true;
+            }
+            p->early->cv.notify_all();
         }
-        p->early->cv.notify_all();
     }
     // start the eigen-solves on the worker threads/streams once the fills are done
     {
