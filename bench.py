@@ -4,10 +4,16 @@ One step = one full NEVPT2 evaluation (or, with --concurrency S, a batch of S co
 evaluations through the request loop) on device-resident inputs: partial traces
 of the packed 4-RDM, every Koopmans matrix (including the O(n^9) A-matrices),
 canonical orbitals, Cholesky integral assembly and the eight perturber classes
-in SC and PC form.  Inputs (seeded synthetic CI vector -> RDMs, Cholesky ERIs)
-are generated once on the host and uploaded before the timed region.
+in SC and PC form.  Inputs: a seeded synthetic CI vector (truncated to its 2e5
+largest determinants for CAS(14,14)) from which the RDMs are built on the GPU,
+and seeded Cholesky ERIs; both are resident in HBM before the timed region.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2] [--prec fp64|fp32] [--concurrency S]
+The default workload is CAS(14,14), nc=30, nv=400.  Both precision forms are
+timed: the plain FP64 path and the mixed form (FP32 4-RDM contractions).
+`value` is the throughput of the fastest form whose per-class energies stay
+within 1 uEh of FP64 on this workload; the other form is reported beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3] [--prec both|fp64|fp32] [--concurrency S]
 """
 from __future__ import annotations
 
@@ -28,10 +34,18 @@ CONFIGS = {   # name: (nact, ncore, nvirt, roots, determinants kept, description
     "c3": (14, 30, 400, 1, 200000, "CAS(14,14) nc=30 nv=400 single state, CI truncated to its 2e5 largest determinants"),
     "c4": (12, 25, 300, 6, None, "CAS(12,12) nc=25 nv=300, 6 roots (QD-NEVPT2)"),
 }
+# CPU oracle sample per workload: bamat (a,c) pairs run (None: one per host thread, so that the
+# threaded pair loop runs one full wave), fraction of each class's outer external loop
+CPU_SAMPLE = {"c1": (5, 0.05), "c2": (5, 0.05), "c3": (None, 0.01), "c4": (None, 0.01)}
 METRIC = "SC+PC-NEVPT2 evals/sec CAS(12-14) mixed vs FP64 at |ΔE|≤1µEh; p50/p99 lat"
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12}
 HBM_PEAK = 8.0e12
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "e4_contractions_cas10_fp64_pmc.md")
+TOL_EH = 1.0e-6
+PROFILES = os.path.join(ROOT, "profiles")
+TRAFFIC = {   # (kernel, config, precision) -> committed rocprofv3 counter summary holding its HBM bytes
+    ("contractions", "c2", "fp64"): ("e4_contractions_cas10_fp64_pmc.md", "traffic_bytes_per_eval:"),
+    ("trd_col", "c3", "fp64"): ("trd_col_cas14_fp64_pmc.md", "traffic_bytes_per_eval:"),
+}
 
 
 def contraction_flops(n, M):
@@ -45,23 +59,34 @@ def contraction_bytes(n, M, esize):
     return M * (2.0 * n ** 8 * esize + 4.0 * 8 * n ** 6)
 
 
-def measured_traffic(config, prec):
-    """HBM bytes of the two passes per evaluation from the committed rocprofv3 counter runs
-    (FETCH_SIZE and WRITE_SIZE, calibrated on a copy kernel), for the workload they were taken on."""
-    if config != "c2" or prec != "fp64" or not os.path.exists(TRAFFIC_PROFILE):
+def trd_bytes(N, nb):
+    """Algorithmic bytes of one tridiagonal reduction (k_trd_col launches k = 0..N-2): launch k reads
+    and writes the trailing block of order m = N-k-1 of each of the nb matrices once."""
+    m = np.arange(1, N, dtype=np.float64)
+    return 16.0 * nb * float((m * m).sum())
+
+
+def measured_traffic(kernel, config, prec):
+    """HBM bytes per evaluation of a kernel family from the committed rocprofv3 counter runs
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), for the workload they were taken on."""
+    ent = TRAFFIC.get((kernel, config, prec))
+    if not ent:
         return None
-    for line in open(TRAFFIC_PROFILE):
-        if line.startswith("traffic_bytes_per_eval:"):
+    path = os.path.join(PROFILES, ent[0])
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(ent[1]):
             return float(line.split(":")[1])
     return None
 
 
-def cpu_baseline(P, frac_pairs=0.05):
+def cpu_baseline(P, pairs=5, frac=0.05):
     """Oracle timed on the host cores on a bounded sample of the same workload.
 
     Stage 1: every Koopmans build in full except bamat, whose (a,c) outer loop is
-    run on a fraction of its pairs and scaled; stage 2: every class with its
-    outer external loop sampled and scaled (oracle.stage2 ``frac``).
+    run on `pairs` of its n^2 pairs and scaled; stage 2: every class with its
+    outer external loop sampled at `frac` and scaled (oracle.stage2 ``frac``).
     """
     from oracle import stage1 as S1, stage2 as S2
     n, nc, M = P.nact, P.ncore, P.metat
@@ -77,17 +102,17 @@ def cpu_baseline(P, frac_pairs=0.05):
     koop["dmatIP"], koop["dmatEA"] = S1.bdmat(n, nc, M, atwo, P.taa, P.dal, aj)
     t_small = time.perf_counter() - t0
     npairs = n * n
-    k = max(1, int(round(frac_pairs * npairs)))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count()))
+    k = max(1, min(pairs if pairs else threads, npairs))
     t0 = time.perf_counter()
     am, at = S1.bamat(n, nc, M, atwo, P.ro4, P.daaa, P.taa, P.dal, aj, pair_limit=k)
     t_bamat = (time.perf_counter() - t0) * npairs / k
     koop["amatIP"], koop["amatEA"] = am, at        # partial; stage-2 timing does not depend on values
-    t0 = time.perf_counter()
-    _, _, t_s2 = S2.state_energies_timed(P, koop, 0, frac=0.05)
+    _, _, t_s2 = S2.state_energies_timed(P, koop, 0, frac=frac)
     t_total = t_small + t_bamat + t_s2
-    sample = (f"stage 1: full except bamat on {k}/{npairs} (a,c) pairs scaled x{npairs / k:.0f}; "
-              f"stage 2: each class's outer external loop on a 5% sample, scaled")
-    return {"value": 1.0 / t_total, "unit": "evals/s", "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
+    sample = (f"stage 1: full except bamat on {k}/{npairs} (a,c) pairs scaled x{npairs / k:.0f}; stage 2: each "
+              f"class's outer external loop on a {frac:.0%} sample, scaled (the metric and K' eigen-solves in full)")
+    return {"value": 1.0 / t_total, "unit": "evals/s", "cores": threads,
             "kind": "port", "sample": sample,
             "seconds_per_eval": {"stage1_small": t_small, "bamat": t_bamat, "stage2": t_s2, "total": t_total}}
 
@@ -102,13 +127,54 @@ def host_rdms(P, ev):
     P.dal = ev.get("1-RDM", (M, n, n))
 
 
+def time_form(evs, prec, steps, warmup, S):
+    """Unloaded latency, per-phase device times and the timed steps of one precision form."""
+    from nevpt2_amd.evaluator import PHASES
+    from nevpt2_amd.serve import Server
+    for ev in evs:
+        for _ in range(warmup):
+            ev.run(prec)
+    lat1, phases, kred = [], np.zeros(len(PHASES)), []
+    for _ in range(steps):
+        t = time.perf_counter()
+        res = evs[0].run(prec)
+        lat1.append((time.perf_counter() - t) * 1e3)
+        phases += np.array(list(evs[0].timings().values()))
+        kred.append(evs[0].pc_timings()[:, 0].sum())
+    phases /= steps
+    latS = []
+    if S == 1:         # the timed steps: one evaluation each, on the calling thread
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t = time.perf_counter()
+            evs[0].run(prec)
+            latS.append((time.perf_counter() - t) * 1e3)
+        elapsed = time.perf_counter() - t0
+    else:              # each step submits S evaluations to the request loop and waits for all of them
+        with Server(slots=S) as srv:
+            for ev in evs:                                 # one untimed round through the server
+                srv.wait(srv.submit_ctx(ev, prec))
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                ids = [srv.submit_ctx(ev, prec) for ev in evs]
+                for rid in ids:
+                    _, _, lat, _ = srv.wait(rid)
+                    latS.append(lat)
+            elapsed = time.perf_counter() - t0
+    return {"res": res, "elapsed": elapsed, "value": S * steps / elapsed, "ms_per_step": elapsed / steps * 1e3,
+            "phases": dict(zip(PHASES, [float(x) for x in phases])), "kred_ms": float(np.mean(kred)),
+            "latency_ms": {"p50": float(np.percentile(latS, 50)), "p99": float(np.percentile(latS, 99)),
+                           "unloaded_p50": float(np.percentile(lat1, 50)),
+                           "unloaded_p99": float(np.percentile(lat1, 99))}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
-    ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
+    ap.add_argument("--prec", default="both", choices=["both", "fp64", "fp32"])
     ap.add_argument("--concurrency", type=int, default=1,
                     help="evaluations in flight per step through the request loop (one resident context each); "
                          "1 runs them one after another on the calling thread")
@@ -119,81 +185,83 @@ def main():
               f"orbital index, see DESIGN.md", file=sys.stderr)
         sys.exit(2)
     from nevpt2_amd import synth
-    from nevpt2_amd.evaluator import Evaluator, PHASES
-    from nevpt2_amd.serve import Server
+    from nevpt2_amd.evaluator import Evaluator
     n, nc, nv, M, keep, desc = CONFIGS[args.config]
     S = max(1, args.concurrency)
     print(f"[bench] generating {desc} ...", file=sys.stderr, flush=True)
     P = synth.Problem(n, nc, nv, seed=1, nroots=M, keep=keep, host_rdms=False)
     t = time.perf_counter()
     evs = [Evaluator.for_problem(P, rdms="gpu") for _ in range(S)]   # each request: its own context and inputs in HBM
-    t_rdm = (time.perf_counter() - t) / S
-    print(f"[bench] RDMs built on the GPU from the CI vector(s): {t_rdm:.2f} s per context", file=sys.stderr, flush=True)
-    prec = 0 if args.prec == "fp64" else 1
-    for ev in evs:
-        for _ in range(args.warmup):
-            ev.run(prec)
-    # unloaded latency and per-phase split (HIP events on the context's stream), one evaluation at a time
-    lat1, phases = [], np.zeros(len(PHASES))
-    for _ in range(args.steps):
-        t = time.perf_counter()
-        res = evs[0].run(prec)
-        lat1.append((time.perf_counter() - t) * 1e3)
-        phases += np.array(list(evs[0].timings().values()))
-    phases /= args.steps
-    latS = []
-    if S == 1:         # the timed steps: one evaluation each, on the calling thread
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            t = time.perf_counter()
-            evs[0].run(prec)
-            latS.append((time.perf_counter() - t) * 1e3)
-        elapsed = time.perf_counter() - t0
-    else:              # each step submits S evaluations to the request loop and waits for all of them
-        with Server(slots=S) as srv:
-            for ev in evs:                                 # one untimed round through the server
-                srv.wait(srv.submit_ctx(ev, prec))
-            t0 = time.perf_counter()
-            for _ in range(args.steps):
-                ids = [srv.submit_ctx(ev, prec) for ev in evs]
-                for rid in ids:
-                    tab, _, lat, _ = srv.wait(rid)
-                    latS.append(lat)
-            elapsed = time.perf_counter() - t0
-    t_contr = phases[PHASES.index("E4 contractions")] * 1e-3
-    fl = contraction_flops(n, M)
-    by = contraction_bytes(n, M, 8 if args.prec == "fp64" else 4)
-    achieved = by / t_contr / 1e9 if t_contr > 0 else 0.0
+    t_setup = (time.perf_counter() - t) / S
+    print(f"[bench] RDMs built on the GPU from the CI vector(s), integrals uploaded: {t_setup:.2f} s per context",
+          file=sys.stderr, flush=True)
+    forms = {}
+    for name, prec in (("fp64", 0), ("fp32", 1)):
+        if args.prec in ("both", name):
+            print(f"[bench] timing the {name} form ...", file=sys.stderr, flush=True)
+            forms[name] = time_form(evs, prec, args.steps, args.warmup, S)
+    if "fp32" in forms:            # the mixed form against the FP64 path on the same resident inputs
+        ref = forms["fp64"]["res"] if "fp64" in forms else evs[0].run(0)
+        got = forms["fp32"]["res"]
+        d = np.abs(got - ref)
+        forms["fp32"]["vs_fp64"] = {
+            "max_abs_dE_sc": float(d[:, :, 1].max()), "max_abs_dE_pc": float(d[:, :, 3].max()),
+            "abs_dE_total_sc": float(abs(got[:, :, 1].sum() - ref[:, :, 1].sum())),
+            "abs_dE_total_pc": float(abs(got[:, :, 3].sum() - ref[:, :, 3].sum())), "unit": "Eh"}
+        forms["fp32"]["within_1uEh"] = bool(max(d[:, :, 1].max(), d[:, :, 3].max()) <= TOL_EH)
+    if "fp64" in forms:
+        forms["fp64"]["within_1uEh"] = True
+    ok = [k for k in forms if forms[k]["within_1uEh"]]
+    head = max(ok, key=lambda k: forms[k]["value"]) if ok else list(forms)[0]
+    F = forms[head]
+    esize = 8 if head == "fp64" else 4
+    # rooflines: the K' tridiagonal reduction of the (+-1)' chain and the bamat contraction passes
+    rl = []
+    Nk = n ** 3
+    if F["kred_ms"] > 0:
+        by = trd_bytes(Nk, 2) * M
+        ach = by / (F["kred_ms"] * 1e-3) / 1e9
+        rl.append({"kernel": "K' tridiagonal reduction (k_trd_col, one launch per column)", "bound": "hbm",
+                   "achieved": ach, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / (HBM_PEAK / 1e9),
+                   "traffic": measured_traffic("trd_col", args.config, head),
+                   "device_ms_per_eval": F["kred_ms"], "launches_per_eval": (Nk - 1) * M,
+                   "algorithmic": f"trailing blocks read and written once per launch, two problems of order {Nk}: "
+                                  f"{by:.3e} B per evaluation"})
+    t_contr = F["phases"]["E4 contractions"] * 1e-3
+    fl, byc = contraction_flops(n, M), contraction_bytes(n, M, esize)
+    achc = byc / t_contr / 1e9 if t_contr > 0 else 0.0
+    rl.append({"kernel": "bamat 4-RDM contractions (k_e4_pass_a + k_e4_pass_b)", "bound": "hbm", "achieved": achc,
+               "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achc / (HBM_PEAK / 1e9),
+               "traffic": measured_traffic("contractions", args.config, head), "device_ms_per_eval": t_contr * 1e3,
+               "launches_per_eval": 2 * M,
+               "algorithmic": f"E4 read once per pass + outputs = {byc:.3e} B per evaluation",
+               "flops": {"algorithmic": fl, "achieved_tflops": fl / t_contr / 1e12 if t_contr > 0 else 0.0,
+                         "peak_tflops": PEAK[head] / 1e12}})
+    dom = max(rl, key=lambda r: r["device_ms_per_eval"])
+    res = F["res"]
     out = {
-        "metric": METRIC, "value": S * args.steps / elapsed, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.prec == "fp64" else "f32",
-        "data": "synthetic (seeded random CI vector -> exact RDMs; seeded Cholesky ERIs); each of the concurrent "
-                "requests runs on its own context with its own device-resident copy of the inputs",
+        "metric": METRIC, "value": F["value"], "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": F["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if head == "fp64" else "f32",
+        "data": "synthetic (seeded random CI vector -> RDMs built on the GPU; seeded Cholesky ERIs); each of the "
+                "concurrent requests runs on its own context with its own device-resident copy of the inputs",
         "config": {"workload": desc, "nact": n, "ncore": nc, "nvirt": nv, "roots": M, "nchovec": P.B.shape[0],
-                   "precision": args.prec, "concurrency": S},
-        "latency_ms": {"p50": float(np.percentile(latS, 50)), "p99": float(np.percentile(latS, 99)),
-                       "unloaded_p50": float(np.percentile(lat1, 50)), "unloaded_p99": float(np.percentile(lat1, 99))},
-        "phases_ms": dict(zip(PHASES, [float(x) for x in phases])),
-        "roofline": {"kernel": "bamat 4-RDM contractions (k_e4_pass_a + k_e4_pass_b)", "bound": "hbm",
-                     "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / (HBM_PEAK / 1e9),
-                     "traffic": measured_traffic(args.config, args.prec),
-                     "algorithmic": f"E4 read once per pass + outputs = {by:.3e} B per evaluation (two launches)",
-                     "flops": {"algorithmic": fl, "achieved_tflops": fl / t_contr / 1e12 if t_contr > 0 else 0.0,
-                               "peak_tflops": PEAK[args.prec] / 1e12}},
+                   "determinants_kept": keep, "precision": head, "concurrency": S},
+        "latency_ms": F["latency_ms"],
+        "phases_ms": F["phases"],
+        "roofline": dom,
+        "rooflines": rl,
+        "forms": {k: {"value": v["value"], "ms_per_step": v["ms_per_step"], "latency_ms": v["latency_ms"],
+                      "within_1uEh_of_fp64": v["within_1uEh"], **({"vs_fp64": v["vs_fp64"]} if "vs_fp64" in v else {})}
+                  for k, v in forms.items()},
         "energies": {"e_sc_total": float(res[0, :, 1].sum()), "e_pc_total": float(res[0, :, 3].sum())},
+        "setup": {"rdm_build_and_upload_s_per_context": t_setup},
     }
-    if prec != 0:      # the mixed form against the FP64 path on the same resident inputs
-        ref = evs[0].run(0)
-        d = np.abs(res - ref)
-        out["vs_fp64"] = {"max_abs_dE_sc": float(d[:, :, 1].max()), "max_abs_dE_pc": float(d[:, :, 3].max()),
-                          "abs_dE_total_sc": float(abs(res[:, :, 1].sum() - ref[:, :, 1].sum())),
-                          "abs_dE_total_pc": float(abs(res[:, :, 3].sum() - ref[:, :, 3].sum())), "unit": "Eh"}
-    out["setup"] = {"rdm_build_s_per_context": t_rdm}
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU oracle on a bounded sample ...", file=sys.stderr, flush=True)
         host_rdms(P, evs[0])
-        out["cpu_baseline"] = cpu_baseline(P)
+        pairs, frac = CPU_SAMPLE[args.config]
+        out["cpu_baseline"] = cpu_baseline(P, pairs, frac)
     for ev in evs:
         ev.close()
     print(json.dumps(out), flush=True)

@@ -20,6 +20,8 @@ struct S2Stats {              // diagnostics for threshold-sensitive seeds
     double min_metric_eig[8];    // on the Cholesky path: thr, a lower bound
     int nnew[8];
     int chol[8];                 // 1: the metric was factored (every eigenvalue at or above thr)
+    double kred_ms;              // device time of the K' reduction of the (+-1)' problems (0: none)
+    int kred_N, kred_nb;         // its order and batch
     double eps_min_virt, eps_max_core;
 };
 

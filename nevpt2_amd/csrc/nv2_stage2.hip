@@ -1073,6 +1073,9 @@ struct Stage2 {
     S2Inputs in;
     Pm1pEarly *early = nullptr;    // the (+-1)' chain when started ahead of stage2_state
     hipEvent_t evS = nullptr, evK = nullptr;   // its metric filled / its Koopmans matrix filled
+    hipEvent_t evKr[2] = {nullptr, nullptr};   // around the K' reduction of the (+-1)' problems (timed)
+    bool kr_recorded = false;
+    int kr_N = 0, kr_nb = 0;
     rocblas_handle h;
     hipStream_t st;
     rocblas_handle wh[NW] = {};
@@ -1376,7 +1379,12 @@ struct EigTiming {
 static int pc_trd_launch(Stage2 *p, PcJob &j) {
     EigTiming tm(j.nmax);
     tm.mark(0, p->ws[j.w]);
+    NV2_HIP(hipEventRecord(p->evKr[0], p->ws[j.w]));
     TRYS(trd_reduce(p->ws[j.w], j.nmax, j.Kp, j.NN(), j.nb, j.trdw));
+    NV2_HIP(hipEventRecord(p->evKr[1], p->ws[j.w]));
+    p->kr_recorded = true;
+    p->kr_N = j.nmax;
+    p->kr_nb = j.nb;
     if (tm.on) {
         for (int k = 1; k < 4; k++) tm.mark(k, p->ws[j.w]);
         tm.report(j.nmax, j.nb);       // the reduction of K' only: the first figure
@@ -1712,6 +1720,8 @@ Stage2 *stage2_create(const S2Inputs &in, rocblas_handle h, hipStream_t st) {
         if (hipEventCreateWithFlags(&p->wev[e], hipEventDisableTiming) != hipSuccess) { delete p; return nullptr; }
     if (hipEventCreateWithFlags(&p->evS, hipEventDisableTiming) != hipSuccess) { delete p; return nullptr; }
     if (hipEventCreateWithFlags(&p->evK, hipEventDisableTiming) != hipSuccess) { delete p; return nullptr; }
+    for (auto &e : p->evKr)
+        if (hipEventCreate(&e) != hipSuccess) { delete p; return nullptr; }
     if (hipStreamCreateWithFlags(&p->xs, hipStreamNonBlocking) != hipSuccess) { delete p; return nullptr; }
     if (rocblas_create_handle(&p->xh) != rocblas_status_success) { delete p; return nullptr; }
     rocblas_set_stream(p->xh, p->xs);
@@ -1743,6 +1753,7 @@ void stage2_destroy(Stage2 *p) {
     delete p->early;
     if (p->evS) (void)hipEventDestroy(p->evS);
     if (p->evK) (void)hipEventDestroy(p->evK);
+    for (auto e : p->evKr) if (e) (void)hipEventDestroy(e);
     for (auto &kv : p->bufs) if (kv.second.p) (void)hipFree(kv.second.p);
     (void)hipDeviceSynchronize();
     (void)hipFree(p->dinfo); (void)hipFree(p->part); (void)hipFree(p->acc); (void)hipFree(p->accq);
@@ -2395,6 +2406,15 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
     // ================================================================ join worker 1: the (+-1)' solves
     TRYS(join_worker(1));
     ms_join = ms_since(t_begin);
+    if (p->kr_recorded) {            // device time of the K' reduction of this state's (+-1)' problems
+        float t = 0.f;
+        NV2_HIP(hipEventSynchronize(p->evKr[1]));
+        NV2_HIP(hipEventElapsedTime(&t, p->evKr[0], p->evKr[1]));
+        sd.kred_ms = t;
+        sd.kred_N = p->kr_N;
+        sd.kred_nb = p->kr_nb;
+        p->kr_recorded = false;
+    }
     const int cls_of[NJOB][2] = {{1, 2}, {3, 4}, {3, 4}, {7, 7}, {5, 6}};
     for (int k = 0; k < NJOB; k++)
         for (int b = 0; b < J[k].nb; b++) {

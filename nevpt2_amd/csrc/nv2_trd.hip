@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
+#include <cstdlib>
+
 #include "nv2_common.h"
 #include "nv2_trd.h"
 
@@ -28,7 +30,6 @@ namespace nv2 {
 
 #define TRYT(x) do { int _r = (x); if (_r) return _r; } while (0)
 
-#define TRD_WG 256      // threads of a k_trd_col workgroup (4 wavefronts)
 
 __global__ void k_trd_sym_upper(double *A, int N, i64 sA) {
     const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -55,7 +56,6 @@ __device__ inline double trd_block_sum(double x, double *red) {
 }
 
 #define TRD_NMAX 4096    // largest order (three vectors of it staged in LDS)
-#define TRD_PRE 8        // trailing-column rows per lane loaded in one pass
 
 // Launch k of the reduction, matrix b = blockIdx.z.  Every workgroup first repeats the
 // Householder step of column k (cheap: O(N - k) loads and two block sums), so no workgroup
@@ -68,6 +68,7 @@ __device__ inline double trd_block_sum(double x, double *red) {
 // the stored column and forms p_k[c] = tau_k (A v_k)_c as the column's dot product with v_k.
 // Workgroup 0 publishes v_k, d_k, e_k, tau_k and writes column k-1 of A in the dsytrd 'L'
 // storage (column k is still being read by the other workgroups of this launch).
+template <int TRD_WG, int TRD_PRE>
 __global__ __launch_bounds__(TRD_WG) void k_trd_col(double *A, int N, i64 sA, int k, int q, double *vbuf,
                                                    double *pbuf, i64 sV, double *d, double *e, double *tau) {
     extern __shared__ double sh[];
@@ -215,19 +216,45 @@ TrdOut trd_outputs(int N, int nb, double *work) {
     return o;
 }
 
+typedef void (*TrdColFn)(double *, int, i64, int, int, double *, double *, i64, double *, double *, double *);
+struct TrdCfg {
+    TrdColFn fn;
+    int wg, waves;
+};
+// launch shape of k_trd_col: workgroup size, rows per lane in flight, wavefronts per launch
+// (NV2_TRD_CFG / NV2_TRD_WAVES select among the compiled variants for measurement)
+static const TrdCfg &trd_cfg() {
+    static const TrdCfg cfg = [] {
+        // 16 wavefronts per workgroup and 16 rows per lane in flight: measured 12% faster than 4 and 8
+        // at order 2744 (about 4096 wavefronts per launch; 8192 gained nothing more)
+        const TrdColFn fns[2] = {k_trd_col<1024, 16>, k_trd_col<256, 8>};
+        const int wgs[2] = {1024, 256};
+        const char *e = getenv("NV2_TRD_CFG");
+        int i = e ? atoi(e) : 0;
+        if (i < 0 || i > 1) i = 0;
+        const char *w = getenv("NV2_TRD_WAVES");
+        int waves = w ? atoi(w) : 4096;
+        if (waves < 64) waves = 64;
+        for (auto f : fns) (void)hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
+        return TrdCfg{fns[i], wgs[i], waves};
+    }();
+    return cfg;
+}
+
 static int trd_launches(hipStream_t st, int N, double *A, i64 sA, int nb, double *work) {
     const i64 sV = N;
     double *vbuf = work, *pbuf = vbuf + (i64)nb * 2 * sV;
     const TrdOut o = trd_outputs(N, nb, work);
     hipLaunchKernelGGL(k_trd_zero, dim3(nblk((i64)4 * N * nb, 256)), dim3(256), 0, st, work, (i64)4 * N * nb);
     hipLaunchKernelGGL(k_trd_sym_upper, dim3(nblk((i64)N * N, 256), 1, nb), dim3(256), 0, st, A, N, sA);
+    const TrdCfg &c = trd_cfg();
     for (int k = 0; k <= N - 2; k++) {
         const int m = N - k - 1;                              // trailing columns
-        int q = (int)(((i64)m * nb + 1535) / 1536);           // about 1536 wavefronts over the batch
+        int q = (int)(((i64)m * nb + c.waves - 1) / c.waves);  // about c.waves wavefronts over the batch
         if (q < 1) q = 1;
-        const unsigned gx = nblk(m, (TRD_WG / 64) * q);
-        hipLaunchKernelGGL(k_trd_col, dim3(gx, 1, nb), dim3(TRD_WG), sizeof(double) * 3 * (N - k), st, A, N, sA, k, q,
-                           vbuf, pbuf, sV, o.d, o.e, o.tau);
+        const unsigned gx = nblk(m, (c.wg / 64) * q);
+        hipLaunchKernelGGL(c.fn, dim3(gx, 1, nb), dim3(c.wg), sizeof(double) * 3 * (N - k), st, A, N, sA, k, q, vbuf,
+                           pbuf, sV, o.d, o.e, o.tau);
     }
     hipLaunchKernelGGL(k_trd_last, dim3(nb), dim3(64), 0, st, A, N, sA, o.d, o.e);
     return 0;
@@ -242,9 +269,6 @@ int trd_reduce(hipStream_t st, int N, double *A, i64 sA, int nb, double *work) {
         return 0;
     }
     if (N > TRD_NMAX) { set_error("trd_reduce: order %d above %d", N, TRD_NMAX); return -2; }
-    static const hipError_t attr =    // once per process (thread-safe static initialisation)
-        hipFuncSetAttribute((const void *)k_trd_col, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
-    NV2_HIP(attr);
     TRYT(trd_launches(st, N, A, sA, nb, work));
     NV2_LAUNCH_CHECK();
     return 0;

@@ -69,6 +69,12 @@ class Evaluator:
         L.check(L.lib().nv2_ctx_get(self._h, b"PC-metric", L.ptr(a), a.size), "nv2_ctx_get(PC-metric)")
         return a.reshape(self.M, 8, 3)
 
+    def pc_timings(self):
+        """Per state: (device ms of the (+-1)' K' tridiagonal reduction, its order, its batch), shape (metat, 3)."""
+        a = np.zeros(3 * self.M)
+        L.check(L.lib().nv2_ctx_get(self._h, b"PC-timings", L.ptr(a), a.size), "nv2_ctx_get(PC-timings)")
+        return a.reshape(self.M, 3)
+
     def set_integrals(self, cholesky, oneint):
         B = L.fin(cholesky)
         one = np.ascontiguousarray(oneint, dtype=np.float64)
