@@ -74,9 +74,13 @@ __global__ void k_bamat_combine(int n, int M, int s, const double *E3f, const do
                                 const double *G5, const double *Y2, const double *Z3, const double *C1,
                                 const double *C2, const double *C3, const double *C4, double *amat, double *atmat) {
     const i64 N = n, n2 = N * N, n4 = n2 * n2, n6 = n4 * n2;
-    i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n6) return;
-    const int P = e % n, Q = (e / n) % n, R = (e / n2) % n, a = (e / (n2 * N)) % n, b = (e / n4) % n, c = e / (n4 * N);
+    // threads run R fastest (the leading index of every E3/G operand below, so their loads are
+    // contiguous), then P, Q, a, b, c; a block's outputs (P-fastest rows of n) are completed by its
+    // own lanes, so the strided stores still fill whole lines in L2
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n6) return;
+    const int R = t % n, P = (t / n) % n, Q = (t / n2) % n, a = (t / (n2 * N)) % n, b = (t / n4) % n, c = t / (n4 * N);
+    const i64 e = P + N * (Q + N * (R + N * (a + N * (b + N * (i64)c))));
     // E3 index helper: (x0..x5) -> x0 + n(x1 + n(x2 + ...))
 #define I6(x0, x1, x2, x3, x4, x5) ((x0) + N * ((x1) + N * ((x2) + N * ((x3) + N * ((x4) + N * (i64)(x5))))))
     double am = G1[(R + N * (P + N * (Q + N * b))) + n4 * (c + N * a)];

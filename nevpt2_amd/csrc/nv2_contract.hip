@@ -11,8 +11,8 @@
 // Every index runs over n <= 16 active orbitals, so each contraction is a set of 16x16 MFMA
 // tiles.  Pass A (one workgroup per (cp,ap,bp), four wavefronts) streams the contiguous
 // sub-slab E4[cp,ap,bp,:,:,:,:,:] (n^5 values) once through LDS in chunks (x4,x5) of n^3
-// values and feeds C1, C2 and C3 from each chunk (one wavefront each for C1 and C2, two for
-// C3); pass B (one
+// values and feeds C1, C2 and C3 from each chunk (the K ranges of C1 and C2 split over two
+// wavefronts each, the C3 tiles over all four); pass B (one
 // workgroup per (cp,ap,bp'), four wavefronts) streams E4[cp,ap,:,bp',...] for C4, whose sum
 // runs over x3.  E4 is read twice instead of four times.  FP64 uses v_mfma_f64_16x16x4f64;
 // the FP32 form stores E4 in float and uses v_mfma_f32_16x16x4f32 with FP32 accumulation.
@@ -62,114 +62,133 @@ __global__ void k_make_bk(int n, const double *atwo, double *B1, double *B2, dou
 
 template <typename V> __device__ inline V vzero() { V z = {0, 0, 0, 0}; return z; }
 
-// pass A: workgroup g = (cp*n + ap)*n + bp; wavefront 0 -> C1, 1 -> C2, 2 and 3 -> C3 (row
-// tiles split between them).  Per chunk (x4,x5) the E4 values and the three B slices the chunk
-// needs are staged together in LDS, so every MFMA operand comes from LDS.
+// pass A: workgroup g = (cp*n + ap)*n + bp.  The chunks (x4,x5) are visited x5-outer, so the B1 and
+// B2 slices of one x5 are staged in LDS once and serve the n chunks (x4,x5); C1 and C2 therefore
+// keep one accumulator tile per x4 in registers.  Wavefronts 0 and 1 split the K range of C1, 2 and
+// 3 that of C2 (their halves are added through LDS at the end), and all four share the C3 row tiles,
+// so every wavefront issues about the same number of MFMAs per chunk.
 template <typename T, int NN>
 __global__ __launch_bounds__(256) void k_e4_pass_a(const T *__restrict__ E4, const double *__restrict__ B1,
                                                    const double *__restrict__ B2, const double *__restrict__ B3,
                                                    int M, int s, double *C1, double *C2, double *C3) {
     constexpr int N2 = NN * NN, N3 = N2 * NN;
     constexpr int KA = (N2 + 3) / 4;          // K steps of C1 / C2 per chunk (K = n^2)
+    constexpr int KH = (KA + 1) / 2;          // K steps of one half
     constexpr int KC = (NN + 3) / 4;          // K steps of C3 per chunk (K = n)
     constexpr int T3 = (N2 + 15) / 16;        // row tiles of C3 (rows (a,c))
-    constexpr int TW = (T3 + 1) / 2;          // C3 tiles per wavefront (2 and 3)
+    constexpr int TW = (T3 + 3) / 4;          // C3 tiles per wavefront
     typedef typename Mfma<T>::V V;
     __shared__ T ech[N3], b1s[N3], b2s[N3];
     __shared__ T b3s[N2];
+    __shared__ T part[2][256];                // the upper K halves of C1 and C2, one x4 tile at a time
     const int g = blockIdx.x;
     const int bp = g % NN, ap = (g / NN) % NN, cp = g / N2;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const bool c2w = wave >= 2;               // this wavefront's half belongs to C2 (else C1)
+    const int k0 = (wave & 1) * KH, k1 = (wave & 1) ? KA : KH;
     const T *slab = E4 + (i64)g * N2 * N3;   // E4[cp,ap,bp,...]
     const i64 obase = ap + (i64)NN * bp + (i64)N2 * cp;
-    V acc3[TW];
+    V accA[NN], acc3[TW];
+#pragma unroll
+    for (int u = 0; u < NN; u++) accA[u] = vzero<V>();
 #pragma unroll
     for (int u = 0; u < TW; u++) acc3[u] = vzero<V>();
-    // the E4 chunk (x4,x5) is chunk x4*n + x5 of the contiguous sub-slab; the next one is loaded
-    // into registers while the current one is multiplied, so its HBM latency is hidden
+    // the next E4 chunk is loaded into registers while the current one is multiplied
     constexpr int PF = (N3 + 255) / 256;
     T pf[PF];
 #pragma unroll
     for (int u = 0; u < PF; u++) {
         const int e = threadIdx.x + 256 * u;
-        pf[u] = e < N3 ? slab[e] : (T)0;
+        pf[u] = e < N3 ? slab[e] : (T)0;          // chunk (x4 = 0, x5 = 0)
     }
-    for (int x4 = 0; x4 < NN; x4++) {
-        V acc = vzero<V>();                    // C1[x4] (wavefront 0) or C2[x4] (wavefront 1)
-        for (int x5 = 0; x5 < NN; x5++) {
-            __syncthreads();                   // the previous chunk is consumed
+    for (int x5 = 0; x5 < NN; x5++) {
+#pragma unroll
+        for (int x4 = 0; x4 < NN; x4++) {
+            __syncthreads();                   // the previous chunk (and B slices) are consumed
 #pragma unroll
             for (int u = 0; u < PF; u++) {
                 const int e = threadIdx.x + 256 * u;
                 if (e < N3) ech[e] = pf[u];
             }
-            for (int e = threadIdx.x; e < N3; e += 256) {
-                b1s[e] = (T)B1[(i64)x5 * N3 + e];
-                b2s[e] = (T)B2[(i64)x5 * N3 + e];
-            }
+            if (x4 == 0)
+                for (int e = threadIdx.x; e < N3; e += 256) {
+                    b1s[e] = (T)B1[(i64)x5 * N3 + e];
+                    b2s[e] = (T)B2[(i64)x5 * N3 + e];
+                }
             for (int e = threadIdx.x; e < N2; e += 256) b3s[e] = (T)B3[((i64)x4 * N2 + x5 * NN) * NN + e];
             __syncthreads();
-            if (x4 * NN + x5 + 1 < N2) {       // prefetch chunk x4*n + x5 + 1
-                const T *nxt = slab + (i64)(x4 * NN + x5 + 1) * N3;
+            {                                  // prefetch the next chunk in visiting order
+                const int nx4 = x4 + 1 < NN ? x4 + 1 : 0, nx5 = x4 + 1 < NN ? x5 : x5 + 1;
+                if (nx5 < NN) {
+                    const T *nxt = slab + (i64)(nx4 * NN + nx5) * N3;
 #pragma unroll
-                for (int u = 0; u < PF; u++) {
-                    const int e = threadIdx.x + 256 * u;
-                    pf[u] = e < N3 ? nxt[e] : (T)0;
+                    for (int u = 0; u < PF; u++) {
+                        const int e = threadIdx.x + 256 * u;
+                        pf[u] = e < N3 ? nxt[e] : (T)0;
+                    }
                 }
             }
-            if (wave == 0) {                   // C1[x4][x8][a] += E[x6,x7,x8] B1[(x5,x6,x7),a], K = (x6,x7)
-#pragma unroll 5
-                for (int ks = 0; ks < KA; ks++) {
+            if (!c2w) {                        // C1[x4][x8][a] += E[x6,x7,x8] B1[(x5,x6,x7),a], K = (x6,x7)
+                for (int ks = k0; ks < k1; ks++) {
                     const int kk = 4 * ks + lk;
                     const bool ok = li < NN && kk < N2;
                     const T a = ok ? ech[kk * NN + li] : (T)0;
                     const T b = ok ? b1s[kk * NN + li] : (T)0;
-                    acc = Mfma<T>::mma(a, b, acc);
+                    accA[x4] = Mfma<T>::mma(a, b, accA[x4]);
                 }
-            } else if (wave == 1) {            // C2[x4][x7][c] += E[x6,x7,x8] B2[(x5,x6,x8),c], K = (x6,x8)
-#pragma unroll 5
-                for (int ks = 0; ks < KA; ks++) {
+            } else {                           // C2[x4][x7][c] += E[x6,x7,x8] B2[(x5,x6,x8),c], K = (x6,x8)
+                for (int ks = k0; ks < k1; ks++) {
                     const int kk = 4 * ks + lk;
                     const bool ok = li < NN && kk < N2;
                     const T a = ok ? ech[(kk / NN) * N2 + li * NN + kk % NN] : (T)0;
                     const T b = ok ? b2s[kk * NN + li] : (T)0;
-                    acc = Mfma<T>::mma(a, b, acc);
-                }
-            } else {                           // C3[(x7,x8)][b] += E[x6,x7,x8] B3[(x4,x5,x6),b], K = x6
-#pragma unroll
-                for (int ks = 0; ks < KC; ks++) {
-                    const int x6 = 4 * ks + lk;
-                    const T b = x6 < NN && li < NN ? b3s[x6 * NN + li] : (T)0;
-#pragma unroll
-                    for (int u = 0; u < TW; u++) {
-                        const int r = 16 * (2 * u + wave - 2) + li;
-                        const T a = x6 < NN && r < N2 ? ech[x6 * N2 + r] : (T)0;
-                        acc3[u] = Mfma<T>::mma(a, b, acc3[u]);
-                    }
+                    accA[x4] = Mfma<T>::mma(a, b, accA[x4]);
                 }
             }
-        }
-        if (wave < 2) {                        // rows x8 (C1) / x7 (C2), columns a (C1) / c (C2)
-            const int j = li;
+            // C3[(x7,x8)][b] += E[x6,x7,x8] B3[(x4,x5,x6),b], K = x6; row tiles 4u + wave
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int i = Mfma<T>::row(lane, r);
-                if (i < NN && j < NN) {
-                    if (wave == 0)
-                        C1[s + (i64)M * (obase + (i64)N3 * j + (i64)N2 * N2 * x4 + (i64)N2 * N3 * i)] = (double)acc[r];
-                    else
-                        C2[s + (i64)M * (obase + (i64)N3 * i + (i64)N2 * N2 * x4 + (i64)N2 * N3 * j)] = (double)acc[r];
+            for (int ks = 0; ks < KC; ks++) {
+                const int x6 = 4 * ks + lk;
+                const T b = x6 < NN && li < NN ? b3s[x6 * NN + li] : (T)0;
+#pragma unroll
+                for (int u = 0; u < TW; u++) {
+                    const int r = 16 * (4 * u + wave) + li;
+                    const T a = x6 < NN && r < N2 ? ech[x6 * N2 + r] : (T)0;
+                    acc3[u] = Mfma<T>::mma(a, b, acc3[u]);
                 }
             }
         }
     }
-    if (wave >= 2) {                           // rows (a,c) = (x7,x8), columns b
+    // C1 / C2: the upper halves (wavefronts 1 and 3) are added to the lower ones through LDS
+#pragma unroll
+    for (int x4 = 0; x4 < NN; x4++) {
+        __syncthreads();
+        if (wave & 1)
+#pragma unroll
+            for (int r = 0; r < 4; r++) part[wave >> 1][lane + 64 * r] = accA[x4][r];
+        __syncthreads();
+        if (!(wave & 1)) {                     // rows x8 (C1) / x7 (C2), columns a (C1) / c (C2)
+            const int j = li;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = Mfma<T>::row(lane, r);
+                const T v = accA[x4][r] + part[wave >> 1][lane + 64 * r];
+                if (i < NN && j < NN) {
+                    if (!c2w)
+                        C1[s + (i64)M * (obase + (i64)N3 * j + (i64)N2 * N2 * x4 + (i64)N2 * N3 * i)] = (double)v;
+                    else
+                        C2[s + (i64)M * (obase + (i64)N3 * i + (i64)N2 * N2 * x4 + (i64)N2 * N3 * j)] = (double)v;
+                }
+            }
+        }
+    }
+    {                                          // C3: rows (a,c) = (x7,x8), columns b
         const int j = li;
 #pragma unroll
         for (int u = 0; u < TW; u++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int rr = 16 * (2 * u + wave - 2) + Mfma<T>::row(lane, r);
+                const int rr = 16 * (4 * u + wave) + Mfma<T>::row(lane, r);
                 if (rr < N2 && j < NN) {
                     const int a = rr / NN, c = rr % NN;
                     C3[s + (i64)M * (obase + (i64)N3 * a + (i64)N2 * N2 * j + (i64)N2 * N3 * c)] = (double)acc3[u][r];
