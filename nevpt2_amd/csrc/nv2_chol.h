@@ -13,5 +13,10 @@ namespace nv2 {
 // chol_workspace(N, nb) doubles.  Asynchronous on st.
 i64 chol_workspace(int N, int nb);
 int chol_upper(rocblas_handle h, hipStream_t st, int N, double *A, i64 sA, int nb, int *info, double *work);
+// Ui_b = U_b^-1 for the nb upper triangular factors U + b*sU (order N, ld N; their strict lower
+// triangle is not read) into Ui + b*sUi (ld N, strict lower triangle zero).  work holds
+// chol_workspace(N, nb) doubles.  Asynchronous on st.
+int chol_inverse_upper(rocblas_handle h, hipStream_t st, int N, const double *U, i64 sU, int nb, double *Ui, i64 sUi,
+                       double *work);
 
 }  // namespace nv2

@@ -91,6 +91,39 @@ __global__ void k_chol_put(double *A, int N, i64 sA, int k0, int kb, int m, cons
     A[b * sA + (k0 + r) + (i64)N * (k0 + kb + c)] = P[b * sP + r + (i64)CHOL_NB * c];
 }
 
+// U_jj^-1 for every diagonal block jj = blockIdx.x of matrix b = blockIdx.z, into the same block of
+// Ui (upper triangle and diagonal; Ui was cleared before)
+__global__ __launch_bounds__(CHOL_WG) void k_trinv_diag(const double *U, int N, i64 sU, double *Ui, i64 sUi) {
+    __shared__ double a[CHOL_NB][CHOL_NB + 1];
+    __shared__ double x[CHOL_NB][CHOL_NB + 1];
+    const int b = blockIdx.z, k0 = blockIdx.x * CHOL_NB, kb = N - k0 < CHOL_NB ? N - k0 : CHOL_NB;
+    const double *Ub = U + b * sU + k0 + (i64)N * k0;
+    double *Xb = Ui + b * sUi + k0 + (i64)N * k0;
+    for (int t = threadIdx.x; t < kb * kb; t += CHOL_WG) {
+        const int r = t % kb, c = t / kb;
+        a[r][c] = r <= c ? Ub[r + (i64)N * c] : 0.0;
+        x[r][c] = 0.0;
+    }
+    __syncthreads();
+    for (int i = kb - 1; i >= 0; i--) {
+        for (int c = i + threadIdx.x; c < kb; c += CHOL_WG) {
+            double s = c == i ? 1.0 : 0.0;
+            for (int k = i + 1; k <= c; k++) s -= a[i][k] * x[k][c];
+            x[i][c] = s / a[i][i];
+        }
+        __syncthreads();
+    }
+    for (int t = threadIdx.x; t < kb * kb; t += CHOL_WG) {
+        const int r = t % kb, c = t / kb;
+        if (r <= c) Xb[r + (i64)N * c] = x[r][c];
+    }
+}
+
+__global__ void k_chol_clear(double *x, i64 cnt) {
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < cnt) x[t] = 0.0;
+}
+
 __global__ void k_chol_zero_info(int *info, int nb) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nb) info[t] = 0;
@@ -99,6 +132,34 @@ __global__ void k_chol_zero_info(int *info, int nb) {
 #define RBH(call) do { if ((call) != rocblas_status_success) { set_error("rocBLAS failed: %s", #call); return -1; } } while (0)
 
 i64 chol_workspace(int N, int nb) { return (i64)nb * CHOL_NB * ((i64)CHOL_NB + N); }
+
+// Ui = U^-1 by block columns: the diagonal blocks are inverted in LDS, then for each block column j
+// (columns k0 .. k0+kb) Ui[0:k0, j] = -(Ui[0:k0, 0:k0] U[0:k0, j]) Ui[j, j], two GEMMs on the
+// already inverted leading block.
+int chol_inverse_upper(rocblas_handle h, hipStream_t st, int N, const double *U, i64 sU, int nb, double *Ui, i64 sUi,
+                       double *work) {
+    if (N < 1 || nb < 1 || nb > 65535) { set_error("chol_inverse_upper: unsupported N=%d nb=%d", N, nb); return -2; }
+    RBH(rocblas_set_stream(h, st));
+    for (int b = 0; b < nb; b++) {
+        hipLaunchKernelGGL(k_chol_clear, dim3((unsigned)(((i64)N * N + 255) / 256)), dim3(256), 0, st, Ui + b * sUi,
+                           (i64)N * N);
+        NV2_LAUNCH_CHECK();
+    }
+    const int nblk = (N + CHOL_NB - 1) / CHOL_NB;
+    hipLaunchKernelGGL(k_trinv_diag, dim3(nblk, 1, nb), dim3(CHOL_WG), 0, st, U, N, sU, Ui, sUi);
+    NV2_LAUNCH_CHECK();
+    double *P = work;                            // k0 x kb panels, ld N, one per matrix
+    const i64 sP = (i64)CHOL_NB * N;
+    const double one = 1.0, mone = -1.0, zero = 0.0;
+    for (int k0 = CHOL_NB; k0 < N; k0 += CHOL_NB) {
+        const int kb = N - k0 < CHOL_NB ? N - k0 : CHOL_NB;
+        RBH(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, k0, kb, k0, &one, Ui, N,
+                                          sUi, U + (i64)N * k0, N, sU, &zero, P, N, sP, nb));
+        RBH(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, k0, kb, kb, &mone, P, N,
+                                          sP, Ui + k0 + (i64)N * k0, N, sUi, &zero, Ui + (i64)N * k0, N, sUi, nb));
+    }
+    return 0;
+}
 
 int chol_upper(rocblas_handle h, hipStream_t st, int N, double *A, i64 sA, int nb, int *info, double *work) {
     if (N < 1 || nb < 1 || nb > 65535) { set_error("chol_upper: unsupported N=%d nb=%d", N, nb); return -2; }

@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <utility>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1469,6 +1470,9 @@ static int pc_metric_launch(Stage2 *p, PcJob &j, double thr) {
         tm.mark(0, st);
         // T_b (S - thr*I) and caux_b (S) factored as one batch: info words b and nb + b
         TRYS(chol_upper(h, st, j.N, j.T, NN, 2 * j.nb, p->dinfo_w + 4 * j.slot, j.cholw));
+        // U^-1 into T (whose factor of S - thr*I has served its purpose: only its info word is read),
+        // while K is not there yet; K' then takes two GEMMs
+        TRYS(chol_inverse_upper(h, st, j.N, j.caux, NN, j.nb, j.T, NN, j.cholw));
         for (int k = 1; k < 4; k++) tm.mark(k, st);
         tm.report(j.N, j.nb);            // the factorisations: the first figure
         NV2_HIP(hipMemcpyAsync(p->hinfo + 4 * j.slot, p->dinfo_w + 4 * j.slot, sizeof(int) * 4, hipMemcpyDeviceToHost,
@@ -1497,14 +1501,19 @@ static int pc_kprime_launch(Stage2 *p, PcJob &j, double thr) {
         if (ok) {
             j.nmax = N;
             for (int b = 0; b < j.nb; b++) { j.nnew[b] = N; j.minw[b] = thr; }   // thr: a lower bound
+            EigTiming tk(N);             // K' formation: printed as the first figure
+            tk.mark(0, st);
+            std::swap(j.caux, j.T);      // caux = U^-1, so caux caux^T = S^-1; T is free scratch
             for (int b = 0; b < j.nb; b++)
-                hipLaunchKernelGGL(k_sym_upper, dim3(nb(NN, 256)), dim3(256), 0, st, j.Kb(b), N, j.Kp + b * NN);
+                hipLaunchKernelGGL(k_sym_upper, dim3(nb(NN, 256)), dim3(256), 0, st, j.Kb(b), N, j.Ks + b * NN);
             NV2_LAUNCH_CHECK();
-            const double one = 1.0;
-            RB(rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
-                                             rocblas_diagonal_non_unit, N, N, &one, j.caux, N, NN, j.Kp, N, NN, j.nb));
-            RB(rocblas_dtrsm_strided_batched(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                                             rocblas_diagonal_non_unit, N, N, &one, j.caux, N, NN, j.Kp, N, NN, j.nb));
+            const double one = 1.0, zero = 0.0;
+            RB(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, N, N, N, &one, j.Ks, N,
+                                             NN, j.caux, N, NN, &zero, j.T, N, NN, j.nb));
+            RB(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, N, N, N, &one,
+                                             j.caux, N, NN, j.T, N, NN, &zero, j.Kp, N, NN, j.nb));
+            for (int k = 1; k < 4; k++) tk.mark(k, st);
+            tk.report(N, j.nb);
             return pc_trd_launch(p, j);
         }
         j.chol = false;                  // some eigenvalue below thr: the eigen path after all
@@ -2441,14 +2450,9 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         const int ncq = 1 + (q.M ? M - 1 : 0);              // right-hand sides: this state, then the QD partners
         GET(Ypc, "pm1_Ypc", (i64)nmax * next * ncq);
         GET(Wq, "pm1_applyq", trd_apply_workspace(nmax, 1, next * ncq));
-        // y = caux^T x, or U^-T x on the Cholesky path (nmax = N3 there)
+        // y = caux^T x (on the Cholesky path caux = U^-1 and nmax = N3)
         auto project = [&](const double *x, double *y) -> int {
-            if (!jb.chol) return gemm(p, true, false, nmax, next, (int)N3, 1.0, caux, N3, x, N3, 0.0, y, nmax);
-            NV2_HIP(hipMemcpyAsync(y, x, sizeof(double) * N3 * next, hipMemcpyDeviceToDevice, st));
-            const double one = 1.0;
-            RB(rocblas_dtrsm(p->h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
-                             rocblas_diagonal_non_unit, (int)N3, next, &one, caux, (int)N3, y, nmax));
-            return 0;
+            return gemm(p, true, false, nmax, next, (int)N3, 1.0, caux, N3, x, N3, 0.0, y, nmax);
         };
         TRYS(project(Xpm[plus], Ypc));
         int col = 1;
