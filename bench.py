@@ -66,6 +66,14 @@ def trd_bytes(N, nb):
     return 16.0 * nb * float((m * m).sum())
 
 
+def band_flops(N, nb):
+    """Algorithmic flops of the reduction to band form (half-bandwidth 64): per panel of m trailing rows,
+    W = A22 Y and the rank-128 update A22 -= Y Z^T + Z Y^T, 6 * 64 m^2, plus the CholeskyQR2 and
+    reflector products on the m x 64 panel, about 10 * 64^2 m."""
+    m = np.arange(N - 64, 64, -64, dtype=np.float64)
+    return nb * float((6.0 * 64 * m * m + 10.0 * 64 * 64 * m).sum())
+
+
 def measured_traffic(kernel, config, prec):
     """HBM bytes per evaluation of a kernel family from the committed rocprofv3 counter runs
     (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), for the workload they were taken on."""
@@ -134,13 +142,15 @@ def time_form(evs, prec, steps, warmup, S):
     for ev in evs:
         for _ in range(warmup):
             ev.run(prec)
-    lat1, phases, kred = [], np.zeros(len(PHASES)), []
+    lat1, phases, kred, kform = [], np.zeros(len(PHASES)), [], 0
     for _ in range(steps):
         t = time.perf_counter()
         res = evs[0].run(prec)
         lat1.append((time.perf_counter() - t) * 1e3)
         phases += np.array(list(evs[0].timings().values()))
-        kred.append(evs[0].pc_timings()[:, 0].sum())
+        pt = evs[0].pc_timings()
+        kred.append(pt[:, 0].sum())
+        kform = int(pt[:, 3].max())
     phases /= steps
     latS = []
     if S == 1:         # the timed steps: one evaluation each, on the calling thread
@@ -162,7 +172,7 @@ def time_form(evs, prec, steps, warmup, S):
                     latS.append(lat)
             elapsed = time.perf_counter() - t0
     return {"res": res, "elapsed": elapsed, "value": S * steps / elapsed, "ms_per_step": elapsed / steps * 1e3,
-            "phases": dict(zip(PHASES, [float(x) for x in phases])), "kred_ms": float(np.mean(kred)),
+            "phases": dict(zip(PHASES, [float(x) for x in phases])), "kred_ms": float(np.mean(kred)), "kred_form": kform,
             "latency_ms": {"p50": float(np.percentile(latS, 50)), "p99": float(np.percentile(latS, 99)),
                            "unloaded_p50": float(np.percentile(lat1, 50)),
                            "unloaded_p99": float(np.percentile(lat1, 99))}}
@@ -215,10 +225,19 @@ def main():
     head = max(ok, key=lambda k: forms[k]["value"]) if ok else list(forms)[0]
     F = forms[head]
     esize = 8 if head == "fp64" else 4
-    # rooflines: the K' tridiagonal reduction of the (+-1)' chain and the bamat contraction passes
+    # rooflines: the K' reduction of the (+-1)' chain (band or tridiagonal form) and the bamat contraction passes
     rl = []
     Nk = n ** 3
-    if F["kred_ms"] > 0:
+    if F["kred_ms"] > 0 and F["kred_form"] == 2:
+        fb = band_flops(Nk, 2) * M
+        ach = fb / (F["kred_ms"] * 1e-3) / 1e12
+        rl.append({"kernel": "K' band reduction (CholeskyQR2 panels, block-reflector GEMMs, two problems batched)",
+                   "bound": "mfma", "achieved": ach, "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
+                   "frac": ach / (PEAK["fp64"] / 1e12), "traffic": None, "device_ms_per_eval": F["kred_ms"],
+                   "launches_per_eval": 15 * (Nk // 64) * M,
+                   "algorithmic": f"6*64*m^2 + 10*64^2*m per panel of m trailing rows, two problems of order {Nk}: "
+                                  f"{fb:.3e} flop per evaluation"})
+    elif F["kred_ms"] > 0:
         by = trd_bytes(Nk, 2) * M
         ach = by / (F["kred_ms"] * 1e-3) / 1e9
         rl.append({"kernel": "K' tridiagonal reduction (k_trd_col, one launch per column)", "bound": "hbm",

@@ -116,9 +116,10 @@ int nv2_ctx_set_ci(nv2_ctx *ctx, int64_t nalpha, int64_t nbeta, const double *ci
  * (smallest metric eigenvalue, eigenvalues kept, 1 if the metric was factored by
  * Cholesky because none falls below thr_lindep -- the smallest is then reported
  * as thr_lindep, a lower bound -- else 0), to flag threshold-sensitive inputs.
- * "PC-timings" (after a run): per state 3 doubles, the device time in ms of the
- * tridiagonal reduction of the (+-1)' Koopmans matrices K' (HIP events on its
- * stream), their order and their number (0 when the state has none). */
+ * "PC-timings" (after a run): per state 4 doubles, the device time in ms of the
+ * reduction of the (+-1)' Koopmans matrices K' (HIP events on its stream), their
+ * order, their number and the form reduced to (1 tridiagonal, 2 band of
+ * half-bandwidth 64); zeros when the state has none. */
 int nv2_ctx_set_trdm(nv2_ctx *ctx, const double *trdm3, const double *trdm2, const double *trdm1);
 int nv2_ctx_run(nv2_ctx *ctx, int32_t prec, double thr_lindep, double *out);
 int nv2_ctx_timings(nv2_ctx *ctx, double *ms);

@@ -5,6 +5,7 @@
 // state (qdnevpt, qdnevpt.F:858-993) -- with per-phase device timings.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -43,7 +44,7 @@ struct nv2_ctx {
     std::vector<double> qd;                 // per state s: 8 classes x (SC e2mp(j,s), PC e2en(j,s)), 16*M
     std::vector<double> heff_sc, heff_pc;   // second-order Heff, H[j + M*i] = e2mp(j,i) / e2en(j,i)
     std::vector<double> pcm;                // per state, per class: (min metric eigenvalue, kept, Cholesky path)
-    std::vector<double> pct;                // per state: (K' reduction ms, its order, its batch)
+    std::vector<double> pct;                // per state: (K' reduction ms, its order, its batch, its form)
     double *find(const char *nm) {
         for (auto &b : bufs) if (b.name == nm) return b.p;
         return nullptr;
@@ -244,7 +245,7 @@ int nv2_ctx_run(nv2_ctx *c, int32_t prec, double thr_lindep, double *out) {
     const bool qd = c->has_trdm && M > 1;
     c->qd.assign(qd ? (size_t)16 * M * M : 0, 0.0);
     c->pcm.assign((size_t)24 * M, 0.0);
-    c->pct.assign((size_t)3 * M, 0.0);
+    c->pct.assign((size_t)4 * M, 0.0);
     for (int s = 0; s < M; s++) {
         S2Stats sd{};
         int rc = stage2_state(c->s2, s, thr_lindep, prec, out + 32 * s, &sd, qd ? c->qd.data() + 16 * M * s : nullptr);
@@ -254,9 +255,10 @@ int nv2_ctx_run(nv2_ctx *c, int32_t prec, double thr_lindep, double *out) {
             c->pcm[24 * s + 3 * k + 1] = sd.nnew[k];
             c->pcm[24 * s + 3 * k + 2] = sd.chol[k];
         }
-        c->pct[3 * s] = sd.kred_ms;
-        c->pct[3 * s + 1] = sd.kred_N;
-        c->pct[3 * s + 2] = sd.kred_nb;
+        c->pct[4 * s] = sd.kred_ms;
+        c->pct[4 * s + 1] = sd.kred_N;
+        c->pct[4 * s + 2] = sd.kred_nb;
+        c->pct[4 * s + 3] = sd.kred_form;
     }
     c->heff_sc.assign((size_t)M * M, 0.0);
     c->heff_pc.assign((size_t)M * M, 0.0);

@@ -22,6 +22,7 @@ struct S2Stats {              // diagnostics for threshold-sensitive seeds
     int chol[8];                 // 1: the metric was factored (every eigenvalue at or above thr)
     double kred_ms;              // device time of the K' reduction of the (+-1)' problems (0: none)
     int kred_N, kred_nb;         // its order and batch
+    int kred_form;               // 1: tridiagonal form, 2: band form (half-bandwidth 64)
     double eps_min_virt, eps_max_core;
 };
 

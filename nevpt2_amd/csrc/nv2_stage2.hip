@@ -26,6 +26,7 @@
 
 #include "nv2_common.h"
 #include "nv2_stage2.h"
+#include "nv2_band.h"
 #include "nv2_chol.h"
 #include "nv2_trd.h"
 #include "../../include/nevpt2_amd.h"
@@ -1061,6 +1062,252 @@ __global__ void k_pm1p_shift(int plus, int nnew, int next, const double *d, cons
     if (q.M) block_storeQ(qsc, qpc, q);
 }
 
+// The same sums on the band form K' = Q B Q^T (nv2_band.hip): one workgroup (one wavefront) per
+// external orbital x solves (B + s_x) u = z_x by Gaussian elimination with partial pivoting.  The
+// active part of step j is the window of rows j..j+BAND_B and columns j..j+2*BAND_B (a pivot row
+// from below the diagonal carries the fill up to 2*BAND_B columns right of it).  The window lives in
+// LDS: its rows by a slot table (perm: lane i holds the slot of row j+i, the last row in a uniform
+// register), its columns circularly (column c at slot c % BAND_W), so a step moves no data: the
+// eliminated row leaves its slot to row j+BAND_B+1, the eliminated column its slot to column
+// j+2*BAND_B+1 (zero in every row present, untouched by earlier steps).  U is stored by rows of
+// BAND_W + 1: the reciprocal of the pivot, the relative columns 1..2*BAND_B, the eliminated
+// right-hand side.  Global data moves in blocks of BSTEP rows (band rows entering the window in the
+// forward pass, rows of U in the backward pass), loaded one block ahead into registers and staged
+// through LDS, so that no step waits for memory.
+// E -= z.u, norm += u.u, and for QD the coupling -z_j.u with z_j = q.a[j] (same layout as Z).
+#define PBR (BAND_B + 1)
+#define UW (BAND_W + 1)
+#define BSTEP 8
+__device__ __forceinline__ double rlane(double v, int i) {   // lane i's value; i uniform
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), i);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), i);
+    return __hiloint2double(hi, lo);
+}
+// barrier of a one-wavefront workgroup that orders LDS only: global loads stay in flight across it
+__device__ __forceinline__ void lds_barrier() { __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+// one class of the launch: band rows, right-hand sides z_x at Z + N*x, orbital energies, U rows and
+// solutions (N * UW and N per orbital), QD partners (q.a[j]: same layout as Z)
+struct BandCls {
+    const double *Br, *Z, *eo;
+    double *U, *y;
+    int plus, next;
+    QDArgs q;
+};
+// blocks 0 .. c0.next-1 solve for the orbitals of class c0, the following c1.next those of class c1
+__global__ __launch_bounds__(64) void k_pm1p_band(int N, BandCls c0, BandCls c1, double *part) {
+    __shared__ double win[PBR][BAND_W];
+    __shared__ double stg[BSTEP][UW];              // staged block: band rows + z, then rows of U
+    __shared__ double ys[BAND_W];
+    __shared__ __align__(16) int ptab[BAND_B];     // slot and multiplier of window row i at i - 1
+    __shared__ __align__(16) double ltab[BAND_B];
+    const bool second = (int)blockIdx.x >= c0.next;
+    const int x = second ? (int)blockIdx.x - c0.next : (int)blockIdx.x, lane = threadIdx.x;
+    const int plus = second ? c1.plus : c0.plus;
+    const double *Br = second ? c1.Br : c0.Br;
+    const double *eo = second ? c1.eo : c0.eo;
+    const QDArgs &q = c0.q;                        // M, self and qpart are those of both classes
+    const double *qa[QM];
+#pragma unroll
+    for (int k = 0; k < QM; k++) qa[k] = second ? c1.q.a[k] : c0.q.a[k];
+    const i64 ldz = N;
+    const double sh = plus ? -eo[x] : eo[x];
+    const double *z = (second ? c1.Z : c0.Z) + ldz * x;
+    double *U = (second ? c1.U : c0.U) + (i64)x * N * UW;
+    double *yx = (second ? c1.y : c0.y) + (i64)x * N;
+    const int ka = lane, kb = lane + 64;           // relative columns of this lane (lane 0 also 2*BAND_B)
+    for (int t = lane; t < PBR * BAND_W; t += 64) {   // (B + s)(r, c), zero outside the matrix and the band
+        const int r = t / BAND_W, c = t % BAND_W, d = c - r;
+        double v = 0.0;
+        if (r < N && c < N && d >= -BAND_B && d <= BAND_B) v = Br[(i64)r * BAND_W + d + BAND_B] + (d == 0 ? sh : 0.0);
+        win[r][c] = v;
+    }
+    int perm = lane, perm64 = 64;                  // slots of window rows lane and BAND_B
+    double zr = lane < N ? z[lane] : 0.0, z64 = BAND_B < N ? z[BAND_B] : 0.0;
+    // block prefetch of band rows r0 .. r0+BSTEP-1 (relative columns ka, kb, 2*BAND_B; z on lanes < BSTEP)
+    double fa[BSTEP], fb[BSTEP], fc[BSTEP], fz;
+    auto fetch_rows = [&](int r0) {
+#pragma unroll
+        for (int u = 0; u < BSTEP; u++) {
+            const bool in = r0 + u < N;
+            const double *row = Br + (i64)(r0 + u) * BAND_W;
+            fa[u] = in ? row[ka] : 0.0;
+            fb[u] = in ? row[kb] : 0.0;
+            fc[u] = in && lane == 0 ? row[2 * BAND_B] : 0.0;
+        }
+        fz = lane < BSTEP && r0 + lane < N ? z[r0 + lane] : 0.0;
+    };
+    fetch_rows(PBR);
+    lds_barrier();
+#pragma unroll 1
+    for (int j0 = 0; j0 < N; j0 += BSTEP) {
+        // rows j0+PBR .. enter during this block: stage them, then fetch the next block
+#pragma unroll
+        for (int u = 0; u < BSTEP; u++) {
+            stg[u][ka] = fa[u];
+            stg[u][kb] = fb[u] + (kb == BAND_B && j0 + PBR + u < N ? sh : 0.0);   // k' = BAND_B: diagonal
+            if (lane == 0) stg[u][2 * BAND_B] = fc[u];
+        }
+        if (lane < BSTEP) stg[lane][BAND_W] = fz;
+        fetch_rows(j0 + PBR + BSTEP);
+        lds_barrier();
+        const int jn = N - j0 < BSTEP ? N - j0 : BSTEP;
+#pragma unroll 1
+        for (int jj = 0; jj < jn; jj++) {
+            const int j = j0 + jj;
+            const int nr = N - j < PBR ? N - j : PBR;
+            const int cs0 = j % BAND_W;
+            // pivot: the largest |entry| of column j.  The key is |entry| with its 7 lowest mantissa bits
+            // replaced by 127 - row: one unsigned maximum, ties (to 2^-45 relative) going to the first row
+            double cv = lane < nr ? win[perm][cs0] : 0.0;
+            double cv64 = nr == PBR ? win[perm64][cs0] : 0.0;
+            auto key = [](double v, int i) -> unsigned long long {
+                return ((unsigned long long)__double_as_longlong(fabs(v)) & ~0x7Full) | (unsigned long long)(127 - i);
+            };
+            unsigned long long kk = lane < nr ? key(cv, lane) : 0ull;
+            if (lane == 0 && nr == PBR) { const unsigned long long k64 = key(cv64, BAND_B); if (k64 > kk) kk = k64; }
+            for (int off = 32; off > 0; off >>= 1) {
+                const unsigned long long o = __shfl_xor(kk, off, 64);
+                if (o > kk) kk = o;
+            }
+            const int pv = 127 - (int)(__builtin_amdgcn_readfirstlane((int)(kk & 0x7F)));
+            const int s0 = __builtin_amdgcn_readlane(perm, 0);
+            const int sp = pv < 64 ? __builtin_amdgcn_readlane(perm, pv) : perm64;
+            const double z0 = rlane(zr, 0), zp = pv < 64 ? rlane(zr, pv) : z64;
+            const double c0v = rlane(cv, 0), piv = pv < 64 ? rlane(cv, pv) : cv64;
+            if (pv != 0) {                         // rows 0 and pv exchange slots, z and column values
+                if (lane == pv) { perm = s0; zr = z0; cv = c0v; }
+                if (pv == BAND_B) { perm64 = s0; z64 = z0; cv64 = c0v; }
+            }
+            if (lane == 0) { perm = sp; zr = zp; cv = piv; }
+            const double rp = 1.0 / piv;
+            const int sa = cs0 + ka < BAND_W ? cs0 + ka : cs0 + ka - BAND_W;
+            const int sb = cs0 + kb < BAND_W ? cs0 + kb : cs0 + kb - BAND_W;
+            const int sc = cs0 + 2 * BAND_B < BAND_W ? cs0 + 2 * BAND_B : cs0 + 2 * BAND_B - BAND_W;
+            const double ua = win[sp][sa], ub = win[sp][sb], uc = lane == 0 ? win[sp][sc] : 0.0;
+            double *Uj = U + (i64)j * UW;
+            Uj[ka] = ka == 0 ? rp : ua;
+            Uj[kb] = ub;
+            if (lane == 0) Uj[2 * BAND_B] = uc;
+            if (lane == 1) Uj[BAND_W] = zp;        // the eliminated right-hand side
+            const double l = lane > 0 && lane < nr ? cv * rp : 0.0;   // rows past the matrix: multiplier 0
+            const double l64 = nr == PBR ? cv64 * rp : 0.0;
+            zr -= l * zp;
+            z64 -= l64 * zp;
+            if (lane > 0) { ptab[lane - 1] = perm; ltab[lane - 1] = l; }
+            else { ptab[BAND_B - 1] = perm64; ltab[BAND_B - 1] = l64; }
+            lds_barrier();                         // pivot row and column read before the updates
+            // rows 1..BAND_B -= l_i * pivot row, columns 1..2*BAND_B; the rows are distinct slots, so
+            // eight at a time: all loads, then all stores.  Slots and multipliers come from LDS
+            // (broadcast reads): moving them lane to lane through scalar registers stalls every row.
+#pragma unroll 1
+            for (int i0 = 1; i0 < PBR; i0 += 8) {
+                int rr[8];
+                double lv[8], xa[8], xb[8], xc[8];
+                {
+                    const int4 pa4 = *(const int4 *)&ptab[i0 - 1], pb4 = *(const int4 *)&ptab[i0 + 3];
+                    rr[0] = pa4.x; rr[1] = pa4.y; rr[2] = pa4.z; rr[3] = pa4.w;
+                    rr[4] = pb4.x; rr[5] = pb4.y; rr[6] = pb4.z; rr[7] = pb4.w;
+#pragma unroll
+                    for (int u = 0; u < 8; u += 2) {
+                        const double2 l2 = *(const double2 *)&ltab[i0 - 1 + u];
+                        lv[u] = l2.x;
+                        lv[u + 1] = l2.y;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    xa[u] = win[rr[u]][sa];
+                    xb[u] = win[rr[u]][sb];
+                    xc[u] = lane == 0 ? win[rr[u]][sc] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    if (ka != 0) win[rr[u]][sa] = xa[u] - lv[u] * ua;
+                    win[rr[u]][sb] = xb[u] - lv[u] * ub;
+                    if (lane == 0) win[rr[u]][sc] = xc[u] - lv[u] * uc;
+                }
+            }
+            // drop row j (its slot takes row j+BAND_B+1) and column j (its slot takes column j+2*BAND_B+1)
+            int np = __shfl_down(perm, 1, 64);
+            double nzr = __shfl_down(zr, 1, 64);
+            if (lane == 63) { np = perm64; nzr = z64; }
+            perm = np;
+            zr = nzr;
+            perm64 = sp;
+            z64 = stg[jj][BAND_W];
+            win[perm][cs0] = 0.0;
+            {   // new row from the stage: relative columns k' of the next window are columns j + 1 + k'
+                const int ta = cs0 + 1 + ka, tb = cs0 + 1 + kb, tc = cs0 + 1 + 2 * BAND_B;
+                win[sp][ta < BAND_W ? ta : ta - BAND_W] = stg[jj][ka];
+                win[sp][tb < BAND_W ? tb : tb - BAND_W] = stg[jj][kb];
+                if (lane == 0) win[sp][tc < BAND_W ? tc : tc - BAND_W] = stg[jj][2 * BAND_B];
+            }
+            lds_barrier();
+        }
+    }
+    // back substitution: u_j = (zf_j - sum_k U[j][k] u_{j+k}) * U[j][0], u_{j+k} at ys[(j + k) % BAND_W],
+    // rows of U by blocks of BSTEP from the bottom (lane 1 also fetches the right-hand side at BAND_W)
+    __threadfence_block();
+    for (int t = lane; t < BAND_W; t += 64) ys[t] = 0.0;
+    auto fetch_u = [&](int jtop) {                 // rows jtop, jtop-1, .. jtop-BSTEP+1
+#pragma unroll
+        for (int u = 0; u < BSTEP; u++) {
+            const bool in = jtop - u >= 0;
+            const double *row = U + (i64)(jtop - u) * UW;
+            fa[u] = in ? row[ka] : 0.0;
+            fb[u] = in ? row[kb] : 0.0;
+            fc[u] = in && lane < 2 ? row[2 * BAND_B + lane] : 0.0;
+        }
+    };
+    fetch_u(N - 1);
+    lds_barrier();
+    double ez = 0.0, uu = 0.0, qsc[QM], qpc[QM];
+#pragma unroll
+    for (int k = 0; k < QM; k++) { qsc[k] = 0.0; qpc[k] = 0.0; }
+#pragma unroll 1
+    for (int jt = N - 1; jt >= 0; jt -= BSTEP) {
+#pragma unroll
+        for (int u = 0; u < BSTEP; u++) {
+            stg[u][ka] = fa[u];
+            stg[u][kb] = fb[u];
+            if (lane < 2) stg[u][2 * BAND_B + lane] = fc[u];
+        }
+        fetch_u(jt - BSTEP);
+        lds_barrier();
+        const int jn = jt + 1 < BSTEP ? jt + 1 : BSTEP;
+#pragma unroll 1
+        for (int u = 0; u < jn; u++) {
+            const int j = jt - u;
+            const int cs = j % BAND_W;
+            const int sa = cs + ka < BAND_W ? cs + ka : cs + ka - BAND_W;
+            const int sb = cs + kb < BAND_W ? cs + kb : cs + kb - BAND_W;
+            const int sc = cs + 2 * BAND_B < BAND_W ? cs + 2 * BAND_B : cs + 2 * BAND_B - BAND_W;
+            double s = stg[u][kb] * ys[sb];
+            if (ka != 0) s += stg[u][ka] * ys[sa];
+            if (lane == 0) s += stg[u][2 * BAND_B] * ys[sc];
+            s = wsum(s);
+            if (lane == 0) {
+                const double y = (stg[u][BAND_W] - s) * stg[u][0];
+                ys[cs] = y;
+                yx[j] = y;
+            }
+            lds_barrier();
+        }
+    }
+    __threadfence();
+    __syncthreads();
+    for (int i = lane; i < N; i += 64) {
+        const double u = yx[i];
+        ez += z[i] * u;
+        uu += u * u;
+#pragma unroll
+        for (int k = 0; k < QM; k++)
+            if (qd_on(q, k)) qpc[k] -= qa[k][ldz * x + i] * u;
+    }
+    block_store4(0.0, 0.0, -ez, uu, part);
+    if (q.M) block_storeQ(qsc, qpc, q);
+}
+
 // ------------------------------------------------------------------ host side
 struct Buf {
     void *p = nullptr;
@@ -1076,7 +1323,7 @@ struct Stage2 {
     hipEvent_t evS = nullptr, evK = nullptr;   // its metric filled / its Koopmans matrix filled
     hipEvent_t evKr[2] = {nullptr, nullptr};   // around the K' reduction of the (+-1)' problems (timed)
     bool kr_recorded = false;
-    int kr_N = 0, kr_nb = 0;
+    int kr_N = 0, kr_nb = 0, kr_form = 0;
     rocblas_handle h;
     hipStream_t st;
     rocblas_handle wh[NW] = {};
@@ -1270,8 +1517,8 @@ static double *parts(Stage2 *p, i64 nblocks) {
     }
     return p->part;
 }
-static int reduce_into(Stage2 *p, i64 nblocks, int cls) {
-    hipLaunchKernelGGL(k_reduce_parts, dim3(4), dim3(256), 0, p->st, p->part, nblocks, p->acc + 4 * cls);
+static int reduce_into(Stage2 *p, i64 nblocks, int cls, i64 first = 0) {   // blocks first .. first+nblocks-1
+    hipLaunchKernelGGL(k_reduce_parts, dim3(4), dim3(256), 0, p->st, p->part + 4 * first, nblocks, p->acc + 4 * cls);
     NV2_LAUNCH_CHECK();
     return 0;
 }
@@ -1289,9 +1536,10 @@ static int qd_prepare(Stage2 *p, QDArgs &q, i64 nblocks) {
     q.qpart = p->qpart;
     return 0;
 }
-static int qd_reduce(Stage2 *p, const QDArgs &q, i64 nblocks, int cls) {
+static int qd_reduce(Stage2 *p, const QDArgs &q, i64 nblocks, int cls, i64 first = 0) {
     if (!q.M) return 0;
-    hipLaunchKernelGGL(k_reduce_q, dim3(2 * QM), dim3(256), 0, p->st, p->qpart, nblocks, p->accq + 2 * QM * cls);
+    hipLaunchKernelGGL(k_reduce_q, dim3(2 * QM), dim3(256), 0, p->st, p->qpart + 2 * QM * first, nblocks,
+                       p->accq + 2 * QM * cls);
     NV2_LAUNCH_CHECK();
     return 0;
 }
@@ -1307,6 +1555,8 @@ struct PcJob {
     int N = 0, nb = 1, w = 0, slot = 0;        // info words at 4*slot + {metric b, K' 2+b}
     bool trd = false;                          // K' only reduced (the (+-1)' sums solve with its T)
     bool chol = false;                         // metric factored as S = U^T U (every eigenvalue kept)
+    bool band = false;                         // K' reduced to band form (nv2_band.hip) instead of T
+    double *bandw = nullptr;                   // band_reduce workspace (trd jobs)
     double *trdw = nullptr;                    // K' reduction scratch; T and tau of problem b from trd_outputs(nmax)
     double *trdm = nullptr, *trde = nullptr;   // eig_trd: reduction scratch, off-diagonal copy for dstedc
     double *trdq = nullptr;                    // scratch of applying Q
@@ -1333,7 +1583,10 @@ static int pc_job_buffers(Stage2 *p, PcJob &j, size_t hoff) {
     if (j.trd) {
         GET(cw, (j.tag + "_cholw").c_str(), chol_workspace(j.N, 2 * j.nb));
         j.cholw = cw;
+        GET(bw, (j.tag + "_bandw").c_str(), band_workspace(j.N, j.nb));
+        j.bandw = bw;
     }
+    j.band = false;
     j.hw1 = p->hpin + hoff;
     if (j.N >= 3) {                  // scratch of eig_trd (every job) and of the K' reduction (trd jobs)
         GET(tm, (j.tag + "_trdm").c_str(), trd_workspace(j.N, j.nb));
@@ -1386,6 +1639,7 @@ static int pc_trd_launch(Stage2 *p, PcJob &j) {
     p->kr_recorded = true;
     p->kr_N = j.nmax;
     p->kr_nb = j.nb;
+    p->kr_form = 1;
     if (tm.on) {
         for (int k = 1; k < 4; k++) tm.mark(k, p->ws[j.w]);
         tm.report(j.nmax, j.nb);       // the reduction of K' only: the first figure
@@ -1450,6 +1704,15 @@ static bool chol_metric() {   // read at every solve, so that a caller can switc
     const char *e = getenv("NV2_METRIC_EIGEN");
     return !(e && atoi(e) > 0);
 }
+// K' of the Cholesky path reduced to band form or to tridiagonal form.  The band form wins from
+// about N = 1500 up (CAS(14,14), N = 2744: 157 vs 165 ms per evaluation; CAS(10,10), N = 1000:
+// 29 vs 25 ms): NV2_K_TRIDIAG=1 / NV2_K_BAND=1 force one form.
+static bool band_kprime(int N) {
+    const char *t = getenv("NV2_K_TRIDIAG"), *b = getenv("NV2_K_BAND");
+    if (t && atoi(t) > 0) return false;
+    if (b && atoi(b) > 0) return true;
+    return N >= 1536;
+}
 __global__ void k_shift_diag(double *A, int N, i64 sA, int nbm, double d) {
     const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (i64)N * nbm) return;
@@ -1508,12 +1771,42 @@ static int pc_kprime_launch(Stage2 *p, PcJob &j, double thr) {
                 hipLaunchKernelGGL(k_sym_upper, dim3(nb(NN, 256)), dim3(256), 0, st, j.Kb(b), N, j.Ks + b * NN);
             NV2_LAUNCH_CHECK();
             const double one = 1.0, zero = 0.0;
-            RB(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, N, N, N, &one, j.Ks, N,
-                                             NN, j.caux, N, NN, &zero, j.T, N, NN, j.nb));
-            RB(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, N, N, N, &one,
-                                             j.caux, N, NN, j.T, N, NN, &zero, j.Kp, N, NN, j.nb));
+            auto kprime = [&]() -> int {
+                RB(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, N, N, N, &one, j.Ks,
+                                                 N, NN, j.caux, N, NN, &zero, j.T, N, NN, j.nb));
+                RB(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, N, N, N, &one,
+                                                 j.caux, N, NN, j.T, N, NN, &zero, j.Kp, N, NN, j.nb));
+                return 0;
+            };
+            TRYS(kprime());
             for (int k = 1; k < 4; k++) tk.mark(k, st);
             tk.report(N, j.nb);
+            if (band_kprime(N)) {
+                // flags in the K' info words (4*slot + 2 + b), read here and cleared: a panel whose
+                // CholeskyQR2 failed sends both problems to the tridiagonal reduction of a fresh K'
+                int *fl = p->dinfo_w + 4 * j.slot + 2;
+                NV2_HIP(hipEventRecord(p->evKr[0], st));
+                TRYS(band_reduce(h, st, N, j.Kp, NN, j.nb, j.bandw, fl));
+                NV2_HIP(hipEventRecord(p->evKr[1], st));
+                NV2_HIP(hipMemcpyAsync(p->hinfo + 4 * j.slot + 2, fl, sizeof(int) * j.nb, hipMemcpyDeviceToHost, st));
+                NV2_HIP(hipStreamSynchronize(st));
+                bool bok = true;
+                for (int b = 0; b < j.nb; b++) {
+                    if (p->hinfo[4 * j.slot + 2 + b] != 0) bok = false;
+                    p->hinfo[4 * j.slot + 2 + b] = 0;
+                }
+                if (stage2_timing()) fprintf(stderr, "[K' %s N=%d] band reduction %s\n", j.tag.c_str(), N,
+                                             bok ? "done" : "failed (CholeskyQR2): tridiagonal reduction");
+                if (bok) {
+                    j.band = true;
+                    p->kr_recorded = true;
+                    p->kr_N = N;
+                    p->kr_nb = j.nb;
+                    p->kr_form = 2;
+                    return 0;
+                }
+                TRYS(kprime());
+            }
             return pc_trd_launch(p, j);
         }
         j.chol = false;                  // some eigenvalue below thr: the eigen path after all
@@ -2422,6 +2715,7 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         sd.kred_ms = t;
         sd.kred_N = p->kr_N;
         sd.kred_nb = p->kr_nb;
+        sd.kred_form = p->kr_form;
         p->kr_recorded = false;
     }
     const int cls_of[NJOB][2] = {{1, 2}, {3, 4}, {3, 4}, {7, 7}, {5, 6}};
@@ -2434,7 +2728,10 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
     // ================================================================ classes (-1)' and (+1)', PC sums
     // sum_mu r_mu^2 / (w_mu + s_x) with r = C^T caux^T X_x and K' = C W C^T (qdnevpt.F:5351-5422,
     // 5130-5205) is y^T (K' + s_x)^-1 y with y = caux^T X_x; with K' = Q T Q^T it is
-    // z^T (T + s_x)^-1 z, z = Q^T y: one tridiagonal solve per external orbital, no eigenvectors of K'
+    // z^T (T + s_x)^-1 z, z = Q^T y: one tridiagonal solve per external orbital, no eigenvectors of K'.
+    // On the band form the solves of both classes go out as one launch after the loop.
+    BandCls bc[2] = {};
+    int band_cls[2] = {-1, -1};
     for (int plus = 0; has4 && plus < 2; plus++) {
         const int cls = plus ? 5 : 6;
         const int next = plus ? nc : nv;
@@ -2445,11 +2742,8 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         if (!next || !nnew) continue;
         const i64 NN = jb.NN();
         const double *caux = jb.caux + bb * NN, *Kp = jb.Kp + bb * NN;
-        const TrdOut to = trd_outputs(nmax, jb.nb, jb.trdw);
-        const double *dT = to.d + (i64)bb * nmax, *eT = to.e + (i64)bb * nmax, *tauT = to.tau + (i64)bb * nmax;
         const int ncq = 1 + (q.M ? M - 1 : 0);              // right-hand sides: this state, then the QD partners
-        GET(Ypc, "pm1_Ypc", (i64)nmax * next * ncq);
-        GET(Wq, "pm1_applyq", trd_apply_workspace(nmax, 1, next * ncq));
+        GET(Ypc, plus ? "pm1_Ypc1" : "pm1_Ypc0", (i64)nmax * next * ncq);
         // y = caux^T x (on the Cholesky path caux = U^-1 and nmax = N3)
         auto project = [&](const double *x, double *y) -> int {
             return gemm(p, true, false, nmax, next, (int)N3, 1.0, caux, N3, x, N3, 0.0, y, nmax);
@@ -2468,6 +2762,22 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             TRYS(project(xj, yj));
             q.a[j] = yj;
         }
+        if (jb.band) {                   // z = Q^T y; the banded solves follow the loop
+            GET(Wq, "pm1_bandq", band_apply_workspace(next * ncq));
+            TRYS(band_apply_qt(p->h, st, nmax, jb.nb, bb, jb.bandw, next * ncq, Ypc, nmax, Wq));
+            BandCls &c = bc[plus];
+            c.Br = band_rows(nmax, jb.nb, bb, jb.bandw);
+            c.Z = Ypc;
+            c.eo = eo;
+            c.plus = plus;
+            c.next = next;
+            c.q = q;
+            band_cls[plus] = cls;
+            continue;
+        }
+        const TrdOut to = trd_outputs(nmax, jb.nb, jb.trdw);
+        const double *dT = to.d + (i64)bb * nmax, *eT = to.e + (i64)bb * nmax, *tauT = to.tau + (i64)bb * nmax;
+        GET(Wq, "pm1_applyq", trd_apply_workspace(nmax, 1, next * ncq));
         TRYS(trd_apply_q(p->h, st, nmax, Kp, NN, 1, tauT, true, next * ncq, Ypc, 0, Wq));   // z = Q^T y
         GET(rows, "pm1_rows", (i64)4 * nnew * next);
         const unsigned nbk = nb(next, 64);
@@ -2478,6 +2788,30 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         NV2_LAUNCH_CHECK();
         TRYS(reduce_into(p, nbk, cls));
         TRYS(qd_reduce(p, q, nbk, cls));
+    }
+    if (band_cls[0] >= 0 || band_cls[1] >= 0) {
+        const int nmax = J[JPM1P].nmax;
+        for (int c = 0; c < 2; c++) if (band_cls[c] < 0) bc[c].next = 0;
+        const int n0 = bc[0].next, n1 = bc[1].next;
+        GET(Ub, "pm1_bandU", (i64)(n0 + n1) * nmax * UW);
+        GET(yb, "pm1_bandy", (i64)(n0 + n1) * nmax);
+        bc[0].U = Ub;
+        bc[0].y = yb;
+        bc[1].U = Ub + (i64)n0 * nmax * UW;
+        bc[1].y = yb + (i64)n0 * nmax;
+        double *pt2 = parts(p, n0 + n1);
+        if (!pt2) { set_error("hipMalloc(parts) failed"); return -1; }
+        TRYS(qd_prepare(p, q, n0 + n1));
+        bc[0].q.qpart = bc[1].q.qpart = q.qpart;
+        bc[0].q.M = bc[1].q.M = q.M;
+        bc[0].q.self = bc[1].q.self = q.self;
+        hipLaunchKernelGGL(k_pm1p_band, dim3(n0 + n1), dim3(64), 0, st, nmax, bc[0], bc[1], pt2);
+        NV2_LAUNCH_CHECK();
+        for (int c = 0; c < 2; c++) {
+            if (!bc[c].next) continue;
+            TRYS(reduce_into(p, bc[c].next, band_cls[c], c ? n0 : 0));
+            TRYS(qd_reduce(p, q, bc[c].next, band_cls[c], c ? n0 : 0));
+        }
     }
     if (stage2_timing()) {
         NV2_HIP(hipStreamSynchronize(st));

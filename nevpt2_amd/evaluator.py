@@ -70,10 +70,11 @@ class Evaluator:
         return a.reshape(self.M, 8, 3)
 
     def pc_timings(self):
-        """Per state: (device ms of the (+-1)' K' tridiagonal reduction, its order, its batch), shape (metat, 3)."""
-        a = np.zeros(3 * self.M)
+        """Per state: (device ms of the (+-1)' K' reduction, its order, its batch, form: 1 tridiagonal,
+        2 band), shape (metat, 4)."""
+        a = np.zeros(4 * self.M)
         L.check(L.lib().nv2_ctx_get(self._h, b"PC-timings", L.ptr(a), a.size), "nv2_ctx_get(PC-timings)")
-        return a.reshape(self.M, 3)
+        return a.reshape(self.M, 4)
 
     def set_integrals(self, cholesky, oneint):
         B = L.fin(cholesky)
