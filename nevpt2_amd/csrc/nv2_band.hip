@@ -28,6 +28,8 @@ namespace nv2 {
 #define BWG 256
 #define KC 256           // rows per workgroup of the A^T B products with BAND_B x BAND_B results
 #define KS 32            // rows per LDS stage of those products
+// keeps the compiler from hoisting the loads of later unrolled iterations (and their registers) upward
+#define BAND_FENCE() __asm__ volatile("" ::: "memory")
 
 struct BandLayout {      // offsets (doubles) inside one problem's part of the workspace
     int npan;            // panels with more than BAND_B rows (CholeskyQR2)
@@ -76,76 +78,62 @@ __global__ void k_band_zero_flags(int *f, int nb) {
     if (threadIdx.x < nb) f[threadIdx.x] = 0;
 }
 
-// R^T R = G (64 x 64, ld 64) and R^-1, problem b = blockIdx.x.  flag[b] is set by a pivot that is not
-// positive and, when dev > 0, by an entry of G - I above dev: the second CholeskyQR pass then starts
-// from a Q1 too far from orthogonal for Q2 to be orthogonal to rounding.
-// G is the sum of the np partial products at Gp + b * sW (BAND_B^2 apart).
-__global__ __launch_bounds__(BWG) void k_bchol(const double *Gp, i64 sW, int np, double *R, double *Ri, int *flag,
-                                               double dev) {
-    __shared__ double a[BB][BB + 1];
-    __shared__ double x[BB][BB + 1];
-    __shared__ int bad;
-    const int b = blockIdx.x;
+// R^T R = G (BAND_B x BAND_B) and R^-1, problem b = blockIdx.x, by one wavefront: lane c keeps column
+// c in registers and row j travels through LDS at step j.  G is the sum of the np partial products at
+// Gp + b * sW (BAND_B^2 apart; symmetric, so lane c reads row c, coalesced).  R and R^-1 are stored by
+// rows (element (i, c) at i * BAND_B + c).  flag[b] is set by a pivot that is not positive and, when
+// dev > 0, by an entry of G - I above dev: the second CholeskyQR pass then starts from a Q1 too far
+// from orthogonal for Q2 to be orthogonal to rounding.
+__global__ __launch_bounds__(64) void k_bchol(const double *Gp, i64 sW, int np, double *R, double *Ri, int *flag,
+                                              double dev) {
+    __shared__ double rb[BB];
+    const int b = blockIdx.x, c = threadIdx.x;
     const double *Gb = Gp + b * sW;
-    if (threadIdx.x == 0) bad = 0;
-    __syncthreads();
-    double g[BB * BB / BWG];             // the partials summed with every load of a partial in flight
+    double a[BB];
 #pragma unroll
-    for (int u = 0; u < BB * BB / BWG; u++) g[u] = 0.0;
+    for (int r = 0; r < BB; r++) a[r] = 0.0;
     for (int q = 0; q < np; q++)
 #pragma unroll
-        for (int u = 0; u < BB * BB / BWG; u++) g[u] += Gb[(i64)q * BB * BB + threadIdx.x + BWG * u];
+        for (int r = 0; r < BB; r++) a[r] += Gb[(i64)q * BB * BB + c + BB * r];
+    bool off = false;
+    if (dev > 0.0)
 #pragma unroll
-    for (int u = 0; u < BB * BB / BWG; u++) {
-        const int t = threadIdx.x + BWG * u;
-        a[t % BB][t / BB] = g[u];
-        if (dev > 0.0 && !(fabs(g[u] - ((t % BB) == (t / BB) ? 1.0 : 0.0)) <= dev)) bad = 1;
-    }
-    __syncthreads();
-    for (int j = 0; j < BB && !bad; j++) {
-        const double d = a[j][j];
-        if (!(d > 0.0)) {
-            if (threadIdx.x == 0) bad = 1;
-            break;
-        }
-        const double u = sqrt(d), inv = 1.0 / u;
+        for (int r = 0; r < BB; r++) off |= !(fabs(a[r] - (r == c ? 1.0 : 0.0)) <= dev);
+    bool bad = __ballot(off) != 0;
+#pragma unroll
+    for (int j = 0; j < BB; j++) {          // right-looking: a[r] = G[r][c] for j <= r <= c
+        rb[c] = a[j];
         __syncthreads();
-        if (threadIdx.x == 0) a[j][j] = u;
-        for (int c = j + 1 + threadIdx.x; c < BB; c += BWG) a[j][c] *= inv;
-        __syncthreads();
-        for (int t = threadIdx.x; t < BB * BB; t += BWG) {
-            const int r = t % BB, c = t / BB;
-            if (r > j && r <= c) a[r][c] -= a[j][r] * a[j][c];
-        }
+        const double d = rb[j];
+        bad |= !(d > 0.0);                  // uniform
+        const double u = sqrt(d > 0.0 ? d : 1.0), inv = 1.0 / u;
+        const double rjc = c == j ? u : a[j] * inv;   // R[j][c]
+        a[j] = rjc;
+#pragma unroll
+        for (int r = j + 1; r < BB; r++) a[r] -= (rb[r] * inv) * rjc;
         __syncthreads();
     }
-    __syncthreads();
-    if (bad) {          // R = Ri = I keeps the rest of the batch finite; the flag sends this problem elsewhere
-        if (threadIdx.x == 0) flag[b] = 1;
-        for (int t = threadIdx.x; t < BB * BB; t += BWG) {
-            const double v = (t % BB) == (t / BB) ? 1.0 : 0.0;
-            R[b * sW + t] = v;
-            Ri[b * sW + t] = v;
-        }
-        return;
-    }
-    for (int t = threadIdx.x; t < BB * BB; t += BWG) x[t % BB][t / BB] = 0.0;
-    __syncthreads();
-    for (int i = BB - 1; i >= 0; i--) {
-        for (int c = i + threadIdx.x; c < BB; c += BWG) {
-            double s = c == i ? 1.0 : 0.0;
-            for (int k = i + 1; k <= c; k++) s -= a[i][k] * x[k][c];
-            x[i][c] = s / a[i][i];
-        }
+    double *Rb = R + b * sW, *Rib = Ri + b * sW;
+#pragma unroll
+    for (int i = 0; i < BB; i++) Rb[i * BB + c] = i <= c ? a[i] : 0.0;
+#pragma unroll
+    for (int i = BB - 1; i >= 0; i--) {     // X = R^-1 in place, rows from the bottom: a[k] = X[k][c], k > i
+        rb[c] = a[i];
+        __syncthreads();
+        double s = c == i ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = i + 1; k < BB; k++) s -= rb[k] * a[k];
+        a[i] = s / rb[i];
         __syncthreads();
     }
-    for (int t = threadIdx.x; t < BB * BB; t += BWG) {
-        const int r = t % BB, c = t / BB;
-        R[b * sW + t] = r <= c ? a[r][c] : 0.0;
-        Ri[b * sW + t] = x[r][c];
+#pragma unroll
+    for (int i = 0; i < BB; i++) Rib[i * BB + c] = a[i];
+    if (bad) {                              // R = R^-1 = I keeps the batch finite; the flag sends this problem elsewhere
+#pragma unroll
+        for (int i = 0; i < BB; i++) { Rb[i * BB + c] = i == c ? 1.0 : 0.0; Rib[i * BB + c] = i == c ? 1.0 : 0.0; }
+        if (c == 0) flag[b] = 1;
     }
 }
-
 // Partial products of A^T B for A, B with BAND_B columns (ld, m rows; problem b = blockIdx.y at A + b*sA,
 // B + b*sB): workgroup c sums rows c*KC .. c*KC+KC-1 into Gp + b*sW + c*BAND_B^2.  rocBLAS gives a
 // BAND_B x BAND_B result with thousands of rows to a single workgroup; here ceil(m / KC) of them share it.
@@ -219,76 +207,90 @@ __global__ __launch_bounds__(BWG) void k_tx(const double *Gp, int np, const doub
         for (int v = 0; v < 4; v++) Vb[(ti + 16 * u) + BB * (tj + 16 * v)] = acc[u][v];
 }
 
-// Householder form of panel k from its CholeskyQR2 factors, problem b = blockIdx.x:
-// R~ = S R2 R1 into the panel's top block of A (zeros below it), Y_top = L, T = -U S L^-T, U^-1.
-__global__ __launch_bounds__(BWG) void k_brecon(double *A, int N, i64 sA, int p0, int r0, const double *Q2,
-                                                const double *R1, const double *R2, double *Y, double *T,
-                                                double *Ui, i64 sW) {
-    __shared__ double a[BB][BB + 1];     // Q2_top - S, then L (strict lower) and U (upper)
-    __shared__ double x[BB][BB + 1];     // U^-1
-    __shared__ double y[BB][BB + 1];     // L^-1
-    __shared__ double sg[BB];
-    const int b = blockIdx.x;
+// Householder form of panel k from its CholeskyQR2 factors, problem b = blockIdx.x, by one wavefront
+// (lane c keeps column c of the matrix at hand in registers; rows and columns that other lanes need
+// travel through LDS): R~ = S R2 R1 into the panel's top block of A (zeros below it), the unpivoted LU
+// Q2_top - S = L U, Y_top = L, U^-1 (stored by rows) and T = -U S L^-T.  R1, R2 come stored by rows.
+__global__ __launch_bounds__(64) void k_brecon(double *A, int N, i64 sA, int p0, int r0, const double *Q2,
+                                               const double *R1, const double *R2, double *Y, double *T,
+                                               double *Ui, i64 sW) {
+    __shared__ double mt[BB][BB + 1];    // R2, then the LU factors
+    __shared__ double sgs[BB], rb[BB], cb[BB];
+    const int b = blockIdx.x, c = threadIdx.x;
     double *Ab = A + b * sA;
     const double *Qb = Q2 + b * sW, *R1b = R1 + b * sW, *R2b = R2 + b * sW;
-    if (threadIdx.x < BB) sg[threadIdx.x] = Qb[threadIdx.x + (i64)N * threadIdx.x] >= 0.0 ? -1.0 : 1.0;
-    for (int t = threadIdx.x; t < BB * BB; t += BWG) {   // R2 and R1 staged in x and y
-        x[t % BB][t / BB] = R2b[t];
-        y[t % BB][t / BB] = R1b[t];
-    }
+    const double sgc = Qb[c + (i64)N * c] >= 0.0 ? -1.0 : 1.0;
+    sgs[c] = sgc;
+    double m[BB];
+#pragma unroll
+    for (int r = 0; r < BB; r++) { mt[r][c] = R2b[r * BB + c]; m[r] = R1b[r * BB + c]; }   // m = column c of R1
     __syncthreads();
-    // R~ = S R2 R1 (upper triangular) into A[r0 + r, p0 + c]; the rest of the panel's columns cleared
-    for (int t = threadIdx.x; t < BB * BB; t += BWG) {
-        const int r = t % BB, c = t / BB;
+    // R~[r][c] = S_r sum_{r <= k <= c} R2[r][k] R1[k][c] (R1[k][c] = 0 for k > c)
+#pragma unroll
+    for (int r = 0; r < BB; r++) {
         double v = 0.0;
-        for (int k = r; k <= c; k++) v += x[r][k] * y[k][c];
-        Ab[(r0 + r) + (i64)N * (p0 + c)] = r <= c ? sg[r] * v : 0.0;
+#pragma unroll
+        for (int k = r; k < BB; k++) v += mt[r][k] * m[k];
+        Ab[(r0 + r) + (i64)N * (p0 + c)] = r <= c ? sgs[r] * v : 0.0;
+        BAND_FENCE();
     }
     const int mb = N - r0 - BB;
-    for (int c = 0; c < BB; c++)
-        for (int r = threadIdx.x; r < mb; r += BWG) Ab[(r0 + BB + r) + (i64)N * (p0 + c)] = 0.0;
-    for (int t = threadIdx.x; t < BB * BB; t += BWG) {
-        const int r = t % BB, c = t / BB;
-        a[r][c] = Qb[r + (i64)N * c] - (r == c ? sg[r] : 0.0);
-    }
-    __syncthreads();
-    for (int k = 0; k < BB; k++) {       // LU without pivoting: |a[k][k]| >= 1 by the choice of S
-        const double piv = a[k][k];
-        for (int i = k + 1 + threadIdx.x; i < BB; i += BWG) a[i][k] /= piv;
-        __syncthreads();
-        for (int t = threadIdx.x; t < BB * BB; t += BWG) {
-            const int i = t % BB, c = t / BB;
-            if (i > k && c > k) a[i][c] -= a[i][k] * a[k][c];
+    for (int cc = 0; cc < BB; cc++)
+        for (int r = c; r < mb; r += 64) Ab[(r0 + BB + r) + (i64)N * (p0 + cc)] = 0.0;
+    // LU without pivoting of Q2_top - S (|pivot| >= 1 by the choice of S); m = column c
+#pragma unroll
+    for (int r = 0; r < BB; r++) m[r] = Qb[r + (i64)N * c] - (r == c ? sgc : 0.0);
+#pragma unroll
+    for (int k = 0; k < BB; k++) {
+        rb[c] = m[k];                    // row k: U[k][c] for c >= k
+        if (c == k) {
+#pragma unroll
+            for (int i = k + 1; i < BB; i++) cb[i] = m[i];   // column k below the pivot
         }
         __syncthreads();
-    }
-    __syncthreads();                     // R~ is written: x and y are free
-    for (int t = threadIdx.x; t < BB * BB; t += BWG) { x[t % BB][t / BB] = 0.0; y[t % BB][t / BB] = 0.0; }
-    __syncthreads();
-    for (int i = BB - 1; i >= 0; i--) {  // x = U^-1, by rows from the bottom
-        for (int c = i + threadIdx.x; c < BB; c += BWG) {
-            double s = c == i ? 1.0 : 0.0;
-            for (int k = i + 1; k <= c; k++) s -= a[i][k] * x[k][c];
-            x[i][c] = s / a[i][i];
-        }
-        __syncthreads();
-    }
-    for (int i = 0; i < BB; i++) {       // y = L^-1 (unit lower), by rows from the top
-        for (int c = threadIdx.x; c <= i; c += BWG) {
-            double s = c == i ? 1.0 : 0.0;
-            for (int k = c; k < i; k++) s -= a[i][k] * y[k][c];
-            y[i][c] = s;
+        const double rp = 1.0 / rb[k], ukc = m[k];
+#pragma unroll
+        for (int i = k + 1; i < BB; i++) {
+            const double l = cb[i] * rp;
+            m[i] = c > k ? m[i] - l * ukc : (c == k ? l : m[i]);
         }
         __syncthreads();
     }
     double *Yb = Y + b * sW, *Tb = T + b * sW, *Uib = Ui + b * sW;
-    for (int t = threadIdx.x; t < BB * BB; t += BWG) {
-        const int r = t % BB, c = t / BB;
-        Yb[r + (i64)N * c] = r > c ? a[r][c] : (r == c ? 1.0 : 0.0);
-        Uib[r + BB * c] = x[r][c];
-        double v = 0.0;                  // T = -U S L^-T: T[r][c] = -sum_{r <= k <= c} U[r][k] S_k L^-1[c][k]
-        for (int k = r; k <= c; k++) v -= a[r][k] * sg[k] * y[c][k];
+#pragma unroll
+    for (int r = 0; r < BB; r++) {
+        mt[r][c] = m[r];
+        Yb[r + (i64)N * c] = r > c ? m[r] : (r == c ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    // U^-1 by rows from the bottom, in m: m[k] = X[k][c] for k > i (zero for k > c)
+#pragma unroll
+    for (int i = BB - 1; i >= 0; i--) {
+        double v = c == i ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = i + 1; k < BB; k++) v -= mt[i][k] * m[k];
+        m[i] = v / mt[i][i];
+        BAND_FENCE();
+    }
+#pragma unroll
+    for (int i = 0; i < BB; i++) Uib[i * BB + c] = m[i];
+    // row c of L^-1: L^T w = e_c, from the bottom (w[k] = 0 for k > c)
+#pragma unroll
+    for (int k = BB - 1; k >= 0; k--) {
+        double v = k == c ? 1.0 : 0.0;
+#pragma unroll
+        for (int t = k + 1; t < BB; t++) v -= mt[t][k] * m[t];
+        m[k] = v;
+        BAND_FENCE();
+    }
+    // T[r][c] = -sum_{r <= k <= c} U[r][k] S_k L^-1[c][k]
+#pragma unroll
+    for (int r = 0; r < BB; r++) {
+        double v = 0.0;
+#pragma unroll
+        for (int k = r; k < BB; k++) v -= mt[r][k] * sgs[k] * m[k];
         Tb[r + BB * c] = r <= c ? v : 0.0;
+        BAND_FENCE();
     }
 }
 
@@ -412,15 +414,15 @@ int band_reduce(rocblas_handle h, hipStream_t st, int N, double *A, i64 sA, int 
         const int np = (m + KC - 1) / KC;
         // CholeskyQR2 of P
         hipLaunchKernelGGL(k_atb_parts, dim3(np, nb), dim3(BWG), 0, st, P, sA, P, sA, N, m, Gp, sW);
-        hipLaunchKernelGGL(k_bchol, dim3(nb), dim3(BWG), 0, st, Gp, sW, np, R1, R1i, dflag, 0.0);
-        RBB(rocblas_dgemm_strided_batched(h, Nn, Nn, m, BB, BB, &one, P, N, sA, R1i, BB, sW, &zero, Q1, N, sW, nb));
+        hipLaunchKernelGGL(k_bchol, dim3(nb), dim3(64), 0, st, Gp, sW, np, R1, R1i, dflag, 0.0);
+        RBB(rocblas_dgemm_strided_batched(h, Nn, Tt, m, BB, BB, &one, P, N, sA, R1i, BB, sW, &zero, Q1, N, sW, nb));
         hipLaunchKernelGGL(k_atb_parts, dim3(np, nb), dim3(BWG), 0, st, Q1, sW, Q1, sW, N, m, Gp, sW);
-        hipLaunchKernelGGL(k_bchol, dim3(nb), dim3(BWG), 0, st, Gp, sW, np, R2, R2i, dflag, 0.1);
-        RBB(rocblas_dgemm_strided_batched(h, Nn, Nn, m, BB, BB, &one, Q1, N, sW, R2i, BB, sW, &zero, Q2, N, sW, nb));
+        hipLaunchKernelGGL(k_bchol, dim3(nb), dim3(64), 0, st, Gp, sW, np, R2, R2i, dflag, 0.1);
+        RBB(rocblas_dgemm_strided_batched(h, Nn, Tt, m, BB, BB, &one, Q1, N, sW, R2i, BB, sW, &zero, Q2, N, sW, nb));
         // reflectors: Y_top, T, U^-1, and S R into the panel
-        hipLaunchKernelGGL(k_brecon, dim3(nb), dim3(BWG), 0, st, A, N, sA, p0, r0, Q2, R1, R2, Yk, Tk, Ui, sW);
+        hipLaunchKernelGGL(k_brecon, dim3(nb), dim3(64), 0, st, A, N, sA, p0, r0, Q2, R1, R2, Yk, Tk, Ui, sW);
         NV2_LAUNCH_CHECK();
-        RBB(rocblas_dgemm_strided_batched(h, Nn, Nn, m - BB, BB, BB, &one, Q2 + BB, N, sW, Ui, BB, sW, &zero, Yk + BB,
+        RBB(rocblas_dgemm_strided_batched(h, Nn, Tt, m - BB, BB, BB, &one, Q2 + BB, N, sW, Ui, BB, sW, &zero, Yk + BB,
                                           N, sW, nb));
         // A22 <- H^T A22 H
         RBB(rocblas_dgemm_strided_batched(h, Nn, Nn, m, BB, m, &one, A22, N, sA, Yk, N, sW, &zero, W, N, sW, nb));
