@@ -1321,6 +1321,7 @@ struct Stage2 {
     S2Inputs in;
     Pm1pEarly *early = nullptr;    // the (+-1)' chain when started ahead of stage2_state
     hipEvent_t evS = nullptr, evK = nullptr;   // its metric filled / its Koopmans matrix filled
+    hipEvent_t evX = nullptr;                  // its PC right-hand sides built (band-form sums on worker 1)
     hipEvent_t evKr[2] = {nullptr, nullptr};   // around the K' reduction of the (+-1)' problems (timed)
     bool kr_recorded = false;
     int kr_N = 0, kr_nb = 0, kr_form = 0;
@@ -1884,6 +1885,12 @@ struct Pm1pEarly {
     std::condition_variable cv;
     bool k_ready = false, abort = false;
     bool k_filled = false;         // K filled and signalled ahead of stage2_state (stage2_pm1p_kready)
+    // Single-state runs on the band form: the PC sums go out on the chain's stream as soon as the
+    // right-hand sides exist (x_ready, event evX), alongside the other classes on the main stream.
+    bool sums_early = false, x_ready = false, sums_done = false;
+    const double *X[2] = {nullptr, nullptr}, *eo[2] = {nullptr, nullptr};
+    int next[2] = {0, 0};
+    double *Ypc[2] = {nullptr, nullptr}, *Wq = nullptr, *Ub = nullptr, *yb = nullptr, *ptb = nullptr;
     int rc = 0;
     char err[512] = "";
     PcJob job;
@@ -1904,6 +1911,43 @@ static void pm1p_stop(Stage2 *p) {   // abort a chain still waiting for K, and j
     if (E->th.joinable()) E->th.join();
     E->active = false;
 }
+// (-1)' and (+1)' PC sums on the band form, on the chain's stream: y = caux^T x, z = Q^T y, one banded
+// solve per external orbital (both classes in one launch), block partials reduced into the classes
+static int pm1p_band_sums(Stage2 *p, Pm1pEarly *E) {
+    const PcJob &jb = E->job;
+    hipStream_t st = p->ws[jb.w];
+    rocblas_handle h = p->wh[jb.w];
+    const int N = jb.nmax;
+    const i64 NN = jb.NN();
+    BandCls bc[2] = {};
+    for (int plus = 0; plus < 2; plus++) {
+        const int next = E->next[plus], bb = plus ? 0 : 1;
+        if (!next || !E->X[plus]) continue;
+        TRYS(gemm_on(h, st, true, false, N, next, N, 1.0, jb.caux + bb * NN, N, E->X[plus], N, 0.0, E->Ypc[plus], N));
+        TRYS(band_apply_qt(h, st, N, jb.nb, bb, jb.bandw, next, E->Ypc[plus], N, E->Wq));
+        BandCls &c = bc[plus];
+        c.Br = band_rows(N, jb.nb, bb, jb.bandw);
+        c.Z = E->Ypc[plus];
+        c.eo = E->eo[plus];
+        c.plus = plus;
+        c.next = next;
+    }
+    const int n0 = bc[0].next, n1 = bc[1].next;
+    if (!n0 && !n1) return 0;
+    bc[0].U = E->Ub;
+    bc[0].y = E->yb;
+    bc[1].U = E->Ub + (i64)n0 * N * UW;
+    bc[1].y = E->yb + (i64)n0 * N;
+    hipLaunchKernelGGL(k_pm1p_band, dim3(n0 + n1), dim3(64), 0, st, N, bc[0], bc[1], E->ptb);
+    NV2_LAUNCH_CHECK();
+    for (int c = 0; c < 2; c++) {
+        if (!bc[c].next) continue;
+        hipLaunchKernelGGL(k_reduce_parts, dim3(4), dim3(256), 0, st, E->ptb + 4 * (c ? n0 : 0), (i64)bc[c].next,
+                           p->acc + 4 * (c ? 5 : 6));
+        NV2_LAUNCH_CHECK();
+    }
+    return 0;
+}
 static int pm1p_worker(Stage2 *p, Pm1pEarly *E) {
     PcJob &j = E->job;
     E->t_start = std::chrono::steady_clock::now();
@@ -1921,6 +1965,17 @@ static int pm1p_worker(Stage2 *p, Pm1pEarly *E) {
     TRYS(pc_kprime_launch(p, j, E->thr));
     NV2_HIP(hipStreamSynchronize(p->ws[j.w]));
     E->t_end = std::chrono::steady_clock::now();
+    if (E->sums_early && j.band) {
+        {
+            std::unique_lock<std::mutex> lk(E->mu);
+            E->cv.wait(lk, [E] { return E->x_ready || E->abort; });
+            if (E->abort) return 0;
+        }
+        NV2_HIP(hipStreamWaitEvent(p->ws[j.w], p->evX, 0));
+        TRYS(pm1p_band_sums(p, E));
+        E->sums_done = true;
+        NV2_HIP(hipStreamSynchronize(p->ws[j.w]));
+    }
     return 0;
 }
 
@@ -1954,6 +2009,16 @@ int stage2_pm1p_begin(Stage2 *p, int s, double thr) {
         NV2_LAUNCH_CHECK();
         NV2_HIP(hipMemcpyAsync(mx, j.Sb(bb), sizeof(double) * N3 * N3, hipMemcpyDeviceToDevice, st));
     }
+    E->sums_early = p->M == 1 && band_kprime((int)N3) && chol_metric() && (p->nc + p->nv) > 0;
+    if (E->sums_early) {              // the sums' buffers, allocated here on the calling thread
+        const i64 nx = (i64)p->nc + p->nv, nmx = p->nc > p->nv ? p->nc : p->nv;
+        GET(y0, "pm1_Ypc0", N3 * (p->nv ? p->nv : 1)); GET(y1, "pm1_Ypc1", N3 * (p->nc ? p->nc : 1));
+        GET(wq, "pm1_bandq", band_apply_workspace((int)nmx));
+        GET(ub, "pm1_bandU", nx * N3 * UW); GET(yb, "pm1_bandy", nx * N3); GET(pb, "pm1_bandparts", 4 * nx);
+        E->Ypc[0] = y0; E->Ypc[1] = y1; E->Wq = wq; E->Ub = ub; E->yb = yb; E->ptb = pb;
+    }
+    E->x_ready = false;
+    E->sums_done = false;
     NV2_HIP(hipEventRecord(p->evS, st));
     E->state = s;
     E->thr = thr;
@@ -2022,6 +2087,7 @@ Stage2 *stage2_create(const S2Inputs &in, rocblas_handle h, hipStream_t st) {
         if (hipEventCreateWithFlags(&p->wev[e], hipEventDisableTiming) != hipSuccess) { delete p; return nullptr; }
     if (hipEventCreateWithFlags(&p->evS, hipEventDisableTiming) != hipSuccess) { delete p; return nullptr; }
     if (hipEventCreateWithFlags(&p->evK, hipEventDisableTiming) != hipSuccess) { delete p; return nullptr; }
+    if (hipEventCreateWithFlags(&p->evX, hipEventDisableTiming) != hipSuccess) { delete p; return nullptr; }
     for (auto &e : p->evKr)
         if (hipEventCreate(&e) != hipSuccess) { delete p; return nullptr; }
     if (hipStreamCreateWithFlags(&p->xs, hipStreamNonBlocking) != hipSuccess) { delete p; return nullptr; }
@@ -2527,6 +2593,20 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         TRYS(reduce_into(p, next, cls));
         TRYS(qd_reduce(p, q, next, cls));
     }
+    if (has4 && p->early && p->early->sums_early) {   // release the chain's band-form sums
+        Pm1pEarly *E = p->early;
+        for (int plus = 0; plus < 2; plus++) {
+            E->X[plus] = Xpm[plus];
+            E->next[plus] = plus ? nc : nv;
+            E->eo[plus] = plus ? ec : ev;
+        }
+        NV2_HIP(hipEventRecord(p->evX, st));
+        {
+            std::lock_guard<std::mutex> lk(E->mu);
+            E->x_ready = true;
+        }
+        E->cv.notify_all();
+    }
 
     // ================================================================ join worker 0: every PC class but (+-1)'
     // (whose n^3 chain on worker 1 is the longest; its sums come last)
@@ -2732,7 +2812,8 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
     // On the band form the solves of both classes go out as one launch after the loop.
     BandCls bc[2] = {};
     int band_cls[2] = {-1, -1};
-    for (int plus = 0; has4 && plus < 2; plus++) {
+    const bool sums_done = has4 && p->early && p->early->sums_done;   // summed on the chain's stream
+    for (int plus = 0; has4 && !sums_done && plus < 2; plus++) {
         const int cls = plus ? 5 : 6;
         const int next = plus ? nc : nv;
         const double *eo = plus ? ec : ev;
