@@ -26,7 +26,7 @@ namespace nv2 {
 
 #define BB BAND_B
 #define BWG 256
-#define KC 256           // rows per workgroup of the A^T B products with BAND_B x BAND_B results
+#define KC 128           // rows per workgroup of the A^T B products with BAND_B x BAND_B results
 #define KS 32            // rows per LDS stage of those products
 // keeps the compiler from hoisting the loads of later unrolled iterations (and their registers) upward
 #define BAND_FENCE() __asm__ volatile("" ::: "memory")
@@ -207,90 +207,102 @@ __global__ __launch_bounds__(BWG) void k_tx(const double *Gp, int np, const doub
         for (int v = 0; v < 4; v++) Vb[(ti + 16 * u) + BB * (tj + 16 * v)] = acc[u][v];
 }
 
-// Householder form of panel k from its CholeskyQR2 factors, problem b = blockIdx.x, by one wavefront
-// (lane c keeps column c of the matrix at hand in registers; rows and columns that other lanes need
-// travel through LDS): R~ = S R2 R1 into the panel's top block of A (zeros below it), the unpivoted LU
-// Q2_top - S = L U, Y_top = L, U^-1 (stored by rows) and T = -U S L^-T.  R1, R2 come stored by rows.
-__global__ __launch_bounds__(64) void k_brecon(double *A, int N, i64 sA, int p0, int r0, const double *Q2,
-                                               const double *R1, const double *R2, double *Y, double *T,
-                                               double *Ui, i64 sW) {
-    __shared__ double mt[BB][BB + 1];    // R2, then the LU factors
+// Householder form of panel k from its CholeskyQR2 factors, problem b = blockIdx.x, by two wavefronts
+// (lane c of a wavefront keeps column c of the matrix at hand in registers; rows and columns that
+// other lanes need travel through LDS).  Wavefront 0 factors Q2_top - S = L U without pivoting and
+// inverts U (stored by rows); wavefront 1 meanwhile forms R~ = S R2 R1 into the panel's top block of A
+// (zeros below it), then, once L and U are there, Y_top = L and T = -U S L^-T.  R1, R2 come stored by
+// rows.  The phases of one wavefront order their LDS traffic by program order (one wavefront, in-order
+// LDS); the two meet at one workgroup barrier.
+#define WAVE_LDS_SYNC() __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory")
+__global__ __launch_bounds__(128) void k_brecon(double *A, int N, i64 sA, int p0, int r0, const double *Q2,
+                                                const double *R1, const double *R2, double *Y, double *T,
+                                                double *Ui, i64 sW) {
+    __shared__ double mt[BB][BB + 1];    // the LU factors
+    __shared__ double r2[BB][BB + 1];    // R2
     __shared__ double sgs[BB], rb[BB], cb[BB];
-    const int b = blockIdx.x, c = threadIdx.x;
+    const int b = blockIdx.x, c = threadIdx.x & 63, wv = threadIdx.x >> 6;
     double *Ab = A + b * sA;
     const double *Qb = Q2 + b * sW, *R1b = R1 + b * sW, *R2b = R2 + b * sW;
     const double sgc = Qb[c + (i64)N * c] >= 0.0 ? -1.0 : 1.0;
-    sgs[c] = sgc;
     double m[BB];
+    if (wv == 0) {
+        sgs[c] = sgc;
+        // LU without pivoting of Q2_top - S (|pivot| >= 1 by the choice of S); m = column c
 #pragma unroll
-    for (int r = 0; r < BB; r++) { mt[r][c] = R2b[r * BB + c]; m[r] = R1b[r * BB + c]; }   // m = column c of R1
-    __syncthreads();
-    // R~[r][c] = S_r sum_{r <= k <= c} R2[r][k] R1[k][c] (R1[k][c] = 0 for k > c)
+        for (int r = 0; r < BB; r++) m[r] = Qb[r + (i64)N * c] - (r == c ? sgc : 0.0);
 #pragma unroll
-    for (int r = 0; r < BB; r++) {
-        double v = 0.0;
+        for (int k = 0; k < BB; k++) {
+            rb[c] = m[k];                // row k: U[k][c] for c >= k
+            if (c == k) {
 #pragma unroll
-        for (int k = r; k < BB; k++) v += mt[r][k] * m[k];
-        Ab[(r0 + r) + (i64)N * (p0 + c)] = r <= c ? sgs[r] * v : 0.0;
-        BAND_FENCE();
-    }
-    const int mb = N - r0 - BB;
-    for (int cc = 0; cc < BB; cc++)
-        for (int r = c; r < mb; r += 64) Ab[(r0 + BB + r) + (i64)N * (p0 + cc)] = 0.0;
-    // LU without pivoting of Q2_top - S (|pivot| >= 1 by the choice of S); m = column c
+                for (int i = k + 1; i < BB; i++) cb[i] = m[i];   // column k below the pivot
+            }
+            WAVE_LDS_SYNC();
+            const double rp = 1.0 / rb[k], ukc = m[k];
 #pragma unroll
-    for (int r = 0; r < BB; r++) m[r] = Qb[r + (i64)N * c] - (r == c ? sgc : 0.0);
-#pragma unroll
-    for (int k = 0; k < BB; k++) {
-        rb[c] = m[k];                    // row k: U[k][c] for c >= k
-        if (c == k) {
-#pragma unroll
-            for (int i = k + 1; i < BB; i++) cb[i] = m[i];   // column k below the pivot
+            for (int i = k + 1; i < BB; i++) {
+                const double l = cb[i] * rp;
+                m[i] = c > k ? m[i] - l * ukc : (c == k ? l : m[i]);
+            }
+            WAVE_LDS_SYNC();
         }
-        __syncthreads();
-        const double rp = 1.0 / rb[k], ukc = m[k];
 #pragma unroll
-        for (int i = k + 1; i < BB; i++) {
-            const double l = cb[i] * rp;
-            m[i] = c > k ? m[i] - l * ukc : (c == k ? l : m[i]);
+        for (int r = 0; r < BB; r++) mt[r][c] = m[r];
+    } else {
+#pragma unroll
+        for (int r = 0; r < BB; r++) { r2[r][c] = R2b[r * BB + c]; m[r] = R1b[r * BB + c]; }   // m = column c of R1
+        WAVE_LDS_SYNC();
+        // R~[r][c] = S_r sum_{r <= k <= c} R2[r][k] R1[k][c] (R1[k][c] = 0 for k > c)
+        const double *Qd = Qb;
+#pragma unroll
+        for (int r = 0; r < BB; r++) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = r; k < BB; k++) v += r2[r][k] * m[k];
+            const double sr = Qd[r + (i64)N * r] >= 0.0 ? -1.0 : 1.0;
+            Ab[(r0 + r) + (i64)N * (p0 + c)] = r <= c ? sr * v : 0.0;
+            BAND_FENCE();
         }
-        __syncthreads();
+        const int mb = N - r0 - BB;
+        for (int cc = 0; cc < BB; cc++)
+            for (int r = c; r < mb; r += 64) Ab[(r0 + BB + r) + (i64)N * (p0 + cc)] = 0.0;
     }
+    __syncthreads();                     // L, U and S in LDS
     double *Yb = Y + b * sW, *Tb = T + b * sW, *Uib = Ui + b * sW;
+    if (wv == 0) {
+        // U^-1 by rows from the bottom, in m: m[k] = X[k][c] for k > i (zero for k > c)
 #pragma unroll
-    for (int r = 0; r < BB; r++) {
-        mt[r][c] = m[r];
-        Yb[r + (i64)N * c] = r > c ? m[r] : (r == c ? 1.0 : 0.0);
-    }
-    __syncthreads();
-    // U^-1 by rows from the bottom, in m: m[k] = X[k][c] for k > i (zero for k > c)
+        for (int i = BB - 1; i >= 0; i--) {
+            double v = c == i ? 1.0 : 0.0;
 #pragma unroll
-    for (int i = BB - 1; i >= 0; i--) {
-        double v = c == i ? 1.0 : 0.0;
+            for (int k = i + 1; k < BB; k++) v -= mt[i][k] * m[k];
+            m[i] = v / mt[i][i];
+            BAND_FENCE();
+        }
 #pragma unroll
-        for (int k = i + 1; k < BB; k++) v -= mt[i][k] * m[k];
-        m[i] = v / mt[i][i];
-        BAND_FENCE();
-    }
+        for (int i = 0; i < BB; i++) Uib[i * BB + c] = m[i];
+    } else {
 #pragma unroll
-    for (int i = 0; i < BB; i++) Uib[i * BB + c] = m[i];
-    // row c of L^-1: L^T w = e_c, from the bottom (w[k] = 0 for k > c)
+        for (int r = 0; r < BB; r++) Yb[r + (i64)N * c] = r > c ? mt[r][c] : (r == c ? 1.0 : 0.0);
+        // row c of L^-1: L^T w = e_c, from the bottom (w[k] = 0 for k > c)
 #pragma unroll
-    for (int k = BB - 1; k >= 0; k--) {
-        double v = k == c ? 1.0 : 0.0;
+        for (int k = BB - 1; k >= 0; k--) {
+            double v = k == c ? 1.0 : 0.0;
 #pragma unroll
-        for (int t = k + 1; t < BB; t++) v -= mt[t][k] * m[t];
-        m[k] = v;
-        BAND_FENCE();
-    }
-    // T[r][c] = -sum_{r <= k <= c} U[r][k] S_k L^-1[c][k]
+            for (int t = k + 1; t < BB; t++) v -= mt[t][k] * m[t];
+            m[k] = v;
+            BAND_FENCE();
+        }
+        // T[r][c] = -sum_{r <= k <= c} U[r][k] S_k L^-1[c][k]
 #pragma unroll
-    for (int r = 0; r < BB; r++) {
-        double v = 0.0;
+        for (int r = 0; r < BB; r++) {
+            double v = 0.0;
 #pragma unroll
-        for (int k = r; k < BB; k++) v -= mt[r][k] * sgs[k] * m[k];
-        Tb[r + BB * c] = r <= c ? v : 0.0;
-        BAND_FENCE();
+            for (int k = r; k < BB; k++) v -= mt[r][k] * sgs[k] * m[k];
+            Tb[r + BB * c] = r <= c ? v : 0.0;
+            BAND_FENCE();
+        }
     }
 }
 
@@ -420,7 +432,7 @@ int band_reduce(rocblas_handle h, hipStream_t st, int N, double *A, i64 sA, int 
         hipLaunchKernelGGL(k_bchol, dim3(nb), dim3(64), 0, st, Gp, sW, np, R2, R2i, dflag, 0.1);
         RBB(rocblas_dgemm_strided_batched(h, Nn, Tt, m, BB, BB, &one, Q1, N, sW, R2i, BB, sW, &zero, Q2, N, sW, nb));
         // reflectors: Y_top, T, U^-1, and S R into the panel
-        hipLaunchKernelGGL(k_brecon, dim3(nb), dim3(64), 0, st, A, N, sA, p0, r0, Q2, R1, R2, Yk, Tk, Ui, sW);
+        hipLaunchKernelGGL(k_brecon, dim3(nb), dim3(128), 0, st, A, N, sA, p0, r0, Q2, R1, R2, Yk, Tk, Ui, sW);
         NV2_LAUNCH_CHECK();
         RBB(rocblas_dgemm_strided_batched(h, Nn, Tt, m - BB, BB, BB, &one, Q2 + BB, N, sW, Ui, BB, sW, &zero, Yk + BB,
                                           N, sW, nb));
