@@ -1062,247 +1062,346 @@ __global__ void k_pm1p_shift(int plus, int nnew, int next, const double *d, cons
     if (q.M) block_storeQ(qsc, qpc, q);
 }
 
-// The same sums on the band form K' = Q B Q^T (nv2_band.hip): one workgroup (one wavefront) per
-// external orbital x solves (B + s_x) u = z_x by Gaussian elimination with partial pivoting.  The
-// active part of step j is the window of rows j..j+BAND_B and columns j..j+2*BAND_B (a pivot row
-// from below the diagonal carries the fill up to 2*BAND_B columns right of it).  The window lives in
-// LDS: its rows by a slot table (perm: lane i holds the slot of row j+i, the last row in a uniform
-// register), its columns circularly (column c at slot c % BAND_W), so a step moves no data: the
-// eliminated row leaves its slot to row j+BAND_B+1, the eliminated column its slot to column
-// j+2*BAND_B+1 (zero in every row present, untouched by earlier steps).  U is stored by rows of
-// BAND_W + 1: the reciprocal of the pivot, the relative columns 1..2*BAND_B, the eliminated
-// right-hand side.  Global data moves in blocks of BSTEP rows (band rows entering the window in the
-// forward pass, rows of U in the backward pass), loaded one block ahead into registers and staged
-// through LDS, so that no step waits for memory.
+// The same sums on the band form K' = Q B Q^T (nv2_band.hip): one workgroup per external orbital x
+// solves (B + s_x) u = z_x by Gaussian elimination with partial pivoting (k_pm1p_bandr below).  The
+// active part of step j is the window of rows j..j+BAND_B and columns j..j+2*BAND_B (a pivot row from
+// below the diagonal carries the fill up to 2*BAND_B columns right of it).  Global data moves in
+// blocks of BSTEP rows (band rows entering the window, rows of U), loaded one block ahead into
+// registers and staged through LDS, so that no step waits for memory.
 // E -= z.u, norm += u.u, and for QD the coupling -z_j.u with z_j = q.a[j] (same layout as Z).
 #define PBR (BAND_B + 1)
 #define UW (BAND_W + 1)
 #define BSTEP 8
-__device__ __forceinline__ double rlane(double v, int i) {   // lane i's value; i uniform
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), i);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), i);
-    return __hiloint2double(hi, lo);
-}
-// barrier of a one-wavefront workgroup that orders LDS only: global loads stay in flight across it
-__device__ __forceinline__ void lds_barrier() { __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+// LDS ordering inside one wavefront of a larger workgroup (in-order LDS; no workgroup barrier)
+#define WAVE_SYNC() __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory")
 // one class of the launch: band rows, right-hand sides z_x at Z + N*x, orbital energies, U rows and
-// solutions (N * UW and N per orbital), QD partners (q.a[j]: same layout as Z)
+// solutions (N * UR and N per orbital), QD partners (q.a[j]: same layout as Z)
 struct BandCls {
     const double *Br, *Z, *eo;
     double *U, *y;
     int plus, next;
     QDArgs q;
 };
-// blocks 0 .. c0.next-1 solve for the orbitals of class c0, the following c1.next those of class c1
-__global__ __launch_bounds__(64) void k_pm1p_band(int N, BandCls c0, BandCls c1, double *part) {
-    __shared__ double win[PBR][BAND_W];
-    __shared__ double stg[BSTEP][UW];              // staged block: band rows + z, then rows of U
+// Blocks 0 .. c0.next-1 solve for the orbitals of class c0, the following c1.next those of class c1.
+// The window lives in registers: thread g of three wavefronts owns column slot g
+// (g = 0..2*BAND_B) or, for g = BAND_W, the right-hand side, and keeps that column for the BAND_B+1
+// row slots of the window in registers.  Rows have no order in the window: partial pivoting picks the
+// largest |entry| of column j over all rows present (rows j..j+BAND_B), the pivot row's slot takes row
+// j+BAND_B+1 after the step, and column j's slot (its owner zeroes it) becomes column j+2*BAND_B+1.  A
+// step moves three small vectors through LDS (column j, the multipliers, the pivot) instead of the
+// whole window.  U rows are stored in slot order: value of column c at slot c % BAND_W (the pivot
+// column's own slot 0), then the eliminated right-hand side and the reciprocal pivot; the back
+// substitution keeps u in a ring by slot, so it reads U rows as they are.
+#define RW_T (3 * 64)
+#define UR (BAND_W + 3)
+__device__ __forceinline__ double rw_get(const double (&w)[PBR], int s) {
+    switch (s) {
+        case 0: return w[0];
+        case 1: return w[1];
+        case 2: return w[2];
+        case 3: return w[3];
+        case 4: return w[4];
+        case 5: return w[5];
+        case 6: return w[6];
+        case 7: return w[7];
+        case 8: return w[8];
+        case 9: return w[9];
+        case 10: return w[10];
+        case 11: return w[11];
+        case 12: return w[12];
+        case 13: return w[13];
+        case 14: return w[14];
+        case 15: return w[15];
+        case 16: return w[16];
+        case 17: return w[17];
+        case 18: return w[18];
+        case 19: return w[19];
+        case 20: return w[20];
+        case 21: return w[21];
+        case 22: return w[22];
+        case 23: return w[23];
+        case 24: return w[24];
+        case 25: return w[25];
+        case 26: return w[26];
+        case 27: return w[27];
+        case 28: return w[28];
+        case 29: return w[29];
+        case 30: return w[30];
+        case 31: return w[31];
+        case 32: return w[32];
+        case 33: return w[33];
+        case 34: return w[34];
+        case 35: return w[35];
+        case 36: return w[36];
+        case 37: return w[37];
+        case 38: return w[38];
+        case 39: return w[39];
+        case 40: return w[40];
+        case 41: return w[41];
+        case 42: return w[42];
+        case 43: return w[43];
+        case 44: return w[44];
+        case 45: return w[45];
+        case 46: return w[46];
+        case 47: return w[47];
+        case 48: return w[48];
+        case 49: return w[49];
+        case 50: return w[50];
+        case 51: return w[51];
+        case 52: return w[52];
+        case 53: return w[53];
+        case 54: return w[54];
+        case 55: return w[55];
+        case 56: return w[56];
+        case 57: return w[57];
+        case 58: return w[58];
+        case 59: return w[59];
+        case 60: return w[60];
+        case 61: return w[61];
+        case 62: return w[62];
+        case 63: return w[63];
+        case 64: return w[64];
+    }
+    return 0.0;
+}
+__device__ __forceinline__ void rw_set(double (&w)[PBR], int s, double v) {
+    switch (s) {
+        case 0: w[0] = v; break;
+        case 1: w[1] = v; break;
+        case 2: w[2] = v; break;
+        case 3: w[3] = v; break;
+        case 4: w[4] = v; break;
+        case 5: w[5] = v; break;
+        case 6: w[6] = v; break;
+        case 7: w[7] = v; break;
+        case 8: w[8] = v; break;
+        case 9: w[9] = v; break;
+        case 10: w[10] = v; break;
+        case 11: w[11] = v; break;
+        case 12: w[12] = v; break;
+        case 13: w[13] = v; break;
+        case 14: w[14] = v; break;
+        case 15: w[15] = v; break;
+        case 16: w[16] = v; break;
+        case 17: w[17] = v; break;
+        case 18: w[18] = v; break;
+        case 19: w[19] = v; break;
+        case 20: w[20] = v; break;
+        case 21: w[21] = v; break;
+        case 22: w[22] = v; break;
+        case 23: w[23] = v; break;
+        case 24: w[24] = v; break;
+        case 25: w[25] = v; break;
+        case 26: w[26] = v; break;
+        case 27: w[27] = v; break;
+        case 28: w[28] = v; break;
+        case 29: w[29] = v; break;
+        case 30: w[30] = v; break;
+        case 31: w[31] = v; break;
+        case 32: w[32] = v; break;
+        case 33: w[33] = v; break;
+        case 34: w[34] = v; break;
+        case 35: w[35] = v; break;
+        case 36: w[36] = v; break;
+        case 37: w[37] = v; break;
+        case 38: w[38] = v; break;
+        case 39: w[39] = v; break;
+        case 40: w[40] = v; break;
+        case 41: w[41] = v; break;
+        case 42: w[42] = v; break;
+        case 43: w[43] = v; break;
+        case 44: w[44] = v; break;
+        case 45: w[45] = v; break;
+        case 46: w[46] = v; break;
+        case 47: w[47] = v; break;
+        case 48: w[48] = v; break;
+        case 49: w[49] = v; break;
+        case 50: w[50] = v; break;
+        case 51: w[51] = v; break;
+        case 52: w[52] = v; break;
+        case 53: w[53] = v; break;
+        case 54: w[54] = v; break;
+        case 55: w[55] = v; break;
+        case 56: w[56] = v; break;
+        case 57: w[57] = v; break;
+        case 58: w[58] = v; break;
+        case 59: w[59] = v; break;
+        case 60: w[60] = v; break;
+        case 61: w[61] = v; break;
+        case 62: w[62] = v; break;
+        case 63: w[63] = v; break;
+        case 64: w[64] = v; break;
+    }
+}
+__global__ __launch_bounds__(RW_T) void k_pm1p_bandr(int N, BandCls c0, BandCls c1, double *part) {
+    __shared__ __align__(16) double stg[BSTEP][UR];    // band rows entering (Br order) + z; then rows of U
+    __shared__ __align__(16) double colj[PBR + 1];
+    __shared__ __align__(16) double ltab[PBR + 1];
     __shared__ double ys[BAND_W];
-    __shared__ __align__(16) int ptab[BAND_B];     // slot and multiplier of window row i at i - 1
-    __shared__ __align__(16) double ltab[BAND_B];
+    __shared__ int sp_s;
+    __shared__ double rp_s;
     const bool second = (int)blockIdx.x >= c0.next;
-    const int x = second ? (int)blockIdx.x - c0.next : (int)blockIdx.x, lane = threadIdx.x;
+    const int x = second ? (int)blockIdx.x - c0.next : (int)blockIdx.x;
+    const int g = threadIdx.x, lane = g & 63, wv = g >> 6;
     const int plus = second ? c1.plus : c0.plus;
     const double *Br = second ? c1.Br : c0.Br;
     const double *eo = second ? c1.eo : c0.eo;
-    const QDArgs &q = c0.q;                        // M, self and qpart are those of both classes
+    const QDArgs &q = c0.q;
     const double *qa[QM];
 #pragma unroll
     for (int k = 0; k < QM; k++) qa[k] = second ? c1.q.a[k] : c0.q.a[k];
     const i64 ldz = N;
     const double sh = plus ? -eo[x] : eo[x];
     const double *z = (second ? c1.Z : c0.Z) + ldz * x;
-    double *U = (second ? c1.U : c0.U) + (i64)x * N * UW;
+    double *U = (second ? c1.U : c0.U) + (i64)x * N * UR;
     double *yx = (second ? c1.y : c0.y) + (i64)x * N;
-    const int ka = lane, kb = lane + 64;           // relative columns of this lane (lane 0 also 2*BAND_B)
-    for (int t = lane; t < PBR * BAND_W; t += 64) {   // (B + s)(r, c), zero outside the matrix and the band
-        const int r = t / BAND_W, c = t % BAND_W, d = c - r;
+    const bool col = g < BAND_W, zl = g == BAND_W;
+    double w[PBR];
+#pragma unroll
+    for (int s = 0; s < PBR; s++) {          // slot s = row s; slot g = column g
         double v = 0.0;
-        if (r < N && c < N && d >= -BAND_B && d <= BAND_B) v = Br[(i64)r * BAND_W + d + BAND_B] + (d == 0 ? sh : 0.0);
-        win[r][c] = v;
+        if (col) {
+            const int d = g - s;
+            if (s < N && g < N && d >= -BAND_B && d <= BAND_B) v = Br[(i64)s * BAND_W + d + BAND_B] + (d == 0 ? sh : 0.0);
+        } else if (zl) {
+            v = s < N ? z[s] : 0.0;
+        }
+        w[s] = v;
     }
-    int perm = lane, perm64 = 64;                  // slots of window rows lane and BAND_B
-    double zr = lane < N ? z[lane] : 0.0, z64 = BAND_B < N ? z[BAND_B] : 0.0;
-    // block prefetch of band rows r0 .. r0+BSTEP-1 (relative columns ka, kb, 2*BAND_B; z on lanes < BSTEP)
-    double fa[BSTEP], fb[BSTEP], fc[BSTEP], fz;
+    // block prefetch of the band rows entering next (Br order, plus z at UW - 1): BSTEP*UW values
+    constexpr int PF = (BSTEP * UW + RW_T - 1) / RW_T;
+    double pf[PF];
     auto fetch_rows = [&](int r0) {
 #pragma unroll
-        for (int u = 0; u < BSTEP; u++) {
-            const bool in = r0 + u < N;
-            const double *row = Br + (i64)(r0 + u) * BAND_W;
-            fa[u] = in ? row[ka] : 0.0;
-            fb[u] = in ? row[kb] : 0.0;
-            fc[u] = in && lane == 0 ? row[2 * BAND_B] : 0.0;
+        for (int k = 0; k < PF; k++) {
+            const int e = g + RW_T * k, u = e / UW, t = e % UW, r = r0 + u;
+            double v = 0.0;
+            if (e < BSTEP * UW && r < N) v = t < BAND_W ? Br[(i64)r * BAND_W + t] : z[r];
+            pf[k] = v;
         }
-        fz = lane < BSTEP && r0 + lane < N ? z[r0 + lane] : 0.0;
     };
     fetch_rows(PBR);
-    lds_barrier();
 #pragma unroll 1
     for (int j0 = 0; j0 < N; j0 += BSTEP) {
-        // rows j0+PBR .. enter during this block: stage them, then fetch the next block
+        __syncthreads();                     // the previous block's stage is consumed
 #pragma unroll
-        for (int u = 0; u < BSTEP; u++) {
-            stg[u][ka] = fa[u];
-            stg[u][kb] = fb[u] + (kb == BAND_B && j0 + PBR + u < N ? sh : 0.0);   // k' = BAND_B: diagonal
-            if (lane == 0) stg[u][2 * BAND_B] = fc[u];
+        for (int k = 0; k < PF; k++) {
+            const int e = g + RW_T * k;
+            if (e < BSTEP * UW) stg[e / UW][e % UW] = pf[k];
         }
-        if (lane < BSTEP) stg[lane][BAND_W] = fz;
         fetch_rows(j0 + PBR + BSTEP);
-        lds_barrier();
         const int jn = N - j0 < BSTEP ? N - j0 : BSTEP;
 #pragma unroll 1
         for (int jj = 0; jj < jn; jj++) {
-            const int j = j0 + jj;
-            const int nr = N - j < PBR ? N - j : PBR;
-            const int cs0 = j % BAND_W;
-            // pivot: the largest |entry| of column j.  The key is |entry| with its 7 lowest mantissa bits
-            // replaced by 127 - row: one unsigned maximum, ties (to 2^-45 relative) going to the first row
-            double cv = lane < nr ? win[perm][cs0] : 0.0;
-            double cv64 = nr == PBR ? win[perm64][cs0] : 0.0;
-            auto key = [](double v, int i) -> unsigned long long {
-                return ((unsigned long long)__double_as_longlong(fabs(v)) & ~0x7Full) | (unsigned long long)(127 - i);
-            };
-            unsigned long long kk = lane < nr ? key(cv, lane) : 0ull;
-            if (lane == 0 && nr == PBR) { const unsigned long long k64 = key(cv64, BAND_B); if (k64 > kk) kk = k64; }
-            for (int off = 32; off > 0; off >>= 1) {
-                const unsigned long long o = __shfl_xor(kk, off, 64);
-                if (o > kk) kk = o;
-            }
-            const int pv = 127 - (int)(__builtin_amdgcn_readfirstlane((int)(kk & 0x7F)));
-            const int s0 = __builtin_amdgcn_readlane(perm, 0);
-            const int sp = pv < 64 ? __builtin_amdgcn_readlane(perm, pv) : perm64;
-            const double z0 = rlane(zr, 0), zp = pv < 64 ? rlane(zr, pv) : z64;
-            const double c0v = rlane(cv, 0), piv = pv < 64 ? rlane(cv, pv) : cv64;
-            if (pv != 0) {                         // rows 0 and pv exchange slots, z and column values
-                if (lane == pv) { perm = s0; zr = z0; cv = c0v; }
-                if (pv == BAND_B) { perm64 = s0; z64 = z0; cv64 = c0v; }
-            }
-            if (lane == 0) { perm = sp; zr = zp; cv = piv; }
-            const double rp = 1.0 / piv;
-            const int sa = cs0 + ka < BAND_W ? cs0 + ka : cs0 + ka - BAND_W;
-            const int sb = cs0 + kb < BAND_W ? cs0 + kb : cs0 + kb - BAND_W;
-            const int sc = cs0 + 2 * BAND_B < BAND_W ? cs0 + 2 * BAND_B : cs0 + 2 * BAND_B - BAND_W;
-            const double ua = win[sp][sa], ub = win[sp][sb], uc = lane == 0 ? win[sp][sc] : 0.0;
-            double *Uj = U + (i64)j * UW;
-            Uj[ka] = ka == 0 ? rp : ua;
-            Uj[kb] = ub;
-            if (lane == 0) Uj[2 * BAND_B] = uc;
-            if (lane == 1) Uj[BAND_W] = zp;        // the eliminated right-hand side
-            const double l = lane > 0 && lane < nr ? cv * rp : 0.0;   // rows past the matrix: multiplier 0
-            const double l64 = nr == PBR ? cv64 * rp : 0.0;
-            zr -= l * zp;
-            z64 -= l64 * zp;
-            if (lane > 0) { ptab[lane - 1] = perm; ltab[lane - 1] = l; }
-            else { ptab[BAND_B - 1] = perm64; ltab[BAND_B - 1] = l64; }
-            lds_barrier();                         // pivot row and column read before the updates
-            // rows 1..BAND_B -= l_i * pivot row, columns 1..2*BAND_B; the rows are distinct slots, so
-            // eight at a time: all loads, then all stores.  Slots and multipliers come from LDS
-            // (broadcast reads): moving them lane to lane through scalar registers stalls every row.
-#pragma unroll 1
-            for (int i0 = 1; i0 < PBR; i0 += 8) {
-                int rr[8];
-                double lv[8], xa[8], xb[8], xc[8];
-                {
-                    const int4 pa4 = *(const int4 *)&ptab[i0 - 1], pb4 = *(const int4 *)&ptab[i0 + 3];
-                    rr[0] = pa4.x; rr[1] = pa4.y; rr[2] = pa4.z; rr[3] = pa4.w;
-                    rr[4] = pb4.x; rr[5] = pb4.y; rr[6] = pb4.z; rr[7] = pb4.w;
+            const int j = j0 + jj, cs0 = j % BAND_W;
+            if (g == cs0) {
 #pragma unroll
-                    for (int u = 0; u < 8; u += 2) {
-                        const double2 l2 = *(const double2 *)&ltab[i0 - 1 + u];
-                        lv[u] = l2.x;
-                        lv[u + 1] = l2.y;
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    xa[u] = win[rr[u]][sa];
-                    xb[u] = win[rr[u]][sb];
-                    xc[u] = lane == 0 ? win[rr[u]][sc] : 0.0;
-                }
-#pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    if (ka != 0) win[rr[u]][sa] = xa[u] - lv[u] * ua;
-                    win[rr[u]][sb] = xb[u] - lv[u] * ub;
-                    if (lane == 0) win[rr[u]][sc] = xc[u] - lv[u] * uc;
-                }
+                for (int s = 0; s < PBR; s++) colj[s] = w[s];
             }
-            // drop row j (its slot takes row j+BAND_B+1) and column j (its slot takes column j+2*BAND_B+1)
-            int np = __shfl_down(perm, 1, 64);
-            double nzr = __shfl_down(zr, 1, 64);
-            if (lane == 63) { np = perm64; nzr = z64; }
-            perm = np;
-            zr = nzr;
-            perm64 = sp;
-            z64 = stg[jj][BAND_W];
-            win[perm][cs0] = 0.0;
-            {   // new row from the stage: relative columns k' of the next window are columns j + 1 + k'
-                const int ta = cs0 + 1 + ka, tb = cs0 + 1 + kb, tc = cs0 + 1 + 2 * BAND_B;
-                win[sp][ta < BAND_W ? ta : ta - BAND_W] = stg[jj][ka];
-                win[sp][tb < BAND_W ? tb : tb - BAND_W] = stg[jj][kb];
-                if (lane == 0) win[sp][tc < BAND_W ? tc : tc - BAND_W] = stg[jj][2 * BAND_B];
+            __syncthreads();                 // column j (and the stage) in LDS
+            if (wv == 0) {
+                // pivot: largest |entry| of column j over the row slots; the key is |entry| with its 7
+                // lowest mantissa bits replaced by 127 - slot (ties to 2^-45 relative go to the lower slot)
+                auto key = [](double v, int i) -> unsigned long long {
+                    return ((unsigned long long)__double_as_longlong(fabs(v)) & ~0x7Full) | (unsigned long long)(127 - i);
+                };
+                const double cv = colj[lane], cv64 = colj[BAND_B];
+                unsigned long long kk = key(cv, lane);
+                if (lane == 0) { const unsigned long long k64 = key(cv64, BAND_B); if (k64 > kk) kk = k64; }
+                for (int off = 32; off > 0; off >>= 1) {
+                    const unsigned long long o = __shfl_xor(kk, off, 64);
+                    if (o > kk) kk = o;
+                }
+                const int pv = 127 - (int)(kk & 0x7F);
+                const double piv = colj[pv];
+                const double rp = 1.0 / piv;
+                ltab[lane] = cv * rp;
+                if (lane == 0) { ltab[BAND_B] = cv64 * rp; sp_s = pv; rp_s = rp; }
             }
-            lds_barrier();
+            __syncthreads();                 // multipliers and pivot slot
+            const int sp = __builtin_amdgcn_readfirstlane(sp_s);
+            const double u = rw_get(w, sp);  // this column's entry of the pivot row
+            double *Uj = U + (i64)j * UR;
+            if (col) Uj[g] = g == cs0 ? 0.0 : u;
+            if (zl) Uj[BAND_W] = u;          // the eliminated right-hand side
+            if (g == 0) Uj[BAND_W + 1] = rp_s;
+#pragma unroll
+            for (int s = 0; s < PBR - 1; s += 2) {
+                const double2 l2 = *(const double2 *)&ltab[s];
+                w[s] -= l2.x * u;
+                w[s + 1] -= l2.y * u;
+            }
+            w[BAND_B] -= ltab[BAND_B] * u;
+            if (g == cs0) {                  // column j leaves; its slot becomes column j + 2*BAND_B + 1
+#pragma unroll
+                for (int s = 0; s < PBR; s++) w[s] = 0.0;
+            }
+            // row j + BAND_B + 1 enters the pivot's slot: column c at slot g is c = j+1+((g-cs0-1) mod BAND_W)
+            double v = 0.0;
+            if (col) {
+                const int t = g - cs0 - 1 < 0 ? g - cs0 - 1 + BAND_W : g - cs0 - 1;
+                v = stg[jj][t] + (t == BAND_B && j + PBR < N ? sh : 0.0);
+            } else if (zl) {
+                v = stg[jj][BAND_W];
+            }
+            rw_set(w, sp, v);
         }
     }
-    // back substitution: u_j = (zf_j - sum_k U[j][k] u_{j+k}) * U[j][0], u_{j+k} at ys[(j + k) % BAND_W],
-    // rows of U by blocks of BSTEP from the bottom (lane 1 also fetches the right-hand side at BAND_W)
-    __threadfence_block();
-    for (int t = lane; t < BAND_W; t += 64) ys[t] = 0.0;
-    auto fetch_u = [&](int jtop) {                 // rows jtop, jtop-1, .. jtop-BSTEP+1
-#pragma unroll
-        for (int u = 0; u < BSTEP; u++) {
-            const bool in = jtop - u >= 0;
-            const double *row = U + (i64)(jtop - u) * UW;
-            fa[u] = in ? row[ka] : 0.0;
-            fb[u] = in ? row[kb] : 0.0;
-            fc[u] = in && lane < 2 ? row[2 * BAND_B + lane] : 0.0;
-        }
-    };
-    fetch_u(N - 1);
-    lds_barrier();
+    __syncthreads();
+    // back substitution by wavefront 0: u_j = (zf_j - sum_g U[j][g] u(slot g)) * rp_j, u in a ring by slot
     double ez = 0.0, uu = 0.0, qsc[QM], qpc[QM];
 #pragma unroll
     for (int k = 0; k < QM; k++) { qsc[k] = 0.0; qpc[k] = 0.0; }
-#pragma unroll 1
-    for (int jt = N - 1; jt >= 0; jt -= BSTEP) {
+    if (wv == 0) {
+        for (int t = lane; t < BAND_W; t += 64) ys[t] = 0.0;
+        double fa[BSTEP], fb[BSTEP], fc[BSTEP];
+        auto fetch_u = [&](int jtop) {
 #pragma unroll
-        for (int u = 0; u < BSTEP; u++) {
-            stg[u][ka] = fa[u];
-            stg[u][kb] = fb[u];
-            if (lane < 2) stg[u][2 * BAND_B + lane] = fc[u];
-        }
-        fetch_u(jt - BSTEP);
-        lds_barrier();
-        const int jn = jt + 1 < BSTEP ? jt + 1 : BSTEP;
-#pragma unroll 1
-        for (int u = 0; u < jn; u++) {
-            const int j = jt - u;
-            const int cs = j % BAND_W;
-            const int sa = cs + ka < BAND_W ? cs + ka : cs + ka - BAND_W;
-            const int sb = cs + kb < BAND_W ? cs + kb : cs + kb - BAND_W;
-            const int sc = cs + 2 * BAND_B < BAND_W ? cs + 2 * BAND_B : cs + 2 * BAND_B - BAND_W;
-            double s = stg[u][kb] * ys[sb];
-            if (ka != 0) s += stg[u][ka] * ys[sa];
-            if (lane == 0) s += stg[u][2 * BAND_B] * ys[sc];
-            s = wsum(s);
-            if (lane == 0) {
-                const double y = (stg[u][BAND_W] - s) * stg[u][0];
-                ys[cs] = y;
-                yx[j] = y;
+            for (int uu2 = 0; uu2 < BSTEP; uu2++) {
+                const bool in = jtop - uu2 >= 0;
+                const double *row = U + (i64)(jtop - uu2) * UR;
+                fa[uu2] = in ? row[lane] : 0.0;
+                fb[uu2] = in ? row[lane + 64] : 0.0;
+                fc[uu2] = in && lane < 3 ? row[2 * BAND_B + lane] : 0.0;
             }
-            lds_barrier();
+        };
+        fetch_u(N - 1);
+        WAVE_SYNC();
+#pragma unroll 1
+        for (int jt = N - 1; jt >= 0; jt -= BSTEP) {
+#pragma unroll
+            for (int uu2 = 0; uu2 < BSTEP; uu2++) {
+                stg[uu2][lane] = fa[uu2];
+                stg[uu2][lane + 64] = fb[uu2];
+                if (lane < 3) stg[uu2][2 * BAND_B + lane] = fc[uu2];
+            }
+            fetch_u(jt - BSTEP);
+            WAVE_SYNC();
+            const int jn = jt + 1 < BSTEP ? jt + 1 : BSTEP;
+#pragma unroll 1
+            for (int uu2 = 0; uu2 < jn; uu2++) {
+                const int j = jt - uu2, cs = j % BAND_W;
+                double sm = stg[uu2][lane] * ys[lane] + stg[uu2][lane + 64] * ys[lane + 64];
+                if (lane == 0) sm += stg[uu2][2 * BAND_B] * ys[2 * BAND_B];
+                sm = wsum(sm);
+                if (lane == 0) {
+                    const double y = (stg[uu2][BAND_W] - sm) * stg[uu2][BAND_W + 1];
+                    ys[cs] = y;
+                    yx[j] = y;
+                }
+                WAVE_SYNC();
+            }
         }
     }
     __threadfence();
     __syncthreads();
-    for (int i = lane; i < N; i += 64) {
-        const double u = yx[i];
-        ez += z[i] * u;
-        uu += u * u;
+    for (int i = g; i < N; i += RW_T) {
+        const double uv = yx[i];
+        ez += z[i] * uv;
+        uu += uv * uv;
 #pragma unroll
         for (int k = 0; k < QM; k++)
-            if (qd_on(q, k)) qpc[k] -= qa[k][ldz * x + i] * u;
+            if (qd_on(q, k)) qpc[k] -= qa[k][ldz * x + i] * uv;
     }
     block_store4(0.0, 0.0, -ez, uu, part);
     if (q.M) block_storeQ(qsc, qpc, q);
@@ -1936,9 +2035,9 @@ static int pm1p_band_sums(Stage2 *p, Pm1pEarly *E) {
     if (!n0 && !n1) return 0;
     bc[0].U = E->Ub;
     bc[0].y = E->yb;
-    bc[1].U = E->Ub + (i64)n0 * N * UW;
+    bc[1].U = E->Ub + (i64)n0 * N * UR;
     bc[1].y = E->yb + (i64)n0 * N;
-    hipLaunchKernelGGL(k_pm1p_band, dim3(n0 + n1), dim3(64), 0, st, N, bc[0], bc[1], E->ptb);
+    hipLaunchKernelGGL(k_pm1p_bandr, dim3(n0 + n1), dim3(RW_T), 0, st, N, bc[0], bc[1], E->ptb);
     NV2_LAUNCH_CHECK();
     for (int c = 0; c < 2; c++) {
         if (!bc[c].next) continue;
@@ -2014,7 +2113,7 @@ int stage2_pm1p_begin(Stage2 *p, int s, double thr) {
         const i64 nx = (i64)p->nc + p->nv, nmx = p->nc > p->nv ? p->nc : p->nv;
         GET(y0, "pm1_Ypc0", N3 * (p->nv ? p->nv : 1)); GET(y1, "pm1_Ypc1", N3 * (p->nc ? p->nc : 1));
         GET(wq, "pm1_bandq", band_apply_workspace((int)nmx));
-        GET(ub, "pm1_bandU", nx * N3 * UW); GET(yb, "pm1_bandy", nx * N3); GET(pb, "pm1_bandparts", 4 * nx);
+        GET(ub, "pm1_bandU", nx * N3 * UR); GET(yb, "pm1_bandy", nx * N3); GET(pb, "pm1_bandparts", 4 * nx);
         E->Ypc[0] = y0; E->Ypc[1] = y1; E->Wq = wq; E->Ub = ub; E->yb = yb; E->ptb = pb;
     }
     E->x_ready = false;
@@ -2874,11 +2973,11 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         const int nmax = J[JPM1P].nmax;
         for (int c = 0; c < 2; c++) if (band_cls[c] < 0) bc[c].next = 0;
         const int n0 = bc[0].next, n1 = bc[1].next;
-        GET(Ub, "pm1_bandU", (i64)(n0 + n1) * nmax * UW);
+        GET(Ub, "pm1_bandU", (i64)(n0 + n1) * nmax * UR);
         GET(yb, "pm1_bandy", (i64)(n0 + n1) * nmax);
         bc[0].U = Ub;
         bc[0].y = yb;
-        bc[1].U = Ub + (i64)n0 * nmax * UW;
+        bc[1].U = Ub + (i64)n0 * nmax * UR;
         bc[1].y = yb + (i64)n0 * nmax;
         double *pt2 = parts(p, n0 + n1);
         if (!pt2) { set_error("hipMalloc(parts) failed"); return -1; }
@@ -2886,7 +2985,7 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         bc[0].q.qpart = bc[1].q.qpart = q.qpart;
         bc[0].q.M = bc[1].q.M = q.M;
         bc[0].q.self = bc[1].q.self = q.self;
-        hipLaunchKernelGGL(k_pm1p_band, dim3(n0 + n1), dim3(64), 0, st, nmax, bc[0], bc[1], pt2);
+        hipLaunchKernelGGL(k_pm1p_bandr, dim3(n0 + n1), dim3(RW_T), 0, st, nmax, bc[0], bc[1], pt2);
         NV2_LAUNCH_CHECK();
         for (int c = 0; c < 2; c++) {
             if (!bc[c].next) continue;
