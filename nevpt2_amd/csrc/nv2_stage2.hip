@@ -1804,14 +1804,15 @@ static bool chol_metric() {   // read at every solve, so that a caller can switc
     const char *e = getenv("NV2_METRIC_EIGEN");
     return !(e && atoi(e) > 0);
 }
-// K' of the Cholesky path reduced to band form or to tridiagonal form.  The band form wins from
-// about N = 1500 up (CAS(14,14), N = 2744: 157 vs 165 ms per evaluation; CAS(10,10), N = 1000:
-// 29 vs 25 ms): NV2_K_TRIDIAG=1 / NV2_K_BAND=1 force one form.
+// K' of the Cholesky path reduced to band form or to tridiagonal form.  The band form wins at the
+// measured orders (per evaluation: CAS(14,14), N = 2744, 136 vs 165 ms; CAS(12,12) x 6 roots, N = 1728,
+// 518 vs 556 ms; CAS(10,10), N = 1000, 22.6 vs 25.4 ms); below N = 512 the tridiagonal form's short
+// chain stays.  NV2_K_TRIDIAG=1 / NV2_K_BAND=1 force one form.
 static bool band_kprime(int N) {
     const char *t = getenv("NV2_K_TRIDIAG"), *b = getenv("NV2_K_BAND");
     if (t && atoi(t) > 0) return false;
     if (b && atoi(b) > 0) return true;
-    return N >= 1536;
+    return N >= 512;
 }
 __global__ void k_shift_diag(double *A, int N, i64 sA, int nbm, double d) {
     const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;

@@ -4,7 +4,7 @@ With the factored metric the library reduces K' = Q B Q^T to half-bandwidth 64 (
 panels, two-sided block-reflector updates) and solves (B + s_x) u = Q^T y_x by banded elimination
 with partial pivoting, one workgroup per external orbital; NV2_K_TRIDIAG=1 reduces K' to
 tridiagonal form instead (one Householder column per launch, tridiagonal solves).  NV2_K_BAND=1
-forces the band form at these small orders (by default it starts at N = 1536).  Both are exact
+forces the band form at these small orders (by default it starts at N = 512).  Both are exact
 rewritings of sum_mu r_mu^2 / (w_mu + s_x) and must agree to relative 1e-10 on every class, the
 QD couplings included.  The orders cover a final Householder block of exactly 64 rows (N = 512)
 and ragged ones (N = 216, 343).  Smaller active spaces (nact 4, 5 at these fillings) have
