@@ -135,15 +135,20 @@ int nv2_koop_matrices(int64_t metat, int64_t nact, int64_t ncore, int64_t norb, 
     DBuf o[14];
     size_t sz[14] = {s2, s2, s4, s4, s4, s4, s4, s2, s4, s4, s4, s4, s2, s2};
     for (int k = 0; k < 14; k++) TRY(o[k].alloc(sz[k]));
-    Stage1Scratch w; w.AJ = AJ.d(); w.Fhalf = Fh.d(); w.Fone = F1.d();
+    DBuf k0p;
+    TRY(k0p.alloc(sizeof(double) * koop0p_workspace((int)n)));
+    Stage1Scratch w; w.AJ = AJ.d(); w.Fhalf = Fh.d(); w.Fone = F1.d(); w.k0p = k0p.d();
+    if (rocblas_create_handle(&w.h) != rocblas_status_success) { set_error("rocblas_create_handle failed"); return -1; }
     Stage1Out so;
     so.ipko1 = o[0].d(); so.eako1 = o[1].d(); so.hrdm = o[2].d(); so.ipko2 = o[3].d(); so.eako2 = o[4].d();
     so.gk0pa = o[5].d(); so.gk0pd = o[6].d(); so.gk0pf = o[7].d();
     so.bmat = o[8].d(); so.btmat = o[9].d(); so.cmat = o[10].d(); so.ctmat = o[11].d();
     so.dmat = o[12].d(); so.dtmat = o[13].d();
     Rdm R{(int)n, (int)M, nwords(n), d3.d(), d2.d(), d1.d(), nullptr};
-    TRY(launch_koop_small(R, (int)ncore, daj.d(), dat.d(), w, so, 0));
-    NV2_HIP(hipDeviceSynchronize());
+    int rc = launch_koop_small(R, (int)ncore, daj.d(), dat.d(), w, so, 0);
+    if (!rc) { hipError_t e = hipDeviceSynchronize(); if (e != hipSuccess) { set_error("koop: %s", hipGetErrorString(e)); rc = -1; } }
+    rocblas_destroy_handle(w.h);
+    if (rc) return rc;
     double *hs[14] = {ipko1, eako1, hrdm, ipko2, eako2, gk0pa, gk0pd, gk0pf, bmat, btmat, cmat, ctmat, dmat, dtmat};
     for (int k = 0; k < 14; k++) TRY(o[k].down(hs[k]));
     return 0;

@@ -121,6 +121,7 @@ nv2_ctx *nv2_ctx_create(int64_t ncore, int64_t nact, int64_t nvirt, int64_t meta
         TRYC(ctx_alloc(c, "1-TRDM", P * n2)); TRYC(ctx_alloc(c, "2-TRDM", P * n4)); TRYC(ctx_alloc(c, "3-TRDM", P * n6));
     }
     TRYC(ctx_alloc(c, "bamat_work", (size_t)bamat_lo_workspace((int)n)));
+    TRYC(ctx_alloc(c, "koop0p_work", (size_t)koop0p_workspace((int)n)));
     c->e4_bytes = sizeof(double) * (size_t)(n4 * n4);
     if (hipMalloc(&c->E4, c->e4_bytes) != hipSuccess) { set_error("hipMalloc(E4) failed"); nv2_ctx_destroy(c); return nullptr; }
     c->plan = contract_plan_create((int)n, (int)M);
@@ -205,6 +206,7 @@ int nv2_ctx_run(nv2_ctx *c, int32_t prec, double thr_lindep, double *out) {
     if (launch_aj_atwo(n, c->nc, (int)c->norb, c->L, c->find("cholesky"), c->find("oneint"), c->find("aj"),
                        c->find("atwo"), st)) return -1;
     Stage1Scratch w; w.AJ = c->find("AJ"); w.Fhalf = c->find("Fhalf"); w.Fone = c->find("Fone");
+    w.h = c->rb; w.k0p = c->find("koop0p_work");
     Stage1Out so;
     so.ipko1 = c->find("1-IPKO"); so.eako1 = c->find("1-EAKO"); so.hrdm = c->find("2-HRDM");
     so.ipko2 = c->find("2-IPKO"); so.eako2 = c->find("2-EAKO"); so.gk0pa = c->find("gk0pa");

@@ -10,7 +10,10 @@ struct Stage1Scratch {
     double *AJ = nullptr;      // active aj block, n x n
     double *Fhalf = nullptr;   // f with alpha = 1/2
     double *Fone = nullptr;    // f with alpha = 1
+    rocblas_handle h = nullptr;   // with k0p: koopman0pE's matrices as GEMMs (else one wavefront per element)
+    double *k0p = nullptr;        // koop0p_workspace(n) doubles
 };
+i64 koop0p_workspace(int n);
 
 struct Stage1Out {            // every array: state index fastest, reference shapes
     double *ipko1 = nullptr, *eako1 = nullptr;                         // (M,n,n)

@@ -287,6 +287,121 @@ __global__ void k_koop0p(Rdm R, const double *atwo, const double *F, double *koo
         koopbb[wid] = kb;
     }
 }
+// koopman0pE's two n^4 matrices as GEMMs (the same sums as k_koop0p, qdnevpt.F koopman0pE): each of
+// its four n^7 terms is sum_{three indices} atwo(..) * X(..) with X built from the 2- and 3-RDM, so X
+// is materialised for one state as an n^3 x n^3 matrix and multiplied with atwo (n^3 x n); the O(n^5)
+// Fock terms and the final index order go to k_k0p_combine.
+//   C1[(bp,ap,b),a] = sum_(c,ie,d) X1 atwo(c,ie,d,a),  X1 = eee2(bp,ap,c,ie,d,b) + eee2(bp,ap,d,b,c,ie)
+//   C2[(bp,ap,a),b] = sum_(ie,c,d) X2 atwo(b,ie,c,d),  X2 = eee2(bp,ap,a,ie,c,d) + eee2(bp,ap,c,d,a,ie)
+//   C3[(a,ap,bp),b] = sum_(c,ie,d) E1 atwo(c,ie,b,d),  E1 = the kernel's e1
+//   C4[(b,ap,bp),a] = sum_(c,ie,d) E2 atwo(c,ie,d,a),  E2 = the kernel's e2
+//   koopaa = fock_a + C1 - C2,  koopbb = fock_b - C3/2 + C4/2
+__global__ void k_k0p_x(Rdm R, int s, int which, double *X) {
+    const int n = R.n;
+    const i64 n3 = (i64)n * n * n, e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n3 * n3) return;
+    const int r = (int)(e % n3), cl = (int)(e / n3);
+    const int r0 = r % n, r1 = (r / n) % n, r2 = r / (n * n);
+    const int q0 = cl % n, q1 = (cl / n) % n, q2 = cl / (n * n);
+    double v;
+    if (which == 0) {                 // rows (bp,ap,b), columns (c,ie,d)
+        const int bp = r0, ap = r1, b = r2, c = q0, ie = q1, d = q2;
+        v = eee2(R, s, bp, ap, c, ie, d, b) + eee2(R, s, bp, ap, d, b, c, ie);
+    } else if (which == 1) {          // rows (bp,ap,a), columns (ie,c,d)
+        const int bp = r0, ap = r1, a = r2, ie = q0, c = q1, d = q2;
+        v = eee2(R, s, bp, ap, a, ie, c, d) + eee2(R, s, bp, ap, c, d, a, ie);
+    } else if (which == 2) {          // rows (a,ap,bp), columns (c,ie,d): e1
+        const int a = r0, ap = r1, bp = r2, c = q0, ie = q1, d = q2;
+        v = -eee2(R, s, c, ie, a, ap, bp, d) - eee2(R, s, a, ap, bp, d, c, ie);
+        if (ap == a) v = v + 2.0 * (ee2(R, s, c, ie, bp, d) + ee2(R, s, bp, d, c, ie));
+        if (bp == ap) v = v + ee2(R, s, c, ie, a, d) + ee2(R, s, a, d, c, ie);
+        if (ap == c) {
+            v = v - ee2(R, s, a, ie, bp, d);
+            if (ie == a) v = v + 2.0 * D1(s, bp, d);
+        }
+        if (bp == ie) {
+            v = v + ee2(R, s, a, ap, c, d);
+            if (ap == a) v = v - 2.0 * D1(s, c, d);
+        }
+    } else {                          // rows (b,ap,bp), columns (c,ie,d): e2
+        const int b = r0, ap = r1, bp = r2, c = q0, ie = q1, d = q2;
+        v = -eee2(R, s, c, ie, d, ap, bp, b) - eee2(R, s, d, ap, bp, b, c, ie);
+        if (ap == d) v = v + 2.0 * (ee2(R, s, c, ie, bp, b) + ee2(R, s, bp, b, c, ie));
+        if (bp == ap) v = v + ee2(R, s, c, ie, d, b) + ee2(R, s, d, b, c, ie);
+        if (ap == c) {
+            v = v - ee2(R, s, d, ie, bp, b);
+            if (d == ie) v = v + 2.0 * D1(s, bp, b);
+        }
+        if (bp == ie) {
+            v = v + ee2(R, s, d, ap, c, b);
+            if (d == ap) v = v - 2.0 * D1(s, c, b);
+        }
+    }
+    X[e] = v;
+}
+// AT3[(c,ie,d), b] = atwo(c,ie,b,d)
+__global__ void k_k0p_at3(int n, const double *atwo, double *AT3) {
+    const i64 n2 = (i64)n * n, e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n2 * n2) return;
+    const int c = (int)(e % n), ie = (int)((e / n) % n), d = (int)((e / n2) % n), b = (int)(e / (n2 * n));
+    AT3[e] = atwo[c + n * (ie + n * (b + (i64)n * d))];
+}
+__global__ void k_k0p_combine(Rdm R, int s, const double *F, const double *C, double *koopaa, double *koopbb) {
+    const int n = R.n, M = R.M;
+    const i64 n3 = (i64)n * n * n, n4 = n3 * n, e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n4) return;
+    const int bp = (int)(e % n), ap = (int)((e / n) % n), a = (int)((e / (n * n)) % n), b = (int)(e / n3);
+    const double *C1 = C, *C2 = C + n4, *C3 = C + 2 * n4, *C4 = C + 3 * n4;
+    double ka = 0.0, kb = 0.0;
+    for (int c = 0; c < n; c++) {     // the terms of the kernel's d = ie = 0 branch
+        ka = ka + 2.0 * (FF(c, a) * ee2(R, s, bp, ap, c, b) - FF(b, c) * ee2(R, s, bp, ap, a, c));
+        double dum1 = -ee2(R, s, a, ap, bp, c);
+        if (a == ap) dum1 = dum1 + 2.0 * D1(s, bp, c);
+        if (bp == ap) dum1 = dum1 + D1(s, a, c);
+        double dum2 = -ee2(R, s, c, ap, bp, b);
+        if (ap == c) dum2 = dum2 + 2.0 * D1(s, bp, b);
+        if (bp == ap) dum2 = dum2 + D1(s, c, b);
+        kb = kb - FF(b, c) * dum1 + FF(c, a) * dum2;
+    }
+    ka = ka + C1[(bp + n * (ap + (i64)n * b)) + n3 * a] - C2[(bp + n * (ap + (i64)n * a)) + n3 * b];
+    kb = kb - 0.5 * C3[(a + n * (ap + (i64)n * bp)) + n3 * b] + 0.5 * C4[(b + n * (ap + (i64)n * bp)) + n3 * a];
+    const i64 o = s + (i64)M * e;
+    koopaa[o] = ka;
+    koopbb[o] = kb;
+}
+i64 koop0p_workspace(int n) {
+    const i64 n3 = (i64)n * n * n;
+    return n3 * n3 + 5 * n3 * n;
+}
+static int launch_koop0p_gemm(const Rdm &R, const double *atwo, const double *F, rocblas_handle h, double *work,
+                              double *koopaa, double *koopbb, hipStream_t st) {
+    const int n = R.n;
+    const i64 n3 = (i64)n * n * n, n4 = n3 * n, n6 = n3 * n3;
+    double *X = work, *AT3 = work + n6, *C = AT3 + n4;
+    if (rocblas_set_stream(h, st) != rocblas_status_success) { set_error("rocblas_set_stream failed"); return -1; }
+    hipLaunchKernelGGL(k_k0p_at3, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, st, n, atwo, AT3);
+    NV2_LAUNCH_CHECK();
+    const double one = 1.0, zero = 0.0;
+    const rocblas_operation Nn = rocblas_operation_none, Tt = rocblas_operation_transpose;
+    const int m3 = (int)n3;
+    for (int s = 0; s < R.M; s++) {
+        for (int which = 0; which < 4; which++) {
+            hipLaunchKernelGGL(k_k0p_x, dim3((unsigned)((n6 + 255) / 256)), dim3(256), 0, st, R, s, which, X);
+            NV2_LAUNCH_CHECK();
+            rocblas_status rs;
+            double *Cw = C + which * n4;
+            if (which == 1)           // B[(ie,c,d), b] = atwo[b + n (ie,c,d)]: atwo as n x n^3, transposed
+                rs = rocblas_dgemm(h, Nn, Tt, m3, n, m3, &one, X, m3, atwo, n, &zero, Cw, m3);
+            else
+                rs = rocblas_dgemm(h, Nn, Nn, m3, n, m3, &one, X, m3, which == 2 ? AT3 : atwo, m3, &zero, Cw, m3);
+            if (rs != rocblas_status_success) { set_error("koop0p GEMM failed (%d)", (int)rs); return -1; }
+        }
+        hipLaunchKernelGGL(k_k0p_combine, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, st, R, s, F, C, koopaa,
+                           koopbb);
+        NV2_LAUNCH_CHECK();
+    }
+    return 0;
+}
 __global__ void k_koop0pf(Rdm R, const double *atwo, const double *F, double *koopab) {
     const int n = R.n, M = R.M;
     WAVE_SETUP((i64)M * n * n);
@@ -477,8 +592,12 @@ int launch_koop_rest(const Rdm &R, const double *atwo, Stage1Scratch &w, Stage1O
     NV2_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_koop2E, dim3(nblk(t4 * 64, 256)), dim3(256), 0, st, R, atwo, w.Fhalf, o.ipko2, o.eako2);
     NV2_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_koop0p, dim3(nblk(t4 * 64, 256)), dim3(256), 0, st, R, atwo, w.Fhalf, o.gk0pa, o.gk0pd);
-    NV2_LAUNCH_CHECK();
+    if (w.h && w.k0p) {
+        if (launch_koop0p_gemm(R, atwo, w.Fhalf, w.h, w.k0p, o.gk0pa, o.gk0pd, st)) return -1;
+    } else {
+        hipLaunchKernelGGL(k_koop0p, dim3(nblk(t4 * 64, 256)), dim3(256), 0, st, R, atwo, w.Fhalf, o.gk0pa, o.gk0pd);
+        NV2_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(k_koop0pf, dim3(nblk(t2 * 64, 256)), dim3(256), 0, st, R, atwo, w.Fhalf, o.gk0pf);
     NV2_LAUNCH_CHECK();
     if (o.bmat) {
