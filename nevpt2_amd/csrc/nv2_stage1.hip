@@ -287,6 +287,64 @@ __global__ void k_koop0p(Rdm R, const double *atwo, const double *F, double *koo
         koopbb[wid] = kb;
     }
 }
+// koop2E's two n^4 matrices as GEMMs (the same sums as k_koop2E): with atwo3[(d,ie,ief), a] =
+// atwo(d,ie,a,ief) (k_k0p_at3),
+//   C1[(ap,bp,b),a] = sum_(d,ie,ief) Y1 atwo(d,ie,a,ief),  Y1 = the kernel's first IP dum
+//   C2[(ap,bp,a),b] = sum_(d,ie,ief) Y2 atwo(d,ie,b,ief),  Y2 = its second IP dum
+//   C3[(ap,bp,b),a] = sum_(c,e2,dd) Z1 atwo(c,e2,dd,a),    Z1, Z2 = its EA dums
+//   C4[(ap,bp,a),b] = sum_(c,e2,dd) Z2 atwo(c,e2,dd,b)
+//   koopaa = fock_ip - C1 - C2,  koopeaa = fock_ea + C3 + C4
+__global__ void k_k2e_x(Rdm R, int s, int which, double *X) {
+    const int n = R.n;
+    const i64 n3 = (i64)n * n * n, e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n3 * n3) return;
+    const int r = (int)(e % n3), cl = (int)(e / n3);
+    const int ap = r % n, bp = (r / n) % n, r2 = r / (n * n);     // rows (ap,bp,b) or (ap,bp,a)
+    const int q0 = cl % n, q1 = (cl / n) % n, q2 = cl / (n * n);
+    double v;
+    if (which == 0) {                 // Y1, columns (d,ie,ief)
+        const int b = r2, d = q0, ie = q1, ief = q2;
+        v = D3(s, ap, bp, d, ief, b, ie);
+        if (d == ief) v = v + 0.5 * D2(s, ap, bp, ie, b);
+        if (d == b) v = v + 0.5 * D2(s, ap, bp, ief, ie);
+    } else if (which == 1) {          // Y2, columns (d,ie,ief)
+        const int a = r2, d = q0, ie = q1, ief = q2;
+        v = D3(s, ap, bp, d, a, ief, ie);
+        if (d == a) v = v + 0.5 * D2(s, ap, bp, ie, ief);
+        if (d == ief) v = v + 0.5 * D2(s, ap, bp, a, ie);
+    } else if (which == 2) {          // Z1, columns (c,e2,dd)
+        const int b = r2, c = q0, e2 = q1, dd = q2;
+        v = -ro3t(R, s, ap, bp, e2, dd, b, c);
+        if (c == e2) v = v + 2.0 * ro2t(R, s, ap, bp, dd, b);
+        if (dd == e2) v = v - 0.5 * ro2t(R, s, ap, bp, c, b);
+        if (b == e2) v = v - 0.5 * ro2t(R, s, ap, bp, dd, c);
+    } else {                          // Z2, columns (c,e2,dd)
+        const int a = r2, c = q0, e2 = q1, dd = q2;
+        v = -ro3t(R, s, ap, bp, e2, a, dd, c);
+        if (c == e2) v = v + 2.0 * ro2t(R, s, ap, bp, a, dd);
+        if (a == e2) v = v - 0.5 * ro2t(R, s, ap, bp, c, dd);
+        if (dd == e2) v = v - 0.5 * ro2t(R, s, ap, bp, a, c);
+    }
+    X[e] = v;
+}
+__global__ void k_k2e_combine(Rdm R, int s, const double *F, const double *C, double *koopaa, double *koopeaa) {
+    const int n = R.n, M = R.M;
+    const i64 n3 = (i64)n * n * n, n4 = n3 * n, e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n4) return;
+    const int ap = (int)(e % n), bp = (int)((e / n) % n), a = (int)((e / (n * n)) % n), b = (int)(e / n3);
+    const double *C1 = C, *C2 = C + n4, *C3 = C + 2 * n4, *C4 = C + 3 * n4;
+    double ip = 0.0, ea = 0.0;
+    for (int d = 0; d < n; d++) {     // the kernel's ie = ief = 0 (dd = e2 = 0) terms
+        ip = ip - FF(a, d) * D2(s, ap, bp, d, b) - FF(b, d) * D2(s, ap, bp, a, d);
+        ea = ea + FF(a, d) * ro2t(R, s, ap, bp, d, b) + FF(b, d) * ro2t(R, s, ap, bp, a, d);
+    }
+    const i64 rb = ap + n * (bp + (i64)n * b), ra = ap + n * (bp + (i64)n * a);
+    ip = ip - C1[rb + n3 * a] - C2[ra + n3 * b];
+    ea = ea + C3[rb + n3 * a] + C4[ra + n3 * b];
+    const i64 o = s + (i64)M * e;
+    koopaa[o] = ip;
+    koopeaa[o] = ea;
+}
 // koopman0pE's two n^4 matrices as GEMMs (the same sums as k_koop0p, qdnevpt.F koopman0pE): each of
 // its four n^7 terms is sum_{three indices} atwo(..) * X(..) with X built from the 2- and 3-RDM, so X
 // is materialised for one state as an n^3 x n^3 matrix and multiplied with atwo (n^3 x n); the O(n^5)
@@ -369,9 +427,9 @@ __global__ void k_k0p_combine(Rdm R, int s, const double *F, const double *C, do
     koopaa[o] = ka;
     koopbb[o] = kb;
 }
-i64 koop0p_workspace(int n) {
+i64 koop0p_workspace(int n) {   // one n^6 term matrix, atwo3, six n^4 products (bcmat's C1..C3t)
     const i64 n3 = (i64)n * n * n;
-    return n3 * n3 + 5 * n3 * n;
+    return n3 * n3 + 7 * n3 * n;
 }
 static int launch_koop0p_gemm(const Rdm &R, const double *atwo, const double *F, rocblas_handle h, double *work,
                               double *koopaa, double *koopbb, hipStream_t st) {
@@ -398,6 +456,30 @@ static int launch_koop0p_gemm(const Rdm &R, const double *atwo, const double *F,
         }
         hipLaunchKernelGGL(k_k0p_combine, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, st, R, s, F, C, koopaa,
                            koopbb);
+        NV2_LAUNCH_CHECK();
+    }
+    return 0;
+}
+static int launch_koop2e_gemm(const Rdm &R, const double *atwo, const double *F, rocblas_handle h, double *work,
+                              double *koopaa, double *koopeaa, hipStream_t st) {
+    const int n = R.n;
+    const i64 n3 = (i64)n * n * n, n4 = n3 * n, n6 = n3 * n3;
+    double *X = work, *AT3 = work + n6, *C = AT3 + n4;
+    if (rocblas_set_stream(h, st) != rocblas_status_success) { set_error("rocblas_set_stream failed"); return -1; }
+    hipLaunchKernelGGL(k_k0p_at3, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, st, n, atwo, AT3);
+    NV2_LAUNCH_CHECK();
+    const double one = 1.0, zero = 0.0;
+    const rocblas_operation Nn = rocblas_operation_none;
+    const int m3 = (int)n3;
+    for (int s = 0; s < R.M; s++) {
+        for (int which = 0; which < 4; which++) {
+            hipLaunchKernelGGL(k_k2e_x, dim3((unsigned)((n6 + 255) / 256)), dim3(256), 0, st, R, s, which, X);
+            NV2_LAUNCH_CHECK();
+            if (rocblas_dgemm(h, Nn, Nn, m3, n, m3, &one, X, m3, which < 2 ? AT3 : atwo, m3, &zero, C + which * n4, m3) !=
+                rocblas_status_success) { set_error("koop2E GEMM failed"); return -1; }
+        }
+        hipLaunchKernelGGL(k_k2e_combine, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, st, R, s, F, C, koopaa,
+                           koopeaa);
         NV2_LAUNCH_CHECK();
     }
     return 0;
@@ -486,6 +568,110 @@ __global__ void k_bcmat(Rdm R, const double *atwo, const double *F, double *cmat
         cmat[wid] = cv;
         ctmat[wid] = ct;
     }
+}
+
+// bbmat and bcmat as GEMMs (the same sums as k_bbmat, k_bcmat): the n^7 terms are products of n^3 x
+// n^3 term matrices with atwo, atwo^T or atwo3; the O(n^6) and O(n^5) terms and the index order go to
+// the combine kernels.
+//   bbmat:  Cb[(ap,bp,cp),a] = sum_(ie,c,ief) eee2(cp,ap,bp,ie,c,ief) atwo(a,ie,c,ief)  (and eee2t: Cbt)
+//   bcmat:  C1[(ap,b,c),a] = sum_(d,ief,ie) eee2(ap,b,d,ief,ie,c) atwo(d,ief,ie,a)
+//           C2[(ap,b,a),c] = sum_(d,ief,ie) eee2(ap,b,d,ief,a,ie) atwo(d,ief,c,ie)
+//           C3[(ap,a,c),b] = sum_(d,ief,ie) eee2(ap,ie,d,ief,a,c) atwo(d,ief,b,ie)   (and eee2tl: C1t..C3t)
+__global__ void k_bbc_x(Rdm R, int s, int which, double *X) {
+    const int n = R.n;
+    const i64 n3 = (i64)n * n * n, e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n3 * n3) return;
+    const int r = (int)(e % n3), cl = (int)(e / n3);
+    const int r0 = r % n, r1 = (r / n) % n, r2 = r / (n * n);
+    const int q0 = cl % n, q1 = (cl / n) % n, q2 = cl / (n * n);
+    double v;
+    switch (which) {
+        case 0: v = eee2(R, s, r2, r0, r1, q0, q1, q2); break;     // rows (ap,bp,cp), columns (ie,c,ief)
+        case 1: v = eee2t(R, s, r2, r0, r1, q0, q1, q2); break;
+        case 2: v = eee2(R, s, r0, r1, q0, q1, q2, r2); break;     // rows (ap,b,c), columns (d,ief,ie)
+        case 3: v = eee2tl(R, s, r0, r1, q0, q1, q2, r2); break;
+        case 4: v = eee2(R, s, r0, r1, q0, q1, r2, q2); break;     // rows (ap,b,a), columns (d,ief,ie)
+        case 5: v = eee2tl(R, s, r0, r1, q0, q1, r2, q2); break;
+        case 6: v = eee2(R, s, r0, q2, q0, q1, r1, r2); break;     // rows (ap,a,c), columns (d,ief,ie)
+        default: v = eee2tl(R, s, r0, q2, q0, q1, r1, r2); break;
+    }
+    X[e] = v;
+}
+__global__ void k_bb_combine(Rdm R, int s, const double *F, const double *AJ, const double *C, double *bmat,
+                             double *btmat) {
+    const int n = R.n, M = R.M;
+    const i64 n3 = (i64)n * n * n, n4 = n3 * n, e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n4) return;
+    const int ap = (int)(e % n), bp = (int)((e / n) % n), cp = (int)((e / (n * n)) % n), a = (int)(e / n3);
+    double bv = 0.0, bt = 0.0;
+    for (int c = 0; c < n; c++) {
+        bv = bv - FF(c, a) * ee2(R, s, cp, ap, bp, c);
+        bt = bt + AJ[c + n * a] * ee2tr(R, s, cp, ap, bp, c);
+    }
+    const i64 o = s + (i64)M * e;
+    bmat[o] = bv - C[e];
+    btmat[o] = bt + C[n4 + e];
+}
+__global__ void k_bc_combine(Rdm R, int s, const double *F, const double *atwo, const double *C, double *cmat,
+                             double *ctmat) {
+    const int n = R.n, M = R.M;
+    const i64 n3 = (i64)n * n * n, n4 = n3 * n, e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n4) return;
+    const int ap = (int)(e % n), a = (int)((e / n) % n), b = (int)((e / (n * n)) % n), c = (int)(e / n3);
+    double cv = 0.0, ct = 0.0;
+    for (int d = 0; d < n; d++) {
+        const double fda = FF(d, a), fcd = FF(c, d), fbd = FF(b, d);
+        cv = cv + fda * ee2(R, s, ap, b, d, c) - fcd * ee2(R, s, ap, b, a, d) - fbd * ee2(R, s, ap, d, a, c);
+        ct = ct + fda * ee2t(R, s, ap, b, d, c) - fcd * ee2t(R, s, ap, b, a, d) + fbd * ee2t(R, s, ap, d, a, c);
+        for (int ie = 0; ie < n; ie++) {   // the kernel's ief = 0 terms
+            const double dint1 = AT(c, ie, d, a), dint2 = AT(d, ie, ie, a), dint3 = AT(c, d, d, ie), dint4 = AT(b, d, d, ie);
+            const double dum1 = ee2(R, s, ap, b, d, ie), dum2 = ee2(R, s, ap, b, d, c);
+            const double dum3 = ee2(R, s, ap, b, a, ie), dum4 = ee2(R, s, ap, ie, a, c);
+            const double dum1t = ee2t(R, s, ap, b, d, ie), dum2t = ee2t(R, s, ap, b, d, c);
+            const double dum3t = ee2t(R, s, ap, b, a, ie), dum4t = ee2t(R, s, ap, ie, a, c);
+            cv = cv + dum1 * dint1 - 0.5 * (dum2 * dint2 + dum3 * dint3 - dum4 * dint4);
+            ct = ct + dum1t * dint1 - 0.5 * (dum2t * dint2 + dum3t * dint3 - dum4t * dint4);
+        }
+    }
+    const double *C1 = C, *C1t = C + n4, *C2 = C + 2 * n4, *C2t = C + 3 * n4, *C3 = C + 4 * n4, *C3t = C + 5 * n4;
+    const i64 i1 = (ap + n * (b + (i64)n * c)) + n3 * a, i2 = (ap + n * (b + (i64)n * a)) + n3 * c;
+    const i64 i3 = (ap + n * (a + (i64)n * c)) + n3 * b;
+    const i64 o = s + (i64)M * e;
+    cmat[o] = cv + C1[i1] - C2[i2] - C3[i3];
+    ctmat[o] = ct + C1t[i1] - C2t[i2] + C3t[i3];
+}
+static int launch_bbc_gemm(const Rdm &R, const double *atwo, const double *Fone, const double *Fhalf, const double *AJ,
+                           rocblas_handle h, double *work, Stage1Out &o, hipStream_t st) {
+    const int n = R.n;
+    const i64 n3 = (i64)n * n * n, n4 = n3 * n, n6 = n3 * n3;
+    double *X = work, *AT3 = work + n6, *C = AT3 + n4;
+    if (rocblas_set_stream(h, st) != rocblas_status_success) { set_error("rocblas_set_stream failed"); return -1; }
+    hipLaunchKernelGGL(k_k0p_at3, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, st, n, atwo, AT3);
+    NV2_LAUNCH_CHECK();
+    const double one = 1.0, zero = 0.0;
+    const rocblas_operation Nn = rocblas_operation_none, Tt = rocblas_operation_transpose;
+    const int m3 = (int)n3;
+    const unsigned g6 = (unsigned)((n6 + 255) / 256), g4 = (unsigned)((n4 + 255) / 256);
+    for (int s = 0; s < R.M; s++) {
+        for (int which = 0; which < 8; which++) {
+            hipLaunchKernelGGL(k_bbc_x, dim3(g6), dim3(256), 0, st, R, s, which, X);
+            NV2_LAUNCH_CHECK();
+            rocblas_status rs;
+            if (which < 2)            // B[(ie,c,ief), a] = atwo[a + n (ie,c,ief)]
+                rs = rocblas_dgemm(h, Nn, Tt, m3, n, m3, &one, X, m3, atwo, n, &zero, C + which * n4, m3);
+            else                      // bcmat: C1 with atwo, C2 and C3 with atwo3
+                rs = rocblas_dgemm(h, Nn, Nn, m3, n, m3, &one, X, m3, which < 4 ? atwo : AT3, m3, &zero,
+                                   C + (which - 2) * n4, m3);   // bbmat's two products are consumed by then
+            if (rs != rocblas_status_success) { set_error("bbmat/bcmat GEMM failed (%d)", (int)rs); return -1; }
+            if (which == 1) {
+                hipLaunchKernelGGL(k_bb_combine, dim3(g4), dim3(256), 0, st, R, s, Fone, AJ, C, o.bmat, o.btmat);
+                NV2_LAUNCH_CHECK();
+            }
+        }
+        hipLaunchKernelGGL(k_bc_combine, dim3(g4), dim3(256), 0, st, R, s, Fhalf, atwo, C, o.cmat, o.ctmat);
+        NV2_LAUNCH_CHECK();
+    }
+    return 0;
 }
 
 // ---------------------------------------------------------------- bdmat (koopro4QD.F90:4399-4459); F alpha=1
@@ -590,8 +776,12 @@ int launch_koop_rest(const Rdm &R, const double *atwo, Stage1Scratch &w, Stage1O
     NV2_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_hrdm, dim3(nblk(t4, 256)), dim3(256), 0, st, R, o.hrdm);
     NV2_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_koop2E, dim3(nblk(t4 * 64, 256)), dim3(256), 0, st, R, atwo, w.Fhalf, o.ipko2, o.eako2);
-    NV2_LAUNCH_CHECK();
+    if (w.h && w.k0p) {
+        if (launch_koop2e_gemm(R, atwo, w.Fhalf, w.h, w.k0p, o.ipko2, o.eako2, st)) return -1;
+    } else {
+        hipLaunchKernelGGL(k_koop2E, dim3(nblk(t4 * 64, 256)), dim3(256), 0, st, R, atwo, w.Fhalf, o.ipko2, o.eako2);
+        NV2_LAUNCH_CHECK();
+    }
     if (w.h && w.k0p) {
         if (launch_koop0p_gemm(R, atwo, w.Fhalf, w.h, w.k0p, o.gk0pa, o.gk0pd, st)) return -1;
     } else {
@@ -600,7 +790,11 @@ int launch_koop_rest(const Rdm &R, const double *atwo, Stage1Scratch &w, Stage1O
     }
     hipLaunchKernelGGL(k_koop0pf, dim3(nblk(t2 * 64, 256)), dim3(256), 0, st, R, atwo, w.Fhalf, o.gk0pf);
     NV2_LAUNCH_CHECK();
-    if (o.bmat) {
+    if (o.bmat && w.h && w.k0p) {
+        if (launch_bbc_gemm(R, atwo, w.Fone, w.Fhalf, w.AJ, w.h, w.k0p, o, st)) return -1;
+        hipLaunchKernelGGL(k_bdmat, dim3(nblk(t2 * 64, 256)), dim3(256), 0, st, R, atwo, w.Fone, w.AJ, o.dmat, o.dtmat);
+        NV2_LAUNCH_CHECK();
+    } else if (o.bmat) {
         hipLaunchKernelGGL(k_bbmat, dim3(nblk(t4 * 64, 256)), dim3(256), 0, st, R, atwo, w.Fone, w.AJ, o.bmat, o.btmat);
         NV2_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_bcmat, dim3(nblk(t4 * 64, 256)), dim3(256), 0, st, R, atwo, w.Fhalf, o.cmat, o.ctmat);
