@@ -710,6 +710,41 @@ __global__ void k_pm2_pc(const double *I, i64 ld, const int2 *pairs, i64 npairs,
     block_store4(0.0, 0.0, E, P, part);
     if (q.M) block_storeQ(qsc, qpc, q);
 }
+// The same sums as k_pm2_pc as one GEMM: the pairs' n^2 integral blocks gathered into Vt (n^2 x npairs,
+// ab fastest), R = Vt^T spc (npairs x nnew, rocBLAS), then one thread per (pair, mu) for the
+// denominators; QD partners get R_j = Vt^T s_j the same way (their pointers replace q.a[j]).
+__global__ void k_pm2_gather(const double *I, i64 ld, const int2 *pairs, i64 npairs, int mode, int nc, int n,
+                             double *Vt) {
+    const int nn = n * n;
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npairs * nn) return;
+    const int ab = (int)(t % nn), a = ab / n, b = ab % n;
+    const int2 pr = pairs[t / nn];
+    const int x = pr.x, y = pr.y;
+    Vt[t] = mode == 0 ? I[(x + (i64)nc * b) + ld * (y + (i64)nc * a)] : I[(b + (i64)n * x) + ld * (a + (i64)n * y)];
+}
+__global__ void k_pm2_denom(const double *R, const int2 *pairs, i64 npairs, int mode, const double *w, int nnew,
+                            const double *eo, double *part, QDArgs q) {
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    double E = 0.0, P = 0.0, qsc[QM], qpc[QM];
+#pragma unroll
+    for (int j = 0; j < QM; j++) { qsc[j] = 0.0; qpc[j] = 0.0; }
+    if (t < npairs * nnew) {
+        const i64 pi = t % npairs;
+        const int mu = (int)(t / npairs);
+        const int2 pr = pairs[pi];
+        const double acc = R[t];
+        const double deno = mode == 0 ? w[mu] - eo[pr.x] - eo[pr.y] : w[mu] + eo[pr.x] + eo[pr.y];
+        const double contri = acc * acc / deno;
+        E = -contri;
+        P = contri / deno;
+#pragma unroll
+        for (int j = 0; j < QM; j++)
+            if (qd_on(q, j)) qpc[j] = -q.a[j][t] * acc / deno;
+    }
+    block_store4(0.0, 0.0, E, P, part);
+    if (q.M) block_storeQ(qsc, qpc, q);
+}
 // diagonal pairs (i=j or r=s) with the n(n+1)/2 metric
 __global__ void k_pm2p_pc(const double *I, i64 ld, int next, int mode, int nc, int n, const double *spc,
                           const double *w, int nnew, const double *eo, double *part, QDArgs q) {
@@ -2810,13 +2845,26 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             }
             if (nprs && nnew) {
                 const i64 tot = nprs * nnew;
-                double *pt = parts(p, nb(tot, 128));
-                TRYS(qd_prepare(p, q, nb(tot, 128)));
-                hipLaunchKernelGGL(k_pm2_pc, dim3(nb(tot, 128)), dim3(128), 0, st, I, ld, prs, nprs, mode, nc, n, spc,
-                                   w2, nnew, eo, pt, q);
+                GET(Vt, "pm2_Vt", n2 * nprs);
+                GET(Rp, "pm2_R", tot);
+                hipLaunchKernelGGL(k_pm2_gather, dim3(nb(n2 * nprs, 256)), dim3(256), 0, st, I, ld, prs, nprs, mode, nc,
+                                   n, Vt);
                 NV2_LAUNCH_CHECK();
-                TRYS(reduce_into(p, nb(tot, 128), cls));
-                TRYS(qd_reduce(p, q, nb(tot, 128), cls));
+                TRYS(gemm(p, true, false, (int)nprs, nnew, (int)n2, 1.0, Vt, n2, spc, n2, 0.0, Rp, nprs));
+                QDArgs qr = q;
+                for (int j = 0; q.M && j < M; j++) {
+                    if (j == s) continue;
+                    QGET(rj, "qd_pm2r_", j, tot);
+                    TRYS(gemm(p, true, false, (int)nprs, nnew, (int)n2, 1.0, Vt, n2, q.a[j], n2, 0.0, rj, nprs));
+                    qr.a[j] = rj;
+                }
+                double *pt = parts(p, nb(tot, 256));
+                TRYS(qd_prepare(p, qr, nb(tot, 256)));
+                hipLaunchKernelGGL(k_pm2_denom, dim3(nb(tot, 256)), dim3(256), 0, st, Rp, prs, nprs, mode, w2, nnew, eo,
+                                   pt, qr);
+                NV2_LAUNCH_CHECK();
+                TRYS(reduce_into(p, nb(tot, 256), cls));
+                TRYS(qd_reduce(p, qr, nb(tot, 256), cls));
             }
         }
         {   // diagonal external pairs: n(n+1)/2 metric
