@@ -18,10 +18,8 @@ i64 band_workspace(int N, int nb);
 // and the band of B_b in work.  dflag[b] becomes nonzero when a panel's CholeskyQR2 loses
 // positive definiteness (the caller then reduces that matrix some other way).  Asynchronous.
 int band_reduce(rocblas_handle h, hipStream_t st, int N, double *A, i64 sA, int nb, double *work, int *dflag);
-// X <- Q_b^T X for problem b after band_reduce (X: N x ncol, ld ldx); scratch: band_apply_workspace(ncol)
-i64 band_apply_workspace(int ncol);
-int band_apply_qt(rocblas_handle h, hipStream_t st, int N, int nb, int b, const double *work, int ncol, double *X,
-                  i64 ldx, double *scratch);
+// G <- G Q_b (G: N x N, ld N): with G = caux, G^T x = Q_b^T caux^T x (band_reduce's workspace scratch)
+int band_apply_right(rocblas_handle h, hipStream_t st, int N, int nb, int b, double *work, double *G);
 // the rows of B_b: entry (r, c), |c - r| <= BAND_B, at [r * BAND_W + (c - r + BAND_B)] (zero outside 0..N-1)
 const double *band_rows(int N, int nb, int b, const double *work);
 

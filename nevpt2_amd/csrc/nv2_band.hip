@@ -459,37 +459,31 @@ int band_reduce(rocblas_handle h, hipStream_t st, int N, double *A, i64 sA, int 
     return 0;
 }
 
-__global__ void k_band_copy(const double *S, int m, int ncol, double *X, i64 ldx) {   // X[0:m, :] = S (ld BB)
-    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (i64)m * ncol) return;
-    X[(t % m) + ldx * (t / m)] = S[(t % m) + (i64)BB * (t / m)];
-}
-
-i64 band_apply_workspace(int ncol) { return (i64)2 * BB * (ncol > 0 ? ncol : 1); }
-
-int band_apply_qt(rocblas_handle h, hipStream_t st, int N, int nb, int b, const double *work, int ncol, double *X,
-                  i64 ldx, double *scratch) {
+// G <- G Q for problem b after band_reduce (G: N x N, ld N, overwritten), Q = H_0 H_1 ... the
+// reduction's reflectors: G <- G H_k is G[:, r0:] -= (G[:, r0:] Y_k) T_k Y_k^T, in panel order, then
+// the last block's explicit Qm.  With G = caux, G^T x = Q^T caux^T x is the right-hand side of the band
+// solves in one GEMM.
+int band_apply_right(rocblas_handle h, hipStream_t st, int N, int nb, int b, double *work, double *G) {
     (void)nb;
-    if (ncol <= 0) return 0;
     RBB(rocblas_set_stream(h, st));
     const BandLayout L = band_layout(N);
-    const double *w0 = work + b * L.size;
-    double *S1 = scratch, *S2 = scratch + (i64)BB * ncol;
+    double *w0 = work + b * L.size;
+    double *W = w0 + L.W, *W2 = w0 + L.W2, *Tmp = w0 + L.Q1;
     const double one = 1.0, zero = 0.0, mone = -1.0;
     const rocblas_operation Nn = rocblas_operation_none, Tt = rocblas_operation_transpose;
-    for (int k = 0; k < L.npan; k++) {          // X[r0:] -= Y_k T_k^T Y_k^T X[r0:], in panel order
+    for (int k = 0; k < L.npan; k++) {
         const int r0 = BB * (k + 1), m = N - r0;
         const double *Yk = w0 + L.Y + (i64)k * BB * N, *Tk = w0 + L.T + (i64)k * BB * BB;
-        double *Xr = X + r0;
-        RBB(rocblas_dgemm(h, Tt, Nn, BB, ncol, m, &one, Yk, N, Xr, (int)ldx, &zero, S1, BB));
-        RBB(rocblas_dgemm(h, Tt, Nn, BB, ncol, BB, &one, Tk, BB, S1, BB, &zero, S2, BB));
-        RBB(rocblas_dgemm(h, Nn, Nn, m, ncol, BB, &mone, Yk, N, S2, BB, &one, Xr, (int)ldx));
+        double *Gr = G + (i64)N * r0;
+        RBB(rocblas_dgemm(h, Nn, Nn, N, BB, m, &one, Gr, N, Yk, N, &zero, W, N));
+        RBB(rocblas_dgemm(h, Nn, Nn, N, BB, BB, &one, W, N, Tk, BB, &zero, W2, N));
+        RBB(rocblas_dgemm(h, Nn, Tt, N, m, BB, &mone, W2, N, Yk, N, &one, Gr, N));
     }
     const int r0 = BB * (L.npan + 1), m = N - r0;
-    if (m >= 1) {                                // X[r0:] <- Qm^T X[r0:]
-        RBB(rocblas_dgemm(h, Tt, Nn, m, ncol, m, &one, w0 + L.Ql, BB, X + r0, (int)ldx, &zero, S1, BB));
-        hipLaunchKernelGGL(k_band_copy, dim3(nbl((i64)m * ncol, 256)), dim3(256), 0, st, S1, m, ncol, X + r0, ldx);
-        NV2_LAUNCH_CHECK();
+    if (m >= 1) {
+        RBB(rocblas_dgemm(h, Nn, Nn, N, m, m, &one, G + (i64)N * r0, N, w0 + L.Ql, BB, &zero, Tmp, N));
+        NV2_HIP(hipMemcpy2DAsync(G + (i64)N * r0, sizeof(double) * N, Tmp, sizeof(double) * N, sizeof(double) * N, m,
+                                 hipMemcpyDeviceToDevice, st));
     }
     return 0;
 }

@@ -1934,6 +1934,8 @@ static int pc_kprime_launch(Stage2 *p, PcJob &j, double thr) {
                 if (stage2_timing()) fprintf(stderr, "[K' %s N=%d] band reduction %s\n", j.tag.c_str(), N,
                                              bok ? "done" : "failed (CholeskyQR2): tridiagonal reduction");
                 if (bok) {
+                    // caux <- caux Q: the sums' projections G^T x then give Q^T caux^T x in one GEMM
+                    for (int b = 0; b < j.nb; b++) TRYS(band_apply_right(h, st, N, j.nb, b, j.bandw, j.caux + b * NN));
                     j.band = true;
                     p->kr_recorded = true;
                     p->kr_N = N;
@@ -2025,7 +2027,7 @@ struct Pm1pEarly {
     bool sums_early = false, x_ready = false, sums_done = false;
     const double *X[2] = {nullptr, nullptr}, *eo[2] = {nullptr, nullptr};
     int next[2] = {0, 0};
-    double *Ypc[2] = {nullptr, nullptr}, *Wq = nullptr, *Ub = nullptr, *yb = nullptr, *ptb = nullptr;
+    double *Ypc[2] = {nullptr, nullptr}, *Ub = nullptr, *yb = nullptr, *ptb = nullptr;
     int rc = 0;
     char err[512] = "";
     PcJob job;
@@ -2058,8 +2060,8 @@ static int pm1p_band_sums(Stage2 *p, Pm1pEarly *E) {
     for (int plus = 0; plus < 2; plus++) {
         const int next = E->next[plus], bb = plus ? 0 : 1;
         if (!next || !E->X[plus]) continue;
+        // z = Q^T caux^T x in one GEMM (caux holds caux Q on the band form)
         TRYS(gemm_on(h, st, true, false, N, next, N, 1.0, jb.caux + bb * NN, N, E->X[plus], N, 0.0, E->Ypc[plus], N));
-        TRYS(band_apply_qt(h, st, N, jb.nb, bb, jb.bandw, next, E->Ypc[plus], N, E->Wq));
         BandCls &c = bc[plus];
         c.Br = band_rows(N, jb.nb, bb, jb.bandw);
         c.Z = E->Ypc[plus];
@@ -2146,11 +2148,10 @@ int stage2_pm1p_begin(Stage2 *p, int s, double thr) {
     }
     E->sums_early = p->M == 1 && band_kprime((int)N3) && chol_metric() && (p->nc + p->nv) > 0;
     if (E->sums_early) {              // the sums' buffers, allocated here on the calling thread
-        const i64 nx = (i64)p->nc + p->nv, nmx = p->nc > p->nv ? p->nc : p->nv;
+        const i64 nx = (i64)p->nc + p->nv;
         GET(y0, "pm1_Ypc0", N3 * (p->nv ? p->nv : 1)); GET(y1, "pm1_Ypc1", N3 * (p->nc ? p->nc : 1));
-        GET(wq, "pm1_bandq", band_apply_workspace((int)nmx));
         GET(ub, "pm1_bandU", nx * N3 * UR); GET(yb, "pm1_bandy", nx * N3); GET(pb, "pm1_bandparts", 4 * nx);
-        E->Ypc[0] = y0; E->Ypc[1] = y1; E->Wq = wq; E->Ub = ub; E->yb = yb; E->ptb = pb;
+        E->Ypc[0] = y0; E->Ypc[1] = y1; E->Ub = ub; E->yb = yb; E->ptb = pb;
     }
     E->x_ready = false;
     E->sums_done = false;
@@ -2973,7 +2974,7 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         const double *caux = jb.caux + bb * NN, *Kp = jb.Kp + bb * NN;
         const int ncq = 1 + (q.M ? M - 1 : 0);              // right-hand sides: this state, then the QD partners
         GET(Ypc, plus ? "pm1_Ypc1" : "pm1_Ypc0", (i64)nmax * next * ncq);
-        // y = caux^T x (on the Cholesky path caux = U^-1 and nmax = N3)
+        // y = caux^T x (on the Cholesky path caux = U^-1 and nmax = N3; on the band form caux Q, so y = z)
         auto project = [&](const double *x, double *y) -> int {
             return gemm(p, true, false, nmax, next, (int)N3, 1.0, caux, N3, x, N3, 0.0, y, nmax);
         };
@@ -2991,9 +2992,7 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             TRYS(project(xj, yj));
             q.a[j] = yj;
         }
-        if (jb.band) {                   // z = Q^T y; the banded solves follow the loop
-            GET(Wq, "pm1_bandq", band_apply_workspace(next * ncq));
-            TRYS(band_apply_qt(p->h, st, nmax, jb.nb, bb, jb.bandw, next * ncq, Ypc, nmax, Wq));
+        if (jb.band) {                   // caux holds caux Q: Ypc is already z; the banded solves follow the loop
             BandCls &c = bc[plus];
             c.Br = band_rows(nmax, jb.nb, bb, jb.bandw);
             c.Z = Ypc;
