@@ -160,7 +160,7 @@ int nv2_bamat(int64_t nact, int64_t ncore, int64_t norb, int64_t metat, const do
     TRY(check_dims(nact, metat));
     if (prec != NV2_PREC_FP64 && prec != NV2_PREC_FP32) { set_error("nv2_bamat: unsupported precision %d", prec); return -2; }
     const i64 n = nact, M = metat, np = norb * (norb + 1) / 2;
-    DBuf dat, dro4, d3, d2, d1, daj, Fh, E4, C[4], dam, dat6, wk;
+    DBuf dat, dro4, d3, d2, d1, daj, Fh, E4, C[4], dam, dat6, wk, d3t;
     TRY(dat.up(atwo, sizeof(double) * p(n, 4)));
     TRY(dro4.up(ro4, sizeof(double) * nwords(n) * p(n, 4) * M));
     TRY(d3.up(daaa, sizeof(double) * M * p(n, 6)));
@@ -173,6 +173,7 @@ int nv2_bamat(int64_t nact, int64_t ncore, int64_t norb, int64_t metat, const do
     TRY(dam.alloc(sizeof(double) * M * p(n, 6)));
     TRY(dat6.alloc(sizeof(double) * M * p(n, 6)));
     TRY(wk.alloc(sizeof(double) * bamat_lo_workspace((int)n)));
+    TRY(d3t.alloc(sizeof(double) * p(n, 6)));
     Rdm R{(int)n, (int)M, nwords(n), d3.d(), d2.d(), d1.d(), dro4.d()};
     TRY(launch_make_f((int)n, (int)ncore, daj.d(), dat.d(), 0.5, Fh.d(), 0));
     ContractPlan *plan = contract_plan_create((int)n, (int)M);
@@ -180,7 +181,7 @@ int nv2_bamat(int64_t nact, int64_t ncore, int64_t norb, int64_t metat, const do
     if (rocblas_create_handle(&h) != rocblas_status_success) { contract_plan_destroy(plan); set_error("rocblas_create_handle failed"); return -1; }
     int rc = plan ? 0 : -1;
     for (int s = 0; s < M && !rc; s++) {
-        rc = launch_build_e4(R, s, E4.p, prec, 0);
+        rc = launch_build_e4(R, s, E4.p, prec, d3t.d(), 0);
         if (!rc) rc = launch_e4_contractions(plan, E4.p, prec, dat.d(), s, C[0].d(), C[1].d(), C[2].d(), C[3].d(), 0);
         if (!rc) rc = launch_bamat_lo(h, R, s, dat.d(), Fh.d(), wk.d(), C[0].d(), C[1].d(), C[2].d(), C[3].d(), dam.d(),
                                       dat6.d(), 0);

@@ -122,6 +122,7 @@ nv2_ctx *nv2_ctx_create(int64_t ncore, int64_t nact, int64_t nvirt, int64_t meta
     }
     TRYC(ctx_alloc(c, "bamat_work", (size_t)bamat_lo_workspace((int)n)));
     TRYC(ctx_alloc(c, "koop0p_work", (size_t)koop0p_workspace((int)n)));
+    TRYC(ctx_alloc(c, "e4_d3t", n6));
     c->e4_bytes = sizeof(double) * (size_t)(n4 * n4);
     if (hipMalloc(&c->E4, c->e4_bytes) != hipSuccess) { set_error("hipMalloc(E4) failed"); nv2_ctx_destroy(c); return nullptr; }
     c->plan = contract_plan_create((int)n, (int)M);
@@ -218,7 +219,7 @@ int nv2_ctx_run(nv2_ctx *c, int32_t prec, double thr_lindep, double *out) {
     for (int s = 0; s < M; s++) {
         hipEvent_t a = c->ev[7], b = c->ev[8], d = c->ev[9];
         NV2_HIP(hipEventRecord(a, st));
-        if (launch_build_e4(R, s, c->E4, prec, st)) return -1;
+        if (launch_build_e4(R, s, c->E4, prec, c->find("e4_d3t"), st)) return -1;
         NV2_HIP(hipEventRecord(b, st));
         if (launch_e4_contractions(c->plan, c->E4, prec, c->find("atwo"), s, c->find("C1"), c->find("C2"),
                                    c->find("C3"), c->find("C4"), st)) return -1;
