@@ -1695,7 +1695,7 @@ struct PcJob {
     double *trdw = nullptr;                    // K' reduction scratch; T and tau of problem b from trd_outputs(nmax)
     double *trdm = nullptr, *trde = nullptr;   // eig_trd: reduction scratch, off-diagonal copy for dstedc
     double *trdq = nullptr;                    // scratch of applying Q
-    double *cholw = nullptr;                   // Cholesky scratch (trd jobs); T and caux are adjacent there
+    double *cholw = nullptr;                   // Cholesky scratch (trd jobs)
     double *S = nullptr, *K = nullptr;         // nb consecutive N x N matrices; S is overwritten
     double *w1 = nullptr, *hw1 = nullptr, *coef = nullptr, *w2 = nullptr;
     double *caux = nullptr, *Ks = nullptr, *T = nullptr, *Kp = nullptr;   // nb x N x N scratch each
@@ -1713,7 +1713,7 @@ static int pc_job_buffers(Stage2 *p, PcJob &j, size_t hoff) {
     GET(w1, (j.tag + "_w1").c_str(), N * j.nb); GET(w2, (j.tag + "_w2").c_str(), N * j.nb);
     GET(coef, (j.tag + "_coef").c_str(), NN);
     GET(Ks, (j.tag + "_Ks").c_str(), NN); GET(Kp, (j.tag + "_Kp").c_str(), NN);
-    GET(Tc, (j.tag + "_Tcaux").c_str(), 2 * NN);   // T then caux: one batch for the two factorisations
+    GET(Tc, (j.tag + "_Tcaux").c_str(), 2 * NN);   // T then caux
     j.w1 = w1; j.w2 = w2; j.coef = coef; j.caux = Tc + NN; j.Ks = Ks; j.T = Tc; j.Kp = Kp;
     if (j.trd) {
         GET(cw, (j.tag + "_cholw").c_str(), chol_workspace(j.N, 2 * j.nb));
@@ -1839,6 +1839,12 @@ static bool chol_metric() {   // read at every solve, so that a caller can switc
     const char *e = getenv("NV2_METRIC_EIGEN");
     return !(e && atoi(e) > 0);
 }
+// NV2_CHOL_EXACT=1: skip the trace test, so that every Cholesky-path metric takes the exact test
+// (S - thr*I factored)
+static bool chol_exact_only() {
+    const char *e = getenv("NV2_CHOL_EXACT");
+    return e && atoi(e) > 0;
+}
 // K' of the Cholesky path reduced to band form or to tridiagonal form.  The band form wins at the
 // measured orders (per evaluation: CAS(14,14), N = 2744, 136 vs 165 ms; CAS(12,12) x 6 roots, N = 1728,
 // 518 vs 556 ms; CAS(10,10), N = 1000, 22.6 vs 25.4 ms); below N = 512 the tridiagonal form's short
@@ -1855,23 +1861,55 @@ __global__ void k_shift_diag(double *A, int N, i64 sA, int nbm, double d) {
     const i64 b = t / N, i = t % N;
     A[b * sA + i + (i64)N * i] += d;
 }
+// Partial sums of squares of the upper triangle of the nb matrices U (order N, ld N): workgroup
+// (x, b) covers columns x, x + gridDim.x, ..; part[b * gridDim.x + x] receives its sum.
+__global__ __launch_bounds__(256) void k_upper_sq_parts(const double *U, int N, i64 sU, double *part) {
+    __shared__ double red[4];
+    const int b = blockIdx.y;
+    const double *Ub = U + b * sU;
+    double acc = 0.0;
+    for (int c = blockIdx.x; c < N; c += gridDim.x)
+        for (int r = threadIdx.x; r <= c; r += 256) {
+            const double v = Ub[r + (i64)N * c];
+            acc += v * v;
+        }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[b * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+// info[b] = 0 when trace(S_b^-1) = ||U_b^-1||_F^2 (the sum of the np parts) is below lim = 1/thr: then
+// 1/lambda_min(S_b) <= trace(S_b^-1) < 1/thr, every eigenvalue is above thr.  1 otherwise (NaN included).
+__global__ __launch_bounds__(256) void k_trace_test(const double *part, int np, double lim, int *info) {
+    __shared__ double red[4];
+    const int b = blockIdx.x;
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < np; k += 256) acc += part[b * np + k];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) info[b] = (red[0] + red[1] + red[2] + red[3] < lim) ? 0 : 1;
+}
+#define TRACE_PARTS 512
 static int pc_metric_launch(Stage2 *p, PcJob &j, double thr) {
-    if (j.trd && j.N >= 3 && chol_metric()) {     // info words: 4*slot + b for S - thr*I, + nb + b for S
+    if (j.trd && j.N >= 3 && chol_metric()) {     // info words: 4*slot + b for the eigenvalue test, + nb + b for S
         j.chol = true;
         hipStream_t st = p->ws[j.w];
         rocblas_handle h = p->wh[j.w];
         const i64 NN = j.NN();
-        NV2_HIP(hipMemcpyAsync(j.T, j.S, sizeof(double) * NN * j.nb, hipMemcpyDeviceToDevice, st));
         NV2_HIP(hipMemcpyAsync(j.caux, j.S, sizeof(double) * NN * j.nb, hipMemcpyDeviceToDevice, st));
-        hipLaunchKernelGGL(k_shift_diag, dim3(nb((i64)j.N * j.nb, 256)), dim3(256), 0, st, j.T, j.N, NN, j.nb, -thr);
-        NV2_LAUNCH_CHECK();
         EigTiming tm(j.N);
         tm.mark(0, st);
-        // T_b (S - thr*I) and caux_b (S) factored as one batch: info words b and nb + b
-        TRYS(chol_upper(h, st, j.N, j.T, NN, 2 * j.nb, p->dinfo_w + 4 * j.slot, j.cholw));
-        // U^-1 into T (whose factor of S - thr*I has served its purpose: only its info word is read),
-        // while K is not there yet; K' then takes two GEMMs
+        // caux_b = U_b with S_b = U_b^T U_b: info words nb + b
+        TRYS(chol_upper(h, st, j.N, j.caux, NN, j.nb, p->dinfo_w + 4 * j.slot + j.nb, j.cholw));
+        // U^-1 into T while K is not there yet (K' then takes two GEMMs), and from it the test that every
+        // eigenvalue of S is above thr: info words b (pc_kprime_launch factors S - thr*I when it fails)
         TRYS(chol_inverse_upper(h, st, j.N, j.caux, NN, j.nb, j.T, NN, j.cholw));
+        hipLaunchKernelGGL(k_upper_sq_parts, dim3(TRACE_PARTS, j.nb), dim3(256), 0, st, j.T, j.N, NN, j.cholw);
+        NV2_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_trace_test, dim3(j.nb), dim3(256), 0, st, j.cholw, TRACE_PARTS, chol_exact_only() ? 0.0 : 1.0 / thr,
+                           p->dinfo_w + 4 * j.slot);
+        NV2_LAUNCH_CHECK();
         for (int k = 1; k < 4; k++) tm.mark(k, st);
         tm.report(j.N, j.nb);            // the factorisations: the first figure
         NV2_HIP(hipMemcpyAsync(p->hinfo + 4 * j.slot, p->dinfo_w + 4 * j.slot, sizeof(int) * 4, hipMemcpyDeviceToHost,
@@ -1887,14 +1925,29 @@ static int pc_metric_launch(Stage2 *p, PcJob &j, double thr) {
 static int pc_kprime_launch(Stage2 *p, PcJob &j, double thr) {
     const int N = j.N;
     if (j.chol) {                        // the factorisations of pc_metric_launch have finished
-        bool ok = true;
+        bool ok = true, exact = false;
         for (int b = 0; b < j.nb; b++) {
-            if (p->hinfo[4 * j.slot + b] != 0 || p->hinfo[4 * j.slot + j.nb + b] != 0) ok = false;
+            if (p->hinfo[4 * j.slot + j.nb + b] != 0) ok = false;       // S itself not positive definite
+            if (p->hinfo[4 * j.slot + b] != 0) exact = true;            // the trace test left it open
             p->hinfo[4 * j.slot + b] = p->hinfo[4 * j.slot + j.nb + b] = 0;
         }
         hipStream_t st = p->ws[j.w];
         rocblas_handle h = p->wh[j.w];
         const i64 NN = j.NN();
+        if (ok && exact) {
+            // the exact test: S - thr*I positive definite, factored in the (still unused) K' buffer
+            NV2_HIP(hipMemcpyAsync(j.Kp, j.S, sizeof(double) * NN * j.nb, hipMemcpyDeviceToDevice, st));
+            hipLaunchKernelGGL(k_shift_diag, dim3(nb((i64)N * j.nb, 256)), dim3(256), 0, st, j.Kp, N, NN, j.nb, -thr);
+            NV2_LAUNCH_CHECK();
+            TRYS(chol_upper(h, st, N, j.Kp, NN, j.nb, p->dinfo_w + 4 * j.slot, j.cholw));
+            NV2_HIP(hipMemcpyAsync(p->hinfo + 4 * j.slot, p->dinfo_w + 4 * j.slot, sizeof(int) * j.nb,
+                                   hipMemcpyDeviceToHost, st));
+            NV2_HIP(hipStreamSynchronize(st));
+            for (int b = 0; b < j.nb; b++) {
+                if (p->hinfo[4 * j.slot + b] != 0) ok = false;
+                p->hinfo[4 * j.slot + b] = 0;
+            }
+        }
         if (stage2_timing() && N >= 500) fprintf(stderr, "[metric %s N=%d] Cholesky path: %s\n", j.tag.c_str(), N,
                                                  ok ? "every eigenvalue above thr" : "not positive definite above thr");
         if (ok) {

@@ -58,3 +58,27 @@ def test_band_form_matches_tridiagonal_form(gpu, n, nele, nc, nv, seed, M):
         for a, b in zip(q_band, q_trd):
             assert np.allclose(a, b, rtol=1e-10, atol=1e-11)
     assert chol.all(), "the Cholesky path (and with it the band form) was not taken"
+
+
+@pytest.mark.gpu
+def test_metric_eigenvalue_test_paths_agree(gpu):
+    """The Cholesky path needs every eigenvalue of the (+-1)' metrics above thr.  The default test
+    reads trace(S^-1) = ||U^-1||_F^2 < 1/thr off the inverse factor the path forms anyway; when that
+    bound is not met the library factors S - thr*I (the exact test).  NV2_CHOL_EXACT=1 sends every
+    metric through the exact test: both must take the Cholesky path and give the same energies."""
+    n, nele, nc, nv, seed = 7, 6, 2, 5, 3
+    print(f"[metric test] nact={n} nele={nele} ncore={nc} nvirt={nv} seed={seed}", flush=True)
+    P = synth.Problem(n, nc, nv, seed=seed, nele=nele, nroots=1, host_rdms=False)
+    ev = Evaluator.for_problem(P, rdms="gpu")
+    try:
+        os.environ.pop("NV2_CHOL_EXACT", None)
+        t_trace = ev.run()
+        c_trace = ev.pc_metric()[:, 5:7, 2]
+        os.environ["NV2_CHOL_EXACT"] = "1"
+        t_exact = ev.run()
+        c_exact = ev.pc_metric()[:, 5:7, 2]
+    finally:
+        os.environ.pop("NV2_CHOL_EXACT", None)
+        ev.close()
+    assert c_trace.all() and c_exact.all(), (c_trace, c_exact)
+    assert np.allclose(t_trace, t_exact, rtol=1e-13, atol=1e-14)
