@@ -67,8 +67,11 @@ template <typename V> __device__ inline V vzero() { V z = {0, 0, 0, 0}; return z
 // keep one accumulator tile per x4 in registers.  Wavefronts 0 and 1 split the K range of C1, 2 and
 // 3 that of C2 (their halves are added through LDS at the end), and all four share the C3 row tiles,
 // so every wavefront issues about the same number of MFMAs per chunk.
+// amdgpu_waves_per_eu(2): two workgroups per CU (the compiler otherwise takes 284 VGPR + AGPR per lane at
+// n = 14 in FP64, one wavefront per SIMD, and every barrier and load of a chunk goes unhidden; with the
+// hint it fits 217 without spilling, and two 70 KB LDS footprints fit a CU)
 template <typename T, int NN>
-__global__ __launch_bounds__(256) void k_e4_pass_a(const T *__restrict__ E4, const double *__restrict__ B1,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_e4_pass_a(const T *__restrict__ E4, const double *__restrict__ B1,
                                                    const double *__restrict__ B2, const double *__restrict__ B3,
                                                    int M, int s, double *C1, double *C2, double *C3) {
     constexpr int N2 = NN * NN, N3 = N2 * NN;
