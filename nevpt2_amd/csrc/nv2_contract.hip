@@ -104,6 +104,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         const int e = threadIdx.x + 256 * u;
         pf[u] = e < N3 ? slab[e] : (T)0;          // chunk (x4 = 0, x5 = 0)
     }
+    // and the B3 slice of the next chunk (N2 <= 256 values: one per thread)
+    static_assert(N2 <= 256, "one B3 value per thread");
+    T pb3 = threadIdx.x < N2 ? (T)B3[threadIdx.x] : (T)0;
     for (int x5 = 0; x5 < NN; x5++) {
 #pragma unroll
         for (int x4 = 0; x4 < NN; x4++) {
@@ -118,11 +121,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                     b1s[e] = (T)B1[(i64)x5 * N3 + e];
                     b2s[e] = (T)B2[(i64)x5 * N3 + e];
                 }
-            for (int e = threadIdx.x; e < N2; e += 256) b3s[e] = (T)B3[((i64)x4 * N2 + x5 * NN) * NN + e];
+            if (threadIdx.x < N2) b3s[threadIdx.x] = pb3;
             __syncthreads();
             {                                  // prefetch the next chunk in visiting order
                 const int nx4 = x4 + 1 < NN ? x4 + 1 : 0, nx5 = x4 + 1 < NN ? x5 : x5 + 1;
                 if (nx5 < NN) {
+                    if (threadIdx.x < N2) pb3 = (T)B3[((i64)nx4 * N2 + nx5 * NN) * NN + threadIdx.x];
                     const T *nxt = slab + (i64)(nx4 * NN + nx5) * N3;
 #pragma unroll
                     for (int u = 0; u < PF; u++) {
