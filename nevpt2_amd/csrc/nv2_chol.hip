@@ -64,17 +64,28 @@ __global__ __launch_bounds__(CHOL_WG) void k_chol_diag(double *A, int N, i64 sA,
         if (threadIdx.x == 0) info[b] = bad;
         return;
     }
-    // U11^-1 by rows from the bottom: x[i][c] = (delta_ic - sum_{i<k<=c} a[i][k] x[k][c]) / a[i][i], c >= i
+    // U11^-1 by columns, all at once: x[i][c] = (delta_ic - sum_{i<k<=c} a[i][k] x[k][c]) / a[i][i] for
+    // i = c down to 0.  Four lanes per column (threads 4c .. 4c+3, always active together) split each sum
+    // by k mod 4 and meet by two shuffles; a column lives in one wavefront, whose in-order LDS (the wait
+    // after each store) orders x[i][c] before its reads, so the columns need no barrier.
     for (int t = threadIdx.x; t < kb * kb; t += CHOL_WG) x[t % kb][t / kb] = 0.0;
     __syncthreads();
-    for (int i = kb - 1; i >= 0; i--) {
-        for (int c = i + threadIdx.x; c < kb; c += CHOL_WG) {
-            double s = c == i ? 1.0 : 0.0;
-            for (int k = i + 1; k <= c; k++) s -= a[i][k] * x[k][c];
-            x[i][c] = s / a[i][i];
+    {
+        static_assert(CHOL_WG == 4 * CHOL_NB, "four lanes per column");
+        const int c = threadIdx.x >> 2, sub = threadIdx.x & 3;
+        if (c < kb) {
+            for (int i = c; i >= 0; i--) {
+                double sm = 0.0;
+                for (int k = i + 1 + sub; k <= c; k += 4) sm += a[i][k] * x[k][c];
+                sm += __shfl_xor(sm, 1, 64);
+                sm += __shfl_xor(sm, 2, 64);
+                const double xi = ((c == i ? 1.0 : 0.0) - sm) / a[i][i];
+                if (sub == 0) x[i][c] = xi;
+                __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            }
         }
-        __syncthreads();
     }
+    __syncthreads();
     double *Ub = Uinv + (i64)b * CHOL_NB * CHOL_NB;
     for (int t = threadIdx.x; t < kb * kb; t += CHOL_WG) {
         const int r = t % kb, c = t / kb;
