@@ -23,26 +23,6 @@ namespace nv2 {
 #define CHOL_NB 64
 #define CHOL_WG 256
 
-// x = a^-1 for the upper triangular kb x kb block a in LDS (x cleared and a barrier passed by the caller),
-// by columns, all at once: x[i][c] = (delta_ic - sum_{i<k<=c} a[i][k] x[k][c]) / a[i][i] for i = c down
-// to 0.  Four lanes per column (threads 4c .. 4c+3, always active together) split each sum by k mod 4 and
-// meet by two shuffles; a column lives in one wavefront, whose in-order LDS (the wait after each store)
-// orders x[i][c] before its reads, so the columns need no barrier.  The caller's barrier follows.
-__device__ __forceinline__ void upper_inverse_lds(const double (*a)[CHOL_NB + 1], double (*x)[CHOL_NB + 1], int kb) {
-    static_assert(CHOL_WG == 4 * CHOL_NB, "four lanes per column");
-    const int c = threadIdx.x >> 2, sub = threadIdx.x & 3;
-    if (c >= kb) return;
-    for (int i = c; i >= 0; i--) {
-        double sm = 0.0;
-        for (int k = i + 1 + sub; k <= c; k += 4) sm += a[i][k] * x[k][c];
-        sm += __shfl_xor(sm, 1, 64);
-        sm += __shfl_xor(sm, 2, 64);
-        const double xi = ((c == i ? 1.0 : 0.0) - sm) / a[i][i];
-        if (sub == 0) x[i][c] = xi;
-        __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-    }
-}
-
 // Factor the kb x kb diagonal block at (k0, k0) of matrix b in LDS and invert it.  info[b] =
 // k0 + j + 1 for the first pivot that is not positive (NaN included); a matrix whose info is
 // already set is left alone (its panel is then scratch).  Uinv + b * CHOL_NB^2 receives U11^-1
@@ -95,7 +75,7 @@ __global__ __launch_bounds__(CHOL_WG) void k_chol_diag(double *A, int N, i64 sA,
     // U11^-1
     for (int t = threadIdx.x; t < kb * kb; t += CHOL_WG) x[t % kb][t / kb] = 0.0;
     __syncthreads();
-    upper_inverse_lds(a, x, kb);
+    upper_inverse_lds<CHOL_NB>(a, x, kb);
     __syncthreads();
     double *Ub = Uinv + (i64)b * CHOL_NB * CHOL_NB;
     for (int t = threadIdx.x; t < kb * kb; t += CHOL_WG) {
@@ -127,7 +107,7 @@ __global__ __launch_bounds__(CHOL_WG) void k_trinv_diag(const double *U, int N, 
         x[r][c] = 0.0;
     }
     __syncthreads();
-    upper_inverse_lds(a, x, kb);
+    upper_inverse_lds<CHOL_NB>(a, x, kb);
     __syncthreads();
     for (int t = threadIdx.x; t < kb * kb; t += CHOL_WG) {
         const int r = t % kb, c = t / kb;

@@ -162,4 +162,25 @@ __device__ __forceinline__ double ro2t(const Rdm &R, int s, int ap, int bp, int 
     if (ap == b && bp == a) v -= 2.0;
     return v;
 }
+// x = a^-1 for the upper triangular kb x kb block a (kb <= NBK, row stride NBK + 1) in LDS, by a workgroup of
+// at least 4 * kb threads (x cleared and a barrier passed by the caller),
+// by columns, all at once: x[i][c] = (delta_ic - sum_{i<k<=c} a[i][k] x[k][c]) / a[i][i] for i = c down
+// to 0.  Four lanes per column (threads 4c .. 4c+3, always active together) split each sum by k mod 4 and
+// meet by two shuffles; a column lives in one wavefront, whose in-order LDS (the wait after each store)
+// orders x[i][c] before its reads, so the columns need no barrier.  The caller's barrier follows.
+template <int NBK>
+__device__ __forceinline__ void upper_inverse_lds(const double (*a)[NBK + 1], double (*x)[NBK + 1], int kb) {
+    const int c = threadIdx.x >> 2, sub = threadIdx.x & 3;
+    if (c >= kb) return;
+    for (int i = c; i >= 0; i--) {
+        double sm = 0.0;
+        for (int k = i + 1 + sub; k <= c; k += 4) sm += a[i][k] * x[k][c];
+        sm += __shfl_xor(sm, 1, 64);
+        sm += __shfl_xor(sm, 2, 64);
+        const double xi = ((c == i ? 1.0 : 0.0) - sm) / a[i][i];
+        if (sub == 0) x[i][c] = xi;
+        __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    }
+}
+
 }  // namespace nv2
