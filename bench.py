@@ -198,8 +198,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path shards by external "
-              f"orbital index, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path would shard by external "
+              f"orbital index, each GPU holding the RDMs and its slice of the external orbitals", file=sys.stderr)
         sys.exit(2)
     from nevpt2_amd import synth
     from nevpt2_amd.evaluator import Evaluator
