@@ -297,9 +297,12 @@ class Problem:
       transition RDMs (ncoppie, ...) when metat > 1: daaaoff, taaoff, daloff
     """
 
-    def __init__(self, nact, ncore, nvirt, seed=1, nroots=1, keep=None, rank_factor=3, nele=None, host_rdms=True):
+    def __init__(self, nact, ncore, nvirt, seed=1, nroots=1, keep=None, rank_factor=3, nele=None, host_rdms=True,
+                 ci=None):
         """host_rdms=False keeps only the CI vectors ``C`` (the RDMs are then built on the GPU from them,
-        ``Evaluator.for_problem(P, rdms="gpu")``); ro4 and the lower RDMs are None."""
+        ``Evaluator.for_problem(P, rdms="gpu")``); ro4 and the lower RDMs are None.  ``ci`` (C[root, I_alpha,
+        I_beta], orthonormal) replaces the seeded random CI vectors, e.g. by eigenvectors of the active
+        Hamiltonian; the integrals still come from ``seed``."""
         from . import rdmtrace
         self.nact, self.ncore, self.nvirt = nact, ncore, nvirt
         self.norb = nact + ncore + nvirt
@@ -307,7 +310,10 @@ class Problem:
         self.metat = nroots
         self.seed = seed
         n, M = nact, nroots
-        self.C = random_ci(n, self.nele, seed, nroots=nroots, keep=keep)
+        if ci is None:
+            self.C = random_ci(n, self.nele, seed, nroots=nroots, keep=keep)
+        else:
+            self.C = np.asarray(ci, dtype=np.float64).reshape(nroots, comb(n, self.nele // 2), -1)
         self.nalpha = self.nbeta = self.nele // 2
         self.ncoppie = M * (M - 1) // 2
         self.B, self.oneint, self.eps0 = random_integrals(ncore, nact, nvirt, seed, rank_factor)

@@ -14,7 +14,11 @@ as in the reference.  The Heff post-processing restates qdnevpt.F:1046-1204
 (qdnevpt.F:8089-8184).
 
 Diagonal entries come from oracle.stage2 unchanged; this module only adds the
-off-diagonal sums.
+off-diagonal sums.  Parity: every class's couplings equal
+<H Psi_j|P (E_0 - H_D)^-1 P|H Psi_i> over state i's SC function and PC
+eigenfunctions, by Fock-space brute force (oracle/fock.py), for CASCI
+eigenfunctions and random CI vectors (tests/test_oracle_brute.py); the
+hermitisation is pinned by its algebra (tests/test_qd_oracle.py).
 """
 from __future__ import annotations
 

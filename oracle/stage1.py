@@ -3,6 +3,12 @@
 Oracle (test infrastructure only).  Every function returns Fortran-ordered
 numpy arrays in the reference layouts (state index first).  The C routines
 restate koop_matrices.F90 and koopro4QD.F90:3900-5049 line by line.
+
+Parity: every Koopmans matrix (1-/2-IPKO, 1-/2-EAKO, 2-HRDM, gk0pa/gk0pd/gk0pf,
+A/B/C/D, IP and EA) equals its operator definition, a spin-summed
+<Psi| O^+ [H_act, O'] |Psi> computed by Fock-space brute force (oracle/fock.py),
+element by element at 1e-12 relative on random CI vectors; the accessors equal
+the reference's own compiled eexx_functions.F90 (oracle/ref) bit for bit.
 """
 from __future__ import annotations
 

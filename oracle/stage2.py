@@ -6,12 +6,14 @@ virtual nc+n..norb-1.  Loops over the outer external index follow the
 reference; inner index sums are numpy reductions.  Accessors are the stage-2
 copies with swapped arguments (qdnevpt.F:6448-6519), built as tensors here.
 
-Parity: the numbers of each class are pinned against the GPU path (same
-inputs), against the textbook limit that class (0) is the MP2-like pair
-formula, and against invariance checks (tests/test_stage2_oracle.py).  The
-reference Fortran itself is not buildable here (hdf5_utils is external), so
-the PC metric solves rely on LAPACK's own semantics (upper triangle, ascending
-eigenvalues) as restated in ``pc_solve``.
+Parity: every class's SC and PC energy and norm equals SC/PC-NEVPT2 computed
+from the first-order functions themselves by Fock-space brute force
+(oracle/fock.py, tests/test_oracle_brute.py): for CASCI eigenfunctions at
+1e-11 relative, and for random CI vectors in commutator form with LAPACK's
+upper triangle of the non-symmetric Koopmans matrix taken in the reference's
+basis order (within 1e-10 relative plus 1e-10 times the class norm).  Class
+(0) also equals the closed-shell MP2 pair formula (tests/test_stage2_oracle.py).
+The reference Fortran itself is not buildable here (hdf5_utils is external).
 """
 from __future__ import annotations
 
