@@ -122,6 +122,12 @@ int nv2_ctx_set_ci(nv2_ctx *ctx, int64_t nalpha, int64_t nbeta, const double *ci
  * half-bandwidth 64); zeros when the state has none. */
 int nv2_ctx_set_trdm(nv2_ctx *ctx, const double *trdm3, const double *trdm2, const double *trdm1);
 int nv2_ctx_run(nv2_ctx *ctx, int32_t prec, double thr_lindep, double *out);
+/* Run options of a context.  "no4rdm" (value != 0) is the reference's .NO4RDM
+ * keyword (nevpt2_cfg no_4rdm_terms): the A-D matrices are not built
+ * (koopro4QD.F90:357-424) and the (+1)' and (-1)' classes and their QD couplings
+ * are not formed (qdnevpt.F:949-991): their table entries are zero and neither
+ * "Heff-SC" nor "Heff-PC" contains them.  Unknown names return -2. */
+int nv2_ctx_set_option(nv2_ctx *ctx, const char *name, int64_t value);
 int nv2_ctx_timings(nv2_ctx *ctx, double *ms);
 int nv2_ctx_get(nv2_ctx *ctx, const char *name, double *host, int64_t count);
 void nv2_ctx_destroy(nv2_ctx *ctx);
@@ -140,8 +146,11 @@ int nv2_ci_rdm(int64_t nact, int64_t nalpha, int64_t nbeta, int32_t k, const dou
  * element (j,i) at j + metat*i), takes the right eigenvectors of the
  * non-Hermitian matrix in ascending eigenvalue order, and returns the
  * hermitised Heff~ = C' diag(w) C'^T, C' = V (V^T V)^{-1/2}, and its
- * eigenvalues (the final QD energies).  Returns -3 when the spectrum is not
- * real.  metat <= 16. */
+ * eigenvalues (the final QD energies).  Degenerate eigenvalues (states of
+ * equal energy) get linearly independent eigenvectors; Heff~ then depends on the
+ * basis chosen inside the degenerate eigenspace, as with dgeev's, its eigenvalues
+ * do not.  Returns -3 when the spectrum is not real or the eigenvectors are
+ * linearly dependent (a defective matrix).  metat <= 16. */
 int nv2_qd_heff(int64_t metat, const double *heff2, const double *e0, double *heff, double *energies);
 
 /* Request loop (no counterpart in the reference, whose drivers evaluate one

@@ -55,6 +55,7 @@ def _declare(L):
     L.nv2_qd_heff.argtypes = [i64, dp, dp, dp, dp]
     L.nv2_ctx_run.argtypes = [C.c_void_p, i32, C.c_double, dp]
     L.nv2_ctx_timings.argtypes = [C.c_void_p, dp]
+    L.nv2_ctx_set_option.argtypes = [C.c_void_p, C.c_char_p, i64]
     L.nv2_ctx_get.argtypes = [C.c_void_p, C.c_char_p, dp, i64]
     L.nv2_ctx_destroy.argtypes = [C.c_void_p]
     L.nv2_ctx_destroy.restype = None

@@ -41,6 +41,7 @@ struct nv2_ctx {
     hipEvent_t ev[10];
     double ms[NV2_NPHASE];
     bool has_trdm = false;
+    bool no4rdm = false;                    // .NO4RDM: no A-D matrices, no (+1)'/(-1)' classes
     std::vector<double> qd;                 // per state s: 8 classes x (SC e2mp(j,s), PC e2en(j,s)), 16*M
     std::vector<double> heff_sc, heff_pc;   // second-order Heff, H[j + M*i] = e2mp(j,i) / e2en(j,i)
     std::vector<double> pcm;                // per state, per class: (min metric eigenvalue, kept, Cholesky path)
@@ -213,10 +214,11 @@ int nv2_ctx_run(nv2_ctx *c, int32_t prec, double thr_lindep, double *out) {
     so.ipko2 = c->find("2-IPKO"); so.eako2 = c->find("2-EAKO"); so.gk0pa = c->find("gk0pa");
     so.gk0pd = c->find("gk0pd"); so.gk0pf = c->find("gk0pf"); so.bmat = c->find("bmatIP"); so.btmat = c->find("bmatEA");
     so.cmat = c->find("cmatIP"); so.ctmat = c->find("cmatEA"); so.dmat = c->find("dmatIP"); so.dtmat = c->find("dmatEA");
+    if (c->no4rdm) so.bmat = nullptr;     // B, C and D serve only the (+1)'/(-1)' classes (koopro4QD.F90:357-424)
     if (launch_koop_prep(n, c->nc, c->find("aj"), c->find("atwo"), w, st)) return -1;
     // phases 2-3: E4 build and the four 4-RDM contractions of bamat, per state
     float t_build = 0.f, t_contr = 0.f;
-    for (int s = 0; s < M; s++) {
+    for (int s = 0; s < M && !c->no4rdm; s++) {
         hipEvent_t a = c->ev[7], b = c->ev[8], d = c->ev[9];
         NV2_HIP(hipEventRecord(a, st));
         if (launch_build_e4(R, s, c->E4, prec, c->find("e4_d3t"), st)) return -1;
@@ -234,13 +236,13 @@ int nv2_ctx_run(nv2_ctx *c, int32_t prec, double thr_lindep, double *out) {
     NV2_HIP(hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
     NV2_HIP(hipEventRecord(c->ev[4], st));
     // phase 4: lower-order part of bamat and final A-matrices
-    for (int s = 0; s < M; s++)
+    for (int s = 0; s < M && !c->no4rdm; s++)
         if (launch_bamat_lo(c->rb, R, s, c->find("atwo"), c->find("Fhalf"), c->find("bamat_work"), c->find("C1"),
                             c->find("C2"), c->find("C3"), c->find("C4"), c->find("amatIP"), c->find("amatEA"), st))
             return -1;
     NV2_HIP(hipEventRecord(c->ev[2], st));
     // the (+-1)' chain of the first state needs amat only: it runs on while the rest of stage 1 is built
-    if (stage2_pm1p_kready(c->s2, 0)) return -1;
+    if (!c->no4rdm && stage2_pm1p_kready(c->s2, 0)) return -1;
     // phase 1 (second part): Koopmans matrices of rank <= 3 and B/C/D
     if (launch_koop_rest(R, c->find("atwo"), w, so, st)) return -1;
     NV2_HIP(hipEventRecord(c->ev[5], st));
@@ -285,6 +287,17 @@ int nv2_ctx_run(nv2_ctx *c, int32_t prec, double thr_lindep, double *out) {
     NV2_HIP(hipEventElapsedTime(&t56, c->ev[5], c->ev[6]));
     c->ms[0] = t01; c->ms[1] = t25; c->ms[2] = t_build; c->ms[3] = t_contr; c->ms[4] = t42; c->ms[5] = t56;
     return 0;
+}
+
+int nv2_ctx_set_option(nv2_ctx *c, const char *name, int64_t value) {
+    if (!c || !name) { set_error("nv2_ctx_set_option: null argument"); return -2; }
+    if (!strcmp(name, "no4rdm")) {
+        c->no4rdm = value != 0;
+        stage2_set_no4rdm(c->s2, c->no4rdm);
+        return 0;
+    }
+    set_error("nv2_ctx_set_option: unknown option '%s'", name);
+    return -2;
 }
 
 int nv2_ctx_timings(nv2_ctx *c, double *ms) {

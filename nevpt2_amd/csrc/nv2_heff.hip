@@ -8,7 +8,11 @@
 // Eigenvalues of the non-symmetric A come from shifted QR iterations on its
 // Hessenberg form (Wilkinson shifts, deflation), eigenvectors from inverse
 // iteration on A; complex eigenvalue pairs are reported as an error, as the
-// hermitisation assumes a real spectrum.  Symmetric sub-problems use cyclic
+// hermitisation assumes a real spectrum.  Roots whose eigenvalues coincide
+// (degenerate states are the common QD case) start from different vectors and
+// are re-orthogonalised against the vectors already found in their cluster at
+// every iteration, so a degenerate eigenspace yields independent vectors; a
+// numerically singular S = V^T V (a defective Heff) is reported as an error.  Symmetric sub-problems use cyclic
 // Jacobi.  Right eigenvectors are normalised to unit length as dgeev returns
 // them; the hermitised matrix does not depend on their signs.
 #include <hip/hip_runtime.h>
@@ -153,8 +157,11 @@ __device__ static int qr_eigenvalues(int m, double *H, double *w) {
     return 0;
 }
 
-// eigenvector of A for eigenvalue lam by inverse iteration (LU with partial pivoting)
-__device__ static void inverse_iteration(int m, const double *A, double lam, double *x) {
+// eigenvector of A for eigenvalue lam by inverse iteration (LU with partial pivoting), started from a
+// vector that depends on the root index k and kept orthogonal to the ncl vectors V(:, cl[0..ncl-1]) of
+// the same eigenvalue cluster
+__device__ static void inverse_iteration(int m, const double *A, double lam, int k, const double *V, const int *cl,
+                                         int ncl, double *x) {
     double L[HM * HM];
     int piv[HM];
     double scale = 0.0;
@@ -163,37 +170,50 @@ __device__ static void inverse_iteration(int m, const double *A, double lam, dou
     const double shift = lam + 1e-13 * (scale > 0.0 ? scale : 1.0);
     for (int i = 0; i < m; i++)
         for (int j = 0; j < m; j++) A2(L, i, j) = A2(A, i, j) - (i == j ? shift : 0.0);
-    for (int k = 0; k < m; k++) {
-        int p = k;
-        for (int i = k + 1; i < m; i++)
-            if (fabs(A2(L, i, k)) > fabs(A2(L, p, k))) p = i;
-        piv[k] = p;
-        if (p != k)
-            for (int j = 0; j < m; j++) { double t = A2(L, k, j); A2(L, k, j) = A2(L, p, j); A2(L, p, j) = t; }
-        if (A2(L, k, k) == 0.0) A2(L, k, k) = 1e-300;
-        for (int i = k + 1; i < m; i++) {
-            const double f = A2(L, i, k) / A2(L, k, k);
-            A2(L, i, k) = f;
-            for (int j = k + 1; j < m; j++) A2(L, i, j) -= f * A2(L, k, j);
+    for (int c = 0; c < m; c++) {
+        int p = c;
+        for (int i = c + 1; i < m; i++)
+            if (fabs(A2(L, i, c)) > fabs(A2(L, p, c))) p = i;
+        piv[c] = p;
+        if (p != c)
+            for (int j = 0; j < m; j++) { double t = A2(L, c, j); A2(L, c, j) = A2(L, p, j); A2(L, p, j) = t; }
+        if (A2(L, c, c) == 0.0) A2(L, c, c) = 1e-300;
+        for (int i = c + 1; i < m; i++) {
+            const double f = A2(L, i, c) / A2(L, c, c);
+            A2(L, i, c) = f;
+            for (int j = c + 1; j < m; j++) A2(L, i, j) -= f * A2(L, c, j);
         }
     }
-    for (int i = 0; i < m; i++) x[i] = 1.0 / sqrt((double)m) * (1.0 + 0.01 * i);
-    for (int it = 0; it < 4; it++) {
-        for (int k = 0; k < m; k++) { const int p = piv[k]; if (p != k) { double t = x[k]; x[k] = x[p]; x[p] = t; } }
+    for (int i = 0; i < m; i++) x[i] = 1.0 + 0.5 * sin(1.7 * (i + 1) * (k + 1)) + (i == k % m ? 1.0 : 0.0);
+    for (int it = 0; it < 6; it++) {
+        for (int c = 0; c < ncl; c++) {      // stay orthogonal to the cluster's vectors found so far
+            const double *v = &A2(V, 0, cl[c]);
+            double d = 0.0;
+            for (int i = 0; i < m; i++) d += v[i] * x[i];
+            for (int i = 0; i < m; i++) x[i] -= d * v[i];
+        }
+        for (int c = 0; c < m; c++) { const int p = piv[c]; if (p != c) { double t = x[c]; x[c] = x[p]; x[p] = t; } }
         for (int i = 0; i < m; i++)
-            for (int k = 0; k < i; k++) x[i] -= A2(L, i, k) * x[k];
+            for (int c = 0; c < i; c++) x[i] -= A2(L, i, c) * x[c];
         for (int i = m - 1; i >= 0; i--) {
-            for (int k = i + 1; k < m; k++) x[i] -= A2(L, i, k) * x[k];
+            for (int c = i + 1; c < m; c++) x[i] -= A2(L, i, c) * x[c];
             x[i] /= A2(L, i, i);
+        }
+        for (int c = 0; c < ncl; c++) {
+            const double *v = &A2(V, 0, cl[c]);
+            double d = 0.0;
+            for (int i = 0; i < m; i++) d += v[i] * x[i];
+            for (int i = 0; i < m; i++) x[i] -= d * v[i];
         }
         double nrm = 0.0;
         for (int i = 0; i < m; i++) nrm += x[i] * x[i];
         nrm = sqrt(nrm);
+        if (!(nrm > 0.0)) nrm = 1.0;
         for (int i = 0; i < m; i++) x[i] /= nrm;
     }
 }
 
-// status[0]: 0 ok, 1 complex or unconverged eigenvalues
+// status[0]: 0 ok, 1 complex or unconverged eigenvalues, 2 linearly dependent eigenvectors (defective Heff)
 __global__ void k_qd_heff(int m, const double *h2, const double *e0, double *heff, double *energies, int *status) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double A[HM * HM], H[HM * HM], V[HM * HM], S[HM * HM], U[HM * HM], w[HM], sw[HM];
@@ -206,7 +226,15 @@ __global__ void k_qd_heff(int m, const double *h2, const double *e0, double *hef
     for (int i = 0; i < m; i++)
         for (int j = i + 1; j < m; j++)
             if (w[j] < w[i]) { double t = w[i]; w[i] = w[j]; w[j] = t; }
-    for (int k = 0; k < m; k++) inverse_iteration(m, A, w[k], &A2(V, 0, k));
+    double scale = 0.0;
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < m; j++) scale = fmax(scale, fabs(A2(A, i, j)));
+    for (int k = 0; k < m; k++) {
+        int cl[HM], ncl = 0;
+        for (int j = 0; j < k; j++)
+            if (fabs(w[j] - w[k]) <= 1e-9 * (scale > 0.0 ? scale : 1.0)) cl[ncl++] = j;
+        inverse_iteration(m, A, w[k], k, V, cl, ncl, &A2(V, 0, k));
+    }
     // S = V^T V and S^{-1/2}
     for (int i = 0; i < m; i++)
         for (int j = 0; j < m; j++) {
@@ -215,6 +243,9 @@ __global__ void k_qd_heff(int m, const double *h2, const double *e0, double *hef
             A2(S, i, j) = d;
         }
     jacobi_sym(m, S, U, sw);
+    double swmax = 0.0, swmin = 1e300;
+    for (int i = 0; i < m; i++) { swmax = fmax(swmax, sw[i]); swmin = fmin(swmin, sw[i]); }
+    if (!(swmin > 1e-10 * swmax)) { status[0] = 2; return; }
     double C[HM * HM];
     for (int i = 0; i < m; i++)          // C' = V U diag(1/sqrt(sw)) U^T
         for (int j = 0; j < m; j++) {
@@ -266,7 +297,11 @@ extern "C" int nv2_qd_heff(int64_t metat, const double *heff2, const double *e0,
                 hipMemcpy(energies, d + 2 * mm + m, sizeof(double) * m, hipMemcpyDeviceToHost) != hipSuccess)) {
         set_error("nv2_qd_heff: download failed"); rc = -1;
     }
-    if (!rc && status) { set_error("nv2_qd_heff: Heff has complex or unconverged eigenvalues"); rc = -3; }
+    if (!rc && status == 1) { set_error("nv2_qd_heff: Heff has complex or unconverged eigenvalues"); rc = -3; }
+    if (!rc && status == 2) {
+        set_error("nv2_qd_heff: the eigenvectors of Heff are linearly dependent (defective matrix)");
+        rc = -3;
+    }
     (void)hipFree(d);
     (void)hipFree(ds);
     return rc;

@@ -16,6 +16,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <exception>
 #include <map>
 #include <mutex>
 #include <thread>
@@ -145,6 +146,15 @@ static void slot_main(nv2_server *s) {
         {
             std::unique_lock<std::mutex> lk(s->mu);
             for (;;) {
+                if (!to_free.empty()) {
+                    // contexts evicted for a request that still did not fit: their memory is already
+                    // off the books, so free it now rather than when (or if) this slot admits one
+                    lk.unlock();
+                    for (nv2_ctx *c : to_free) nv2_ctx_destroy(c);
+                    to_free.clear();
+                    lk.lock();
+                    continue;
+                }
                 if (s->stop) return;
                 if (!s->queue.empty()) {
                     ServeReq &h = s->queue.front();
@@ -159,19 +169,29 @@ static void slot_main(nv2_server *s) {
                         continue;
                     }
                     if (try_admit(s, q, pc, to_free)) break;
+                    if (!to_free.empty()) continue;     // free what was evicted first, then look again
                 }
                 s->cv_work.wait(lk);
             }
             s->running++;
         }
-        for (nv2_ctx *c : to_free) nv2_ctx_destroy(c);
         const Clock::time_point t_start = Clock::now();
         int rc;
-        if (q.ctx) rc = nv2_ctx_run(q.ctx, q.prec, q.thr, q.out);
-        else rc = run_host_request(s, q, pc);
         ServeDone d;
+        // nothing may escape a slot thread: an exception there would end the process (std::terminate)
+        try {
+            for (nv2_ctx *c : to_free) nv2_ctx_destroy(c);
+            if (q.ctx) rc = nv2_ctx_run(q.ctx, q.prec, q.thr, q.out);
+            else rc = run_host_request(s, q, pc);
+            if (rc) nv2_last_error(d.err, sizeof(d.err));
+        } catch (const std::exception &x) {
+            rc = -1;
+            snprintf(d.err, sizeof(d.err), "request failed with an exception: %s", x.what());
+        } catch (...) {
+            rc = -1;
+            snprintf(d.err, sizeof(d.err), "request failed with an unknown exception");
+        }
         d.status = rc;
-        if (rc) nv2_last_error(d.err, sizeof(d.err));
         const Clock::time_point t_end = Clock::now();
         d.latency_ms = std::chrono::duration<double, std::milli>(t_end - q.t_submit).count();
         d.queue_ms = std::chrono::duration<double, std::milli>(t_start - q.t_submit).count();

@@ -36,6 +36,9 @@ void stage2_destroy(Stage2 *p);
 int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *stats, double *qd_out = nullptr);
 // starts the (+-1)' metric solve of state s on worker 1 ahead of stage2_state (it needs the 3-RDM
 // only); stage2_state(s) then waits for it instead of starting it
+// Without the 4-RDM terms (the reference's .NO4RDM) the (+1)' and (-1)' classes and their QD
+// couplings are not formed; their table entries stay zero (qdnevpt.F:949-991).
+void stage2_set_no4rdm(Stage2 *p, bool on);
 int stage2_pm1p_begin(Stage2 *p, int s, double thr);
 // Fills the chain's Koopmans matrix from amat of state s (bamat done) and releases the chain;
 // stage2_state then skips that fill.  No-op without a chain started for state s.

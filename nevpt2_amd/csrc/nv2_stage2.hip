@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <map>
 #include <utility>
 #include <string>
@@ -1453,6 +1454,7 @@ struct Buf {
 struct Pm1pEarly;
 struct Stage2 {
     S2Inputs in;
+    bool no4 = false;              // .NO4RDM: no (+-1)' classes
     Pm1pEarly *early = nullptr;    // the (+-1)' chain when started ahead of stage2_state
     hipEvent_t evS = nullptr, evK = nullptr;   // its metric filled / its Koopmans matrix filled
     hipEvent_t evX = nullptr;                  // its PC right-hand sides built (band-form sums on worker 1)
@@ -1485,11 +1487,21 @@ struct Stage2 {
     i64 np_c, np_cs, np_v, np_vs;
     bool blocks_ready = false;
 
+    // Waits for this evaluation's own streams only: other contexts on the device (concurrent
+    // requests of the request loop) keep running.
+    void sync_own() {
+        (void)hipStreamSynchronize(st);
+        for (int w = 0; w < NW; w++)
+            if (ws[w]) (void)hipStreamSynchronize(ws[w]);
+        if (xs) (void)hipStreamSynchronize(xs);
+    }
+    // Buffers are looked up and grown on the thread that drives the evaluation only; worker threads
+    // use the pointers handed to them.
     double *get(const std::string &name, i64 count) {
         Buf &b = bufs[name];
         size_t bytes = sizeof(double) * (size_t)(count > 0 ? count : 1);
         if (b.cap < bytes) {
-            if (b.p) { (void)hipDeviceSynchronize(); (void)hipFree(b.p); }
+            if (b.p) { sync_own(); (void)hipFree(b.p); }
             b.p = nullptr;
             if (hipMalloc(&b.p, bytes) != hipSuccess) { b.cap = 0; return nullptr; }
             b.cap = bytes;
@@ -1561,11 +1573,10 @@ static int gemm_sb(Stage2 *p, int m, int nn, int k, const double *A, i64 lda, i6
 // eigen-decomposition in place (ascending eigenvalues in w), reference triangle choice.
 // NV2_EIGEN selects the rocSOLVER algorithm: "dc" (syevd, default), "jacobi" (syevj), "dcj" (syevdj)
 static int eigen_algo() {
-    static int algo = -1;
-    if (algo < 0) {
+    static const int algo = [] {      // initialised once, thread-safely (several contexts run concurrently)
         const char *e = getenv("NV2_EIGEN");
-        algo = (e && !strcmp(e, "jacobi")) ? 1 : (e && !strcmp(e, "dcj")) ? 2 : 0;
-    }
+        return (e && !strcmp(e, "jacobi")) ? 1 : (e && !strcmp(e, "dcj")) ? 2 : 0;
+    }();
     return algo;
 }
 static int syevd(Stage2 *p, bool upper, int N, double *A, double *w) {
@@ -1802,12 +1813,17 @@ static int eig_trd(Stage2 *p, PcJob &j, int N, double *A, i64 sA, double *ev, i6
     if (split) {
         NV2_HIP(hipEventRecord(p->xev[0], st));
         t1 = std::thread([&]() {
-            if (hipStreamWaitEvent(p->xs, p->xev[0], 0) != hipSuccess ||
-                rocsolver_dstedc(p->xh, rocblas_evect_tridiagonal, N, ev + sD, j.trde + N, j.T + sA, N,
-                                 p->dinfo_w + idx + 1) != rocblas_status_success ||
-                hipEventRecord(p->xev[1], p->xs) != hipSuccess) {
+            try {
+                if (hipStreamWaitEvent(p->xs, p->xev[0], 0) != hipSuccess ||
+                    rocsolver_dstedc(p->xh, rocblas_evect_tridiagonal, N, ev + sD, j.trde + N, j.T + sA, N,
+                                     p->dinfo_w + idx + 1) != rocblas_status_success ||
+                    hipEventRecord(p->xev[1], p->xs) != hipSuccess) {
+                    rc1 = -1;
+                    snprintf(err1, sizeof(err1), "dstedc of the second problem (N=%d) failed", N);
+                }
+            } catch (...) {
                 rc1 = -1;
-                snprintf(err1, sizeof(err1), "dstedc of the second problem (N=%d) failed", N);
+                snprintf(err1, sizeof(err1), "dstedc of the second problem (N=%d): exception", N);
             }
         });
     }
@@ -2169,8 +2185,10 @@ static int pm1p_worker(Stage2 *p, Pm1pEarly *E) {
     return 0;
 }
 
+void stage2_set_no4rdm(Stage2 *p, bool on) { p->no4 = on; }
+
 int stage2_pm1p_begin(Stage2 *p, int s, double thr) {
-    if (!(p->in.amatIP && p->in.amatEA)) return 0;
+    if (p->no4 || !(p->in.amatIP && p->in.amatEA)) return 0;
     if (!p->early) p->early = new Pm1pEarly();
     Pm1pEarly *E = p->early;
     if (E->active) {
@@ -2218,7 +2236,15 @@ int stage2_pm1p_begin(Stage2 *p, int s, double thr) {
     E->err[0] = 0;
     E->active = true;
     E->th = std::thread([p, E]() {
-        E->rc = pm1p_worker(p, E);
+        try {
+            E->rc = pm1p_worker(p, E);
+        } catch (const std::exception &x) {
+            set_error("(+-1)' chain: %s", x.what());
+            E->rc = -1;
+        } catch (...) {
+            set_error("(+-1)' chain: unknown exception");
+            E->rc = -1;
+        }
         if (E->rc) nv2_last_error(E->err, sizeof(E->err));
     });
     return 0;
@@ -2311,8 +2337,8 @@ void stage2_destroy(Stage2 *p) {
     if (p->evS) (void)hipEventDestroy(p->evS);
     if (p->evK) (void)hipEventDestroy(p->evK);
     for (auto e : p->evKr) if (e) (void)hipEventDestroy(e);
+    p->sync_own();      // this evaluation's streams only; other contexts keep running
     for (auto &kv : p->bufs) if (kv.second.p) (void)hipFree(kv.second.p);
-    (void)hipDeviceSynchronize();
     (void)hipFree(p->dinfo); (void)hipFree(p->part); (void)hipFree(p->acc); (void)hipFree(p->accq);
     (void)hipFree(p->qpart); (void)hipFree(p->dinfo_w);
     if (p->hinfo) (void)hipHostFree(p->hinfo);
@@ -2437,7 +2463,7 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
     enum { JPM1, JPM2, JPM2P, J0P, JPM1P, NJOB };
     PcJob J[NJOB];
     const int m = n * (n + 1) / 2;
-    const bool has4 = p->in.amatIP && p->in.amatEA;
+    const bool has4 = !p->no4 && p->in.amatIP && p->in.amatEA;
     const i64 N3 = n2 * n;
     // the (+-1)' chain runs on worker 1 from its own start (stage2_pm1p_begin, possibly called
     // right after the partial traces); every return from here on stops it if it still waits
@@ -2555,7 +2581,15 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
     } joiner{workers};
     for (int w = 0; w < (has4 ? 1 : NW); w++)          // with the (+-1)' chain, worker 1 is its thread
         workers.emplace_back([p, &J, w, thr, &wrc, &werr, &ms_wdone, t_begin]() {
-            wrc[w] = pc_worker(p, J, NJOB, w, thr, &ms_wdone[w], t_begin);
+            try {
+                wrc[w] = pc_worker(p, J, NJOB, w, thr, &ms_wdone[w], t_begin);
+            } catch (const std::exception &x) {
+                set_error("stage-2 worker %d: %s", w, x.what());
+                wrc[w] = -1;
+            } catch (...) {
+                set_error("stage-2 worker %d: unknown exception", w);
+                wrc[w] = -1;
+            }
             if (wrc[w]) nv2_last_error(werr[w], sizeof(werr[w]));
         });
 

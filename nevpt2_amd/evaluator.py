@@ -76,6 +76,10 @@ class Evaluator:
         L.check(L.lib().nv2_ctx_get(self._h, b"PC-timings", L.ptr(a), a.size), "nv2_ctx_get(PC-timings)")
         return a.reshape(self.M, 4)
 
+    def set_option(self, name, value):
+        """Run option: "no4rdm" (the reference's .NO4RDM) skips the A-D matrices and the (+-1)' classes."""
+        L.check(L.lib().nv2_ctx_set_option(self._h, name.encode(), int(value)), f"nv2_ctx_set_option({name})")
+
     def set_integrals(self, cholesky, oneint):
         B = L.fin(cholesky)
         one = np.ascontiguousarray(oneint, dtype=np.float64)

@@ -44,6 +44,13 @@ module nevpt2_hip
       real(c_double), intent(out) :: out(*)
       integer(c_int) :: rc
     end function
+    function nv2_ctx_set_option(ctx, name, value) bind(C, name="nv2_ctx_set_option") result(rc)
+      import :: c_ptr, c_int, c_char, c_int64_t
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: name(*)
+      integer(c_int64_t), value :: value
+      integer(c_int) :: rc
+    end function
     function nv2_ctx_timings(ctx, ms) bind(C, name="nv2_ctx_timings") result(rc)
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx

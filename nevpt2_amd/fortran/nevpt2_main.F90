@@ -53,6 +53,17 @@ program nevpt2_amd_main
   rc = nv2_ctx_set_integrals(ctx, chol, oneint)
   if (rc == 0) rc = nv2_ctx_set_ro4(ctx, ro4)
   if (rc == 0 .and. qd) rc = nv2_ctx_set_trdm(ctx, t3, t2, t1)
+  ! .NO4RDM: no A-D matrices and no (+1)'/(-1)' classes (koopro4QD.F90:357, qdnevpt.F:949-991)
+  if (cfg%no_4rdm_terms) then
+    write(6, '(a)') ' WARNING: .NO4RDM has been specified. V(+1p) and'
+    write(6, '(a)') ' V(-1p) terms will not be calculated.'
+  end if
+  if (cfg%prec /= 0) then
+    write(6, '(a)') ' WARNING: .MIXED runs the 4-RDM contractions in FP32. On classes with near-zero'
+    write(6, '(a)') ' denominators (large class norms) the energies can differ from the FP64 path by'
+    write(6, '(a)') ' far more than 1 microhartree; compare with a run without .MIXED.'
+  end if
+  if (rc == 0 .and. cfg%no_4rdm_terms) rc = nv2_ctx_set_option(ctx, 'no4rdm' // c_null_char, 1_c_int64_t)
   allocate(tab(32 * metat), ms(NV2_NPHASE))
   if (rc == 0) rc = nv2_ctx_run(ctx, int(cfg%prec, c_int32_t), real(cfg%thr_lindep, c_double), tab)
   if (rc /= 0) then
@@ -81,7 +92,6 @@ program nevpt2_amd_main
     nsc_tot = 0; sc_tot = 0; npc_tot = 0; pc_tot = 0
     do k = 1, 8
       i = 32 * (s - 1) + 4 * (k - 1)
-      if (cfg%no_4rdm_terms .and. (k == 6 .or. k == 7)) tab(i + 1:i + 4) = 0
       write(6, '(1x,a,4f20.15)') lab(k)(1:5), tab(i + 1:i + 4)
       nsc_tot = nsc_tot + tab(i + 1); sc_tot = sc_tot + tab(i + 2)
       npc_tot = npc_tot + tab(i + 3); pc_tot = pc_tot + tab(i + 4)

@@ -14,7 +14,7 @@ EXE = os.path.join(ROOT, "nevpt2_amd", "nevpt2_amd.x")
 
 INPUT = """*MODEL
 .CHOLES
-*KOOPRO
+{model}*KOOPRO
 .STATES
 {nstates}
 .NACTEL
@@ -47,7 +47,7 @@ def test_host_end_to_end(gpu, tmp_path):
     print("[fortran host] CAS(4,4) nc=2 nv=5", flush=True)
     P = synth.Problem(4, 2, 5, seed=3)
     bundle.write_bundle(tmp_path / "in.bin", P, e0=[-1.5])
-    (tmp_path / "input.nevpt").write_text(INPUT.format(nstates=1, nele=P.nele, bundle="in.bin"))
+    (tmp_path / "input.nevpt").write_text(INPUT.format(nstates=1, nele=P.nele, bundle="in.bin", model=""))
     r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     tab = bundle.read_table(r.stdout)
@@ -68,7 +68,7 @@ def test_host_qd_two_roots(gpu, tmp_path):
     P = synth.Problem(4, 2, 3, seed=1, nroots=2)     # a seed whose SC and PC Heff have real spectra
     e0 = np.array([-1.5, -1.2])
     bundle.write_bundle(tmp_path / "in.bin", P, e0=e0)
-    (tmp_path / "input.nevpt").write_text(INPUT.format(nstates=2, nele=P.nele, bundle="in.bin"))
+    (tmp_path / "input.nevpt").write_text(INPUT.format(nstates=2, nele=P.nele, bundle="in.bin", model=""))
     r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "QD-NEVPT2: FINAL RESULTS" in r.stdout
@@ -80,5 +80,31 @@ def test_host_qd_two_roots(gpu, tmp_path):
     epc, _ = Q.diagonalize_heff(Hpc, e0)
     # the summary prints 7 decimals
     assert np.abs(summ[:, 0] - e0).max() < 1e-6
+    assert np.abs(summ[:, 1] - esc).max() < 2e-6 * max(1.0, np.abs(esc).max())
+    assert np.abs(summ[:, 2] - epc).max() < 2e-6 * max(1.0, np.abs(epc).max())
+
+
+@pytest.mark.gpu
+def test_host_no4rdm_skips_the_prime_classes(gpu, tmp_path):
+    """.NO4RDM: the (+1)' and (-1)' classes are never formed (qdnevpt.F:949-991), so they are absent
+    from the table, from Heff and from the summary energies alike."""
+    _need_exe()
+    print("[fortran host] .NO4RDM QD CAS(4,4) nc=2 nv=3, 2 roots", flush=True)
+    P = synth.Problem(4, 2, 3, seed=1, nroots=2)
+    e0 = np.array([-1.5, -1.2])
+    bundle.write_bundle(tmp_path / "in.bin", P, e0=e0)
+    (tmp_path / "input.nevpt").write_text(INPUT.format(nstates=2, nele=P.nele, bundle="in.bin", model=".NO4RDM\n"))
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "V(-1p) terms will not be calculated" in r.stdout
+    tabs = bundle.read_table(r.stdout)
+    koop = S1.koopmans_all(P)
+    Hsc, Hpc, tables, _ = Q.heff_matrices(P, koop, with_4rdm=False)
+    for s in range(2):
+        assert np.all(tabs[s][[5, 6]] == 0.0)
+        assert (np.abs(tabs[s][:, 1] - tables[s][:, 1]) <= energy_tol(tables[s][:, 0])).all()
+    summ = bundle.read_qd_summary(r.stdout)
+    esc, _ = Q.diagonalize_heff(Hsc, e0)
+    epc, _ = Q.diagonalize_heff(Hpc, e0)
     assert np.abs(summ[:, 1] - esc).max() < 2e-6 * max(1.0, np.abs(esc).max())
     assert np.abs(summ[:, 2] - epc).max() < 2e-6 * max(1.0, np.abs(epc).max())

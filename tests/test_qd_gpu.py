@@ -67,3 +67,25 @@ def test_qd_heff_diagonalisation(gpu, qdcase):
         scale = max(1.0, np.abs(ref_e).max())
         assert np.abs(w - ref_e).max() <= 1e-10 * scale, (w, ref_e)
         assert np.abs(Ht - ref_h).max() <= 1e-8 * scale, (Ht, ref_h)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("symmetric", [True, False], ids=["symmetric", "nonsymmetric"])
+def test_qd_heff_degenerate_roots(gpu, symmetric):
+    """Two states of equal energy (the common QD case): independent eigenvectors, the right energies;
+    for a symmetric Heff the hermitised matrix is Heff itself whatever basis of the degenerate pair."""
+    rng = np.random.default_rng(4)
+    lam = np.array([-1.25, -1.25, -0.75, -0.5])
+    Q, _ = np.linalg.qr(rng.standard_normal((4, 4)))
+    X = Q if symmetric else Q + 0.2 * rng.standard_normal((4, 4))
+    A = X @ np.diag(lam) @ np.linalg.inv(X)
+    if symmetric:
+        A = 0.5 * (A + A.T)
+    e0 = np.array([-1.0, -1.0, -1.0, -1.0])
+    w, Ht = qdnevpt.qd_heff(A - np.diag(e0), e0)
+    print(f"  energies {w}", flush=True)
+    assert np.abs(np.sort(w) - lam).max() < 1e-10
+    assert np.abs(Ht - Ht.T).max() < 1e-12
+    assert np.abs(np.linalg.eigvalsh(Ht) - lam).max() < 1e-10
+    if symmetric:
+        assert np.abs(Ht - A).max() < 1e-10

@@ -9,10 +9,6 @@ and solver calls run in both cases, so they agree to round-off).
 import numpy as np
 import pytest
 
-# The first GPU run of this file aborted the process (exit 134, no HIP/ROCr message) during the
-# fixture or the first test; the cause is not shown yet, so the file stays off the GPU.
-pytestmark = pytest.mark.skip(reason="aborted on the GPU in its first run (exit 134, cause not yet shown)")
-
 from nevpt2_amd import synth
 from nevpt2_amd._lib import NV2Error
 from nevpt2_amd.evaluator import Evaluator
