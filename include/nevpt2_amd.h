@@ -54,6 +54,22 @@ int nv2_koop_matrices(int64_t metat, int64_t nact, int64_t ncore, int64_t norb, 
                       double *gk0pf, double *bmat, double *btmat, double *cmat, double *ctmat, double *dmat,
                       double *dtmat);
 
+/* Allocation-free form of nv2_koop_matrices.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  Every array argument is DEVICE memory the
+ * caller owns, in the layouts of nv2_koop_matrices; the work is enqueued on
+ * `stream` (a hipStream_t, NULL for the default stream) and the call returns
+ * without waiting.  It calls no hipMalloc/hipFree and no allocating library
+ * routine: the only scratch, the active-space Dyall f matrices (3 nact^2
+ * doubles), lives in a static device array sized for nact <= 16; larger nact
+ * returns -2.  All calls on one GPU share that array, so calls that overlap on
+ * one GPU race. */
+int nv2_koop_matrices_noalloc(int64_t metat, int64_t nact, int64_t ncore, int64_t norb, const double *atwo,
+                              const double *daaa, const double *taa, const double *dal, const double *aj,
+                              double *ipko1, double *eako1, double *hrdm, double *ipko2, double *eako2,
+                              double *gk0pa, double *gk0pd, double *gk0pf, double *bmat, double *btmat,
+                              double *cmat, double *ctmat, double *dmat, double *dtmat, void *stream);
+
 /* A-matrices of the (+-1)' classes, amat/atmat(metat,n,n,n,n,n,n) (amatIP/amatEA).
  * Replaces bamat (koopro4QD.F90:3974), C1 branch. */
 int nv2_bamat(int64_t nact, int64_t ncore, int64_t norb, int64_t metat, const double *atwo, const double *ro4,

@@ -43,6 +43,7 @@ def _declare(L):
     L.nv2_bro.argtypes = [i64, i64, i64, dp, dp, dp, dp]
     L.nv2_get_aj_atwo.argtypes = [i64, i64, i64, i64, dp, dp, dp, dp]
     L.nv2_koop_matrices.argtypes = [i64, i64, i64, i64] + [dp] * 19
+    L.nv2_koop_matrices_noalloc.argtypes = [i64, i64, i64, i64] + [C.c_void_p] * 19 + [C.c_void_p]
     L.nv2_bamat.argtypes = [i64, i64, i64, i64] + [dp] * 8 + [i32]
     L.nv2_qdnevpt_state.argtypes = [i64] * 5 + [dp] * 21 + [i64, C.c_double, dp]
     L.nv2_ctx_create.restype = C.c_void_p

@@ -32,3 +32,12 @@ def test_error_channel_without_gpu():
     rc = L.nv2_bro(0, 1, 6, None, None, None, None)
     assert rc == -2
     assert "nact" in _lib.last_error()
+
+
+def test_noalloc_rejects_sizes_beyond_its_scratch():
+    """The allocation-free entry point refuses nact above its static scratch before any device work."""
+    from nevpt2_amd import _lib
+    L = _lib.lib()
+    rc = L.nv2_koop_matrices_noalloc(1, 17, 2, 20, *([None] * 19), None)
+    assert rc == -2
+    assert "nact=17" in _lib.last_error() and "scratch" in _lib.last_error()
