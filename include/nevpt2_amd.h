@@ -28,6 +28,7 @@ extern "C" {
 /* arithmetic of the 4-RDM contractions (bamat) and PC S-builds */
 #define NV2_PREC_FP64 0   /* drop-in form, matches the reference */
 #define NV2_PREC_FP32 1   /* FP32 storage and arithmetic of the 4-RDM contractions */
+#define NV2_PREC_BF16X3 2 /* FP32 storage; split-BF16x3 products on the bf16 matrix cores, FP32 accumulation */
 
 int nv2_last_error(char *buf, int64_t len);
 int nv2_device_count(void);

@@ -19,6 +19,7 @@ dp = C.POINTER(C.c_double)
 
 PREC_FP64 = 0
 PREC_FP32 = 1
+PREC_BF16X3 = 2
 
 _lib = None
 

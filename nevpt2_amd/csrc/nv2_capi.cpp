@@ -158,7 +158,7 @@ int nv2_bamat(int64_t nact, int64_t ncore, int64_t norb, int64_t metat, const do
               const double *daaa, const double *taa, const double *dal, const double *aj, double *amat,
               double *atmat, int32_t prec) {
     TRY(check_dims(nact, metat));
-    if (prec != NV2_PREC_FP64 && prec != NV2_PREC_FP32) { set_error("nv2_bamat: unsupported precision %d", prec); return -2; }
+    if (prec != NV2_PREC_FP64 && prec != NV2_PREC_FP32 && prec != NV2_PREC_BF16X3) { set_error("nv2_bamat: unsupported precision %d", prec); return -2; }
     const i64 n = nact, M = metat, np = norb * (norb + 1) / 2;
     DBuf dat, dro4, d3, d2, d1, daj, Fh, E4, C[4], dam, dat6, wk, d3t;
     TRY(dat.up(atwo, sizeof(double) * p(n, 4)));

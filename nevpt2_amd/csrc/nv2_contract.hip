@@ -15,7 +15,11 @@
 // wavefronts each, the C3 tiles over all four); pass B (one
 // workgroup per (cp,ap,bp'), four wavefronts) streams E4[cp,ap,:,bp',...] for C4, whose sum
 // runs over x3.  E4 is read twice instead of four times.  FP64 uses v_mfma_f64_16x16x4f64;
-// the FP32 form stores E4 in float and uses v_mfma_f32_16x16x4f32 with FP32 accumulation.
+// the FP32 form stores E4 in float and uses v_mfma_f32_16x16x4f32 with FP32 accumulation; the
+// split-BF16x3 form stores E4 in float too and splits every operand exactly into three bf16 terms
+// (x = h + m + l, by truncation: 8 + 8 + 8 significant bits), forming the six products of order
+// up to 2^-16 -- hh, hm, mh, mm, hl, lh -- with two v_mfma_f32_16x16x16_bf16 per K-step of four and
+// FP32 accumulation (the three dropped products are of order 2^-24, the FP32 rounding level).
 // Outputs land directly in amat's layout (state fastest).
 #include "nv2_common.h"
 #include "nv2_stage1.h"
@@ -24,6 +28,7 @@ namespace nv2 {
 
 typedef double mf_d4 __attribute__((ext_vector_type(4)));
 typedef float mf_f4 __attribute__((ext_vector_type(4)));
+typedef short mf_s4 __attribute__((ext_vector_type(4)));
 
 // A operand of lane l is A[l % 16][l / 16], B operand B[l / 16][l % 16] (one K step of 4);
 // accumulator register r of lane l holds D[row(l, r)][l % 16]
@@ -36,6 +41,32 @@ template <> struct Mfma<double> {
 template <> struct Mfma<float> {
     typedef mf_f4 V;
     __device__ static V mma(float a, float b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+// split-BF16x3 on the bf16 matrix cores.  v_mfma_f32_16x16x16_bf16 takes four consecutive K values per
+// lane (lane l: A[l % 16][4 (l / 16) + t], B[4 (l / 16) + t][l % 16], t = 0..3) and keeps the FP32
+// accumulator layout of v_mfma_f32_16x16x4f32, so one K index of the FP32 form (lane group l / 16)
+// spreads its split products over the four slots t of its lane group.
+struct MfmaSplit3 {
+    typedef mf_f4 V;
+    __device__ static void split(float x, short &h, short &m, short &l) {
+        const unsigned uh = __float_as_uint(x) & 0xffff0000u;
+        const float r1 = x - __uint_as_float(uh);              // exact: at most 16 significant bits
+        const unsigned um = __float_as_uint(r1) & 0xffff0000u;
+        const float r2 = r1 - __uint_as_float(um);             // exact: at most 8 significant bits
+        const unsigned ul = __float_as_uint(r2) & 0xffff0000u;   // == r2
+        h = (short)(uh >> 16); m = (short)(um >> 16); l = (short)(ul >> 16);
+    }
+    __device__ static V mma(float a, float b, V c) {
+        short ah, am, al, bh, bm, bl;
+        split(a, ah, am, al);
+        split(b, bh, bm, bl);
+        const mf_s4 A0 = {ah, ah, am, am}, B0 = {bh, bm, bh, bm};      // hh + hm + mh + mm
+        const mf_s4 A1 = {ah, al, 0, 0}, B1 = {bl, bh, 0, 0};          // hl + lh
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A0, B0, c, 0, 0, 0);
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A1, B1, c, 0, 0, 0);
+    }
     __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
@@ -70,7 +101,7 @@ template <typename V> __device__ inline V vzero() { V z = {0, 0, 0, 0}; return z
 // amdgpu_waves_per_eu(2): two workgroups per CU (the compiler otherwise takes 284 VGPR + AGPR per lane at
 // n = 14 in FP64, one wavefront per SIMD, and every barrier and load of a chunk goes unhidden; with the
 // hint it fits 217 without spilling, and two 70 KB LDS footprints fit a CU)
-template <typename T, int NN>
+template <typename T, int NN, typename MM = Mfma<T>>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_e4_pass_a(const T *__restrict__ E4, const double *__restrict__ B1,
                                                    const double *__restrict__ B2, const double *__restrict__ B3,
                                                    int M, int s, double *C1, double *C2, double *C3) {
@@ -80,7 +111,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     constexpr int KC = (NN + 3) / 4;          // K steps of C3 per chunk (K = n)
     constexpr int T3 = (N2 + 15) / 16;        // row tiles of C3 (rows (a,c))
     constexpr int TW = (T3 + 3) / 4;          // C3 tiles per wavefront
-    typedef typename Mfma<T>::V V;
+    typedef typename MM::V V;
     __shared__ T ech[N3], b1s[N3], b2s[N3];
     __shared__ T b3s[N2];
     __shared__ T part[2][256];                // the upper K halves of C1 and C2, one x4 tile at a time
@@ -141,7 +172,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                     const bool ok = li < NN && kk < N2;
                     const T a = ok ? ech[kk * NN + li] : (T)0;
                     const T b = ok ? b1s[kk * NN + li] : (T)0;
-                    accA[x4] = Mfma<T>::mma(a, b, accA[x4]);
+                    accA[x4] = MM::mma(a, b, accA[x4]);
                 }
             } else {                           // C2[x4][x7][c] += E[x6,x7,x8] B2[(x5,x6,x8),c], K = (x6,x8)
                 for (int ks = k0; ks < k1; ks++) {
@@ -149,7 +180,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                     const bool ok = li < NN && kk < N2;
                     const T a = ok ? ech[(kk / NN) * N2 + li * NN + kk % NN] : (T)0;
                     const T b = ok ? b2s[kk * NN + li] : (T)0;
-                    accA[x4] = Mfma<T>::mma(a, b, accA[x4]);
+                    accA[x4] = MM::mma(a, b, accA[x4]);
                 }
             }
             // C3[(x7,x8)][b] += E[x6,x7,x8] B3[(x4,x5,x6),b], K = x6; row tiles 4u + wave
@@ -161,7 +192,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 for (int u = 0; u < TW; u++) {
                     const int r = 16 * (4 * u + wave) + li;
                     const T a = x6 < NN && r < N2 ? ech[x6 * N2 + r] : (T)0;
-                    acc3[u] = Mfma<T>::mma(a, b, acc3[u]);
+                    acc3[u] = MM::mma(a, b, acc3[u]);
                 }
             }
         }
@@ -178,7 +209,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             const int j = li;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int i = Mfma<T>::row(lane, r);
+                const int i = MM::row(lane, r);
                 const T v = accA[x4][r] + part[wave >> 1][lane + 64 * r];
                 if (i < NN && j < NN) {
                     if (!c2w)
@@ -195,7 +226,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         for (int u = 0; u < TW; u++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int rr = 16 * (4 * u + wave) + Mfma<T>::row(lane, r);
+                const int rr = 16 * (4 * u + wave) + MM::row(lane, r);
                 if (rr < N2 && j < NN) {
                     const int a = rr / NN, c = rr % NN;
                     C3[s + (i64)M * (obase + (i64)N3 * a + (i64)N2 * N2 * j + (i64)N2 * N3 * c)] = (double)acc3[u][r];
@@ -206,14 +237,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 
 // pass B: workgroup g = (cp*n + ap)*n + bp; C4[(x7,x8)][b] = sum over chunks (x3,x5) and x6; the
 // row tiles are spread over the four wavefronts
-template <typename T, int NN>
+template <typename T, int NN, typename MM = Mfma<T>>
 __global__ __launch_bounds__(256) void k_e4_pass_b(const T *__restrict__ E4, const double *__restrict__ B3, int M,
                                                    int s, double *C4) {
     constexpr int N2 = NN * NN, N3 = N2 * NN;
     constexpr int KC = (NN + 3) / 4;
     constexpr int T3 = (N2 + 15) / 16;
     constexpr int TW = (T3 + 3) / 4;           // tiles per wavefront
-    typedef typename Mfma<T>::V V;
+    typedef typename MM::V V;
     __shared__ T ech[N3];
     __shared__ T b3s[N2];
     const int g = blockIdx.x;
@@ -260,7 +291,7 @@ __global__ __launch_bounds__(256) void k_e4_pass_b(const T *__restrict__ E4, con
                 for (int u = 0; u < TW; u++) {
                     const int r = 16 * (wave + 4 * u) + li;
                     const T a = x6 < NN && r < N2 ? ech[x6 * N2 + r] : (T)0;
-                    acc[u] = Mfma<T>::mma(a, b, acc[u]);
+                    acc[u] = MM::mma(a, b, acc[u]);
                 }
             }
         }
@@ -269,7 +300,7 @@ __global__ __launch_bounds__(256) void k_e4_pass_b(const T *__restrict__ E4, con
     for (int u = 0; u < TW; u++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int rr = 16 * (wave + 4 * u) + Mfma<T>::row(lane, r);
+            const int rr = 16 * (wave + 4 * u) + MM::row(lane, r);
             if (rr < N2 && j < NN) {
                 const int a = rr / NN, c = rr % NN;
                 C4[s + (i64)M * (obase + (i64)N3 * a + (i64)N2 * N2 * j + (i64)N2 * N3 * c)] = (double)acc[u][r];
@@ -292,20 +323,20 @@ void contract_plan_destroy(ContractPlan *p) {
     delete p;
 }
 
-template <typename T, int NN>
+template <typename T, int NN, typename MM>
 static int run_passes(ContractPlan *p, const T *E4, int s, double **outs, hipStream_t st) {
     const unsigned nwg = NN * NN * NN;
-    hipLaunchKernelGGL((k_e4_pass_a<T, NN>), dim3(nwg), dim3(256), 0, st, E4, p->Bk[0], p->Bk[1], p->Bk[2], p->M, s,
-                       outs[0], outs[1], outs[2]);
-    hipLaunchKernelGGL((k_e4_pass_b<T, NN>), dim3(nwg), dim3(256), 0, st, E4, p->Bk[2], p->M, s, outs[3]);
+    hipLaunchKernelGGL((k_e4_pass_a<T, NN, MM>), dim3(nwg), dim3(256), 0, st, E4, p->Bk[0], p->Bk[1], p->Bk[2], p->M,
+                       s, outs[0], outs[1], outs[2]);
+    hipLaunchKernelGGL((k_e4_pass_b<T, NN, MM>), dim3(nwg), dim3(256), 0, st, E4, p->Bk[2], p->M, s, outs[3]);
     NV2_LAUNCH_CHECK();
     return 0;
 }
 
-template <typename T>
+template <typename T, typename MM = Mfma<T>>
 static int dispatch(ContractPlan *p, const T *E4, int s, double **outs, hipStream_t st) {
     switch (p->n) {
-#define CASE(k) case k: return run_passes<T, k>(p, E4, s, outs, st);
+#define CASE(k) case k: return run_passes<T, k, MM>(p, E4, s, outs, st);
         CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
         CASE(14) CASE(15) CASE(16)
 #undef CASE
@@ -322,6 +353,7 @@ int launch_e4_contractions(ContractPlan *p, const void *E4, int prec, const doub
     NV2_LAUNCH_CHECK();
     double *outs[4] = {C1, C2, C3, C4};
     if (prec == 0) return dispatch<double>(p, (const double *)E4, s, outs, st);
+    if (prec == 2) return dispatch<float, MfmaSplit3>(p, (const float *)E4, s, outs, st);
     return dispatch<float>(p, (const float *)E4, s, outs, st);
 }
 

@@ -194,7 +194,7 @@ int nv2_ctx_set_ro4(nv2_ctx *c, const double *ro4) {
 
 int nv2_ctx_run(nv2_ctx *c, int32_t prec, double thr_lindep, double *out) {
     if (!c) return -2;
-    if (prec != NV2_PREC_FP64 && prec != NV2_PREC_FP32) { set_error("nv2_ctx_run: unsupported precision %d", prec); return -2; }
+    if (prec != NV2_PREC_FP64 && prec != NV2_PREC_FP32 && prec != NV2_PREC_BF16X3) { set_error("nv2_ctx_run: unsupported precision %d", prec); return -2; }
     hipStream_t st = c->st;
     const int n = c->n, M = c->M;
     Rdm R{n, M, c->nw, c->find("3-RDM"), c->find("2-RDM"), c->find("1-RDM"), c->find("ro4")};

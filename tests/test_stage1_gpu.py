@@ -65,3 +65,20 @@ def test_bamat_fp32(gpu, case):
     am, at = koopro.bamat(ref["atwo"], P.ro4, P.daaa, P.taa, P.dal, P.nact, P.ncore, ref["aj"], P.norb, P.metat,
                           prec=1)
     _close(am, ref["amatIP"], rel=2e-5, name="amatIP fp32"); _close(at, ref["amatEA"], rel=2e-5, name="amatEA fp32")
+
+
+@pytest.mark.gpu
+def test_bamat_bf16x3(gpu, case):
+    """Split-BF16x3 contractions (three exact bf16 terms per FP32 operand, six products, FP32 accumulation)
+    are as accurate as the FP32 form: FP32 E4 storage and FP32 accumulation set the error, not the split."""
+    P, ref = case
+    am, at = koopro.bamat(ref["atwo"], P.ro4, P.daaa, P.taa, P.dal, P.nact, P.ncore, ref["aj"], P.norb, P.metat,
+                          prec=2)
+    a32, t32 = koopro.bamat(ref["atwo"], P.ro4, P.daaa, P.taa, P.dal, P.nact, P.ncore, ref["aj"], P.norb, P.metat,
+                            prec=1)
+    for got, g32, r, nm in ((am, a32, ref["amatIP"], "amatIP"), (at, t32, ref["amatEA"], "amatEA")):
+        scale = float(np.abs(r).max())
+        e16, e32 = float(np.abs(got - r).max()), float(np.abs(g32 - r).max())
+        print(f"  {nm}: max|bf16x3 - fp64| = {e16:.2e}, max|fp32 - fp64| = {e32:.2e} (scale {scale:.2e})")
+        _close(got, r, rel=2e-5, name=nm + " bf16x3")
+        assert e16 <= 4.0 * e32 + 1e-12 * scale
