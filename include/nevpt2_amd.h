@@ -145,6 +145,13 @@ int nv2_ctx_run(nv2_ctx *ctx, int32_t prec, double thr_lindep, double *out);
  * are not formed (qdnevpt.F:949-991): their table entries are zero and neither
  * "Heff-SC" nor "Heff-PC" contains them.  Unknown names return -2. */
 int nv2_ctx_set_option(nv2_ctx *ctx, const char *name, int64_t value);
+/* Frozen core orbitals: the reference's .FROZEN list (qdnevpt.F:733-745,
+ * input_reader_sections.F90:110-114), 1-based orbital numbers of canonical core
+ * orbitals (ascending orbital energy); zero entries are ignored.  No class uses
+ * them as holes, in SC and PC form and in the QD couplings; they stay occupied
+ * in the Fock operator.  nfrozen = 0 clears the list.  -2 for an orbital outside
+ * the core. */
+int nv2_ctx_set_frozen(nv2_ctx *ctx, int64_t nfrozen, const int64_t *orbitals);
 int nv2_ctx_timings(nv2_ctx *ctx, double *ms);
 int nv2_ctx_get(nv2_ctx *ctx, const char *name, double *host, int64_t count);
 void nv2_ctx_destroy(nv2_ctx *ctx);
@@ -157,6 +164,24 @@ void nv2_ctx_destroy(nv2_ctx *ctx);
  * builders behind nv2_ctx_set_ci (koopro4QD.F90:921-3014). */
 int nv2_ci_rdm(int64_t nact, int64_t nalpha, int64_t nbeta, int32_t k, const double *bra, const double *ket,
                double *out);
+
+/* CI vectors in the reference's file04 form (the record read at
+ * koopro4QD.F90:147-176) converted to the layout nv2_ctx_set_ci takes.
+ * Determinant J (1..ncf) is the closed-shell reference (nocc = NOCA orbitals,
+ * core included, doubly occupied) with the excitations trou(nd(J)+k) ->
+ * part(nd(J)+k), k = 1..ne(J): spin-orbitals 1..norb and norb+1..2norb, the
+ * pairs a+_part a_trou applied in listed order (the phase convention of
+ * Jdifgen, koopro4QD.F90:3763-3886), particle 2*norb+1 removing the electron
+ * (an ion, :188-204).  Replaces giveocc2 (:3123-3167) and the determinant
+ * bookkeeping of the RDM builders.  c holds C(J, state) (J fastest); ci
+ * receives metat vectors C[I_alpha][I_beta] over the active strings (alpha =
+ * spin-orbitals 1..norb), and nalpha/nbeta their electron counts.  ci = NULL
+ * only returns nalpha/nbeta (to size ci: metat * C(nact,nalpha) *
+ * C(nact,nbeta) doubles).  Host code only, no GPU.  -2 when a determinant
+ * changes a core or virtual occupation or the electron counts differ. */
+int nv2_ci_from_file04(int64_t norb, int64_t ncore, int64_t nact, int64_t nocc, int64_t metat, int64_t ncf,
+                       const int16_t *ne, const int64_t *nd, const int16_t *trou, const int16_t *part,
+                       const double *c, double *ci, int64_t *nalpha, int64_t *nbeta);
 
 /* QD-NEVPT2 effective Hamiltonian (qdnevpt.F:1046-1204 with hermitian_heff,
  * :8089-8184): adds e0 to the diagonal of the second-order heff2 (metat x metat,

@@ -58,6 +58,8 @@ def _declare(L):
     L.nv2_ctx_run.argtypes = [C.c_void_p, i32, C.c_double, dp]
     L.nv2_ctx_timings.argtypes = [C.c_void_p, dp]
     L.nv2_ctx_set_option.argtypes = [C.c_void_p, C.c_char_p, i64]
+    L.nv2_ctx_set_frozen.argtypes = [C.c_void_p, i64, C.POINTER(C.c_int64)]
+    L.nv2_ci_from_file04.argtypes = [i64] * 6 + [C.c_void_p] * 6 + [C.POINTER(C.c_int64)] * 2
     L.nv2_ctx_get.argtypes = [C.c_void_p, C.c_char_p, dp, i64]
     L.nv2_ctx_destroy.argtypes = [C.c_void_p]
     L.nv2_ctx_destroy.restype = None

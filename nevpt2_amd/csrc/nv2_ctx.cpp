@@ -300,6 +300,16 @@ int nv2_ctx_set_option(nv2_ctx *c, const char *name, int64_t value) {
     return -2;
 }
 
+int nv2_ctx_set_frozen(nv2_ctx *c, int64_t nfrozen, const int64_t *orbitals) {
+    if (!c || nfrozen < 0 || (nfrozen > 0 && !orbitals)) { set_error("nv2_ctx_set_frozen: invalid argument"); return -2; }
+    std::vector<int> idx;
+    for (int64_t k = 0; k < nfrozen; k++) {
+        if (orbitals[k] == 0) continue;            // the reference ignores zero entries (qdnevpt.F:735)
+        idx.push_back((int)(orbitals[k] - 1));
+    }
+    return stage2_set_frozen(c->s2, (int)idx.size(), idx.data());
+}
+
 int nv2_ctx_timings(nv2_ctx *c, double *ms) {
     if (!c || !ms) return -2;
     for (int k = 0; k < NV2_NPHASE; k++) ms[k] = c->ms[k];

@@ -39,6 +39,11 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
 // Without the 4-RDM terms (the reference's .NO4RDM) the (+1)' and (-1)' classes and their QD
 // couplings are not formed; their table entries stay zero (qdnevpt.F:949-991).
 void stage2_set_no4rdm(Stage2 *p, bool on);
+// Frozen core orbitals (the reference's .FROZEN, qdnevpt.F:733-745): canonical core orbitals, 0-based in
+// ascending orbital energy, that no class uses as a hole.  They stay in the Fock operator; their orbital
+// energy is replaced by -1e30 after canonicalisation, so every term with a hole in one has a denominator
+// of ~1e30 and vanishes to rounding.  Returns -2 for an index outside the core.
+int stage2_set_frozen(Stage2 *p, int nfrozen, const int *core_index);
 int stage2_pm1p_begin(Stage2 *p, int s, double thr);
 // Fills the chain's Koopmans matrix from amat of state s (bamat done) and releases the chain;
 // stage2_state then skips that fill.  No-op without a chain started for state s.

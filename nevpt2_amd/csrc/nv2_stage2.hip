@@ -1455,6 +1455,7 @@ struct Pm1pEarly;
 struct Stage2 {
     S2Inputs in;
     bool no4 = false;              // .NO4RDM: no (+-1)' classes
+    std::vector<int> frozen;       // .FROZEN: canonical core orbitals that are never holes
     Pm1pEarly *early = nullptr;    // the (+-1)' chain when started ahead of stage2_state
     hipEvent_t evS = nullptr, evK = nullptr;   // its metric filled / its Koopmans matrix filled
     hipEvent_t evX = nullptr;                  // its PC right-hand sides built (band-form sums on worker 1)
@@ -2187,6 +2188,24 @@ static int pm1p_worker(Stage2 *p, Pm1pEarly *E) {
 
 void stage2_set_no4rdm(Stage2 *p, bool on) { p->no4 = on; }
 
+int stage2_set_frozen(Stage2 *p, int nfrozen, const int *core_index) {
+    std::vector<int> f;
+    for (int k = 0; k < nfrozen; k++) {
+        if (core_index[k] < 0 || core_index[k] >= p->nc) {
+            set_error("frozen orbital %d is not a core orbital (ncore = %d)", core_index[k] + 1, p->nc);
+            return -2;
+        }
+        f.push_back(core_index[k]);
+    }
+    p->frozen = f;
+    return 0;
+}
+
+__global__ void k_freeze(double *eps, const int *idx, int nf) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nf) eps[idx[k]] = -1.0e30;
+}
+
 int stage2_pm1p_begin(Stage2 *p, int s, double thr) {
     if (p->no4 || !(p->in.amatIP && p->in.amatEA)) return 0;
     if (!p->early) p->early = new Pm1pEarly();
@@ -2624,6 +2643,14 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         GET(wc, "wcore", nc);
         TRYS(syevd(p, false, nc, Cc, wc));
         NV2_HIP(hipMemcpyAsync(eps, wc, sizeof(double) * nc, hipMemcpyDeviceToDevice, st));
+        if (!p->frozen.empty()) {   // .FROZEN: these canonical core orbitals are never holes
+            const int nf = (int)p->frozen.size();
+            int *fi = (int *)p->get("frozen_idx", (nf + 1) / 2);
+            if (!fi) { set_error("hipMalloc failed for frozen_idx"); return -1; }
+            NV2_HIP(hipMemcpyAsync(fi, p->frozen.data(), sizeof(int) * nf, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_freeze, dim3(1), dim3(256), 0, st, eps, fi, nf);
+            NV2_LAUNCH_CHECK();
+        }
     }
     {
         NV2_HIP(hipMemcpy2DAsync(Cv, sizeof(double) * nv, hf + (i64)(nc + n) * norb + nc + n, sizeof(double) * norb,

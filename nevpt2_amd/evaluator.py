@@ -7,6 +7,8 @@ in HBM.
 """
 from __future__ import annotations
 
+import ctypes as C
+
 import numpy as np
 
 from . import _lib as L
@@ -79,6 +81,12 @@ class Evaluator:
     def set_option(self, name, value):
         """Run option: "no4rdm" (the reference's .NO4RDM) skips the A-D matrices and the (+-1)' classes."""
         L.check(L.lib().nv2_ctx_set_option(self._h, name.encode(), int(value)), f"nv2_ctx_set_option({name})")
+
+    def set_frozen(self, orbitals):
+        """The reference's .FROZEN: 1-based canonical core orbital numbers that are never holes."""
+        a = np.ascontiguousarray(orbitals, dtype=np.int64)
+        L.check(L.lib().nv2_ctx_set_frozen(self._h, int(a.size), a.ctypes.data_as(C.POINTER(C.c_int64))),
+                "nv2_ctx_set_frozen")
 
     def set_integrals(self, cholesky, oneint):
         B = L.fin(cholesky)

@@ -51,6 +51,31 @@ module nevpt2_hip
       integer(c_int64_t), value :: value
       integer(c_int) :: rc
     end function
+    function nv2_ctx_set_frozen(ctx, nfrozen, orbitals) bind(C, name="nv2_ctx_set_frozen") result(rc)
+      import :: c_ptr, c_int, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: nfrozen
+      integer(c_int64_t), intent(in) :: orbitals(*)
+      integer(c_int) :: rc
+    end function
+    function nv2_ctx_set_ci(ctx, nalpha, nbeta, ci) bind(C, name="nv2_ctx_set_ci") result(rc)
+      import :: c_ptr, c_int, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: nalpha, nbeta
+      real(c_double), intent(in) :: ci(*)
+      integer(c_int) :: rc
+    end function
+    function nv2_ci_from_file04(norb, ncore, nact, nocc, metat, ncf, ne, nd, trou, part, c, ci, nalpha, nbeta) &
+        bind(C, name="nv2_ci_from_file04") result(rc)
+      import :: c_ptr, c_int, c_int16_t, c_int64_t, c_double
+      integer(c_int64_t), value :: norb, ncore, nact, nocc, metat, ncf
+      integer(c_int16_t), intent(in) :: ne(*), trou(*), part(*)
+      integer(c_int64_t), intent(in) :: nd(*)
+      real(c_double), intent(in) :: c(*)
+      type(c_ptr), value :: ci
+      integer(c_int64_t), intent(out) :: nalpha, nbeta
+      integer(c_int) :: rc
+    end function
     function nv2_ctx_timings(ctx, ms) bind(C, name="nv2_ctx_timings") result(rc)
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx
@@ -95,7 +120,10 @@ module nevpt2_hip
     logical :: no_pc = .false.
     logical :: zorder = .true.
     integer :: prec = 0
-    character(len=256) :: file04 = ' '
+    character(len=256) :: file04 = ' '          ! the reference's file04 (CI vectors, koopro4QD.F90:147-176)
+    character(len=256) :: bundle = ' '          ! this host's integral bundle (nevpt2_amd/bundle.py)
+    integer :: nr_frozen = 0
+    integer(8), allocatable :: frozen(:)        ! .FROZEN orbital numbers (qdnevpt.F:733-745)
   end type
 
 contains
@@ -160,15 +188,49 @@ contains
         case ('.LINDEP'); read(u, *) cfg%thr_lindep
         case ('.STATES'); read(u, *) cfg%nr_states
         case ('.FILE04'); read(u, '(a)') cfg%file04
+        case ('.BUNDLE'); read(u, '(a)') cfg%bundle     ! integrals (+ RDMs without .FILE04): host input
         case ('.NO PC '); cfg%no_pc = .true.
         case ('.NATORB')
-        case ('.FROZEN'); call bad_kw(word, section)
+        case ('.FROZEN')                  ! count, then the orbital numbers (input_reader_sections.F90:110-114)
+          read(u, *) cfg%nr_frozen
+          if (allocated(cfg%frozen)) deallocate(cfg%frozen)
+          allocate(cfg%frozen(max(cfg%nr_frozen, 1)))
+          cfg%frozen = 0
+          if (cfg%nr_frozen > 0) read(u, *) cfg%frozen(1:cfg%nr_frozen)
         case default; call bad_kw(word, section)
         end select
       case default
       end select
     end do
     close(u)
+  end subroutine
+
+  ! The file04 record (koopro4QD.F90:147-176): header, then the excitation lists, the CI coefficients
+  ! and the state energies; the fields after E are not needed here.  NOCA/NOCB are equalised as the
+  ! reference does (koopro4QD.F90:199-201).
+  subroutine read_file04(fname, norb, nocc, metat, ncf, ne, nd, trou, part, c, e)
+    character(*), intent(in) :: fname
+    integer(8), intent(out) :: norb, nocc, metat, ncf
+    integer(2), allocatable, intent(out) :: ne(:), trou(:), part(:)
+    integer(8), allocatable, intent(out) :: nd(:)
+    real(8), allocatable, intent(out) :: c(:, :), e(:)
+    integer(8) :: noca, nocb, nexc, ncoef, j, k
+    integer :: u, ios
+    open(newunit=u, file=fname, form='unformatted', access='sequential', status='old', iostat=ios)
+    if (ios /= 0) stop 'ERROR: cannot open file04'
+    read(u) norb, noca, nocb, metat, ncf, nexc, ncoef
+    rewind(u)
+    allocate(ne(ncf), nd(ncf), trou(max(nexc, 1_8)), part(max(nexc, 1_8)), c(ncf, metat), e(metat))
+    read(u) norb, noca, nocb, metat, ncf, nexc, ncoef, (ne(j), nd(j), j = 1, ncf), (trou(j), part(j), j = 1, nexc), &
+        ((c(j, k), j = 1, ncf), k = 1, metat), (e(j), j = 1, metat)
+    close(u)
+    nocc = min(noca, nocb)
+    write(6, '(a)') ' FILE04 informations:'
+    write(6, '(a,i8)') ' NORB = ', norb
+    write(6, '(a,i8)') ' NOCA = ', noca
+    write(6, '(a,i8)') ' NOCB = ', nocb
+    write(6, '(a,i8)') ' METAT= ', metat
+    write(6, '(a,i8)') ' NCF  = ', ncf
   end subroutine
 
   subroutine bad_kw(word, section)

@@ -1,8 +1,10 @@
-! Fortran host of the MI355X NEVPT2 hot path: reads input.nevpt, the input
-! bundle named by .FILE04 (packed 4-RDM, Cholesky vectors, one-electron
-! integrals, reference energies and, for several roots, the transition RDMs),
-! runs one device-resident evaluation and prints the reference's per-class
-! table, the QD effective Hamiltonians and the summary (qdnevpt.F:1004-1230).
+! Fortran host of the MI355X NEVPT2 hot path: reads input.nevpt, the integral
+! bundle named by .BUNDLE (Cholesky vectors, one-electron integrals, reference
+! energies and, without .FILE04, the packed 4-RDM and for several roots the
+! transition RDMs), and with .FILE04 the reference's file04 CI vectors, whose
+! RDMs are then built on the GPU; runs one device-resident evaluation and prints
+! the reference's per-class table, the QD effective Hamiltonians and the summary
+! (qdnevpt.F:1004-1230).
 program nevpt2_amd_main
   use iso_c_binding
   use nevpt2_hip
@@ -13,8 +15,14 @@ program nevpt2_amd_main
   real(c_double), allocatable :: e0(:), oneint(:), chol(:), ro4(:), tab(:), ms(:)
   real(c_double), allocatable :: t3(:), t2(:), t1(:), hsc(:), hpc(:), hh(:), esc(:), epc(:)
   integer(c_int64_t) :: npr
-  logical :: qd
+  logical :: qd, from_ci
   character(len=8) :: magic
+  integer(8) :: f_norb, f_nocc, f_metat, f_ncf
+  integer(2), allocatable :: f_ne(:), f_trou(:), f_part(:)
+  integer(8), allocatable :: f_nd(:)
+  real(8), allocatable :: f_c(:, :), f_e(:)
+  real(c_double), allocatable, target :: ci(:)
+  integer(c_int64_t) :: nalpha, nbeta
   integer :: u, ios, s, k, i
   integer(c_int) :: rc
   real(8) :: sc_tot, pc_tot, nsc_tot, npc_tot
@@ -22,8 +30,9 @@ program nevpt2_amd_main
                                            "(+1)'", "(-1)'", "(0)' "]
 
   call read_input('input.nevpt', cfg)
-  if (cfg%file04 == ' ') stop 'ERROR: .FILE04 (input bundle) is mandatory input'
-  open(newunit=u, file=trim(cfg%file04), access='stream', form='unformatted', status='old', iostat=ios)
+  if (cfg%bundle == ' ') stop 'ERROR: .BUNDLE (integral bundle) is mandatory input'
+  from_ci = cfg%file04 /= ' '
+  open(newunit=u, file=trim(cfg%bundle), access='stream', form='unformatted', status='old', iostat=ios)
   if (ios /= 0) stop 'ERROR: cannot open the input bundle'
   read(u) magic
   if (magic /= 'NV2BUNDL') stop 'ERROR: not an nevpt2_amd input bundle'
@@ -34,11 +43,16 @@ program nevpt2_amd_main
   norb = ncore + nact + nvirt
   npair = norb * (norb + 1) / 2
   nwords = nact * (nact + 1) * (nact + 2) * (nact + 3) / 24
-  allocate(e0(metat), oneint(npair), chol(nchovec * npair), ro4(nwords * nact**4 * metat))
-  read(u) e0, oneint, chol, ro4
-  qd = metat > 1 .and. hdr(1) >= 2
+  allocate(e0(metat), oneint(npair), chol(nchovec * npair))
+  read(u) e0, oneint, chol
+  if (.not. from_ci) then
+    if (hdr(1) == 3) stop 'ERROR: the bundle holds integrals only: give the CI vectors with .FILE04'
+    allocate(ro4(nwords * nact**4 * metat))
+    read(u) ro4
+  end if
+  qd = metat > 1 .and. (hdr(1) == 2 .or. from_ci)
   if (metat > 1 .and. .not. qd) stop 'ERROR: several roots need the transition RDMs (bundle version 2)'
-  if (qd) then
+  if (qd .and. .not. from_ci) then
     npr = metat * (metat - 1) / 2
     allocate(t3(npr * nact**6), t2(npr * nact**4), t1(npr * nact**2))
     read(u) t3, t2, t1
@@ -51,8 +65,24 @@ program nevpt2_amd_main
     stop 'ERROR: device context'
   end if
   rc = nv2_ctx_set_integrals(ctx, chol, oneint)
-  if (rc == 0) rc = nv2_ctx_set_ro4(ctx, ro4)
-  if (rc == 0 .and. qd) rc = nv2_ctx_set_trdm(ctx, t3, t2, t1)
+  if (from_ci) then
+    ! file04 CI vectors -> string-ordered arrays -> RDMs (and transition RDMs) on the GPU; E0 from file04
+    call read_file04(trim(cfg%file04), f_norb, f_nocc, f_metat, f_ncf, f_ne, f_nd, f_trou, f_part, f_c, f_e)
+    if (f_norb /= norb .or. f_metat /= metat) stop 'ERROR: file04 does not match the integral bundle'
+    if (rc == 0) rc = nv2_ci_from_file04(norb, ncore, nact, f_nocc, metat, f_ncf, f_ne, f_nd, f_trou, f_part, &
+                                         f_c, c_null_ptr, nalpha, nbeta)
+    if (rc == 0) then
+      if (nalpha + nbeta /= nele) stop 'ERROR: file04 electron count does not match .NACTEL / the bundle'
+      allocate(ci(metat * nstr(nact, nalpha) * nstr(nact, nbeta)))
+      rc = nv2_ci_from_file04(norb, ncore, nact, f_nocc, metat, f_ncf, f_ne, f_nd, f_trou, f_part, f_c, &
+                              c_loc(ci), nalpha, nbeta)
+    end if
+    if (rc == 0) rc = nv2_ctx_set_ci(ctx, nalpha, nbeta, ci)
+    e0 = f_e
+  else
+    if (rc == 0) rc = nv2_ctx_set_ro4(ctx, ro4)
+    if (rc == 0 .and. qd) rc = nv2_ctx_set_trdm(ctx, t3, t2, t1)
+  end if
   ! .NO4RDM: no A-D matrices and no (+1)'/(-1)' classes (koopro4QD.F90:357, qdnevpt.F:949-991)
   if (cfg%no_4rdm_terms) then
     write(6, '(a)') ' WARNING: .NO4RDM has been specified. V(+1p) and'
@@ -64,6 +94,10 @@ program nevpt2_amd_main
     write(6, '(a)') ' far more than 1 microhartree; compare with a run without .MIXED.'
   end if
   if (rc == 0 .and. cfg%no_4rdm_terms) rc = nv2_ctx_set_option(ctx, 'no4rdm' // c_null_char, 1_c_int64_t)
+  if (rc == 0 .and. cfg%nr_frozen > 0) then
+    write(6, '(/a/(5x,18i4))') '     Frozen orbitals:', cfg%frozen(1:cfg%nr_frozen)
+    rc = nv2_ctx_set_frozen(ctx, int(cfg%nr_frozen, c_int64_t), cfg%frozen)
+  end if
   allocate(tab(32 * metat), ms(NV2_NPHASE))
   if (rc == 0) rc = nv2_ctx_run(ctx, int(cfg%prec, c_int32_t), real(cfg%thr_lindep, c_double), tab)
   if (rc /= 0) then
@@ -135,6 +169,16 @@ program nevpt2_amd_main
   write(6, '(a,6f10.3)') ' device phase times (ms): ', ms
 
 contains
+
+  ! number of strings of k electrons in n orbitals
+  integer(c_int64_t) function nstr(n, k)
+    integer(c_int64_t), intent(in) :: n, k
+    integer(c_int64_t) :: i
+    nstr = 1
+    do i = 1, k
+      nstr = nstr * (n - k + i) / i
+    end do
+  end function
 
   ! matout layout (qdnevpt.F:1639-1659): five columns per block
   subroutine matout(a, m)

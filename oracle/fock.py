@@ -285,6 +285,7 @@ class FullSpace:
         self.occ = np.stack([((b >> p) & 1) + ((b >> (norb + p)) & 1) for p in range(norb)], axis=1)
         self.h0, self.g0 = dense_integrals(P)
         self.Es = {}
+        self.frozen = ()
         self.triangle = "upper"
         # the reference's PC basis order for (+2) and (-2) (pairs (b, a) with b slow) and classes whose K
         # it stores with kets as rows ((+1)' and (-1)': coef(abc, a'b'c') = amat(a',b',c',a,b,c))
@@ -379,8 +380,9 @@ class FullSpace:
         return np.all(self.occ[:, ext] == want[ext][None, :], axis=1)
 
     def labels(self, k):
+        """External label sets of class k; frozen core orbitals (the reference's .FROZEN) are never holes."""
         nc, n, norb = self.nc, self.n, self.norb
-        core = range(nc)
+        core = [i for i in range(nc) if i not in self.frozen]
         virt = range(nc + n, norb)
         if k == 0:
             return [((i, j), (r, s)) for i in core for j in core if i <= j for r in virt for s in virt if r <= s]
@@ -532,10 +534,12 @@ class FullSpace:
         return sc, pc
 
 
-def nevpt2(P, s=0, thr=1.0e-6, classes=range(8)):
+def nevpt2(P, s=0, thr=1.0e-6, classes=range(8), frozen=()):
     """Per-class (norm_sc, e_sc, norm_pc, e_pc) of state s, in the reference's class order, plus the
-    state's E_0 residual |H_D Psi - E_0 Psi| and the metric eigenvalue gap per class."""
+    state's E_0 residual |H_D Psi - E_0 Psi| and the metric eigenvalue gap per class.  ``frozen``:
+    canonical core orbitals (0-based, ascending energy) excluded as holes."""
     fsp = FullSpace(P, [P.C[t] for t in range(P.metat)], thr).setup_state(s)
+    fsp.frozen = tuple(frozen)
     out = np.zeros((8, 4))
     gaps = {}
     for k in classes:

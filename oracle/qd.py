@@ -532,7 +532,7 @@ CLASS_OFF = [(1, v1k_off), (2, vm1k_off), (3, v2mod_off), (4, vm2mod_off), (7, v
              (5, lambda c, T, thr: pm1_prime_off(c, T, True, thr))]
 
 
-def heff_matrices(P, koop, thr=THR_LINDEP, with_4rdm=True):
+def heff_matrices(P, koop, thr=THR_LINDEP, with_4rdm=True, frozen=()):
     """Second-order Heff (SC, PC) without E0, e2mp/e2en of qdnevpt.F (column i = perturbers of state i),
     the per-state class tables, and the per-class off-diagonal contributions off[k] (SC, PC)."""
     M = P.metat
@@ -541,7 +541,7 @@ def heff_matrices(P, koop, thr=THR_LINDEP, with_4rdm=True):
     tables = []
     off = {}
     for i in range(M):
-        tab, c = S2.state_energies(P, koop, i, thr, with_4rdm)
+        tab, c = S2.state_energies(P, koop, i, thr, with_4rdm, frozen)
         tables.append(tab)
         Hsc[i, i] = tab[:, 1].sum()
         Hpc[i, i] = tab[:, 3].sum()

@@ -70,10 +70,19 @@ def pc_solve(S, K, thr=THR_LINDEP):
     return caux @ V, w2, int(keep.sum()), w1
 
 
-class Ctx:
-    """Per-state quantities shared by the classes."""
+FROZEN_EPS = -1.0e30
 
-    def __init__(self, P, koop, s=0):
+
+class Ctx:
+    """Per-state quantities shared by the classes.
+
+    ``frozen`` lists canonical core orbitals (0-based, ascending orbital energy) that the reference's
+    .FROZEN keyword keeps out of every class's external core loops (``if (zgel(ii)) goto``,
+    qdnevpt.F:733-745, e.g. :1831-1840).  They stay in the Fock operator; here their orbital energy is
+    set to FROZEN_EPS, which gives every term with a hole in them a denominator of ~1e30, so each
+    such term vanishes to rounding -- the skip, without touching each class's loops."""
+
+    def __init__(self, P, koop, s=0, frozen=()):
         nc, n, nv, norb = P.ncore, P.nact, P.nvirt, P.norb
         self.nc, self.n, self.nv, self.norb = nc, n, nv, norb
         self.s = s
@@ -86,6 +95,10 @@ class Ctx:
         self.h, self.Bf = h, Bf
         self.fock(P)
         self.transform()
+        for i in frozen:
+            if not 0 <= i < nc:
+                raise ValueError(f"frozen orbital {i} is not a core orbital (ncore={nc})")
+            self.eps[i] = FROZEN_EPS
 
     # ------------------------------------------------------------ Fock + canonicalisation (qdnevpt.F:369-784)
     def fock(self, P):
@@ -539,10 +552,10 @@ def vmupE(c, thr=THR_LINDEP):
 CLASS_NAMES = ["(0)", "(+1)", "(-1)", "(+2)", "(-2)", "(+1)'", "(-1)'", "(0)'"]
 
 
-def state_energies(P, koop, s=0, thr=THR_LINDEP, with_4rdm=True):
+def state_energies(P, koop, s=0, thr=THR_LINDEP, with_4rdm=True, frozen=()):
     """Per-class (norm_sc, e_sc, norm_pc, e_pc) for state s, in the reference's class order
-    (0),(+1),(-1),(+2),(-2),(+1)',(-1)',(0)' (qdnevpt.F:1019-1045)."""
-    c = Ctx(P, koop, s)
+    (0),(+1),(-1),(+2),(-2),(+1)',(-1)',(0)' (qdnevpt.F:1019-1045); ``frozen``: see Ctx."""
+    c = Ctx(P, koop, s, frozen)
     out = np.zeros((8, 4))
     for k, fn in ((0, v0), (1, v1k), (2, vm1k), (3, v2mod), (4, vm2mod), (7, v0pmod)):
         e_sc, p_sc, e_pc, p_pc = fn(c, thr) if fn is not v0 else fn(c)

@@ -23,7 +23,7 @@ INPUT = """*MODEL
 *QDNEVP
 .LINDEP
 1.0d-6
-.FILE04
+.BUNDLE
 {bundle}
 *END OF
 """
@@ -108,3 +108,56 @@ def test_host_no4rdm_skips_the_prime_classes(gpu, tmp_path):
     epc, _ = Q.diagonalize_heff(Hpc, e0)
     assert np.abs(summ[:, 1] - esc).max() < 2e-6 * max(1.0, np.abs(esc).max())
     assert np.abs(summ[:, 2] - epc).max() < 2e-6 * max(1.0, np.abs(epc).max())
+
+
+FILE04_INPUT = """*MODEL
+.CHOLES
+*KOOPRO
+.STATES
+{nstates}
+.NACTEL
+{nele}
+.FILE04
+file04
+*QDNEVP
+.LINDEP
+1.0d-6
+.BUNDLE
+ints.bin
+.FILE04
+file04
+*END OF
+"""
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("roots", [1, 2])
+def test_host_reads_file04(gpu, tmp_path, roots):
+    """CI vectors from the reference's file04 (excitation lists, shuffled hole/particle order), integrals
+    from an integrals-only bundle: the host converts the lists, builds the RDMs (and transition RDMs) on the
+    GPU and reproduces the oracle's table and QD summary; E0 comes from file04."""
+    from nevpt2_amd import file04
+    from oracle import cipsi
+    _need_exe()
+    print(f"[fortran host] file04 CAS(4,4) nc=2 nv=3, {roots} root(s)", flush=True)
+    P = synth.Problem(4, 2, 3, seed=1, nroots=roots)
+    e0 = np.array([-1.5, -1.2])[:roots]
+    f = cipsi.lists_from_ci(P.C, P.nact, P.ncore, P.norb, 2, 2, seed=7)
+    file04.write(tmp_path / "file04", P.norb, f["noca"], f["ne"], f["nd"], f["trou"], f["part"], f["c"], e0)
+    bundle.write_bundle(tmp_path / "ints.bin", P, e0=np.zeros(roots), rdms=False)
+    (tmp_path / "input.nevpt").write_text(FILE04_INPUT.format(nstates=roots, nele=P.nele))
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "FILE04 informations" in r.stdout
+    tabs = bundle.read_table(r.stdout)
+    koop = S1.koopmans_all(P)
+    for s in range(roots):
+        ref, _ = S2.state_energies(P, koop, s)
+        assert (np.abs(tabs[s][:, 1] - ref[:, 1]) <= energy_tol(ref[:, 0])).all()
+        assert (np.abs(tabs[s][:, 3] - ref[:, 3]) <= energy_tol(ref[:, 2])).all()
+    if roots > 1:
+        summ = bundle.read_qd_summary(r.stdout)
+        Hsc, Hpc, _, _ = Q.heff_matrices(P, koop)
+        esc, _ = Q.diagonalize_heff(Hsc, e0)
+        assert np.abs(summ[:, 0] - e0).max() < 1e-6
+        assert np.abs(summ[:, 1] - esc).max() < 2e-6 * max(1.0, np.abs(esc).max())

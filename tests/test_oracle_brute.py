@@ -227,3 +227,16 @@ def test_plausible_slips_fail(slip):
     ref, _ = S2.state_energies(P, k2, 0)
     d = max(abs(ref[cls, 1] - bf[cls, 1]), abs(ref[cls, 3] - bf[cls, 3]))
     assert d > 1e-9, (slip, d)
+
+
+@pytest.mark.parametrize("frozen", [(0,), (0, 2)], ids=["one", "two"])
+def test_frozen_core_orbitals_are_never_holes(frozen):
+    """.FROZEN (qdnevpt.F:733-745): the oracle's orbital-energy device for frozen core orbitals equals the
+    reference's skip, i.e. NEVPT2 from first-order functions whose external labels exclude those holes."""
+    P = _eigen_problem(4, 3, 2, 11, 4)
+    koop = S1.koopmans_all(P)
+    ref, _ = S2.state_energies(P, koop, 0, frozen=frozen)
+    bf, _, _ = fock.nevpt2(P, frozen=frozen)
+    full, _ = S2.state_energies(P, koop, 0)
+    _compare(ref, bf, rel=1e-11, norm_rel=0.0)
+    assert abs(full[:, 1].sum() - ref[:, 1].sum()) > 1e-3       # freezing changes the energy
