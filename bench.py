@@ -47,7 +47,8 @@ TOL_EH = 1.0e-6
 CLASS_NAMES = ["(0)", "(+1)", "(-1)", "(+2)", "(-2)", "(+1)'", "(-1)'", "(0)'"]
 PROFILES = os.path.join(ROOT, "profiles")
 TRAFFIC = {   # (kernel, config, precision) -> committed rocprofv3 counter summary holding its HBM bytes
-    ("contractions", "c2", "fp64"): ("e4_contractions_cas10_fp64_pmc.md", "traffic_bytes_per_eval:"),
+    ("contractions", "c3", "fp64"): ("e4_contractions_cas14_fp64_pmc.md", "traffic_bytes_per_eval:"),
+    ("e4_build", "c3", "fp64"): ("e4_contractions_cas14_fp64_pmc.md", "e4_build_traffic_bytes_per_eval:"),
     ("trd_col", "c3", "fp64"): ("trd_col_cas14_fp64_pmc.md", "traffic_bytes_per_eval:"),
 }
 
@@ -283,7 +284,8 @@ def main():
     by4 = e4_build_bytes(n, M, esize)
     ach4 = by4 / t_e4 / 1e9 if t_e4 > 0 else 0.0
     rl.append({"kernel": "E4 build (k_e4_d3t + k_build_e4, pair multisets)", "bound": "hbm", "achieved": ach4,
-               "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach4 / (HBM_PEAK / 1e9), "traffic": None,
+               "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach4 / (HBM_PEAK / 1e9),
+               "traffic": measured_traffic("e4_build", args.config, head),
                "device_ms_per_eval": t_e4 * 1e3, "launches_per_eval": 2 * M,
                "algorithmic": f"E4 written once + packed Gamma4 and 3-RDM read once = {by4:.3e} B per evaluation"})
     # the headline line is the hot path's own MFMA kernel pair (the 4-RDM contractions): a single kernel family
