@@ -143,7 +143,18 @@ int nv2_ctx_run(nv2_ctx *ctx, int32_t prec, double thr_lindep, double *out);
  * keyword (nevpt2_cfg no_4rdm_terms): the A-D matrices are not built
  * (koopro4QD.F90:357-424) and the (+1)' and (-1)' classes and their QD couplings
  * are not formed (qdnevpt.F:949-991): their table entries are zero and neither
- * "Heff-SC" nor "Heff-PC" contains them.  Unknown names return -2. */
+ * "Heff-SC" nor "Heff-PC" contains them.
+ * "natorb" (value != 0) is the reference's .NATORB keyword (compute_rho1st and
+ * compute_nooccn, qdnevpt.F:1240-1467): each run also forms, per state, the density
+ * matrix to first order that the reference diagonalises -- core block 2, active
+ * block the 1-RDM, and the core-virtual, active-virtual and core-active blocks
+ * <Psi_0|E_pq|Psi_1> accumulated by the PC classes (0)', (-1)' and (+1)'
+ * (qdnevpt.F:4823-4988, 5326-5424, 5113-5203; zero without the 4-RDM classes),
+ * virtual block 0.  nv2_ctx_get then returns "rho1" (norb x norb per state, in the
+ * state's canonical core and virtual orbitals, as the reference prints it),
+ * "rho1-mo" (the same matrix in the input orbitals, U rho U^T with U the
+ * canonicalising rotation) and "natocc" (norb per state: its eigenvalues in
+ * ascending order, the natural occupation numbers).  Unknown names return -2. */
 int nv2_ctx_set_option(nv2_ctx *ctx, const char *name, int64_t value);
 /* Frozen core orbitals: the reference's .FROZEN list (qdnevpt.F:733-745,
  * input_reader_sections.F90:110-114), 1-based orbital numbers of canonical core

@@ -42,6 +42,8 @@ struct nv2_ctx {
     double ms[NV2_NPHASE];
     bool has_trdm = false;
     bool no4rdm = false;                    // .NO4RDM: no A-D matrices, no (+1)'/(-1)' classes
+    bool natorb = false;                    // .NATORB: first-order density matrices and their eigenvalues
+    std::vector<double> rho1, rho1_mo, natocc;   // per state: norb^2, norb^2, norb
     std::vector<double> qd;                 // per state s: 8 classes x (SC e2mp(j,s), PC e2en(j,s)), 16*M
     std::vector<double> heff_sc, heff_pc;   // second-order Heff, H[j + M*i] = e2mp(j,i) / e2en(j,i)
     std::vector<double> pcm;                // per state, per class: (min metric eigenvalue, kept, Cholesky path)
@@ -251,10 +253,17 @@ int nv2_ctx_run(nv2_ctx *c, int32_t prec, double thr_lindep, double *out) {
     c->qd.assign(qd ? (size_t)16 * M * M : 0, 0.0);
     c->pcm.assign((size_t)24 * M, 0.0);
     c->pct.assign((size_t)4 * M, 0.0);
+    const size_t no = (size_t)c->norb;
+    c->rho1.assign(c->natorb ? no * no * M : 0, 0.0);
+    c->rho1_mo.assign(c->natorb ? no * no * M : 0, 0.0);
+    c->natocc.assign(c->natorb ? no * M : 0, 0.0);
     for (int s = 0; s < M; s++) {
         S2Stats sd{};
         int rc = stage2_state(c->s2, s, thr_lindep, prec, out + 32 * s, &sd, qd ? c->qd.data() + 16 * M * s : nullptr);
         if (rc) return rc;
+        if (c->natorb &&
+            stage2_rho1_get(c->s2, c->rho1.data() + no * no * s, c->rho1_mo.data() + no * no * s, c->natocc.data() + no * s))
+            return -1;
         for (int k = 0; k < 8; k++) {
             c->pcm[24 * s + 3 * k] = sd.min_metric_eig[k];
             c->pcm[24 * s + 3 * k + 1] = sd.nnew[k];
@@ -296,6 +305,11 @@ int nv2_ctx_set_option(nv2_ctx *c, const char *name, int64_t value) {
         stage2_set_no4rdm(c->s2, c->no4rdm);
         return 0;
     }
+    if (!strcmp(name, "natorb")) {
+        c->natorb = value != 0;
+        stage2_set_rho1(c->s2, c->natorb);
+        return 0;
+    }
     set_error("nv2_ctx_set_option: unknown option '%s'", name);
     return -2;
 }
@@ -320,7 +334,9 @@ int nv2_ctx_get(nv2_ctx *c, const char *name, double *host, int64_t count) {
     if (!c || !name) return -2;
     const std::vector<double> *hv = !strcmp(name, "Heff-SC") ? &c->heff_sc : !strcmp(name, "Heff-PC") ? &c->heff_pc
                                   : !strcmp(name, "QD") ? &c->qd : !strcmp(name, "PC-metric") ? &c->pcm
-                                  : !strcmp(name, "PC-timings") ? &c->pct : nullptr;
+                                  : !strcmp(name, "PC-timings") ? &c->pct : !strcmp(name, "rho1") ? &c->rho1
+                                  : !strcmp(name, "rho1-mo") ? &c->rho1_mo : !strcmp(name, "natocc") ? &c->natocc
+                                  : nullptr;
     if (hv) {
         if ((size_t)count != hv->size()) {
             set_error("nv2_ctx_get(%s): count %lld != %zu (run the context first)", name, (long long)count, hv->size());

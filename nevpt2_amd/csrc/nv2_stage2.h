@@ -44,6 +44,13 @@ void stage2_set_no4rdm(Stage2 *p, bool on);
 // energy is replaced by -1e30 after canonicalisation, so every term with a hole in one has a denominator
 // of ~1e30 and vanishes to rounding.  Returns -2 for an index outside the core.
 int stage2_set_frozen(Stage2 *p, int nfrozen, const int *core_index);
+// .NATORB (qdnevpt.F:1240-1467): with it on, stage2_state also forms the state's density matrix to first
+// order -- core block 2, active block the 1-RDM, and the core-virtual, active-virtual and core-active
+// blocks <Psi_0|E_pq|Psi_1> of the PC classes (0)', (-1)' and (+1)' -- in its canonical core and virtual
+// orbitals; stage2_rho1_get copies the last state's matrix (norb x norb), the same matrix back in the
+// input orbitals, and its eigenvalues in ascending order (any of the three pointers may be null).
+void stage2_set_rho1(Stage2 *p, bool on);
+int stage2_rho1_get(Stage2 *p, double *can, double *mo, double *occ);
 int stage2_pm1p_begin(Stage2 *p, int s, double thr);
 // Fills the chain's Koopmans matrix from amat of state s (bamat done) and releases the chain;
 // stage2_state then skips that fill.  No-op without a chain started for state s.

@@ -925,9 +925,11 @@ __global__ void k_v0p_sc(const double *Icvaa, i64 ldj, const double *Icaav, i64 
 __global__ void k_v0p_pc(const double *Icvaa, i64 ldj, const double *Icaav, i64 ldk, int nc, int n, int nv,
                          const double *sm, const double *sp, const double *spp, const double *dc1, const double *dc2,
                          const double *w, int nnew, const double *heff, const double *ec, const double *ev,
-                         double *part, QDArgs q) {
+                         double *part, QDArgs q, const double *vx, double *rcv) {
     // QD: q.a/b/c[j] = s, s', s'' of pair (i,j) (nn x nnew); q.d/e[j] = ro.coef[:nn], ro.coef[nn:] (nnew)
+    // .NATORB (rcv non-null): rcv[i + nc r] = -sum_mu c_ir,mu vx[mu] (qdnevpt.F:4971-4976)
     __shared__ double dj[256], dk[256];
+    __shared__ double sh16[16];
     const int i = blockIdx.x % nc, r = blockIdx.x / nc;
     const int nn = n * n;
     for (int e = threadIdx.x; e < nn; e += blockDim.x) {
@@ -937,7 +939,7 @@ __global__ void k_v0p_pc(const double *Icvaa, i64 ldj, const double *Icaav, i64 
     }
     __syncthreads();
     const double hh = heff[i + (i64)nc * r];
-    double E = 0.0, P = 0.0, qsc[QM], qpc[QM];
+    double E = 0.0, P = 0.0, qsc[QM], qpc[QM], racc = 0.0;
 #pragma unroll
     for (int j = 0; j < QM; j++) { qsc[j] = 0.0; qpc[j] = 0.0; }
     for (int mu = threadIdx.x; mu < nnew; mu += blockDim.x) {
@@ -953,6 +955,7 @@ __global__ void k_v0p_pc(const double *Icvaa, i64 ldj, const double *Icaav, i64 
         const double co = rimu + rimup, cir = co / deno;
         E -= co * cir;
         P += cir * cir;
+        if (rcv) racc += cir * vx[mu];
 #pragma unroll
         for (int j = 0; j < QM; j++) {
             if (!qd_on(q, j)) continue;
@@ -969,6 +972,24 @@ __global__ void k_v0p_pc(const double *Icvaa, i64 ldj, const double *Icaav, i64 
     }
     block_store4(0.0, 0.0, E, P, part);
     if (q.M) block_storeQ(qsc, qpc, q);
+    if (rcv) {                               // uniform over the block
+        const double R = block_sum(racc, sh16);
+        if (threadIdx.x == 0) rcv[i + (i64)nc * r] = -R;
+    }
+}
+// .NATORB, class (0)' (qdnevpt.F:4854-4861): vx[mu] = sum_ab da(a,b) (2 coef[ab, mu] - coef[n^2 + ab, mu]),
+// ab = a*n+b: the overlap of PC function mu with E_ri Psi_0 (per unit of the external pair)
+__global__ void k_rho_vaux0p(const double *da, int n, const double *coef, int ldc, int nnew, double *vx) {
+    const int mu = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mu >= nnew) return;
+    const int nn = n * n;
+    double s = 0.0;
+    for (int a = 0; a < n; a++)
+        for (int b = 0; b < n; b++) {
+            const i64 o = a * n + b + (i64)ldc * mu;
+            s += da[a + n * b] * (2.0 * coef[o] - coef[nn + o]);
+        }
+    vx[mu] = s;
 }
 // dc[mu] = sum_ab da(b,a) coef[off + ab, mu], ab = a*n+b
 __global__ void k_dacoef(const double *da, int n, const double *coef, int ldc, int off, int nnew, double *dc) {
@@ -1024,6 +1045,42 @@ __global__ void k_fill_pm1p_v(const double *I, int plus, int n, int nc, int next
     int a = row / (n * n), b = (row / n) % n, c = row % n;
     V[e] = plus ? I[(a + (i64)n * c) + ld * (x + (i64)nc * b)] : I[(a + (i64)n * c) + ld * (b + (i64)n * x)];
 }
+// .NATORB, classes (+1)' / (-1)': T[abc + N3 d] = ee2tl(d,b,a,c) (qdnevpt.F:5113-5130) or ee2(d,b,a,c)
+// (:5326-5341), so that vaux(d, mu) = (coef^T T)[mu, d]
+__global__ void k_fill_rho_t(Sl S, int plus, double *T) {
+    const int n = S.n;
+    const i64 N = (i64)n * n * n;
+    const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N * n) return;
+    const i64 row = e % N;
+    const int d = (int)(e / N);
+    const int a = row / (n * n), b = (row / n) % n, c = row % n;
+    T[e] = plus ? s_ee2tl(S, d, b, a, c) : s_ee2(S, d, b, a, c);
+}
+// The .NATORB density matrix of one state in the canonical orbitals (qdnevpt.F:1247-1296): core block
+// 2, active block the 1-RDM, the first-order core-virtual, active-virtual and core-active blocks
+// (zero when absent) mirrored, virtual block 0; and U = diag(C_core, 1, C_virt) for the back-transform.
+__global__ void k_rho_assemble(int nc, int n, int nv, const double *da, const double *rcv, const double *rca,
+                               const double *rav, double *R, double *U, const double *Cc, const double *Cv) {
+    const int norb = nc + n + nv;
+    const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (i64)norb * norb) return;
+    const int p = e % norb, q = e / norb;
+    auto cls = [&](int o) { return o < nc ? 0 : o < nc + n ? 1 : 2; };
+    const int cp = cls(p), cq = cls(q);
+    double v = 0.0, u = 0.0;
+    if (cp == 0 && cq == 0) { v = p == q ? 2.0 : 0.0; u = Cc[p + (i64)nc * q]; }
+    else if (cp == 1 && cq == 1) { v = da[(p - nc) + n * (q - nc)]; u = p == q ? 1.0 : 0.0; }
+    else if (cp == 2 && cq == 2) { u = Cv[(p - nc - n) + (i64)nv * (q - nc - n)]; }
+    else if (cp == 0 && cq == 2) v = rcv ? rcv[p + (i64)nc * (q - nc - n)] : 0.0;
+    else if (cp == 2 && cq == 0) v = rcv ? rcv[q + (i64)nc * (p - nc - n)] : 0.0;
+    else if (cp == 1 && cq == 2) v = rav ? rav[(p - nc) + (i64)n * (q - nc - n)] : 0.0;
+    else if (cp == 2 && cq == 1) v = rav ? rav[(q - nc) + (i64)n * (p - nc - n)] : 0.0;
+    else if (cp == 0 && cq == 1) v = rca ? rca[p + (i64)nc * (q - nc)] : 0.0;
+    else v = rca ? rca[q + (i64)nc * (p - nc)] : 0.0;      // cp == 1, cq == 0
+    R[e] = v;
+    U[e] = u;
+}
 // SC sums per external orbital x
 __global__ void k_pm1p_sc(int plus, int n, int next, const double *H, const double *W1, const double *W2,
                           const double *V, const double *Z1, const double *Z2, const double *Y1, const double *Y2,
@@ -1074,12 +1131,17 @@ __global__ void k_pm1p_sc(int plus, int n, int next, const double *H, const doub
 // (+-1)' PC sums of one external orbital x per thread: (T + s_x) u = z_x with s_x = -e_x for (+1)'
 // (denominator w_mu - e_i) and +e_x for (-1)' (w_mu + e_r); E -= z.u, norm += u.u, and for QD the
 // coupling -z_j.u with z_j = q.a[j] (same layout as Z)
+// .NATORB (rho non-null): rho[x rsx + d rsd] = -u.vt_d for the nr columns vt_d = Vt + ldz d, the
+// projections of the density vectors (qdnevpt.F:5187-5192, 5407-5412)
 __global__ void k_pm1p_shift(int plus, int nnew, int next, const double *d, const double *e, const double *Z, i64 ldz,
-                             const double *eo, double *rows, double *part, QDArgs q) {
+                             const double *eo, double *rows, double *part, QDArgs q, const double *Vt, int nr,
+                             double *rho, i64 rsx, i64 rsd) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    double E = 0.0, P = 0.0, qsc[QM], qpc[QM];
+    double E = 0.0, P = 0.0, qsc[QM], qpc[QM], racc[16];
 #pragma unroll
     for (int j = 0; j < QM; j++) { qsc[j] = 0.0; qpc[j] = 0.0; }
+#pragma unroll
+    for (int k = 0; k < 16; k++) racc[k] = 0.0;
     if (x < next) {
         const double sh = plus ? -eo[x] : eo[x];
         const double *z = Z + ldz * x;
@@ -1090,9 +1152,19 @@ __global__ void k_pm1p_shift(int plus, int nnew, int next, const double *d, cons
 #pragma unroll
             for (int j = 0; j < QM; j++)
                 if (qd_on(q, j)) qpc[j] -= q.a[j][ldz * x + i] * u;
+            if (rho) {
+#pragma unroll
+                for (int k = 0; k < 16; k++)
+                    if (k < nr) racc[k] += u * Vt[i + ldz * k];
+            }
         });
         E = -ez;
         P = uu;
+        if (rho) {
+#pragma unroll
+            for (int k = 0; k < 16; k++)
+                if (k < nr) rho[x * rsx + k * rsd] = -racc[k];
+        }
     }
     block_store4(0.0, 0.0, E, P, part);
     if (q.M) block_storeQ(qsc, qpc, q);
@@ -1455,6 +1527,7 @@ struct Pm1pEarly;
 struct Stage2 {
     S2Inputs in;
     bool no4 = false;              // .NO4RDM: no (+-1)' classes
+    bool rho1 = false;             // .NATORB: first-order density matrix of each state
     std::vector<int> frozen;       // .FROZEN: canonical core orbitals that are never holes
     Pm1pEarly *early = nullptr;    // the (+-1)' chain when started ahead of stage2_state
     hipEvent_t evS = nullptr, evK = nullptr;   // its metric filled / its Koopmans matrix filled
@@ -2098,6 +2171,10 @@ struct Pm1pEarly {
     const double *X[2] = {nullptr, nullptr}, *eo[2] = {nullptr, nullptr};
     int next[2] = {0, 0};
     double *Ypc[2] = {nullptr, nullptr}, *Ub = nullptr, *yb = nullptr, *ptb = nullptr;
+    // .NATORB: density vectors T (N3 x n), their projections, and the density block of each class
+    const double *Tr[2] = {nullptr, nullptr};
+    double *Vr[2] = {nullptr, nullptr}, *rho[2] = {nullptr, nullptr};
+    int nact = 0;
     int rc = 0;
     char err[512] = "";
     PcJob job;
@@ -2118,6 +2195,13 @@ static void pm1p_stop(Stage2 *p) {   // abort a chain still waiting for K, and j
     if (E->th.joinable()) E->th.join();
     E->active = false;
 }
+// .NATORB density block from the banded solutions u_x (Y = [u_x], N x next): -sum_i u_x[i] V[i, d] with
+// V = (caux Q)^T T, as rho_ca[x, d] (nc x n) for (+1)' and rho_av[d, x] (n x nv) for (-1)'
+static int band_rho(rocblas_handle h, hipStream_t st, int plus, int N, int n, int next, const double *Y,
+                    const double *V, double *rho) {
+    if (plus) return gemm_on(h, st, true, false, next, n, N, -1.0, Y, N, V, N, 0.0, rho, next);
+    return gemm_on(h, st, true, false, n, next, N, -1.0, V, N, Y, N, 0.0, rho, n);
+}
 // (-1)' and (+1)' PC sums on the band form, on the chain's stream: y = caux^T x, z = Q^T y, one banded
 // solve per external orbital (both classes in one launch), block partials reduced into the classes
 static int pm1p_band_sums(Stage2 *p, Pm1pEarly *E) {
@@ -2132,6 +2216,8 @@ static int pm1p_band_sums(Stage2 *p, Pm1pEarly *E) {
         if (!next || !E->X[plus]) continue;
         // z = Q^T caux^T x in one GEMM (caux holds caux Q on the band form)
         TRYS(gemm_on(h, st, true, false, N, next, N, 1.0, jb.caux + bb * NN, N, E->X[plus], N, 0.0, E->Ypc[plus], N));
+        if (E->rho[plus])
+            TRYS(gemm_on(h, st, true, false, N, E->nact, N, 1.0, jb.caux + bb * NN, N, E->Tr[plus], N, 0.0, E->Vr[plus], N));
         BandCls &c = bc[plus];
         c.Br = band_rows(N, jb.nb, bb, jb.bandw);
         c.Z = E->Ypc[plus];
@@ -2152,6 +2238,7 @@ static int pm1p_band_sums(Stage2 *p, Pm1pEarly *E) {
         hipLaunchKernelGGL(k_reduce_parts, dim3(4), dim3(256), 0, st, E->ptb + 4 * (c ? n0 : 0), (i64)bc[c].next,
                            p->acc + 4 * (c ? 5 : 6));
         NV2_LAUNCH_CHECK();
+        if (E->rho[c]) TRYS(band_rho(h, st, c, N, E->nact, bc[c].next, bc[c].y, E->Vr[c], E->rho[c]));
     }
     return 0;
 }
@@ -2187,6 +2274,21 @@ static int pm1p_worker(Stage2 *p, Pm1pEarly *E) {
 }
 
 void stage2_set_no4rdm(Stage2 *p, bool on) { p->no4 = on; }
+
+void stage2_set_rho1(Stage2 *p, bool on) { p->rho1 = on; }
+
+int stage2_rho1_get(Stage2 *p, double *can, double *mo, double *occ) {
+    const char *nm[3] = {"rho1_R", "rho1_mo", "rho1_occ"};
+    double *dst[3] = {can, mo, occ};
+    const size_t cnt[3] = {(size_t)p->norb * p->norb, (size_t)p->norb * p->norb, (size_t)p->norb};
+    for (int k = 0; k < 3; k++) {
+        auto it = p->bufs.find(nm[k]);
+        if (it == p->bufs.end() || !it->second.p) { set_error("no .NATORB density computed"); return -2; }
+        if (dst[k]) NV2_HIP(hipMemcpyAsync(dst[k], it->second.p, sizeof(double) * cnt[k], hipMemcpyDeviceToHost, p->st));
+    }
+    NV2_HIP(hipStreamSynchronize(p->st));
+    return 0;
+}
 
 int stage2_set_frozen(Stage2 *p, int nfrozen, const int *core_index) {
     std::vector<int> f;
@@ -2444,6 +2546,17 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
     GET(da, "da", n2); GET(taa, "taa", n4); GET(ro3, "ro3", n6);
     TRYS(slice(p, p->in.dal, s, n2, da)); TRYS(slice(p, p->in.taa, s, n4, taa)); TRYS(slice(p, p->in.daaa, s, n6, ro3));
     Sl S{n, da, taa, ro3};
+    // ---- .NATORB: the first-order density blocks of (0)', (+1)' and (-1)' (zero where a class is absent)
+    double *rho_cv = nullptr, *rho_ca = nullptr, *rho_av = nullptr;
+    if (p->rho1) {
+        rho_cv = p->get("rho1_cv", (i64)nc * nv);
+        rho_ca = p->get("rho1_ca", (i64)nc * n);
+        rho_av = p->get("rho1_av", (i64)n * nv);
+        if (!rho_cv || !rho_ca || !rho_av) { set_error("hipMalloc failed for the .NATORB blocks"); return -1; }
+        NV2_HIP(hipMemsetAsync(rho_cv, 0, sizeof(double) * ((i64)nc * nv > 0 ? (i64)nc * nv : 1), st));
+        NV2_HIP(hipMemsetAsync(rho_ca, 0, sizeof(double) * ((i64)nc * n > 0 ? (i64)nc * n : 1), st));
+        NV2_HIP(hipMemsetAsync(rho_av, 0, sizeof(double) * (i64)n * nv, st));
+    }
     // ---- Koopmans matrices of this state
     GET(k_eako1, "k_eako1", n2); GET(k_ipko1, "k_ipko1", n2); GET(k_hrdm, "k_hrdm", n4); GET(k_eako2, "k_eako2", n4);
     GET(k_ipko2, "k_ipko2", n4); GET(k_ga, "k_ga", n4); GET(k_gd, "k_gd", n4); GET(k_gf, "k_gf", n2);
@@ -2780,6 +2893,7 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
     // (-1)' and (+1)' SC
     double *Vpm[2] = {nullptr, nullptr}, *Hpm[2] = {nullptr, nullptr}, *Q2pm[2] = {nullptr, nullptr};
     double *Xpm[2] = {nullptr, nullptr};
+    double *Trm[2] = {nullptr, nullptr}, *Vrm[2] = {nullptr, nullptr};   // .NATORB density vectors, projections
     for (int plus = 0; has4 && plus < 2; plus++) {
         const int cls = plus ? 5 : 6;
         const int next = plus ? nc : nv;
@@ -2792,6 +2906,13 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         NV2_LAUNCH_CHECK();
         Q2pm[plus] = Q2;
         if (!next) continue;
+        if (p->rho1) {
+            GET(Tr, ("rho1_T" + t).c_str(), N3 * n); GET(Vr, ("rho1_V" + t).c_str(), N3 * n);
+            hipLaunchKernelGGL(k_fill_rho_t, dim3(nb(N3 * n, 256)), dim3(256), 0, st, S, plus, Tr);
+            NV2_LAUNCH_CHECK();
+            Trm[plus] = Tr;
+            Vrm[plus] = Vr;
+        }
         GET(V, ("pm1_V" + t).c_str(), N3 * next);
         hipLaunchKernelGGL(k_fill_pm1p_v, dim3(nb(N3 * next, 256)), dim3(256), 0, st, plus ? Iaaca : Iaaav, plus, n, nc,
                            next, V);
@@ -2849,7 +2970,11 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             E->X[plus] = Xpm[plus];
             E->next[plus] = plus ? nc : nv;
             E->eo[plus] = plus ? ec : ev;
+            E->Tr[plus] = Trm[plus];
+            E->Vr[plus] = Vrm[plus];
+            E->rho[plus] = Trm[plus] ? (plus ? rho_ca : rho_av) : nullptr;
         }
+        E->nact = n;
         NV2_HIP(hipEventRecord(p->evX, st));
         {
             std::lock_guard<std::mutex> lk(E->mu);
@@ -3038,11 +3163,18 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
                 TRYS(gemm(p, true, false, 1, nnew, (int)n2, 1.0, roj, n2, coef + n2, N, 0.0, r1j, 1));
                 q.a[j] = smj; q.b[j] = spj; q.c[j] = sppj; q.d[j] = r0j; q.e[j] = r1j;
             }
+            double *vx = nullptr;
+            if (p->rho1) {
+                vx = p->get("rho1_vx", nnew);
+                if (!vx) { set_error("hipMalloc failed for rho1_vx"); return -1; }
+                hipLaunchKernelGGL(k_rho_vaux0p, dim3(nb(nnew, 128)), dim3(128), 0, st, da, n, coef, N, nnew, vx);
+                NV2_LAUNCH_CHECK();
+            }
             const i64 nbk = (i64)nc * nv;
             double *pt = parts(p, nbk);
             TRYS(qd_prepare(p, q, nbk));
             hipLaunchKernelGGL(k_v0p_pc, dim3((unsigned)nbk), dim3(256), 0, st, Icvaa, Ncv, Icaav, Nca, nc, n, nv, sm,
-                               sp, spp, dc1, dc2, w, nnew, h0p, ec, ev, pt, q);
+                               sp, spp, dc1, dc2, w, nnew, h0p, ec, ev, pt, q, vx, p->rho1 ? rho_cv : nullptr);
             NV2_LAUNCH_CHECK();
             TRYS(reduce_into(p, nbk, 7));
             TRYS(qd_reduce(p, q, nbk, 7));
@@ -3107,6 +3239,7 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             q.a[j] = yj;
         }
         if (jb.band) {                   // caux holds caux Q: Ypc is already z; the banded solves follow the loop
+            if (Trm[plus]) TRYS(gemm(p, true, false, nmax, n, (int)N3, 1.0, caux, N3, Trm[plus], N3, 0.0, Vrm[plus], nmax));
             BandCls &c = bc[plus];
             c.Br = band_rows(nmax, jb.nb, bb, jb.bandw);
             c.Z = Ypc;
@@ -3121,12 +3254,19 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         const double *dT = to.d + (i64)bb * nmax, *eT = to.e + (i64)bb * nmax, *tauT = to.tau + (i64)bb * nmax;
         GET(Wq, "pm1_applyq", trd_apply_workspace(nmax, 1, next * ncq));
         TRYS(trd_apply_q(p->h, st, nmax, Kp, NN, 1, tauT, true, next * ncq, Ypc, 0, Wq));   // z = Q^T y
+        double *rho_b = nullptr;
+        if (Trm[plus]) {                 // the density vectors take the path of the right-hand sides: Q^T caux^T T
+            TRYS(gemm(p, true, false, nmax, n, (int)N3, 1.0, caux, N3, Trm[plus], N3, 0.0, Vrm[plus], nmax));
+            GET(Wr, "pm1_applyq_rho", trd_apply_workspace(nmax, 1, n));
+            TRYS(trd_apply_q(p->h, st, nmax, Kp, NN, 1, tauT, true, n, Vrm[plus], 0, Wr));
+            rho_b = plus ? rho_ca : rho_av;
+        }
         GET(rows, "pm1_rows", (i64)4 * nnew * next);
         const unsigned nbk = nb(next, 64);
         double *pt2 = parts(p, nbk);
         TRYS(qd_prepare(p, q, nbk));
         hipLaunchKernelGGL(k_pm1p_shift, dim3(nbk), dim3(64), 0, st, plus, nnew, next, dT, eT, Ypc, (i64)nmax, eo, rows,
-                           pt2, q);
+                           pt2, q, Vrm[plus], n, rho_b, plus ? (i64)1 : (i64)n, plus ? (i64)nc : (i64)1);
         NV2_LAUNCH_CHECK();
         TRYS(reduce_into(p, nbk, cls));
         TRYS(qd_reduce(p, q, nbk, cls));
@@ -3153,6 +3293,7 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             if (!bc[c].next) continue;
             TRYS(reduce_into(p, bc[c].next, band_cls[c], c ? n0 : 0));
             TRYS(qd_reduce(p, q, bc[c].next, band_cls[c], c ? n0 : 0));
+            if (Trm[c]) TRYS(band_rho(p->h, st, c, nmax, n, bc[c].next, bc[c].y, Vrm[c], c ? rho_ca : rho_av));
         }
     }
     if (stage2_timing()) {
@@ -3166,6 +3307,20 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             fprintf(stderr, "[stage2 s=%d] (+-1)' chain: start %.2f  metric solved %.2f  K ready %.2f  K' reduced %.2f ms\n",
                     s, rel(p->early->t_start), rel(p->early->t_metric), rel(p->early->t_kready), rel(p->early->t_end));
         }
+    }
+    // ---- .NATORB: the density matrix in the canonical orbitals, back in the input orbitals, and its
+    // eigenvalues (dsyev 'L', qdnevpt.F:1414)
+    if (p->rho1) {
+        const i64 no2 = (i64)norb * norb;
+        GET(Rc, "rho1_R", no2); GET(Uf, "rho1_U", no2); GET(Rt, "rho1_tmp", no2); GET(Rmo, "rho1_mo", no2);
+        GET(Rw, "rho1_w", no2); GET(occ, "rho1_occ", norb);
+        hipLaunchKernelGGL(k_rho_assemble, dim3(nb(no2, 256)), dim3(256), 0, st, nc, n, nv, da, rho_cv,
+                           has4 ? rho_ca : nullptr, has4 ? rho_av : nullptr, Rc, Uf, Cc, Cv);
+        NV2_LAUNCH_CHECK();
+        TRYS(gemm(p, false, false, norb, norb, norb, 1.0, Uf, norb, Rc, norb, 0.0, Rt, norb));
+        TRYS(gemm(p, false, true, norb, norb, norb, 1.0, Rt, norb, Uf, norb, 0.0, Rmo, norb));
+        NV2_HIP(hipMemcpyAsync(Rw, Rc, sizeof(double) * no2, hipMemcpyDeviceToDevice, st));
+        TRYS(syevd(p, false, norb, Rw, occ));
     }
     // ---- results: (norm_sc, e_sc, norm_pc, e_pc) per class, SC==PC for class (0)
     double accs[32];

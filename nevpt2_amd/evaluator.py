@@ -79,8 +79,21 @@ class Evaluator:
         return a.reshape(self.M, 4)
 
     def set_option(self, name, value):
-        """Run option: "no4rdm" (the reference's .NO4RDM) skips the A-D matrices and the (+-1)' classes."""
+        """Run options: "no4rdm" (the reference's .NO4RDM) skips the A-D matrices and the (+-1)' classes;
+        "natorb" (.NATORB) also forms each state's first-order density matrix (see natorb())."""
         L.check(L.lib().nv2_ctx_set_option(self._h, name.encode(), int(value)), f"nv2_ctx_set_option({name})")
+
+    def natorb(self):
+        """After a run with the "natorb" option: per state the first-order density matrix in its canonical
+        orbitals, the same matrix in the input orbitals, and its eigenvalues in ascending order; shapes
+        (metat, norb, norb), (metat, norb, norb), (metat, norb)."""
+        no, M = self.norb, self.M
+        out = []
+        for name, cnt in (("rho1", no * no), ("rho1-mo", no * no), ("natocc", no)):
+            a = np.zeros(cnt * M)
+            L.check(L.lib().nv2_ctx_get(self._h, name.encode(), L.ptr(a), a.size), f"nv2_ctx_get({name})")
+            out.append(a.reshape(M, no, no).transpose(0, 2, 1).copy() if cnt == no * no else a.reshape(M, no))
+        return tuple(out)
 
     def set_frozen(self, orbitals):
         """The reference's .FROZEN: 1-based canonical core orbital numbers that are never holes."""

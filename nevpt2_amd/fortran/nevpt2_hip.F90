@@ -119,6 +119,7 @@ module nevpt2_hip
     logical :: no_4rdm_terms = .false.
     logical :: no_pc = .false.
     logical :: zorder = .true.
+    logical :: natorb = .false.                 ! .NATORB: first-order density and natural occupations
     integer :: prec = 0
     character(len=256) :: file04 = ' '          ! the reference's file04 (CI vectors, koopro4QD.F90:147-176)
     character(len=256) :: bundle = ' '          ! this host's integral bundle (nevpt2_amd/bundle.py)
@@ -190,7 +191,7 @@ contains
         case ('.FILE04'); read(u, '(a)') cfg%file04
         case ('.BUNDLE'); read(u, '(a)') cfg%bundle     ! integrals (+ RDMs without .FILE04): host input
         case ('.NO PC '); cfg%no_pc = .true.
-        case ('.NATORB')
+        case ('.NATORB'); cfg%natorb = .true.
         case ('.FROZEN')                  ! count, then the orbital numbers (input_reader_sections.F90:110-114)
           read(u, *) cfg%nr_frozen
           if (allocated(cfg%frozen)) deallocate(cfg%frozen)

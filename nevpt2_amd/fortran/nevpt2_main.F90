@@ -3,8 +3,9 @@
 ! energies and, without .FILE04, the packed 4-RDM and for several roots the
 ! transition RDMs), and with .FILE04 the reference's file04 CI vectors, whose
 ! RDMs are then built on the GPU; runs one device-resident evaluation and prints
-! the reference's per-class table, the QD effective Hamiltonians and the summary
-! (qdnevpt.F:1004-1230).
+! the reference's per-class table, the QD effective Hamiltonians, the summary
+! (qdnevpt.F:1004-1230) and, with .NATORB, the first-order density matrices and
+! natural occupation numbers (qdnevpt.F:1240-1467).
 program nevpt2_amd_main
   use iso_c_binding
   use nevpt2_hip
@@ -14,6 +15,7 @@ program nevpt2_amd_main
   integer(c_int64_t) :: hdr(8), ncore, nact, nvirt, metat, nele, nchovec, norb, npair, nwords
   real(c_double), allocatable :: e0(:), oneint(:), chol(:), ro4(:), tab(:), ms(:)
   real(c_double), allocatable :: t3(:), t2(:), t1(:), hsc(:), hpc(:), hh(:), esc(:), epc(:)
+  real(c_double), allocatable :: rho(:), occ(:)
   integer(c_int64_t) :: npr
   logical :: qd, from_ci
   character(len=8) :: magic
@@ -94,6 +96,7 @@ program nevpt2_amd_main
     write(6, '(a)') ' far more than 1 microhartree; compare with a run without .MIXED.'
   end if
   if (rc == 0 .and. cfg%no_4rdm_terms) rc = nv2_ctx_set_option(ctx, 'no4rdm' // c_null_char, 1_c_int64_t)
+  if (rc == 0 .and. cfg%natorb) rc = nv2_ctx_set_option(ctx, 'natorb' // c_null_char, 1_c_int64_t)
   if (rc == 0 .and. cfg%nr_frozen > 0) then
     write(6, '(/a/(5x,18i4))') '     Frozen orbitals:', cfg%frozen(1:cfg%nr_frozen)
     rc = nv2_ctx_set_frozen(ctx, int(cfg%nr_frozen, c_int64_t), cfg%frozen)
@@ -109,6 +112,11 @@ program nevpt2_amd_main
   allocate(hsc(metat * metat), hpc(metat * metat), hh(metat * metat), esc(metat), epc(metat))
   rc = nv2_ctx_get(ctx, 'Heff-SC' // c_null_char, hsc, metat * metat)
   if (rc == 0) rc = nv2_ctx_get(ctx, 'Heff-PC' // c_null_char, hpc, metat * metat)
+  if (rc == 0 .and. cfg%natorb) then
+    allocate(rho(norb * norb * metat), occ(norb * metat))
+    rc = nv2_ctx_get(ctx, 'rho1' // c_null_char, rho, norb * norb * metat)
+    if (rc == 0) rc = nv2_ctx_get(ctx, 'natocc' // c_null_char, occ, norb * metat)
+  end if
   call nv2_ctx_destroy(ctx)
   if (rc /= 0) stop 'ERROR: effective Hamiltonian'
 
@@ -166,9 +174,56 @@ program nevpt2_amd_main
     end if
   end do
   write(6, '(1x,72("-"))')
+  if (cfg%natorb) call print_natorb()
   write(6, '(a,6f10.3)') ' device phase times (ms): ', ms
 
 contains
+
+  ! .NATORB output (qdnevpt.F:1297-1460): each state's density matrix to first order, then per state its
+  ! eigenvalues as dsyev returns them and their sums over the core, active and virtual counts after the
+  ! descending-modulus order of order_eigvalvec (one irrep: the reference's C1 path)
+  subroutine print_natorb()
+    real(c_double) :: w(norb), t, tot_loc, tot_sum, sum_c, sum_a, sum_v
+    integer :: s, i, j, m
+    do s = 1, int(metat)
+      write(6, '(a,i12)') ' Density matrix 0   for state ', s
+      call matout(rho(1 + norb * norb * (s - 1):norb * norb * s), norb)
+    end do
+    do s = 1, int(metat)
+      tot_sum = 0
+      write(6, '(/a,i3)') ' Natural orbital occupation numbers for state ', s
+      write(6, '(a)') ' !NOTE! Approximate numbers --> no i-i/v-v terms!'
+      write(6, '(a/)') ' ------------------------------------------------'
+      write(6, '(a,i3,a/)') ' irrep ', 1, ': orbital occupations'
+      w = occ(1 + norb * (s - 1):norb * s)
+      write(6, '(6f12.8)') w
+      do i = 1, int(norb) - 1          ! order_eigvalvec: the largest modulus first, later ones win ties
+        m = i
+        t = w(i)
+        do j = i + 1, int(norb)
+          if (abs(w(j)) >= abs(t)) then
+            t = w(j)
+            m = j
+          end if
+        end do
+        if (m /= i) then
+          w(m) = w(i)
+          w(i) = t
+        end if
+      end do
+      sum_c = sum(w(1:ncore))
+      sum_a = sum(w(ncore + 1:ncore + nact))
+      sum_v = sum(w(ncore + nact + 1:norb))
+      tot_loc = sum(w)
+      tot_sum = tot_sum + tot_loc
+      write(6, '(/a)') ' ------------------------------------------------'
+      write(6, '(a,f15.6/)') ' Total sum of occupation numbers =', tot_loc
+      write(6, '(a,f15.6)') ' = core    occupation numbers    :', sum_c
+      write(6, '(a,f15.6)') ' + active  occupation numbers    :', sum_a
+      write(6, '(a,f15.6/)') ' + virtual occupation numbers    :', sum_v
+      write(6, '(a,f15.6/)') ' Final sum of occupation numbers =', tot_sum
+    end do
+  end subroutine
 
   ! number of strings of k electrons in n orbitals
   integer(c_int64_t) function nstr(n, k)

@@ -340,7 +340,7 @@ class FullSpace:
         self.HD = (Hact + sp.diags(ext)).tocsr()
         self.E0 = float(psi @ (self.HD @ psi))
         self.residual = float(np.linalg.norm(self.HD @ psi - self.E0 * psi))
-        self.psi, self.Hpsi, self.eps, self.gam = psi, Hpsi, eps, gam
+        self.psi, self.Hpsi, self.eps, self.gam, self.U = psi, Hpsi, eps, gam, U
         # for a state that is not an eigenfunction of H_D the excitation operators' commutators need
         # H and the excitations applied to H_D|Psi_0> as well: [H_D, O]|Psi> = H_D O|Psi> - O H_D|Psi>
         self.HDpsi = self.HD @ psi
@@ -501,6 +501,39 @@ class FullSpace:
             e_pc -= float(np.sum(y * y / lam))
             n_pc += float(np.sum(y * y / lam ** 2))
         return n_sc, e_sc, n_pc, e_pc, kept_min, drop_max
+
+    def class_rho1(self, k):
+        """First-order density block of PC class k in {5, 6, 7}: <Psi_0|E_pq|Psi_1> with Psi_1 the class's
+        PC first-order function, -sum_mu y_mu / lam_mu Phi_mu over the H_D eigenfunctions Phi_mu of each
+        external label's span.  (0)': [i, r] with E_ir; (+1)': [i, d] with E_id; (-1)': [d, r] with E_dr
+        (canonical core and virtual orbitals, rows and columns as the .NATORB blocks)."""
+        nc, n, nv = self.nc, self.n, self.nv
+        out = {5: np.zeros((nc, n)), 6: np.zeros((n, nv)), 7: np.zeros((nc, nv))}[k]
+        HD, psi = self.HD, self.psi
+        for holes, parts in self.labels(k):
+            Phi, blocks = self.pc_functions(k, holes, parts)
+            S = Phi @ Phi.T
+            PhiH, _ = self.pc_functions(k, holes, parts, self.HDpsi)
+            Kd = self.symmetrise(k, Phi @ (HD @ Phi.T) - Phi @ PhiH.T, blocks)
+            w, Uw = np.linalg.eigh(S)
+            keep = w >= self.thr
+            X = Uw[:, keep] / np.sqrt(w[keep])[None, :]
+            lam, V = np.linalg.eigh(X.T @ Kd @ X)
+            T = V.T @ X.T                                     # Phi_mu = sum_a T[mu, a] Phi_a
+            y = T @ (Phi @ self.Hpsi)
+            psi1 = -((y / lam) @ T) @ Phi
+            if k == 7:
+                (i,), (r,) = holes, parts
+                out[i, r - nc - n] = float(psi1 @ (self.E(r, i) @ psi))
+            elif k == 5:
+                (i,) = holes
+                for d in range(n):
+                    out[i, d] = float(psi1 @ (self.E(nc + d, i) @ psi))
+            else:
+                (r,) = parts
+                for d in range(n):
+                    out[d, r - nc - n] = float(psi1 @ (self.E(r, nc + d) @ psi))
+        return out
 
     def class_couplings(self, k):
         """QD-NEVPT2 couplings of class k into column s of Heff (this state's perturbers):

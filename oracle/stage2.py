@@ -72,6 +72,9 @@ def pc_solve(S, K, thr=THR_LINDEP):
 
 FROZEN_EPS = -1.0e30
 
+# planted mistakes for the tests that show the .NATORB pins catch them (None in every real use)
+_RHO_SLIP = None
+
 
 class Ctx:
     """Per-state quantities shared by the classes.
@@ -401,8 +404,11 @@ def vm2mod(c: Ctx, thr=THR_LINDEP):
     return Esc, Psc, Epc, Ppc
 
 
-def v0pmod(c: Ctx, thr=THR_LINDEP):
-    """Class (0)': SC (qdnevpt.F:4326-4611), PC fillc0p/e20pcont (:4195-4264, :4771-4999)."""
+def v0pmod(c: Ctx, thr=THR_LINDEP, rho=None):
+    """Class (0)': SC (qdnevpt.F:4326-4611), PC fillc0p/e20pcont (:4195-4264, :4771-4999).
+    With a dict ``rho``, also the core-virtual block of the first-order density matrix that e20pcont
+    accumulates for .NATORB (:4823-4861, :4971-4976): rho['cv'][i, r] = -sum_mu c_ir,mu vaux_mu with
+    vaux_mu = sum_ab da(a,b) (2 coef(ab,mu) - coef(ab+n^2,mu))."""
     nc, n, nv = c.nc, c.n, c.nv
     ec, ev = c.eps[:nc], c.eps[nc + n:]
     da, daa = c.da, c.daa
@@ -460,6 +466,11 @@ def v0pmod(c: Ctx, thr=THR_LINDEP):
     sp = Vm @ coef[nn:]
     spp = V2p @ coef[nn:]
     daT = da.T.reshape(-1)                                   # da(b,a) at ab = a*n+b
+    if rho is not None:
+        vaux = (2.0 * coef[:nn] - coef[nn:]).T @ da.reshape(-1)       # da(a,b) at ab = a*n+b
+        if _RHO_SLIP == "v0p_one_coef":
+            vaux = (coef[:nn] - coef[nn:]).T @ da.reshape(-1)
+        rho["cv"] = np.zeros((nc, nv))
     Epc = Ppc = 0.0
     for i in _outer(nc):
         Dj = np.einsum("lr,lab->rab", c.Bcv[:, i, :], c.Baa).reshape(nv, nn)
@@ -472,11 +483,17 @@ def v0pmod(c: Ctx, thr=THR_LINDEP):
         cir = co / deno
         Epc -= float(np.sum(co * cir))
         Ppc += float(np.sum(cir ** 2))
+        if rho is not None:
+            rho["cv"][i] = -(cir @ vaux)
     return Esc, Psc, Epc, Ppc
 
 
-def _pm1_prime(c: Ctx, plus: bool, thr=THR_LINDEP):
-    """Classes (+1)' (vpupE, qdnevpt.F:5437-5867) and (-1)' (vmupE, :5869-6445)."""
+def _pm1_prime(c: Ctx, plus: bool, thr=THR_LINDEP, rho=None):
+    """Classes (+1)' (vpupE, qdnevpt.F:5437-5867) and (-1)' (vmupE, :5869-6445).
+    With a dict ``rho``, also the block of the first-order density matrix that e21pcont (core-active,
+    :5052-5055, :5113-5130, :5187-5192) or e2m1pcont (active-virtual, :5266-5269, :5326-5341, :5407-5412)
+    accumulates for .NATORB: rho[x, d] = -sum_mu co_mu vaux(d, mu), vaux(d, mu) = sum_abc coef(mu, abc)
+    ee2tl(d,b,a,c) for (+1)' and ee2(d,b,a,c) for (-1)'; stored as rho['ca'][i, d] and rho['av'][d, r]."""
     nc, n, nv = c.nc, c.n, c.nv
     ec, ev = c.eps[:nc], c.eps[nc + n:]
     da = c.da
@@ -530,6 +547,18 @@ def _pm1_prime(c: Ctx, plus: bool, thr=THR_LINDEP):
     coefT = coef.T                                                   # coefT[mu, abc]
     Sb = coefT @ M                                                   # S[mu, abc] = sum coefT[mu,a'b'c'] X3(c',a',b',b,a,c)
     sp = coefT @ Q2                                                  # sp[mu, a]
+    if rho is not None:
+        I = np.eye(n)
+        if plus:       # T[abc, d] = ee2tl(d,b,a,c) = -ee2(b,d,a,c) + 2 delta(d,b) da(c,a)
+            T = -np.einsum("bdac->abcd", E2) + 2.0 * np.einsum("db,ca->abcd", I, da)
+            if _RHO_SLIP == "plus_as_minus":
+                T = np.einsum("dbac->abcd", E2)
+        else:          # T[abc, d] = ee2(d,b,a,c)
+            T = np.einsum("dbac->abcd", E2)
+            if _RHO_SLIP == "minus_bc_swapped":
+                T = np.einsum("dcab->abcd", E2)
+        vaux = coefT @ T.reshape(n ** 3, n)                          # vaux[mu, d]
+        blk = np.zeros((next_, n))
     Epc = Ppc = 0.0
     for x in _outer(next_):
         v = np.einsum("acb->abc", D[x]).reshape(-1)                  # dint1(a,c,b) at abc
@@ -538,15 +567,67 @@ def _pm1_prime(c: Ctx, plus: bool, thr=THR_LINDEP):
         co = rmu / deno
         Epc -= float(np.sum(rmu * co))
         Ppc += float(np.sum(co ** 2))
+        if rho is not None:
+            blk[x] = -(co @ vaux)
+    if rho is not None:
+        if plus:
+            rho["ca"] = blk
+        else:
+            rho["av"] = blk.T.copy()
     return Esc, Psc, Epc, Ppc
 
 
-def vpupE(c, thr=THR_LINDEP):
-    return _pm1_prime(c, True, thr)
+def vpupE(c, thr=THR_LINDEP, rho=None):
+    return _pm1_prime(c, True, thr, rho)
 
 
-def vmupE(c, thr=THR_LINDEP):
-    return _pm1_prime(c, False, thr)
+def vmupE(c, thr=THR_LINDEP, rho=None):
+    return _pm1_prime(c, False, thr, rho)
+
+
+def rho1(P, koop, s=0, thr=THR_LINDEP, with_4rdm=True, frozen=()):
+    """The .NATORB density matrix of state s (qdnevpt.F:1240-1303): core block 2, active block the 1-RDM,
+    core-virtual / active-virtual / core-active blocks the first-order terms of the PC classes (0)',
+    (-1)' and (+1)' (zero without the 4-RDM classes), symmetric, virtual block 0; in the canonical core and
+    virtual orbitals of state s.  Returns (rho, rho_mo, occ): rho_mo = U rho U^T back in the input orbitals
+    (U = diag(ccore, 1, cvirt)), occ the eigenvalues of rho in ascending order (the dsyev print of
+    qdnevpt.F:1414-1423)."""
+    c = Ctx(P, koop, s, frozen)
+    nc, n, nv, norb = c.nc, c.n, c.nv, c.norb
+    blocks = {}
+    v0pmod(c, thr, rho=blocks)
+    if with_4rdm:
+        vpupE(c, thr, rho=blocks)
+        vmupE(c, thr, rho=blocks)
+    A, V = slice(nc, nc + n), slice(nc + n, norb)
+    R = np.zeros((norb, norb))
+    R[:nc, :nc] = 2.0 * np.eye(nc)
+    R[A, A] = c.da
+    for key, (r0, c0) in (("cv", (slice(0, nc), V)), ("av", (A, V)), ("ca", (slice(0, nc), A))):
+        if key in blocks:
+            R[r0, c0] = blocks[key]
+            R[c0, r0] = blocks[key].T
+    U = np.zeros((norb, norb))
+    U[:nc, :nc] = c.ccore
+    U[A, A] = np.eye(n)
+    U[V, V] = c.cvirt
+    return R, U @ R @ U.T, np.linalg.eigvalsh(R)
+
+
+def order_occupations(w):
+    """order_eigvalvec (qdnevpt.F:1484-1519): selection sort by descending |w|, the later of equal moduli
+    first (its test is >=)."""
+    w = list(w)
+    m = len(w)
+    for i in range(m - 1):
+        emin, imin = w[i], i
+        for j in range(i + 1, m):
+            if abs(w[j]) >= abs(emin):
+                emin, imin = w[j], j
+        if imin != i:
+            w[imin] = w[i]
+            w[i] = emin
+    return np.array(w)
 
 
 CLASS_NAMES = ["(0)", "(+1)", "(-1)", "(+2)", "(-2)", "(+1)'", "(-1)'", "(0)'"]

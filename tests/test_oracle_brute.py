@@ -12,6 +12,9 @@
 4. The QD couplings of oracle/qd.py equal <H Psi_j|P (E_0 - H_D)^-1 P|H Psi_i> over state i's SC
    function / PC eigenfunctions.
 5. Plausible slips (a transposed A-matrix, a B/C mix-up, a 1e-6 error in a Koopmans matrix) fail (3).
+6. The .NATORB density blocks of oracle/stage2.py (qdnevpt.F:4823-4988, 5052-5203, 5266-5424) equal
+   <Psi_0|E_pq|Psi_1> with Psi_1 the PC first-order function of classes (0)', (+1)' and (-1)', and slips in
+   their density vectors fail.
 """
 import numpy as np
 import pytest
@@ -240,3 +243,61 @@ def test_frozen_core_orbitals_are_never_holes(frozen):
     full, _ = S2.state_energies(P, koop, 0)
     _compare(ref, bf, rel=1e-11, norm_rel=0.0)
     assert abs(full[:, 1].sum() - ref[:, 1].sum()) > 1e-3       # freezing changes the energy
+
+
+def _brute_rho1(P, s=0, frozen=()):
+    """The .NATORB density matrix from the first-order functions, back in the input orbitals."""
+    fsp = fock.FullSpace(P, [P.C[t] for t in range(P.metat)]).setup_state(s)
+    fsp.frozen = tuple(frozen)
+    nc, n, norb = P.ncore, P.nact, P.norb
+    A, V = slice(nc, nc + n), slice(nc + n, norb)
+    B = np.zeros((norb, norb))
+    B[:nc, :nc] = 2.0 * np.eye(nc)
+    B[A, A] = fsp.gam
+    for k, (r0, c0) in ((7, (slice(0, nc), V)), (5, (slice(0, nc), A)), (6, (A, V))):
+        blk = fsp.class_rho1(k)
+        B[r0, c0] = blk
+        B[c0, r0] = blk.T
+    return fsp.U @ B @ fsp.U.T
+
+
+@pytest.mark.parametrize("case", ["eigen_a", "eigen_b", "random", "frozen"])
+def test_natorb_density_equals_first_order_functions(case):
+    """rho_pq = <Psi_0|E_pq|Psi_1> in every first-order block, for the input orbitals (the canonical
+    orbitals' signs cancel in U rho U^T)."""
+    frozen = ()
+    if case == "eigen_a":
+        P = _eigen_problem(4, 2, 2, 11, 4)
+    elif case == "eigen_b":
+        P = _eigen_problem(5, 1, 2, 7, 6)
+    elif case == "random":
+        P = synth.Problem(4, 2, 3, seed=13, nele=4)
+    else:
+        P, frozen = _eigen_problem(4, 3, 2, 11, 4), (1,)
+    koop = S1.koopmans_all(P)
+    _, Rmo, occ = S2.rho1(P, koop, 0, frozen=frozen)
+    Bmo = _brute_rho1(P, 0, frozen)
+    off = np.abs(Rmo - np.diag(np.diag(Rmo))).max()
+    d = float(np.abs(Rmo - Bmo).max())
+    print(f"{case}: max|rho offdiag| {off:.3e}  |d| {d:.2e}")
+    assert off > 1e-2
+    assert d <= (1e-12 if case != "random" else 1e-9) * max(1.0, float(np.abs(Bmo).max()))
+    assert np.abs(occ - np.linalg.eigvalsh(Bmo)).max() <= 1e-9 * max(1.0, float(np.abs(Bmo).max()))
+
+
+@pytest.mark.parametrize("slip", ["v0p_one_coef", "plus_as_minus", "minus_bc_swapped"])
+def test_natorb_slips_fail(slip, monkeypatch):
+    P = _eigen_problem(4, 2, 2, 11, 4)
+    koop = S1.koopmans_all(P)
+    Bmo = _brute_rho1(P)
+    monkeypatch.setattr(S2, "_RHO_SLIP", slip)
+    _, Rmo, _ = S2.rho1(P, koop, 0)
+    d = float(np.abs(Rmo - Bmo).max())
+    assert d > 1e-6, (slip, d)
+
+
+def test_order_occupations_is_order_eigvalvec():
+    """order_eigvalvec (qdnevpt.F:1484-1519) sorts by descending modulus and, with >=, lets the later of
+    two equal moduli go first."""
+    got = S2.order_occupations([0.1, -1.9, 1.9, 0.5])
+    assert list(got) == [1.9, -1.9, 0.5, 0.1]
