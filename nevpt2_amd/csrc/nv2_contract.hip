@@ -93,38 +93,113 @@ __global__ void k_make_bk(int n, const double *atwo, double *B1, double *B2, dou
 
 template <typename V> __device__ inline V vzero() { V z = {0, 0, 0, 0}; return z; }
 
-// pass A: workgroup g = (cp*n + ap)*n + bp.  The chunks (x4,x5) are visited x5-outer, so the B1 and
-// B2 slices of one x5 are staged in LDS once and serve the n chunks (x4,x5); C1 and C2 therefore
-// keep one accumulator tile per x4 in registers.  Wavefronts 0 and 1 split the K range of C1, 2 and
-// 3 that of C2 (their halves are added through LDS at the end), and all four share the C3 row tiles,
-// so every wavefront issues about the same number of MFMAs per chunk.
-// amdgpu_waves_per_eu(2): two workgroups per CU (the compiler otherwise takes 284 VGPR + AGPR per lane at
-// n = 14 in FP64, one wavefront per SIMD, and every barrier and load of a chunk goes unhidden; with the
-// hint it fits 217 without spilling, and two 70 KB LDS footprints fit a CU)
+// LDS layout of one E4 chunk E[x6][x7][x8] (n^3 values): each x7 row holds x8 at stride 1 in RS = 17
+// slots, each x6 block is n rows (S6 = 17 n slots), and the blocks run on to a multiple of 4 with
+// zero blocks.  Every slot the chunk does not fill is zeroed once at the start and never written
+// again, so each MFMA operand is read at a per-lane base plus a compile-time offset (the offset field
+// of ds_read), with no per-lane condition:
+//   C1: A = E[x6,x7,x8 = lane%16] with K = (x6,x7) in order: slot 17 k + lane%16 (past the last k:
+//       the zero blocks);
+//   C2: A = E[x6, x7 = lane%16, x8] with K = (x6, x8) and x8 run to 16 (B zero and A a zero pad slot
+//       for x8 >= n): slot S6 x6 + 17 (lane%16) + x8; 17 being odd, the 16 lanes of one K value hit
+//       16 different bank pairs;
+//   C3/C4: A = E[x6, x7 = tile, x8 = lane%16] with K = x6 run to a multiple of 4 (zero blocks).
+// Rows or columns beyond n land in output rows and columns that are never stored.
+template <int NN> struct ChunkLayout {
+    static constexpr int RS = 17;
+    static constexpr int S6 = NN * RS;
+    static constexpr int KC = (NN + 3) / 4;
+    static constexpr int KA = (NN * NN + 3) / 4;
+    static constexpr int cmax(int a, int b) { return a > b ? a : b; }
+    static constexpr int SIZE = cmax(cmax(4 * KC * S6, 4 * KA * RS + 16), cmax((NN - 1) * S6 + 15 * RS + 16, 512));
+    __device__ static int at(int x6, int x7, int x8) { return x6 * S6 + x7 * RS + x8; }
+};
+
+// One wavefront's share of C1 for one chunk: K steps K0 .. K1-1 of K = (x6,x7) (n^2 values, padded to
+// a multiple of 4 with zero rows of B1).  Fully unrolled, reads at fixed offsets from two bases.
+template <typename T, int NN, typename MM, int K0, int K1>
+__device__ __forceinline__ void kloop_c1(const T *ech, const T *bs, int li, int lk, typename MM::V &acc) {
+    const T *pa = ech + lk * ChunkLayout<NN>::RS + li, *pb = bs + lk * NN + li;
+#pragma unroll
+    for (int ks = K0; ks < K1; ks++) acc = MM::mma(pa[4 * ks * ChunkLayout<NN>::RS], pb[4 * ks * NN], acc);
+}
+// C2 likewise, K = (x6, x8) with x8 in 0..15 (4 n steps; the B2 slice is staged in that order, rows
+// x8 >= n zero)
+template <typename T, int NN, typename MM, int K0, int K1>
+__device__ __forceinline__ void kloop_c2(const T *ech, const T *bs, int li, int lk, typename MM::V &acc) {
+    typedef ChunkLayout<NN> CL;
+    const T *pa = ech + li * CL::RS + lk, *pb = bs + lk * NN + li;
+#pragma unroll
+    for (int ks = K0; ks < K1; ks++) acc = MM::mma(pa[(ks / 4) * CL::S6 + 4 * (ks % 4)], pb[4 * ks * NN], acc);
+}
+
+// C3 (pass A) / C4 (pass B) for one chunk: rows (a,c) = (x7, x8) as one tile per x7, columns b, K = x6
+// (padded to a multiple of 4: zero blocks of the chunk, zero rows of B3).  Wavefront W owns the tiles
+// x7 = W + 4u, u < (n - W + 3) / 4: a compile-time count, so every MFMA is unconditional.
+template <int NN, int W> struct C3Tiles { static constexpr int value = (NN - W + 3) / 4; };
+template <typename T, int NN, typename MM, int W, int TW>
+__device__ __forceinline__ void kloop_c3(const T *ech, const T *b3s, int li, int lk, typename MM::V (&acc)[TW]) {
+    typedef ChunkLayout<NN> CL;
+    constexpr int TWW = C3Tiles<NN, W>::value;
+    static_assert(TWW <= TW, "accumulator tiles");
+    const T *pa = ech + lk * CL::S6 + W * CL::RS + li, *pb = b3s + lk * NN + li;
+#pragma unroll
+    for (int ks = 0; ks < CL::KC; ks++) {
+        const T b = pb[4 * ks * NN];
+#pragma unroll
+        for (int u = 0; u < TWW; u++) acc[u] = MM::mma(pa[4 * ks * CL::S6 + 4 * u * CL::RS], b, acc[u]);
+    }
+}
+// one chunk of pass A for wavefront W: its half of C1 (W = 0, 1) or C2 (W = 2, 3), and its C3 tiles
+template <typename T, int NN, typename MM, int W, int TW>
+__device__ __forceinline__ void chunk_a(const T *ech, const T *b1s, const T *b2s, const T *b3s, int li, int lk,
+                                        typename MM::V &accA, typename MM::V (&acc3)[TW]) {
+    constexpr int KA = ChunkLayout<NN>::KA, KH = (KA + 1) / 2, KA2 = 4 * NN;
+    if (W == 0) kloop_c1<T, NN, MM, 0, KH>(ech, b1s, li, lk, accA);
+    if (W == 1) kloop_c1<T, NN, MM, KH, KA>(ech, b1s, li, lk, accA);
+    if (W == 2) kloop_c2<T, NN, MM, 0, KA2 / 2>(ech, b2s, li, lk, accA);
+    if (W == 3) kloop_c2<T, NN, MM, KA2 / 2, KA2>(ech, b2s, li, lk, accA);
+    kloop_c3<T, NN, MM, W, TW>(ech, b3s, li, lk, acc3);
+}
+
+// pass A: workgroup g = (cp*n + ap)*n + bp.  The chunks (x4,x5) are visited in two groups of x4
+// (G4 values each), x5-outer inside a group, so the B1 and B2 slices of one x5 are staged in LDS once
+// per group and serve its G4 chunks, and C1 and C2 keep one accumulator tile per x4 of the group in
+// registers (G4 tiles instead of n: the register budget of two workgroups per CU).  Wavefronts 0 and 1
+// split the K range of C1, 2 and 3 that of C2 (their halves are added through LDS after each group),
+// and all four share the C3 tiles (one per x7), so every wavefront issues about the same number of
+// MFMAs per chunk.  Each chunk is read from HBM once.
+// amdgpu_waves_per_eu(2): two workgroups per CU, so one's barriers and chunk stores overlap the
+// other's MFMAs (the LDS footprint, 77 KB at n = 14 in FP64, allows two).
 template <typename T, int NN, typename MM = Mfma<T>>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_e4_pass_a(const T *__restrict__ E4, const double *__restrict__ B1,
                                                    const double *__restrict__ B2, const double *__restrict__ B3,
                                                    int M, int s, double *C1, double *C2, double *C3) {
+    typedef ChunkLayout<NN> CL;
     constexpr int N2 = NN * NN, N3 = N2 * NN;
     constexpr int KA = (N2 + 3) / 4;          // K steps of C1 / C2 per chunk (K = n^2)
-    constexpr int KH = (KA + 1) / 2;          // K steps of one half
+    constexpr int KH = (KA + 1) / 2;          // K steps of the lower half
     constexpr int KC = (NN + 3) / 4;          // K steps of C3 per chunk (K = n)
-    constexpr int T3 = (N2 + 15) / 16;        // row tiles of C3 (rows (a,c))
-    constexpr int TW = (T3 + 3) / 4;          // C3 tiles per wavefront
+    constexpr int TW = (NN + 3) / 4;          // C3 tiles per wavefront (one tile per x7)
+    constexpr int G4 = (NN + 1) / 2;          // x4 values per group
+    constexpr int KA2 = 4 * NN;               // K steps of C2 (x8 run to 16)
+    constexpr int BS1 = 4 * KA * NN + 16;     // B1 slice with zero rows and tail
+    constexpr int BS20 = 16 * NN * NN + 16;   // B2 slice in the (x6, x8 < 16) order
+    constexpr int BS2 = BS20 > 512 ? BS20 : 512;   // (and room for two partial tiles)
+    constexpr int B3S = 4 * KC * NN + 16;
     typedef typename MM::V V;
-    __shared__ T ech[N3], b1s[N3], b2s[N3];
-    __shared__ T b3s[N2];
-    __shared__ T part[2][256];                // the upper K halves of C1 and C2, one x4 tile at a time
+    __shared__ T ech[CL::SIZE], b1s[BS1], b2s[BS2], b3s[B3S];
     const int g = blockIdx.x;
     const int bp = g % NN, ap = (g / NN) % NN, cp = g / N2;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
-    const bool c2w = wave >= 2;               // this wavefront's half belongs to C2 (else C1)
-    const int k0 = (wave & 1) * KH, k1 = (wave & 1) ? KA : KH;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: scalar branches
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
     const T *slab = E4 + (i64)g * N2 * N3;   // E4[cp,ap,bp,...]
     const i64 obase = ap + (i64)NN * bp + (i64)N2 * cp;
-    V accA[NN], acc3[TW];
-#pragma unroll
-    for (int u = 0; u < NN; u++) accA[u] = vzero<V>();
+    for (int e = threadIdx.x; e < CL::SIZE; e += 256) ech[e] = (T)0;
+    for (int e = threadIdx.x; e < BS1; e += 256) b1s[e] = (T)0;
+    for (int e = threadIdx.x; e < BS2; e += 256) b2s[e] = (T)0;
+    for (int e = threadIdx.x; e < B3S; e += 256) b3s[e] = (T)0;
+    V acc3[TW];
 #pragma unroll
     for (int u = 0; u < TW; u++) acc3[u] = vzero<V>();
     // the next E4 chunk is loaded into registers while the current one is multiplied
@@ -138,120 +213,123 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     // and the B3 slice of the next chunk (N2 <= 256 values: one per thread)
     static_assert(N2 <= 256, "one B3 value per thread");
     T pb3 = threadIdx.x < N2 ? (T)B3[threadIdx.x] : (T)0;
-    for (int x5 = 0; x5 < NN; x5++) {
+    const bool c2w = wave >= 2;
+    for (int x4g = 0; x4g < NN; x4g += G4) {
+        const int g4 = NN - x4g < G4 ? NN - x4g : G4;       // x4 values in this group (uniform)
+        V accA[G4];
 #pragma unroll
-        for (int x4 = 0; x4 < NN; x4++) {
-            __syncthreads();                   // the previous chunk (and B slices) are consumed
+        for (int u = 0; u < G4; u++) accA[u] = vzero<V>();
+        for (int x5 = 0; x5 < NN; x5++) {
 #pragma unroll
-            for (int u = 0; u < PF; u++) {
-                const int e = threadIdx.x + 256 * u;
-                if (e < N3) ech[e] = pf[u];
-            }
-            if (x4 == 0)
-                for (int e = threadIdx.x; e < N3; e += 256) {
-                    b1s[e] = (T)B1[(i64)x5 * N3 + e];
-                    b2s[e] = (T)B2[(i64)x5 * N3 + e];
+            for (int u = 0; u < G4; u++) {
+                if (u >= g4) break;
+                const int x4 = x4g + u;
+                __syncthreads();               // the previous chunk (and B slices) are consumed
+#pragma unroll
+                for (int v = 0; v < PF; v++) {
+                    const int e = threadIdx.x + 256 * v;
+                    if (e < N3) ech[CL::at(e / N2, (e / NN) % NN, e % NN)] = pf[v];
                 }
-            if (threadIdx.x < N2) b3s[threadIdx.x] = pb3;
-            __syncthreads();
-            {                                  // prefetch the next chunk in visiting order
-                const int nx4 = x4 + 1 < NN ? x4 + 1 : 0, nx5 = x4 + 1 < NN ? x5 : x5 + 1;
-                if (nx5 < NN) {
-                    if (threadIdx.x < N2) pb3 = (T)B3[((i64)nx4 * N2 + nx5 * NN) * NN + threadIdx.x];
-                    const T *nxt = slab + (i64)(nx4 * NN + nx5) * N3;
+                if (u == 0)
+                    for (int e = threadIdx.x; e < N3; e += 256) {
+                        b1s[e] = (T)B1[(i64)x5 * N3 + e];
+                        b2s[((e / N2) * 16 + (e / NN) % NN) * NN + e % NN] = (T)B2[(i64)x5 * N3 + e];
+                    }
+                if (threadIdx.x < N2) b3s[threadIdx.x] = pb3;
+                __syncthreads();
+                {                              // prefetch the next chunk in visiting order
+                    int nx4 = x4 + 1, nx5 = x5;
+                    if (u + 1 >= g4) {         // end of the group's x4 run: next x5, or the next group
+                        nx4 = x4g;
+                        nx5 = x5 + 1;
+                        if (nx5 == NN) { nx4 = x4g + G4; nx5 = 0; }
+                    }
+                    if (nx4 < NN) {
+                        if (threadIdx.x < N2) pb3 = (T)B3[((i64)nx4 * N2 + nx5 * NN) * NN + threadIdx.x];
+                        const T *nxt = slab + (i64)(nx4 * NN + nx5) * N3;
 #pragma unroll
-                    for (int u = 0; u < PF; u++) {
-                        const int e = threadIdx.x + 256 * u;
-                        pf[u] = e < N3 ? nxt[e] : (T)0;
+                        for (int v = 0; v < PF; v++) {
+                            const int e = threadIdx.x + 256 * v;
+                            pf[v] = e < N3 ? nxt[e] : (T)0;
+                        }
+                    }
+                }
+                // C1[x4][x8][a] += E[x6,x7,x8] B1[(x5,x6,x7),a]; C2[x4][x7][c] += E[x6,x7,x8] B2[(x5,x6,x8),c]
+                // and C3[(x7,x8)][b] += E[x6,x7,x8] B3[(x4,x5,x6),b], K = x6
+                switch (wave) {
+                    case 0: chunk_a<T, NN, MM, 0, TW>(ech, b1s, b2s, b3s, li, lk, accA[u], acc3); break;
+                    case 1: chunk_a<T, NN, MM, 1, TW>(ech, b1s, b2s, b3s, li, lk, accA[u], acc3); break;
+                    case 2: chunk_a<T, NN, MM, 2, TW>(ech, b1s, b2s, b3s, li, lk, accA[u], acc3); break;
+                    default: chunk_a<T, NN, MM, 3, TW>(ech, b1s, b2s, b3s, li, lk, accA[u], acc3); break;
+                }
+            }
+        }
+        // C1 / C2 of the group: the upper halves (wavefronts 1 and 3) are added to the lower ones through
+        // the B2 slice buffer (restaged before its next use; its zero rows meet zero pad slots of the
+        // chunk, so what this leaves there does not matter)
+        T *part = b2s;
+#pragma unroll
+        for (int u = 0; u < G4; u++) {
+            if (u >= g4) break;
+            const int x4 = x4g + u;
+            __syncthreads();
+            if (wave & 1)
+#pragma unroll
+                for (int r = 0; r < 4; r++) part[256 * (wave >> 1) + lane + 64 * r] = accA[u][r];
+            __syncthreads();
+            if (!(wave & 1)) {                 // rows x8 (C1) / x7 (C2), columns a (C1) / c (C2)
+                const int j = li;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int i = MM::row(lane, r);
+                    const T v = accA[u][r] + part[256 * (wave >> 1) + lane + 64 * r];
+                    if (i < NN && j < NN) {
+                        if (!c2w)
+                            C1[s + (i64)M * (obase + (i64)N3 * j + (i64)N2 * N2 * x4 + (i64)N2 * N3 * i)] = (double)v;
+                        else
+                            C2[s + (i64)M * (obase + (i64)N3 * i + (i64)N2 * N2 * x4 + (i64)N2 * N3 * j)] = (double)v;
                     }
                 }
             }
-            if (!c2w) {                        // C1[x4][x8][a] += E[x6,x7,x8] B1[(x5,x6,x7),a], K = (x6,x7)
-                for (int ks = k0; ks < k1; ks++) {
-                    const int kk = 4 * ks + lk;
-                    const bool ok = li < NN && kk < N2;
-                    const T a = ok ? ech[kk * NN + li] : (T)0;
-                    const T b = ok ? b1s[kk * NN + li] : (T)0;
-                    accA[x4] = MM::mma(a, b, accA[x4]);
-                }
-            } else {                           // C2[x4][x7][c] += E[x6,x7,x8] B2[(x5,x6,x8),c], K = (x6,x8)
-                for (int ks = k0; ks < k1; ks++) {
-                    const int kk = 4 * ks + lk;
-                    const bool ok = li < NN && kk < N2;
-                    const T a = ok ? ech[(kk / NN) * N2 + li * NN + kk % NN] : (T)0;
-                    const T b = ok ? b2s[kk * NN + li] : (T)0;
-                    accA[x4] = MM::mma(a, b, accA[x4]);
-                }
-            }
-            // C3[(x7,x8)][b] += E[x6,x7,x8] B3[(x4,x5,x6),b], K = x6; row tiles 4u + wave
-#pragma unroll
-            for (int ks = 0; ks < KC; ks++) {
-                const int x6 = 4 * ks + lk;
-                const T b = x6 < NN && li < NN ? b3s[x6 * NN + li] : (T)0;
-#pragma unroll
-                for (int u = 0; u < TW; u++) {
-                    const int r = 16 * (4 * u + wave) + li;
-                    const T a = x6 < NN && r < N2 ? ech[x6 * N2 + r] : (T)0;
-                    acc3[u] = MM::mma(a, b, acc3[u]);
-                }
-            }
         }
     }
-    // C1 / C2: the upper halves (wavefronts 1 and 3) are added to the lower ones through LDS
-#pragma unroll
-    for (int x4 = 0; x4 < NN; x4++) {
-        __syncthreads();
-        if (wave & 1)
-#pragma unroll
-            for (int r = 0; r < 4; r++) part[wave >> 1][lane + 64 * r] = accA[x4][r];
-        __syncthreads();
-        if (!(wave & 1)) {                     // rows x8 (C1) / x7 (C2), columns a (C1) / c (C2)
-            const int j = li;
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int i = MM::row(lane, r);
-                const T v = accA[x4][r] + part[wave >> 1][lane + 64 * r];
-                if (i < NN && j < NN) {
-                    if (!c2w)
-                        C1[s + (i64)M * (obase + (i64)N3 * j + (i64)N2 * N2 * x4 + (i64)N2 * N3 * i)] = (double)v;
-                    else
-                        C2[s + (i64)M * (obase + (i64)N3 * i + (i64)N2 * N2 * x4 + (i64)N2 * N3 * j)] = (double)v;
-                }
-            }
-        }
-    }
-    {                                          // C3: rows (a,c) = (x7,x8), columns b
+    {                                          // C3: rows (a,c) = (x7 = tile, x8), columns b
         const int j = li;
 #pragma unroll
-        for (int u = 0; u < TW; u++)
+        for (int u = 0; u < TW; u++) {
+            const int a = wave + 4 * u;
+            if (a >= NN) continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int rr = 16 * (4 * u + wave) + MM::row(lane, r);
-                if (rr < N2 && j < NN) {
-                    const int a = rr / NN, c = rr % NN;
+                const int c = MM::row(lane, r);
+                if (c < NN && j < NN)
                     C3[s + (i64)M * (obase + (i64)N3 * a + (i64)N2 * N2 * j + (i64)N2 * N3 * c)] = (double)acc3[u][r];
-                }
             }
+        }
     }
 }
 
-// pass B: workgroup g = (cp*n + ap)*n + bp; C4[(x7,x8)][b] = sum over chunks (x3,x5) and x6; the
-// row tiles are spread over the four wavefronts
+// pass B: workgroup g = (cp*n + ap)*n + bp; C4[(x7,x8)][b] = sum over chunks (x3,x5) and x6, one tile per
+// x7, the tiles spread over the four wavefronts
 template <typename T, int NN, typename MM = Mfma<T>>
 __global__ __launch_bounds__(256) void k_e4_pass_b(const T *__restrict__ E4, const double *__restrict__ B3, int M,
                                                    int s, double *C4) {
+    typedef ChunkLayout<NN> CL;
     constexpr int N2 = NN * NN, N3 = N2 * NN;
     constexpr int KC = (NN + 3) / 4;
-    constexpr int T3 = (N2 + 15) / 16;
-    constexpr int TW = (T3 + 3) / 4;           // tiles per wavefront
+    constexpr int TW = (NN + 3) / 4;           // tiles per wavefront
+    constexpr int B3S = 4 * KC * NN + 16;
     typedef typename MM::V V;
-    __shared__ T ech[N3];
-    __shared__ T b3s[N2];
+    __shared__ T ech[CL::SIZE];
+    __shared__ T b3s[B3S];
     const int g = blockIdx.x;
     const int bp = g % NN, ap = (g / NN) % NN, cp = g / N2;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: scalar branches
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
     const T *slab = E4 + (i64)(g / NN) * N3 * N3 + (i64)bp * N2 * N2;    // E4[cp,ap,0,bp,...]
     const i64 obase = ap + (i64)NN * bp + (i64)N2 * cp;
+    for (int e = threadIdx.x; e < CL::SIZE; e += 256) ech[e] = (T)0;
+    for (int e = threadIdx.x; e < B3S; e += 256) b3s[e] = (T)0;
     V acc[TW];
 #pragma unroll
     for (int u = 0; u < TW; u++) acc[u] = vzero<V>();
@@ -262,19 +340,21 @@ __global__ __launch_bounds__(256) void k_e4_pass_b(const T *__restrict__ E4, con
         const int e = threadIdx.x + 256 * u;
         pf[u] = e < N3 ? slab[e] : (T)0;
     }
+    T pb3 = threadIdx.x < N2 ? (T)B3[threadIdx.x] : (T)0;
     for (int x3 = 0; x3 < NN; x3++)
         for (int x5 = 0; x5 < NN; x5++) {
             __syncthreads();
 #pragma unroll
             for (int u = 0; u < PF; u++) {
                 const int e = threadIdx.x + 256 * u;
-                if (e < N3) ech[e] = pf[u];
+                if (e < N3) ech[CL::at(e / N2, (e / NN) % NN, e % NN)] = pf[u];
             }
-            for (int e = threadIdx.x; e < N2; e += 256) b3s[e] = (T)B3[((i64)x3 * N2 + x5 * NN) * NN + e];
+            if (threadIdx.x < N2) b3s[threadIdx.x] = pb3;
             __syncthreads();
             {
                 const int nx5 = x5 + 1 < NN ? x5 + 1 : 0, nx3 = x5 + 1 < NN ? x3 : x3 + 1;
                 if (nx3 < NN) {
+                    if (threadIdx.x < N2) pb3 = (T)B3[((i64)nx3 * N2 + nx5 * NN) * NN + threadIdx.x];
                     const T *nxt = slab + (i64)nx3 * N2 * N3 + (i64)nx5 * N3;
 #pragma unroll
                     for (int u = 0; u < PF; u++) {
@@ -283,29 +363,25 @@ __global__ __launch_bounds__(256) void k_e4_pass_b(const T *__restrict__ E4, con
                     }
                 }
             }
-#pragma unroll
-            for (int ks = 0; ks < KC; ks++) {
-                const int x6 = 4 * ks + lk;
-                const T b = x6 < NN && li < NN ? b3s[x6 * NN + li] : (T)0;
-#pragma unroll
-                for (int u = 0; u < TW; u++) {
-                    const int r = 16 * (wave + 4 * u) + li;
-                    const T a = x6 < NN && r < N2 ? ech[x6 * N2 + r] : (T)0;
-                    acc[u] = MM::mma(a, b, acc[u]);
-                }
+            switch (wave) {
+                case 0: kloop_c3<T, NN, MM, 0, TW>(ech, b3s, li, lk, acc); break;
+                case 1: kloop_c3<T, NN, MM, 1, TW>(ech, b3s, li, lk, acc); break;
+                case 2: kloop_c3<T, NN, MM, 2, TW>(ech, b3s, li, lk, acc); break;
+                default: kloop_c3<T, NN, MM, 3, TW>(ech, b3s, li, lk, acc); break;
             }
         }
     const int j = li;
 #pragma unroll
-    for (int u = 0; u < TW; u++)
+    for (int u = 0; u < TW; u++) {
+        const int a = wave + 4 * u;
+        if (a >= NN) continue;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int rr = 16 * (wave + 4 * u) + MM::row(lane, r);
-            if (rr < N2 && j < NN) {
-                const int a = rr / NN, c = rr % NN;
+            const int c = MM::row(lane, r);
+            if (c < NN && j < NN)
                 C4[s + (i64)M * (obase + (i64)N3 * a + (i64)N2 * N2 * j + (i64)N2 * N3 * c)] = (double)acc[u][r];
-            }
         }
+    }
 }
 
 ContractPlan *contract_plan_create(int n, int M) {

@@ -82,3 +82,19 @@ def test_bamat_bf16x3(gpu, case):
         print(f"  {nm}: max|bf16x3 - fp64| = {e16:.2e}, max|fp32 - fp64| = {e32:.2e} (scale {scale:.2e})")
         _close(got, r, rel=2e-5, name=nm + " bf16x3")
         assert e16 <= 4.0 * e32 + 1e-12 * scale
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [5, 7, 8, 9])
+def test_bamat_sizes(gpu, n):
+    """The 4-RDM contraction passes at sizes whose paddings differ: n^2 not a multiple of 4 (odd n) pads
+    the K range of C1 with zero rows, n not a multiple of 4 pads the x6 blocks of C3/C4, and n = 8 pads
+    neither."""
+    print(f"[stage1 bamat] nact={n} ncore=1 nvirt=2 nele={n - n % 2}", flush=True)
+    P = synth.Problem(n, 1, 2, seed=3, nele=n - n % 2)
+    ref = S.koopmans_all(P)
+    for prec, rel in ((0, 1e-11), (1, 2e-5), (2, 2e-5)):
+        am, at = koopro.bamat(ref["atwo"], P.ro4, P.daaa, P.taa, P.dal, P.nact, P.ncore, ref["aj"], P.norb,
+                              P.metat, prec=prec)
+        _close(am, ref["amatIP"], rel=rel, name=f"amatIP prec {prec}")
+        _close(at, ref["amatEA"], rel=rel, name=f"amatEA prec {prec}")
