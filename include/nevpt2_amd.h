@@ -206,6 +206,17 @@ int nv2_ci_from_file04(int64_t norb, int64_t ncore, int64_t nact, int64_t nocc, 
  * linearly dependent (a defective matrix).  metat <= 16. */
 int nv2_qd_heff(int64_t metat, const double *heff2, const double *e0, double *heff, double *energies);
 
+/* Eigen-decomposition of nprob real symmetric matrices of mixed orders
+ * (1 <= N[q] <= 1024) in one batched solve: A[q] (order N[q], column-major,
+ * upper triangle read) is overwritten by its orthonormal eigenvectors and w[q]
+ * receives the eigenvalues in ascending order, as dsyev('V','U') returns them
+ * (up to the sign of each eigenvector and the basis inside a degenerate
+ * eigenspace).  Replaces the per-class metric and Koopmans-matrix dsyev calls of
+ * the PC classes (qdnevpt.F:1995-2037 for (+1), the same pair in every class
+ * driver, thresholding at :4744-4769), which stage 2 now issues as one batch.
+ * Host arrays. */
+int nv2_syev_batched(int64_t nprob, const int64_t *N, double *const *A, double *const *w);
+
 /* Request loop (no counterpart in the reference, whose drivers evaluate one
  * problem per process run: koopro4QD_main.F90:71, qdnevpt_main.F90:75).  A
  * server runs `slots` evaluations concurrently on one GPU; requests are admitted

@@ -73,6 +73,7 @@ def _declare(L):
     L.nv2_server_destroy.argtypes = [C.c_void_p]
     L.nv2_server_destroy.restype = None
     L.nv2_ctx_bytes.restype = i64
+    L.nv2_syev_batched.argtypes = [i64, C.POINTER(C.c_int64), C.POINTER(dp), C.POINTER(dp)]
     L.nv2_ctx_bytes.argtypes = [i64] * 5
 
 
@@ -96,6 +97,19 @@ def ptr(a: np.ndarray):
 def fin(a) -> np.ndarray:
     """Input array as Fortran-contiguous float64 (the reference layout)."""
     return np.asfortranarray(a, dtype=np.float64)
+
+
+def syev_batched(mats):
+    """Eigen-decomposition of every symmetric matrix of ``mats`` (upper triangles read) in one
+    batched GPU solve; returns [(w ascending, V columns)], as numpy.linalg.eigh does."""
+    n = len(mats)
+    As = [np.array(m, dtype=np.float64, order="F", copy=True) for m in mats]
+    ws = [np.zeros(a.shape[0]) for a in As]
+    N = (C.c_int64 * n)(*[a.shape[0] for a in As])
+    Ap = (dp * n)(*[ptr(a) for a in As])
+    wp = (dp * n)(*[ptr(w) for w in ws])
+    check(lib().nv2_syev_batched(n, N, Ap, wp), "nv2_syev_batched")
+    return list(zip(ws, As))
 
 
 def zeros(*shape) -> np.ndarray:

@@ -11,6 +11,7 @@
 #include "nv2_common.h"
 #include "nv2_stage1.h"
 #include "nv2_stage2.h"
+#include "nv2_syevb.h"
 #include "../../include/nevpt2_amd.h"
 
 namespace nv2 {
@@ -224,6 +225,30 @@ int nv2_qdnevpt_state(int64_t ncore, int64_t nact, int64_t nvirt, int64_t metat,
     stage2_destroy(st2);
     rocblas_destroy_handle(h);
     return rc;
+}
+
+int nv2_syev_batched(int64_t nprob, const int64_t *N, double *const *A, double *const *w) {
+    if (nprob < 0 || (nprob > 0 && (!N || !A || !w))) { set_error("nv2_syev_batched: bad arguments"); return -2; }
+    for (i64 q = 0; q < nprob; q++)
+        if (N[q] < 1 || N[q] > SYEVB_NMAX) { set_error("nv2_syev_batched: order %lld outside 1..%d", (long long)N[q], SYEVB_NMAX); return -2; }
+    std::vector<DBuf> dA(nprob), dw(nprob), dwk(nprob);
+    std::vector<SyevbProb> P(nprob);
+    for (i64 q = 0; q < nprob; q++) {
+        const i64 n = N[q];
+        TRY(dA[q].up(A[q], sizeof(double) * n * n));
+        TRY(dw[q].alloc(sizeof(double) * n));
+        TRY(dwk[q].alloc(sizeof(double) * syevb_workspace((int)n)));
+        P[q] = SyevbProb{dA[q].d(), dw[q].d(), dwk[q].d(), (int)n};
+    }
+    rocblas_handle h;
+    if (rocblas_create_handle(&h) != rocblas_status_success) { set_error("rocblas_create_handle failed"); return -1; }
+    int rc = rocblas_set_stream(h, 0) == rocblas_status_success ? 0 : -1;
+    if (!rc) rc = syevb(h, 0, (int)nprob, P.data());
+    if (!rc) { hipError_t e = hipDeviceSynchronize(); if (e != hipSuccess) { set_error("syevb: %s", hipGetErrorString(e)); rc = -1; } }
+    rocblas_destroy_handle(h);
+    if (rc) return rc;
+    for (i64 q = 0; q < nprob; q++) { TRY(dA[q].down(A[q])); TRY(dw[q].down(w[q])); }
+    return 0;
 }
 
 }  // extern "C"

@@ -30,6 +30,7 @@
 #include "nv2_band.h"
 #include "nv2_chol.h"
 #include "nv2_trd.h"
+#include "nv2_syevb.h"
 #include "../../include/nevpt2_amd.h"
 
 namespace nv2 {
@@ -1781,6 +1782,7 @@ struct PcJob {
     double *trdm = nullptr, *trde = nullptr;   // eig_trd: reduction scratch, off-diagonal copy for dstedc
     double *trdq = nullptr;                    // scratch of applying Q
     double *cholw = nullptr;                   // Cholesky scratch (trd jobs)
+    double *sbw = nullptr;                     // batched-eigensolver scratch, nb problems (other jobs)
     double *S = nullptr, *K = nullptr;         // nb consecutive N x N matrices; S is overwritten
     double *w1 = nullptr, *hw1 = nullptr, *coef = nullptr, *w2 = nullptr;
     double *caux = nullptr, *Ks = nullptr, *T = nullptr, *Kp = nullptr;   // nb x N x N scratch each
@@ -1800,13 +1802,20 @@ static int pc_job_buffers(Stage2 *p, PcJob &j, size_t hoff) {
     GET(Ks, (j.tag + "_Ks").c_str(), NN); GET(Kp, (j.tag + "_Kp").c_str(), NN);
     GET(Tc, (j.tag + "_Tcaux").c_str(), 2 * NN);   // T then caux
     j.w1 = w1; j.w2 = w2; j.coef = coef; j.caux = Tc + NN; j.Ks = Ks; j.T = Tc; j.Kp = Kp;
-    if (j.trd) {
+    if (j.N >= 3) {                  // the factored metric (every job on the Cholesky path)
         GET(cw, (j.tag + "_cholw").c_str(), chol_workspace(j.N, 2 * j.nb));
         j.cholw = cw;
+    }
+    if (j.trd) {
         GET(bw, (j.tag + "_bandw").c_str(), band_workspace(j.N, j.nb));
         j.bandw = bw;
     }
     j.band = false;
+    j.sbw = nullptr;
+    if (!j.trd && j.N >= 1 && j.N <= SYEVB_NMAX) {
+        GET(sb, (j.tag + "_sbw").c_str(), syevb_workspace(j.N) * j.nb);
+        j.sbw = sb;
+    }
     j.hw1 = p->hpin + hoff;
     if (j.N >= 3) {                  // scratch of eig_trd (every job) and of the K' reduction (trd jobs)
         GET(tm, (j.tag + "_trdm").c_str(), trd_workspace(j.N, j.nb));
@@ -1981,8 +1990,10 @@ __global__ __launch_bounds__(256) void k_trace_test(const double *part, int np, 
     if (threadIdx.x == 0) info[b] = (red[0] + red[1] + red[2] + red[3] < lim) ? 0 : 1;
 }
 #define TRACE_PARTS 512
-static int pc_metric_launch(Stage2 *p, PcJob &j, double thr) {
-    if (j.trd && j.N >= 3 && chol_metric()) {     // info words: 4*slot + b for the eigenvalue test, + nb + b for S
+// The factored metric: S_b = U_b^T U_b, U_b^-1 into T and the trace test of every eigenvalue
+// above thr.  Info words: 4*slot + b for the eigenvalue test, + nb + b for the factorisation of S.
+static int pc_metric_chol_launch(Stage2 *p, PcJob &j, double thr) {
+    {
         j.chol = true;
         hipStream_t st = p->ws[j.w];
         rocblas_handle h = p->wh[j.w];
@@ -2006,11 +2017,15 @@ static int pc_metric_launch(Stage2 *p, PcJob &j, double thr) {
                                st));
         return 0;
     }
+}
+static int pc_metric_launch(Stage2 *p, PcJob &j, double thr) {
+    if (j.trd && j.N >= 3 && chol_metric()) return pc_metric_chol_launch(p, j, thr);
     if (j.N >= 3 && eigen_algo() == 0) TRYS(eig_trd(p, j, j.N, j.S, j.NN(), j.w1, j.N, 4 * j.slot));
     else TRYS(syevd_async(p, j.w, 4 * j.slot, true, j.N, j.S, j.NN(), j.w1, j.N, j.nb));
     if (j.N) NV2_HIP(hipMemcpyAsync(j.hw1, j.w1, sizeof(double) * j.N * j.nb, hipMemcpyDeviceToHost, p->ws[j.w]));
     return 0;
 }
+static int pc_kprime_form(Stage2 *p, PcJob &j, double thr);
 // after the worker stream has finished: threshold the metric spectra, launch K' and its solves
 static int pc_kprime_launch(Stage2 *p, PcJob &j, double thr) {
     const int N = j.N;
@@ -2095,6 +2110,25 @@ static int pc_kprime_launch(Stage2 *p, PcJob &j, double thr) {
         NV2_HIP(hipMemcpyAsync(j.hw1, j.w1, sizeof(double) * N * j.nb, hipMemcpyDeviceToHost, st));
         NV2_HIP(hipStreamSynchronize(st));
     }
+    TRYS(pc_kprime_form(p, j, thr));
+    if (j.nmax == 0) return 0;
+    if (j.trd) return pc_trd_launch(p, j);
+    const int nmax = j.nmax;
+    hipStream_t st = p->ws[j.w];
+    rocblas_handle h = p->wh[j.w];
+    const i64 NN = j.NN();
+    const double one = 1.0, zero = 0.0;
+    if (nmax >= 3 && eigen_algo() == 0) TRYS(eig_trd(p, j, nmax, j.Kp, NN, j.w2, N, 4 * j.slot + 2));
+    else TRYS(syevd_async(p, j.w, 4 * j.slot + 2, true, nmax, j.Kp, NN, j.w2, N, j.nb));
+    RB(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, N, nmax, nmax, &one, j.caux, N,
+                                     NN, j.Kp, nmax, NN, &zero, j.coef, N, NN, j.nb));
+    return 0;
+}
+// The thresholding of the metric spectra (host copies in hw1, info words in hinfo), caux =
+// U_kept w^-1/2 and K' = caux^T K caux of a job on the eigen path, padded to a common order when
+// the two problems keep different counts; nmax = 0 when nothing is kept.
+static int pc_kprime_form(Stage2 *p, PcJob &j, double thr) {
+    const int N = j.N;
     j.nmax = 0;
     for (int b = 0; b < j.nb; b++) {
         const int inf = p->hinfo[4 * j.slot + b];
@@ -2129,17 +2163,134 @@ static int pc_kprime_launch(Stage2 *p, PcJob &j, double thr) {
         hipLaunchKernelGGL(k_pad_gersh, dim3(1), dim3(256), 0, st, j.Kp, nmax, NN, j.nb, j.nnew[0], j.nnew[1], nmax);
         NV2_LAUNCH_CHECK();
     }
-    if (j.trd) return pc_trd_launch(p, j);
-    if (nmax >= 3 && eigen_algo() == 0) TRYS(eig_trd(p, j, nmax, j.Kp, NN, j.w2, N, 4 * j.slot + 2));
-    else TRYS(syevd_async(p, j.w, 4 * j.slot + 2, true, nmax, j.Kp, NN, j.w2, N, j.nb));
-    RB(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, N, nmax, nmax, &one, j.caux, N,
-                                     NN, j.Kp, nmax, NN, &zero, j.coef, N, NN, j.nb));
     return 0;
 }
 // one host thread per worker stream: the solver library blocks its calling thread at times,
 // so separate threads are what lets the chains of different problems overlap on the device
+// NV2_SYEVB=0: the worker-0 classes take the per-matrix path (trd_reduce + dstedc) instead of
+// the batched eigensolver
+static bool syevb_on() {
+    const char *e = getenv("NV2_SYEVB");
+    return !(e && atoi(e) == 0);
+}
+// The jobs of worker w (the classes other than (+-1)') as two batched eigen-solves: every metric
+// of every job in one syevb call, then every K' (after the host thresholding) in another, then
+// coef = caux V per job.
+static int pc_worker_batched(Stage2 *p, PcJob *J, int njob, int w, double thr, double *ms_done,
+                             std::chrono::steady_clock::time_point t0) {
+    hipStream_t st = p->ws[w];
+    rocblas_handle h = p->wh[w];
+    // metrics of order >= 3 factored first (S = U^T U, U^-1, trace test of every eigenvalue above thr)
+    for (int k = 0; k < njob; k++)
+        if (J[k].w == w && J[k].N >= 3 && chol_metric()) TRYS(pc_metric_chol_launch(p, J[k], thr));
+    NV2_HIP(hipStreamSynchronize(st));
+    // a job whose metric passes keeps caux = U^-1 (every eigenvalue kept); the others, and any metric
+    // the trace test leaves open, take the eigen path: one batched solve of their metrics
+    // the trace test is sufficient, not necessary: where it leaves a metric open that factors, the
+    // exact test (S - thr*I positive definite) decides, as on the (+-1)' chain
+    bool open = false;
+    std::vector<char> pass(njob, 0);
+    for (int k = 0; k < njob; k++) {
+        PcJob &j = J[k];
+        if (j.w != w || !j.chol) continue;
+        bool ok = true, exact = false;
+        for (int b = 0; b < j.nb; b++) {
+            if (p->hinfo[4 * j.slot + j.nb + b] != 0) ok = false;
+            if (p->hinfo[4 * j.slot + b] != 0) exact = true;
+        }
+        for (int q = 0; q < 4; q++) p->hinfo[4 * j.slot + q] = 0;   // all four words were copied
+        pass[k] = ok && !exact;
+        if (ok && exact) {
+            const i64 NN = j.NN();
+            NV2_HIP(hipMemcpyAsync(j.Kp, j.S, sizeof(double) * NN * j.nb, hipMemcpyDeviceToDevice, st));
+            hipLaunchKernelGGL(k_shift_diag, dim3(nb((i64)j.N * j.nb, 256)), dim3(256), 0, st, j.Kp, j.N, NN, j.nb, -thr);
+            NV2_LAUNCH_CHECK();
+            TRYS(chol_upper(h, st, j.N, j.Kp, NN, j.nb, p->dinfo_w + 4 * j.slot, j.cholw));
+            NV2_HIP(hipMemcpyAsync(p->hinfo + 4 * j.slot, p->dinfo_w + 4 * j.slot, sizeof(int) * j.nb,
+                                   hipMemcpyDeviceToHost, st));
+            pass[k] = 2;
+            open = true;
+        }
+    }
+    if (open) NV2_HIP(hipStreamSynchronize(st));
+    std::vector<SyevbProb> pr;
+    for (int k = 0; k < njob; k++) {
+        PcJob &j = J[k];
+        if (j.w != w) continue;
+        if (j.chol) {
+            bool ok = pass[k] == 1;
+            if (pass[k] == 2) {
+                ok = true;
+                for (int b = 0; b < j.nb; b++) {
+                    if (p->hinfo[4 * j.slot + b] != 0) ok = false;
+                    p->hinfo[4 * j.slot + b] = 0;
+                }
+            }
+            if (ok) {
+                j.nmax = j.N;
+                for (int b = 0; b < j.nb; b++) { j.nnew[b] = j.N; j.minw[b] = thr; }   // thr: a lower bound
+                std::swap(j.caux, j.T);
+                continue;
+            }
+            j.chol = false;
+        }
+        for (int b = 0; b < j.nb && j.N; b++)
+            pr.push_back(SyevbProb{j.Sb(b), j.w1 + (i64)b * j.N, j.sbw + b * syevb_workspace(j.N), j.N});
+    }
+    TRYS(syevb(h, st, (int)pr.size(), pr.data()));
+    for (int k = 0; k < njob; k++)
+        if (J[k].w == w && J[k].N && !J[k].chol)
+            NV2_HIP(hipMemcpyAsync(J[k].hw1, J[k].w1, sizeof(double) * J[k].N * J[k].nb, hipMemcpyDeviceToHost, st));
+    NV2_HIP(hipStreamSynchronize(st));
+    const double ms_metric = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    // K' of every job, then one batched solve of all of them
+    pr.clear();
+    const double one = 1.0, zero = 0.0;
+    for (int k = 0; k < njob; k++) {
+        PcJob &j = J[k];
+        if (j.w != w) continue;
+        if (j.chol) {                    // K' = U^-T K U^-1
+            const int N = j.N;
+            const i64 NN = j.NN();
+            for (int b = 0; b < j.nb; b++)
+                hipLaunchKernelGGL(k_sym_upper, dim3(nb(NN, 256)), dim3(256), 0, st, j.Kb(b), N, j.Ks + b * NN);
+            NV2_LAUNCH_CHECK();
+            RB(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, N, N, N, &one, j.Ks, N,
+                                             NN, j.caux, N, NN, &zero, j.T, N, NN, j.nb));
+            RB(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, N, N, N, &one,
+                                             j.caux, N, NN, j.T, N, NN, &zero, j.Kp, N, NN, j.nb));
+        } else {
+            TRYS(pc_kprime_form(p, j, thr));
+        }
+        for (int b = 0; b < j.nb && j.nmax; b++)
+            pr.push_back(SyevbProb{j.Kp + b * j.NN(), j.w2 + (i64)b * j.N, j.sbw + b * syevb_workspace(j.N), j.nmax});
+    }
+    TRYS(syevb(h, st, (int)pr.size(), pr.data()));
+    for (int k = 0; k < njob; k++) {
+        const PcJob &j = J[k];
+        if (j.w != w || !j.nmax) continue;
+        const i64 NN = j.NN();
+        RB(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, j.N, j.nmax, j.nmax, &one,
+                                         j.caux, j.N, NN, j.Kp, j.nmax, NN, &zero, j.coef, j.N, NN, j.nb));
+    }
+    NV2_HIP(hipStreamSynchronize(st));
+    *ms_done = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stage2_timing()) {
+        fprintf(stderr, "[worker %d batched] metrics at %.2f ms, K' spectra and coef at %.2f ms\n", w, ms_metric,
+                *ms_done);
+        for (int k = 0; k < njob; k++)
+            if (J[k].w == w)
+                fprintf(stderr, "[metric %s N=%d] %s\n", J[k].tag.c_str(), J[k].N,
+                        J[k].chol ? "Cholesky path: every eigenvalue above thr" : "eigen path");
+    }
+    return 0;
+}
 static int pc_worker(Stage2 *p, PcJob *J, int njob, int w, double thr, double *ms_done,
                      std::chrono::steady_clock::time_point t0) {
+    bool batched = syevb_on();
+    for (int k = 0; k < njob; k++)
+        if (J[k].w == w && (J[k].trd || !J[k].sbw)) batched = false;
+    if (batched) return pc_worker_batched(p, J, njob, w, thr, ms_done, t0);
     for (int k = 0; k < njob; k++)
         if (J[k].w == w) TRYS(pc_metric_launch(p, J[k], thr));
     NV2_HIP(hipStreamSynchronize(p->ws[w]));

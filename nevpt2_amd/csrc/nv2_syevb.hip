@@ -1,0 +1,708 @@
+// Batched symmetric eigensolver for the small PC-class problems of stage 2.
+//
+// Every PC class diagonalises its metric S and then its Koopmans matrix K' (dsyev('V','U'),
+// e.g. qdnevpt.F:1995-2037 for class (+1), 4744-4769 for the thresholding).  Apart from the
+// n^3 problems of the (+-1)' classes these are small (orders n, n(n+1)/2, n^2, 2n^2: 14..392 at
+// n = 14) and many (two per class and state).  Solved one by one through a LAPACK-style library
+// path each costs hundreds of dependent launches of small kernels; here every problem of a phase
+// (any mix of orders up to 1024) goes through five launches in all:
+//
+//   k_syevb_trd    Householder tridiagonalisation A = Q T Q^T, one workgroup (16 wavefronts)
+//                  per matrix.  While the trailing block is larger than SB_MLDS it stays in
+//                  global memory (L2-resident at these orders); every column step is one pass
+//                  over it that applies the previous step's rank-2 update and forms the next
+//                  matrix-vector product together, thread (row, share of columns) walking a row
+//                  with SB_UNR loads in flight.  The last SB_MLDS columns run on a copy in LDS.
+//                  Reflectors are left below the subdiagonal as dsytrd('L') does.
+//   k_syevb_vec    eigenvalues of T by bisection on Sturm counts and eigenvectors by inverse
+//                  iteration with a pivoted LU of T - lambda I, one thread per eigenpair,
+//                  SB_VT eigenpairs per workgroup (the approach of LAPACK dstebz + dstein).
+//   k_syevb_clus   Gram-Schmidt inside every cluster of eigenvalues closer than 1e-4 ||T||,
+//                  where independent inverse iteration does not give orthogonal vectors.
+//   k_syevb_back   U = Q Z, one workgroup per column panel of Z (held in LDS), the reflectors
+//                  applied from the last to the first, the next one read while the current one
+//                  is applied.
+//   k_syevb_copy   U into A.
+// Workgroups stay below a whole CU (64 VGPRs at 16 wavefronts, 64 KB LDS for the reduction;
+// 256-thread workgroups elsewhere) so that they are placed beside the other streams' kernels:
+// a first form with 1024-thread, 128-VGPR workgroups waited tens of milliseconds for idle CUs
+// while the rest of stage 2 ran.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdlib>
+
+#include "nv2_common.h"
+#include "nv2_syevb.h"
+
+namespace nv2 {
+
+#define SB_TB 1024               // threads per workgroup of the tridiagonalisation
+#define SB_WAVES (SB_TB / 64)
+#define SB_VT 256                // threads (eigenpairs) per workgroup of k_syevb_vec
+#define SB_SMALLC 32             // clusters up to this size: one wavefront each
+#define SB_BT 256                // threads per workgroup of the back-transformation
+#define SB_META 8                // per-problem scalars between tau and X
+#define SB_MLDS 64               // trailing blocks of at most this order are reduced in LDS
+#define SB_LAUNCH 16             // problems per launch (kernel-argument arrays)
+#define SB_ITERS 2               // inverse-iteration solves per eigenvector
+#define SB_UNR 8                 // loads in flight per thread in the streaming loops
+#define SB_HMAX 8                // largest split of a row walk of the global phase
+#define SB_IU 4                  // steps of loads in flight in the inverse-iteration solves:
+#define SB_IB 4                  // SB_IU forward, SB_IB in the back substitution (four loads a step)
+
+struct SyevbArgs {
+    int n;
+    int N[SB_LAUNCH];
+    int lower[SB_LAUNCH];
+    double *A[SB_LAUNCH];
+    double *w[SB_LAUNCH];
+    double *work[SB_LAUNCH];
+};
+
+// scratch of one problem: d, e, tau (3N), SB_META doubles (meta[0]: the Gershgorin bound of
+// ||T||), X (N^2, eigenvectors of T by rows: X[i*N + j]),
+// F (5N^2: the pivoted LU of T - w_j I of every j, F[(5i + c)*N + j]), Z (N^2, the eigenvectors
+// of T column-major, written while other threads still read F); the compact-WY scratch of the
+// back-transformation (3N^2) reuses F
+i64 syevb_workspace(int N) { return (i64)3 * N + SB_META + (i64)7 * N * N; }
+
+__device__ inline double sb_wave_sum(double x) {
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+__device__ inline double sb_wave_max(double x) {
+    for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+    return x;
+}
+__device__ inline double sb_wave_min(double x) {
+    for (int o = 32; o > 0; o >>= 1) x = fmin(x, __shfl_xor(x, o, 64));
+    return x;
+}
+// sum over the workgroup, the same value in every thread (fixed order: deterministic)
+__device__ inline double sb_block_sum(double x, double *red) {
+    x = sb_wave_sum(x);
+    __syncthreads();                       // red[] may still be read by the previous call
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+    __syncthreads();
+    double s = 0.0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += red[i];
+    return s;
+}
+__device__ inline double sb_block_max(double x, double *red) {
+    x = sb_wave_max(x);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+    __syncthreads();
+    double s = red[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); i++) s = fmax(s, red[i]);
+    return s;
+}
+__device__ inline double sb_block_min(double x, double *red) {
+    x = sb_wave_min(x);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+    __syncthreads();
+    double s = red[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); i++) s = fmin(s, red[i]);
+    return s;
+}
+
+// Householder vector of x (length m, x[0] = alpha): H = I - t v v^T with v[0] = 1 maps x to
+// (beta, 0, ..); returns t, beta and the factor that turns x[1..] into v[1..] (dlarfg's
+// choice beta = -sign(alpha) ||x||; t = 0 when x[1..] = 0)
+__device__ inline void sb_house(double alpha, double sigma, double &t, double &beta, double &scl) {
+    if (sigma == 0.0) {
+        t = 0.0; beta = alpha; scl = 1.0;
+    } else {
+        beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
+        t = (beta - alpha) / beta;
+        scl = 1.0 / (alpha - beta);
+    }
+}
+
+// ---------------------------------------------------------------- tridiagonalisation
+__global__ __launch_bounds__(SB_TB, 8) void k_syevb_trd(SyevbArgs a) {
+    extern __shared__ double sh[];         // L: SB_MLDS^2, then v/w of two steps (SYEVB_NMAX each)
+    __shared__ double red[SB_WAVES];
+    const int N = a.N[blockIdx.x];
+    double *A = a.A[blockIdx.x];
+    double *d = a.work[blockIdx.x], *e = d + N, *tau = e + N;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    double *L = sh;
+    double *vb0 = sh + SB_MLDS * SB_MLDS, *vb1 = vb0 + SYEVB_NMAX;
+    double *wb0 = vb1 + SYEVB_NMAX, *wb1 = wb0 + SYEVB_NMAX;
+    const i64 ld = N;
+    // the full matrix from the triangle the caller names (dsyev('V','U') reads the upper one),
+    // scaled by a power of two (exact) to a largest entry in [0.5, 1) so that no sum of squares
+    // of the Householder steps under- or overflows; d and e are scaled back at the end (the
+    // reflectors do not depend on the scale)
+    const bool lower = a.lower[blockIdx.x] != 0;
+    double amax = 0.0;
+    for (i64 idx = tid; idx < (i64)N * N; idx += SB_TB) {
+        const int i = (int)(idx % N), j = (int)(idx / N);
+        if (lower ? i >= j : i <= j) amax = fmax(amax, fabs(A[idx]));
+    }
+    amax = sb_block_max(amax, red);
+    int ex = 0;
+    if (amax > 0.0) (void)frexp(amax, &ex);
+    const double sc = ldexp(1.0, -ex);
+    for (i64 idx = tid; idx < (i64)N * N; idx += SB_TB) {
+        const int i = (int)(idx % N), j = (int)(idx / N);
+        if (lower ? i >= j : i <= j) {
+            const double x = A[idx] * sc;
+            A[idx] = x;
+            if (i != j) A[j + ld * i] = x;
+        }
+    }
+    __syncthreads();
+    if (N <= 2) {
+        if (tid == 0) {
+            d[0] = ldexp(A[0], ex);
+            e[0] = N > 1 ? ldexp(A[1], ex) : 0.0;
+            tau[0] = 0.0;
+            if (N > 1) { d[1] = ldexp(A[1 + ld], ex); e[1] = 0.0; tau[1] = 0.0; }
+        }
+        return;
+    }
+    // global phase: step k reads column k (bringing it up to date with the pending update of
+    // step k-1), forms v_k, then one pass over the trailing block applies update k-1 and forms
+    // p_k = t_k A v_k column by column; w_k = p_k - (t_k/2)(p_k.v_k) v_k is the pending update
+    int k = 0;
+    bool pend = false;
+    double *vp = vb1, *wp = wb1, *vc = vb0, *wc = wb0;    // previous step (index 0 = row k), this step (row k+1)
+    for (; k <= N - 3 && N - k > SB_MLDS; k++) {
+        const int m = N - k - 1;
+        double s = 0.0;
+        for (int r = tid; r < m; r += SB_TB) {
+            double x = A[(k + 1 + r) + ld * k];
+            if (pend) x -= vp[r + 1] * wp[0] + wp[r + 1] * vp[0];
+            vc[r] = x;
+            if (r >= 1) s += x * x;
+        }
+        const double sigma = sb_block_sum(s, red);
+        double dk = A[k + ld * k];
+        if (pend) dk -= 2.0 * vp[0] * wp[0];
+        double t, beta, scl;
+        sb_house(vc[0], sigma, t, beta, scl);
+        __syncthreads();                   // vc[0] read everywhere before it becomes 1
+        for (int r = tid; r < m; r += SB_TB) {
+            const double v = r == 0 ? 1.0 : vc[r] * scl;
+            vc[r] = v;
+            if (r >= 1) A[(k + 1 + r) + ld * k] = v;
+        }
+        if (tid == 0) { d[k] = dk; e[k] = beta; tau[k] = t; }
+        __syncthreads();
+        // A is symmetric, so (A v)_c is also row c of A times v: thread (c, h) walks row c over
+        // its share h of the columns, reading column after column of the block contiguously
+        // across the lanes, SB_UNR loads in flight; the H partial sums meet in LDS (L is free)
+        {
+            const int H = m >= SB_TB ? 1 : (SB_TB / m > SB_HMAX ? SB_HMAX : SB_TB / m);
+            for (int slot = tid; slot < H * m; slot += SB_TB) {
+                const int c = slot % m, h = slot / m;
+                const int r0 = (int)((i64)m * h / H), r1 = (int)((i64)m * (h + 1) / H);
+                double *row = A + (k + 1 + c) + ld * (k + 1);
+                const double vpc = pend ? vp[c + 1] : 0.0, wpc = pend ? wp[c + 1] : 0.0;
+                double acc = 0.0;
+                for (int r = r0; r < r1; r += SB_UNR) {
+                    double x[SB_UNR];
+#pragma unroll
+                    for (int u = 0; u < SB_UNR; u++) x[u] = r + u < r1 ? row[ld * (r + u)] : 0.0;
+#pragma unroll
+                    for (int u = 0; u < SB_UNR; u++) {
+                        if (r + u < r1) {
+                            if (pend) {
+                                x[u] -= vpc * wp[r + u + 1] + wpc * vp[r + u + 1];
+                                row[ld * (r + u)] = x[u];
+                            }
+                            acc += x[u] * vc[r + u];
+                        }
+                    }
+                }
+                L[slot] = acc;
+            }
+            __syncthreads();
+            for (int c = tid; c < m; c += SB_TB) {
+                double acc = 0.0;
+                for (int h = 0; h < H; h++) acc += L[h * m + c];
+                wc[c] = t * acc;
+            }
+        }
+        __syncthreads();
+        double pv = 0.0;
+        for (int r = tid; r < m; r += SB_TB) pv += wc[r] * vc[r];
+        pv = sb_block_sum(pv, red);
+        const double K = 0.5 * t * pv;
+        for (int r = tid; r < m; r += SB_TB) wc[r] -= K * vc[r];
+        __syncthreads();
+        pend = true;
+        double *tv = vp, *tw = wp;
+        vp = vc; wp = wc; vc = tv; wc = tw;
+    }
+    // LDS phase: rows/columns k0..N-1 (order s <= SB_MLDS) copied with the pending update applied
+    const int k0 = k, s = N - k0;
+    for (int idx = tid; idx < s * s; idx += SB_TB) {
+        const int r = idx % s, c = idx / s;
+        double x = A[(k0 + r) + ld * (k0 + c)];
+        if (pend) x -= vp[r] * wp[c] + wp[r] * vp[c];
+        L[idx] = x;
+    }
+    __syncthreads();
+    double *v = vb0, *w = wb0;
+    for (; k <= N - 3; k++) {
+        const int m = N - k - 1, o = k - k0;
+        double sg = 0.0;
+        for (int r = tid; r < m; r += SB_TB) {
+            const double x = L[(o + 1 + r) + s * o];
+            v[r] = x;
+            if (r >= 1) sg += x * x;
+        }
+        const double sigma = sb_block_sum(sg, red);
+        const double dk = L[o + s * o];
+        double t, beta, scl;
+        sb_house(v[0], sigma, t, beta, scl);
+        __syncthreads();
+        for (int r = tid; r < m; r += SB_TB) {
+            const double vv = r == 0 ? 1.0 : v[r] * scl;
+            v[r] = vv;
+            if (r >= 1) A[(k + 1 + r) + ld * k] = vv;
+        }
+        if (tid == 0) { d[k] = dk; e[k] = beta; tau[k] = t; }
+        __syncthreads();
+        for (int c = wv; c < m; c += SB_WAVES) {
+            const double *col = L + (o + 1) + s * (o + 1 + c);
+            double acc = 0.0;
+            for (int r = lane; r < m; r += 64) acc += col[r] * v[r];
+            acc = sb_wave_sum(acc);
+            if (lane == 0) w[c] = t * acc;
+        }
+        __syncthreads();
+        double pv = 0.0;
+        for (int r = tid; r < m; r += SB_TB) pv += w[r] * v[r];
+        pv = sb_block_sum(pv, red);
+        const double K = 0.5 * t * pv;
+        for (int r = tid; r < m; r += SB_TB) w[r] -= K * v[r];
+        __syncthreads();
+        for (int c = wv; c < m; c += SB_WAVES) {
+            double *col = L + (o + 1) + s * (o + 1 + c);
+            const double vcc = v[c], wcc = w[c];
+            for (int r = lane; r < m; r += 64) col[r] -= v[r] * wcc + w[r] * vcc;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {                        // the last 2 x 2 block
+        const int o = N - 2 - k0;
+        d[N - 2] = L[o + s * o];
+        d[N - 1] = L[(o + 1) + s * (o + 1)];
+        e[N - 2] = L[(o + 1) + s * o];
+        e[N - 1] = 0.0;
+        tau[N - 2] = 0.0;
+        tau[N - 1] = 0.0;
+    }
+    __syncthreads();
+    for (int i = tid; i < N; i += SB_TB) { d[i] = ldexp(d[i], ex); e[i] = ldexp(e[i], ex); }
+}
+
+// ---------------------------------------------------------------- eigenpairs of T
+// pseudo-random start vector entry in [-1, 1) (the splitmix64 finaliser of (i, j))
+__device__ inline double sb_rand(unsigned i, unsigned j) {
+    unsigned long long z = ((unsigned long long)i << 32) ^ (unsigned long long)j ^ 0x9E3779B97F4A7C15ull;
+    z ^= z >> 30;
+    z *= 0xBF58476D1CE4E5B9ull;
+    z ^= z >> 27;
+    z *= 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+// Eigenpairs of T, SB_VT eigenvalues per workgroup (grid.x) of problem grid.y: w_j by bisection on
+// Sturm counts, then the eigenvector by inverse iteration with a pivoted LU of T - w_j I, one
+// thread per eigenpair, written to column j of Z with unit norm.  T is scaled by a power of two
+// (exact) to a norm in [0.5, 1) first, so the inverse iteration's growth stays far from overflow.
+__global__ __launch_bounds__(SB_VT) void k_syevb_vec(SyevbArgs a) {
+    __shared__ double sd[SYEVB_NMAX], se[SYEVB_NMAX], se2[SYEVB_NMAX];
+    __shared__ double red[SB_VT / 64];
+    const int N = a.N[blockIdx.y];
+    if ((int)blockIdx.x * SB_VT >= N) return;      // uniform: no eigenvalue of this problem here
+    double *work = a.work[blockIdx.y];
+    const double *d = work, *e = work + N;
+    double *meta = work + (i64)3 * N;
+    double *X = meta + SB_META, *F = X + (i64)N * N, *Z = F + (i64)5 * N * N;
+    double *wout = a.w[blockIdx.y];
+    const int tid = threadIdx.x;
+    double glo = DBL_MAX, ghi = -DBL_MAX, em2 = 0.0;
+    for (int i = tid; i < N; i += SB_VT) {
+        const double ei = i < N - 1 ? e[i] : 0.0, eim = i > 0 ? e[i - 1] : 0.0;
+        sd[i] = d[i];
+        se[i] = ei;
+        const double r = fabs(ei) + fabs(eim);
+        glo = fmin(glo, d[i] - r);
+        ghi = fmax(ghi, d[i] + r);
+    }
+    glo = sb_block_min(glo, red);
+    ghi = sb_block_max(ghi, red);
+    int ex = 0;
+    const double tn0 = fmax(fabs(glo), fabs(ghi));
+    if (tn0 > 0.0) (void)frexp(tn0, &ex);
+    const double sc = ldexp(1.0, -ex);
+    for (int i = tid; i < N; i += SB_VT) {
+        sd[i] *= sc;
+        se[i] *= sc;
+        se2[i] = se[i] * se[i];
+        em2 = fmax(em2, se2[i]);
+    }
+    em2 = sb_block_max(em2, red);                  // its barriers publish sd/se/se2
+    if (blockIdx.x == 0 && tid == 0) meta[0] = tn0;
+    glo *= sc;
+    ghi *= sc;
+    const double eps = DBL_EPSILON;
+    const double tn = fmax(fabs(glo), fabs(ghi));
+    const double pivmin = DBL_MIN * fmax(1.0, em2);
+    glo -= 2.0 * eps * tn * N + pivmin;
+    ghi += 2.0 * eps * tn * N + pivmin;
+    const int j = blockIdx.x * SB_VT + tid;
+    if (j >= N) return;                            // no barrier below
+    // bisection: w_j is the (j+1)-th smallest eigenvalue (Sturm count of T - x I)
+    double lo = glo, hi = ghi;
+    for (int it = 0; it < 256; it++) {
+        const double mid = 0.5 * (lo + hi);
+        if (hi - lo <= fmax(eps * tn, 2.0 * eps * fmax(fabs(lo), fabs(hi))) || mid <= lo || mid >= hi) break;
+        double q = sd[0] - mid;
+        if (fabs(q) < pivmin) q = -pivmin;
+        int cnt = q < 0.0;
+        for (int i = 1; i < N; i++) {
+            q = (sd[i] - mid) - se2[i - 1] / q;
+            if (fabs(q) < pivmin) q = -pivmin;
+            cnt += q < 0.0;
+        }
+        if (cnt > j) hi = mid;
+        else lo = mid;
+    }
+    const double lam = 0.5 * (lo + hi);
+    wout[j] = ldexp(lam, ex);
+    // inverse iteration: rows of U and the multipliers in F, SB_ITERS solves from a
+    // pseudo-random start with loads SB_IU / SB_IB steps ahead; 32-bit element offsets from the
+    // uniform bases X and F (N <= 1024: 5 N^2 < 2^32)
+    const double tiny = tn > 0.0 ? eps * tn : eps;
+    const unsigned NU = (unsigned)N, ju = (unsigned)j;
+    double di = sd[0] - lam, dui = N > 1 ? se[0] : 0.0;
+    for (int i = 0; i < N - 1; i++) {
+        const double dl = se[i], dn = sd[i + 1] - lam, dun = i < N - 2 ? se[i + 1] : 0.0;
+        const unsigned o = 5u * (unsigned)i * NU + ju;
+        if (fabs(di) >= fabs(dl)) {
+            const double fc = di != 0.0 ? dl / di : 0.0;
+            F[o] = di; F[o + NU] = dui; F[o + 2 * NU] = 0.0; F[o + 3 * NU] = fc; F[o + 4 * NU] = 0.0;
+            di = dn - fc * dui;
+            dui = dun;
+        } else {
+            const double fc = di / dl;
+            F[o] = dl; F[o + NU] = dn; F[o + 2 * NU] = dun; F[o + 3 * NU] = fc; F[o + 4 * NU] = 1.0;
+            di = dui - fc * dn;
+            dui = -fc * dun;
+        }
+        X[(unsigned)i * NU + ju] = sb_rand((unsigned)i, ju);
+    }
+    {
+        const unsigned o = 5u * (NU - 1) * NU + ju;
+        F[o] = di; F[o + NU] = 0.0; F[o + 2 * NU] = 0.0;
+        X[(NU - 1) * NU + ju] = sb_rand(NU - 1, ju);
+    }
+    double scl = 1.0, s2 = 1.0;
+    for (int it = 0; it < SB_ITERS; it++) {
+        double bi = X[ju] * scl;
+        for (int i0 = 0; i0 < N - 1; i0 += SB_IU) {      // row operations of L on b
+            double fc[SB_IU], fs[SB_IU], bn[SB_IU];
+#pragma unroll
+            for (int u = 0; u < SB_IU; u++) {
+                const int i = i0 + u;
+                if (i < N - 1) {
+                    const unsigned o = 5u * (unsigned)i * NU + ju;
+                    fc[u] = F[o + 3 * NU];
+                    fs[u] = F[o + 4 * NU];
+                    bn[u] = X[(unsigned)(i + 1) * NU + ju];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < SB_IU; u++) {
+                const int i = i0 + u;
+                if (i < N - 1) {
+                    const double b = bn[u] * scl;
+                    double piv;
+                    if (fs[u] != 0.0) { piv = b; bi = bi - fc[u] * b; }
+                    else { piv = bi; bi = b - fc[u] * bi; }
+                    X[(unsigned)i * NU + ju] = piv;
+                }
+            }
+        }
+        X[(NU - 1) * NU + ju] = bi;
+        double x1 = 0.0, x2 = 0.0, xm = 0.0;
+        s2 = 0.0;
+        for (int i0 = N - 1; i0 >= 0; i0 -= SB_IB) {     // back substitution with U
+            double u0[SB_IB], u1[SB_IB], u2[SB_IB], bb[SB_IB];
+#pragma unroll
+            for (int u = 0; u < SB_IB; u++) {
+                const int i = i0 - u;
+                if (i >= 0) {
+                    const unsigned o = 5u * (unsigned)i * NU + ju;
+                    u0[u] = F[o];
+                    u1[u] = F[o + NU];
+                    u2[u] = F[o + 2 * NU];
+                    bb[u] = X[(unsigned)i * NU + ju];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < SB_IB; u++) {
+                const int i = i0 - u;
+                if (i >= 0) {
+                    double p0 = u0[u];
+                    if (fabs(p0) < tiny) p0 = p0 < 0.0 ? -tiny : tiny;
+                    const double x = (bb[u] - u1[u] * x1 - u2[u] * x2) / p0;
+                    X[(unsigned)i * NU + ju] = x;
+                    x2 = x1;
+                    x1 = x;
+                    xm = fmax(xm, fabs(x));
+                    s2 += x * x;
+                }
+            }
+        }
+        scl = xm > 0.0 ? 1.0 / xm : 1.0;
+    }
+    // unit 2-norm (s2: the squared norm of the last solve), into column j of Z
+    const double sn = s2 > 0.0 ? 1.0 / sqrt(s2) : 1.0;
+    for (int i0 = 0; i0 < N; i0 += SB_IU) {
+        double xv[SB_IU];
+#pragma unroll
+        for (int u = 0; u < SB_IU; u++) if (i0 + u < N) xv[u] = X[(unsigned)(i0 + u) * NU + ju];
+#pragma unroll
+        for (int u = 0; u < SB_IU; u++) if (i0 + u < N) Z[(unsigned)(i0 + u) + NU * ju] = xv[u] * sn;
+    }
+}
+
+// Clusters of eigenvalues with consecutive gaps <= 1e-4 ||T||, where independent inverse
+// iteration does not give orthogonal vectors (their angle errors are about eps ||T|| / gap):
+// Gram-Schmidt over the cluster's columns of Z, in eigenvalue order (the approach of LAPACK
+// dstein).  Clusters of at most SB_SMALLC vectors take one wavefront each (modified Gram-Schmidt,
+// twice); a larger one takes the whole workgroup, classical Gram-Schmidt applied twice per vector
+// (one wavefront per earlier vector for the dot products, one thread per row for the update).
+// Eigenvalues equal up to rounding give linearly independent vectors here because every one
+// started from its own pseudo-random vector.
+__global__ __launch_bounds__(SB_TB) void k_syevb_clus(SyevbArgs a) {
+    __shared__ double sw[SYEVB_NMAX], dts[SYEVB_NMAX];
+    __shared__ double red[SB_WAVES];
+    __shared__ int cls[SYEVB_NMAX / 2 + 1], cln[SYEVB_NMAX / 2 + 1];
+    __shared__ int ncl;
+    const int N = a.N[blockIdx.x];
+    double *work = a.work[blockIdx.x];
+    const double *meta = work + (i64)3 * N;
+    double *Z = work + (i64)3 * N + SB_META + (i64)6 * N * N;
+    const double *w = a.w[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int j = tid; j < N; j += SB_TB) sw[j] = w[j];
+    __syncthreads();
+    if (tid == 0) {
+        const double ctol = 1e-4 * meta[0];
+        int nc = 0, j = 0;
+        while (j < N) {
+            int k2 = j + 1;
+            while (k2 < N && sw[k2] - sw[k2 - 1] <= ctol) k2++;
+            if (k2 - j > 1) { cls[nc] = j; cln[nc] = k2 - j; nc++; }
+            j = k2;
+        }
+        ncl = nc;
+    }
+    __syncthreads();
+    const i64 NL = N;
+    for (int c = wv; c < ncl; c += SB_WAVES) {
+        const int j0 = cls[c], nn = cln[c];
+        if (nn > SB_SMALLC) continue;
+        for (int pass = 0; pass < 2; pass++)
+            for (int ja = j0; ja < j0 + nn; ja++) {
+                double *za = Z + NL * ja;
+                for (int jb = j0; jb < ja; jb++) {
+                    const double *zb = Z + NL * jb;
+                    double dot = 0.0;
+                    for (int i = lane; i < N; i += 64) dot += zb[i] * za[i];
+                    dot = sb_wave_sum(dot);
+                    for (int i = lane; i < N; i += 64) za[i] -= dot * zb[i];
+                }
+                double nr = 0.0;
+                for (int i = lane; i < N; i += 64) nr += za[i] * za[i];
+                nr = sb_wave_sum(nr);
+                const double sn = nr > 0.0 ? 1.0 / sqrt(nr) : 0.0;
+                for (int i = lane; i < N; i += 64) za[i] *= sn;
+            }
+    }
+    for (int c = 0; c < ncl; c++) {                  // uniform over the workgroup
+        const int j0 = cls[c], nn = cln[c];
+        if (nn <= SB_SMALLC) continue;
+        for (int ja = j0; ja < j0 + nn; ja++) {
+            double *za = Z + NL * ja;
+            const int np = ja - j0;
+            for (int pass = 0; pass < 2 && np > 0; pass++) {
+                for (int kb = wv; kb < np; kb += SB_WAVES) {
+                    const double *zb = Z + NL * (j0 + kb);
+                    double dot = 0.0;
+                    for (int i = lane; i < N; i += 64) dot += zb[i] * za[i];
+                    dot = sb_wave_sum(dot);
+                    if (lane == 0) dts[kb] = dot;
+                }
+                __syncthreads();
+                for (int i = tid; i < N; i += SB_TB) {
+                    double acc = 0.0;
+                    for (int kb = 0; kb < np; kb++) acc += dts[kb] * Z[i + NL * (j0 + kb)];
+                    za[i] -= acc;
+                }
+                __syncthreads();
+            }
+            double nr = 0.0;
+            for (int i = tid; i < N; i += SB_TB) nr += za[i] * za[i];
+            nr = sb_block_sum(nr, red);
+            const double sn = nr > 0.0 ? 1.0 / sqrt(nr) : 0.0;
+            for (int i = tid; i < N; i += SB_TB) za[i] *= sn;
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------- back-transformation
+// U = Q Z = H_0 H_1 .. H_{N-3} Z: workgroup (x, problem) holds the column panel x of Z (N x PW,
+// column-major in LDS) and applies the reflectors from the last to the first; reflector k is
+// v = e_{k+1} + A(k+2:N, k) with tau_k.  Each wavefront owns PW / 4 columns (the product v^T z
+// is a wavefront sum, no barrier), and the next reflector is read from global memory into
+// registers while the current one is applied, one barrier per reflector.  The panel is written
+// back to Z; k_syevb_copy then moves every Z into A (the reflectors in A are read by all panels
+// until the last one is done).
+template <int PW>
+__global__ __launch_bounds__(SB_BT) void k_syevb_back(SyevbArgs a) {
+    extern __shared__ double zs[];             // N x PW panel, then v of two reflectors (N each)
+    const int N = a.N[blockIdx.y];
+    const int c0 = blockIdx.x * PW;
+    if (c0 >= N) return;                       // uniform
+    const int pw = N - c0 < PW ? N - c0 : PW;
+    const double *A = a.A[blockIdx.y];
+    double *work = a.work[blockIdx.y];
+    const double *tau = work + (i64)2 * N;
+    double *Z = work + (i64)3 * N + SB_META + (i64)6 * N * N;
+    double *vb[2] = {zs + (i64)N * PW, zs + (i64)N * PW + N};
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const i64 NL = N;
+    for (int idx = tid; idx < N * pw; idx += SB_BT) zs[idx] = Z[(idx % N) + NL * (c0 + idx / N)];
+    if (N < 3) {                               // no reflectors
+        __syncthreads();
+        for (int idx = tid; idx < N * pw; idx += SB_BT) Z[(idx % N) + NL * (c0 + idx / N)] = zs[idx];
+        return;
+    }
+    int cur = 0;
+    {
+        const int k = N - 3;
+        for (int r = k + 1 + tid; r < N; r += SB_BT) vb[0][r] = r == k + 1 ? 1.0 : A[r + NL * k];
+    }
+    __syncthreads();
+    for (int k = N - 3; k >= 0; k--) {
+        double pre[SYEVB_NMAX / SB_BT];           // reflector k-1 (rows k..N-1), in flight
+#pragma unroll
+        for (int t = 0; t < SYEVB_NMAX / SB_BT; t++) {
+            const int r = k + tid + SB_BT * t;
+            pre[t] = 0.0;
+            if (k >= 1 && r < N) pre[t] = r == k ? 1.0 : A[r + NL * (k - 1)];
+        }
+        const double tk = tau[k];
+        const double *v = vb[cur];
+        if (tk != 0.0)
+            for (int c = wv; c < pw; c += SB_BT / 64) {
+                double *z = zs + NL * c;
+                double sacc = 0.0;
+                for (int r = k + 1 + lane; r < N; r += 64) sacc += v[r] * z[r];
+                sacc = sb_wave_sum(sacc) * tk;
+                for (int r = k + 1 + lane; r < N; r += 64) z[r] -= sacc * v[r];
+            }
+        if (k >= 1) {
+#pragma unroll
+            for (int t = 0; t < SYEVB_NMAX / SB_BT; t++) {
+                const int r = k + tid + SB_BT * t;
+                if (r < N) vb[cur ^ 1][r] = pre[t];
+            }
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    for (int idx = tid; idx < N * pw; idx += SB_BT) Z[(idx % N) + NL * (c0 + idx / N)] = zs[idx];
+}
+
+__global__ void k_syevb_copy(SyevbArgs a) {
+    const int N = a.N[blockIdx.y];
+    const i64 NN = (i64)N * N;
+    const double *Z = a.work[blockIdx.y] + (i64)3 * N + SB_META + (i64)6 * N * N;
+    double *A = a.A[blockIdx.y];
+    for (i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x; e < NN; e += (i64)gridDim.x * blockDim.x) A[e] = Z[e];
+}
+
+static size_t trd_lds_bytes() { return sizeof(double) * ((size_t)SB_MLDS * SB_MLDS + 4 * (size_t)SYEVB_NMAX); }
+static size_t back_lds_bytes(int N, int pw) { return sizeof(double) * ((size_t)N * pw + 2 * (size_t)N); }
+
+typedef void (*BackFn)(SyevbArgs);
+static BackFn back_fn(int pw) {
+    switch (pw) {
+        case 4: return k_syevb_back<4>;
+        case 8: return k_syevb_back<8>;
+        default: return k_syevb_back<16>;
+    }
+}
+
+// panel width of the back-transformation (columns of Z per workgroup): 16 up to order 512, 8 above
+// (the LDS panel stays at most 64 KB); NV2_SYEVB_PW=4|8|16 overrides it
+static int back_pw(int nmax) {
+    static const int env = [] {
+        const char *e = getenv("NV2_SYEVB_PW");
+        const int v = e ? atoi(e) : 0;
+        return v == 4 || v == 8 || v == 16 ? v : 0;
+    }();
+    if (env) return env;
+    return nmax <= 512 ? 16 : 8;
+}
+
+int syevb(rocblas_handle h, hipStream_t st, int np, const SyevbProb *P) {
+    (void)h;
+    static const bool attr_ok = [] {
+        bool ok = hipFuncSetAttribute((const void *)k_syevb_trd, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)trd_lds_bytes()) == hipSuccess;
+        for (int pw : {4, 8, 16})
+            ok = ok && hipFuncSetAttribute((const void *)back_fn(pw), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)back_lds_bytes(SYEVB_NMAX, pw)) == hipSuccess;
+        return ok;
+    }();
+    if (!attr_ok) { set_error("syevb: the LDS sizes of its kernels are not available"); return -1; }
+    for (int q = 0; q < np; q++)
+        if (P[q].N < 0 || P[q].N > SYEVB_NMAX) { set_error("syevb: order %d outside 0..%d", P[q].N, SYEVB_NMAX); return -2; }
+    for (int q0 = 0; q0 < np;) {
+        SyevbArgs a{};
+        int q = q0;
+        for (; q < np && a.n < SB_LAUNCH; q++) {
+            if (P[q].N == 0) continue;
+            a.N[a.n] = P[q].N; a.lower[a.n] = P[q].lower; a.A[a.n] = P[q].A; a.w[a.n] = P[q].w; a.work[a.n] = P[q].work;
+            a.n++;
+        }
+        if (a.n) {
+            int nmax = 1;
+            for (int b = 0; b < a.n; b++) nmax = a.N[b] > nmax ? a.N[b] : nmax;
+            hipLaunchKernelGGL(k_syevb_trd, dim3(a.n), dim3(SB_TB), trd_lds_bytes(), st, a);
+            NV2_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_syevb_vec, dim3((nmax + SB_VT - 1) / SB_VT, a.n), dim3(SB_VT), 0, st, a);
+            NV2_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_syevb_clus, dim3(a.n), dim3(SB_TB), 0, st, a);
+            NV2_LAUNCH_CHECK();
+            const int pw = back_pw(nmax);
+            hipLaunchKernelGGL(back_fn(pw), dim3((nmax + pw - 1) / pw, a.n), dim3(SB_BT), back_lds_bytes(nmax, pw), st,
+                               a);
+            NV2_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_syevb_copy, dim3(64, a.n), dim3(256), 0, st, a);
+            NV2_LAUNCH_CHECK();
+        }
+        q0 = q;
+    }
+    return 0;
+}
+
+}  // namespace nv2

@@ -1,4 +1,5 @@
-"""The (+-1)' PC sums through the Cholesky-factored metric equal those through its eigen-decomposition.
+"""The PC sums through the Cholesky-factored metric equal those through its eigen-decomposition
+(the (+-1)' classes and, through the batched solves, every other PC class).
 
 When no metric eigenvalue falls below thr_lindep the reference's thresholded inverse is S^-1, and
 the PC sums are invariant under caux -> caux O (O orthogonal); the library then factors S = U^T U
@@ -64,4 +65,7 @@ def test_metric_below_threshold_takes_eigen_path(gpu):
     print("kept", m_eig[0, 5:7, 1], "min eigenvalue", m_eig[0, 5:7, 0], flush=True)
     assert min(m_eig[0, 5:7, 0]) < 0.1 and np.all(m_def[0, 5:7, 2] == 0)
     assert np.all(m_def[0, 5:7, 1] == m_eig[0, 5:7, 1]) and m_eig[0, 5:7, 1].min() < n ** 3
-    assert np.allclose(t_def, t_eig, rtol=1e-12, atol=1e-12)
+    # (+-1)': the eigen path both times; the other PC classes may keep their factored metric (their
+    # eigenvalues can all lie above 0.1), which rounds differently
+    assert np.allclose(t_def[:, 5:7], t_eig[:, 5:7], rtol=1e-12, atol=1e-12)
+    assert np.allclose(t_def, t_eig, rtol=1e-9, atol=1e-10)
