@@ -212,9 +212,9 @@ int nv2_qd_heff(int64_t metat, const double *heff2, const double *e0, double *he
  * receives the eigenvalues in ascending order, as dsyev('V','U') returns them
  * (up to the sign of each eigenvector and the basis inside a degenerate
  * eigenspace).  Replaces the per-class metric and Koopmans-matrix dsyev calls of
- * the PC classes (qdnevpt.F:1995-2037 for (+1), the same pair in every class
- * driver, thresholding at :4744-4769), which stage 2 now issues as one batch.
- * Host arrays. */
+ * the PC classes (qdnevpt.F:1989 and 2023 for (+1), the same pair in every class
+ * driver, thresholding at :4744-4769) and the core and virtual Fock dsyev calls
+ * (:519, 671), which stage 2 now issues as batches.  Host arrays. */
 int nv2_syev_batched(int64_t nprob, const int64_t *N, double *const *A, double *const *w);
 
 /* Request loop (no counterpart in the reference, whose drivers evaluate one

@@ -1645,6 +1645,12 @@ static int gemm_sb(Stage2 *p, int m, int nn, int k, const double *A, i64 lda, i6
                                      sA, B, (int)ldb, sB, &zero, C, (int)ldc, sC, batch));
     return 0;
 }
+// NV2_SYEVB=0: the Fock blocks and the worker-0 classes take the per-matrix paths (rocSOLVER dsyevd,
+// trd_reduce + dstedc) instead of the batched eigensolver
+static bool syevb_on() {
+    const char *e = getenv("NV2_SYEVB");
+    return !(e && atoi(e) == 0);
+}
 // eigen-decomposition in place (ascending eigenvalues in w), reference triangle choice.
 // NV2_EIGEN selects the rocSOLVER algorithm: "dc" (syevd, default), "jacobi" (syevj), "dcj" (syevdj)
 static int eigen_algo() {
@@ -2167,12 +2173,6 @@ static int pc_kprime_form(Stage2 *p, PcJob &j, double thr) {
 }
 // one host thread per worker stream: the solver library blocks its calling thread at times,
 // so separate threads are what lets the chains of different problems overlap on the device
-// NV2_SYEVB=0: the worker-0 classes take the per-matrix path (trd_reduce + dstedc) instead of
-// the batched eigensolver
-static bool syevb_on() {
-    const char *e = getenv("NV2_SYEVB");
-    return !(e && atoi(e) == 0);
-}
 // The jobs of worker w (the classes other than (+-1)') as two batched eigen-solves: every metric
 // of every job in one syevb call, then every K' (after the host thresholding) in another, then
 // coef = caux V per job.
@@ -2904,8 +2904,30 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         GET(G, "fockG", (i64)L * n * (nc > nv ? nc : nv));
         TRYS(gemm_sb(p, L, n, n, Bac, L, (i64)L * n, da, n, 0, G, L, (i64)L * n, nc));
         TRYS(gemm(p, true, false, nc, nc, L * n, -0.5, G, (i64)L * n, Bac, (i64)L * n, 1.0, Cc, nc));
-        GET(wc, "wcore", nc);
-        TRYS(syevd(p, false, nc, Cc, wc));
+    }
+    GET(wc, "wcore", nc);
+    {
+        NV2_HIP(hipMemcpy2DAsync(Cv, sizeof(double) * nv, hf + (i64)(nc + n) * norb + nc + n, sizeof(double) * norb,
+                                 sizeof(double) * nv, nv, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_coul, dim3(nb((i64)nv * nv, 4)), dim3(256), 0, st, p->in.B, L, nc + n, nv, Jt, Cv);
+        NV2_LAUNCH_CHECK();
+        if (nc) TRYS(gemm(p, true, false, nv, nv, L * nc, -1.0, Bcv, (i64)L * nc, Bcv, (i64)L * nc, 1.0, Cv, nv));
+        GET(G, "fockG", (i64)L * n * (nc > nv ? nc : nv));
+        TRYS(gemm_sb(p, L, n, n, Bav, L, (i64)L * n, da, n, 0, G, L, (i64)L * n, nv));
+        TRYS(gemm(p, true, false, nv, nv, L * n, -0.5, G, (i64)L * n, Bav, (i64)L * n, 1.0, Cv, nv));
+    }
+    // canonical core and virtual orbitals: dsyev('V','L') of both Fock blocks (qdnevpt.F:519, 671), as one
+    // batched solve when both orders allow it
+    GET(wv, "wvirt", nv);
+    if (syevb_on() && nc <= SYEVB_NMAX && nv <= SYEVB_NMAX) {
+        GET(fw, "fock_sbw", syevb_workspace(nc) + syevb_workspace(nv));
+        SyevbProb fp[2] = {SyevbProb{Cc, wc, fw, nc, 1}, SyevbProb{Cv, wv, fw + syevb_workspace(nc), nv, 1}};
+        TRYS(syevb(p->h, st, 2, fp));
+    } else {
+        if (nc) TRYS(syevd(p, false, nc, Cc, wc));
+        TRYS(syevd(p, false, nv, Cv, wv));
+    }
+    if (nc) {
         NV2_HIP(hipMemcpyAsync(eps, wc, sizeof(double) * nc, hipMemcpyDeviceToDevice, st));
         if (!p->frozen.empty()) {   // .FROZEN: these canonical core orbitals are never holes
             const int nf = (int)p->frozen.size();
@@ -2916,19 +2938,7 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             NV2_LAUNCH_CHECK();
         }
     }
-    {
-        NV2_HIP(hipMemcpy2DAsync(Cv, sizeof(double) * nv, hf + (i64)(nc + n) * norb + nc + n, sizeof(double) * norb,
-                                 sizeof(double) * nv, nv, hipMemcpyDeviceToDevice, st));
-        hipLaunchKernelGGL(k_coul, dim3(nb((i64)nv * nv, 4)), dim3(256), 0, st, p->in.B, L, nc + n, nv, Jt, Cv);
-        NV2_LAUNCH_CHECK();
-        if (nc) TRYS(gemm(p, true, false, nv, nv, L * nc, -1.0, Bcv, (i64)L * nc, Bcv, (i64)L * nc, 1.0, Cv, nv));
-        GET(G, "fockG", (i64)L * n * (nc > nv ? nc : nv));
-        TRYS(gemm_sb(p, L, n, n, Bav, L, (i64)L * n, da, n, 0, G, L, (i64)L * n, nv));
-        TRYS(gemm(p, true, false, nv, nv, L * n, -0.5, G, (i64)L * n, Bav, (i64)L * n, 1.0, Cv, nv));
-        GET(wv, "wvirt", nv);
-        TRYS(syevd(p, false, nv, Cv, wv));
-        NV2_HIP(hipMemcpyAsync(eps + nc + n, wv, sizeof(double) * nv, hipMemcpyDeviceToDevice, st));
-    }
+    NV2_HIP(hipMemcpyAsync(eps + nc + n, wv, sizeof(double) * nv, hipMemcpyDeviceToDevice, st));
     const double *ec = eps, *ev = eps + nc + n;
     // ---- Cholesky transform (qdnevpt.F:7599-7717)
     GET(T1, "T1", (i64)L * nc * (nv > nc ? nv : nc));

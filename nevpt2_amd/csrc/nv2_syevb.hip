@@ -1,7 +1,7 @@
 // Batched symmetric eigensolver for the small PC-class problems of stage 2.
 //
 // Every PC class diagonalises its metric S and then its Koopmans matrix K' (dsyev('V','U'),
-// e.g. qdnevpt.F:1995-2037 for class (+1), 4744-4769 for the thresholding).  Apart from the
+// e.g. qdnevpt.F:1989 and 2023 for class (+1), 4744-4769 for the thresholding).  Apart from the
 // n^3 problems of the (+-1)' classes these are small (orders n, n(n+1)/2, n^2, 2n^2: 14..392 at
 // n = 14) and many (two per class and state).  Solved one by one through a LAPACK-style library
 // path each costs hundreds of dependent launches of small kernels; here every problem of a phase
