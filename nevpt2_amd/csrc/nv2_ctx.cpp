@@ -332,6 +332,12 @@ int nv2_ctx_timings(nv2_ctx *c, double *ms) {
 
 int nv2_ctx_get(nv2_ctx *c, const char *name, double *host, int64_t count) {
     if (!c || !name) return -2;
+    if (!strcmp(name, "PC-graphs")) {        // hipGraph launches of the batched eigensolver call sites
+        if (count != 3) { set_error("nv2_ctx_get(PC-graphs): count %lld != 3", (long long)count); return -2; }
+        if (!c->s2) { for (int k = 0; k < 3; k++) host[k] = 0.0; return 0; }
+        stage2_graph_launches(c->s2, host);
+        return 0;
+    }
     const std::vector<double> *hv = !strcmp(name, "Heff-SC") ? &c->heff_sc : !strcmp(name, "Heff-PC") ? &c->heff_pc
                                   : !strcmp(name, "QD") ? &c->qd : !strcmp(name, "PC-metric") ? &c->pcm
                                   : !strcmp(name, "PC-timings") ? &c->pct : !strcmp(name, "rho1") ? &c->rho1

@@ -29,6 +29,8 @@ struct S2Stats {              // diagnostics for threshold-sensitive seeds
 struct Stage2;
 Stage2 *stage2_create(const S2Inputs &in, rocblas_handle h, hipStream_t st);
 void stage2_destroy(Stage2 *p);
+// hipGraph launches so far of the batched eigensolver call sites: worker-0 metrics, worker-0 K', Fock blocks
+void stage2_graph_launches(Stage2 *p, double *out3);
 // out[k*4 + {0,1,2,3}] = (norm_sc, e_sc, norm_pc, e_pc) for class k in the reference order
 // (0),(+1),(-1),(+2),(-2),(+1)',(-1)',(0)'.  With qd_out (16*M doubles) and transition RDMs
 // present, the QD couplings of state s: qd_out[k*2M + j] = SC e2mp(j,s), qd_out[k*2M + M + j] =

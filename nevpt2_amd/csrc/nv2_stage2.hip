@@ -1546,6 +1546,11 @@ struct Stage2 {
     int *hinfo = nullptr;          // pinned host copies of solver info words
     int *dinfo_w = nullptr;        // device info words, 2 per solve slot
     double *wE[NW] = {};           // syevd workspace per worker
+    SyevbPlan *sbplan[3] = {};     // batched eigensolver call sites: worker-0 metrics, worker-0 K', Fock blocks
+    SyevbPlan *plan(int k) {
+        if (!sbplan[k]) sbplan[k] = syevb_plan_create();
+        return sbplan[k];
+    }
     hipStream_t xs = nullptr;      // second stream + handle for the "-" problem's dstedc of the n^3 metric
     rocblas_handle xh = nullptr;
     hipEvent_t xev[2] = {};
@@ -2237,7 +2242,7 @@ static int pc_worker_batched(Stage2 *p, PcJob *J, int njob, int w, double thr, d
         for (int b = 0; b < j.nb && j.N; b++)
             pr.push_back(SyevbProb{j.Sb(b), j.w1 + (i64)b * j.N, j.sbw + b * syevb_workspace(j.N), j.N});
     }
-    TRYS(syevb(h, st, (int)pr.size(), pr.data()));
+    TRYS(syevb(h, st, (int)pr.size(), pr.data(), p->plan(0)));
     for (int k = 0; k < njob; k++)
         if (J[k].w == w && J[k].N && !J[k].chol)
             NV2_HIP(hipMemcpyAsync(J[k].hw1, J[k].w1, sizeof(double) * J[k].N * J[k].nb, hipMemcpyDeviceToHost, st));
@@ -2265,7 +2270,7 @@ static int pc_worker_batched(Stage2 *p, PcJob *J, int njob, int w, double thr, d
         for (int b = 0; b < j.nb && j.nmax; b++)
             pr.push_back(SyevbProb{j.Kp + b * j.NN(), j.w2 + (i64)b * j.N, j.sbw + b * syevb_workspace(j.N), j.nmax});
     }
-    TRYS(syevb(h, st, (int)pr.size(), pr.data()));
+    TRYS(syevb(h, st, (int)pr.size(), pr.data(), p->plan(1)));
     for (int k = 0; k < njob; k++) {
         const PcJob &j = J[k];
         if (j.w != w || !j.nmax) continue;
@@ -2424,6 +2429,9 @@ static int pm1p_worker(Stage2 *p, Pm1pEarly *E) {
     return 0;
 }
 
+void stage2_graph_launches(Stage2 *p, double *out) {
+    for (int k = 0; k < 3; k++) out[k] = (double)syevb_plan_graph_launches(p->sbplan[k]);
+}
 void stage2_set_no4rdm(Stage2 *p, bool on) { p->no4 = on; }
 
 void stage2_set_rho1(Stage2 *p, bool on) { p->rho1 = on; }
@@ -2610,6 +2618,7 @@ void stage2_destroy(Stage2 *p) {
     if (p->evK) (void)hipEventDestroy(p->evK);
     for (auto e : p->evKr) if (e) (void)hipEventDestroy(e);
     p->sync_own();      // this evaluation's streams only; other contexts keep running
+    for (auto &pl : p->sbplan) syevb_plan_destroy(pl);
     for (auto &kv : p->bufs) if (kv.second.p) (void)hipFree(kv.second.p);
     (void)hipFree(p->dinfo); (void)hipFree(p->part); (void)hipFree(p->acc); (void)hipFree(p->accq);
     (void)hipFree(p->qpart); (void)hipFree(p->dinfo_w);
@@ -2922,7 +2931,7 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
     if (syevb_on() && nc <= SYEVB_NMAX && nv <= SYEVB_NMAX) {
         GET(fw, "fock_sbw", syevb_workspace(nc) + syevb_workspace(nv));
         SyevbProb fp[2] = {SyevbProb{Cc, wc, fw, nc, 1}, SyevbProb{Cv, wv, fw + syevb_workspace(nc), nv, 1}};
-        TRYS(syevb(p->h, st, 2, fp));
+        TRYS(syevb(p->h, st, 2, fp, p->plan(2)));
     } else {
         if (nc) TRYS(syevd(p, false, nc, Cc, wc));
         TRYS(syevd(p, false, nv, Cv, wv));

@@ -20,8 +20,16 @@ struct SyevbProb {
     int lower = 0;
 };
 i64 syevb_workspace(int N);
-// Every problem of P (any number, any mix of orders 1..SYEVB_NMAX) on stream st; h must run
-// on st.  Two kernel launches per 16 problems plus the back-transformation GEMMs per problem.
-int syevb(rocblas_handle h, hipStream_t st, int np, const SyevbProb *P);
+// Launch shapes and captured launches of one call site: with a plan, the first call with a list of
+// problems times the candidate shapes (once per list of orders in the process) and solves; the
+// second call with the same buffers captures its launches in a hipGraph, and later ones replay it.
+struct SyevbPlan;
+SyevbPlan *syevb_plan_create();
+void syevb_plan_destroy(SyevbPlan *plan);
+i64 syevb_plan_graph_launches(const SyevbPlan *plan);   // hipGraph launches so far (0 for no plan)
+// Every problem of P (any number, any mix of orders 1..SYEVB_NMAX) on stream st: five kernel
+// launches per 16 problems.  h is not used (no library call is made).  Without a plan the launches
+// go to st directly (shapes from the process's tuning table as well).
+int syevb(rocblas_handle h, hipStream_t st, int np, const SyevbProb *P, SyevbPlan *plan = nullptr);
 
 }  // namespace nv2

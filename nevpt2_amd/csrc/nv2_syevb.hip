@@ -16,13 +16,20 @@
 //                  Reflectors are left below the subdiagonal as dsytrd('L') does.
 //   k_syevb_vec    eigenvalues of T by bisection on Sturm counts and eigenvectors by inverse
 //                  iteration with a pivoted LU of T - lambda I, one thread per eigenpair,
-//                  SB_VT eigenpairs per workgroup (the approach of LAPACK dstebz + dstein).
+//                  64, 128 or 256 eigenpairs per workgroup (the approach of LAPACK dstebz +
+//                  dstein).
 //   k_syevb_clus   Gram-Schmidt inside every cluster of eigenvalues closer than 1e-4 ||T||,
 //                  where independent inverse iteration does not give orthogonal vectors.
-//   k_syevb_back   U = Q Z, one workgroup per column panel of Z (held in LDS), the reflectors
-//                  applied from the last to the first, the next one read while the current one
-//                  is applied.
+//   k_syevb_back   U = Q Z, one workgroup per column panel of Z (4, 8 or 16 columns, held in
+//                  LDS), the reflectors applied from the last to the first, the next one read
+//                  while the current one is applied.
 //   k_syevb_copy   U into A.
+// A SyevbPlan (one per call site and problem list) picks the two launch shapes the first time it
+// sees a list, by timing every candidate on that call's own data (k_syevb_vec and k_syevb_back
+// read their inputs without overwriting them, so each candidate reruns the same step); the
+// choices are kept per list of orders for the process (and in the file NV2_TUNE_FILE when set).
+// From the second call with the same buffers on, the plan replays a hipGraph of the five launches
+// captured on that call.
 // Workgroups stay below a whole CU (64 VGPRs at 16 wavefronts, 64 KB LDS for the reduction;
 // 256-thread workgroups elsewhere) so that they are placed beside the other streams' kernels:
 // a first form with 1024-thread, 128-VGPR workgroups waited tens of milliseconds for idle CUs
@@ -32,7 +39,13 @@
 
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
 
 #include "nv2_common.h"
 #include "nv2_syevb.h"
@@ -41,7 +54,6 @@ namespace nv2 {
 
 #define SB_TB 1024               // threads per workgroup of the tridiagonalisation
 #define SB_WAVES (SB_TB / 64)
-#define SB_VT 256                // threads (eigenpairs) per workgroup of k_syevb_vec
 #define SB_SMALLC 32             // clusters up to this size: one wavefront each
 #define SB_BT 256                // threads per workgroup of the back-transformation
 #define SB_META 8                // per-problem scalars between tau and X
@@ -317,15 +329,16 @@ __device__ inline double sb_rand(unsigned i, unsigned j) {
     return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
 }
 
-// Eigenpairs of T, SB_VT eigenvalues per workgroup (grid.x) of problem grid.y: w_j by bisection on
+// Eigenpairs of T, VT eigenvalues per workgroup (grid.x) of problem grid.y: w_j by bisection on
 // Sturm counts, then the eigenvector by inverse iteration with a pivoted LU of T - w_j I, one
 // thread per eigenpair, written to column j of Z with unit norm.  T is scaled by a power of two
 // (exact) to a norm in [0.5, 1) first, so the inverse iteration's growth stays far from overflow.
-__global__ __launch_bounds__(SB_VT) void k_syevb_vec(SyevbArgs a) {
+template <int VT>
+__global__ __launch_bounds__(VT) void k_syevb_vec(SyevbArgs a) {
     __shared__ double sd[SYEVB_NMAX], se[SYEVB_NMAX], se2[SYEVB_NMAX];
-    __shared__ double red[SB_VT / 64];
+    __shared__ double red[VT / 64];
     const int N = a.N[blockIdx.y];
-    if ((int)blockIdx.x * SB_VT >= N) return;      // uniform: no eigenvalue of this problem here
+    if ((int)blockIdx.x * VT >= N) return;         // uniform: no eigenvalue of this problem here
     double *work = a.work[blockIdx.y];
     const double *d = work, *e = work + N;
     double *meta = work + (i64)3 * N;
@@ -333,7 +346,7 @@ __global__ __launch_bounds__(SB_VT) void k_syevb_vec(SyevbArgs a) {
     double *wout = a.w[blockIdx.y];
     const int tid = threadIdx.x;
     double glo = DBL_MAX, ghi = -DBL_MAX, em2 = 0.0;
-    for (int i = tid; i < N; i += SB_VT) {
+    for (int i = tid; i < N; i += VT) {
         const double ei = i < N - 1 ? e[i] : 0.0, eim = i > 0 ? e[i - 1] : 0.0;
         sd[i] = d[i];
         se[i] = ei;
@@ -347,7 +360,7 @@ __global__ __launch_bounds__(SB_VT) void k_syevb_vec(SyevbArgs a) {
     const double tn0 = fmax(fabs(glo), fabs(ghi));
     if (tn0 > 0.0) (void)frexp(tn0, &ex);
     const double sc = ldexp(1.0, -ex);
-    for (int i = tid; i < N; i += SB_VT) {
+    for (int i = tid; i < N; i += VT) {
         sd[i] *= sc;
         se[i] *= sc;
         se2[i] = se[i] * se[i];
@@ -362,7 +375,7 @@ __global__ __launch_bounds__(SB_VT) void k_syevb_vec(SyevbArgs a) {
     const double pivmin = DBL_MIN * fmax(1.0, em2);
     glo -= 2.0 * eps * tn * N + pivmin;
     ghi += 2.0 * eps * tn * N + pivmin;
-    const int j = blockIdx.x * SB_VT + tid;
+    const int j = blockIdx.x * VT + tid;
     if (j >= N) return;                            // no barrier below
     // bisection: w_j is the (j+1)-th smallest eigenvalue (Sturm count of T - x I)
     double lo = glo, hi = ghi;
@@ -571,9 +584,10 @@ __global__ __launch_bounds__(SB_TB) void k_syevb_clus(SyevbArgs a) {
 // column-major in LDS) and applies the reflectors from the last to the first; reflector k is
 // v = e_{k+1} + A(k+2:N, k) with tau_k.  Each wavefront owns PW / 4 columns (the product v^T z
 // is a wavefront sum, no barrier), and the next reflector is read from global memory into
-// registers while the current one is applied, one barrier per reflector.  The panel is written
-// back to Z; k_syevb_copy then moves every Z into A (the reflectors in A are read by all panels
-// until the last one is done).
+// registers while the current one is applied, one barrier per reflector.  The panel goes to U (the
+// F scratch, free after k_syevb_vec), so that Z stays intact and the step can be rerun;
+// k_syevb_copy then moves every U into A (the reflectors in A are read by all panels until the last
+// one is done).
 template <int PW>
 __global__ __launch_bounds__(SB_BT) void k_syevb_back(SyevbArgs a) {
     extern __shared__ double zs[];             // N x PW panel, then v of two reflectors (N each)
@@ -584,14 +598,15 @@ __global__ __launch_bounds__(SB_BT) void k_syevb_back(SyevbArgs a) {
     const double *A = a.A[blockIdx.y];
     double *work = a.work[blockIdx.y];
     const double *tau = work + (i64)2 * N;
-    double *Z = work + (i64)3 * N + SB_META + (i64)6 * N * N;
+    const double *Z = work + (i64)3 * N + SB_META + (i64)6 * N * N;
+    double *U = work + (i64)3 * N + SB_META + (i64)N * N;
     double *vb[2] = {zs + (i64)N * PW, zs + (i64)N * PW + N};
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const i64 NL = N;
     for (int idx = tid; idx < N * pw; idx += SB_BT) zs[idx] = Z[(idx % N) + NL * (c0 + idx / N)];
     if (N < 3) {                               // no reflectors
         __syncthreads();
-        for (int idx = tid; idx < N * pw; idx += SB_BT) Z[(idx % N) + NL * (c0 + idx / N)] = zs[idx];
+        for (int idx = tid; idx < N * pw; idx += SB_BT) U[(idx % N) + NL * (c0 + idx / N)] = zs[idx];
         return;
     }
     int cur = 0;
@@ -628,79 +643,283 @@ __global__ __launch_bounds__(SB_BT) void k_syevb_back(SyevbArgs a) {
         __syncthreads();
         cur ^= 1;
     }
-    for (int idx = tid; idx < N * pw; idx += SB_BT) Z[(idx % N) + NL * (c0 + idx / N)] = zs[idx];
+    for (int idx = tid; idx < N * pw; idx += SB_BT) U[(idx % N) + NL * (c0 + idx / N)] = zs[idx];
 }
 
 __global__ void k_syevb_copy(SyevbArgs a) {
     const int N = a.N[blockIdx.y];
     const i64 NN = (i64)N * N;
-    const double *Z = a.work[blockIdx.y] + (i64)3 * N + SB_META + (i64)6 * N * N;
+    const double *U = a.work[blockIdx.y] + (i64)3 * N + SB_META + (i64)N * N;
     double *A = a.A[blockIdx.y];
-    for (i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x; e < NN; e += (i64)gridDim.x * blockDim.x) A[e] = Z[e];
+    for (i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x; e < NN; e += (i64)gridDim.x * blockDim.x) A[e] = U[e];
 }
 
 static size_t trd_lds_bytes() { return sizeof(double) * ((size_t)SB_MLDS * SB_MLDS + 4 * (size_t)SYEVB_NMAX); }
 static size_t back_lds_bytes(int N, int pw) { return sizeof(double) * ((size_t)N * pw + 2 * (size_t)N); }
 
-typedef void (*BackFn)(SyevbArgs);
-static BackFn back_fn(int pw) {
-    switch (pw) {
-        case 4: return k_syevb_back<4>;
-        case 8: return k_syevb_back<8>;
-        default: return k_syevb_back<16>;
+typedef void (*SbFn)(SyevbArgs);
+static const int BACK_PW[3] = {4, 8, 16};
+static const int VEC_VT[3] = {64, 128, 256};
+static SbFn back_fn(int pw) { return pw == 4 ? k_syevb_back<4> : pw == 8 ? k_syevb_back<8> : k_syevb_back<16>; }
+static SbFn vec_fn(int vt) { return vt == 64 ? k_syevb_vec<64> : vt == 128 ? k_syevb_vec<128> : k_syevb_vec<256>; }
+
+// ---------------------------------------------------------------- launch-shape tuning
+struct SbShape {
+    int vt = 256, pw = 16;
+};
+// tuned shapes per list of orders (the key "N1,N2,..."), for the process; NV2_TUNE_FILE persists them
+static std::mutex g_tune_mu;
+static std::map<std::string, SbShape> g_tune;
+static bool g_tune_loaded = false;
+static std::string order_key(const SyevbArgs &a) {
+    std::string k;
+    for (int b = 0; b < a.n; b++) k += (b ? "," : "") + std::to_string(a.N[b]);
+    return k;
+}
+static void tune_load() {               // caller holds g_tune_mu
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    const char *f = getenv("NV2_TUNE_FILE");
+    if (!f) return;
+    FILE *fp = fopen(f, "r");
+    if (!fp) return;
+    char key[4096];
+    int vt, pw;
+    while (fscanf(fp, " syevb %4095s %d %d", key, &vt, &pw) == 3) g_tune[key] = SbShape{vt, pw};
+    fclose(fp);
+}
+static void tune_store(const std::string &key, const SbShape &sh) {   // caller holds g_tune_mu
+    g_tune[key] = sh;
+    const char *f = getenv("NV2_TUNE_FILE");
+    if (!f) return;
+    FILE *fp = fopen(f, "a");
+    if (!fp) return;
+    fprintf(fp, "syevb %s %d %d\n", key.c_str(), sh.vt, sh.pw);
+    fclose(fp);
+}
+// NV2_SYEVB_TUNE=0: no timing, the fixed shapes (256 eigenpairs, panels of 16 up to order 512, else 8)
+static bool tune_on() {
+    const char *e = getenv("NV2_SYEVB_TUNE");
+    return !(e && atoi(e) == 0);
+}
+// NV2_SYEVB_GRAPH=0: every call launches directly
+static bool graph_on() {
+    const char *e = getenv("NV2_SYEVB_GRAPH");
+    return !(e && atoi(e) == 0);
+}
+
+static size_t back_lds_max() { return back_lds_bytes(SYEVB_NMAX, 8); }
+static int back_pw_ok(int pw, int nmax) { return back_lds_bytes(nmax, pw) <= back_lds_max(); }
+
+static int enqueue_chunk(hipStream_t st, const SyevbArgs &a, const SbShape &sh) {
+    int nmax = 1;
+    for (int b = 0; b < a.n; b++) nmax = a.N[b] > nmax ? a.N[b] : nmax;
+    hipLaunchKernelGGL(k_syevb_trd, dim3(a.n), dim3(SB_TB), trd_lds_bytes(), st, a);
+    NV2_LAUNCH_CHECK();
+    hipLaunchKernelGGL(vec_fn(sh.vt), dim3((nmax + sh.vt - 1) / sh.vt, a.n), dim3(sh.vt), 0, st, a);
+    NV2_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_syevb_clus, dim3(a.n), dim3(SB_TB), 0, st, a);
+    NV2_LAUNCH_CHECK();
+    hipLaunchKernelGGL(back_fn(sh.pw), dim3((nmax + sh.pw - 1) / sh.pw, a.n), dim3(SB_BT), back_lds_bytes(nmax, sh.pw),
+                       st, a);
+    NV2_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_syevb_copy, dim3(64, a.n), dim3(256), 0, st, a);
+    NV2_LAUNCH_CHECK();
+    return 0;
+}
+
+// One chunk with the shapes chosen by timing: the tridiagonalisation once, then every k_syevb_vec
+// candidate (the last one timed leaves the same results as any other: each eigenpair is computed
+// by one thread whatever the workgroup size), the cluster step, every k_syevb_back candidate that
+// fits the LDS limit, and the copy.  Synchronises st (first call of a new list of orders only).
+static int tune_chunk(hipStream_t st, const SyevbArgs &a, SbShape &best) {
+    int nmax = 1;
+    for (int b = 0; b < a.n; b++) nmax = a.N[b] > nmax ? a.N[b] : nmax;
+    hipEvent_t e0, e1;
+    NV2_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); set_error("syevb: hipEventCreate"); return -1; }
+    int rc = 0;
+    auto timed = [&](auto launch, float &ms) -> int {
+        if (hipEventRecord(e0, st) != hipSuccess) return -1;
+        launch();
+        if (hipGetLastError() != hipSuccess || hipEventRecord(e1, st) != hipSuccess ||
+            hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            return -1;
+        return 0;
+    };
+    hipLaunchKernelGGL(k_syevb_trd, dim3(a.n), dim3(SB_TB), trd_lds_bytes(), st, a);
+    float bt = 1e30f;
+    for (int vt : VEC_VT) {
+        float ms = 0.f;
+        if (timed([&] { hipLaunchKernelGGL(vec_fn(vt), dim3((nmax + vt - 1) / vt, a.n), dim3(vt), 0, st, a); }, ms)) {
+            rc = -1;
+            break;
+        }
+        if (ms < bt) { bt = ms; best.vt = vt; }
     }
+    if (!rc) {
+        hipLaunchKernelGGL(k_syevb_clus, dim3(a.n), dim3(SB_TB), 0, st, a);
+        float bb = 1e30f;
+        for (int pw : BACK_PW) {
+            if (!back_pw_ok(pw, nmax)) continue;
+            float ms = 0.f;
+            if (timed([&] {
+                    hipLaunchKernelGGL(back_fn(pw), dim3((nmax + pw - 1) / pw, a.n), dim3(SB_BT),
+                                       back_lds_bytes(nmax, pw), st, a);
+                }, ms)) {
+                rc = -1;
+                break;
+            }
+            if (ms < bb) { bb = ms; best.pw = pw; }
+        }
+    }
+    if (!rc) hipLaunchKernelGGL(k_syevb_copy, dim3(64, a.n), dim3(256), 0, st, a);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc || hipGetLastError() != hipSuccess) { set_error("syevb: a launch failed while tuning"); return -1; }
+    return 0;
 }
 
-// panel width of the back-transformation (columns of Z per workgroup): 16 up to order 512, 8 above
-// (the LDS panel stays at most 64 KB); NV2_SYEVB_PW=4|8|16 overrides it
-static int back_pw(int nmax) {
-    static const int env = [] {
-        const char *e = getenv("NV2_SYEVB_PW");
-        const int v = e ? atoi(e) : 0;
-        return v == 4 || v == 8 || v == 16 ? v : 0;
-    }();
-    if (env) return env;
-    return nmax <= 512 ? 16 : 8;
+static SbShape default_shape(const SyevbArgs &a) {
+    int nmax = 1;
+    for (int b = 0; b < a.n; b++) nmax = a.N[b] > nmax ? a.N[b] : nmax;
+    SbShape sh;
+    sh.vt = 256;
+    sh.pw = nmax <= 512 ? 16 : 8;
+    return sh;
 }
 
-int syevb(rocblas_handle h, hipStream_t st, int np, const SyevbProb *P) {
+struct SyevbPlan {
+    std::vector<SyevbArgs> chunks;       // the problem list of the captured graph
+    std::vector<SbShape> shapes;
+    int calls = 0;                       // calls with this list so far
+    i64 launches = 0;                    // graph launches over the plan's life
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cap = nullptr;           // private stream the launches are captured on
+    void drop() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+};
+SyevbPlan *syevb_plan_create() { return new (std::nothrow) SyevbPlan(); }
+i64 syevb_plan_graph_launches(const SyevbPlan *p) { return p ? p->launches : 0; }
+void syevb_plan_destroy(SyevbPlan *p) {
+    if (!p) return;
+    p->drop();
+    if (p->cap) (void)hipStreamDestroy(p->cap);
+    delete p;
+}
+
+static bool same_args(const SyevbArgs &x, const SyevbArgs &y) {
+    if (x.n != y.n) return false;
+    for (int b = 0; b < x.n; b++)
+        if (x.N[b] != y.N[b] || x.lower[b] != y.lower[b] || x.A[b] != y.A[b] || x.w[b] != y.w[b] ||
+            x.work[b] != y.work[b])
+            return false;
+    return true;
+}
+
+int syevb(rocblas_handle h, hipStream_t st, int np, const SyevbProb *P, SyevbPlan *plan) {
     (void)h;
     static const bool attr_ok = [] {
         bool ok = hipFuncSetAttribute((const void *)k_syevb_trd, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)trd_lds_bytes()) == hipSuccess;
-        for (int pw : {4, 8, 16})
+        for (int pw : BACK_PW)
             ok = ok && hipFuncSetAttribute((const void *)back_fn(pw), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)back_lds_bytes(SYEVB_NMAX, pw)) == hipSuccess;
+                                           (int)back_lds_max()) == hipSuccess;
         return ok;
     }();
     if (!attr_ok) { set_error("syevb: the LDS sizes of its kernels are not available"); return -1; }
+    std::vector<SyevbArgs> chunks;
     for (int q = 0; q < np; q++)
         if (P[q].N < 0 || P[q].N > SYEVB_NMAX) { set_error("syevb: order %d outside 0..%d", P[q].N, SYEVB_NMAX); return -2; }
-    for (int q0 = 0; q0 < np;) {
+    for (int q = 0; q < np;) {
         SyevbArgs a{};
-        int q = q0;
         for (; q < np && a.n < SB_LAUNCH; q++) {
             if (P[q].N == 0) continue;
             a.N[a.n] = P[q].N; a.lower[a.n] = P[q].lower; a.A[a.n] = P[q].A; a.w[a.n] = P[q].w; a.work[a.n] = P[q].work;
             a.n++;
         }
-        if (a.n) {
-            int nmax = 1;
-            for (int b = 0; b < a.n; b++) nmax = a.N[b] > nmax ? a.N[b] : nmax;
-            hipLaunchKernelGGL(k_syevb_trd, dim3(a.n), dim3(SB_TB), trd_lds_bytes(), st, a);
-            NV2_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_syevb_vec, dim3((nmax + SB_VT - 1) / SB_VT, a.n), dim3(SB_VT), 0, st, a);
-            NV2_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_syevb_clus, dim3(a.n), dim3(SB_TB), 0, st, a);
-            NV2_LAUNCH_CHECK();
-            const int pw = back_pw(nmax);
-            hipLaunchKernelGGL(back_fn(pw), dim3((nmax + pw - 1) / pw, a.n), dim3(SB_BT), back_lds_bytes(nmax, pw), st,
-                               a);
-            NV2_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_syevb_copy, dim3(64, a.n), dim3(256), 0, st, a);
-            NV2_LAUNCH_CHECK();
+        if (a.n) chunks.push_back(a);
+    }
+    if (chunks.empty()) return 0;
+    if (plan && plan->chunks.size() == chunks.size()) {
+        bool same = true;
+        for (size_t c = 0; c < chunks.size() && same; c++) same = same_args(plan->chunks[c], chunks[c]);
+        if (!same) { plan->drop(); plan->calls = 0; plan->shapes.clear(); }
+    } else if (plan) {
+        plan->drop();
+        plan->calls = 0;
+        plan->shapes.clear();
+    }
+    if (plan && plan->exec && graph_on()) {                   // third call on: replay
+        NV2_HIP(hipGraphLaunch(plan->exec, st));
+        plan->calls++;
+        plan->launches++;
+        return 0;
+    }
+    // launch shapes: tuned per list of orders (timing on this call's data the first time a list is seen)
+    std::vector<SbShape> shapes(chunks.size());
+    for (size_t c = 0; c < chunks.size(); c++) {
+        const std::string key = order_key(chunks[c]);
+        bool have = false;
+        if (tune_on()) {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            tune_load();
+            auto it = g_tune.find(key);
+            if (it != g_tune.end()) { shapes[c] = it->second; have = true; }
+        } else {
+            shapes[c] = default_shape(chunks[c]);
+            have = true;
         }
-        q0 = q;
+        if (!have) {
+            SbShape sh = default_shape(chunks[c]);
+            if (tune_chunk(st, chunks[c], sh)) return -1;      // this chunk is solved by the tuning run itself
+            {
+                std::lock_guard<std::mutex> lk(g_tune_mu);
+                tune_store(key, sh);
+            }
+            shapes[c] = sh;
+            chunks[c].n = -chunks[c].n;                         // done: skip it below
+        }
+    }
+    // The capture runs on the plan's own stream, which no other stream waits on: a capture on the
+    // caller's stream would turn every event recorded there meanwhile into a graph node, and a wait
+    // on such an event from another thread's stream fails ("dependency created on uncaptured work").
+    bool capture = plan && graph_on() && plan->calls >= 1;
+    for (size_t c = 0; c < chunks.size() && capture; c++) capture = chunks[c].n > 0;   // nothing solved while tuning
+    if (capture && !plan->cap) NV2_HIP(hipStreamCreateWithFlags(&plan->cap, hipStreamNonBlocking));
+    hipStream_t es = capture ? plan->cap : st;
+    if (capture) NV2_HIP(hipStreamBeginCapture(es, hipStreamCaptureModeThreadLocal));
+    for (size_t c = 0; c < chunks.size(); c++) {
+        if (chunks[c].n < 0) { chunks[c].n = -chunks[c].n; continue; }
+        const int rc = enqueue_chunk(es, chunks[c], shapes[c]);
+        if (rc) {
+            if (capture) { hipGraph_t g = nullptr; (void)hipStreamEndCapture(es, &g); if (g) (void)hipGraphDestroy(g); }
+            return rc;
+        }
+    }
+    if (capture) {
+        hipGraph_t g = nullptr;
+        NV2_HIP(hipStreamEndCapture(es, &g));
+        hipGraphExec_t x = nullptr;
+        if (hipGraphInstantiateWithFlags(&x, g, 0) != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            set_error("syevb: hipGraphInstantiate failed");
+            return -1;
+        }
+        plan->graph = g;
+        plan->exec = x;
+        NV2_HIP(hipGraphLaunch(x, st));
+        plan->launches++;
+    }
+    if (plan) {
+        plan->chunks = chunks;
+        plan->shapes = shapes;
+        plan->calls++;
     }
     return 0;
 }

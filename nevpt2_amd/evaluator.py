@@ -78,6 +78,14 @@ class Evaluator:
         L.check(L.lib().nv2_ctx_get(self._h, b"PC-timings", L.ptr(a), a.size), "nv2_ctx_get(PC-timings)")
         return a.reshape(self.M, 4)
 
+    def pc_graphs(self):
+        """hipGraph launches so far of the batched eigensolver call sites (worker-0 metrics, worker-0 K',
+        Fock blocks): the first call with a problem list tunes and solves, the second captures, later
+        ones replay."""
+        a = np.zeros(3)
+        L.check(L.lib().nv2_ctx_get(self._h, b"PC-graphs", L.ptr(a), 3), "nv2_ctx_get(PC-graphs)")
+        return a
+
     def set_option(self, name, value):
         """Run options: "no4rdm" (the reference's .NO4RDM) skips the A-D matrices and the (+-1)' classes;
         "natorb" (.NATORB) also forms each state's first-order density matrix (see natorb())."""

@@ -4,10 +4,14 @@ spectra on both sides of the LDS/global switch (order 120), exact degeneracies, 
 metric (a cluster of zero eigenvalues), a decoupled padding block, tight clusters (one of 100
 vectors, and the spectrum of a metric close to 0.5 I, which is one cluster), graded
 eigenvalues, a diagonal and a zero matrix, matrices of norm 1e-200 and 1e150, and order 1."""
+import os
+
 import numpy as np
 import pytest
 
 from nevpt2_amd import _lib as L
+from nevpt2_amd import synth
+from nevpt2_amd.evaluator import Evaluator
 
 
 def _sym(rng, n):
@@ -67,3 +71,34 @@ def test_syev_batched_matches_lapack(gpu):
 def test_syev_batched_rejects_orders_above_its_limit(gpu):
     with pytest.raises(L.NV2Error, match="outside"):
         L.syev_batched([np.eye(1025)])
+
+
+@pytest.mark.gpu
+def test_graph_replays_match_direct_launches(gpu):
+    """Inside an evaluation the batched solves run through plans: the first call with a problem list
+    picks the launch shapes by timing and solves, the second captures its launches in a hipGraph, later
+    calls replay it.  Every replay, with other evaluations and eager work between them, gives the same
+    class table bit for bit as direct launches (NV2_SYEVB_GRAPH=0) on the same inputs."""
+    print("[syevb graphs] nact=6 ncore=3 nvirt=6 seed=4 (and a CAS(5,5) context between replays)", flush=True)
+    P = synth.Problem(6, 3, 6, seed=4, nele=6)
+    Q = synth.Problem(5, 2, 4, seed=7, nele=4)
+    ev, other = Evaluator.for_problem(P), Evaluator.for_problem(Q)
+    try:
+        os.environ["NV2_SYEVB_GRAPH"] = "0"
+        ref = ev.run()
+        ref_other = other.run()
+        os.environ.pop("NV2_SYEVB_GRAPH")
+        outs, launches = [], []
+        for _ in range(5):
+            outs.append(ev.run())
+            assert np.array_equal(other.run(), ref_other)
+            launches.append(ev.pc_graphs().copy())
+    finally:
+        os.environ.pop("NV2_SYEVB_GRAPH", None)
+        ev.close()
+        other.close()
+    print("  graph launches after each run (metrics, K', Fock):", [l.tolist() for l in launches])
+    for k, o in enumerate(outs):
+        assert np.array_equal(o, ref), k
+    # the K' and Fock call sites replay from the third run with graphs on (the direct run was the first)
+    assert launches[-1][1] >= 3 and launches[-1][2] >= 3, launches
