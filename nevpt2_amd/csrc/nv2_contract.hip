@@ -174,7 +174,7 @@ __device__ __forceinline__ void chunk_a(const T *ech, const T *b1s, const T *b2s
 template <typename T, int NN, typename MM = Mfma<T>>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_e4_pass_a(const T *__restrict__ E4, const double *__restrict__ B1,
                                                    const double *__restrict__ B2, const double *__restrict__ B3,
-                                                   int M, int s, double *C1, double *C2, double *C3) {
+                                                   int M, int s, double *C1, double *C2, double *C3, int g0) {
     typedef ChunkLayout<NN> CL;
     constexpr int N2 = NN * NN, N3 = N2 * NN;
     constexpr int KA = (N2 + 3) / 4;          // K steps of C1 / C2 per chunk (K = n^2)
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     constexpr int B3S = 4 * KC * NN + 16;
     typedef typename MM::V V;
     __shared__ T ech[CL::SIZE], b1s[BS1], b2s[BS2], b3s[B3S];
-    const int g = blockIdx.x;
+    const int g = g0 + blockIdx.x;
     const int bp = g % NN, ap = (g / NN) % NN, cp = g / N2;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: scalar branches
     const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // x7, the tiles spread over the four wavefronts
 template <typename T, int NN, typename MM = Mfma<T>>
 __global__ __launch_bounds__(256) void k_e4_pass_b(const T *__restrict__ E4, const double *__restrict__ B3, int M,
-                                                   int s, double *C4) {
+                                                   int s, double *C4, int g0) {
     typedef ChunkLayout<NN> CL;
     constexpr int N2 = NN * NN, N3 = N2 * NN;
     constexpr int KC = (NN + 3) / 4;
@@ -322,7 +322,7 @@ __global__ __launch_bounds__(256) void k_e4_pass_b(const T *__restrict__ E4, con
     typedef typename MM::V V;
     __shared__ T ech[CL::SIZE];
     __shared__ T b3s[B3S];
-    const int g = blockIdx.x;
+    const int g = g0 + blockIdx.x;
     const int bp = g % NN, ap = (g / NN) % NN, cp = g / N2;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: scalar branches
     const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
@@ -402,9 +402,14 @@ void contract_plan_destroy(ContractPlan *p) {
 template <typename T, int NN, typename MM>
 static int run_passes(ContractPlan *p, const T *E4, int s, double **outs, hipStream_t st) {
     const unsigned nwg = NN * NN * NN;
-    hipLaunchKernelGGL((k_e4_pass_a<T, NN, MM>), dim3(nwg), dim3(256), 0, st, E4, p->Bk[0], p->Bk[1], p->Bk[2], p->M,
-                       s, outs[0], outs[1], outs[2]);
-    hipLaunchKernelGGL((k_e4_pass_b<T, NN, MM>), dim3(nwg), dim3(256), 0, st, E4, p->Bk[2], p->M, s, outs[3]);
+    const int nch = stage1_chunks();
+    const unsigned chunk = ((nwg + nch - 1) / nch + 7) / 8 * 8;
+    for (unsigned g0 = 0; g0 < nwg; g0 += chunk)
+        hipLaunchKernelGGL((k_e4_pass_a<T, NN, MM>), dim3(nwg - g0 < chunk ? nwg - g0 : chunk), dim3(256), 0, st, E4,
+                           p->Bk[0], p->Bk[1], p->Bk[2], p->M, s, outs[0], outs[1], outs[2], (int)g0);
+    for (unsigned g0 = 0; g0 < nwg; g0 += chunk)
+        hipLaunchKernelGGL((k_e4_pass_b<T, NN, MM>), dim3(nwg - g0 < chunk ? nwg - g0 : chunk), dim3(256), 0, st, E4,
+                           p->Bk[2], p->M, s, outs[3], (int)g0);
     NV2_LAUNCH_CHECK();
     return 0;
 }

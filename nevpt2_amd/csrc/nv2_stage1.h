@@ -38,6 +38,8 @@ int launch_koop_small(const Rdm &R, int ncore, const double *aj, const double *a
                       Stage1Out &o, hipStream_t st);
 // d3t: n^6 doubles of scratch (the 3-RDM of state s transposed for the delta terms)
 int launch_build_e4(const Rdm &R, int s, void *E4, int prec, double *d3t, hipStream_t st);
+// launches each contraction pass is split into (NV2_PASS_CHUNKS)
+int stage1_chunks();
 // lower-order part of bamat for state s as GEMMs + one combine kernel (nv2_bamat_lo.hip);
 // work holds bamat_lo_workspace(n) doubles
 i64 bamat_lo_workspace(int n);

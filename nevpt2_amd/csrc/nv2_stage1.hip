@@ -769,11 +769,11 @@ __device__ __forceinline__ unsigned tri2(unsigned x) { return x * (x + 1) / 2u; 
 #define E4B_MS 4
 template <typename T, int NN>
 __global__ __launch_bounds__(256) void k_build_e4(Rdm R, int s, const double *__restrict__ D3T, T *__restrict__ E4,
-                                                  unsigned G, unsigned NMS) {
+                                                  unsigned G, unsigned NMS, unsigned B0) {
     constexpr int NP = NN * NN;
     __shared__ double L[E4B_MS][NP];
     __shared__ int X[E4B_MS][6][4];   // pairs at positions 0..2 of arrangement k, and whether it is written
-    const unsigned G8 = G / 8, i0 = blockIdx.x;
+    const unsigned G8 = G / 8, i0 = B0 + blockIdx.x;
     const unsigned blk = i0 < 8 * G8 ? (i0 % 8) * G8 + i0 / 8 : i0;
     if (threadIdx.x < E4B_MS) {
         const int m = threadIdx.x;
@@ -927,15 +927,30 @@ int launch_koop_small(const Rdm &R, int ncore, const double *aj, const double *a
     return launch_koop_rest(R, atwo, w, o, st);
 }
 
+// Launches the big stage-1 kernels are split into (NV2_E4_CHUNKS for the E4 build, NV2_PASS_CHUNKS for
+// each contraction pass; default 1): between two launches the dispatcher serves the other streams'
+// pending kernels
+static int env_chunks(const char *name) {
+    const char *e = getenv(name);
+    const int c = e ? atoi(e) : 1;
+    return c < 1 ? 1 : (c > 64 ? 64 : c);
+}
+int stage1_chunks() { return env_chunks("NV2_PASS_CHUNKS"); }
+static int e4_chunks() { return env_chunks("NV2_E4_CHUNKS"); }
 template <typename T>
 static int build_e4_n(const Rdm &R, int s, T *E4, double *d3t, hipStream_t st) {
+    const int nch = e4_chunks();
     switch (R.n) {
 #define CASE(k)                                                                                                 \
     case k: {                                                                                                   \
         constexpr unsigned np = k * k, nms = np * (np + 1) * (np + 2) / 6, g = (nms + E4B_MS - 1) / E4B_MS;    \
+        const unsigned chunk = ((g + nch - 1) / nch + 7) / 8 * 8;   /* whole XCD rounds per launch */       \
         hipLaunchKernelGGL((k_e4_d3t<k>), dim3((k * k * k * k * k * k + 255) / 256), dim3(256), 0, st, R, s, d3t);    \
         NV2_LAUNCH_CHECK();                                                                                     \
-        hipLaunchKernelGGL((k_build_e4<T, k>), dim3(g), dim3(256), 0, st, R, s, d3t, E4, g, nms);             \
+        for (unsigned b0 = 0; b0 < g; b0 += chunk) {                                                            \
+            const unsigned gc = g - b0 < chunk ? g - b0 : chunk;                                               \
+            hipLaunchKernelGGL((k_build_e4<T, k>), dim3(gc), dim3(256), 0, st, R, s, d3t, E4, g, nms, b0);     \
+        }                                                                                                       \
         break;                                                                                                  \
     }
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12)

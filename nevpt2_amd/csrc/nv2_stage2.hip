@@ -2321,6 +2321,7 @@ struct Pm1pEarly {
     std::condition_variable cv;
     bool k_ready = false, abort = false;
     bool k_filled = false;         // K filled and signalled ahead of stage2_state (stage2_pm1p_kready)
+    bool kred_done = false;        // the K' reduction has finished on the device (or the chain has ended)
     // Single-state runs on the band form: the PC sums go out on the chain's stream as soon as the
     // right-hand sides exist (x_ready, event evX), alongside the other classes on the main stream.
     bool sums_early = false, x_ready = false, sums_done = false;
@@ -2415,6 +2416,11 @@ static int pm1p_worker(Stage2 *p, Pm1pEarly *E) {
     TRYS(pc_kprime_launch(p, j, E->thr));
     NV2_HIP(hipStreamSynchronize(p->ws[j.w]));
     E->t_end = std::chrono::steady_clock::now();
+    {
+        std::lock_guard<std::mutex> lk(E->mu);
+        E->kred_done = true;
+    }
+    E->cv.notify_all();
     if (E->sums_early && j.band) {
         {
             std::unique_lock<std::mutex> lk(E->mu);
@@ -2514,6 +2520,7 @@ int stage2_pm1p_begin(Stage2 *p, int s, double thr) {
     E->abort = false;
     E->rc = 0;
     E->err[0] = 0;
+    E->kred_done = false;
     E->active = true;
     E->th = std::thread([p, E]() {
         try {
@@ -2526,6 +2533,11 @@ int stage2_pm1p_begin(Stage2 *p, int s, double thr) {
             E->rc = -1;
         }
         if (E->rc) nv2_last_error(E->err, sizeof(E->err));
+        {
+            std::lock_guard<std::mutex> lk(E->mu);
+            E->kred_done = true;       // also on every early exit: nothing may wait for it forever
+        }
+        E->cv.notify_all();
     });
     return 0;
 }
@@ -2685,6 +2697,13 @@ static double *qd_fill(Stage2 *p, const Sl &S, int kind, int lt, i64 total, cons
 static bool stage2_timing() {
     static const bool on = getenv("NV2_STAGE2_TIMING") && atoi(getenv("NV2_STAGE2_TIMING")) > 0;
     return on;
+}
+// The SC sums wait until the (+-1)' K' reduction has finished: they otherwise share the device with
+// its chain of small launches and stretch it (CAS(14,14): reduction done 50 instead of 60 ms into
+// stage 2, evaluation 83.3 against 84.4 ms).  NV2_HOLD_SC=0 starts them as soon as the integrals exist.
+static bool hold_sc() {
+    const char *e = getenv("NV2_HOLD_SC");
+    return !(e && atoi(e) == 0);
 }
 static double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -3005,6 +3024,12 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         TRYS(gemm(p, true, false, nc, n, L * nc, -1.0, Tcc, (i64)L * nc, Tca, (i64)L * nc, 1.0, hp1p, nc));
     }
     // ================================================================ SC sums that need no PC solve (main stream)
+    // the SC sums wait until the (+-1)' K' reduction has finished (hold_sc)
+    if (hold_sc() && has4 && p->early && p->early->active) {
+        Pm1pEarly *E = p->early;
+        std::unique_lock<std::mutex> lk(E->mu);
+        E->cv.wait(lk, [E] { return E->kred_done || E->abort; });
+    }
     // class (0): SC == PC
     if (nc && nv) {
         dim3 g(nb(nv, 128), (unsigned)p->np_c);
