@@ -78,23 +78,37 @@ __global__ void k_band_zero_flags(int *f, int nb) {
     if (threadIdx.x < nb) f[threadIdx.x] = 0;
 }
 
-// R^T R = G (BAND_B x BAND_B) and R^-1, problem b = blockIdx.x, by one wavefront: lane c keeps column
-// c in registers and row j travels through LDS at step j.  G is the sum of the np partial products at
-// Gp + b * sW (BAND_B^2 apart; symmetric, so lane c reads row c, coalesced).  R and R^-1 are stored by
-// rows (element (i, c) at i * BAND_B + c).  flag[b] is set by a pivot that is not positive and, when
-// dev > 0, by an entry of G - I above dev: the second CholeskyQR pass then starts from a Q1 too far
-// from orthogonal for Q2 to be orthogonal to rounding.
-__global__ __launch_bounds__(64) void k_bchol(const double *Gp, i64 sW, int np, double *R, double *Ri, int *flag,
-                                              double dev) {
+// R^T R = G (BAND_B x BAND_B) and R^-1, problem b = blockIdx.x.  G is the sum of the np partial
+// products at Gp + b * sW (BAND_B^2 apart): the four wavefronts of the workgroup add them into LDS
+// (consecutive threads on consecutive entries, every partial's loads in flight together), then one
+// wavefront factors: lane c keeps column c in registers and row j travels through LDS at step j (one
+// wavefront, in-order LDS: no workgroup barrier).  R and R^-1 are stored by rows (element (i, c) at
+// i * BAND_B + c).  flag[b] is set by a pivot that is not positive and, when dev > 0, by an entry of
+// G - I above dev: the second CholeskyQR pass then starts from a Q1 too far from orthogonal for Q2 to
+// be orthogonal to rounding.
+#define WAVE_LDS_SYNC() __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory")
+#define BCH_T 256
+__global__ __launch_bounds__(BCH_T) void k_bchol(const double *Gp, i64 sW, int np, double *R, double *Ri, int *flag,
+                                                 double dev) {
+    __shared__ double gs[BB * BB];
     __shared__ double rb[BB];
-    const int b = blockIdx.x, c = threadIdx.x;
+    const int b = blockIdx.x, t = threadIdx.x, c = t & 63;
     const double *Gb = Gp + b * sW;
+    {
+        double g[BB * BB / BCH_T];
+#pragma unroll
+        for (int u = 0; u < BB * BB / BCH_T; u++) g[u] = 0.0;
+        for (int q = 0; q < np; q++)
+#pragma unroll
+            for (int u = 0; u < BB * BB / BCH_T; u++) g[u] += Gb[(i64)q * BB * BB + t + BCH_T * u];
+#pragma unroll
+        for (int u = 0; u < BB * BB / BCH_T; u++) gs[t + BCH_T * u] = g[u];
+    }
+    __syncthreads();
+    if (t >= 64) return;
     double a[BB];
 #pragma unroll
-    for (int r = 0; r < BB; r++) a[r] = 0.0;
-    for (int q = 0; q < np; q++)
-#pragma unroll
-        for (int r = 0; r < BB; r++) a[r] += Gb[(i64)q * BB * BB + c + BB * r];
+    for (int r = 0; r < BB; r++) a[r] = gs[c + BB * r];   // G[r][c] (symmetric: row c)
     bool off = false;
     if (dev > 0.0)
 #pragma unroll
@@ -103,7 +117,7 @@ __global__ __launch_bounds__(64) void k_bchol(const double *Gp, i64 sW, int np, 
 #pragma unroll
     for (int j = 0; j < BB; j++) {          // right-looking: a[r] = G[r][c] for j <= r <= c
         rb[c] = a[j];
-        __syncthreads();
+        WAVE_LDS_SYNC();
         const double d = rb[j];
         bad |= !(d > 0.0);                  // uniform
         const double u = sqrt(d > 0.0 ? d : 1.0), inv = 1.0 / u;
@@ -111,7 +125,7 @@ __global__ __launch_bounds__(64) void k_bchol(const double *Gp, i64 sW, int np, 
         a[j] = rjc;
 #pragma unroll
         for (int r = j + 1; r < BB; r++) a[r] -= (rb[r] * inv) * rjc;
-        __syncthreads();
+        WAVE_LDS_SYNC();
     }
     double *Rb = R + b * sW, *Rib = Ri + b * sW;
 #pragma unroll
@@ -119,12 +133,12 @@ __global__ __launch_bounds__(64) void k_bchol(const double *Gp, i64 sW, int np, 
 #pragma unroll
     for (int i = BB - 1; i >= 0; i--) {     // X = R^-1 in place, rows from the bottom: a[k] = X[k][c], k > i
         rb[c] = a[i];
-        __syncthreads();
+        WAVE_LDS_SYNC();
         double s = c == i ? 1.0 : 0.0;
 #pragma unroll
         for (int k = i + 1; k < BB; k++) s -= rb[k] * a[k];
         a[i] = s / rb[i];
-        __syncthreads();
+        WAVE_LDS_SYNC();
     }
 #pragma unroll
     for (int i = 0; i < BB; i++) Rib[i * BB + c] = a[i];
@@ -211,10 +225,9 @@ __global__ __launch_bounds__(BWG) void k_tx(const double *Gp, int np, const doub
 // (lane c of a wavefront keeps column c of the matrix at hand in registers; rows and columns that
 // other lanes need travel through LDS).  Wavefront 0 factors Q2_top - S = L U without pivoting and
 // inverts U (stored by rows); wavefront 1 meanwhile forms R~ = S R2 R1 into the panel's top block of A
-// (zeros below it), then, once L and U are there, Y_top = L and T = -U S L^-T.  R1, R2 come stored by
+// (the rows below it are outside the band and never read again), then, once L and U are there, Y_top = L and T = -U S L^-T.  R1, R2 come stored by
 // rows.  The phases of one wavefront order their LDS traffic by program order (one wavefront, in-order
 // LDS); the two meet at one workgroup barrier.
-#define WAVE_LDS_SYNC() __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory")
 __global__ __launch_bounds__(128) void k_brecon(double *A, int N, i64 sA, int p0, int r0, const double *Q2,
                                                 const double *R1, const double *R2, double *Y, double *T,
                                                 double *Ui, i64 sW) {
@@ -264,9 +277,7 @@ __global__ __launch_bounds__(128) void k_brecon(double *A, int N, i64 sA, int p0
             Ab[(r0 + r) + (i64)N * (p0 + c)] = r <= c ? sr * v : 0.0;
             BAND_FENCE();
         }
-        const int mb = N - r0 - BB;
-        for (int cc = 0; cc < BB; cc++)
-            for (int r = c; r < mb; r += 64) Ab[(r0 + BB + r) + (i64)N * (p0 + cc)] = 0.0;
+        // rows r0 + BB .. N-1 of the panel's columns lie outside the band: nothing reads them again
     }
     __syncthreads();                     // L, U and S in LDS
     double *Yb = Y + b * sW, *Tb = T + b * sW, *Uib = Ui + b * sW;
@@ -401,6 +412,96 @@ __global__ void k_band_extract(const double *A, int N, i64 sA, double *Br, i64 s
     Br[b * sW + t] = v;
 }
 
+// z = Q_b^T y for column sets of right-hand sides (one launch for every set; QT_C columns per
+// workgroup, y read and written in place in global memory).  Q_b = H_0 H_1 ... H_{np-1} diag(I, Qm)
+// from band_reduce, so Q_b^T y applies H_0^T first: for panel k (rows r0 = BB (k+1) .. N-1, m of them)
+// w = Y_k^T y[r0:], w' = T_k^T w, y[r0:] -= Y_k w'; then y[r0:] <- Qm^T y[r0:] for the last block.
+// Thread (i = t % 64, c = t / 64) owns w[i][c]; the m x BB reflector block streams through LDS in chunks
+// of QT_R rows, once for the product and once for the update.  Every workgroup of a set walks the
+// same panels in the same order, so a chunk is shared through L2.
+#define QT_C 4
+#define QT_R 64
+__global__ __launch_bounds__(256) void k_band_qt(BandQtArgs a) {
+    __shared__ double yk[QT_R][BB + 1];
+    __shared__ double tk[BB][BB + 1];
+    __shared__ double ys[QT_R][QT_C];
+    __shared__ double wv[BB][QT_C], wv2[BB][QT_C];
+    int si = 0;
+    while (si + 1 < a.nset && (int)blockIdx.x >= a.s[si + 1].wg0) si++;
+    const BandQtSet S = a.s[si];
+    const int N = a.N, t = threadIdx.x, i = t & 63, c = t >> 6;
+    const int c0 = ((int)blockIdx.x - S.wg0) * QT_C;
+    const int nc = S.ncols - c0 < QT_C ? S.ncols - c0 : QT_C;
+    double *Y = S.Y + S.ld * c0;
+    for (int k = 0; k < a.npan; k++) {
+        const int r0 = BB * (k + 1), m = N - r0;
+        const double *Yk = S.w + a.offY + (i64)k * BB * N, *Tk = S.w + a.offT + (i64)k * BB * BB;
+        double acc = 0.0;
+        for (int rc = 0; rc < m; rc += QT_R) {
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < QT_R * BB / 256; u++) {
+                const int e = t + 256 * u, rr = e % QT_R, ii = e / QT_R;
+                yk[rr][ii] = rc + rr < m ? Yk[(rc + rr) + (i64)N * ii] : 0.0;
+            }
+            {
+                const int rr = t % QT_R, cc = t / QT_R;
+                ys[rr][cc] = rc + rr < m && cc < nc ? Y[(r0 + rc + rr) + S.ld * cc] : 0.0;
+            }
+            __syncthreads();
+#pragma unroll 16
+            for (int rr = 0; rr < QT_R; rr++) acc += yk[rr][i] * ys[rr][c];
+        }
+#pragma unroll
+        for (int u = 0; u < BB * BB / 256; u++) {
+            const int e = t + 256 * u, jj = e % BB, ii = e / BB;
+            tk[jj][ii] = Tk[jj + BB * ii];
+        }
+        wv[i][c] = acc;
+        __syncthreads();
+        double w2 = 0.0;
+#pragma unroll 16
+        for (int jj = 0; jj < BB; jj++) w2 += tk[jj][i] * wv[jj][c];
+        wv2[i][c] = w2;
+        for (int rc = 0; rc < m; rc += QT_R) {
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < QT_R * BB / 256; u++) {
+                const int e = t + 256 * u, rr = e % QT_R, ii = e / QT_R;
+                yk[rr][ii] = rc + rr < m ? Yk[(rc + rr) + (i64)N * ii] : 0.0;
+            }
+            __syncthreads();
+            const int rr = t % QT_R, cc = t / QT_R;
+            if (rc + rr < m && cc < nc) {
+                double v = 0.0;
+#pragma unroll 16
+                for (int ii = 0; ii < BB; ii++) v += yk[rr][ii] * wv2[ii][cc];
+                Y[(r0 + rc + rr) + S.ld * cc] -= v;
+            }
+        }
+        __syncthreads();
+    }
+    const int r0 = BB * (a.npan + 1), m = N - r0;
+    if (m >= 1) {
+        const double *Ql = S.w + a.offQl;
+#pragma unroll
+        for (int u = 0; u < BB * BB / 256; u++) {
+            const int e = t + 256 * u, jj = e % BB, ii = e / BB;
+            tk[jj][ii] = jj < m && ii < m ? Ql[jj + BB * ii] : 0.0;
+        }
+        {
+            const int rr = t % QT_R, cc = t / QT_R;
+            ys[rr][cc] = rr < m && cc < nc ? Y[(r0 + rr) + S.ld * cc] : 0.0;
+        }
+        __syncthreads();
+        if (i < m && c < nc) {
+            double v = 0.0;
+            for (int jj = 0; jj < m; jj++) v += tk[jj][i] * ys[jj][c];
+            Y[(r0 + i) + S.ld * c] = v;
+        }
+    }
+}
+
 #define RBB(call) do { if ((call) != rocblas_status_success) { set_error("rocBLAS failed: %s", #call); return -1; } } while (0)
 
 static inline unsigned nbl(i64 tot, int bs) { return (unsigned)((tot + bs - 1) / bs); }
@@ -426,10 +527,10 @@ int band_reduce(rocblas_handle h, hipStream_t st, int N, double *A, i64 sA, int 
         const int np = (m + KC - 1) / KC;
         // CholeskyQR2 of P
         hipLaunchKernelGGL(k_atb_parts, dim3(np, nb), dim3(BWG), 0, st, P, sA, P, sA, N, m, Gp, sW);
-        hipLaunchKernelGGL(k_bchol, dim3(nb), dim3(64), 0, st, Gp, sW, np, R1, R1i, dflag, 0.0);
+        hipLaunchKernelGGL(k_bchol, dim3(nb), dim3(BCH_T), 0, st, Gp, sW, np, R1, R1i, dflag, 0.0);
         RBB(rocblas_dgemm_strided_batched(h, Nn, Tt, m, BB, BB, &one, P, N, sA, R1i, BB, sW, &zero, Q1, N, sW, nb));
         hipLaunchKernelGGL(k_atb_parts, dim3(np, nb), dim3(BWG), 0, st, Q1, sW, Q1, sW, N, m, Gp, sW);
-        hipLaunchKernelGGL(k_bchol, dim3(nb), dim3(64), 0, st, Gp, sW, np, R2, R2i, dflag, 0.1);
+        hipLaunchKernelGGL(k_bchol, dim3(nb), dim3(BCH_T), 0, st, Gp, sW, np, R2, R2i, dflag, 0.1);
         RBB(rocblas_dgemm_strided_batched(h, Nn, Tt, m, BB, BB, &one, Q1, N, sW, R2i, BB, sW, &zero, Q2, N, sW, nb));
         // reflectors: Y_top, T, U^-1, and S R into the panel
         hipLaunchKernelGGL(k_brecon, dim3(nb), dim3(128), 0, st, A, N, sA, p0, r0, Q2, R1, R2, Yk, Tk, Ui, sW);
@@ -459,32 +560,32 @@ int band_reduce(rocblas_handle h, hipStream_t st, int N, double *A, i64 sA, int 
     return 0;
 }
 
-// G <- G Q for problem b after band_reduce (G: N x N, ld N, overwritten), Q = H_0 H_1 ... the
-// reduction's reflectors: G <- G H_k is G[:, r0:] -= (G[:, r0:] Y_k) T_k Y_k^T, in panel order, then
-// the last block's explicit Qm.  With G = caux, G^T x = Q^T caux^T x is the right-hand side of the band
-// solves in one GEMM.
-int band_apply_right(rocblas_handle h, hipStream_t st, int N, int nb, int b, double *work, double *G) {
-    (void)nb;
-    RBB(rocblas_set_stream(h, st));
+// z = Q_b^T y in place for up to BAND_QT_SETS column sets (set s: problem b_s's Q, ncols_s columns of
+// order N at Y_s, leading dimension ld_s), one launch
+int band_apply_qt(hipStream_t st, int N, const double *work, int nset, const int *b, double *const *Y,
+                  const i64 *ld, const int *ncols) {
+    if (nset < 0 || nset > BAND_QT_SETS) { set_error("band_apply_qt: %d column sets", nset); return -2; }
     const BandLayout L = band_layout(N);
-    double *w0 = work + b * L.size;
-    double *W = w0 + L.W, *W2 = w0 + L.W2, *Tmp = w0 + L.Q1;
-    const double one = 1.0, zero = 0.0, mone = -1.0;
-    const rocblas_operation Nn = rocblas_operation_none, Tt = rocblas_operation_transpose;
-    for (int k = 0; k < L.npan; k++) {
-        const int r0 = BB * (k + 1), m = N - r0;
-        const double *Yk = w0 + L.Y + (i64)k * BB * N, *Tk = w0 + L.T + (i64)k * BB * BB;
-        double *Gr = G + (i64)N * r0;
-        RBB(rocblas_dgemm(h, Nn, Nn, N, BB, m, &one, Gr, N, Yk, N, &zero, W, N));
-        RBB(rocblas_dgemm(h, Nn, Nn, N, BB, BB, &one, W, N, Tk, BB, &zero, W2, N));
-        RBB(rocblas_dgemm(h, Nn, Tt, N, m, BB, &mone, W2, N, Yk, N, &one, Gr, N));
+    BandQtArgs a{};
+    a.N = N;
+    a.npan = L.npan;
+    a.offY = L.Y;
+    a.offT = L.T;
+    a.offQl = L.Ql;
+    int wg = 0;
+    for (int q = 0; q < nset; q++) {
+        if (ncols[q] <= 0) continue;
+        BandQtSet &S = a.s[a.nset++];
+        S.w = work + b[q] * L.size;
+        S.Y = Y[q];
+        S.ld = ld[q];
+        S.ncols = ncols[q];
+        S.wg0 = wg;
+        wg += (ncols[q] + QT_C - 1) / QT_C;
     }
-    const int r0 = BB * (L.npan + 1), m = N - r0;
-    if (m >= 1) {
-        RBB(rocblas_dgemm(h, Nn, Nn, N, m, m, &one, G + (i64)N * r0, N, w0 + L.Ql, BB, &zero, Tmp, N));
-        NV2_HIP(hipMemcpy2DAsync(G + (i64)N * r0, sizeof(double) * N, Tmp, sizeof(double) * N, sizeof(double) * N, m,
-                                 hipMemcpyDeviceToDevice, st));
-    }
+    if (!wg) return 0;
+    hipLaunchKernelGGL(k_band_qt, dim3(wg), dim3(256), 0, st, a);
+    NV2_LAUNCH_CHECK();
     return 0;
 }
 

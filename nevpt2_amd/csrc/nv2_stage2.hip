@@ -2038,6 +2038,7 @@ static int pc_metric_launch(Stage2 *p, PcJob &j, double thr) {
 }
 static int pc_kprime_form(Stage2 *p, PcJob &j, double thr);
 // after the worker stream has finished: threshold the metric spectra, launch K' and its solves
+static void early_kred_signal(Stage2 *p, const PcJob &j);
 static int pc_kprime_launch(Stage2 *p, PcJob &j, double thr) {
     const int N = j.N;
     if (j.chol) {                        // the factorisations of pc_metric_launch have finished
@@ -2103,8 +2104,8 @@ static int pc_kprime_launch(Stage2 *p, PcJob &j, double thr) {
                 if (stage2_timing()) fprintf(stderr, "[K' %s N=%d] band reduction %s\n", j.tag.c_str(), N,
                                              bok ? "done" : "failed (CholeskyQR2): tridiagonal reduction");
                 if (bok) {
-                    // caux <- caux Q: the sums' projections G^T x then give Q^T caux^T x in one GEMM
-                    for (int b = 0; b < j.nb; b++) TRYS(band_apply_right(h, st, N, j.nb, b, j.bandw, j.caux + b * NN));
+                    early_kred_signal(p, j);     // the device-heavy part of the chain is over
+                    // caux stays U^-1: the sums project y = caux^T x and apply Q^T to y (band_apply_qt)
                     j.band = true;
                     p->kr_recorded = true;
                     p->kr_N = N;
@@ -2368,13 +2369,19 @@ static int pm1p_band_sums(Stage2 *p, Pm1pEarly *E) {
     const int N = jb.nmax;
     const i64 NN = jb.NN();
     BandCls bc[2] = {};
+    int qb[BAND_QT_SETS], qn[BAND_QT_SETS], nq = 0;
+    double *qy[BAND_QT_SETS];
+    i64 qld[BAND_QT_SETS];
     for (int plus = 0; plus < 2; plus++) {
         const int next = E->next[plus], bb = plus ? 0 : 1;
         if (!next || !E->X[plus]) continue;
-        // z = Q^T caux^T x in one GEMM (caux holds caux Q on the band form)
+        // y = caux^T x, then z = Q^T y for every class in one launch below
         TRYS(gemm_on(h, st, true, false, N, next, N, 1.0, jb.caux + bb * NN, N, E->X[plus], N, 0.0, E->Ypc[plus], N));
-        if (E->rho[plus])
+        qb[nq] = bb; qy[nq] = E->Ypc[plus]; qld[nq] = N; qn[nq++] = next;
+        if (E->rho[plus]) {
             TRYS(gemm_on(h, st, true, false, N, E->nact, N, 1.0, jb.caux + bb * NN, N, E->Tr[plus], N, 0.0, E->Vr[plus], N));
+            qb[nq] = bb; qy[nq] = E->Vr[plus]; qld[nq] = N; qn[nq++] = E->nact;
+        }
         BandCls &c = bc[plus];
         c.Br = band_rows(N, jb.nb, bb, jb.bandw);
         c.Z = E->Ypc[plus];
@@ -2384,6 +2391,7 @@ static int pm1p_band_sums(Stage2 *p, Pm1pEarly *E) {
     }
     const int n0 = bc[0].next, n1 = bc[1].next;
     if (!n0 && !n1) return 0;
+    TRYS(band_apply_qt(st, N, jb.bandw, nq, qb, qy, qld, qn));
     bc[0].U = E->Ub;
     bc[0].y = E->yb;
     bc[1].U = E->Ub + (i64)n0 * N * UR;
@@ -2398,6 +2406,16 @@ static int pm1p_band_sums(Stage2 *p, Pm1pEarly *E) {
         if (E->rho[c]) TRYS(band_rho(h, st, c, N, E->nact, bc[c].next, bc[c].y, E->Vr[c], E->rho[c]));
     }
     return 0;
+}
+// Releases the SC sums held for the (+-1)' K' reduction (the job of the chain only)
+static void early_kred_signal(Stage2 *p, const PcJob &j) {
+    Pm1pEarly *E = p->early;
+    if (!E || &j != &E->job) return;
+    {
+        std::lock_guard<std::mutex> lk(E->mu);
+        E->kred_done = true;
+    }
+    E->cv.notify_all();
 }
 static int pm1p_worker(Stage2 *p, Pm1pEarly *E) {
     PcJob &j = E->job;
@@ -2698,9 +2716,10 @@ static bool stage2_timing() {
     static const bool on = getenv("NV2_STAGE2_TIMING") && atoi(getenv("NV2_STAGE2_TIMING")) > 0;
     return on;
 }
-// The SC sums wait until the (+-1)' K' reduction has finished: they otherwise share the device with
-// its chain of small launches and stretch it (CAS(14,14): reduction done 50 instead of 60 ms into
-// stage 2, evaluation 83.3 against 84.4 ms).  NV2_HOLD_SC=0 starts them as soon as the integrals exist.
+// The SC sums other than (+-1)' wait until the (+-1)' K' band reduction has finished: they otherwise
+// share the device with its chain of small launches and stretch it.  The (+-1)' SC sums, which also
+// form the right-hand sides of the chain's PC sums, run before the wait.  NV2_HOLD_SC=0 starts every
+// SC sum as soon as the integrals exist.
 static bool hold_sc() {
     const char *e = getenv("NV2_HOLD_SC");
     return !(e && atoi(e) == 0);
@@ -3023,69 +3042,8 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         TRYS(gemm(p, true, false, 1, (int)Nca, L, 2.0, Jc2, L, Tca, L, 1.0, hp1p, 1));
         TRYS(gemm(p, true, false, nc, n, L * nc, -1.0, Tcc, (i64)L * nc, Tca, (i64)L * nc, 1.0, hp1p, nc));
     }
-    // ================================================================ SC sums that need no PC solve (main stream)
-    // the SC sums wait until the (+-1)' K' reduction has finished (hold_sc)
-    if (hold_sc() && has4 && p->early && p->early->active) {
-        Pm1pEarly *E = p->early;
-        std::unique_lock<std::mutex> lk(E->mu);
-        E->cv.wait(lk, [E] { return E->kred_done || E->abort; });
-    }
-    // class (0): SC == PC
-    if (nc && nv) {
-        dim3 g(nb(nv, 128), (unsigned)p->np_c);
-        double *pt = parts(p, (i64)g.x * g.y);
-        hipLaunchKernelGGL(k_v0, g, dim3(128), 0, st, Icvcv, Ncv, p->pairs_c, nc, nv, ec, ev, pt);
-        NV2_LAUNCH_CHECK();
-        TRYS(reduce_into(p, (i64)g.x * g.y, 0));
-    }
-    // (+2)/(-2) SC
-    for (int mode = 0; mode < 2; mode++) {
-        const int cls = mode == 0 ? 3 : 4;
-        const double *X = mode == 0 ? k_hrdm : taa;
-        const double *K = mode == 0 ? k_eako2 : k_ipko2;
-        const double *I = mode == 0 ? Icaca : Iavav;
-        const i64 ld = mode == 0 ? Nca : Nav;
-        const int next = mode == 0 ? nc : nv;
-        const double *eo = mode == 0 ? ec : ev;
-        const int2 *pr = mode == 0 ? p->pairs_c : p->pairs_v;
-        const i64 npr = mode == 0 ? p->np_c : p->np_v;
-        for (int j = 0; q.M && j < M; j++) {
-            if (j == s) continue;
-            QFILL(xj, mode == 0 ? QK_P2_X : QK_M2_X, j, n4, "qd_pm2x_");
-            QFILL(xsj, mode == 0 ? QK_P2_XS : QK_M2_XS, j, n4, "qd_pm2xs_");
-            q.a[j] = xj; q.b[j] = xsj;
-        }
-        if (next && npr) {
-            double *pt = parts(p, npr);
-            TRYS(qd_prepare(p, q, npr));
-            hipLaunchKernelGGL(k_pm2_sc, dim3((unsigned)npr), dim3(256), 0, st, I, ld, pr, mode, nc, n, X, K, eo, pt, q);
-            NV2_LAUNCH_CHECK();
-            TRYS(reduce_into(p, npr, cls));
-            TRYS(qd_reduce(p, q, npr, cls));
-        }
-    }
-    // (0)' SC
-    GET(X1, "x1", n4); GET(X2, "x2", n4); GET(G1, "g1", n4); GET(G2, "g2", n4);
-    hipLaunchKernelGGL(k_v0p_prep, dim3(nb(n4, 256)), dim3(256), 0, st, S, k_ga, k_gd, X1, X2, G1, G2);
-    NV2_LAUNCH_CHECK();
-    for (int j = 0; q.M && j < M; j++) {
-        if (j == s) continue;
-        QFILL(x1j, QK_0P_X1, j, n4, "qd_0px1_");
-        QFILL(x2j, QK_0P_X2, j, n4, "qd_0px2_");
-        QFILL(dsj, QK_0P_DS, j, n2, "qd_0pds_");
-        q.a[j] = x1j; q.b[j] = x2j; q.c[j] = dsj;
-    }
-    if (nc && nv) {
-        const i64 nbk = (i64)nc * nv;
-        double *pt = parts(p, nbk);
-        TRYS(qd_prepare(p, q, nbk));
-        hipLaunchKernelGGL(k_v0p_sc, dim3((unsigned)nbk), dim3(256), 0, st, Icvaa, Ncv, Icaav, Nca, nc, n, nv, X1,
-                           X2, G1, G2, da, k_gf, h0p, ec, ev, pt, q);
-        NV2_LAUNCH_CHECK();
-        TRYS(reduce_into(p, nbk, 7));
-        TRYS(qd_reduce(p, q, nbk, 7));
-    }
-    // (-1)' and (+1)' SC
+    // ================================================================ (-1)' and (+1)' SC (main stream)
+    // first, and not held: the right-hand sides X of the (+-1)' PC sums are ready when the chain needs them
     double *Vpm[2] = {nullptr, nullptr}, *Hpm[2] = {nullptr, nullptr}, *Q2pm[2] = {nullptr, nullptr};
     double *Xpm[2] = {nullptr, nullptr};
     double *Trm[2] = {nullptr, nullptr}, *Vrm[2] = {nullptr, nullptr};   // .NATORB density vectors, projections
@@ -3178,6 +3136,68 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         E->cv.notify_all();
     }
 
+    // ================================================================ SC sums that need no PC solve (main stream)
+    // the SC sums wait until the (+-1)' K' reduction has finished (hold_sc)
+    if (hold_sc() && has4 && p->early && p->early->active) {
+        Pm1pEarly *E = p->early;
+        std::unique_lock<std::mutex> lk(E->mu);
+        E->cv.wait(lk, [E] { return E->kred_done || E->abort; });
+    }
+    // class (0): SC == PC
+    if (nc && nv) {
+        dim3 g(nb(nv, 128), (unsigned)p->np_c);
+        double *pt = parts(p, (i64)g.x * g.y);
+        hipLaunchKernelGGL(k_v0, g, dim3(128), 0, st, Icvcv, Ncv, p->pairs_c, nc, nv, ec, ev, pt);
+        NV2_LAUNCH_CHECK();
+        TRYS(reduce_into(p, (i64)g.x * g.y, 0));
+    }
+    // (+2)/(-2) SC
+    for (int mode = 0; mode < 2; mode++) {
+        const int cls = mode == 0 ? 3 : 4;
+        const double *X = mode == 0 ? k_hrdm : taa;
+        const double *K = mode == 0 ? k_eako2 : k_ipko2;
+        const double *I = mode == 0 ? Icaca : Iavav;
+        const i64 ld = mode == 0 ? Nca : Nav;
+        const int next = mode == 0 ? nc : nv;
+        const double *eo = mode == 0 ? ec : ev;
+        const int2 *pr = mode == 0 ? p->pairs_c : p->pairs_v;
+        const i64 npr = mode == 0 ? p->np_c : p->np_v;
+        for (int j = 0; q.M && j < M; j++) {
+            if (j == s) continue;
+            QFILL(xj, mode == 0 ? QK_P2_X : QK_M2_X, j, n4, "qd_pm2x_");
+            QFILL(xsj, mode == 0 ? QK_P2_XS : QK_M2_XS, j, n4, "qd_pm2xs_");
+            q.a[j] = xj; q.b[j] = xsj;
+        }
+        if (next && npr) {
+            double *pt = parts(p, npr);
+            TRYS(qd_prepare(p, q, npr));
+            hipLaunchKernelGGL(k_pm2_sc, dim3((unsigned)npr), dim3(256), 0, st, I, ld, pr, mode, nc, n, X, K, eo, pt, q);
+            NV2_LAUNCH_CHECK();
+            TRYS(reduce_into(p, npr, cls));
+            TRYS(qd_reduce(p, q, npr, cls));
+        }
+    }
+    // (0)' SC
+    GET(X1, "x1", n4); GET(X2, "x2", n4); GET(G1, "g1", n4); GET(G2, "g2", n4);
+    hipLaunchKernelGGL(k_v0p_prep, dim3(nb(n4, 256)), dim3(256), 0, st, S, k_ga, k_gd, X1, X2, G1, G2);
+    NV2_LAUNCH_CHECK();
+    for (int j = 0; q.M && j < M; j++) {
+        if (j == s) continue;
+        QFILL(x1j, QK_0P_X1, j, n4, "qd_0px1_");
+        QFILL(x2j, QK_0P_X2, j, n4, "qd_0px2_");
+        QFILL(dsj, QK_0P_DS, j, n2, "qd_0pds_");
+        q.a[j] = x1j; q.b[j] = x2j; q.c[j] = dsj;
+    }
+    if (nc && nv) {
+        const i64 nbk = (i64)nc * nv;
+        double *pt = parts(p, nbk);
+        TRYS(qd_prepare(p, q, nbk));
+        hipLaunchKernelGGL(k_v0p_sc, dim3((unsigned)nbk), dim3(256), 0, st, Icvaa, Ncv, Icaav, Nca, nc, n, nv, X1,
+                           X2, G1, G2, da, k_gf, h0p, ec, ev, pt, q);
+        NV2_LAUNCH_CHECK();
+        TRYS(reduce_into(p, nbk, 7));
+        TRYS(qd_reduce(p, q, nbk, 7));
+    }
     // ================================================================ join worker 0: every PC class but (+-1)'
     // (whose n^3 chain on worker 1 is the longest; its sums come last)
     if (stage2_timing()) {           // the SC sums are done on the device
@@ -3415,7 +3435,7 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         const double *caux = jb.caux + bb * NN, *Kp = jb.Kp + bb * NN;
         const int ncq = 1 + (q.M ? M - 1 : 0);              // right-hand sides: this state, then the QD partners
         GET(Ypc, plus ? "pm1_Ypc1" : "pm1_Ypc0", (i64)nmax * next * ncq);
-        // y = caux^T x (on the Cholesky path caux = U^-1 and nmax = N3; on the band form caux Q, so y = z)
+        // y = caux^T x (on the Cholesky path caux = U^-1 and nmax = N3)
         auto project = [&](const double *x, double *y) -> int {
             return gemm(p, true, false, nmax, next, (int)N3, 1.0, caux, N3, x, N3, 0.0, y, nmax);
         };
@@ -3433,8 +3453,12 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             TRYS(project(xj, yj));
             q.a[j] = yj;
         }
-        if (jb.band) {                   // caux holds caux Q: Ypc is already z; the banded solves follow the loop
+        if (jb.band) {                   // z = Q^T y in place; the banded solves follow the loop
+            int qb[2] = {bb, bb}, qn[2] = {next * ncq, n};
+            double *qy[2] = {Ypc, Vrm[plus]};
+            i64 qld[2] = {nmax, nmax};
             if (Trm[plus]) TRYS(gemm(p, true, false, nmax, n, (int)N3, 1.0, caux, N3, Trm[plus], N3, 0.0, Vrm[plus], nmax));
+            TRYS(band_apply_qt(st, nmax, jb.bandw, Trm[plus] ? 2 : 1, qb, qy, qld, qn));
             BandCls &c = bc[plus];
             c.Br = band_rows(nmax, jb.nb, bb, jb.bandw);
             c.Z = Ypc;
