@@ -21,6 +21,7 @@
 #include <rocblas/rocblas.h>
 
 #include "nv2_band.h"
+#include "nv2_chol64.h"
 
 namespace nv2 {
 
@@ -79,20 +80,18 @@ __global__ void k_band_zero_flags(int *f, int nb) {
 }
 
 // R^T R = G (BAND_B x BAND_B) and R^-1, problem b = blockIdx.x.  G is the sum of the np partial
-// products at Gp + b * sW (BAND_B^2 apart): the four wavefronts of the workgroup add them into LDS
-// (consecutive threads on consecutive entries, every partial's loads in flight together), then one
-// wavefront factors: lane c keeps column c in registers and row j travels through LDS at step j (one
-// wavefront, in-order LDS: no workgroup barrier).  R and R^-1 are stored by rows (element (i, c) at
-// i * BAND_B + c).  flag[b] is set by a pivot that is not positive and, when dev > 0, by an entry of
-// G - I above dev: the second CholeskyQR pass then starts from a Q1 too far from orthogonal for Q2 to
-// be orthogonal to rounding.
+// products at Gp + b * sW (BAND_B^2 apart): the four wavefronts add them into LDS (consecutive threads
+// on consecutive entries, every partial's loads in flight together), then factor it together
+// (chol64_4w).  R and R^-1 are stored by rows (element (i, c) at i * BAND_B + c).  flag[b] is set by a
+// pivot that is not positive and, when dev > 0, by an entry of G - I above dev: the second CholeskyQR
+// pass then starts from a Q1 too far from orthogonal for Q2 to be orthogonal to rounding.
 #define WAVE_LDS_SYNC() __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory")
 #define BCH_T 256
 __global__ __launch_bounds__(BCH_T) void k_bchol(const double *Gp, i64 sW, int np, double *R, double *Ri, int *flag,
                                                  double dev) {
     __shared__ double gs[BB * BB];
-    __shared__ double rb[BB];
-    const int b = blockIdx.x, t = threadIdx.x, c = t & 63;
+    __shared__ Chol64Lds L;
+    const int b = blockIdx.x, t = threadIdx.x, w = t >> 6, c = t & 63;
     const double *Gb = Gp + b * sW;
     {
         double g[BB * BB / BCH_T];
@@ -105,48 +104,25 @@ __global__ __launch_bounds__(BCH_T) void k_bchol(const double *Gp, i64 sW, int n
         for (int u = 0; u < BB * BB / BCH_T; u++) gs[t + BCH_T * u] = g[u];
     }
     __syncthreads();
-    if (t >= 64) return;
-    double a[BB];
-#pragma unroll
-    for (int r = 0; r < BB; r++) a[r] = gs[c + BB * r];   // G[r][c] (symmetric: row c)
+    double a[16], x[16];
     bool off = false;
-    if (dev > 0.0)
 #pragma unroll
-        for (int r = 0; r < BB; r++) off |= !(fabs(a[r] - (r == c ? 1.0 : 0.0)) <= dev);
-    bool bad = __ballot(off) != 0;
-#pragma unroll
-    for (int j = 0; j < BB; j++) {          // right-looking: a[r] = G[r][c] for j <= r <= c
-        rb[c] = a[j];
-        WAVE_LDS_SYNC();
-        const double d = rb[j];
-        bad |= !(d > 0.0);                  // uniform
-        const double u = sqrt(d > 0.0 ? d : 1.0), inv = 1.0 / u;
-        const double rjc = c == j ? u : a[j] * inv;   // R[j][c]
-        a[j] = rjc;
-#pragma unroll
-        for (int r = j + 1; r < BB; r++) a[r] -= (rb[r] * inv) * rjc;
-        WAVE_LDS_SYNC();
+    for (int i = 0; i < 16; i++) {
+        const int r = 16 * w + i;
+        a[i] = gs[c + BB * r];
+        if (dev > 0.0) off |= !(fabs(a[i] - (r == c ? 1.0 : 0.0)) <= dev);
     }
+    const bool offall = __syncthreads_or(off) != 0;
+    const bool bad = chol64_4w(a, x, L) != 0 || offall;
     double *Rb = R + b * sW, *Rib = Ri + b * sW;
 #pragma unroll
-    for (int i = 0; i < BB; i++) Rb[i * BB + c] = i <= c ? a[i] : 0.0;
-#pragma unroll
-    for (int i = BB - 1; i >= 0; i--) {     // X = R^-1 in place, rows from the bottom: a[k] = X[k][c], k > i
-        rb[c] = a[i];
-        WAVE_LDS_SYNC();
-        double s = c == i ? 1.0 : 0.0;
-#pragma unroll
-        for (int k = i + 1; k < BB; k++) s -= rb[k] * a[k];
-        a[i] = s / rb[i];
-        WAVE_LDS_SYNC();
+    for (int i = 0; i < 16; i++) {
+        const int r = 16 * w + i;
+        // R = R^-1 = I keeps the batch finite when the flag sends this problem elsewhere
+        Rb[r * BB + c] = bad ? (r == c ? 1.0 : 0.0) : a[i];
+        Rib[r * BB + c] = bad ? (r == c ? 1.0 : 0.0) : x[i];
     }
-#pragma unroll
-    for (int i = 0; i < BB; i++) Rib[i * BB + c] = a[i];
-    if (bad) {                              // R = R^-1 = I keeps the batch finite; the flag sends this problem elsewhere
-#pragma unroll
-        for (int i = 0; i < BB; i++) { Rb[i * BB + c] = i == c ? 1.0 : 0.0; Rib[i * BB + c] = i == c ? 1.0 : 0.0; }
-        if (c == 0) flag[b] = 1;
-    }
+    if (bad && t == 0) flag[b] = 1;
 }
 // Partial products of A^T B for A, B with BAND_B columns (ld, m rows; problem b = blockIdx.y at A + b*sA,
 // B + b*sB): workgroup c sums rows c*KC .. c*KC+KC-1 into Gp + b*sW + c*BAND_B^2.  rocBLAS gives a
@@ -412,94 +388,167 @@ __global__ void k_band_extract(const double *A, int N, i64 sA, double *Br, i64 s
     Br[b * sW + t] = v;
 }
 
-// z = Q_b^T y for column sets of right-hand sides (one launch for every set; QT_C columns per
-// workgroup, y read and written in place in global memory).  Q_b = H_0 H_1 ... H_{np-1} diag(I, Qm)
-// from band_reduce, so Q_b^T y applies H_0^T first: for panel k (rows r0 = BB (k+1) .. N-1, m of them)
-// w = Y_k^T y[r0:], w' = T_k^T w, y[r0:] -= Y_k w'; then y[r0:] <- Qm^T y[r0:] for the last block.
-// Thread (i = t % 64, c = t / 64) owns w[i][c]; the m x BB reflector block streams through LDS in chunks
-// of QT_R rows, once for the product and once for the update.  Every workgroup of a set walks the
-// same panels in the same order, so a chunk is shared through L2.
-#define QT_C 4
-#define QT_R 64
-__global__ __launch_bounds__(256) void k_band_qt(BandQtArgs a) {
-    __shared__ double yk[QT_R][BB + 1];
-    __shared__ double tk[BB][BB + 1];
-    __shared__ double ys[QT_R][QT_C];
-    __shared__ double wv[BB][QT_C], wv2[BB][QT_C];
-    int si = 0;
-    while (si + 1 < a.nset && (int)blockIdx.x >= a.s[si + 1].wg0) si++;
-    const BandQtSet S = a.s[si];
-    const int N = a.N, t = threadIdx.x, i = t & 63, c = t >> 6;
-    const int c0 = ((int)blockIdx.x - S.wg0) * QT_C;
-    const int nc = S.ncols - c0 < QT_C ? S.ncols - c0 : QT_C;
-    double *Y = S.Y + S.ld * c0;
-    for (int k = 0; k < a.npan; k++) {
-        const int r0 = BB * (k + 1), m = N - r0;
-        const double *Yk = S.w + a.offY + (i64)k * BB * N, *Tk = S.w + a.offT + (i64)k * BB * BB;
-        double acc = 0.0;
-        for (int rc = 0; rc < m; rc += QT_R) {
-            __syncthreads();
-#pragma unroll
-            for (int u = 0; u < QT_R * BB / 256; u++) {
-                const int e = t + 256 * u, rr = e % QT_R, ii = e / QT_R;
-                yk[rr][ii] = rc + rr < m ? Yk[(rc + rr) + (i64)N * ii] : 0.0;
-            }
-            {
-                const int rr = t % QT_R, cc = t / QT_R;
-                ys[rr][cc] = rc + rr < m && cc < nc ? Y[(r0 + rc + rr) + S.ld * cc] : 0.0;
-            }
-            __syncthreads();
-#pragma unroll 16
-            for (int rr = 0; rr < QT_R; rr++) acc += yk[rr][i] * ys[rr][c];
-        }
-#pragma unroll
-        for (int u = 0; u < BB * BB / 256; u++) {
-            const int e = t + 256 * u, jj = e % BB, ii = e / BB;
-            tk[jj][ii] = Tk[jj + BB * ii];
-        }
-        wv[i][c] = acc;
-        __syncthreads();
-        double w2 = 0.0;
-#pragma unroll 16
-        for (int jj = 0; jj < BB; jj++) w2 += tk[jj][i] * wv[jj][c];
-        wv2[i][c] = w2;
-        for (int rc = 0; rc < m; rc += QT_R) {
-            __syncthreads();
-#pragma unroll
-            for (int u = 0; u < QT_R * BB / 256; u++) {
-                const int e = t + 256 * u, rr = e % QT_R, ii = e / QT_R;
-                yk[rr][ii] = rc + rr < m ? Yk[(rc + rr) + (i64)N * ii] : 0.0;
-            }
-            __syncthreads();
-            const int rr = t % QT_R, cc = t / QT_R;
-            if (rc + rr < m && cc < nc) {
-                double v = 0.0;
-#pragma unroll 16
-                for (int ii = 0; ii < BB; ii++) v += yk[rr][ii] * wv2[ii][cc];
-                Y[(r0 + rc + rr) + S.ld * cc] -= v;
-            }
-        }
-        __syncthreads();
+// Q_b^T Z for right-hand sides Z (N x ncols per problem), panel by panel, two launches per panel.
+// k_band_qt_step (ku, kp = ku + 1) covers rows base = BB (ku + 1) .. N-1 in blocks of QB rows and the
+// columns in blocks of QB; workgroup (rb, cb, b) holds its Z block in LDS and
+//   * (ku >= 0) updates it with panel ku: Z -= Y_ku W2, W2 = T_ku^T Y_ku^T Z from k_band_qt_reduce
+//     (rows of the block relative to panel ku: QB rb ..);
+//   * (kp < npan, rb >= 1) forms its partial Y_kp^T Z over its rows (relative to panel kp: QB (rb-1) ..).
+// k_band_qt_reduce then adds the partials of each column block in row-block order (a quarter of the
+// block's columns per workgroup) and forms W2 = T_kp^T (sum) for the next step.  No workgroup waits for
+// another inside a launch.
+#define QB 64
+struct BandQtK {
+    int N, npan, ku, kp, nrb, ncols, ncb, npmax;
+    i64 sW, offY, offT, ldZ, sZ;
+    const double *work;
+    double *Z, *Gp, *W2;         // Gp: [b][cb][rb] QB x QB partials; W2: [b][cb] QB x QB
+};
+__global__ __launch_bounds__(256) void k_band_qt_step(BandQtK a) {
+    __shared__ double zs[QB][QB + 1];
+    __shared__ double ys[QB][BB + 1];
+    __shared__ double ws[BB][QB + 1];
+    const int rb = blockIdx.x, cb = blockIdx.y, b = blockIdx.z, t = threadIdx.x;
+    const int N = a.N, base = BB * (a.ku + 1), rs = base + QB * rb;
+    const int nr = N - rs < QB ? N - rs : QB, cs = QB * cb;
+    const int ncl = a.ncols - cs < QB ? a.ncols - cs : QB;
+    const double *w0 = a.work + b * a.sW;
+    double *Zb = a.Z + b * a.sZ + a.ldZ * cs + rs;
+    const int tr = t % 16, tc = t / 16;
+    for (int e = t; e < QB * QB; e += 256) {
+        const int r = e % QB, c = e / QB;
+        zs[r][c] = r < nr && c < ncl ? Zb[r + a.ldZ * c] : 0.0;
     }
-    const int r0 = BB * (a.npan + 1), m = N - r0;
-    if (m >= 1) {
-        const double *Ql = S.w + a.offQl;
-#pragma unroll
-        for (int u = 0; u < BB * BB / 256; u++) {
-            const int e = t + 256 * u, jj = e % BB, ii = e / BB;
-            tk[jj][ii] = jj < m && ii < m ? Ql[jj + BB * ii] : 0.0;
+    if (a.ku >= 0) {
+        const double *Yk = w0 + a.offY + (i64)a.ku * BB * N + QB * rb;
+        const double *W2 = a.W2 + ((i64)b * a.ncb + cb) * QB * QB;
+        for (int e = t; e < QB * BB; e += 256) {
+            const int r = e % QB, i = e / QB;
+            ys[r][i] = r < nr ? Yk[r + (i64)N * i] : 0.0;
         }
-        {
-            const int rr = t % QT_R, cc = t / QT_R;
-            ys[rr][cc] = rr < m && cc < nc ? Y[(r0 + rr) + S.ld * cc] : 0.0;
+        for (int e = t; e < BB * QB; e += 256) ws[e % BB][e / BB] = W2[e];
+        __syncthreads();
+        double acc[4][4] = {};
+        for (int i = 0; i < BB; i++) {
+            double yv[4], wv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) { yv[u] = ys[tr + 16 * u][i]; wv[u] = ws[i][tc + 16 * u]; }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) acc[u][v] += yv[u] * wv[v];
         }
         __syncthreads();
-        if (i < m && c < nc) {
-            double v = 0.0;
-            for (int jj = 0; jj < m; jj++) v += tk[jj][i] * ys[jj][c];
-            Y[(r0 + i) + S.ld * c] = v;
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) zs[tr + 16 * u][tc + 16 * v] -= acc[u][v];
+    }
+    __syncthreads();
+    if (a.ku >= 0)
+        for (int e = t; e < QB * QB; e += 256) {
+            const int r = e % QB, c = e / QB;
+            if (r < nr && c < ncl) Zb[r + a.ldZ * c] = zs[r][c];
+        }
+    if (a.kp >= a.npan || rb == 0) return;
+    {
+        const double *Yk = w0 + a.offY + (i64)a.kp * BB * N + QB * (rb - 1);
+        for (int e = t; e < QB * BB; e += 256) {
+            const int r = e % QB, i = e / QB;
+            ys[r][i] = r < nr ? Yk[r + (i64)N * i] : 0.0;
         }
     }
+    __syncthreads();
+    double acc[4][4] = {};
+    for (int r = 0; r < QB; r++) {
+        double yv[4], zv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) { yv[u] = ys[r][tr + 16 * u]; zv[u] = zs[r][tc + 16 * u]; }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) acc[u][v] += yv[u] * zv[v];
+    }
+    double *G = a.Gp + (((i64)b * a.ncb + cb) * a.npmax + (rb - 1)) * QB * QB;
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) G[(tr + 16 * u) + BB * (tc + 16 * v)] = acc[u][v];
+}
+// W2 = T_kp^T (sum of the np partials) for columns 16 q .. 16 q + 15 of column block cb of problem b,
+// workgroup (q, cb, b)
+__global__ __launch_bounds__(256) void k_band_qt_reduce(BandQtK a, int np) {
+    __shared__ double ts[BB][BB + 1];
+    __shared__ double ws[BB][17];
+    const int q = blockIdx.x, cb = blockIdx.y, b = blockIdx.z, t = threadIdx.x;
+    const double *Gb = a.Gp + ((i64)b * a.ncb + cb) * a.npmax * QB * QB + (i64)BB * 16 * q;
+    const double *Tk = a.work + b * a.sW + a.offT + (i64)a.kp * BB * BB;
+    for (int e = t; e < BB * BB; e += 256) ts[e % BB][e / BB] = Tk[e];     // ts[j][i] = T[j][i]
+    double g[4] = {0.0, 0.0, 0.0, 0.0};
+    int p = 0;
+    for (; p + 4 <= np; p += 4) {
+        double x[4][4];
+#pragma unroll
+        for (int pp = 0; pp < 4; pp++)
+#pragma unroll
+            for (int u = 0; u < 4; u++) x[pp][u] = Gb[(i64)(p + pp) * QB * QB + t + 256 * u];
+#pragma unroll
+        for (int pp = 0; pp < 4; pp++)
+#pragma unroll
+            for (int u = 0; u < 4; u++) g[u] += x[pp][u];
+    }
+    for (; p < np; p++)
+#pragma unroll
+        for (int u = 0; u < 4; u++) g[u] += Gb[(i64)p * QB * QB + t + 256 * u];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int e = t + 256 * u;                // entry (j, c) of the quarter: j = e % BB, c = e / BB
+        ws[e % BB][e / BB] = g[u];
+    }
+    __syncthreads();
+    const int i = t % BB, c0 = t / BB;             // outputs (i, c0 + 4 v)
+    double o[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < BB; j++) {
+        const double tv = ts[j][i];
+#pragma unroll
+        for (int v = 0; v < 4; v++) o[v] += tv * ws[j][c0 + 4 * v];
+    }
+    double *W2 = a.W2 + ((i64)b * a.ncb + cb) * QB * QB + (i64)BB * 16 * q;
+#pragma unroll
+    for (int v = 0; v < 4; v++) W2[i + BB * (c0 + 4 * v)] = o[v];
+}
+// the last block: Z[r0:] <- Qm^T Z[r0:] (m <= BB rows), workgroup (column block, problem)
+__global__ __launch_bounds__(256) void k_band_qt_last(BandQtK a, int r0, int m, i64 offQl) {
+    __shared__ double zs[QB][QB + 1];
+    __shared__ double qs[BB][BB + 1];
+    const int cb = blockIdx.x, b = blockIdx.y, t = threadIdx.x, cs = QB * cb;
+    const int ncl = a.ncols - cs < QB ? a.ncols - cs : QB;
+    double *Zb = a.Z + b * a.sZ + a.ldZ * cs + r0;
+    const double *Ql = a.work + b * a.sW + offQl;
+    for (int e = t; e < QB * QB; e += 256) {
+        const int r = e % QB, c = e / QB;
+        zs[r][c] = r < m && c < ncl ? Zb[r + a.ldZ * c] : 0.0;
+        qs[r][c] = r < m && c < m ? Ql[r + BB * c] : 0.0;        // qs[j][i] = Qm[j][i]
+    }
+    __syncthreads();
+    const int tr = t % 16, tc = t / 16;
+    double o[4][4] = {};
+    for (int j = 0; j < BB; j++) {
+        double qv[4], zv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) { qv[u] = qs[j][tr + 16 * u]; zv[u] = zs[j][tc + 16 * u]; }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) o[u][v] += qv[u] * zv[v];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = tr + 16 * u, c = tc + 16 * v;
+            if (i < m && c < ncl) Zb[i + a.ldZ * c] = o[u][v];
+        }
 }
 
 #define RBB(call) do { if ((call) != rocblas_status_success) { set_error("rocBLAS failed: %s", #call); return -1; } } while (0)
@@ -560,32 +609,51 @@ int band_reduce(rocblas_handle h, hipStream_t st, int N, double *A, i64 sA, int 
     return 0;
 }
 
-// z = Q_b^T y in place for up to BAND_QT_SETS column sets (set s: problem b_s's Q, ncols_s columns of
-// order N at Y_s, leading dimension ld_s), one launch
-int band_apply_qt(hipStream_t st, int N, const double *work, int nset, const int *b, double *const *Y,
-                  const i64 *ld, const int *ncols) {
-    if (nset < 0 || nset > BAND_QT_SETS) { set_error("band_apply_qt: %d column sets", nset); return -2; }
+static inline int qt_ncb(int ncols) { return (ncols + QB - 1) / QB; }
+static inline int qt_npmax(int N) { return (N + QB - 1) / QB - 1; }
+i64 band_qt_workspace(int N, int nb, int ncols) {
+    const i64 ncb = qt_ncb(ncols);
+    return (i64)nb * ncb * qt_npmax(N) * QB * QB + (i64)nb * ncb * QB * QB;
+}
+// Z_b <- Q_b^T Z_b in place for the nb problems b0 .. b0+nb-1 (Z_b = Z + (b - b0) sZ: N x ncols, ld ldZ).
+// Q_b = H_0 H_1 ... H_{np-1} diag(I, Qm), so Q_b^T applies H_0^T first: per panel k, Z[r0:] -= Y_k T_k^T
+// Y_k^T Z[r0:] (k_band_qt_step and k_band_qt_reduce), then Z[r0:] <- Qm^T Z[r0:] (k_band_qt_last).
+int band_apply_qt(hipStream_t st, int N, const double *work, int b0, int nb, double *Z, i64 ldZ, i64 sZ, int ncols,
+                  double *scratch) {
+    if (nb < 1 || nb > 65535 || ncols < 0) { set_error("band_apply_qt: nb=%d ncols=%d", nb, ncols); return -2; }
+    if (ncols == 0) return 0;
     const BandLayout L = band_layout(N);
-    BandQtArgs a{};
+    BandQtK a{};
     a.N = N;
     a.npan = L.npan;
+    a.ncols = ncols;
+    a.ncb = qt_ncb(ncols);
+    a.npmax = qt_npmax(N);
+    a.sW = L.size;
     a.offY = L.Y;
     a.offT = L.T;
-    a.offQl = L.Ql;
-    int wg = 0;
-    for (int q = 0; q < nset; q++) {
-        if (ncols[q] <= 0) continue;
-        BandQtSet &S = a.s[a.nset++];
-        S.w = work + b[q] * L.size;
-        S.Y = Y[q];
-        S.ld = ld[q];
-        S.ncols = ncols[q];
-        S.wg0 = wg;
-        wg += (ncols[q] + QT_C - 1) / QT_C;
+    a.ldZ = ldZ;
+    a.sZ = sZ;
+    a.work = work + b0 * L.size;
+    a.Z = Z;
+    a.Gp = scratch;
+    a.W2 = scratch + (i64)nb * a.ncb * a.npmax * QB * QB;
+    for (int ku = -1; ku < L.npan; ku++) {
+        a.ku = ku;
+        a.kp = ku + 1;
+        a.nrb = (N - BB * (ku + 1) + QB - 1) / QB;
+        hipLaunchKernelGGL(k_band_qt_step, dim3(a.nrb, a.ncb, nb), dim3(256), 0, st, a);
+        NV2_LAUNCH_CHECK();
+        if (a.kp < L.npan) {
+            hipLaunchKernelGGL(k_band_qt_reduce, dim3(QB / 16, a.ncb, nb), dim3(256), 0, st, a, a.nrb - 1);
+            NV2_LAUNCH_CHECK();
+        }
     }
-    if (!wg) return 0;
-    hipLaunchKernelGGL(k_band_qt, dim3(wg), dim3(256), 0, st, a);
-    NV2_LAUNCH_CHECK();
+    const int r0 = BB * (L.npan + 1), m = N - r0;
+    if (m >= 1) {
+        hipLaunchKernelGGL(k_band_qt_last, dim3(a.ncb, nb), dim3(256), 0, st, a, r0, m, L.Ql);
+        NV2_LAUNCH_CHECK();
+    }
     return 0;
 }
 

@@ -17,6 +17,7 @@
 #include <rocblas/rocblas.h>
 
 #include "nv2_chol.h"
+#include "nv2_chol64.h"
 
 namespace nv2 {
 
@@ -85,62 +86,36 @@ __global__ __launch_bounds__(CHOL_WG) void k_chol_diag(double *A, int N, i64 sA,
     }
 }
 
-// The same in one wavefront with the block in registers: lane c holds column c of the symmetric
-// block (kb < CHOL_NB: padded with the identity, so the factor of the pad is the identity), and at
-// step j row j travels through LDS (a right-looking update of every column at once, no barrier
-// beyond the wavefront's own).  The inverse is formed the same way from the bottom row up.  The
-// 256-thread LDS form above spent most of its time in 64 barrier-separated steps; this one is the
-// form the (+-1)' chain and the smaller classes' factorisations use.
-__global__ __launch_bounds__(64) void k_chol_diag1(double *A, int N, i64 sA, int k0, int kb, int *info,
-                                                   double *Uinv) {
-    __shared__ double rb[CHOL_NB];
-    const int b = blockIdx.x, c = threadIdx.x;
-    if (info[b] != 0) return;                       // uniform over the wavefront
+// The same by the four wavefronts of a 256-thread workgroup with the block in registers (chol64_4w;
+// kb < CHOL_NB: padded with the identity, so the factor of the pad is the identity).  The 256-thread
+// LDS form above spent most of its time in 64 barrier-separated steps over LDS; this one is the form
+// the (+-1)' chain and the smaller classes' factorisations use.
+__global__ __launch_bounds__(256) void k_chol_diag1(double *A, int N, i64 sA, int k0, int kb, int *info,
+                                                    double *Uinv) {
+    __shared__ Chol64Lds L;
+    const int b = blockIdx.x, w = threadIdx.x >> 6, c = threadIdx.x & 63;
+    if (info[b] != 0) return;                       // uniform over the workgroup
     double *Ab = A + b * sA + k0 + (i64)N * k0;
-    double a[CHOL_NB];
+    double a[16], x[16];
 #pragma unroll
-    for (int r = 0; r < CHOL_NB; r++) {
+    for (int i = 0; i < 16; i++) {
+        const int r = 16 * w + i;
         double v = r == c ? 1.0 : 0.0;
         if (r < kb && c < kb) v = r <= c ? Ab[r + (i64)N * c] : Ab[c + (i64)N * r];
-        a[r] = v;
+        a[i] = v;
     }
-    int bad = 0;
-#pragma unroll
-    for (int j = 0; j < CHOL_NB; j++) {             // a[r] = S[r][c] for j <= r, c; then row j of U
-        rb[c] = a[j];
-        __syncthreads();
-        const double d = rb[j];
-        if (!(d > 0.0) && bad == 0) bad = k0 + j + 1;   // uniform: every lane reads the same d
-        const double u = sqrt(d > 0.0 ? d : 1.0), inv = 1.0 / u;
-        const double rjc = c == j ? u : a[j] * inv;      // U[j][c] (used for c > j)
-        a[j] = rjc;
-#pragma unroll
-        for (int r = j + 1; r < CHOL_NB; r++) a[r] -= (rb[r] * inv) * rjc;
-        __syncthreads();
-    }
+    const int bad = chol64_4w(a, x, L);
     if (bad) {
-        if (c == 0) info[b] = bad;
+        if (threadIdx.x == 0) info[b] = k0 + bad;
         return;
-    }
-    // U in the upper triangle of the block (a[i] = U[i][c] for i <= c)
-#pragma unroll
-    for (int i = 0; i < CHOL_NB; i++)
-        if (i <= c && c < kb && i < kb) Ab[i + (i64)N * c] = a[i];
-    // X = U^-1 in place, rows from the bottom: after step i, a[k] = X[k][c] for k >= i
-#pragma unroll
-    for (int i = CHOL_NB - 1; i >= 0; i--) {
-        rb[c] = a[i];                               // U[i][c]
-        __syncthreads();
-        double sm = c == i ? 1.0 : 0.0;
-#pragma unroll
-        for (int k = i + 1; k < CHOL_NB; k++) sm -= rb[k] * a[k];
-        a[i] = c >= i ? sm / rb[i] : 0.0;
-        __syncthreads();
     }
     double *Ub = Uinv + (i64)b * CHOL_NB * CHOL_NB;
 #pragma unroll
-    for (int i = 0; i < CHOL_NB; i++)
-        if (c < kb && i < kb) Ub[i + CHOL_NB * c] = a[i];
+    for (int i = 0; i < 16; i++) {
+        const int r = 16 * w + i;
+        if (r <= c && c < kb && r < kb) Ab[r + (i64)N * c] = a[i];   // U in the upper triangle of the block
+        if (c < kb && r < kb) Ub[r + CHOL_NB * c] = x[i];
+    }
 }
 
 // block row of U: A[k0 + r, k0 + kb + c] = P[r + CHOL_NB * c], r < kb, c < m
@@ -225,7 +200,7 @@ int chol_upper(rocblas_handle h, hipStream_t st, int N, double *A, i64 sA, int n
     const double one = 1.0, mone = -1.0, zero = 0.0;
     for (int k0 = 0; k0 < N; k0 += CHOL_NB) {
         const int kb = N - k0 < CHOL_NB ? N - k0 : CHOL_NB;
-        hipLaunchKernelGGL(k_chol_diag1, dim3(nb), dim3(64), 0, st, A, N, sA, k0, kb, info, Uinv);
+        hipLaunchKernelGGL(k_chol_diag1, dim3(nb), dim3(256), 0, st, A, N, sA, k0, kb, info, Uinv);
         NV2_LAUNCH_CHECK();
         const int m = N - k0 - kb;
         if (m <= 0) break;

@@ -1533,6 +1533,7 @@ struct Stage2 {
     Pm1pEarly *early = nullptr;    // the (+-1)' chain when started ahead of stage2_state
     hipEvent_t evS = nullptr, evK = nullptr;   // its metric filled / its Koopmans matrix filled
     hipEvent_t evX = nullptr;                  // its PC right-hand sides built (band-form sums on worker 1)
+    hipEvent_t evQt = nullptr;                 // Q^T applied to them (the held SC sums start after it)
     hipEvent_t evKr[2] = {nullptr, nullptr};   // around the K' reduction of the (+-1)' problems (timed)
     bool kr_recorded = false;
     int kr_N = 0, kr_nb = 0, kr_form = 0;
@@ -2038,7 +2039,6 @@ static int pc_metric_launch(Stage2 *p, PcJob &j, double thr) {
 }
 static int pc_kprime_form(Stage2 *p, PcJob &j, double thr);
 // after the worker stream has finished: threshold the metric spectra, launch K' and its solves
-static void early_kred_signal(Stage2 *p, const PcJob &j);
 static int pc_kprime_launch(Stage2 *p, PcJob &j, double thr) {
     const int N = j.N;
     if (j.chol) {                        // the factorisations of pc_metric_launch have finished
@@ -2104,7 +2104,6 @@ static int pc_kprime_launch(Stage2 *p, PcJob &j, double thr) {
                 if (stage2_timing()) fprintf(stderr, "[K' %s N=%d] band reduction %s\n", j.tag.c_str(), N,
                                              bok ? "done" : "failed (CholeskyQR2): tridiagonal reduction");
                 if (bok) {
-                    early_kred_signal(p, j);     // the device-heavy part of the chain is over
                     // caux stays U^-1: the sums project y = caux^T x and apply Q^T to y (band_apply_qt)
                     j.band = true;
                     p->kr_recorded = true;
@@ -2323,12 +2322,15 @@ struct Pm1pEarly {
     bool k_ready = false, abort = false;
     bool k_filled = false;         // K filled and signalled ahead of stage2_state (stage2_pm1p_kready)
     bool kred_done = false;        // the K' reduction has finished on the device (or the chain has ended)
+    bool qt_event = false;         // with kred_done: the held SC sums also wait for evQt on the device
     // Single-state runs on the band form: the PC sums go out on the chain's stream as soon as the
     // right-hand sides exist (x_ready, event evX), alongside the other classes on the main stream.
     bool sums_early = false, x_ready = false, sums_done = false;
     const double *X[2] = {nullptr, nullptr}, *eo[2] = {nullptr, nullptr};
     int next[2] = {0, 0};
     double *Ypc[2] = {nullptr, nullptr}, *Ub = nullptr, *yb = nullptr, *ptb = nullptr;
+    int ncm = 0;                   // column stride of the two classes' right-hand sides (Ypc[0] = Ypc[1] + N3 ncm)
+    double *qtw = nullptr;         // band_apply_qt scratch
     // .NATORB: density vectors T (N3 x n), their projections, and the density block of each class
     const double *Tr[2] = {nullptr, nullptr};
     double *Vr[2] = {nullptr, nullptr}, *rho[2] = {nullptr, nullptr};
@@ -2355,6 +2357,15 @@ static void pm1p_stop(Stage2 *p) {   // abort a chain still waiting for K, and j
 }
 // .NATORB density block from the banded solutions u_x (Y = [u_x], N x next): -sum_i u_x[i] V[i, d] with
 // V = (caux Q)^T T, as rho_ca[x, d] (nc x n) for (+1)' and rho_av[d, x] (n x nv) for (-1)'
+// Releases the SC sums held for the (+-1)' K' reduction; with qt, they also wait for evQt on the device
+static void early_kred_signal(Pm1pEarly *E, bool qt) {
+    {
+        std::lock_guard<std::mutex> lk(E->mu);
+        E->qt_event = qt;
+        E->kred_done = true;
+    }
+    E->cv.notify_all();
+}
 static int band_rho(rocblas_handle h, hipStream_t st, int plus, int N, int n, int next, const double *Y,
                     const double *V, double *rho) {
     if (plus) return gemm_on(h, st, true, false, next, n, N, -1.0, Y, N, V, N, 0.0, rho, next);
@@ -2369,18 +2380,14 @@ static int pm1p_band_sums(Stage2 *p, Pm1pEarly *E) {
     const int N = jb.nmax;
     const i64 NN = jb.NN();
     BandCls bc[2] = {};
-    int qb[BAND_QT_SETS], qn[BAND_QT_SETS], nq = 0;
-    double *qy[BAND_QT_SETS];
-    i64 qld[BAND_QT_SETS];
     for (int plus = 0; plus < 2; plus++) {
         const int next = E->next[plus], bb = plus ? 0 : 1;
         if (!next || !E->X[plus]) continue;
-        // y = caux^T x, then z = Q^T y for every class in one launch below
+        // y = caux^T x; z = Q^T y below, for both classes in the same GEMMs when both are there
         TRYS(gemm_on(h, st, true, false, N, next, N, 1.0, jb.caux + bb * NN, N, E->X[plus], N, 0.0, E->Ypc[plus], N));
-        qb[nq] = bb; qy[nq] = E->Ypc[plus]; qld[nq] = N; qn[nq++] = next;
         if (E->rho[plus]) {
             TRYS(gemm_on(h, st, true, false, N, E->nact, N, 1.0, jb.caux + bb * NN, N, E->Tr[plus], N, 0.0, E->Vr[plus], N));
-            qb[nq] = bb; qy[nq] = E->Vr[plus]; qld[nq] = N; qn[nq++] = E->nact;
+            TRYS(band_apply_qt(st, N, jb.bandw, bb, 1, E->Vr[plus], N, 0, E->nact, E->qtw));
         }
         BandCls &c = bc[plus];
         c.Br = band_rows(N, jb.nb, bb, jb.bandw);
@@ -2390,8 +2397,15 @@ static int pm1p_band_sums(Stage2 *p, Pm1pEarly *E) {
         c.next = next;
     }
     const int n0 = bc[0].next, n1 = bc[1].next;
-    if (!n0 && !n1) return 0;
-    TRYS(band_apply_qt(st, N, jb.bandw, nq, qb, qy, qld, qn));
+    if (!n0 && !n1) {
+        early_kred_signal(E, false);
+        return 0;
+    }
+    if (n0 && n1) TRYS(band_apply_qt(st, N, jb.bandw, 0, 2, E->Ypc[1], N, (i64)N * E->ncm, E->ncm, E->qtw));
+    else if (n0) TRYS(band_apply_qt(st, N, jb.bandw, 1, 1, E->Ypc[0], N, 0, n0, E->qtw));
+    else TRYS(band_apply_qt(st, N, jb.bandw, 0, 1, E->Ypc[1], N, 0, n1, E->qtw));
+    NV2_HIP(hipEventRecord(p->evQt, st));
+    early_kred_signal(E, true);
     bc[0].U = E->Ub;
     bc[0].y = E->yb;
     bc[1].U = E->Ub + (i64)n0 * N * UR;
@@ -2406,16 +2420,6 @@ static int pm1p_band_sums(Stage2 *p, Pm1pEarly *E) {
         if (E->rho[c]) TRYS(band_rho(h, st, c, N, E->nact, bc[c].next, bc[c].y, E->Vr[c], E->rho[c]));
     }
     return 0;
-}
-// Releases the SC sums held for the (+-1)' K' reduction (the job of the chain only)
-static void early_kred_signal(Stage2 *p, const PcJob &j) {
-    Pm1pEarly *E = p->early;
-    if (!E || &j != &E->job) return;
-    {
-        std::lock_guard<std::mutex> lk(E->mu);
-        E->kred_done = true;
-    }
-    E->cv.notify_all();
 }
 static int pm1p_worker(Stage2 *p, Pm1pEarly *E) {
     PcJob &j = E->job;
@@ -2434,11 +2438,9 @@ static int pm1p_worker(Stage2 *p, Pm1pEarly *E) {
     TRYS(pc_kprime_launch(p, j, E->thr));
     NV2_HIP(hipStreamSynchronize(p->ws[j.w]));
     E->t_end = std::chrono::steady_clock::now();
-    {
-        std::lock_guard<std::mutex> lk(E->mu);
-        E->kred_done = true;
-    }
-    E->cv.notify_all();
+    // on the band form with the sums on this stream, the SC sums are released once Q^T is applied to
+    // the right-hand sides (pm1p_band_sums): its workgroups need whole CUs, which the SC sums would hold
+    if (!(E->sums_early && j.band)) early_kred_signal(E, false);
     if (E->sums_early && j.band) {
         {
             std::unique_lock<std::mutex> lk(E->mu);
@@ -2524,9 +2526,15 @@ int stage2_pm1p_begin(Stage2 *p, int s, double thr) {
     E->sums_early = p->M == 1 && band_kprime((int)N3) && chol_metric() && (p->nc + p->nv) > 0;
     if (E->sums_early) {              // the sums' buffers, allocated here on the calling thread
         const i64 nx = (i64)p->nc + p->nv;
-        GET(y0, "pm1_Ypc0", N3 * (p->nv ? p->nv : 1)); GET(y1, "pm1_Ypc1", N3 * (p->nc ? p->nc : 1));
+        // both classes' right-hand sides in one buffer, problem b (the (+1)' class for b = 0) at b * N3 * ncm,
+        // so that Q^T goes to both in the same batched GEMMs
+        const i64 ncm = std::max<i64>(std::max(p->nc, p->nv), 1);
+        GET(yy, "pm1_Ypcb", 2 * N3 * ncm);
         GET(ub, "pm1_bandU", nx * N3 * UR); GET(yb, "pm1_bandy", nx * N3); GET(pb, "pm1_bandparts", 4 * nx);
-        E->Ypc[0] = y0; E->Ypc[1] = y1; E->Ub = ub; E->yb = yb; E->ptb = pb;
+        E->Ypc[1] = yy; E->Ypc[0] = yy + N3 * ncm; E->ncm = (int)ncm;
+        GET(qtw, "pm1_qtw", std::max(band_qt_workspace((int)N3, 2, (int)ncm), band_qt_workspace((int)N3, 1, n)));
+        E->qtw = qtw;
+        E->Ub = ub; E->yb = yb; E->ptb = pb;
     }
     E->x_ready = false;
     E->sums_done = false;
@@ -2539,6 +2547,7 @@ int stage2_pm1p_begin(Stage2 *p, int s, double thr) {
     E->rc = 0;
     E->err[0] = 0;
     E->kred_done = false;
+    E->qt_event = false;
     E->active = true;
     E->th = std::thread([p, E]() {
         try {
@@ -2613,6 +2622,7 @@ Stage2 *stage2_create(const S2Inputs &in, rocblas_handle h, hipStream_t st) {
     if (hipEventCreateWithFlags(&p->evS, hipEventDisableTiming) != hipSuccess) { delete p; return nullptr; }
     if (hipEventCreateWithFlags(&p->evK, hipEventDisableTiming) != hipSuccess) { delete p; return nullptr; }
     if (hipEventCreateWithFlags(&p->evX, hipEventDisableTiming) != hipSuccess) { delete p; return nullptr; }
+    if (hipEventCreateWithFlags(&p->evQt, hipEventDisableTiming) != hipSuccess) { delete p; return nullptr; }
     for (auto &e : p->evKr)
         if (hipEventCreate(&e) != hipSuccess) { delete p; return nullptr; }
     if (hipStreamCreateWithFlags(&p->xs, hipStreamNonBlocking) != hipSuccess) { delete p; return nullptr; }
@@ -2646,6 +2656,8 @@ void stage2_destroy(Stage2 *p) {
     delete p->early;
     if (p->evS) (void)hipEventDestroy(p->evS);
     if (p->evK) (void)hipEventDestroy(p->evK);
+    if (p->evX) (void)hipEventDestroy(p->evX);
+    if (p->evQt) (void)hipEventDestroy(p->evQt);
     for (auto e : p->evKr) if (e) (void)hipEventDestroy(e);
     p->sync_own();      // this evaluation's streams only; other contexts keep running
     for (auto &pl : p->sbplan) syevb_plan_destroy(pl);
@@ -3140,8 +3152,13 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
     // the SC sums wait until the (+-1)' K' reduction has finished (hold_sc)
     if (hold_sc() && has4 && p->early && p->early->active) {
         Pm1pEarly *E = p->early;
-        std::unique_lock<std::mutex> lk(E->mu);
-        E->cv.wait(lk, [E] { return E->kred_done || E->abort; });
+        bool qt;
+        {
+            std::unique_lock<std::mutex> lk(E->mu);
+            E->cv.wait(lk, [E] { return E->kred_done || E->abort; });
+            qt = E->qt_event && !E->abort;
+        }
+        if (qt) NV2_HIP(hipStreamWaitEvent(st, p->evQt, 0));
     }
     // class (0): SC == PC
     if (nc && nv) {
@@ -3454,11 +3471,12 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             q.a[j] = yj;
         }
         if (jb.band) {                   // z = Q^T y in place; the banded solves follow the loop
-            int qb[2] = {bb, bb}, qn[2] = {next * ncq, n};
-            double *qy[2] = {Ypc, Vrm[plus]};
-            i64 qld[2] = {nmax, nmax};
-            if (Trm[plus]) TRYS(gemm(p, true, false, nmax, n, (int)N3, 1.0, caux, N3, Trm[plus], N3, 0.0, Vrm[plus], nmax));
-            TRYS(band_apply_qt(st, nmax, jb.bandw, Trm[plus] ? 2 : 1, qb, qy, qld, qn));
+            GET(qtw, "pm1_qtw", std::max(band_qt_workspace(nmax, 1, next * ncq), band_qt_workspace(nmax, 1, n)));
+            TRYS(band_apply_qt(st, nmax, jb.bandw, bb, 1, Ypc, nmax, 0, next * ncq, qtw));
+            if (Trm[plus]) {
+                TRYS(gemm(p, true, false, nmax, n, (int)N3, 1.0, caux, N3, Trm[plus], N3, 0.0, Vrm[plus], nmax));
+                TRYS(band_apply_qt(st, nmax, jb.bandw, bb, 1, Vrm[plus], nmax, 0, n, qtw));
+            }
             BandCls &c = bc[plus];
             c.Br = band_rows(nmax, jb.nb, bb, jb.bandw);
             c.Z = Ypc;
