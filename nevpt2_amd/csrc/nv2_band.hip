@@ -197,98 +197,151 @@ __global__ __launch_bounds__(BWG) void k_tx(const double *Gp, int np, const doub
         for (int v = 0; v < 4; v++) Vb[(ti + 16 * u) + BB * (tj + 16 * v)] = acc[u][v];
 }
 
-// Householder form of panel k from its CholeskyQR2 factors, problem b = blockIdx.x, by two wavefronts
-// (lane c of a wavefront keeps column c of the matrix at hand in registers; rows and columns that
-// other lanes need travel through LDS).  Wavefront 0 factors Q2_top - S = L U without pivoting and
-// inverts U (stored by rows); wavefront 1 meanwhile forms R~ = S R2 R1 into the panel's top block of A
-// (the rows below it are outside the band and never read again), then, once L and U are there, Y_top = L and T = -U S L^-T.  R1, R2 come stored by
-// rows.  The phases of one wavefront order their LDS traffic by program order (one wavefront, in-order
-// LDS); the two meet at one workgroup barrier.
-__global__ __launch_bounds__(128) void k_brecon(double *A, int N, i64 sA, int p0, int r0, const double *Q2,
+// Householder form of panel k from its CholeskyQR2 factors, problem b = blockIdx.x, by the four
+// wavefronts of a 256-thread workgroup (Ballard et al.): with S = -sign(diag(Q2_top)), the LU factors of
+// Q2_top - S without pivoting give the reflectors Y_top = L (unit lower) and T = -U S L^-T.  Thread
+// (w, c) holds rows 16 w .. 16 w + 15 of column c.  LU step k: the wavefront with row k publishes it,
+// lane k of every wavefront publishes its rows of column k, and the rows below k are eliminated (one
+// barrier per step, the published row and column double-buffered).  Then U^-1 (rows from the bottom)
+// and L^-1 (rows from the top) in the same 64 steps, and the triangular products T and R~ = S R2 R1
+// (written into the panel's top block of A; the rows below it are outside the band and never read
+// again) from LDS.  R1, R2 and U^-1 are stored by rows.
+__global__ __launch_bounds__(256) void k_brecon(double *A, int N, i64 sA, int p0, int r0, const double *Q2,
                                                 const double *R1, const double *R2, double *Y, double *T,
                                                 double *Ui, i64 sW) {
-    __shared__ double mt[BB][BB + 1];    // the LU factors
+    __shared__ double lu[BB][BB + 1];    // U of Q2_top - S (zeros below the diagonal)
+    __shared__ double ll[BB][BB + 1];    // L (strictly lower part, zeros elsewhere)
+    __shared__ double xl[BB][BB + 1];    // L^-1; before the LU: R1
     __shared__ double r2[BB][BB + 1];    // R2
-    __shared__ double sgs[BB], rb[BB], cb[BB];
-    const int b = blockIdx.x, c = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __shared__ double rowb[2][BB], colb[2][BB], rub[2][BB], rlb[2][BB], sg[BB];
+    const int b = blockIdx.x, t = threadIdx.x, c = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);     // uniform over the wavefront
     double *Ab = A + b * sA;
     const double *Qb = Q2 + b * sW, *R1b = R1 + b * sW, *R2b = R2 + b * sW;
-    const double sgc = Qb[c + (i64)N * c] >= 0.0 ? -1.0 : 1.0;
-    double m[BB];
-    if (wv == 0) {
-        sgs[c] = sgc;
-        // LU without pivoting of Q2_top - S (|pivot| >= 1 by the choice of S); m = column c
-#pragma unroll
-        for (int r = 0; r < BB; r++) m[r] = Qb[r + (i64)N * c] - (r == c ? sgc : 0.0);
-#pragma unroll
-        for (int k = 0; k < BB; k++) {
-            rb[c] = m[k];                // row k: U[k][c] for c >= k
-            if (c == k) {
-#pragma unroll
-                for (int i = k + 1; i < BB; i++) cb[i] = m[i];   // column k below the pivot
-            }
-            WAVE_LDS_SYNC();
-            const double rp = 1.0 / rb[k], ukc = m[k];
-#pragma unroll
-            for (int i = k + 1; i < BB; i++) {
-                const double l = cb[i] * rp;
-                m[i] = c > k ? m[i] - l * ukc : (c == k ? l : m[i]);
-            }
-            WAVE_LDS_SYNC();
-        }
-#pragma unroll
-        for (int r = 0; r < BB; r++) mt[r][c] = m[r];
-    } else {
-#pragma unroll
-        for (int r = 0; r < BB; r++) { r2[r][c] = R2b[r * BB + c]; m[r] = R1b[r * BB + c]; }   // m = column c of R1
-        WAVE_LDS_SYNC();
-        // R~[r][c] = S_r sum_{r <= k <= c} R2[r][k] R1[k][c] (R1[k][c] = 0 for k > c)
-        const double *Qd = Qb;
-#pragma unroll
-        for (int r = 0; r < BB; r++) {
-            double v = 0.0;
-#pragma unroll
-            for (int k = r; k < BB; k++) v += r2[r][k] * m[k];
-            const double sr = Qd[r + (i64)N * r] >= 0.0 ? -1.0 : 1.0;
-            Ab[(r0 + r) + (i64)N * (p0 + c)] = r <= c ? sr * v : 0.0;
-            BAND_FENCE();
-        }
-        // rows r0 + BB .. N-1 of the panel's columns lie outside the band: nothing reads them again
+    if (t < BB) sg[t] = Qb[t + (i64)N * t] >= 0.0 ? -1.0 : 1.0;
+    for (int e = t; e < BB * BB; e += 256) {
+        xl[e / BB][e % BB] = R1b[e];     // R1[k][c], stored by rows
+        r2[e / BB][e % BB] = R2b[e];
     }
-    __syncthreads();                     // L, U and S in LDS
+    __syncthreads();
+    // R~[r][c] = S_r sum_{r <= k <= c} R2[r][k] R1[k][c] into the panel's top block (R1, R2 upper
+    // triangular with zeros below the diagonal: the full sum)
+    {
+        double v[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) v[i] = 0.0;
+#pragma unroll 4
+        for (int k = 0; k < BB; k++) {
+            const double bk = xl[k][c];
+#pragma unroll
+            for (int i = 0; i < 16; i++) v[i] += r2[16 * w + i][k] * bk;
+        }
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int r = 16 * w + i;
+            Ab[(r0 + r) + (i64)N * (p0 + c)] = r <= c ? sg[r] * v[i] : 0.0;
+        }
+    }
+    double mm[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const int r = 16 * w + i;
+        mm[i] = Qb[r + (i64)N * c] - (r == c ? sg[c] : 0.0);
+    }
+    // LU without pivoting (|pivot| >= 1 by the choice of S)
+#pragma unroll 1
+    for (int kw = 0; kw < 4; kw++) {
+#pragma unroll
+        for (int ik = 0; ik < 16; ik++) {
+            const int k = 16 * kw + ik, pb = ik & 1;
+            if (w == kw) rowb[pb][c] = mm[ik];
+            if (c == k)
+#pragma unroll
+                for (int i = 0; i < 16; i++) colb[pb][16 * w + i] = mm[i];
+            __syncthreads();
+            const double rp = 1.0 / rowb[pb][k], ukc = rowb[pb][c];
+            if (w > kw) {
+#pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    const double l = colb[pb][16 * w + i] * rp;
+                    mm[i] = c > k ? mm[i] - l * ukc : (c == k ? l : mm[i]);
+                }
+            } else if (w == kw) {
+#pragma unroll
+                for (int i = ik + 1; i < 16; i++) {
+                    const double l = colb[pb][16 * w + i] * rp;
+                    mm[i] = c > k ? mm[i] - l * ukc : (c == k ? l : mm[i]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const int r = 16 * w + i;
+        lu[r][c] = r <= c ? mm[i] : 0.0;
+        ll[r][c] = r > c ? mm[i] : 0.0;
+    }
+    __syncthreads();
+    // U^-1 from the bottom row up and L^-1 from the top row down, one step of each per barrier
+    double xu[16], xv[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) { xu[i] = 16 * w + i == c ? 1.0 : 0.0; xv[i] = xu[i]; }
+#pragma unroll 1
+    for (int sw = 0; sw < 4; sw++) {
+#pragma unroll
+        for (int is = 0; is < 16; is++) {
+            const int kl = 16 * sw + is, ku = BB - 1 - kl, pb = is & 1;
+            const int iu = 15 - is, wu = 3 - sw;          // owner of row ku
+            if (w == wu) {
+                const double x = c < ku ? 0.0 : xu[iu] / lu[ku][ku];
+                xu[iu] = x;
+                rub[pb][c] = x;
+            }
+            if (w == sw) rlb[pb][c] = xv[is];             // row kl of L^-1 is final (unit diagonal)
+            __syncthreads();
+            const double xuc = rub[pb][c], xlc = rlb[pb][c];
+            if (w < wu) {                                 // rows below ku: every row of the block
+#pragma unroll
+                for (int i = 0; i < 16; i++) xu[i] -= lu[16 * w + i][ku] * xuc;
+            } else if (w == wu) {
+#pragma unroll
+                for (int i = 0; i < iu; i++) xu[i] -= lu[16 * w + i][ku] * xuc;
+            }
+            if (w > sw) {                                 // rows above kl
+#pragma unroll
+                for (int i = 0; i < 16; i++) xv[i] -= ll[16 * w + i][kl] * xlc;
+            } else if (w == sw) {
+#pragma unroll
+                for (int i = is + 1; i < 16; i++) xv[i] -= ll[16 * w + i][kl] * xlc;
+            }
+        }
+    }
+    __syncthreads();                     // xl (R1) is no longer read
+#pragma unroll
+    for (int i = 0; i < 16; i++) xl[16 * w + i][c] = xv[i];
     double *Yb = Y + b * sW, *Tb = T + b * sW, *Uib = Ui + b * sW;
-    if (wv == 0) {
-        // U^-1 by rows from the bottom, in m: m[k] = X[k][c] for k > i (zero for k > c)
 #pragma unroll
-        for (int i = BB - 1; i >= 0; i--) {
-            double v = c == i ? 1.0 : 0.0;
+    for (int i = 0; i < 16; i++) {
+        const int r = 16 * w + i;
+        Uib[r * BB + c] = xu[i];
+        Yb[r + (i64)N * c] = r > c ? mm[i] : (r == c ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    // T[r][c] = -sum_{r <= k <= c} U[r][k] S_k L^-1[c][k] (U upper, L^-1 lower: the full sum)
+    {
+        double v[16];
 #pragma unroll
-            for (int k = i + 1; k < BB; k++) v -= mt[i][k] * m[k];
-            m[i] = v / mt[i][i];
-            BAND_FENCE();
+        for (int i = 0; i < 16; i++) v[i] = 0.0;
+#pragma unroll 4
+        for (int k = 0; k < BB; k++) {
+            const double zk = sg[k] * xl[c][k];
+#pragma unroll
+            for (int i = 0; i < 16; i++) v[i] -= lu[16 * w + i][k] * zk;
         }
 #pragma unroll
-        for (int i = 0; i < BB; i++) Uib[i * BB + c] = m[i];
-    } else {
-#pragma unroll
-        for (int r = 0; r < BB; r++) Yb[r + (i64)N * c] = r > c ? mt[r][c] : (r == c ? 1.0 : 0.0);
-        // row c of L^-1: L^T w = e_c, from the bottom (w[k] = 0 for k > c)
-#pragma unroll
-        for (int k = BB - 1; k >= 0; k--) {
-            double v = k == c ? 1.0 : 0.0;
-#pragma unroll
-            for (int t = k + 1; t < BB; t++) v -= mt[t][k] * m[t];
-            m[k] = v;
-            BAND_FENCE();
-        }
-        // T[r][c] = -sum_{r <= k <= c} U[r][k] S_k L^-1[c][k]
-#pragma unroll
-        for (int r = 0; r < BB; r++) {
-            double v = 0.0;
-#pragma unroll
-            for (int k = r; k < BB; k++) v -= mt[r][k] * sgs[k] * m[k];
-            Tb[r + BB * c] = r <= c ? v : 0.0;
-            BAND_FENCE();
+        for (int i = 0; i < 16; i++) {
+            const int r = 16 * w + i;
+            Tb[r + BB * c] = r <= c ? v[i] : 0.0;
         }
     }
 }
@@ -582,7 +635,7 @@ int band_reduce(rocblas_handle h, hipStream_t st, int N, double *A, i64 sA, int 
         hipLaunchKernelGGL(k_bchol, dim3(nb), dim3(BCH_T), 0, st, Gp, sW, np, R2, R2i, dflag, 0.1);
         RBB(rocblas_dgemm_strided_batched(h, Nn, Tt, m, BB, BB, &one, Q1, N, sW, R2i, BB, sW, &zero, Q2, N, sW, nb));
         // reflectors: Y_top, T, U^-1, and S R into the panel
-        hipLaunchKernelGGL(k_brecon, dim3(nb), dim3(128), 0, st, A, N, sA, p0, r0, Q2, R1, R2, Yk, Tk, Ui, sW);
+        hipLaunchKernelGGL(k_brecon, dim3(nb), dim3(256), 0, st, A, N, sA, p0, r0, Q2, R1, R2, Yk, Tk, Ui, sW);
         NV2_LAUNCH_CHECK();
         RBB(rocblas_dgemm_strided_batched(h, Nn, Tt, m - BB, BB, BB, &one, Q2 + BB, N, sW, Ui, BB, sW, &zero, Yk + BB,
                                           N, sW, nb));

@@ -21,7 +21,7 @@ struct Chol64Lds {
 // Returns the 1-based first step whose pivot is not positive (0 when the matrix factors); then a and x
 // hold finite values without meaning.  Uniform over the workgroup.  Every thread of the 256 calls it.
 __device__ inline int chol64_4w(double (&a)[16], double (&x)[16], Chol64Lds &L) {
-    const int w = threadIdx.x >> 6, c = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), c = threadIdx.x & 63;   // w uniform
     int bad = 0;
 #pragma unroll 1
     for (int jw = 0; jw < 4; jw++) {

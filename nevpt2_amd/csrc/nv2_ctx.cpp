@@ -235,6 +235,7 @@ int nv2_ctx_run(nv2_ctx *c, int32_t prec, double thr_lindep, double *out) {
         t_build += x; t_contr += y;
     }
     float t01 = 0.f;
+    NV2_HIP(hipEventSynchronize(c->ev[1]));   // already complete unless no E4 phase synchronised above
     NV2_HIP(hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
     NV2_HIP(hipEventRecord(c->ev[4], st));
     // phase 4: lower-order part of bamat and final A-matrices
