@@ -675,6 +675,65 @@ __global__ void k_pm2_sc(const double *I, i64 ld, const int2 *pairs, int mode, i
     if (q.M) block_storeQ(qsc, qpc, q);
 }
 
+// The same sums for the pairs x < y as one GEMM: T (npairs x nn, pair-major: the pair's integral block
+// T(a,b) at a + n b) times [X | K | X_j ...] (nn x ncol) gives U = T [X | K ...]; then dn = T_p . U_p[X],
+// dk = T_p . U_p[K] and the QD forms T_p . U_p[X_j] (k_pm2_dot, one wavefront per pair).
+__global__ void k_pm2_tgather(const double *I, i64 ld, const int2 *pairs, i64 npairs, int mode, int nc, int n,
+                             double *T) {
+    const int nn = n * n;
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npairs * nn) return;
+    const int e = (int)(t % nn), a = e % n, b = e / n;
+    const i64 pi = t / nn;
+    const int2 pr = pairs[pi];
+    const int x = pr.x, y = pr.y;
+    T[t] = mode == 0 ? I[(x + (i64)nc * a) + ld * (y + (i64)nc * b)] : I[(a + (i64)n * y) + ld * (b + (i64)n * x)];
+}
+__global__ void k_pm2_dot(const double *T, const double *U, int nn, int ncol, const int2 *pairs, i64 npairs, int mode,
+                          const double *eo, double *part, QDArgs q) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const i64 pi = (i64)blockIdx.x * (blockDim.x >> 6) + wv;
+    double E = 0.0, P = 0.0, qsc[QM], qpc[QM];
+#pragma unroll
+    for (int j = 0; j < QM; j++) { qsc[j] = 0.0; qpc[j] = 0.0; }
+    if (pi < npairs) {
+        const double *Tp = T + pi * nn, *Up = U + pi * ncol;
+        double dn = 0.0, dk = 0.0, dnj[QM];
+#pragma unroll
+        for (int j = 0; j < QM; j++) dnj[j] = 0.0;
+        for (int e = lane; e < nn; e += 64) {
+            const double te = Tp[e];
+            dn += te * Up[e];
+            dk += te * Up[nn + e];
+            int col = 2;
+#pragma unroll
+            for (int j = 0; j < QM; j++)
+                if (qd_on(q, j)) dnj[j] += te * Up[(i64)nn * col++ + e];
+        }
+        dn = wsum(dn);
+        dk = wsum(dk);
+#pragma unroll
+        for (int j = 0; j < QM; j++)
+            if (qd_on(q, j)) dnj[j] = wsum(dnj[j]);
+        if (lane == 0 && fabs(dn) >= 1.0e-14) {
+            const int2 pr = pairs[pi];
+            const double den = mode == 0 ? eo[pr.x] + eo[pr.y] - dk / dn : -dk / dn - eo[pr.x] - eo[pr.y];
+            E = dn / den;
+            P = E / den;
+#pragma unroll
+            for (int j = 0; j < QM; j++)
+                if (qd_on(q, j)) qsc[j] = dnj[j] / den;
+        }
+    }
+    if (lane != 0) {
+        E = 0.0; P = 0.0;
+#pragma unroll
+        for (int j = 0; j < QM; j++) qsc[j] = 0.0;
+    }
+    block_store4(E, P, 0.0, 0.0, part);
+    if (q.M) block_storeQ(qsc, qpc, q);
+}
+
 // ------------------------------------------------------------------ classes (+2)/(-2), PC (qdnevpt.F:2563-3072)
 // off-diagonal pairs: v(ab) = T(b,a) with ab = a*n+b ; contribution -(v.s_mu)^2/deno
 __global__ void k_pm2_pc(const double *I, i64 ld, const int2 *pairs, i64 npairs, int mode, int nc, int n,
@@ -1565,6 +1624,7 @@ struct Stage2 {
     double *qpart = nullptr;
     size_t qpart_cap = 0;
     int2 *pairs_c = nullptr, *pairs_cs = nullptr, *pairs_v = nullptr, *pairs_vs = nullptr, *up = nullptr;
+    int2 *pairs_cd = nullptr, *pairs_vd = nullptr;   // diagonal pairs (i, i), (r, r)
     i64 np_c, np_cs, np_v, np_vs;
     bool blocks_ready = false;
 
@@ -2632,11 +2692,13 @@ Stage2 *stage2_create(const S2Inputs &in, rocblas_handle h, hipStream_t st) {
     for (int e = 0; e < 2; e++)
         if (hipEventCreateWithFlags(&p->xev[e], hipEventDisableTiming) != hipSuccess) { delete p; return nullptr; }
     // pair tables: core i<=j, core i<j, virt r<=s, virt r<s, active upper pairs
-    std::vector<int2> pc, pcs, pv, pvs, up;
+    std::vector<int2> pc, pcs, pv, pvs, up, pcd, pvd;
     for (int i = 0; i < in.nc; i++)
         for (int j = i; j < in.nc; j++) { pc.push_back({i, j}); if (j > i) pcs.push_back({i, j}); }
     for (int r = 0; r < in.nv; r++)
         for (int s = r; s < in.nv; s++) { pv.push_back({r, s}); if (s > r) pvs.push_back({r, s}); }
+    for (int i = 0; i < in.nc; i++) pcd.push_back({i, i});
+    for (int r = 0; r < in.nv; r++) pvd.push_back({r, r});
     for (int a = 0; a < in.n; a++)
         for (int b = a; b < in.n; b++) up.push_back({a, b});
     auto upl = [&](std::vector<int2> &v, int2 **d) {
@@ -2646,6 +2708,7 @@ Stage2 *stage2_create(const S2Inputs &in, rocblas_handle h, hipStream_t st) {
         return true;
     };
     upl(pc, &p->pairs_c); upl(pcs, &p->pairs_cs); upl(pv, &p->pairs_v); upl(pvs, &p->pairs_vs); upl(up, &p->up);
+    upl(pcd, &p->pairs_cd); upl(pvd, &p->pairs_vd);
     p->np_c = pc.size(); p->np_cs = pcs.size(); p->np_v = pv.size(); p->np_vs = pvs.size();
     return p;
 }
@@ -2675,6 +2738,7 @@ void stage2_destroy(Stage2 *p) {
     if (p->xs) (void)hipStreamDestroy(p->xs);
     for (int e = 0; e < 2; e++) if (p->xev[e]) (void)hipEventDestroy(p->xev[e]);
     (void)hipFree(p->pairs_c); (void)hipFree(p->pairs_cs); (void)hipFree(p->pairs_v); (void)hipFree(p->pairs_vs);
+    (void)hipFree(p->pairs_cd); (void)hipFree(p->pairs_vd);
     (void)hipFree(p->up);
     delete p;
 }
@@ -3185,13 +3249,38 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
             QFILL(xsj, mode == 0 ? QK_P2_XS : QK_M2_XS, j, n4, "qd_pm2xs_");
             q.a[j] = xj; q.b[j] = xsj;
         }
+        (void)pr;
         if (next && npr) {
-            double *pt = parts(p, npr);
-            TRYS(qd_prepare(p, q, npr));
-            hipLaunchKernelGGL(k_pm2_sc, dim3((unsigned)npr), dim3(256), 0, st, I, ld, pr, mode, nc, n, X, K, eo, pt, q);
+            // diagonal pairs x == y by k_pm2_sc (symmetrised forms), pairs x < y by one GEMM + k_pm2_dot
+            const int2 *prd = mode == 0 ? p->pairs_cd : p->pairs_vd, *prs = mode == 0 ? p->pairs_cs : p->pairs_vs;
+            const i64 nd = next, ns = mode == 0 ? p->np_cs : p->np_vs;
+            const int nqj = q.M ? M - 1 : 0, ncol = (int)n2 * (2 + nqj);
+            const i64 nbs = (ns + 3) / 4;
+            double *pt = parts(p, nd + nbs);
+            if (!pt) { set_error("hipMalloc(parts) failed"); return -1; }
+            TRYS(qd_prepare(p, q, nd + nbs));
+            hipLaunchKernelGGL(k_pm2_sc, dim3((unsigned)nd), dim3(256), 0, st, I, ld, prd, mode, nc, n, X, K, eo, pt, q);
             NV2_LAUNCH_CHECK();
-            TRYS(reduce_into(p, npr, cls));
-            TRYS(qd_reduce(p, q, npr, cls));
+            if (ns) {
+                GET(Tp, "pm2_T", ns * n2);
+                GET(XK, "pm2_XK", n2 * ncol);
+                GET(Up, "pm2_U", ns * ncol);
+                NV2_HIP(hipMemcpyAsync(XK, X, sizeof(double) * n4, hipMemcpyDeviceToDevice, st));
+                NV2_HIP(hipMemcpyAsync(XK + n4, K, sizeof(double) * n4, hipMemcpyDeviceToDevice, st));
+                int col = 2;
+                for (int j = 0; q.M && j < M; j++)
+                    if (j != s) NV2_HIP(hipMemcpyAsync(XK + n4 * col++, q.a[j], sizeof(double) * n4, hipMemcpyDeviceToDevice, st));
+                hipLaunchKernelGGL(k_pm2_tgather, dim3(nb(ns * n2, 256)), dim3(256), 0, st, I, ld, prs, ns, mode, nc, n, Tp);
+                NV2_LAUNCH_CHECK();
+                TRYS(gemm(p, true, false, ncol, (int)ns, (int)n2, 1.0, XK, n2, Tp, n2, 0.0, Up, ncol));
+                QDArgs qs = q;
+                if (q.M) qs.qpart = q.qpart + 2 * QM * nd;
+                hipLaunchKernelGGL(k_pm2_dot, dim3((unsigned)nbs), dim3(256), 0, st, Tp, Up, (int)n2, ncol, prs, ns, mode,
+                                   eo, pt + 4 * nd, qs);
+                NV2_LAUNCH_CHECK();
+            }
+            TRYS(reduce_into(p, nd + nbs, cls));
+            TRYS(qd_reduce(p, q, nd + nbs, cls));
         }
     }
     // (0)' SC
