@@ -1037,6 +1037,88 @@ __global__ void k_v0p_pc(const double *Icvaa, i64 ldj, const double *Icaav, i64 
         if (threadIdx.x == 0) rcv[i + (i64)nc * r] = -R;
     }
 }
+// The (0)' sums of a single state as GEMMs over all (i, r) pairs p = i + nc r.  DJK holds, pair-major,
+// dj(ab) = (i r | a b) and dk(ab) = (i a | r b): row p = [dj | dk], with ab = a + n b (conv 0, the SC
+// sums) or a n + b (conv 1, the PC sums).
+__global__ void k_v0p_gather(const double *Icvaa, i64 ldj, const double *Icaav, i64 ldk, int nc, int n, i64 npairs,
+                             int conv, double *DJK) {
+    const int nn = n * n;
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npairs * 2 * nn) return;
+    const int k = (int)(t % (2 * nn)), e = k % nn;
+    const i64 pp = t / (2 * nn);
+    const int i = (int)(pp % nc), r = (int)(pp / nc);
+    const int a = conv ? e / n : e % n, b = conv ? e % n : e / n;
+    DJK[t] = k < nn ? Icvaa[(i + (i64)nc * r) + ldj * (a + (i64)n * b)] : Icaav[(i + (i64)nc * a) + ldk * (b + (i64)n * r)];
+}
+// SC: W (2 npairs x 4nn, row 2p: dj^T [X1 | X2 | G1 | G2], row 2p+1: dk^T [...]); one wavefront per pair
+// forms the quadratic forms of k_v0p_sc and its denominator
+__global__ void k_v0p_scdot(const double *W, const double *DJK, int n, i64 npairs, int nc, const double *da,
+                            const double *gf, const double *heff, const double *ec, const double *ev, double *part) {
+    const int lane = threadIdx.x & 63, nn = n * n;
+    const i64 pp = (i64)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    double E = 0.0, P = 0.0;
+    if (pp < npairs) {
+        const double *dj = DJK + pp * 2 * nn, *dk = dj + nn;
+        const double *wj = W + pp * 8 * nn, *wk = wj + 4 * nn;      // blocks X1 | X2 | G1 | G2
+        double s1 = 0.0, s2 = 0.0, dum = 0.0, dumk = 0.0;
+        for (int e = lane; e < nn; e += 64) {
+            const double dje = dj[e], dke = dk[e];
+            s1 += 2.0 * wj[e] * dje - wj[e] * dke - wk[e] * dje + wk[nn + e] * dke;
+            s2 += wj[2 * nn + e] * dje - 0.5 * wj[2 * nn + e] * dke - 0.5 * wk[2 * nn + e] * dje + wk[3 * nn + e] * dke;
+            const int a = e % n, b = e / n;
+            const double d1 = 2.0 * dje - dke;
+            dum += da[b + n * a] * d1;
+            dumk += gf[a + n * b] * d1 * 0.5;
+        }
+        s1 = wsum(s1); s2 = wsum(s2); dum = wsum(dum); dumk = wsum(dumk);
+        if (lane == 0) {
+            const int i = (int)(pp % nc), r = (int)(pp / nc);
+            const double hh = heff[pp];
+            const double dn = s1 + 2.0 * hh * dum + 2.0 * hh * hh, dko = s2 + hh * dumk;
+            if (fabs(dn) > 1.0e-14) {
+                const double den = ec[i] - ev[r] - dko / dn;
+                E = dn / den;
+                P = E / den;
+            }
+        }
+    }
+    if (lane != 0) { E = 0.0; P = 0.0; }
+    block_store4(E, P, 0.0, 0.0, part);
+}
+// PC: CM (2nn x nnew) = [2 sm - sp ; spp - sm], so that CO = CM^T [dj | dk] is rimu + rimup of k_v0p_pc
+__global__ void k_v0p_cm(const double *sm, const double *sp, const double *spp, int nn, int nnew, double *CM) {
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (i64)2 * nn * nnew) return;
+    const int k = (int)(t % (2 * nn));
+    const i64 mu = t / (2 * nn), o = (k % nn) + (i64)nn * mu;
+    CM[t] = k < nn ? 2.0 * sm[o] - sp[o] : spp[o] - sm[o];
+}
+// PC: CO (nnew x npairs); one wavefront per pair over mu for the denominators of k_v0p_pc
+__global__ void k_v0p_pcden(const double *CO, int nnew, i64 npairs, int nc, const double *dc1, const double *dc2,
+                            const double *w, const double *heff, const double *ec, const double *ev, double *part,
+                            const double *vx, double *rcv) {
+    const int lane = threadIdx.x & 63;
+    const i64 pp = (i64)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    double E = 0.0, P = 0.0, racc = 0.0;
+    if (pp < npairs) {
+        const int i = (int)(pp % nc), r = (int)(pp / nc);
+        const double hh = heff[pp], sh = ev[r] - ec[i];
+        const double *co = CO + pp * nnew;
+        for (int mu = lane; mu < nnew; mu += 64) {
+            const double c = co[mu] + 2.0 * hh * dc1[mu] - hh * dc2[mu];
+            const double cir = c / (w[mu] + sh);
+            E -= c * cir;
+            P += cir * cir;
+            if (rcv) racc += cir * vx[mu];
+        }
+        if (rcv) {
+            racc = wsum(racc);
+            if (lane == 0) rcv[pp] = -racc;
+        }
+    }
+    block_store4(0.0, 0.0, E, P, part);
+}
 // .NATORB, class (0)' (qdnevpt.F:4854-4861): vx[mu] = sum_ab da(a,b) (2 coef[ab, mu] - coef[n^2 + ab, mu]),
 // ab = a*n+b: the overlap of PC function mu with E_ri Psi_0 (per unit of the external pair)
 __global__ void k_rho_vaux0p(const double *da, int n, const double *coef, int ldc, int nnew, double *vx) {
@@ -3284,7 +3366,8 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         }
     }
     // (0)' SC
-    GET(X1, "x1", n4); GET(X2, "x2", n4); GET(G1, "g1", n4); GET(G2, "g2", n4);
+    GET(XG, "v0p_xg", 4 * n4);      // X1 | X2 | G1 | G2, contiguous for the GEMM of the SC sums
+    double *X1 = XG, *X2 = XG + n4, *G1 = XG + 2 * n4, *G2 = XG + 3 * n4;
     hipLaunchKernelGGL(k_v0p_prep, dim3(nb(n4, 256)), dim3(256), 0, st, S, k_ga, k_gd, X1, X2, G1, G2);
     NV2_LAUNCH_CHECK();
     for (int j = 0; q.M && j < M; j++) {
@@ -3294,7 +3377,21 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         QFILL(dsj, QK_0P_DS, j, n2, "qd_0pds_");
         q.a[j] = x1j; q.b[j] = x2j; q.c[j] = dsj;
     }
-    if (nc && nv) {
+    if (nc && nv && !q.M) {          // one state: GEMM over all (i, r) pairs
+        const i64 npp = (i64)nc * nv, nbk = (npp + 3) / 4;
+        GET(DJK, "v0p_djk", npp * 2 * n2);
+        GET(W, "v0p_W", npp * 8 * n2);
+        hipLaunchKernelGGL(k_v0p_gather, dim3(nb(npp * 2 * n2, 256)), dim3(256), 0, st, Icvaa, Ncv, Icaav, Nca, nc, n,
+                           npp, 0, DJK);
+        NV2_LAUNCH_CHECK();
+        TRYS(gemm(p, true, false, (int)(4 * n2), (int)(2 * npp), (int)n2, 1.0, X1, n2, DJK, n2, 0.0, W, 4 * n2));
+        double *pt = parts(p, nbk);
+        if (!pt) { set_error("hipMalloc(parts) failed"); return -1; }
+        hipLaunchKernelGGL(k_v0p_scdot, dim3((unsigned)nbk), dim3(256), 0, st, W, DJK, n, npp, nc, da, k_gf, h0p, ec,
+                           ev, pt);
+        NV2_LAUNCH_CHECK();
+        TRYS(reduce_into(p, nbk, 7));
+    } else if (nc && nv) {
         const i64 nbk = (i64)nc * nv;
         double *pt = parts(p, nbk);
         TRYS(qd_prepare(p, q, nbk));
@@ -3491,14 +3588,33 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
                 hipLaunchKernelGGL(k_rho_vaux0p, dim3(nb(nnew, 128)), dim3(128), 0, st, da, n, coef, N, nnew, vx);
                 NV2_LAUNCH_CHECK();
             }
-            const i64 nbk = (i64)nc * nv;
-            double *pt = parts(p, nbk);
-            TRYS(qd_prepare(p, q, nbk));
-            hipLaunchKernelGGL(k_v0p_pc, dim3((unsigned)nbk), dim3(256), 0, st, Icvaa, Ncv, Icaav, Nca, nc, n, nv, sm,
-                               sp, spp, dc1, dc2, w, nnew, h0p, ec, ev, pt, q, vx, p->rho1 ? rho_cv : nullptr);
-            NV2_LAUNCH_CHECK();
-            TRYS(reduce_into(p, nbk, 7));
-            TRYS(qd_reduce(p, q, nbk, 7));
+            if (!q.M) {              // one state: CO = CM^T [dj | dk] over all (i, r) pairs
+                const i64 npp = (i64)nc * nv, nbk = (npp + 3) / 4;
+                GET(DJK, "v0p_djk_pc", npp * 2 * n2);
+                GET(CM, "v0p_cm", 2 * n2 * nnew);
+                GET(CO, "v0p_co", npp * nnew);
+                hipLaunchKernelGGL(k_v0p_gather, dim3(nb(npp * 2 * n2, 256)), dim3(256), 0, st, Icvaa, Ncv, Icaav, Nca,
+                                   nc, n, npp, 1, DJK);
+                hipLaunchKernelGGL(k_v0p_cm, dim3(nb(2 * n2 * nnew, 256)), dim3(256), 0, st, sm, sp, spp, (int)n2, nnew,
+                                   CM);
+                NV2_LAUNCH_CHECK();
+                TRYS(gemm(p, true, false, nnew, (int)npp, (int)(2 * n2), 1.0, CM, 2 * n2, DJK, 2 * n2, 0.0, CO, nnew));
+                double *pt = parts(p, nbk);
+                if (!pt) { set_error("hipMalloc(parts) failed"); return -1; }
+                hipLaunchKernelGGL(k_v0p_pcden, dim3((unsigned)nbk), dim3(256), 0, st, CO, nnew, npp, nc, dc1, dc2, w,
+                                   h0p, ec, ev, pt, vx, p->rho1 ? rho_cv : nullptr);
+                NV2_LAUNCH_CHECK();
+                TRYS(reduce_into(p, nbk, 7));
+            } else {
+                const i64 nbk = (i64)nc * nv;
+                double *pt = parts(p, nbk);
+                TRYS(qd_prepare(p, q, nbk));
+                hipLaunchKernelGGL(k_v0p_pc, dim3((unsigned)nbk), dim3(256), 0, st, Icvaa, Ncv, Icaav, Nca, nc, n, nv,
+                                   sm, sp, spp, dc1, dc2, w, nnew, h0p, ec, ev, pt, q, vx, p->rho1 ? rho_cv : nullptr);
+                NV2_LAUNCH_CHECK();
+                TRYS(reduce_into(p, nbk, 7));
+                TRYS(qd_reduce(p, q, nbk, 7));
+            }
         }
     }
     // ================================================================ join worker 1: the (+-1)' solves
