@@ -422,30 +422,41 @@ __global__ void k_qd_fill(Sl S, int kind, int lt, const int2 *up, double *out, i
 }
 
 // ------------------------------------------------------------------ class (0) (qdnevpt.F:1661-1773)
-__global__ void k_v0(const double *I, i64 ld, const int2 *pairs, int nc, int nv, const double *ec, const double *ev,
-                     double *part) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+// E = -sum dint^2 / (e_r + e_s - e_i - e_j) over core pairs i <= j and virtual pairs r <= s, from the pair
+// blocks B_p(r, s) = (i r | j s) of the core pairs p = (i <= j) (pairs_c order),
+// nv x nv each at Bb + p nv^2: traint(r,s) = (j r | s i) = B_p(s, r), (j s | r i) = B_p(r, s).  Workgroup
+// (tile pair t, pair p) loads the tiles (r0.., s0..) and (s0.., r0..) of B_p (s0 >= r0) and sums r <= s.
+#define V0T 32
+__global__ __launch_bounds__(256) void k_v0b(const double *Bb, const int2 *pairs, int nv, const double *ec,
+                                             const double *ev, double *part) {
+    __shared__ double ta[V0T][V0T + 1], tb[V0T][V0T + 1];
+    const int nt = (nv + V0T - 1) / V0T;
+    int t = blockIdx.x, rt = 0;                      // tile pair t -> (rt <= st)
+    while (t >= nt - rt) { t -= nt - rt; rt++; }
+    const int st = rt + t, r0 = V0T * rt, s0 = V0T * st;
     const int2 pr = pairs[blockIdx.y];
-    const int ii = pr.x, ij = pr.y;
+    const double *B = Bb + (i64)blockIdx.y * nv * nv;
+    for (int e = threadIdx.x; e < V0T * V0T; e += 256) {
+        const int a = e % V0T, b = e / V0T;
+        ta[a][b] = r0 + a < nv && s0 + b < nv ? B[(r0 + a) + (i64)nv * (s0 + b)] : 0.0;   // B(r, s)
+        tb[b][a] = s0 + a < nv && r0 + b < nv ? B[(s0 + a) + (i64)nv * (r0 + b)] : 0.0;   // tb[i][j] = B(s0 + j, r0 + i)
+    }
+    __syncthreads();
+    const double dij = ec[pr.x] + ec[pr.y];
+    const bool zij = pr.x == pr.y;
     double E = 0.0, P = 0.0;
-    if (r < nv) {
-        const double dij = ec[ii] + ec[ij];
-        const bool zij = ii == ij;
-        for (int s = r; s < nv; s++) {
-            // traint(r,s) = (ij r | s ii)
-            double ab1 = I[(ij + (i64)nc * r) + ld * (ii + (i64)nc * s)];
-            double dint;
-            const bool zrs = r == s;
-            if (zrs && zij) dint = ab1;
-            else if (zrs != zij) dint = sqrt(2.0) * ab1;
-            else {
-                double ab2 = I[(ij + (i64)nc * s) + ld * (ii + (i64)nc * r)];
-                dint = 2.0 * sqrt(ab1 * ab1 + ab2 * ab2 - ab1 * ab2);
-            }
-            double co = -dint / (ev[r] + ev[s] - dij);
-            E += dint * co;
-            P += co * co;
-        }
+    for (int e = threadIdx.x; e < V0T * V0T; e += 256) {
+        const int a = e % V0T, b = e / V0T, r = r0 + a, s = s0 + b;
+        if (r >= nv || s >= nv || s < r) continue;
+        const double ab1 = tb[a][b], ab2 = ta[a][b];   // (j r | s i) = B(s, r), (j s | r i) = B(r, s)
+        double dint;
+        const bool zrs = r == s;
+        if (zrs && zij) dint = ab1;
+        else if (zrs != zij) dint = sqrt(2.0) * ab1;
+        else dint = 2.0 * sqrt(ab1 * ab1 + ab2 * ab2 - ab1 * ab2);
+        const double co = -dint / (ev[r] + ev[s] - dij);
+        E += dint * co;
+        P += co * co;
     }
     block_store4(E, P, E, P, part);
 }
@@ -3160,10 +3171,23 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
     const double *Taa = Baa;
     // ---- integral blocks: I_xy[u,v] = sum_L T_x[L,u] T_y[L,v]
     const i64 Ncv = (i64)nc * nv, Nca = (i64)nc * n, Nav = (i64)n * nv, Naa = n2;
-    GET(Icvcv, "Icvcv", Ncv * Ncv); GET(Icacv, "Icacv", Nca * Ncv); GET(Icvav, "Icvav", Ncv * Nav);
+    GET(Icacv, "Icacv", Nca * Ncv); GET(Icvav, "Icvav", Ncv * Nav);
     GET(Icaca, "Icaca", Nca * Nca); GET(Iavav, "Iavav", Nav * Nav); GET(Icvaa, "Icvaa", Ncv * Naa);
     GET(Icaav, "Icaav", Nca * Nav); GET(Iaaav, "Iaaav", Naa * Nav); GET(Iaaca, "Iaaca", Naa * Nca);
-    TRYS(gemm(p, true, false, Ncv, Ncv, L, 1.0, Tcv, L, Tcv, L, 0.0, Icvcv, Ncv));
+    // class (0) needs (i r | j s) for i <= j only: per core orbital i, the blocks of j = i .. nc-1 in one
+    // strided-batched GEMM, stored pair-major in pairs_c order
+    const i64 nv2 = (i64)nv * nv;
+    GET(V0b, "v0_pairblocks", (p->np_c ? p->np_c : 1) * nv2);
+    if (nc && nv) {
+        const double one = 1.0, zero = 0.0;
+        i64 p0 = 0;
+        for (int i = 0; i < nc; i++) {
+            RB(rocblas_dgemm_strided_batched(p->h, rocblas_operation_transpose, rocblas_operation_none, nv, nv, L, &one,
+                                             Tcv + (i64)L * i, L * nc, 0, Tcv + (i64)L * i, L * nc, L, &zero,
+                                             V0b + p0 * nv2, nv, nv2, nc - i));
+            p0 += nc - i;
+        }
+    }
     TRYS(gemm(p, true, false, Nca, Ncv, L, 1.0, Tca, L, Tcv, L, 0.0, Icacv, Nca));
     TRYS(gemm(p, true, false, Ncv, Nav, L, 1.0, Tcv, L, Tav, L, 0.0, Icvav, Ncv));
     TRYS(gemm(p, true, false, Nca, Nca, L, 1.0, Tca, L, Tca, L, 0.0, Icaca, Nca));
@@ -3308,9 +3332,11 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
     }
     // class (0): SC == PC
     if (nc && nv) {
-        dim3 g(nb(nv, 128), (unsigned)p->np_c);
+        const int nt = (nv + V0T - 1) / V0T;
+        dim3 g((unsigned)(nt * (nt + 1) / 2), (unsigned)p->np_c);
         double *pt = parts(p, (i64)g.x * g.y);
-        hipLaunchKernelGGL(k_v0, g, dim3(128), 0, st, Icvcv, Ncv, p->pairs_c, nc, nv, ec, ev, pt);
+        if (!pt) { set_error("hipMalloc(parts) failed"); return -1; }
+        hipLaunchKernelGGL(k_v0b, g, dim3(256), 0, st, V0b, p->pairs_c, nv, ec, ev, pt);
         NV2_LAUNCH_CHECK();
         TRYS(reduce_into(p, (i64)g.x * g.y, 0));
     }
