@@ -923,132 +923,7 @@ __global__ void k_v0p_vmats(Sl S, double *V, double *V2) {
     if (ap == a) v2 += 2.0 * DA(S, b, bp);
     V2[e] = v2;
 }
-__global__ void k_v0p_sc(const double *Icvaa, i64 ldj, const double *Icaav, i64 ldk, int nc, int n, int nv,
-                         const double *X1, const double *X2, const double *G1, const double *G2, const double *da,
-                         const double *gf, const double *heff, const double *ec, const double *ev, double *part,
-                         QDArgs q) {
-    // QD: q.a[j] = X1, q.b[j] = X2 (layout of X1/X2), q.c[j] = daoff(b,a)+daoff(a,b) at a+n*b
-    __shared__ double dj[256], dk[256], sh[16];
-    const int i = blockIdx.x % nc, r = blockIdx.x / nc;
-    const int nn = n * n;
-    for (int e = threadIdx.x; e < nn; e += blockDim.x) {
-        int a = e % n, b = e / n;
-        dj[e] = Icvaa[(i + (i64)nc * r) + ldj * (a + (i64)n * b)];       // (i r | a b)
-        dk[e] = Icaav[(i + (i64)nc * a) + ldk * (b + (i64)n * r)];       // (i a | r b)
-    }
-    __syncthreads();
-    double dn = 0.0, dko = 0.0, dummy = 0.0, dummyk = 0.0, dnj[QM];
-#pragma unroll
-    for (int j = 0; j < QM; j++) dnj[j] = 0.0;
-    for (int e = threadIdx.x; e < nn; e += blockDim.x) {      // (a',b')
-        const double djp = dj[e], dkp = dk[e];
-        double s1 = 0, s2 = 0;
-        for (int f = 0; f < nn; f++) {                        // (a,b)
-            const i64 o = e + (i64)nn * f;
-            const double djv = dj[f], dkv = dk[f];
-            const double dint1 = 2.0 * djp * djv - djp * dkv - dkp * djv;
-            const double dint2 = dkp * dkv;
-            const double dint11 = djp * djv - 0.5 * djp * dkv - 0.5 * djv * dkp;
-            s1 += dint1 * X1[o] + dint2 * X2[o];
-            s2 += dint11 * G1[o] + dint2 * G2[o];
-#pragma unroll
-            for (int j = 0; j < QM; j++)
-                if (qd_on(q, j)) dnj[j] += dint1 * q.a[j][o] + dint2 * q.b[j][o];
-        }
-        dn += s1;
-        dko += s2;
-        const int a = e % n, b = e / n;
-        const double d1 = 2.0 * dj[e] - dk[e];
-        dummy += da[b + n * a] * d1;
-        dummyk += gf[a + n * b] * d1 * 0.5;
-        const double hh = heff[i + (i64)nc * r];
-#pragma unroll
-        for (int j = 0; j < QM; j++)
-            if (qd_on(q, j)) dnj[j] += hh * q.c[j][a + n * b] * d1;
-    }
-    dn = block_sum(dn, sh);
-    dko = block_sum(dko, sh);
-    dummy = block_sum(dummy, sh);
-    dummyk = block_sum(dummyk, sh);
-#pragma unroll
-    for (int j = 0; j < QM; j++)
-        if (qd_on(q, j)) dnj[j] = block_sum(dnj[j], sh);
-    double E = 0.0, P = 0.0, qsc[QM], qpc[QM];
-#pragma unroll
-    for (int j = 0; j < QM; j++) { qsc[j] = 0.0; qpc[j] = 0.0; }
-    if (threadIdx.x == 0) {
-        const double hh = heff[i + (i64)nc * r];
-        dn = dn + 2.0 * hh * dummy + 2.0 * hh * hh;
-        dko = dko + hh * dummyk;
-        if (fabs(dn) > 1.0e-14) {
-            double den = ec[i] - ev[r] - dko / dn;
-            double co = dn / den;
-            E = co;
-            P = co / den;
-#pragma unroll
-            for (int j = 0; j < QM; j++)
-                if (qd_on(q, j)) qsc[j] = dnj[j] / den;
-        }
-    }
-    block_store4(E, P, 0.0, 0.0, part);
-    if (q.M) block_storeQ(qsc, qpc, q);
-}
-__global__ void k_v0p_pc(const double *Icvaa, i64 ldj, const double *Icaav, i64 ldk, int nc, int n, int nv,
-                         const double *sm, const double *sp, const double *spp, const double *dc1, const double *dc2,
-                         const double *w, int nnew, const double *heff, const double *ec, const double *ev,
-                         double *part, QDArgs q, const double *vx, double *rcv) {
-    // QD: q.a/b/c[j] = s, s', s'' of pair (i,j) (nn x nnew); q.d/e[j] = ro.coef[:nn], ro.coef[nn:] (nnew)
-    // .NATORB (rcv non-null): rcv[i + nc r] = -sum_mu c_ir,mu vx[mu] (qdnevpt.F:4971-4976)
-    __shared__ double dj[256], dk[256];
-    __shared__ double sh16[16];
-    const int i = blockIdx.x % nc, r = blockIdx.x / nc;
-    const int nn = n * n;
-    for (int e = threadIdx.x; e < nn; e += blockDim.x) {
-        int a = e / n, b = e % n;                                          // ref order ab = a*n+b
-        dj[e] = Icvaa[(i + (i64)nc * r) + ldj * (a + (i64)n * b)];
-        dk[e] = Icaav[(i + (i64)nc * a) + ldk * (b + (i64)n * r)];
-    }
-    __syncthreads();
-    const double hh = heff[i + (i64)nc * r];
-    double E = 0.0, P = 0.0, qsc[QM], qpc[QM], racc = 0.0;
-#pragma unroll
-    for (int j = 0; j < QM; j++) { qsc[j] = 0.0; qpc[j] = 0.0; }
-    for (int mu = threadIdx.x; mu < nnew; mu += blockDim.x) {
-        double rimu = 0.0, rimup = 0.0;
-        for (int ab = 0; ab < nn; ab++) {
-            const i64 o = ab + (i64)nn * mu;
-            rimu += (2.0 * dj[ab] - dk[ab]) * sm[o];
-            rimup += -dj[ab] * sp[o] + dk[ab] * spp[o];
-        }
-        rimu += 2.0 * hh * dc1[mu];
-        rimup -= hh * dc2[mu];
-        const double deno = w[mu] + ev[r] - ec[i];
-        const double co = rimu + rimup, cir = co / deno;
-        E -= co * cir;
-        P += cir * cir;
-        if (rcv) racc += cir * vx[mu];
-#pragma unroll
-        for (int j = 0; j < QM; j++) {
-            if (!qd_on(q, j)) continue;
-            double rj = 0.0, rjp = 0.0;
-            for (int ab = 0; ab < nn; ab++) {
-                const i64 o = ab + (i64)nn * mu;
-                rj += (2.0 * dj[ab] - dk[ab]) * q.a[j][o];
-                rjp += -dj[ab] * q.b[j][o] + dk[ab] * q.c[j][o];
-            }
-            rj += 2.0 * hh * q.d[j][mu];
-            rjp -= hh * q.e[j][mu];
-            qpc[j] -= (rj + rjp) * cir;
-        }
-    }
-    block_store4(0.0, 0.0, E, P, part);
-    if (q.M) block_storeQ(qsc, qpc, q);
-    if (rcv) {                               // uniform over the block
-        const double R = block_sum(racc, sh16);
-        if (threadIdx.x == 0) rcv[i + (i64)nc * r] = -R;
-    }
-}
-// The (0)' sums of a single state as GEMMs over all (i, r) pairs p = i + nc r.  DJK holds, pair-major,
+// The (0)' sums as GEMMs over all (i, r) pairs p = i + nc r.  DJK holds, pair-major,
 // dj(ab) = (i r | a b) and dk(ab) = (i a | r b): row p = [dj | dk], with ab = a + n b (conv 0, the SC
 // sums) or a n + b (conv 1, the PC sums).
 __global__ void k_v0p_gather(const double *Icvaa, i64 ldj, const double *Icaav, i64 ldk, int nc, int n, i64 npairs,
@@ -1062,17 +937,25 @@ __global__ void k_v0p_gather(const double *Icvaa, i64 ldj, const double *Icaav, 
     const int a = conv ? e / n : e % n, b = conv ? e % n : e / n;
     DJK[t] = k < nn ? Icvaa[(i + (i64)nc * r) + ldj * (a + (i64)n * b)] : Icaav[(i + (i64)nc * a) + ldk * (b + (i64)n * r)];
 }
-// SC: W (2 npairs x 4nn, row 2p: dj^T [X1 | X2 | G1 | G2], row 2p+1: dk^T [...]); one wavefront per pair
-// forms the quadratic forms of k_v0p_sc and its denominator
-__global__ void k_v0p_scdot(const double *W, const double *DJK, int n, i64 npairs, int nc, const double *da,
-                            const double *gf, const double *heff, const double *ec, const double *ev, double *part) {
+// SC: W (2 npairs x ncw, row 2p: dj^T [X1 | X2 | G1 | G2 | A_j | B_j ...], row 2p+1: dk^T [...], ncw = nn (4 +
+// 2 partners)); one wavefront per pair forms e20pcont's SC quadratic forms (qdnevpt.F:4436-4611), the
+// denominator and, for QD,
+// the couplings with A_j = q.a[j], B_j = q.b[j] (the partners' X1, X2) and q.c[j]
+__global__ void k_v0p_scdot(const double *W, int ncw, const double *DJK, int n, i64 npairs, int nc, const double *da,
+                            const double *gf, const double *heff, const double *ec, const double *ev, double *part,
+                            QDArgs q) {
     const int lane = threadIdx.x & 63, nn = n * n;
     const i64 pp = (i64)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    double E = 0.0, P = 0.0;
+    double E = 0.0, P = 0.0, qsc[QM], qpc[QM];
+#pragma unroll
+    for (int j = 0; j < QM; j++) { qsc[j] = 0.0; qpc[j] = 0.0; }
     if (pp < npairs) {
         const double *dj = DJK + pp * 2 * nn, *dk = dj + nn;
-        const double *wj = W + pp * 8 * nn, *wk = wj + 4 * nn;      // blocks X1 | X2 | G1 | G2
-        double s1 = 0.0, s2 = 0.0, dum = 0.0, dumk = 0.0;
+        const double *wj = W + pp * 2 * ncw, *wk = wj + ncw;      // blocks X1 | X2 | G1 | G2 | A_j | B_j ...
+        double s1 = 0.0, s2 = 0.0, dum = 0.0, dumk = 0.0, dnj[QM];
+#pragma unroll
+        for (int j = 0; j < QM; j++) dnj[j] = 0.0;
+        const double hh = heff[pp];
         for (int e = lane; e < nn; e += 64) {
             const double dje = dj[e], dke = dk[e];
             s1 += 2.0 * wj[e] * dje - wj[e] * dke - wk[e] * dje + wk[nn + e] * dke;
@@ -1081,23 +964,43 @@ __global__ void k_v0p_scdot(const double *W, const double *DJK, int n, i64 npair
             const double d1 = 2.0 * dje - dke;
             dum += da[b + n * a] * d1;
             dumk += gf[a + n * b] * d1 * 0.5;
+            int blk = 4;
+#pragma unroll
+            for (int j = 0; j < QM; j++)
+                if (qd_on(q, j)) {
+                    const i64 oa = (i64)nn * blk + e, ob = oa + nn;
+                    dnj[j] += 2.0 * wj[oa] * dje - wj[oa] * dke - wk[oa] * dje + wk[ob] * dke + hh * q.c[j][e] * d1;
+                    blk += 2;
+                }
         }
         s1 = wsum(s1); s2 = wsum(s2); dum = wsum(dum); dumk = wsum(dumk);
+#pragma unroll
+        for (int j = 0; j < QM; j++)
+            if (qd_on(q, j)) dnj[j] = wsum(dnj[j]);
         if (lane == 0) {
             const int i = (int)(pp % nc), r = (int)(pp / nc);
-            const double hh = heff[pp];
             const double dn = s1 + 2.0 * hh * dum + 2.0 * hh * hh, dko = s2 + hh * dumk;
             if (fabs(dn) > 1.0e-14) {
                 const double den = ec[i] - ev[r] - dko / dn;
                 E = dn / den;
                 P = E / den;
+#pragma unroll
+                for (int j = 0; j < QM; j++)
+                    if (qd_on(q, j)) qsc[j] = dnj[j] / den;
             }
         }
     }
-    if (lane != 0) { E = 0.0; P = 0.0; }
+    if (lane != 0) {
+        E = 0.0; P = 0.0;
+#pragma unroll
+        for (int j = 0; j < QM; j++) qsc[j] = 0.0;
+    }
     block_store4(E, P, 0.0, 0.0, part);
+    if (q.M) block_storeQ(qsc, qpc, q);
 }
-// PC: CM (2nn x nnew) = [2 sm - sp ; spp - sm], so that CO = CM^T [dj | dk] is rimu + rimup of k_v0p_pc
+// PC (qdnevpt.F:4892-4986): CM (2nn x nnew) = [2 sm - sp ; spp - sm], so that CO = CM^T [dj | dk] is
+// r_mu = sum_ab (2 dj - dk) sm + (-dj sp + dk spp), with sm = V coef[:nn], sp = V coef[nn:], spp = V2 coef[nn:]
+// (the QD partners' s, s', s'' give the next nnew-column blocks of CM the same way)
 __global__ void k_v0p_cm(const double *sm, const double *sp, const double *spp, int nn, int nnew, double *CM) {
     const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (i64)2 * nn * nnew) return;
@@ -1105,23 +1008,35 @@ __global__ void k_v0p_cm(const double *sm, const double *sp, const double *spp, 
     const i64 mu = t / (2 * nn), o = (k % nn) + (i64)nn * mu;
     CM[t] = k < nn ? 2.0 * sm[o] - sp[o] : spp[o] - sm[o];
 }
-// PC: CO (nnew x npairs); one wavefront per pair over mu for the denominators of k_v0p_pc
-__global__ void k_v0p_pcden(const double *CO, int nnew, i64 npairs, int nc, const double *dc1, const double *dc2,
-                            const double *w, const double *heff, const double *ec, const double *ev, double *part,
-                            const double *vx, double *rcv) {
+// PC: CO (nco x npairs, nco = nnew (1 + partners)); one wavefront per pair over mu for the denominators of
+// the PC sum (c_mu = r_mu + 2 h ro.coef[:nn] - h ro.coef[nn:], cir = c_mu / (w_mu + e_r - e_i)) and, for
+// QD, the couplings -r_j cir with r_j from the partner's block of CO, q.d[j] and q.e[j]
+__global__ void k_v0p_pcden(const double *CO, int nnew, int nco, i64 npairs, int nc, const double *dc1,
+                            const double *dc2, const double *w, const double *heff, const double *ec, const double *ev,
+                            double *part, const double *vx, double *rcv, QDArgs q) {
     const int lane = threadIdx.x & 63;
     const i64 pp = (i64)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    double E = 0.0, P = 0.0, racc = 0.0;
+    double E = 0.0, P = 0.0, racc = 0.0, qsc[QM], qpc[QM];
+#pragma unroll
+    for (int j = 0; j < QM; j++) { qsc[j] = 0.0; qpc[j] = 0.0; }
     if (pp < npairs) {
         const int i = (int)(pp % nc), r = (int)(pp / nc);
         const double hh = heff[pp], sh = ev[r] - ec[i];
-        const double *co = CO + pp * nnew;
+        const double *co = CO + pp * nco;
         for (int mu = lane; mu < nnew; mu += 64) {
             const double c = co[mu] + 2.0 * hh * dc1[mu] - hh * dc2[mu];
             const double cir = c / (w[mu] + sh);
             E -= c * cir;
             P += cir * cir;
             if (rcv) racc += cir * vx[mu];
+            int blk = 1;
+#pragma unroll
+            for (int j = 0; j < QM; j++)
+                if (qd_on(q, j)) {
+                    const double rj = co[(i64)nnew * blk + mu] + 2.0 * hh * q.d[j][mu] - hh * q.e[j][mu];
+                    qpc[j] -= rj * cir;
+                    blk++;
+                }
         }
         if (rcv) {
             racc = wsum(racc);
@@ -1129,6 +1044,7 @@ __global__ void k_v0p_pcden(const double *CO, int nnew, i64 npairs, int nc, cons
         }
     }
     block_store4(0.0, 0.0, E, P, part);
+    if (q.M) block_storeQ(qsc, qpc, q);
 }
 // .NATORB, class (0)' (qdnevpt.F:4854-4861): vx[mu] = sum_ab da(a,b) (2 coef[ab, mu] - coef[n^2 + ab, mu]),
 // ab = a*n+b: the overlap of PC function mu with E_ri Psi_0 (per unit of the external pair)
@@ -3392,7 +3308,8 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         }
     }
     // (0)' SC
-    GET(XG, "v0p_xg", 4 * n4);      // X1 | X2 | G1 | G2, contiguous for the GEMM of the SC sums
+    const int nq0 = q.M ? M - 1 : 0;   // QD partners
+    GET(XG, "v0p_xg", (4 + 2 * nq0) * n4);   // X1 | X2 | G1 | G2 | the partners' X1, X2: one GEMM operand
     double *X1 = XG, *X2 = XG + n4, *G1 = XG + 2 * n4, *G2 = XG + 3 * n4;
     hipLaunchKernelGGL(k_v0p_prep, dim3(nb(n4, 256)), dim3(256), 0, st, S, k_ga, k_gd, X1, X2, G1, G2);
     NV2_LAUNCH_CHECK();
@@ -3403,26 +3320,25 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
         QFILL(dsj, QK_0P_DS, j, n2, "qd_0pds_");
         q.a[j] = x1j; q.b[j] = x2j; q.c[j] = dsj;
     }
-    if (nc && nv && !q.M) {          // one state: GEMM over all (i, r) pairs
-        const i64 npp = (i64)nc * nv, nbk = (npp + 3) / 4;
+    if (nc && nv) {                  // GEMM over all (i, r) pairs
+        const i64 npp = (i64)nc * nv, nbk = (npp + 3) / 4, ncw = (4 + 2 * nq0) * n2;
+        int blk = 4;
+        for (int j = 0; q.M && j < M; j++) {
+            if (j == s) continue;
+            NV2_HIP(hipMemcpyAsync(XG + n4 * blk++, q.a[j], sizeof(double) * n4, hipMemcpyDeviceToDevice, st));
+            NV2_HIP(hipMemcpyAsync(XG + n4 * blk++, q.b[j], sizeof(double) * n4, hipMemcpyDeviceToDevice, st));
+        }
         GET(DJK, "v0p_djk", npp * 2 * n2);
-        GET(W, "v0p_W", npp * 8 * n2);
+        GET(W, "v0p_W", npp * 2 * ncw);
         hipLaunchKernelGGL(k_v0p_gather, dim3(nb(npp * 2 * n2, 256)), dim3(256), 0, st, Icvaa, Ncv, Icaav, Nca, nc, n,
                            npp, 0, DJK);
         NV2_LAUNCH_CHECK();
-        TRYS(gemm(p, true, false, (int)(4 * n2), (int)(2 * npp), (int)n2, 1.0, X1, n2, DJK, n2, 0.0, W, 4 * n2));
+        TRYS(gemm(p, true, false, (int)ncw, (int)(2 * npp), (int)n2, 1.0, XG, n2, DJK, n2, 0.0, W, ncw));
         double *pt = parts(p, nbk);
         if (!pt) { set_error("hipMalloc(parts) failed"); return -1; }
-        hipLaunchKernelGGL(k_v0p_scdot, dim3((unsigned)nbk), dim3(256), 0, st, W, DJK, n, npp, nc, da, k_gf, h0p, ec,
-                           ev, pt);
-        NV2_LAUNCH_CHECK();
-        TRYS(reduce_into(p, nbk, 7));
-    } else if (nc && nv) {
-        const i64 nbk = (i64)nc * nv;
-        double *pt = parts(p, nbk);
         TRYS(qd_prepare(p, q, nbk));
-        hipLaunchKernelGGL(k_v0p_sc, dim3((unsigned)nbk), dim3(256), 0, st, Icvaa, Ncv, Icaav, Nca, nc, n, nv, X1,
-                           X2, G1, G2, da, k_gf, h0p, ec, ev, pt, q);
+        hipLaunchKernelGGL(k_v0p_scdot, dim3((unsigned)nbk), dim3(256), 0, st, W, (int)ncw, DJK, n, npp, nc, da, k_gf,
+                           h0p, ec, ev, pt, q);
         NV2_LAUNCH_CHECK();
         TRYS(reduce_into(p, nbk, 7));
         TRYS(qd_reduce(p, q, nbk, 7));
@@ -3614,29 +3530,29 @@ int stage2_state(Stage2 *p, int s, double thr, int prec, double *out, S2Stats *s
                 hipLaunchKernelGGL(k_rho_vaux0p, dim3(nb(nnew, 128)), dim3(128), 0, st, da, n, coef, N, nnew, vx);
                 NV2_LAUNCH_CHECK();
             }
-            if (!q.M) {              // one state: CO = CM^T [dj | dk] over all (i, r) pairs
-                const i64 npp = (i64)nc * nv, nbk = (npp + 3) / 4;
+            {                        // CO = CM^T [dj | dk] over all (i, r) pairs, the QD partners' blocks after
+                const int nq = q.M ? M - 1 : 0, nco = nnew * (1 + nq);
+                const i64 npp = (i64)nc * nv, nbk = (npp + 3) / 4, ncm = 2 * n2 * nnew;
                 GET(DJK, "v0p_djk_pc", npp * 2 * n2);
-                GET(CM, "v0p_cm", 2 * n2 * nnew);
-                GET(CO, "v0p_co", npp * nnew);
+                GET(CM, "v0p_cm", ncm * (1 + nq));
+                GET(CO, "v0p_co", npp * nco);
                 hipLaunchKernelGGL(k_v0p_gather, dim3(nb(npp * 2 * n2, 256)), dim3(256), 0, st, Icvaa, Ncv, Icaav, Nca,
                                    nc, n, npp, 1, DJK);
-                hipLaunchKernelGGL(k_v0p_cm, dim3(nb(2 * n2 * nnew, 256)), dim3(256), 0, st, sm, sp, spp, (int)n2, nnew,
-                                   CM);
+                hipLaunchKernelGGL(k_v0p_cm, dim3(nb(ncm, 256)), dim3(256), 0, st, sm, sp, spp, (int)n2, nnew, CM);
                 NV2_LAUNCH_CHECK();
-                TRYS(gemm(p, true, false, nnew, (int)npp, (int)(2 * n2), 1.0, CM, 2 * n2, DJK, 2 * n2, 0.0, CO, nnew));
+                int blk = 1;
+                for (int j = 0; q.M && j < M; j++) {
+                    if (j == s) continue;
+                    hipLaunchKernelGGL(k_v0p_cm, dim3(nb(ncm, 256)), dim3(256), 0, st, q.a[j], q.b[j], q.c[j], (int)n2,
+                                       nnew, CM + ncm * blk++);
+                    NV2_LAUNCH_CHECK();
+                }
+                TRYS(gemm(p, true, false, nco, (int)npp, (int)(2 * n2), 1.0, CM, 2 * n2, DJK, 2 * n2, 0.0, CO, nco));
                 double *pt = parts(p, nbk);
                 if (!pt) { set_error("hipMalloc(parts) failed"); return -1; }
-                hipLaunchKernelGGL(k_v0p_pcden, dim3((unsigned)nbk), dim3(256), 0, st, CO, nnew, npp, nc, dc1, dc2, w,
-                                   h0p, ec, ev, pt, vx, p->rho1 ? rho_cv : nullptr);
-                NV2_LAUNCH_CHECK();
-                TRYS(reduce_into(p, nbk, 7));
-            } else {
-                const i64 nbk = (i64)nc * nv;
-                double *pt = parts(p, nbk);
                 TRYS(qd_prepare(p, q, nbk));
-                hipLaunchKernelGGL(k_v0p_pc, dim3((unsigned)nbk), dim3(256), 0, st, Icvaa, Ncv, Icaav, Nca, nc, n, nv,
-                                   sm, sp, spp, dc1, dc2, w, nnew, h0p, ec, ev, pt, q, vx, p->rho1 ? rho_cv : nullptr);
+                hipLaunchKernelGGL(k_v0p_pcden, dim3((unsigned)nbk), dim3(256), 0, st, CO, nnew, nco, npp, nc, dc1, dc2,
+                                   w, h0p, ec, ev, pt, vx, p->rho1 ? rho_cv : nullptr, q);
                 NV2_LAUNCH_CHECK();
                 TRYS(reduce_into(p, nbk, 7));
                 TRYS(qd_reduce(p, q, nbk, 7));
