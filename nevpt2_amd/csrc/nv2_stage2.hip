@@ -52,6 +52,44 @@ __device__ __forceinline__ double wsum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     return v;
 }
+// Wavefront reductions by DPP lane permutes (VALU latency, no LDS round trip): xor 1 and 2 by quad
+// permutes, then the half-row and row mirrors, then rows 0 + 1 and 2 + 3 by row_bcast15 and the halves by
+// row_bcast31; lane 63 ends with the total, which readlane returns to every lane (a wave-uniform value).
+// Lanes outside a broadcast's row mask receive 0, the identity of both reductions below.
+template <int CTRL, int RM>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)x, CTRL, RM, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(x >> 32), CTRL, RM, 0xf, false);
+    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+__device__ __forceinline__ unsigned long long umax64(unsigned long long a, unsigned long long b) { return a > b ? a : b; }
+__device__ __forceinline__ unsigned long long wmax_u64(unsigned long long x) {   // every lane active
+    x = umax64(x, dpp_u64<0xb1, 0xf>(x));       // quad_perm [1,0,3,2]
+    x = umax64(x, dpp_u64<0x4e, 0xf>(x));       // quad_perm [2,3,0,1]
+    x = umax64(x, dpp_u64<0x141, 0xf>(x));      // row_half_mirror
+    x = umax64(x, dpp_u64<0x140, 0xf>(x));      // row_mirror
+    x = umax64(x, dpp_u64<0x142, 0xa>(x));      // row_bcast15
+    x = umax64(x, dpp_u64<0x143, 0xc>(x));      // row_bcast31
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), 63);
+    return ((unsigned long long)hi << 32) | lo;
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ double dpp_f64(double v) {
+    return __longlong_as_double((long long)dpp_u64<CTRL, RM>((unsigned long long)__double_as_longlong(v)));
+}
+__device__ __forceinline__ double wsum_dpp(double v) {                    // every lane active
+    v += dpp_f64<0xb1, 0xf>(v);
+    v += dpp_f64<0x4e, 0xf>(v);
+    v += dpp_f64<0x141, 0xf>(v);
+    v += dpp_f64<0x140, 0xf>(v);
+    v += dpp_f64<0x142, 0xa>(v);
+    v += dpp_f64<0x143, 0xc>(v);
+    const unsigned long long x = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), 63);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 // block sum of four values; thread 0 writes part[4*block + k]
 __device__ void block_store4(double a, double b, double c, double d, double *part) {
     __shared__ double sh[4][16];
@@ -1485,10 +1523,7 @@ __global__ __launch_bounds__(RW_T) void k_pm1p_bandr(int N, BandCls c0, BandCls 
                 const double cv = colj[lane], cv64 = colj[BAND_B];
                 unsigned long long kk = key(cv, lane);
                 if (lane == 0) { const unsigned long long k64 = key(cv64, BAND_B); if (k64 > kk) kk = k64; }
-                for (int off = 32; off > 0; off >>= 1) {
-                    const unsigned long long o = __shfl_xor(kk, off, 64);
-                    if (o > kk) kk = o;
-                }
+                kk = wmax_u64(kk);
                 const int pv = 127 - (int)(kk & 0x7F);
                 const double piv = colj[pv];
                 const double rp = 1.0 / piv;
@@ -1560,7 +1595,7 @@ __global__ __launch_bounds__(RW_T) void k_pm1p_bandr(int N, BandCls c0, BandCls 
                 const int j = jt - uu2, cs = j % BAND_W;
                 double sm = stg[uu2][lane] * ys[lane] + stg[uu2][lane + 64] * ys[lane + 64];
                 if (lane == 0) sm += stg[uu2][2 * BAND_B] * ys[2 * BAND_B];
-                sm = wsum(sm);
+                sm = wsum_dpp(sm);
                 if (lane == 0) {
                     const double y = (stg[uu2][BAND_W] - sm) * stg[uu2][BAND_W + 1];
                     ys[cs] = y;
