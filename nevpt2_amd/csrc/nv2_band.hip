@@ -120,7 +120,7 @@ __global__ __launch_bounds__(BCH_T) void k_bchol(const double *Gp, i64 sW, int n
         const int r = 16 * w + i;
         // R = R^-1 = I keeps the batch finite when the flag sends this problem elsewhere
         Rb[r * BB + c] = bad ? (r == c ? 1.0 : 0.0) : a[i];
-        Rib[r * BB + c] = bad ? (r == c ? 1.0 : 0.0) : x[i];
+        Rib[c * BB + r] = bad ? (r == c ? 1.0 : 0.0) : x[i];     // x[i] = R^-1[c][r]
     }
     if (bad && t == 0) flag[b] = 1;
 }

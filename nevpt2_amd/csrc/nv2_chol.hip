@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void k_chol_diag1(double *A, int N, i64 sA, in
     for (int i = 0; i < 16; i++) {
         const int r = 16 * w + i;
         if (r <= c && c < kb && r < kb) Ab[r + (i64)N * c] = a[i];   // U in the upper triangle of the block
-        if (c < kb && r < kb) Ub[r + CHOL_NB * c] = x[i];
+        if (c < kb && r < kb) Ub[c + CHOL_NB * r] = x[i];       // x[i] = U^-1[c][r]
     }
 }
 
