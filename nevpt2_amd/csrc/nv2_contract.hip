@@ -333,43 +333,44 @@ __global__ __launch_bounds__(256) void k_e4_pass_b(const T *__restrict__ E4, con
     V acc[TW];
 #pragma unroll
     for (int u = 0; u < TW; u++) acc[u] = vzero<V>();
-    constexpr int PF = (N3 + 255) / 256;       // next chunk prefetched into registers
-    T pf[PF];
+    constexpr int PF = (N3 + 255) / 256;       // the next two chunks prefetched into registers
+    constexpr int NQ = NN * NN;                // chunks q = x3 * NN + x5
+    T pfa[PF], pfb[PF], pba, pbb;
+    auto fetch = [&](int q, T (&pf)[PF], T &pb) {
+        if (q >= NQ) return;
+        const int x3 = q / NN, x5 = q % NN;
+        if (threadIdx.x < N2) pb = (T)B3[((i64)x3 * N2 + x5 * NN) * NN + threadIdx.x];
+        const T *src = slab + (i64)x3 * N2 * N3 + (i64)x5 * N3;
 #pragma unroll
-    for (int u = 0; u < PF; u++) {
-        const int e = threadIdx.x + 256 * u;
-        pf[u] = e < N3 ? slab[e] : (T)0;
-    }
-    T pb3 = threadIdx.x < N2 ? (T)B3[threadIdx.x] : (T)0;
-    for (int x3 = 0; x3 < NN; x3++)
-        for (int x5 = 0; x5 < NN; x5++) {
-            __syncthreads();
-#pragma unroll
-            for (int u = 0; u < PF; u++) {
-                const int e = threadIdx.x + 256 * u;
-                if (e < N3) ech[CL::at(e / N2, (e / NN) % NN, e % NN)] = pf[u];
-            }
-            if (threadIdx.x < N2) b3s[threadIdx.x] = pb3;
-            __syncthreads();
-            {
-                const int nx5 = x5 + 1 < NN ? x5 + 1 : 0, nx3 = x5 + 1 < NN ? x3 : x3 + 1;
-                if (nx3 < NN) {
-                    if (threadIdx.x < N2) pb3 = (T)B3[((i64)nx3 * N2 + nx5 * NN) * NN + threadIdx.x];
-                    const T *nxt = slab + (i64)nx3 * N2 * N3 + (i64)nx5 * N3;
-#pragma unroll
-                    for (int u = 0; u < PF; u++) {
-                        const int e = threadIdx.x + 256 * u;
-                        pf[u] = e < N3 ? nxt[e] : (T)0;
-                    }
-                }
-            }
-            switch (wave) {
-                case 0: kloop_c3<T, NN, MM, 0, TW>(ech, b3s, li, lk, acc); break;
-                case 1: kloop_c3<T, NN, MM, 1, TW>(ech, b3s, li, lk, acc); break;
-                case 2: kloop_c3<T, NN, MM, 2, TW>(ech, b3s, li, lk, acc); break;
-                default: kloop_c3<T, NN, MM, 3, TW>(ech, b3s, li, lk, acc); break;
-            }
+        for (int u = 0; u < PF; u++) {
+            const int e = threadIdx.x + 256 * u;
+            pf[u] = e < N3 ? src[e] : (T)0;
         }
+    };
+    pba = pbb = (T)0;
+    fetch(0, pfa, pba);
+    fetch(1, pfb, pbb);
+    auto step = [&](int q, T (&pf)[PF], T &pb) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            const int e = threadIdx.x + 256 * u;
+            if (e < N3) ech[CL::at(e / N2, (e / NN) % NN, e % NN)] = pf[u];
+        }
+        if (threadIdx.x < N2) b3s[threadIdx.x] = pb;
+        __syncthreads();
+        fetch(q + 2, pf, pb);
+        switch (wave) {
+            case 0: kloop_c3<T, NN, MM, 0, TW>(ech, b3s, li, lk, acc); break;
+            case 1: kloop_c3<T, NN, MM, 1, TW>(ech, b3s, li, lk, acc); break;
+            case 2: kloop_c3<T, NN, MM, 2, TW>(ech, b3s, li, lk, acc); break;
+            default: kloop_c3<T, NN, MM, 3, TW>(ech, b3s, li, lk, acc); break;
+        }
+    };
+    for (int q = 0; q < NQ; q += 2) {
+        step(q, pfa, pba);
+        if (q + 1 < NQ) step(q + 1, pfb, pbb);
+    }
     const int j = li;
 #pragma unroll
     for (int u = 0; u < TW; u++) {
