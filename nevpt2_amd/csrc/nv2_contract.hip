@@ -162,13 +162,16 @@ __device__ __forceinline__ void chunk_a(const T *ech, const T *b1s, const T *b2s
     kloop_c3<T, NN, MM, W, TW>(ech, b3s, li, lk, acc3);
 }
 
-// pass A: workgroup g = (cp*n + ap)*n + bp.  The chunks (x4,x5) are visited in two groups of x4
-// (G4 values each), x5-outer inside a group, so the B1 and B2 slices of one x5 are staged in LDS once
-// per group and serve its G4 chunks, and C1 and C2 keep one accumulator tile per x4 of the group in
-// registers (G4 tiles instead of n: the register budget of two workgroups per CU).  Wavefronts 0 and 1
-// split the K range of C1, 2 and 3 that of C2 (their halves are added through LDS after each group),
-// and all four share the C3 tiles (one per x7), so every wavefront issues about the same number of
-// MFMAs per chunk.  Each chunk is read from HBM once.
+// pass A: workgroup 2 g + h, g = (cp*n + ap)*n + bp, takes the chunks (x4,x5) of x4 group h (x4 in
+// h G4 .. h G4 + G4 - 1), x5-outer, so the B1 and B2 slices of one x5 are staged in LDS once and serve
+// its G4 chunks, and C1 and C2 keep one accumulator tile per x4 of the group in registers (G4 tiles
+// instead of n: the register budget of two workgroups per CU).  Two workgroups per slab make 2 n^3
+// equal workgroups, so the last round of the grid is a smaller fraction of it.  C3 sums over both
+// groups: each workgroup adds its half to the zeroed output with an FP64 atomic add, and two addends
+// onto zero give a+b = b+a whichever lands first.  Wavefronts 0 and 1 split the K range of C1, 2 and
+// 3 that of C2 (their halves are added through LDS at the end), and all four share the C3 tiles (one
+// per x7), so every wavefront issues about the same number of MFMAs per chunk.  Each chunk is read from
+// HBM once.
 // amdgpu_waves_per_eu(2): two workgroups per CU, so one's barriers and chunk stores overlap the
 // other's MFMAs (the LDS footprint, 77 KB at n = 14 in FP64, allows two).
 template <typename T, int NN, typename MM = Mfma<T>>
@@ -189,7 +192,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     constexpr int B3S = 4 * KC * NN + 16;
     typedef typename MM::V V;
     __shared__ T ech[CL::SIZE], b1s[BS1], b2s[BS2], b3s[B3S];
-    const int g = g0 + blockIdx.x;
+    const int gb = g0 + blockIdx.x;
+    const int g = gb >> 1, x4g = (gb & 1) * G4;    // this workgroup's x4 group: x4g .. x4g + g4 - 1
+    const int g4 = NN - x4g < G4 ? NN - x4g : G4;
     const int bp = g % NN, ap = (g / NN) % NN, cp = g / N2;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: scalar branches
     const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
@@ -208,14 +213,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
     for (int u = 0; u < PF; u++) {
         const int e = threadIdx.x + 256 * u;
-        pf[u] = e < N3 ? slab[e] : (T)0;          // chunk (x4 = 0, x5 = 0)
+        pf[u] = e < N3 ? slab[(i64)x4g * NN * N3 + e] : (T)0;          // chunk (x4 = x4g, x5 = 0)
     }
     // and the B3 slice of the next chunk (N2 <= 256 values: one per thread)
     static_assert(N2 <= 256, "one B3 value per thread");
-    T pb3 = threadIdx.x < N2 ? (T)B3[threadIdx.x] : (T)0;
+    T pb3 = threadIdx.x < N2 ? (T)B3[(i64)x4g * N2 * NN + threadIdx.x] : (T)0;
     const bool c2w = wave >= 2;
-    for (int x4g = 0; x4g < NN; x4g += G4) {
-        const int g4 = NN - x4g < G4 ? NN - x4g : G4;       // x4 values in this group (uniform)
+    {
         V accA[G4];
 #pragma unroll
         for (int u = 0; u < G4; u++) accA[u] = vzero<V>();
@@ -239,10 +243,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 __syncthreads();
                 {                              // prefetch the next chunk in visiting order
                     int nx4 = x4 + 1, nx5 = x5;
-                    if (u + 1 >= g4) {         // end of the group's x4 run: next x5, or the next group
+                    if (u + 1 >= g4) {         // end of the group's x4 run: next x5, or none
                         nx4 = x4g;
                         nx5 = x5 + 1;
-                        if (nx5 == NN) { nx4 = x4g + G4; nx5 = 0; }
+                        if (nx5 == NN) nx4 = NN;
                     }
                     if (nx4 < NN) {
                         if (threadIdx.x < N2) pb3 = (T)B3[((i64)nx4 * N2 + nx5 * NN) * NN + threadIdx.x];
@@ -265,8 +269,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             }
         }
         // C1 / C2 of the group: the upper halves (wavefronts 1 and 3) are added to the lower ones through
-        // the B2 slice buffer (restaged before its next use; its zero rows meet zero pad slots of the
-        // chunk, so what this leaves there does not matter)
+        // the B2 slice buffer (its zero rows meet zero pad slots of the chunk, so what this leaves there
+        // does not matter)
         T *part = b2s;
 #pragma unroll
         for (int u = 0; u < G4; u++) {
@@ -293,8 +297,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             }
         }
     }
-    {                                          // C3: rows (a,c) = (x7 = tile, x8), columns b
-        const int j = li;
+    {                                          // C3: rows (a,c) = (x7 = tile, x8), columns b; the two x4
+        const int j = li;                      // groups' halves are added to the zeroed output
 #pragma unroll
         for (int u = 0; u < TW; u++) {
             const int a = wave + 4 * u;
@@ -303,14 +307,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             for (int r = 0; r < 4; r++) {
                 const int c = MM::row(lane, r);
                 if (c < NN && j < NN)
-                    C3[s + (i64)M * (obase + (i64)N3 * a + (i64)N2 * N2 * j + (i64)N2 * N3 * c)] = (double)acc3[u][r];
+                    unsafeAtomicAdd(&C3[s + (i64)M * (obase + (i64)N3 * a + (i64)N2 * N2 * j + (i64)N2 * N3 * c)],
+                                    (double)acc3[u][r]);
             }
         }
     }
 }
 
 // pass B: workgroup g = (cp*n + ap)*n + bp; C4[(x7,x8)][b] = sum over chunks (x3,x5) and x6, one tile per
-// x7, the tiles spread over the four wavefronts
+// x7, the tiles spread over the four wavefronts.  (Two workgroups per slab, as in pass A, made this
+// streaming pass slower.)
 template <typename T, int NN, typename MM = Mfma<T>>
 __global__ __launch_bounds__(256) void k_e4_pass_b(const T *__restrict__ E4, const double *__restrict__ B3, int M,
                                                    int s, double *C4, int g0) {
@@ -400,13 +406,22 @@ void contract_plan_destroy(ContractPlan *p) {
     delete p;
 }
 
+// C[s + M k] = 0 for k < cnt: one state's entries of a contraction output
+__global__ void k_zero_state(double *C, int M, int s, i64 cnt) {
+    const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < cnt) C[s + (i64)M * k] = 0.0;
+}
+
 template <typename T, int NN, typename MM>
 static int run_passes(ContractPlan *p, const T *E4, int s, double **outs, hipStream_t st) {
     const unsigned nwg = NN * NN * NN;
     const int nch = stage1_chunks();
     const unsigned chunk = ((nwg + nch - 1) / nch + 7) / 8 * 8;
-    for (unsigned g0 = 0; g0 < nwg; g0 += chunk)
-        hipLaunchKernelGGL((k_e4_pass_a<T, NN, MM>), dim3(nwg - g0 < chunk ? nwg - g0 : chunk), dim3(256), 0, st, E4,
+    const i64 n6 = (i64)nwg * nwg;
+    hipLaunchKernelGGL(k_zero_state, dim3((unsigned)((n6 + 255) / 256)), dim3(256), 0, st, outs[2], p->M, s, n6);
+    const unsigned nwa = 2 * nwg, chunka = 2 * chunk;    // pass A: two workgroups per slab
+    for (unsigned g0 = 0; g0 < nwa; g0 += chunka)
+        hipLaunchKernelGGL((k_e4_pass_a<T, NN, MM>), dim3(nwa - g0 < chunka ? nwa - g0 : chunka), dim3(256), 0, st, E4,
                            p->Bk[0], p->Bk[1], p->Bk[2], p->M, s, outs[0], outs[1], outs[2], (int)g0);
     for (unsigned g0 = 0; g0 < nwg; g0 += chunk)
         hipLaunchKernelGGL((k_e4_pass_b<T, NN, MM>), dim3(nwg - g0 < chunk ? nwg - g0 : chunk), dim3(256), 0, st, E4,
