@@ -13,8 +13,8 @@
 //     the reflectors Y (unit lower trapezoidal) and T = -U S Y_top^-T, so that H = I - Y T Y^T has
 //     H[:, :BAND_B] = Q2 S and H^T P = [S R; 0];
 //   * the trailing block A22 <- H^T A22 H as W = A22 Y T, Z = W - 1/2 Y (T^T Y^T W),
-//     A22 -= Y Z^T + Z Y^T (rocBLAS GEMMs; the BAND_B x BAND_B products over the panel's rows
-//     P^T P, Q1^T Q1 and Y^T W are split over row blocks, k_atb_parts).
+//     A22 -= [Y Z] [Z Y]^T (rocBLAS GEMMs, the last one of rank 2 BAND_B; the BAND_B x BAND_B products
+//     over the panel's rows P^T P, Q1^T Q1 and Y^T W are split over row blocks, k_atb_parts).
 // The last panel (m <= BAND_B rows) is factored by Householder steps in LDS, with its orthogonal
 // factor kept explicitly.  The panels of every matrix of the batch advance together.
 #include <hip/hip_runtime.h>
@@ -34,7 +34,7 @@ namespace nv2 {
 
 struct BandLayout {      // offsets (doubles) inside one problem's part of the workspace
     int npan;            // panels with more than BAND_B rows (CholeskyQR2)
-    i64 Y, T, Ql, Bl, Gp, R1, R1i, R2, R2i, Ui, V1, Q1, Q2, W, W2, size;
+    i64 Y, T, Ql, Bl, Gp, R1, R1i, R2, R2i, Ui, V1, Q1, Q2, W, X, size;   // X = [Y | W2 | Y], m x 3 BAND_B
 };
 static BandLayout band_layout(int N) {
     BandLayout L;
@@ -57,7 +57,7 @@ static BandLayout band_layout(int N) {
     L.Q1 = o; o += col;
     L.Q2 = o; o += col;
     L.W = o; o += col;
-    L.W2 = o; o += col;
+    L.X = o; o += 3 * col;
     L.size = o;
     return L;
 }
@@ -74,6 +74,15 @@ __global__ void k_band_sym(double *A, int N, i64 sA) {   // lower <- upper
     const int i = (int)(e % N), j = (int)(e / N);
     double *Ab = A + blockIdx.z * sA;
     if (i > j) Ab[e] = Ab[j + (i64)N * i];
+}
+// X[:, 0:BB] = X[:, 2BB:3BB] = Y (rows 0..m-1, leading dimension N), problem blockIdx.y
+__global__ void k_band_ycopy(const double *Y, double *X, int N, int m, i64 sW) {
+    const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (i64)m * BB) return;
+    const i64 r = e % m, c = e / m, o = (i64)blockIdx.y * sW + r + (i64)N * c;
+    const double v = Y[o];
+    X[o] = v;
+    X[o + (i64)2 * BB * N] = v;
 }
 __global__ void k_band_zero_flags(int *f, int nb) {
     if (threadIdx.x < nb) f[threadIdx.x] = 0;
@@ -625,7 +634,7 @@ int band_reduce(rocblas_handle h, hipStream_t st, int N, double *A, i64 sA, int 
         double *Yk = w0 + L.Y + (i64)k * BB * N, *Tk = w0 + L.T + (i64)k * BB * BB;
         double *Gp = w0 + L.Gp, *R1 = w0 + L.R1, *R1i = w0 + L.R1i, *R2 = w0 + L.R2, *R2i = w0 + L.R2i;
         double *Ui = w0 + L.Ui, *V1 = w0 + L.V1, *Q1 = w0 + L.Q1, *Q2 = w0 + L.Q2;
-        double *W = w0 + L.W, *W2 = w0 + L.W2;
+        double *W = w0 + L.W, *X = w0 + L.X, *W2 = X + (i64)BB * N;
         const int np = (m + KC - 1) / KC;
         // CholeskyQR2 of P
         hipLaunchKernelGGL(k_atb_parts, dim3(np, nb), dim3(BWG), 0, st, P, sA, P, sA, N, m, Gp, sW);
@@ -646,8 +655,12 @@ int band_reduce(rocblas_handle h, hipStream_t st, int N, double *A, i64 sA, int 
         hipLaunchKernelGGL(k_tx, dim3(nb), dim3(BWG), 0, st, Gp, np, Tk, V1, sW);
         NV2_LAUNCH_CHECK();
         RBB(rocblas_dgemm_strided_batched(h, Nn, Nn, m, BB, BB, &mhalf, Yk, N, sW, V1, BB, sW, &one, W2, N, sW, nb));
-        RBB(rocblas_dgemm_strided_batched(h, Nn, Tt, m, m, BB, &mone, Yk, N, sW, W2, N, sW, &one, A22, N, sA, nb));
-        RBB(rocblas_dgemm_strided_batched(h, Nn, Tt, m, m, BB, &mone, W2, N, sW, Yk, N, sW, &one, A22, N, sA, nb));
+        // A22 -= Y W2^T + W2 Y^T as one rank-2 BAND_B update: [Y | W2] [W2 | Y]^T, both operands inside
+        // X = [Y | W2 | Y] (one pass over A22 instead of two)
+        hipLaunchKernelGGL(k_band_ycopy, dim3(nbl((i64)m * BB, 256), nb), dim3(256), 0, st, Yk, X, N, m, sW);
+        NV2_LAUNCH_CHECK();
+        RBB(rocblas_dgemm_strided_batched(h, Nn, Tt, m, m, 2 * BB, &mone, X, N, sW, X + (i64)BB * N, N, sW, &one, A22, N,
+                                          sA, nb));
     }
     {
         const int p0 = BB * L.npan, r0 = p0 + BB, m = N - r0;
