@@ -75,15 +75,6 @@ __global__ void k_band_sym(double *A, int N, i64 sA) {   // lower <- upper
     double *Ab = A + blockIdx.z * sA;
     if (i > j) Ab[e] = Ab[j + (i64)N * i];
 }
-// X[:, 0:BB] = X[:, 2BB:3BB] = Y (rows 0..m-1, leading dimension N), problem blockIdx.y
-__global__ void k_band_ycopy(const double *Y, double *X, int N, int m, i64 sW) {
-    const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (i64)m * BB) return;
-    const i64 r = e % m, c = e / m, o = (i64)blockIdx.y * sW + r + (i64)N * c;
-    const double v = Y[o];
-    X[o] = v;
-    X[o + (i64)2 * BB * N] = v;
-}
 __global__ void k_band_zero_flags(int *f, int nb) {
     if (threadIdx.x < nb) f[threadIdx.x] = 0;
 }
@@ -136,8 +127,10 @@ __global__ __launch_bounds__(BCH_T) void k_bchol(const double *Gp, i64 sW, int n
 // Partial products of A^T B for A, B with BAND_B columns (ld, m rows; problem b = blockIdx.y at A + b*sA,
 // B + b*sB): workgroup c sums rows c*KC .. c*KC+KC-1 into Gp + b*sW + c*BAND_B^2.  rocBLAS gives a
 // BAND_B x BAND_B result with thousands of rows to a single workgroup; here ceil(m / KC) of them share it.
+// With Ac set, the rows of A it reads are also copied to Ac + b*sW and Ac + b*sW + 2*BAND_B*ld (the outer
+// blocks of [Y | W2 | Y]).
 __global__ __launch_bounds__(BWG) void k_atb_parts(const double *A, i64 sA, const double *B, i64 sB, int ld, int m,
-                                                   double *Gp, i64 sW) {
+                                                   double *Gp, i64 sW, double *Ac) {
     __shared__ double sa[KS][BB + 1];
     __shared__ double sb[KS][BB + 1];
     const int c = blockIdx.x, b = blockIdx.y, t = threadIdx.x;
@@ -149,8 +142,14 @@ __global__ __launch_bounds__(BWG) void k_atb_parts(const double *A, i64 sA, cons
         __syncthreads();                 // the previous stage is consumed
         for (int e = t; e < KS * BB; e += BWG) {
             const int k = e % KS, i = e / KS;
-            sa[k][i] = s0 + k < kn ? Ab[s0 + k + (i64)ld * i] : 0.0;
+            const double av = s0 + k < kn ? Ab[s0 + k + (i64)ld * i] : 0.0;
+            sa[k][i] = av;
             sb[k][i] = s0 + k < kn ? Bb[s0 + k + (i64)ld * i] : 0.0;
+            if (Ac && s0 + k < kn) {
+                double *Acb = Ac + b * sW + k0 + s0 + k + (i64)ld * i;
+                Acb[0] = av;
+                Acb[(i64)2 * BB * ld] = av;
+            }
         }
         __syncthreads();
         for (int k = 0; k < KS; k++) {
@@ -637,10 +636,10 @@ int band_reduce(rocblas_handle h, hipStream_t st, int N, double *A, i64 sA, int 
         double *W = w0 + L.W, *X = w0 + L.X, *W2 = X + (i64)BB * N;
         const int np = (m + KC - 1) / KC;
         // CholeskyQR2 of P
-        hipLaunchKernelGGL(k_atb_parts, dim3(np, nb), dim3(BWG), 0, st, P, sA, P, sA, N, m, Gp, sW);
+        hipLaunchKernelGGL(k_atb_parts, dim3(np, nb), dim3(BWG), 0, st, P, sA, P, sA, N, m, Gp, sW, nullptr);
         hipLaunchKernelGGL(k_bchol, dim3(nb), dim3(BCH_T), 0, st, Gp, sW, np, R1, R1i, dflag, 0.0);
         RBB(rocblas_dgemm_strided_batched(h, Nn, Tt, m, BB, BB, &one, P, N, sA, R1i, BB, sW, &zero, Q1, N, sW, nb));
-        hipLaunchKernelGGL(k_atb_parts, dim3(np, nb), dim3(BWG), 0, st, Q1, sW, Q1, sW, N, m, Gp, sW);
+        hipLaunchKernelGGL(k_atb_parts, dim3(np, nb), dim3(BWG), 0, st, Q1, sW, Q1, sW, N, m, Gp, sW, nullptr);
         hipLaunchKernelGGL(k_bchol, dim3(nb), dim3(BCH_T), 0, st, Gp, sW, np, R2, R2i, dflag, 0.1);
         RBB(rocblas_dgemm_strided_batched(h, Nn, Tt, m, BB, BB, &one, Q1, N, sW, R2i, BB, sW, &zero, Q2, N, sW, nb));
         // reflectors: Y_top, T, U^-1, and S R into the panel
@@ -651,14 +650,13 @@ int band_reduce(rocblas_handle h, hipStream_t st, int N, double *A, i64 sA, int 
         // A22 <- H^T A22 H
         RBB(rocblas_dgemm_strided_batched(h, Nn, Nn, m, BB, m, &one, A22, N, sA, Yk, N, sW, &zero, W, N, sW, nb));
         RBB(rocblas_dgemm_strided_batched(h, Nn, Nn, m, BB, BB, &one, W, N, sW, Tk, BB, sW, &zero, W2, N, sW, nb));
-        hipLaunchKernelGGL(k_atb_parts, dim3(np, nb), dim3(BWG), 0, st, Yk, sW, W2, sW, N, m, Gp, sW);
+        // Y^T W2 partials; the same reads copy Y into the outer blocks of X = [Y | W2 | Y]
+        hipLaunchKernelGGL(k_atb_parts, dim3(np, nb), dim3(BWG), 0, st, Yk, sW, W2, sW, N, m, Gp, sW, X);
         hipLaunchKernelGGL(k_tx, dim3(nb), dim3(BWG), 0, st, Gp, np, Tk, V1, sW);
         NV2_LAUNCH_CHECK();
         RBB(rocblas_dgemm_strided_batched(h, Nn, Nn, m, BB, BB, &mhalf, Yk, N, sW, V1, BB, sW, &one, W2, N, sW, nb));
         // A22 -= Y W2^T + W2 Y^T as one rank-2 BAND_B update: [Y | W2] [W2 | Y]^T, both operands inside
         // X = [Y | W2 | Y] (one pass over A22 instead of two)
-        hipLaunchKernelGGL(k_band_ycopy, dim3(nbl((i64)m * BB, 256), nb), dim3(256), 0, st, Yk, X, N, m, sW);
-        NV2_LAUNCH_CHECK();
         RBB(rocblas_dgemm_strided_batched(h, Nn, Tt, m, m, 2 * BB, &mone, X, N, sW, X + (i64)BB * N, N, sW, &one, A22, N,
                                           sA, nb));
     }
