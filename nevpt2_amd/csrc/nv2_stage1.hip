@@ -930,13 +930,16 @@ int launch_koop_small(const Rdm &R, int ncore, const double *aj, const double *a
 // Launches the big stage-1 kernels are split into (NV2_E4_CHUNKS for the E4 build, NV2_PASS_CHUNKS for
 // each contraction pass; default 1): between two launches the dispatcher serves the other streams'
 // pending kernels
-static int env_chunks(const char *name) {
+static int env_chunks(const char *name, int dflt) {
     const char *e = getenv(name);
-    const int c = e ? atoi(e) : 1;
+    const int c = e ? atoi(e) : dflt;
     return c < 1 ? 1 : (c > 64 ? 64 : c);
 }
-int stage1_chunks() { return env_chunks("NV2_PASS_CHUNKS"); }
-static int e4_chunks() { return env_chunks("NV2_E4_CHUNKS"); }
+int stage1_chunks() { return env_chunks("NV2_PASS_CHUNKS", 1); }
+// The E4 build goes out as 32 launches (about 10 000 workgroups each at n = 14): between two launches
+// the (+-1)' chain's pending kernels, which a single launch of 318 000 workgroups held back for its
+// whole duration, get the CUs (CAS(14,14): 64.2 against 64.9 ms per evaluation).
+static int e4_chunks() { return env_chunks("NV2_E4_CHUNKS", 32); }
 template <typename T>
 static int build_e4_n(const Rdm &R, int s, T *E4, double *d3t, hipStream_t st) {
     const int nch = e4_chunks();
