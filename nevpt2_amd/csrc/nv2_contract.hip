@@ -9,10 +9,10 @@
 // (B1[(d,j,i),a] = w(d,j,i,a), B2[(d,j,i),c] = w(d,j,c,i), B3[(i,d,j),b] = w(d,j,b,i)).
 //
 // Every index runs over n <= 16 active orbitals, so each contraction is a set of 16x16 MFMA
-// tiles.  Pass A (one workgroup per (cp,ap,bp), four wavefronts) streams the contiguous
-// sub-slab E4[cp,ap,bp,:,:,:,:,:] (n^5 values) once through LDS in chunks (x4,x5) of n^3
-// values and feeds C1, C2 and C3 from each chunk (the K ranges of C1 and C2 split over two
-// wavefronts each, the C3 tiles over all four); pass B (one
+// tiles.  Pass A (two workgroups per (cp,ap,bp), one per half of x4, four wavefronts each)
+// streams the contiguous sub-slab E4[cp,ap,bp,:,:,:,:,:] (n^5 values) once through LDS in chunks
+// (x4,x5) of n^3 values and feeds C1, C2 and C3 from each chunk (the K ranges of C1 and C2 split
+// over two wavefronts each, the C3 tiles over all four); pass B (one
 // workgroup per (cp,ap,bp'), four wavefronts) streams E4[cp,ap,:,bp',...] for C4, whose sum
 // runs over x3.  E4 is read twice instead of four times.  FP64 uses v_mfma_f64_16x16x4f64;
 // the FP32 form stores E4 in float and uses v_mfma_f32_16x16x4f32 with FP32 accumulation; the
